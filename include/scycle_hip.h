@@ -1,0 +1,243 @@
+/*
+ * scycle_hip.h -- C ABI of the MI355X (gfx950) SCycle step library,
+ * libscycle_hip.so.
+ *
+ * Every entry point returns int: 0 on success, a non-zero SC_ERR_* code
+ * otherwise (the reference propagates PetscErrorCode the same way), and
+ * sc_last_error() returns a message for the last failure on the calling
+ * thread.  No C++ exception crosses this boundary.
+ *
+ * Pointers named *_d are device pointers (hipMalloc'd, FP64 unless the
+ * argument says otherwise); *_h are host pointers.  All work is enqueued on
+ * the context's own HIP stream; sc_ctx_sync() waits for it.  A context must
+ * not be used from two host threads at once (the reference is single-threaded
+ * SPMD per rank, too).
+ *
+ * Layout of every 2-D field: index I = j*Nz + k (z fastest, j along y); the
+ * fault is j = 0, the surface k = 0 (domain.cpp:379-383, 449-501).
+ * Units are the reference's: km, GPa, g/cm^3, MPa, m, m/s, s.
+ */
+#ifndef SCYCLE_HIP_H
+#define SCYCLE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SC_OK 0
+#define SC_ERR_ARG 1        /* invalid argument / unsupported option      */
+#define SC_ERR_HIP 2        /* HIP runtime error                          */
+#define SC_ERR_NOCONV 3     /* iterative solve or root solve not converged */
+#define SC_ERR_NONFINITE 4  /* NaN/Inf produced (reference asserts)       */
+#define SC_ERR_IO 5         /* file / parse error                         */
+
+#define SC_DIRICHLET 0
+#define SC_NEUMANN 1
+
+#define SC_FP64 0
+#define SC_FP32 1
+
+typedef struct sc_ctx sc_ctx;
+
+/* Grid and operator options; fields mirror the setters of SbpOps
+ * (sbpOps.hpp:42-104) and the Domain keys that select the operator
+ * (domain.cpp:167-188). */
+typedef struct sc_grid_desc {
+  int Ny, Nz;
+  double Ly, Lz;
+  int order;           /* 2 or 4                                          */
+  int var_grid;        /* 1 = variableGridSpacing, 0 = constantGridSpacing */
+  int compat;          /* 0 = fullyCompatible, 1 = compatible             */
+  int laplace;         /* 0 = "yz", 1 = "y", 2 = "z" (setLaplaceType)      */
+  int bcR, bcT, bcL, bcB; /* SC_DIRICHLET / SC_NEUMANN (setBCTypes order R,T,L,B) */
+} sc_grid_desc;
+
+const char* sc_last_error(void);
+int sc_version(int* major, int* minor);
+
+/* Device memory helpers (the caller may also pass its own allocations). */
+int sc_malloc(void** ptr_d, unsigned long long bytes);
+int sc_free(void* ptr_d);
+int sc_memcpy_h2d(void* dst_d, const void* src_h, unsigned long long bytes);
+int sc_memcpy_d2h(void* dst_h, const void* src_d, unsigned long long bytes);
+
+/* Creates the operator context: replaces the SbpOps constructor +
+ * setGrid/setBCTypes/setMultiplyByH(1)/setLaplaceType/computeMatrices
+ * sequence of LinearElastic::setUpSBPContext (linearElastic.cpp:426-461).
+ * y_h, z_h: N-length host coordinate fields (NULL => y = Ly*q, z = Lz*r);
+ * mu_h: N-length shear modulus.  The grid must be a tensor-product grid
+ * (y depends on j only, z on k only); otherwise SC_ERR_ARG. */
+int sc_ctx_create(const sc_grid_desc* desc, const double* y_h, const double* z_h,
+                  const double* mu_h, sc_ctx** out);
+int sc_ctx_destroy(sc_ctx* ctx);
+int sc_ctx_sync(sc_ctx* ctx);
+/* Returns the context's hipStream_t (as void*). */
+int sc_ctx_stream(sc_ctx* ctx, void** stream);
+
+/* SbpOps::updateVarCoeff (sbpOps.hpp:65; sbpOps_m_varGrid.cpp:531-560). */
+int sc_update_var_coeff(sc_ctx* ctx, const double* mu_h);
+/* SbpOps::changeBCTypes (sbpOps.hpp:61; sbpOps_m_varGrid.cpp:187-213). */
+int sc_change_bc_types(sc_ctx* ctx, int bcR, int bcT, int bcL, int bcB);
+/* SbpOps::geth11 (sbpOps.hpp:68). */
+int sc_geth11(sc_ctx* ctx, double* h11y, double* h11z);
+/* Diagonals of SbpOps::getCoordTrans yq (Ny values) and zr (Nz values)
+ * (sbpOps.hpp:81; sbpOps_m_varGrid.cpp:333-385). */
+int sc_get_metric(sc_ctx* ctx, double* yq_h, double* zr_h);
+
+/* out = A*in, A = getA (sbpOps.hpp:71) as used by KSPSolve and by
+ * propagateWaves (strikeSlip_linearElastic_fd.cpp:459-460).
+ * prec = SC_FP64 (double buffers) or SC_FP32 (float buffers). */
+int sc_sbp_apply(sc_ctx* ctx, int prec, const void* in_d, void* out_d);
+/* diag(A) (FP64). */
+int sc_sbp_diag(sc_ctx* ctx, double* out_d);
+/* SbpOps::setRhs(rhs, bcL, bcR, bcT, bcB) (sbpOps.hpp:84;
+ * sbpOps_m_varGrid.cpp:1046-1083).  bcL/bcR: Nz values, bcT/bcB: Ny. */
+int sc_sbp_set_rhs(sc_ctx* ctx, double* rhs_d, const double* bcL_d, const double* bcR_d,
+                   const double* bcT_d, const double* bcB_d);
+/* SbpOps::Dy, muxDy, Dz, muxDz, H, Hinv (sbpOps.hpp:87-96). */
+int sc_sbp_dy(sc_ctx* ctx, const double* in_d, double* out_d);
+int sc_sbp_mux_dy(sc_ctx* ctx, const double* in_d, double* out_d);
+int sc_sbp_dz(sc_ctx* ctx, const double* in_d, double* out_d);
+int sc_sbp_mux_dz(sc_ctx* ctx, const double* in_d, double* out_d);
+int sc_sbp_h(sc_ctx* ctx, const double* in_d, double* out_d);
+int sc_sbp_hinv(sc_ctx* ctx, const double* in_d, double* out_d);
+
+/* Measurement: mean duration (ms) of `reps` back-to-back launches timed with
+ * HIP events on the context stream, and the nodes they write.  which:
+ * 0 FP64 apply, 1 FP32 apply, 2 FP64 interior kernel, 3 FP32 interior kernel.
+ * No counterpart in the reference (its timers are MPI_Wtime accumulators,
+ * e.g. linearElastic.cpp:474-477). */
+int sc_bench_apply(sc_ctx* ctx, int which, int reps, double* ms_per_launch, double* nodes);
+
+/* ---------------------------------------------------------------- solve */
+#define SC_SOLVE_CG64 0  /* FP64 Jacobi-PCG                                */
+#define SC_SOLVE_IR32 1  /* FP32 Jacobi-PCG inside FP64 iterative refinement */
+
+typedef struct sc_solve_opts {
+  int mode;            /* SC_SOLVE_CG64 / SC_SOLVE_IR32                   */
+  double rtol;         /* ||b - A u||_2 <= rtol * ||b||_2 (FP64 residual)  */
+  double inner_rtol;   /* IR32: relative reduction per FP32 inner solve    */
+  int max_iters;       /* cap on (inner) CG iterations in total           */
+  int max_outer;       /* IR32: cap on refinement sweeps                  */
+  int check_every;     /* iterations between host convergence checks      */
+} sc_solve_opts;
+
+typedef struct sc_solve_stats {
+  int iters;           /* total CG iterations (FP64 or FP32)              */
+  int outer;           /* refinement sweeps (IR32)                        */
+  int converged;
+  double relres;       /* final FP64 relative residual                    */
+} sc_solve_stats;
+
+/* LinearElastic::computeU -> KSPSolve(A, rhs, u) (linearElastic.cpp:465-489)
+ * with u as the initial guess (KSPSetInitialGuessNonzero) and the solution. */
+int sc_solve(sc_ctx* ctx, const double* rhs_d, double* u_d, const sc_solve_opts* opts,
+             sc_solve_stats* stats);
+
+/* ---------------------------------------------------------------- fault */
+#define SC_AGING_LAW 0
+#define SC_SLIP_LAW 1
+
+typedef struct sc_fault_qd_args {
+  int N;                     /* fault nodes (Nz)                          */
+  double v0, vL, f0, rootTol;
+  int maxNumIts;
+  int stateLaw;              /* SC_AGING_LAW / SC_SLIP_LAW               */
+  int jac_mode;              /* 0: reference's Newton slope (fault.cpp:1051), 1: exact */
+  const double *eta_d, *sN_d, *a_d, *b_d, *Dc_d, *locked_d, *prestress_d;
+  const double *psi_d;
+  double *tauQS_d;           /* in: sxy on the fault; out: + prestress    */
+  double *V_d;               /* in: previous slip velocity (Newton start); out: V */
+  double *dpsi_d, *tau_d, *strength_d;
+  int* status_d;             /* device int: set non-zero on failure (may be NULL) */
+} sc_fault_qd_args;
+
+/* Fault_qd::d_dt minus the scatter (fault.cpp:861-918): tauQS += prestress,
+ * BracketedNewton slip-velocity solve (fault.cpp:954-1059,
+ * rootFinder.cpp:174-257), aging/slip state rate (fault.cpp:1984-2088),
+ * tau = tauQS - eta*V, strength (fault.cpp:2246-2283). */
+int sc_fault_qd_rates(sc_ctx* ctx, const sc_fault_qd_args* args);
+
+/* ------------------------------------------------------- QD integration */
+typedef struct sc_qd sc_qd;
+
+typedef struct sc_qd_desc {
+  sc_grid_desc grid;          /* material SAT types: typically R=Dirichlet (remoteLoading),
+                                 T=Neumann, L=Dirichlet (symmFault), B=Neumann */
+  int remoteLoadingR;         /* bcR = vL*t/faultTypeScale + bcRShift      */
+  double faultTypeScale, vL, f0, v0, rootTol;
+  int maxNumIts, stateLaw, jac_mode;
+  sc_solve_opts solve;
+} sc_qd_desc;
+
+typedef struct sc_qd_run {
+  int integrator;             /* 0 RK32, 1 RK43 (odeSolver.cpp)           */
+  int control;                /* 0 P, 1 PID                               */
+  int normType;               /* 0 L2_absolute, 1 L2_relative             */
+  int maxSteps;
+  double initTime, maxTime, initDt, minDt, maxDt, tol;
+  int nReplay;                /* >0: force the step sizes in replayDt       */
+  const double* replayDt_h;
+} sc_qd_run;
+
+typedef struct sc_qd_counters {
+  long long d_dt_calls, solves, cg_iters, outer_sweeps, rejected_steps;
+  double solve_seconds, total_seconds;
+} sc_qd_counters;
+
+/* StrikeSlip_LinearElastic_qd (strikeSlip_linearElastic_qd.cpp:7-78) with
+ * Fault_qd and LinearElastic, over host arrays: mu/y/z (N), fault a, b, Dc,
+ * sN, eta, locked, prestress, bcRShift (Nz; locked/prestress/bcRShift may be
+ * NULL = zero). */
+int sc_qd_create(const sc_qd_desc* desc, const double* mu_h, const double* y_h,
+                 const double* z_h, const double* a_h, const double* b_h, const double* Dc_h,
+                 const double* sN_h, const double* eta_h, const double* locked_h,
+                 const double* prestress_h, const double* bcRShift_h, sc_qd** out);
+/* Reads a reference .in deck (Domain/LinearElastic/Fault/mediator keys,
+ * binary Vec overrides from inputDir) and builds the same mediator. */
+int sc_qd_create_from_file(const char* in_path, sc_qd** out);
+int sc_qd_destroy(sc_qd* q);
+sc_ctx* sc_qd_ctx(sc_qd* q);
+/* Integrated state (psi, slip) and the persistent slip velocity (Nz each). */
+int sc_qd_set_state(sc_qd* q, const double* psi_h, const double* slip_h, const double* V_h);
+int sc_qd_get_state(sc_qd* q, double* psi_h, double* slip_h, double* V_h, double* tau_h);
+int sc_qd_get_u(sc_qd* q, double* u_h);
+/* One StrikeSlip_LinearElastic_qd::d_dt (strikeSlip_linearElastic_qd.cpp:756-798)
+ * at time t on the current state: rates dpsi_h, dslip_h (= V). */
+int sc_qd_ddt(sc_qd* q, double t, double* dpsi_h, double* dslip_h);
+/* OdeSolver::integrate (odeSolver.cpp:505-627 RK32, :973-1175 RK43).
+ * Returns accepted step sizes in dts_h[0..min(steps,dtsCap)). */
+int sc_qd_integrate(sc_qd* q, const sc_qd_run* run, double* tEnd_h, int* steps_h,
+                    double* dts_h, int dtsCap);
+int sc_qd_get_counters(sc_qd* q, sc_qd_counters* c);
+/* Time-integration settings read from the deck (timeIntegrator,
+ * timeControlType, maxStepCount, init/maxTime, min/max/initDeltaT with
+ * computeMinTimeStep, timeStepTol, normType), for sc_qd_create_from_file
+ * drivers. */
+int sc_qd_file_run(sc_qd* q, sc_qd_run* run_out);
+/* StrikeSlip_LinearElastic_qd::solveSS (strikeSlip_linearElastic_qd.cpp:895-1006):
+ * steady-state guess at vL (Fault::guessSS fault.cpp:633-709; psi recomputed
+ * unless have_state_vals), Neumann-fault solve, shift u >= 0, bcRShift and
+ * slip from the solution, back to the configured BC types.  psi_h in/out,
+ * slip_h out (Nz); also sets the driver state (psi, slip, V = vL). */
+int sc_qd_solve_ss(sc_qd* q, double* psi_h, int have_state_vals, double* slip_h);
+
+/* --------------------------------------------- host-only input utilities */
+/* Value of `key` in a .in deck, cut at its first space like every
+ * loadSettings (e.g. domain.cpp:151-165); the last occurrence wins. */
+int sc_in_get(const char* in_path, const char* key, char* buf, int buflen);
+/* Vector value "[v1 v2 ...]" of `key` (loadVectorFromInputFile,
+ * genFuncs.cpp:413-440; a one-element list yields 0 values). */
+int sc_in_get_vector(const char* in_path, const char* key, double* vals, int cap, int* n);
+/* PETSc binary Vec files (big-endian, classid 1211214; genFuncs.cpp:74-94,
+ * 380-410, 1024-1042): number of appended records, read record `rec`,
+ * write or append one record. */
+int sc_vec_count(const char* path, int* nrec);
+int sc_vec_read(const char* path, int rec, double* vals, int cap, int* n);
+int sc_vec_write(const char* path, const double* vals, int n, int append);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCYCLE_HIP_H */

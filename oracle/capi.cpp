@@ -1,0 +1,321 @@
+// ctypes entry points of the CPU oracle (test infrastructure only; see
+// oracle.hpp).  Loaded by tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg; never by the product library.
+#include <chrono>
+#include <cstring>
+
+#include "oracle.hpp"
+#include "physics.hpp"
+
+using namespace orc;
+
+namespace {
+struct OrcSbp {
+  Sbp2D s;
+  BandChol chol;
+  bool chol_ok = false;
+};
+const Csr* pick(const OrcSbp* o, int which) {
+  switch (which) {
+    case 0: return &o->s.A;
+    case 1: return &o->s.H;
+    case 2: return &o->s.Hinv;
+    case 3: return &o->s.Dy_Iz;
+    case 4: return &o->s.Iy_Dz;
+    case 5: return &o->s.D2;
+    case 6: return &o->s.rhsL;
+    case 7: return &o->s.rhsR;
+    case 8: return &o->s.rhsT;
+    case 9: return &o->s.rhsB;
+    case 10: return &o->s.Jinv;
+    default: return nullptr;
+  }
+}
+void dense(const Sp1& m, int n, double* out) {
+  if (!out) return;
+  std::memset(out, 0, sizeof(double) * n * n);
+  for (auto& r : m.m)
+    for (auto& c : r.second) out[(size_t)r.first * n + c.first] = c.second;
+}
+}  // namespace
+
+extern "C" {
+
+void* orc_sbp_new(int order, int Ny, int Nz, double Ly, double Lz, int var_grid, int compat,
+                  int laplace, int bcR, int bcT, int bcL, int bcB, const double* mu,
+                  const double* y, const double* z) {
+  SbpDesc d;
+  d.order = order;
+  d.Ny = Ny;
+  d.Nz = Nz;
+  d.Ly = Ly;
+  d.Lz = Lz;
+  d.var_grid = var_grid != 0;
+  d.compat = compat ? "compatible" : "fullyCompatible";
+  d.laplace = laplace == 0 ? "yz" : (laplace == 1 ? "y" : "z");
+  d.bcR = bcR;
+  d.bcT = bcT;
+  d.bcL = bcL;
+  d.bcB = bcB;
+  auto* o = new OrcSbp;
+  if (sbp2d_build(o->s, d, mu, y, z)) {
+    delete o;
+    return nullptr;
+  }
+  return o;
+}
+void orc_sbp_free(void* h) { delete (OrcSbp*)h; }
+int orc_sbp_change_bc(void* h, int bcR, int bcT, int bcL, int bcB) {
+  auto* o = (OrcSbp*)h;
+  o->chol_ok = false;
+  return sbp2d_change_bc(o->s, bcR, bcT, bcL, bcB);
+}
+long orc_sbp_nnz(void* h, int which) {
+  const Csr* m = pick((OrcSbp*)h, which);
+  return m ? (long)m->rp[m->nr] : -1;
+}
+int orc_sbp_shape(void* h, int which, int* nr, int* nc) {
+  const Csr* m = pick((OrcSbp*)h, which);
+  if (!m) return 1;
+  *nr = m->nr;
+  *nc = m->nc;
+  return 0;
+}
+void orc_sbp_csr(void* h, int which, long* rp, int* ci, double* v) {
+  const Csr* m = pick((OrcSbp*)h, which);
+  for (int i = 0; i <= m->nr; ++i) rp[i] = (long)m->rp[i];
+  std::memcpy(ci, m->ci.data(), sizeof(int) * m->ci.size());
+  std::memcpy(v, m->v.data(), sizeof(double) * m->v.size());
+}
+void orc_sbp_apply(void* h, const double* in, double* out) { spmv(((OrcSbp*)h)->s.A, in, out); }
+void orc_sbp_set_rhs(void* h, double* rhs, const double* bcL, const double* bcR, const double* bcT,
+                     const double* bcB) {
+  sbp2d_set_rhs(((OrcSbp*)h)->s, rhs, bcL, bcR, bcT, bcB);
+}
+void orc_sbp_muxdy(void* h, const double* in, double* out) { sbp2d_muxDy(((OrcSbp*)h)->s, in, out); }
+void orc_sbp_muxdz(void* h, const double* in, double* out) { sbp2d_muxDz(((OrcSbp*)h)->s, in, out); }
+void orc_sbp_dy(void* h, const double* in, double* out) { sbp2d_Dy(((OrcSbp*)h)->s, in, out); }
+void orc_sbp_dz(void* h, const double* in, double* out) { sbp2d_Dz(((OrcSbp*)h)->s, in, out); }
+void orc_sbp_h(void* h, const double* in, double* out) { spmv(((OrcSbp*)h)->s.H, in, out); }
+void orc_sbp_hinv(void* h, const double* in, double* out) { spmv(((OrcSbp*)h)->s.Hinv, in, out); }
+void orc_sbp_diag(void* h, double* out) {
+  auto d = diagonal(((OrcSbp*)h)->s.A);
+  std::memcpy(out, d.data(), sizeof(double) * d.size());
+}
+void orc_sbp_metric(void* h, double* yq, double* zr) {
+  auto* o = (OrcSbp*)h;
+  auto a = diagonal(o->s.yq), b = diagonal(o->s.zr);
+  std::memcpy(yq, a.data(), sizeof(double) * a.size());
+  std::memcpy(zr, b.data(), sizeof(double) * b.size());
+}
+void orc_sbp_scalars(void* h, double* out) {
+  auto& s = ((OrcSbp*)h)->s;
+  out[0] = s.dy; out[1] = s.dz; out[2] = s.alphaDy; out[3] = s.alphaDz;
+  out[4] = s.h11y; out[5] = s.h11z; out[6] = s.alphaT;
+}
+int orc_sbp1d(int order, int N, double scale, int compat, double* H, double* Hinv, double* D1,
+              double* BS, double* D2, double* C2, double* D3, double* D4, double* C3, double* C4) {
+  Sbp1D o(N);
+  int e = sbp_spmat(order, N, scale, o, compat ? "compatible" : "fullyCompatible");
+  if (e) return e;
+  if (order == 2) e = sbp_spmat2(N, scale, o);
+  else e = sbp_spmat4(N, scale, o);
+  if (e) return e;
+  dense(o.H, N, H); dense(o.Hinv, N, Hinv); dense(o.D1, N, D1); dense(o.BS, N, BS);
+  dense(o.D2, N, D2); dense(o.C2, N, C2); dense(o.D3, N, D3); dense(o.D4, N, D4);
+  dense(o.C3, N, C3); dense(o.C4, N, C4);
+  return 0;
+}
+int orc_pcg(void* h, const double* b, double* u, double rtol, int maxit, double* relres) {
+  PcgStats st = pcg(((OrcSbp*)h)->s.A, b, u, rtol, maxit);
+  if (relres) *relres = st.relres;
+  return st.iters;
+}
+int orc_chol_solve(void* h, const double* b, double* u) {
+  auto* o = (OrcSbp*)h;
+  if (!o->chol_ok) {
+    if (o->chol.factor(o->s.A)) return 1;
+    o->chol_ok = true;
+  }
+  o->chol.solve(b, u);
+  return 0;
+}
+
+double orc_strength(double sN, double psi, double V, double a, double v0) { return strength_psi(sN, psi, V, a, v0); }
+double orc_aging(double psi, double V, double b, double f0, double v0, double Dc) { return agingLaw_psi(psi, V, b, f0, v0, Dc); }
+double orc_sliplaw(double psi, double V, double a, double b, double f0, double v0, double Dc) { return slipLaw_psi(psi, V, a, b, f0, v0, Dc); }
+int orc_vel_qd(int N, const double* eta, const double* tauQS, const double* sN, const double* psi,
+               const double* a, const double* locked, double v0, double vL, double tol, int maxIts,
+               int jac_mode, double* V, long* its) {
+  return computeVel_qd(N, eta, tauQS, sN, psi, a, locked, v0, vL, tol, maxIts, jac_mode, V, its);
+}
+
+// ------------------------------------------------------------- QD run ----
+struct OrcQdCfg {
+  int order, Ny, Nz, var_grid;
+  double Ly, Lz;
+  int bcR, bcT, bcL, bcB;          // material SAT types
+  int remoteLoadingR;
+  double faultTypeScale, vL, f0, v0, rootTol;
+  int stateLaw, jac_mode, solver;
+  double cg_rtol;
+};
+struct OrcQdRun {
+  int integrator;  // 0 RK32, 1 RK43
+  int control;     // 0 P, 1 PID
+  int normType;    // 0 L2_absolute, 1 L2_relative
+  int maxSteps;
+  double initTime, maxTime, initDt, minDt, maxDt, tol;
+  int nReplay;
+  const double* replayDt;
+};
+
+void* orc_qd_new(const OrcQdCfg* c, const double* mu, const double* y, const double* z,
+                 const double* a, const double* b, const double* Dc, const double* sN,
+                 const double* eta, const double* locked, const double* prestress,
+                 const double* bcRShift) {
+  QdConfig q;
+  q.sbp.order = c->order;
+  q.sbp.Ny = c->Ny;
+  q.sbp.Nz = c->Nz;
+  q.sbp.Ly = c->Ly;
+  q.sbp.Lz = c->Lz;
+  q.sbp.var_grid = c->var_grid != 0;
+  q.sbp.bcR = c->bcR;
+  q.sbp.bcT = c->bcT;
+  q.sbp.bcL = c->bcL;
+  q.sbp.bcB = c->bcB;
+  q.remoteLoadingR = c->remoteLoadingR;
+  q.faultTypeScale = c->faultTypeScale;
+  q.vL = c->vL;
+  q.f0 = c->f0;
+  q.v0 = c->v0;
+  q.rootTol = c->rootTol;
+  q.stateLaw = c->stateLaw;
+  q.jac_mode = c->jac_mode;
+  q.solver = c->solver;
+  q.cg_rtol = c->cg_rtol;
+  auto* m = new QdModel;
+  int Nz = c->Nz;
+  m->a.assign(a, a + Nz);
+  m->b.assign(b, b + Nz);
+  m->Dc.assign(Dc, Dc + Nz);
+  m->sN.assign(sN, sN + Nz);
+  m->eta.assign(eta, eta + Nz);
+  if (locked) m->locked.assign(locked, locked + Nz);
+  if (prestress) m->prestress.assign(prestress, prestress + Nz);
+  if (m->init(q, mu, y, z)) {
+    delete m;
+    return nullptr;
+  }
+  if (bcRShift) m->bcRShift.assign(bcRShift, bcRShift + Nz);
+  return m;
+}
+void orc_qd_free(void* h) { delete (QdModel*)h; }
+
+// One d_dt evaluation (the reference's StrikeSlip_LinearElastic_qd::d_dt).
+// V is the root-solve warm start on entry (persisting fault state).
+int orc_qd_ddt(void* h, double t, const double* psi, const double* slip, double* V_inout,
+               double* dpsi, double* dslip, double* tau, double* u) {
+  auto* m = (QdModel*)h;
+  int Nz = m->Nz;
+  if (V_inout) m->V.assign(V_inout, V_inout + Nz);
+  VarMap var, dvar;
+  var["psi"].assign(psi, psi + Nz);
+  var["slip"].assign(slip, slip + Nz);
+  int e = m->d_dt(t, var, dvar);
+  if (e) return e;
+  if (V_inout) std::memcpy(V_inout, m->V.data(), sizeof(double) * Nz);
+  if (dpsi) std::memcpy(dpsi, dvar["psi"].data(), sizeof(double) * Nz);
+  if (dslip) std::memcpy(dslip, dvar["slip"].data(), sizeof(double) * Nz);
+  if (tau) std::memcpy(tau, m->tau.data(), sizeof(double) * Nz);
+  if (u) std::memcpy(u, m->u.data(), sizeof(double) * m->N);
+  return 0;
+}
+
+// Integrate from (psi, slip); on return psi/slip hold the final state, and
+// dts[0..steps) the accepted step sizes.  Returns the number of steps or <0.
+int orc_qd_integrate(void* h, const OrcQdRun* r, double* psi, double* slip, double* V,
+                     double* tau, double* u, double* tEnd, double* dts, int dtsCap,
+                     double* seconds) {
+  auto* m = (QdModel*)h;
+  int Nz = m->Nz;
+  RkSolver rk(r->integrator == 0 ? 3 : 4, r->maxSteps, r->maxTime, r->initDt,
+              r->control == 0 ? "P" : "PID");
+  rk.initT = rk.currT = r->initTime;
+  rk.totTol = r->tol;
+  rk.minDeltaT = r->minDt;
+  rk.maxDeltaT = r->maxDt;
+  rk.normType = r->normType == 0 ? "L2_absolute" : "L2_relative";
+  rk.errInds = {"psi", "slip"};
+  rk.var["psi"].assign(psi, psi + Nz);
+  rk.var["slip"].assign(slip, slip + Nz);
+  if (r->nReplay > 0) {
+    rk.force_dt = 1;
+    rk.replayDt.assign(r->replayDt, r->replayDt + r->nReplay);
+    if (rk.maxNumSteps > r->nReplay) rk.maxNumSteps = r->nReplay;
+  }
+  if (V) m->V.assign(V, V + Nz);
+  auto t0 = std::chrono::steady_clock::now();
+  int e = rk.integrate(m);
+  auto t1 = std::chrono::steady_clock::now();
+  if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+  if (e) return -e;
+  std::memcpy(psi, rk.var["psi"].data(), sizeof(double) * Nz);
+  std::memcpy(slip, rk.var["slip"].data(), sizeof(double) * Nz);
+  if (V) std::memcpy(V, m->V.data(), sizeof(double) * Nz);
+  if (tau) std::memcpy(tau, m->tau.data(), sizeof(double) * Nz);
+  if (u) std::memcpy(u, m->u.data(), sizeof(double) * m->N);
+  if (tEnd) *tEnd = rk.currT;
+  for (int i = 0; i < (int)rk.dtHistory.size() && i < dtsCap; ++i) dts[i] = rk.dtHistory[i];
+  return rk.stepCount;
+}
+long orc_qd_counter(void* h, int which) {
+  auto* m = (QdModel*)h;
+  return which == 0 ? m->nSolves : (which == 1 ? m->cgIters : m->rootIts);
+}
+
+// ------------------------------------------------------------------ FD ---
+int orc_fd_propagate(void* h, double dt, const double* u, const double* uPrev, const double* rho,
+                     const double* ay, double* uNext, double* d2u) {
+  fd_propagate(((OrcSbp*)h)->s, dt, u, uPrev, rho, ay, uNext, d2u);
+  return 0;
+}
+void orc_fd_ay(void* h, const double* cs, int outR, int outT, int outB, int faultL, double* ay) {
+  fd_ay(((OrcSbp*)h)->s, cs, outR, outT, outB, faultL, ay);
+}
+void* orc_fdfault_new(int N, double v0, double f0, double vL, double rootTol, double fts,
+                      int stateLaw, const double* a, const double* b, const double* Dc,
+                      const double* sNEff, const double* rho, const double* alphay,
+                      const double* tau0, const double* slip0) {
+  auto* f = new FdFault;
+  f->N = N; f->v0 = v0; f->f0 = f0; f->vL = vL; f->rootTol = rootTol; f->faultTypeScale = fts;
+  f->stateLaw = stateLaw;
+  f->a.assign(a, a + N); f->b.assign(b, b + N); f->Dc.assign(Dc, Dc + N);
+  f->sNEff.assign(sNEff, sNEff + N); f->rho.assign(rho, rho + N); f->alphay.assign(alphay, alphay + N);
+  f->tau0.assign(tau0, tau0 + N); f->slip0.assign(slip0, slip0 + N);
+  f->psi.assign(N, 0); f->slip.assign(N, 0); f->V.assign(N, 0); f->u.assign(N, 0);
+  f->uPrev.assign(N, 0); f->d2u.assign(N, 0); f->an.assign(N, 0); f->Phi.assign(N, 0);
+  f->fricPen.assign(N, 0); f->strength.assign(N, 0);
+  return f;
+}
+void orc_fdfault_free(void* h) { delete (FdFault*)h; }
+int orc_fdfault_step(void* h, double dt, const double* d2u, double* V_inout, const double* psi,
+                     const double* psiPrev, const double* slip, const double* slipPrev,
+                     double* psiNext, double* slipNext, double* uFault) {
+  auto* f = (FdFault*)h;
+  std::memcpy(f->d2u.data(), d2u, sizeof(double) * f->N);
+  std::memcpy(f->V.data(), V_inout, sizeof(double) * f->N);
+  int e = f->step(dt, psi, psiPrev, slip, slipPrev, psiNext, slipNext);
+  std::memcpy(V_inout, f->V.data(), sizeof(double) * f->N);
+  if (uFault) std::memcpy(uFault, f->u.data(), sizeof(double) * f->N);
+  return e;
+}
+
+void orc_powerlaw(int N, const double* sxy, const double* sxz, const double* T, const double* A,
+                  const double* B, const double* n, const double* mu, const double* sat,
+                  double cap, double* effVisc, double* gxy_t, double* gxz_t) {
+  powerlaw_rates(N, sxy, sxz, T, A, B, n, mu, sat, cap, effVisc, gxy_t, gxz_t);
+}
+
+}  // extern "C"

@@ -1,0 +1,477 @@
+// Oracle restatement of the fault, integrator, wave-step and power-law parts
+// of the SCycle step (test infrastructure only; see oracle.hpp).
+#include "physics.hpp"
+
+#include <algorithm>
+#include <cassert>
+#include <cmath>
+
+namespace orc {
+
+// fault.cpp:2246-2250
+double strength_psi(double sN, double psi, double V, double a, double v0) {
+  return a * sN * std::asinh((double)(0.5 * V / v0) * std::exp(psi / a));
+}
+
+// fault.cpp:1984-1995
+double agingLaw_psi(double psi, double V, double b, double f0, double v0, double Dc) {
+  double A = std::exp((double)(f0 - psi) / b);
+  double dstate = 0.;
+  if (!std::isinf(A) && b > 1e-3) dstate = (double)(b * v0 / Dc) * (A - V / v0);
+  return dstate;
+}
+
+// fault.cpp:2072-2088
+double slipLaw_psi(double psi, double V, double a, double b, double f0, double v0, double Dc) {
+  if (V == 0) return 0.0;
+  double absV = std::fabs(V);
+  double fss = f0 + (a - b) * std::log(absV / v0);
+  double f = a * std::asinh((double)(absV / 2. / v0) * std::exp(psi / a));
+  return -absV / Dc * (f - fss);
+}
+
+// rootFinder.cpp:174-257 (the first residual check at the uninitialised
+// member _x, rootFinder.cpp:182, is not reproduced).
+int bracketed_newton(const std::function<void(double, double*, double*)>& resid, double left,
+                     double right, double x0, double atol, int maxIts, double* out, int* its) {
+  double f, fp, fLeft, fRight, dummy;
+  resid(left, &fLeft, &dummy);
+  resid(right, &fRight, &dummy);
+  *its = 0;
+  if (std::sqrt(fLeft * fLeft) <= atol) { *out = left; return 0; }
+  if (std::sqrt(fRight * fRight) <= atol) { *out = right; return 0; }
+  if (fLeft > 0) {
+    std::swap(left, right);
+    std::swap(fLeft, fRight);
+  }
+  int numIts = 0;
+  double x = x0;
+  double dxOld = std::fabs(right - left);
+  double dx = dxOld;
+  resid(x, &f, &fp);
+  while ((numIts <= maxIts) && (std::fabs(f) >= atol)) {
+    if (((x - right) * fp - f) * ((x - left) * fp - f) > 0.0 ||
+        std::fabs(2.0 * f) > std::fabs(dxOld * fp)) {
+      dxOld = dx;
+      dx = 0.5 * (right - left);
+      x = left + dx;
+    } else {
+      dxOld = dx;
+      dx = f / fp;
+      x -= dx;
+    }
+    resid(x, &f, &fp);
+    if (f < 0.0) left = x; else right = x;
+    numIts++;
+  }
+  *out = x;
+  *its = numIts;
+  return std::fabs(f) > atol ? 1 : 0;
+}
+
+// fault.cpp:954-1059
+int computeVel_qd(int N, const double* eta, const double* tauQS, const double* sN,
+                  const double* psi, const double* a, const double* locked, double v0, double vL,
+                  double rootTol, int maxIts, int jac_mode, double* V, long* rootIts) {
+  int bad = 0;
+  for (int j = 0; j < N; ++j) {
+    if (locked && locked[j] > 0.5) { V[j] = 0.; continue; }
+    if (locked && locked[j] < -0.5) { V[j] = vL; continue; }
+    double left = 0., right = tauQS[j] / eta[j];
+    if (std::isnan(left) || std::isnan(right)) return -1;
+    double out = V[j];
+    if (std::fabs(left - right) < 1e-14) {
+      out = left;
+    } else {
+      auto resid = [&](double vel, double* f, double* J) {
+        double strength = strength_psi(sN[j], psi[j], vel, a[j], v0);
+        double stress = tauQS[j] - eta[j] * vel;
+        *f = strength - stress;
+        double A = a[j] * sN[j];
+        double B = std::exp(psi[j] / a[j]) / (2. * v0);
+        if (jac_mode == 0) *J = A * vel / std::sqrt(B * B * vel * vel + 1.) + eta[j];
+        else *J = A * B / std::sqrt(B * B * vel * vel + 1.) + eta[j];
+      };
+      int its = 0;
+      bad |= bracketed_newton(resid, left, right, V[j], rootTol, maxIts, &out, &its);
+      if (rootIts) *rootIts += its;
+    }
+    V[j] = out;
+  }
+  return bad;
+}
+
+// ------------------------------------------------------------------ RK ----
+static void waxpy(std::vector<double>& w, double a, const std::vector<double>& x,
+                  const std::vector<double>& y) {
+  w.resize(x.size());
+  for (size_t i = 0; i < x.size(); ++i) w[i] = a * x[i] + y[i];
+}
+static void vaxpy(std::vector<double>& y, double a, const std::vector<double>& x) {
+  for (size_t i = 0; i < x.size(); ++i) y[i] = y[i] + a * x[i];
+}
+
+// odeSolver.cpp:395-445 / 842-898
+double RkSolver::computeStepSize(double totErr) {
+  double stepRatio;
+  if (controlType == "P") {
+    stepRatio = kappa * std::pow(totTol / totErr, 1. / (1. + ord));
+  } else {
+    double alpha = 0.49 / ord, beta = 0.34 / ord, gamma = 0.1 / ord;
+    if (stepCount < 4) stepRatio = kappa * std::pow(totTol / totErr, 1. / (1. + ord));
+    else
+      stepRatio = kappa * std::pow(totTol / totErr, alpha) * std::pow(errA[0] / totTol, beta) *
+                  std::pow(totTol / errA[1], gamma);
+  }
+  double dt = stepRatio * deltaT;
+  dt = std::min(deltaT * 5.0, dt);
+  dt = std::min(maxDeltaT, dt);
+  dt = std::max(minDeltaT, dt);
+  return dt;
+}
+
+// odeSolver.cpp:449-501 (RK32, lo=y2 hi=y3) / 902-969 (RK43, lo=y3 hi=y4)
+double RkSolver::computeError(const VarMap& lo, const VarMap& hi) {
+  double tot = 0;
+  for (size_t i = 0; i < errInds.size(); ++i) {
+    const auto& l = lo.at(errInds[i]);
+    const auto& h = hi.at(errInds[i]);
+    if (normType == "L2_absolute") {
+      double s = 0;
+      for (size_t k = 0; k < l.size(); ++k) { double e = -1.0 * h[k] + l[k]; s += e * e; }
+      tot += std::sqrt(s) / (std::sqrt((double)h.size()) * scaleV[i]);
+    } else if (ord == 3) {
+      double s = 0, n = 0;
+      for (size_t k = 0; k < l.size(); ++k) { double e = -1.0 * h[k] + l[k]; s += e * e; n += h[k] * h[k]; }
+      tot += std::sqrt(s) / (std::sqrt(n) * scaleV[i]);
+    } else {
+      double m = -INFINITY;
+      for (size_t k = 0; k < l.size(); ++k) m = std::max(m, std::fabs(-1.0 * h[k] + l[k]) / h[k]);
+      tot += m / scaleV[i];
+    }
+  }
+  return tot;
+}
+
+int RkSolver::integrate(OdeCtx* obj) {
+  if (errInds.empty())
+    for (auto& kv : var) errInds.push_back(kv.first);
+  if (scaleV.empty()) scaleV.assign(errInds.size(), 1.0);
+  if (finalT == initT) return 0;
+  if (deltaT == 0) deltaT = (finalT - initT) / maxNumSteps;
+  if (maxNumSteps == 0) return 0;
+  for (auto& kv : var) dvar[kv.first].assign(kv.second.size(), 0.0);
+  int ierr = obj->d_dt(currT, var, dvar);
+  if (ierr) return ierr;
+  int stop = 0;
+  double totErr = 0;
+  while (stepCount < maxNumSteps && currT < finalT) {
+    stepCount++;
+    int attempt = 0;
+    VarMap y2, y3;
+    while (attempt < 100) {
+      attempt++;
+      if (force_dt) deltaT = replayDt[stepCount - 1];
+      if (currT + deltaT > finalT) deltaT = finalT - currT;
+      if (ord == 3) {
+        // odeSolver.cpp:554-597
+        VarMap k1, f1, k2, f2;
+        for (auto& kv : var) waxpy(k1[kv.first], 0.5 * deltaT, dvar[kv.first], kv.second);
+        for (auto& kv : var) f1[kv.first].assign(kv.second.size(), 0.0);
+        if ((ierr = obj->d_dt(currT + 0.5 * deltaT, k1, f1))) return ierr;
+        for (auto& kv : var) {
+          waxpy(k2[kv.first], -deltaT, dvar[kv.first], kv.second);
+          vaxpy(k2[kv.first], 2 * deltaT, f1[kv.first]);
+        }
+        for (auto& kv : var) f2[kv.first].assign(kv.second.size(), 0.0);
+        if ((ierr = obj->d_dt(currT + deltaT, k2, f2))) return ierr;
+        for (auto& kv : var) {
+          const auto& k = kv.first;
+          waxpy(y2[k], 0.5 * deltaT, dvar[k], kv.second);
+          vaxpy(y2[k], 0.5 * deltaT, f2[k]);
+          waxpy(y3[k], deltaT / 6.0, dvar[k], kv.second);
+          vaxpy(y3[k], 2 * deltaT / 3.0, f1[k]);
+          vaxpy(y3[k], deltaT / 6.0, f2[k]);
+        }
+        totErr = computeError(y2, y3);
+        if (force_dt) break;
+        if (totErr <= totTol) break;
+      } else {
+        // odeSolver.cpp:986-1148 (Kennedy-Carpenter ARK4(3)6L[2]SA, explicit part)
+        const double c2 = 1. / 2., c3 = 83. / 250., c4 = 31. / 50., c5 = 17. / 20., c6 = 1.;
+        const double b1 = 82889. / 524892., b3 = 15625. / 83664., b4 = 69875. / 102672.,
+                     b5 = -2260. / 8211., b6 = 1. / 4.;
+        const double hb1 = 4586570599. / 29645900160., hb3 = 178811875. / 945068544.,
+                     hb4 = 814220225. / 1159782912., hb5 = -3700637. / 11593932.,
+                     hb6 = 61727. / 225920.;
+        const double a21 = 1. / 2., a31 = 13861. / 62500., a32 = 6889. / 62500.;
+        const double a41 = -116923316275. / 2393684061468., a42 = -2731218467317. / 15368042101831.,
+                     a43 = 9408046702089. / 11113171139209.;
+        const double a51 = -451086348788. / 2902428689909., a52 = -2682348792572. / 7519795681897.,
+                     a53 = 12662868775082. / 11960479115383., a54 = 3355817975965. / 11060851509271.;
+        const double a61 = 647845179188. / 3216320057751., a62 = 73281519250. / 8382639484533.,
+                     a63 = 552539513391. / 3454668386233., a64 = 3354512671639. / 8306763924573.,
+                     a65 = 4040. / 17871.;
+        VarMap k2, k3, k4, k5, k6, f2, f3, f4, f5, f6;
+        const VarMap& f1 = dvar;
+        auto zero = [&](VarMap& m) { for (auto& kv : var) m[kv.first].assign(kv.second.size(), 0.0); };
+        for (auto& kv : var) waxpy(k2[kv.first], a21 * deltaT, f1.at(kv.first), kv.second);
+        zero(f2);
+        if ((ierr = obj->d_dt(currT + c2 * deltaT, k2, f2))) return ierr;
+        for (auto& kv : var) {
+          waxpy(k3[kv.first], a31 * deltaT, f1.at(kv.first), kv.second);
+          vaxpy(k3[kv.first], a32 * deltaT, f2[kv.first]);
+        }
+        zero(f3);
+        if ((ierr = obj->d_dt(currT + c3 * deltaT, k3, f3))) return ierr;
+        for (auto& kv : var) {
+          const auto& k = kv.first;
+          waxpy(k4[k], a41 * deltaT, f1.at(k), kv.second);
+          vaxpy(k4[k], a42 * deltaT, f2[k]);
+          vaxpy(k4[k], a43 * deltaT, f3[k]);
+        }
+        zero(f4);
+        if ((ierr = obj->d_dt(currT + c4 * deltaT, k4, f4))) return ierr;
+        for (auto& kv : var) {
+          const auto& k = kv.first;
+          waxpy(k5[k], a51 * deltaT, f1.at(k), kv.second);
+          vaxpy(k5[k], a52 * deltaT, f2[k]);
+          vaxpy(k5[k], a53 * deltaT, f3[k]);
+          vaxpy(k5[k], a54 * deltaT, f4[k]);
+        }
+        zero(f5);
+        if ((ierr = obj->d_dt(currT + c5 * deltaT, k5, f5))) return ierr;
+        for (auto& kv : var) {
+          const auto& k = kv.first;
+          waxpy(k6[k], a61 * deltaT, f1.at(k), kv.second);
+          vaxpy(k6[k], a62 * deltaT, f2[k]);
+          vaxpy(k6[k], a63 * deltaT, f3[k]);
+          vaxpy(k6[k], a64 * deltaT, f4[k]);
+          vaxpy(k6[k], a65 * deltaT, f5[k]);
+        }
+        zero(f6);
+        if ((ierr = obj->d_dt(currT + c6 * deltaT, k6, f6))) return ierr;
+        for (auto& kv : var) {
+          const auto& k = kv.first;
+          waxpy(y2[k], hb1 * deltaT, f1.at(k), kv.second);  // y2 holds the 3rd-order y3
+          vaxpy(y2[k], hb3 * deltaT, f3[k]);
+          vaxpy(y2[k], hb4 * deltaT, f4[k]);
+          vaxpy(y2[k], hb5 * deltaT, f5[k]);
+          vaxpy(y2[k], hb6 * deltaT, f6[k]);
+          waxpy(y3[k], b1 * deltaT, f1.at(k), kv.second);   // y3 holds the 4th-order y4
+          vaxpy(y3[k], b3 * deltaT, f3[k]);
+          vaxpy(y3[k], b4 * deltaT, f4[k]);
+          vaxpy(y3[k], b5 * deltaT, f5[k]);
+          vaxpy(y3[k], b6 * deltaT, f6[k]);
+        }
+        totErr = computeError(y2, y3);
+        if (force_dt) break;
+        if (totErr < totTol) break;
+      }
+      deltaT = computeStepSize(totErr);
+      if (minDeltaT == deltaT) break;
+      numRejected++;
+    }
+    currT = currT + deltaT;
+    for (auto& kv : var) {
+      kv.second = y3[kv.first];
+      std::fill(dvar[kv.first].begin(), dvar[kv.first].end(), 0.0);
+    }
+    if ((ierr = obj->d_dt(currT, var, dvar))) return ierr;
+    if (totErr != 0.0) newDeltaT = computeStepSize(totErr);
+    errA[1] = errA[0];
+    errA[0] = totErr;
+    dtHistory.push_back(deltaT);
+    obj->timeMonitor(currT, deltaT, stepCount, stop);
+    deltaT = newDeltaT;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------- QD -----
+int QdModel::init(const QdConfig& cfg, const double* mu, const double* y, const double* z) {
+  c = cfg;
+  int e = sbp2d_build(sbp, cfg.sbp, mu, y, z);
+  if (e) return e;
+  Ny = cfg.sbp.Ny;
+  Nz = cfg.sbp.Nz;
+  N = Ny * Nz;
+  bcL.assign(Nz, 0.0); bcR.assign(Nz, 0.0); bcRShift.assign(Nz, 0.0);
+  bcT.assign(Ny, 0.0); bcB.assign(Ny, 0.0);
+  rhs.assign(N, 0.0); u.assign(N, 0.0); sxy.assign(N, 0.0);
+  tauQS.assign(Nz, 0.0); tau.assign(Nz, 0.0); V.assign(Nz, 0.0); strength.assign(Nz, 0.0);
+  psi.assign(Nz, 0.0); slip.assign(Nz, 0.0);
+  if (locked.empty()) locked.assign(Nz, 0.0);
+  if (prestress.empty()) prestress.assign(Nz, 0.0);
+  if (c.solver == 0) {
+    if (chol.factor(sbp.A)) return 10;
+  }
+  return 0;
+}
+
+// solveMomentumBalance (strikeSlip_linearElastic_qd.cpp:874-891)
+int QdModel::solveMomentumBalance(double /*t*/) {
+  sbp2d_set_rhs(sbp, rhs.data(), bcL.data(), bcR.data(), bcT.data(), bcB.data());
+  if (c.solver == 0) {
+    chol.solve(rhs.data(), u.data());
+  } else {
+    PcgStats st = pcg(sbp.A, rhs.data(), u.data(), c.cg_rtol, c.cg_maxit);
+    cgIters += st.iters;
+    if (st.relres > c.cg_rtol) return 20;
+  }
+  nSolves++;
+  sbp2d_muxDy(sbp, u.data(), sxy.data());
+  return 0;
+}
+
+// strikeSlip_linearElastic_qd.cpp:756-798 + Fault_qd::d_dt fault.cpp:861-918
+int QdModel::d_dt(double t, const VarMap& var, VarMap& dvar) {
+  const auto& vslip = var.at("slip");
+  for (int k = 0; k < Nz; ++k) bcL[k] = vslip[k] * (1.0 / c.faultTypeScale);
+  if (c.remoteLoadingR)
+    for (int k = 0; k < Nz; ++k) bcR[k] = c.vL * t / c.faultTypeScale + 1.0 * bcRShift[k];
+  psi = var.at("psi");
+  slip = vslip;
+  int e = solveMomentumBalance(t);
+  if (e) return e;
+  for (int k = 0; k < Nz; ++k) tauQS[k] = sxy[k];
+  for (int k = 0; k < Nz; ++k) tauQS[k] = tauQS[k] + 1.0 * prestress[k];
+  if (computeVel_qd(Nz, eta.data(), tauQS.data(), sN.data(), psi.data(), a.data(), locked.data(),
+                    c.v0, c.vL, c.rootTol, c.maxNumIts, c.jac_mode, V.data(), &rootIts))
+    return 30;
+  dvar["slip"] = V;
+  auto& dpsi = dvar["psi"];
+  dpsi.resize(Nz);
+  for (int k = 0; k < Nz; ++k)
+    dpsi[k] = c.stateLaw == 0 ? agingLaw_psi(psi[k], V[k], b[k], c.f0, c.v0, Dc[k])
+                              : slipLaw_psi(psi[k], V[k], a[k], b[k], c.f0, c.v0, Dc[k]);
+  for (int k = 0; k < Nz; ++k) tau[k] = -1.0 * (eta[k] * V[k]) + tauQS[k];
+  for (int k = 0; k < Nz; ++k) strength[k] = strength_psi(sN[k], psi[k], V[k], a[k], c.v0);
+  return 0;
+}
+
+// ---------------------------------------------------------------- FD -----
+void fd_propagate(const Sbp2D& s, double dt, const double* u, const double* uPrev,
+                  const double* rho, const double* ay, double* uNext, double* d2u) {
+  const int N = s.N();
+  std::vector<double> temp(N), t2(N);
+  spmv(s.A, u, temp.data());
+  spmv(s.Hinv, temp.data(), d2u);
+  if (s.d.var_grid) {
+    spmv(s.Jinv, d2u, t2.data());
+    std::copy(t2.begin(), t2.end(), d2u);
+  }
+  for (int i = 0; i < N; ++i) {
+    double c1 = dt * dt / rho[i];
+    double c2 = dt * ay[i] - 1.0;
+    double c3 = dt * ay[i] + 1.0;
+    uNext[i] = (c1 * d2u[i] + 2. * u[i] + c2 * uPrev[i]) / c3;
+  }
+}
+
+void fd_ay(const Sbp2D& s, const double* cs, int outR, int outT, int outB, int faultL, double* ay) {
+  const int Ny = s.d.Ny, Nz = s.d.Nz, N = Ny * Nz;
+  for (int i = 0; i < N; ++i) {
+    double v = 0;
+    if ((i / Nz == Ny - 1) && outR) v += 0.5 / s.h11y;
+    if ((i % Nz == 0) && outT) v += 0.5 / s.h11z;
+    if (((i + 1) % Nz == 0) && outB) v += 0.5 / s.h11z;
+    if ((i / Nz == 0) && faultL) v += 0.5 / s.h11y;
+    ay[i] = v * cs[i];
+  }
+}
+
+// Fault_fd::d_dt (fault.cpp:1436-1523), setPhi (:1527-1575), ComputeVel_fd
+// (:1588-1685), ComputeAging_fd / ComputeSlipLaw_fd (:1697-1879)
+int FdFault::step(double dt, const double* psiCur, const double* psiPrev, const double* slipCur,
+                  const double* slipPrev, double* psiNext, double* slipNext) {
+  int bad = 0;
+  for (int j = 0; j < N; ++j) {
+    psi[j] = psiCur[j];
+    slip[j] = slipCur[j];
+    uPrev[j] = (-1.0 * slip0[j] + slipPrev[j]) * (1.0 / faultTypeScale);
+    u[j] = (-1.0 * slip0[j] + slipCur[j]) * (1.0 / faultTypeScale);
+  }
+  for (int j = 0; j < N; ++j) {
+    an[j] = d2u[j] + tau0[j] / alphay[j];
+    Phi[j] = 2.0 / dt * (u[j] - uPrev[j]) + dt * an[j] / rho[j];
+    fricPen[j] = dt / alphay[j] / rho[j];
+  }
+  for (int j = 0; j < N; ++j) {
+    if (locked.size() && locked[j] > 0.5) { V[j] = 0.; continue; }
+    if (locked.size() && locked[j] < -0.5) { V[j] = vL; continue; }
+    double left = 0., right = std::fabs(Phi[j]);
+    double out;
+    if (std::fabs(left - right) < 1e-14) out = left;
+    else {
+      auto resid = [&](double vel, double* f, double* J) {
+        double cons = fricPen[j] * strength_psi(sNEff[j], psi[j], vel, a[j], v0);
+        double P = Phi[j] < 0 ? -Phi[j] : Phi[j];
+        *f = cons - (P - vel);
+        double A = a[j] * sNEff[j];
+        double B = std::exp(psi[j] / a[j]) / (2. * v0);
+        *J = 1 + fricPen[j] * A * B / std::sqrt(1. + B * B * vel * vel);
+      };
+      int its;
+      bad |= bracketed_newton(resid, left, right, std::fabs(V[j]), rootTol, maxNumIts, &out, &its);
+      rootIts += its;
+    }
+    V[j] = out;
+  }
+  for (int j = 0; j < N; ++j) {
+    if (V[j] < 1e-14) continue;
+    double fric = strength_psi(sNEff[j], psi[j], V[j], a[j], v0);
+    double alpha = 1.0 / (rho[j] * alphay[j]) * fric / V[j];
+    double A = 1.0 + alpha * dt;
+    V[j] = Phi[j] / (1. + dt * alpha);
+    u[j] = (2. * u[j] + (an[j] * dt * dt / rho[j]) + (dt * alpha - 1.) * uPrev[j]) / A;
+  }
+  for (int j = 0; j < N; ++j) {
+    auto resid = [&](double state, double* f, double* J) {
+      double mid = (psiPrev[j] + state) / 2.0;
+      if (stateLaw == 0) {
+        double G = agingLaw_psi(mid, V[j], b[j], f0, v0, Dc[j]);
+        *f = -2 * dt * G + (state - psiPrev[j]);
+        *J = 1 + dt * v0 / Dc[j] * std::exp((f0 - mid) / b[j]);
+      } else {
+        double G = slipLaw_psi(mid, V[j], a[j], b[j], f0, v0, Dc[j]);
+        *f = -2 * dt * G + (state - psiPrev[j]);
+        double A = std::fabs(V[j]) / 2. / v0 * std::exp((psiPrev[j] + state) / 2.0 / a[j]);
+        *J = 1 + dt * std::fabs(V[j]) / Dc[j] * A / std::sqrt(1 + A * A);
+      }
+    };
+    int its;
+    double out;
+    bad |= bracketed_newton(resid, -2., 2., psiCur[j], rootTol, maxNumIts, &out, &its);
+    rootIts += its;
+    psiNext[j] = out;
+    psi[j] = out;
+  }
+  for (int j = 0; j < N; ++j) {
+    slip[j] = faultTypeScale * u[j] + slip0[j];
+    slipNext[j] = slip[j];
+    strength[j] = strength_psi(sNEff[j], psi[j], V[j], a[j], v0);
+  }
+  return bad;
+}
+
+// ---------------------------------------------------------- power law ----
+void powerlaw_rates(int N, const double* sxy, const double* sxz, const double* T,
+                    const double* A, const double* B, const double* n, const double* mu,
+                    const double* sat, double effViscCap, double* effVisc, double* gxy_t,
+                    double* gxz_t) {
+  for (int i = 0; i < N; ++i) {
+    double sdev = sxy[i] * sxy[i];
+    if (sxz) sdev += sxz[i] * sxz[i];
+    sdev = std::sqrt(std::fabs(sdev));
+    double inv = 1e3 * A[i] * std::pow(sdev, n[i] - 1.0) * std::exp(-B[i] / T[i]);
+    double ev = 1.0 / effViscCap;
+    ev = ev + 1.0 * inv;
+    ev = 1.0 / ev;
+    effVisc[i] = ev;
+    gxy_t[i] = (mu[i] * (sat ? sat[i] : 0.0) + 1.0 * sxy[i]) / ev;
+    if (sxz && gxz_t) gxz_t[i] = sxz[i] / ev;
+  }
+}
+
+}  // namespace orc

@@ -1,0 +1,147 @@
+"""ctypes binding of libscycle_hip.so (the C ABI declared in include/scycle_hip.h).
+
+The product path is the HIP library: if it is not built, or no GPU is visible
+when a compute entry point is called, this module raises instead of falling
+back to anything else.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libscycle_hip.so")
+
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+class ScError(RuntimeError):
+    pass
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("Ny", C.c_int), ("Nz", C.c_int), ("Ly", C.c_double), ("Lz", C.c_double),
+                ("order", C.c_int), ("var_grid", C.c_int), ("compat", C.c_int),
+                ("laplace", C.c_int), ("bcR", C.c_int), ("bcT", C.c_int), ("bcL", C.c_int),
+                ("bcB", C.c_int)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("mode", C.c_int), ("rtol", C.c_double), ("inner_rtol", C.c_double),
+                ("max_iters", C.c_int), ("max_outer", C.c_int), ("check_every", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("outer", C.c_int), ("converged", C.c_int),
+                ("relres", C.c_double)]
+
+
+class FaultQdArgs(C.Structure):
+    _fields_ = [("N", C.c_int), ("v0", C.c_double), ("vL", C.c_double), ("f0", C.c_double),
+                ("rootTol", C.c_double), ("maxNumIts", C.c_int), ("stateLaw", C.c_int),
+                ("jac_mode", C.c_int),
+                ("eta_d", _vp), ("sN_d", _vp), ("a_d", _vp), ("b_d", _vp), ("Dc_d", _vp),
+                ("locked_d", _vp), ("prestress_d", _vp), ("psi_d", _vp), ("tauQS_d", _vp),
+                ("V_d", _vp), ("dpsi_d", _vp), ("tau_d", _vp), ("strength_d", _vp),
+                ("status_d", _vp)]
+
+
+class QdDesc(C.Structure):
+    _fields_ = [("grid", GridDesc), ("remoteLoadingR", C.c_int), ("faultTypeScale", C.c_double),
+                ("vL", C.c_double), ("f0", C.c_double), ("v0", C.c_double),
+                ("rootTol", C.c_double), ("maxNumIts", C.c_int), ("stateLaw", C.c_int),
+                ("jac_mode", C.c_int), ("solve", SolveOpts)]
+
+
+class QdRun(C.Structure):
+    _fields_ = [("integrator", C.c_int), ("control", C.c_int), ("normType", C.c_int),
+                ("maxSteps", C.c_int), ("initTime", C.c_double), ("maxTime", C.c_double),
+                ("initDt", C.c_double), ("minDt", C.c_double), ("maxDt", C.c_double),
+                ("tol", C.c_double), ("nReplay", C.c_int), ("replayDt_h", _dp)]
+
+
+class QdCounters(C.Structure):
+    _fields_ = [("d_dt_calls", C.c_longlong), ("solves", C.c_longlong),
+                ("cg_iters", C.c_longlong), ("outer_sweeps", C.c_longlong),
+                ("rejected_steps", C.c_longlong), ("solve_seconds", C.c_double),
+                ("total_seconds", C.c_double)]
+
+
+# Every symbol include/scycle_hip.h declares, with (restype, argtypes).
+SIGNATURES = {
+    "sc_last_error": (C.c_char_p, []),
+    "sc_version": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "sc_malloc": (C.c_int, [C.POINTER(_vp), C.c_ulonglong]),
+    "sc_free": (C.c_int, [_vp]),
+    "sc_memcpy_h2d": (C.c_int, [_vp, _vp, C.c_ulonglong]),
+    "sc_memcpy_d2h": (C.c_int, [_vp, _vp, C.c_ulonglong]),
+    "sc_ctx_create": (C.c_int, [C.POINTER(GridDesc), _dp, _dp, _dp, C.POINTER(_vp)]),
+    "sc_ctx_destroy": (C.c_int, [_vp]),
+    "sc_ctx_sync": (C.c_int, [_vp]),
+    "sc_ctx_stream": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "sc_update_var_coeff": (C.c_int, [_vp, _dp]),
+    "sc_change_bc_types": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "sc_geth11": (C.c_int, [_vp, _dp, _dp]),
+    "sc_get_metric": (C.c_int, [_vp, _dp, _dp]),
+    "sc_sbp_apply": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "sc_sbp_diag": (C.c_int, [_vp, _vp]),
+    "sc_sbp_set_rhs": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "sc_sbp_dy": (C.c_int, [_vp, _vp, _vp]),
+    "sc_sbp_mux_dy": (C.c_int, [_vp, _vp, _vp]),
+    "sc_sbp_dz": (C.c_int, [_vp, _vp, _vp]),
+    "sc_sbp_mux_dz": (C.c_int, [_vp, _vp, _vp]),
+    "sc_sbp_h": (C.c_int, [_vp, _vp, _vp]),
+    "sc_sbp_hinv": (C.c_int, [_vp, _vp, _vp]),
+    "sc_bench_apply": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
+    "sc_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "sc_fault_qd_rates": (C.c_int, [_vp, C.POINTER(FaultQdArgs)]),
+    "sc_qd_create": (C.c_int, [C.POINTER(QdDesc)] + [_dp] * 11 + [C.POINTER(_vp)]),
+    "sc_qd_create_from_file": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
+    "sc_qd_destroy": (C.c_int, [_vp]),
+    "sc_qd_ctx": (_vp, [_vp]),
+    "sc_qd_set_state": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "sc_qd_get_state": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    "sc_qd_get_u": (C.c_int, [_vp, _dp]),
+    "sc_qd_ddt": (C.c_int, [_vp, C.c_double, _dp, _dp]),
+    "sc_qd_integrate": (C.c_int, [_vp, C.POINTER(QdRun), _dp, C.POINTER(C.c_int), _dp, C.c_int]),
+    "sc_qd_get_counters": (C.c_int, [_vp, C.POINTER(QdCounters)]),
+    "sc_qd_file_run": (C.c_int, [_vp, C.POINTER(QdRun)]),
+    "sc_qd_solve_ss": (C.c_int, [_vp, _dp, C.c_int, _dp]),
+    "sc_in_get": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]),
+    "sc_in_get_vector": (C.c_int, [C.c_char_p, C.c_char_p, _dp, C.c_int, C.POINTER(C.c_int)]),
+    "sc_vec_count": (C.c_int, [C.c_char_p, C.POINTER(C.c_int)]),
+    "sc_vec_read": (C.c_int, [C.c_char_p, C.c_int, _dp, C.c_int, C.POINTER(C.c_int)]),
+    "sc_vec_write": (C.c_int, [C.c_char_p, _dp, C.c_int, C.c_int]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libscycle_hip.so; raises ScError if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ScError(f"libscycle_hip.so is not built ({LIB_PATH}); run "
+                          "`python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != 0:
+        msg = load().sc_last_error()
+        raise ScError(f"scycle_hip error {code}: {msg.decode() if msg else ''}")
+
+
+def dptr(a):
+    """double* of a contiguous float64 numpy array (or None)."""
+    if a is None:
+        return None
+    import numpy as np
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)

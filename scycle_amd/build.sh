@@ -1,0 +1,26 @@
+#!/usr/bin/env bash
+# Builds libscycle_hip.so for gfx950 in-tree (scycle_amd/libscycle_hip.so).
+set -euo pipefail
+here="$(cd "$(dirname "$0")" && pwd)"
+src="$here/csrc"
+out="$here/libscycle_hip.so"
+HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
+ARCH="${SC_OFFLOAD_ARCH:-gfx950}"
+objdir="$here/_build"
+mkdir -p "$objdir"
+flags=(-O3 -std=c++17 -fPIC --offload-arch="$ARCH" -Wall -Wno-unused-function
+       -Xclang -target-feature -Xclang -packed-fp32-ops -I"$here/../include")
+objs=()
+for f in "$src"/*.hip "$src"/*.cpp; do
+  o="$objdir/$(basename "$f").o"
+  if [[ "$f" == *.hip ]]; then
+    "$HIPCC" "${flags[@]}" -x hip -c "$f" -o "$o" &
+  else
+    "$HIPCC" "${flags[@]}" -c "$f" -o "$o" &
+  fi
+  objs+=("$o")
+done
+wait
+"$HIPCC" -shared --offload-arch="$ARCH" -fPIC -o "$out.tmp" "${objs[@]}"
+mv "$out.tmp" "$out"
+echo "built $out"

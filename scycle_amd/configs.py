@@ -1,0 +1,131 @@
+"""Synthetic inputs of the benchmark configurations (quasi-dynamic strike-slip
+cycle on a 2-D antiplane domain), built the way the reference builds them from
+a .in deck: depth profiles are piecewise-linear through (vals, depths) knots
+with linear extrapolation past the last knot (genFuncs.cpp:574-617 setVec).
+
+Units: km, GPa, g/cm^3, MPa, m, m/s, s.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+MASK64 = (1 << 64) - 1
+
+
+def splitmix64(seed, n):
+    """n uniforms in [0,1) from SplitMix64 (53-bit mantissa)."""
+    out = np.empty(n)
+    x = seed & MASK64
+    for i in range(n):
+        x = (x + 0x9E3779B97F4A7C15) & MASK64
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        z = z ^ (z >> 31)
+        out[i] = (z >> 11) * (1.0 / (1 << 53))
+    return out
+
+
+def set_vec(coord, vals, depths):
+    """Reference setVec: piecewise-linear interpolation, last segment extended."""
+    coord = np.asarray(coord, dtype=np.float64)
+    out = np.full(coord.shape, float(vals[0]))
+    if coord.size == 1:
+        return out
+    n = len(depths)
+    for ind in range(n - 1):
+        z0, z1, v0, v1 = depths[ind], depths[ind + 1], vals[ind], vals[ind + 1]
+        inside = (coord >= z0) & (coord <= z1)
+        beyond = (coord > z1) & (ind == n - 2)
+        m = inside | beyond
+        out[m] = (v1 - v0) / (z1 - z0) * (coord[m] - z0) + v0
+    return out
+
+
+@dataclass
+class QdConfig:
+    name: str
+    Ny: int
+    Nz: int
+    Ly: float
+    Lz: float
+    order: int
+    var_grid: bool
+    mu: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    a: np.ndarray
+    b: np.ndarray
+    Dc: np.ndarray
+    sN: np.ndarray
+    eta: np.ndarray
+    psi0: np.ndarray
+    slip0: np.ndarray
+    locked: np.ndarray = None
+    prestress: np.ndarray = None
+    bcRShift: np.ndarray = None
+    mat_bc: tuple = (0, 1, 0, 1)          # R, T, L, B: remoteLoading, freeSurface, symmFault, freeSurface
+    remote_loading_R: bool = True
+    fault_type_scale: float = 2.0
+    vL: float = 1e-9
+    f0: float = 0.6
+    v0: float = 1e-6
+    root_tol: float = 1e-12
+    state_law: int = 0                      # 0 aging, 1 slip
+    steps: int = 2000
+    integrator: str = "RK32"
+    tol: float = 1e-8
+    min_dt: float = 0.0
+    init_dt: float = 1e-3
+    extra: dict = field(default_factory=dict)
+
+
+def _grid(Ny, Nz, Ly, Lz, zstretch=None):
+    q = np.repeat(np.arange(Ny) * (1.0 / (Ny - 1.0)), Nz)
+    r = np.tile(np.arange(Nz) * (1.0 / (Nz - 1.0)), Ny)
+    y = Ly * q
+    if zstretch:
+        z = Lz * np.sinh(zstretch * r) / np.sinh(zstretch)
+    else:
+        z = Lz * r
+    return y, z
+
+
+def strike_slip(Ny, Nz, order=2, var_grid=False, zstretch=None, layered_mu=False,
+                Ly=30.0, Lz=30.0, seed=0x5C7C1E, name="custom", steps=2000):
+    """The C1/C2 physics: aging law, f0=0.6, v0=1e-6, b=0.02, Dc=8mm, sN=50 MPa,
+    a = 0.015 above 12 km ramping to 0.025 at 17 km and below; vL = 1e-9;
+    psi0 = psi_ss(vL)*(1 + eps), eps uniform in [-1%, 1%] from SplitMix64."""
+    y, z = _grid(Ny, Nz, Ly, Lz, zstretch)
+    zf = z[:Nz]
+    if layered_mu:
+        t = np.clip(z / 4.0, 0.0, 1.0)
+        mu = np.where(z >= 4.0, 36.0, 30.0 + 6.0 * (3 * t ** 2 - 2 * t ** 3))
+    else:
+        mu = np.full(Ny * Nz, 36.0)
+    rho = np.full(Nz, 3.0)
+    mu_f = mu[:Nz]
+    a = set_vec(zf, [0.015, 0.015, 0.025, 0.025], [0, 12, 17, 30])
+    b = np.full(Nz, 0.02)
+    Dc = np.full(Nz, 8e-3)
+    sN = np.full(Nz, 50.0)
+    fts = 2.0
+    eta = np.sqrt(mu_f * rho) / fts
+    f0, v0, vL = 0.6, 1e-6, 1e-9
+    psi_ss = f0 - b * np.log(vL / v0)
+    eps = 0.02 * splitmix64(seed, Nz) - 0.01
+    psi0 = psi_ss * (1.0 + eps)
+    return QdConfig(name=name, Ny=Ny, Nz=Nz, Ly=Ly, Lz=Lz, order=order, var_grid=var_grid,
+                    mu=mu, y=y, z=z, a=a, b=b, Dc=Dc, sN=sN, eta=eta, psi0=psi0,
+                    slip0=np.zeros(Nz), steps=steps)
+
+
+def config_c1(Ny=101, Nz=101):
+    """101^2, order 2, constant grid, constant mu = 36 GPa."""
+    return strike_slip(Ny, Nz, order=2, var_grid=False, name=f"C1-{Ny}x{Nz}", steps=2000)
+
+
+def config_c2(Ny=1001, Nz=1001):
+    """1001^2, order 4, sinh-stretched z (b = 5), layered mu 30 -> 36 GPa over 4 km."""
+    return strike_slip(Ny, Nz, order=4, var_grid=True, zstretch=5.0, layered_mu=True,
+                       name=f"C2-{Ny}x{Nz}", steps=500)

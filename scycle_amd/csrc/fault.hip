@@ -1,0 +1,152 @@
+// Fault-side kernels of the quasi-dynamic step: rate-and-state slip-velocity
+// root solve, state evolution, shear stress and strength (Fault_qd::d_dt,
+// fault.cpp:861-918), plus the small vector kernels of the explicit
+// Runge-Kutta driver (odeSolver.cpp) and the boundary-data update of
+// StrikeSlip_LinearElastic_qd::d_dt (strikeSlip_linearElastic_qd.cpp:763-770).
+// All of it is FP64: the argument of asinh reaches ~1e30.
+#include <cmath>
+
+#include "fault_dev.hpp"
+#include "sc_internal.hpp"
+
+namespace sc {
+
+__global__ __launch_bounds__(256) void k_fault_qd(sc_fault_qd_args a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.N) return;
+  const double tq = a.tauQS_d[k] + 1.0 * (a.prestress_d ? a.prestress_d[k] : 0.0);
+  a.tauQS_d[k] = tq;
+  const double psi = a.psi_d[k], aa = a.a_d[k], sN = a.sN_d[k], eta = a.eta_d[k];
+  const double lk = a.locked_d ? a.locked_d[k] : 0.0;
+  double V = a.V_d[k];
+  int bad = 0;
+  if (lk > 0.5) {
+    V = 0.;
+  } else if (lk < -0.5) {
+    V = a.vL;
+  } else {
+    const double left = 0., right = tq / eta;
+    if (isnan(right)) {
+      bad = SC_ERR_NONFINITE;
+    } else if (fabs(left - right) < 1e-14) {
+      V = left;
+    } else {
+      QdResid r{psi, aa, sN, eta, tq, a.v0, a.jac_mode};
+      double out;
+      int its;
+      if (bracketed_newton(r, left, right, V, a.rootTol, a.maxNumIts, &out, &its)) bad = SC_ERR_NOCONV;
+      V = out;
+    }
+  }
+  a.V_d[k] = V;
+  double dpsi = a.stateLaw == SC_AGING_LAW ? aging_law_psi(psi, V, a.b_d[k], a.f0, a.v0, a.Dc_d[k])
+                                           : slip_law_psi(psi, V, aa, a.b_d[k], a.f0, a.v0, a.Dc_d[k]);
+  if (!isfinite(dpsi)) bad = SC_ERR_NONFINITE;
+  a.dpsi_d[k] = dpsi;
+  a.tau_d[k] = -1.0 * (eta * V) + tq;
+  a.strength_d[k] = strength_psi(sN, psi, V, aa, a.v0);
+  if (bad && a.status_d) atomicOr(a.status_d, bad);
+}
+
+int fault_qd_rates(hipStream_t s, const sc_fault_qd_args& a) {
+  if (a.N <= 0) return SC_OK;
+  hipLaunchKernelGGL(k_fault_qd, dim3((a.N + 255) / 256), dim3(256), 0, s, a);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
+// ------------------------------------------------------ QD driver helpers
+// bcL = slip / fts ; bcR = vL*t/fts + bcRShift
+__global__ void k_qd_bc(int Nz, const double* __restrict__ slip, const double* __restrict__ shift,
+                        double t, double vL, double fts, int remote, double* __restrict__ bcL,
+                        double* __restrict__ bcR) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= Nz) return;
+  bcL[k] = slip[k] * (1.0 / fts);
+  if (remote) bcR[k] = vL * t / fts + 1.0 * shift[k];
+}
+
+// out = base + sum_i c[i]*f[i]   (VecWAXPY followed by VecAXPYs, odeSolver.cpp)
+struct LinComb {
+  const double* f[6];
+  double c[6];
+  int nt;
+};
+__global__ void k_lincomb(int n, double* __restrict__ out, const double* __restrict__ base,
+                          LinComb lc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = lc.c[0] * lc.f[0][i] + base[i];
+  for (int t = 1; t < lc.nt; ++t) v = v + lc.c[t] * lc.f[t][i];
+  out[i] = v;
+}
+
+// Error measures over one vector pair (lo, hi): [sum (lo-hi)^2, sum hi^2, max |lo-hi|/hi]
+__global__ __launch_bounds__(256) void k_err(int n, const double* __restrict__ lo,
+                                             const double* __restrict__ hi, double* __restrict__ out) {
+  __shared__ double s0[256], s1[256], s2[256];
+  double a = 0, b = 0, m = -INFINITY;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double e = -1.0 * hi[i] + lo[i];
+    a += e * e;
+    b += hi[i] * hi[i];
+    m = fmax(m, fabs(e) / hi[i]);
+  }
+  s0[threadIdx.x] = a;
+  s1[threadIdx.x] = b;
+  s2[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+      s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s0[0];
+    out[1] = s1[0];
+    out[2] = s2[0];
+  }
+}
+
+int launch_qd_bc(hipStream_t s, int Nz, const double* slip, const double* shift, double t,
+                 double vL, double fts, int remote, double* bcL, double* bcR) {
+  hipLaunchKernelGGL(k_qd_bc, dim3((Nz + 255) / 256), dim3(256), 0, s, Nz, slip, shift, t, vL, fts,
+                     remote, bcL, bcR);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
+int launch_lincomb(hipStream_t s, int n, double* out, const double* base, int nt,
+                   const double* const* f, const double* c) {
+  LinComb lc{};
+  lc.nt = nt;
+  for (int i = 0; i < nt; ++i) {
+    lc.f[i] = f[i];
+    lc.c[i] = c[i];
+  }
+  hipLaunchKernelGGL(k_lincomb, dim3((n + 255) / 256), dim3(256), 0, s, n, out, base, lc);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
+int launch_err(hipStream_t s, int n, const double* lo, const double* hi, double* out3) {
+  hipLaunchKernelGGL(k_err, dim3(1), dim3(256), 0, s, n, lo, hi, out3);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
+}  // namespace sc
+
+extern "C" int sc_fault_qd_rates(sc_ctx* h, const sc_fault_qd_args* args) {
+  if (!h || !args) return sc::fail(SC_ERR_ARG, "sc_fault_qd_rates: null argument");
+  const sc_fault_qd_args& a = *args;
+  if (!a.eta_d || !a.sN_d || !a.a_d || !a.b_d || !a.Dc_d || !a.psi_d || !a.tauQS_d || !a.V_d ||
+      !a.dpsi_d || !a.tau_d || !a.strength_d)
+    return sc::fail(SC_ERR_ARG, "sc_fault_qd_rates: null array");
+  if (a.stateLaw != SC_AGING_LAW && a.stateLaw != SC_SLIP_LAW)
+    return sc::fail(SC_ERR_ARG, "sc_fault_qd_rates: stateLaw must be aging or slip");
+  return sc::fault_qd_rates(h->c.stream, a);
+}

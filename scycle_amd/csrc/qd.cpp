@@ -1,0 +1,421 @@
+// Quasi-dynamic mediator + explicit adaptive Runge-Kutta driver on the GPU.
+//
+// QdDriver plays StrikeSlip_LinearElastic_qd (strikeSlip_linearElastic_qd.cpp)
+// with its LinearElastic and Fault_qd members; integrate() is the host loop of
+// OdeSolver RK32/RK43 (odeSolver.cpp:505-627, 973-1175) with every vector on
+// the device.  Per d_dt: boundary data -> setRhs -> solve (warm start from the
+// previous displacement, as KSPSetInitialGuessNonzero) -> sigma_xy on the
+// fault line -> rate-and-state kernel.  The host reads back one scalar block
+// per RK attempt (the error estimate) and the CG convergence flags.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "qd_internal.hpp"
+
+namespace sc {
+
+int launch_qd_bc(hipStream_t s, int Nz, const double* slip, const double* shift, double t,
+                 double vL, double fts, int remote, double* bcL, double* bcR);
+int launch_lincomb(hipStream_t s, int n, double* out, const double* base, int nt,
+                   const double* const* f, const double* c);
+int launch_err(hipStream_t s, int n, const double* lo, const double* hi, double* out3);
+
+namespace {
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+QdDriver::~QdDriver() {
+  if (ctx) {
+    (void)hipStreamSynchronize(ctx->c.stream);
+    for (void* p : owned) (void)hipFree(p);
+    if (host_scratch) (void)hipHostFree(host_scratch);
+    sc_ctx_destroy(ctx);
+  }
+}
+
+double* QdDriver::dalloc(size_t n, const double* init_h) {
+  double* p = nullptr;
+  if (hipMalloc(&p, n * sizeof(double)) != hipSuccess) return nullptr;
+  owned.push_back(p);
+  if (init_h) (void)hipMemcpy(p, init_h, n * sizeof(double), hipMemcpyHostToDevice);
+  else (void)hipMemset(p, 0, n * sizeof(double));
+  return p;
+}
+
+int QdDriver::init(const sc_qd_desc& d, const double* mu, const double* y, const double* z,
+                   const double* a, const double* b, const double* Dc, const double* sN,
+                   const double* eta, const double* locked, const double* prestress,
+                   const double* bcRShift) {
+  desc = d;
+  int e = sc_ctx_create(&d.grid, y, z, mu, &ctx);
+  if (e) return e;
+  Ny = d.grid.Ny;
+  Nz = d.grid.Nz;
+  N = Ny * Nz;
+  a_d = dalloc(Nz, a);
+  b_d = dalloc(Nz, b);
+  Dc_d = dalloc(Nz, Dc);
+  sN_d = dalloc(Nz, sN);
+  eta_d = dalloc(Nz, eta);
+  locked_d = dalloc(Nz, locked);
+  prestress_d = dalloc(Nz, prestress);
+  shift_d = dalloc(Nz, bcRShift);
+  bcL_d = dalloc(Nz, nullptr);
+  bcR_d = dalloc(Nz, nullptr);
+  bcT_d = dalloc(Ny, nullptr);
+  bcB_d = dalloc(Ny, nullptr);
+  rhs_d = dalloc(N, nullptr);
+  u_d = dalloc(N, nullptr);
+  tauQS_d = dalloc(Nz, nullptr);
+  tau_d = dalloc(Nz, nullptr);
+  strength_d = dalloc(Nz, nullptr);
+  V_d = dalloc(Nz, nullptr);
+  err_d = dalloc(8, nullptr);
+  int* st = nullptr;
+  if (hipMalloc(&st, sizeof(int)) != hipSuccess) return fail(SC_ERR_HIP, "hipMalloc status");
+  owned.push_back(st);
+  (void)hipMemset(st, 0, sizeof(int));
+  status_d = st;
+  for (const char* key : {"psi", "slip"}) {
+    VecSet& v = vars[key];
+    v.var = dalloc(Nz, nullptr);
+    v.dvar = dalloc(Nz, nullptr);
+    for (int i = 0; i < 7; ++i) {
+      v.k[i] = dalloc(Nz, nullptr);
+      v.f[i] = dalloc(Nz, nullptr);
+    }
+    v.lo = dalloc(Nz, nullptr);
+    v.hi = dalloc(Nz, nullptr);
+  }
+  if (hipHostMalloc(&host_scratch, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    return fail(SC_ERR_HIP, "hipHostMalloc");
+  for (void* p : owned)
+    if (!p) return fail(SC_ERR_HIP, "device allocation failed");
+  return SC_OK;
+}
+
+// StrikeSlip_LinearElastic_qd::d_dt (strikeSlip_linearElastic_qd.cpp:756-798)
+int QdDriver::d_dt(double t, const double* psi, const double* slip, double* dpsi, double* dslip) {
+  hipStream_t s = ctx->c.stream;
+  const double t0 = now();
+  ++counters.d_dt_calls;
+  int e = launch_qd_bc(s, Nz, slip, shift_d, t, desc.vL, desc.faultTypeScale, desc.remoteLoadingR,
+                       bcL_d, bcR_d);
+  if (!e) e = launch_set_rhs(ctx->c, rhs_d, bcL_d, bcR_d, bcT_d, bcB_d, s);
+  if (e) return e;
+  const double ts = now();
+  sc_solve_stats st;
+  e = solve(ctx->c, rhs_d, u_d, desc.solve, st);
+  counters.solve_seconds += now() - ts;
+  counters.solves += 1;
+  counters.cg_iters += st.iters;
+  counters.outer_sweeps += st.outer;
+  if (e) return e;
+  e = launch_fault_sxy(ctx->c, u_d, tauQS_d, s);
+  if (e) return e;
+  sc_fault_qd_args fa{};
+  fa.N = Nz;
+  fa.v0 = desc.v0;
+  fa.vL = desc.vL;
+  fa.f0 = desc.f0;
+  fa.rootTol = desc.rootTol;
+  fa.maxNumIts = desc.maxNumIts;
+  fa.stateLaw = desc.stateLaw;
+  fa.jac_mode = desc.jac_mode;
+  fa.eta_d = eta_d;
+  fa.sN_d = sN_d;
+  fa.a_d = a_d;
+  fa.b_d = b_d;
+  fa.Dc_d = Dc_d;
+  fa.locked_d = locked_d;
+  fa.prestress_d = prestress_d;
+  fa.psi_d = psi;
+  fa.tauQS_d = tauQS_d;
+  fa.V_d = V_d;
+  fa.dpsi_d = dpsi;
+  fa.tau_d = tau_d;
+  fa.strength_d = strength_d;
+  fa.status_d = status_d;
+  e = fault_qd_rates(s, fa);
+  if (e) return e;
+  SC_HIP(hipMemcpyAsync(dslip, V_d, Nz * sizeof(double), hipMemcpyDeviceToDevice, s));
+  counters.total_seconds += now() - t0;
+  return SC_OK;
+}
+
+int QdDriver::check_status() {
+  hipStream_t s = ctx->c.stream;
+  SC_HIP(hipMemcpyAsync(host_scratch, status_d, sizeof(int), hipMemcpyDeviceToHost, s));
+  SC_HIP(hipStreamSynchronize(s));
+  int st = *(int*)host_scratch;
+  if (st) {
+    (void)hipMemsetAsync(status_d, 0, sizeof(int), s);
+    return fail(st & SC_ERR_NONFINITE ? SC_ERR_NONFINITE : SC_ERR_NOCONV,
+                "fault root solve failed (status " + std::to_string(st) + ")");
+  }
+  return SC_OK;
+}
+
+// computeStepSize (odeSolver.cpp:395-445, 842-898)
+double QdDriver::step_size(const RkState& r, double totErr) const {
+  double ratio;
+  if (r.control == 0 || r.stepCount < 4) {
+    ratio = r.kappa * std::pow(r.tol / totErr, 1. / (1. + r.ord));
+  } else {
+    const double al = 0.49 / r.ord, be = 0.34 / r.ord, ga = 0.1 / r.ord;
+    ratio = r.kappa * std::pow(r.tol / totErr, al) * std::pow(r.errA[0] / r.tol, be) *
+            std::pow(r.tol / r.errA[1], ga);
+  }
+  double dt = ratio * r.deltaT;
+  dt = std::min(r.deltaT * 5.0, dt);
+  dt = std::min(r.maxDt, dt);
+  dt = std::max(r.minDt, dt);
+  return dt;
+}
+
+// computeError (odeSolver.cpp:449-501, 902-969) over errInds = [psi slip]
+int QdDriver::error(const RkState& r, double* totErr) {
+  hipStream_t s = ctx->c.stream;
+  int e = launch_err(s, Nz, vars["psi"].lo, vars["psi"].hi, err_d);
+  if (!e) e = launch_err(s, Nz, vars["slip"].lo, vars["slip"].hi, err_d + 3);
+  if (e) return e;
+  SC_HIP(hipMemcpyAsync(host_scratch, err_d, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
+  SC_HIP(hipStreamSynchronize(s));
+  const double* h = (const double*)host_scratch;
+  double tot = 0;
+  for (int i = 0; i < 2; ++i) {
+    const double* v = h + 3 * i;
+    if (r.normType == 0) tot += std::sqrt(v[0]) / (std::sqrt((double)Nz) * 1.0);
+    else if (r.ord == 3) tot += std::sqrt(v[0]) / (std::sqrt(v[1]) * 1.0);
+    else tot += v[2] / 1.0;
+  }
+  *totErr = tot;
+  return check_status();
+}
+
+int QdDriver::lincomb_all(const char* out_sel, int stage, int nt, const int* fidx, const double* c) {
+  hipStream_t s = ctx->c.stream;
+  for (auto& kv : vars) {
+    VecSet& v = kv.second;
+    double* out = nullptr;
+    if (!std::strcmp(out_sel, "k")) out = v.k[stage];
+    else if (!std::strcmp(out_sel, "lo")) out = v.lo;
+    else out = v.hi;
+    const double* f[6];
+    for (int i = 0; i < nt; ++i) f[i] = fidx[i] == 0 ? v.dvar : v.f[fidx[i]];
+    int e = launch_lincomb(s, Nz, out, v.var, nt, f, c);
+    if (e) return e;
+  }
+  return SC_OK;
+}
+
+int QdDriver::stage(int st, double t) {
+  VecSet& p = vars["psi"];
+  VecSet& q = vars["slip"];
+  return d_dt(t, p.k[st], q.k[st], p.f[st], q.f[st]);
+}
+
+int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* dts, int dtsCap) {
+  RkState r;
+  r.ord = run.integrator == 0 ? 3 : 4;
+  r.control = run.control;
+  r.normType = run.normType;
+  r.tol = run.tol;
+  r.minDt = run.minDt;
+  r.maxDt = run.maxDt;
+  r.deltaT = run.initDt;
+  r.currT = run.initTime;
+  const double finalT = run.maxTime;
+  int maxSteps = run.maxSteps;
+  const bool replay = run.nReplay > 0 && run.replayDt_h;
+  if (replay && maxSteps > run.nReplay) maxSteps = run.nReplay;
+  hipStream_t s = ctx->c.stream;
+  VecSet& P = vars["psi"];
+  VecSet& S = vars["slip"];
+  int e;
+  if (finalT == r.currT || maxSteps == 0) {
+    *tEnd = r.currT;
+    *steps = 0;
+    return SC_OK;
+  }
+  if (r.deltaT == 0) r.deltaT = (finalT - r.currT) / maxSteps;
+  e = d_dt(r.currT, P.var, S.var, P.dvar, S.dvar);
+  if (e) return e;
+  double totErr = 0, newDt = r.deltaT;
+  int nOut = 0;
+  while (r.stepCount < maxSteps && r.currT < finalT) {
+    r.stepCount++;
+    int attempt = 0;
+    while (attempt < 100) {
+      attempt++;
+      if (replay) r.deltaT = run.replayDt_h[r.stepCount - 1];
+      if (r.currT + r.deltaT > finalT) r.deltaT = finalT - r.currT;
+      const double h = r.deltaT;
+      if (r.ord == 3) {
+        { const int fi[1] = {0}; const double c[1] = {0.5 * h}; if ((e = lincomb_all("k", 1, 1, fi, c))) return e; }
+        if ((e = stage(1, r.currT + 0.5 * h))) return e;
+        { const int fi[2] = {0, 1}; const double c[2] = {-h, 2 * h}; if ((e = lincomb_all("k", 2, 2, fi, c))) return e; }
+        if ((e = stage(2, r.currT + h))) return e;
+        { const int fi[2] = {0, 2}; const double c[2] = {0.5 * h, 0.5 * h}; if ((e = lincomb_all("lo", 0, 2, fi, c))) return e; }
+        { const int fi[3] = {0, 1, 2}; const double c[3] = {h / 6.0, 2 * h / 3.0, h / 6.0}; if ((e = lincomb_all("hi", 0, 3, fi, c))) return e; }
+        if ((e = error(r, &totErr))) return e;
+        if (replay || totErr <= r.tol) break;
+      } else {
+        const double c2 = 1. / 2., c3 = 83. / 250., c4 = 31. / 50., c5 = 17. / 20., c6 = 1.;
+        const double b1 = 82889. / 524892., b3 = 15625. / 83664., b4 = 69875. / 102672.,
+                     b5 = -2260. / 8211., b6 = 1. / 4.;
+        const double hb1 = 4586570599. / 29645900160., hb3 = 178811875. / 945068544.,
+                     hb4 = 814220225. / 1159782912., hb5 = -3700637. / 11593932.,
+                     hb6 = 61727. / 225920.;
+        const double a21 = 1. / 2., a31 = 13861. / 62500., a32 = 6889. / 62500.;
+        const double a41 = -116923316275. / 2393684061468., a42 = -2731218467317. / 15368042101831.,
+                     a43 = 9408046702089. / 11113171139209.;
+        const double a51 = -451086348788. / 2902428689909., a52 = -2682348792572. / 7519795681897.,
+                     a53 = 12662868775082. / 11960479115383., a54 = 3355817975965. / 11060851509271.;
+        const double a61 = 647845179188. / 3216320057751., a62 = 73281519250. / 8382639484533.,
+                     a63 = 552539513391. / 3454668386233., a64 = 3354512671639. / 8306763924573.,
+                     a65 = 4040. / 17871.;
+        { const int fi[1] = {0}; const double c[1] = {a21 * h}; if ((e = lincomb_all("k", 2, 1, fi, c))) return e; }
+        if ((e = stage(2, r.currT + c2 * h))) return e;
+        { const int fi[2] = {0, 2}; const double c[2] = {a31 * h, a32 * h}; if ((e = lincomb_all("k", 3, 2, fi, c))) return e; }
+        if ((e = stage(3, r.currT + c3 * h))) return e;
+        { const int fi[3] = {0, 2, 3}; const double c[3] = {a41 * h, a42 * h, a43 * h}; if ((e = lincomb_all("k", 4, 3, fi, c))) return e; }
+        if ((e = stage(4, r.currT + c4 * h))) return e;
+        { const int fi[4] = {0, 2, 3, 4}; const double c[4] = {a51 * h, a52 * h, a53 * h, a54 * h}; if ((e = lincomb_all("k", 5, 4, fi, c))) return e; }
+        if ((e = stage(5, r.currT + c5 * h))) return e;
+        { const int fi[5] = {0, 2, 3, 4, 5}; const double c[5] = {a61 * h, a62 * h, a63 * h, a64 * h, a65 * h}; if ((e = lincomb_all("k", 6, 5, fi, c))) return e; }
+        if ((e = stage(6, r.currT + c6 * h))) return e;
+        { const int fi[5] = {0, 3, 4, 5, 6}; const double c[5] = {hb1 * h, hb3 * h, hb4 * h, hb5 * h, hb6 * h}; if ((e = lincomb_all("lo", 0, 5, fi, c))) return e; }
+        { const int fi[5] = {0, 3, 4, 5, 6}; const double c[5] = {b1 * h, b3 * h, b4 * h, b5 * h, b6 * h}; if ((e = lincomb_all("hi", 0, 5, fi, c))) return e; }
+        if ((e = error(r, &totErr))) return e;
+        if (replay || totErr < r.tol) break;
+      }
+      r.deltaT = step_size(r, totErr);
+      counters.rejected_steps++;
+      if (r.minDt == r.deltaT) break;
+    }
+    r.currT = r.currT + r.deltaT;
+    for (auto& kv : vars) {
+      SC_HIP(hipMemcpyAsync(kv.second.var, kv.second.hi, Nz * sizeof(double), hipMemcpyDeviceToDevice, s));
+      SC_HIP(hipMemsetAsync(kv.second.dvar, 0, Nz * sizeof(double), s));
+    }
+    if ((e = d_dt(r.currT, P.var, S.var, P.dvar, S.dvar))) return e;
+    if (totErr != 0.0) newDt = step_size(r, totErr);
+    r.errA[1] = r.errA[0];
+    r.errA[0] = totErr;
+    if (dts && nOut < dtsCap) dts[nOut++] = r.deltaT;
+    r.deltaT = newDt;
+  }
+  SC_HIP(hipStreamSynchronize(s));
+  if ((e = check_status())) return e;
+  *tEnd = r.currT;
+  *steps = r.stepCount;
+  return SC_OK;
+}
+
+}  // namespace sc
+
+// ---------------------------------------------------------------- C ABI ---
+using sc::fail;
+
+extern "C" int sc_qd_create(const sc_qd_desc* desc, const double* mu_h, const double* y_h,
+                            const double* z_h, const double* a_h, const double* b_h,
+                            const double* Dc_h, const double* sN_h, const double* eta_h,
+                            const double* locked_h, const double* prestress_h,
+                            const double* bcRShift_h, sc_qd** out) {
+  if (!desc || !mu_h || !a_h || !b_h || !Dc_h || !sN_h || !eta_h || !out)
+    return fail(SC_ERR_ARG, "sc_qd_create: null argument");
+  if (desc->stateLaw != SC_AGING_LAW && desc->stateLaw != SC_SLIP_LAW)
+    return fail(SC_ERR_ARG, "stateLaw must be agingLaw or slipLaw");
+  if (!(desc->faultTypeScale > 0)) return fail(SC_ERR_ARG, "faultTypeScale must be > 0");
+  auto* h = new sc_qd;
+  h->q.reset(new sc::QdDriver);
+  int e = h->q->init(*desc, mu_h, y_h, z_h, a_h, b_h, Dc_h, sN_h, eta_h, locked_h, prestress_h,
+                     bcRShift_h);
+  if (e) {
+    delete h;
+    return e;
+  }
+  *out = h;
+  return SC_OK;
+}
+
+extern "C" int sc_qd_destroy(sc_qd* q) {
+  delete q;
+  return SC_OK;
+}
+
+extern "C" sc_ctx* sc_qd_ctx(sc_qd* q) { return q ? q->q->ctx : nullptr; }
+
+extern "C" int sc_qd_set_state(sc_qd* h, const double* psi_h, const double* slip_h,
+                               const double* V_h) {
+  if (!h) return fail(SC_ERR_ARG, "null handle");
+  auto& q = *h->q;
+  const size_t bytes = (size_t)q.Nz * sizeof(double);
+  if (psi_h) SC_HIP(hipMemcpy(q.vars["psi"].var, psi_h, bytes, hipMemcpyHostToDevice));
+  if (slip_h) SC_HIP(hipMemcpy(q.vars["slip"].var, slip_h, bytes, hipMemcpyHostToDevice));
+  if (V_h) SC_HIP(hipMemcpy(q.V_d, V_h, bytes, hipMemcpyHostToDevice));
+  return SC_OK;
+}
+
+extern "C" int sc_qd_get_state(sc_qd* h, double* psi_h, double* slip_h, double* V_h,
+                               double* tau_h) {
+  if (!h) return fail(SC_ERR_ARG, "null handle");
+  auto& q = *h->q;
+  SC_HIP(hipStreamSynchronize(q.ctx->c.stream));
+  const size_t bytes = (size_t)q.Nz * sizeof(double);
+  if (psi_h) SC_HIP(hipMemcpy(psi_h, q.vars["psi"].var, bytes, hipMemcpyDeviceToHost));
+  if (slip_h) SC_HIP(hipMemcpy(slip_h, q.vars["slip"].var, bytes, hipMemcpyDeviceToHost));
+  if (V_h) SC_HIP(hipMemcpy(V_h, q.V_d, bytes, hipMemcpyDeviceToHost));
+  if (tau_h) SC_HIP(hipMemcpy(tau_h, q.tau_d, bytes, hipMemcpyDeviceToHost));
+  return SC_OK;
+}
+
+extern "C" int sc_qd_get_u(sc_qd* h, double* u_h) {
+  if (!h || !u_h) return fail(SC_ERR_ARG, "null argument");
+  auto& q = *h->q;
+  SC_HIP(hipStreamSynchronize(q.ctx->c.stream));
+  SC_HIP(hipMemcpy(u_h, q.u_d, (size_t)q.N * sizeof(double), hipMemcpyDeviceToHost));
+  return SC_OK;
+}
+
+extern "C" int sc_qd_ddt(sc_qd* h, double t, double* dpsi_h, double* dslip_h) {
+  if (!h) return fail(SC_ERR_ARG, "null handle");
+  auto& q = *h->q;
+  auto& P = q.vars["psi"];
+  auto& S = q.vars["slip"];
+  int e = q.d_dt(t, P.var, S.var, P.dvar, S.dvar);
+  if (e) return e;
+  SC_HIP(hipStreamSynchronize(q.ctx->c.stream));
+  if ((e = q.check_status())) return e;
+  const size_t bytes = (size_t)q.Nz * sizeof(double);
+  if (dpsi_h) SC_HIP(hipMemcpy(dpsi_h, P.dvar, bytes, hipMemcpyDeviceToHost));
+  if (dslip_h) SC_HIP(hipMemcpy(dslip_h, S.dvar, bytes, hipMemcpyDeviceToHost));
+  return SC_OK;
+}
+
+extern "C" int sc_qd_integrate(sc_qd* h, const sc_qd_run* run, double* tEnd_h, int* steps_h,
+                               double* dts_h, int dtsCap) {
+  if (!h || !run) return fail(SC_ERR_ARG, "null argument");
+  if (run->integrator != 0 && run->integrator != 1) return fail(SC_ERR_ARG, "integrator must be RK32 (0) or RK43 (1)");
+  if (!(run->tol >= 1e-14)) return fail(SC_ERR_ARG, "timeStepTol must be >= 1e-14");
+  double tEnd = 0;
+  int steps = 0;
+  int e = h->q->integrate(*run, &tEnd, &steps, dts_h, dtsCap);
+  if (tEnd_h) *tEnd_h = tEnd;
+  if (steps_h) *steps_h = steps;
+  return e;
+}
+
+extern "C" int sc_qd_get_counters(sc_qd* h, sc_qd_counters* c) {
+  if (!h || !c) return fail(SC_ERR_ARG, "null argument");
+  *c = h->q->counters;
+  return SC_OK;
+}

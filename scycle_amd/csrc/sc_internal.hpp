@@ -1,0 +1,117 @@
+// Internal declarations of libscycle_hip (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/scycle_hip.h"
+
+namespace sc {
+
+constexpr int kBand = 9;     // band width of every 1-D table (offsets -4..+4)
+constexpr int kFrame = 8;    // frame width handled by the generic kernel
+
+// -------------------------------------------------------- error handling --
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+#define SC_HIP(call)                                                          \
+  do {                                                                        \
+    hipError_t _e = (call);                                                   \
+    if (_e != hipSuccess)                                                     \
+      return ::sc::fail(SC_ERR_HIP, std::string(#call " -> ") + hipGetErrorString(_e)); \
+  } while (0)
+
+// ------------------------------------------------------------ 1-D tables --
+// One direction of the SBP operator, in banded form: entry (i, c) of a 1-D
+// matrix sits at [i*kBand + (c - i + 4)].  Built from the reference's 1-D
+// stencil definitions (spmat.cpp:310-627).
+struct Dir1D {
+  int n = 0, order = 4;
+  double h = 1, alphaD = 0, h11 = 0;
+  std::vector<double> H, Hinv;       // n
+  std::vector<double> D1;            // n*9
+  std::vector<double> DR, DRT;       // D3 (order 4) / D2 (order 2) and transpose
+  std::vector<double> D4, D4T;       // order 4 only (zeros otherwise)
+  std::vector<double> wR, w4;        // C3/(18h) or C2*h^3/4 ; C4/(144h)
+  std::vector<double> BS0, BSN;      // boundary rows of BS: cols 0..8 / n-9..n-1
+};
+int build_dir(int order, int n, double h, bool compatible, Dir1D& d);
+
+// ---------------------------------------------------- device parameters --
+// Raw pointers into context-owned device memory, passed by value to kernels.
+template <typename T>
+struct DevDir {
+  int n;
+  const T *H, *Hinv, *D1, *DR, *DRT, *D4, *D4T, *wR, *w4, *BS0, *BSN;
+  const T* met;    // yq (y) or zr (z), n values
+  const T* imet;   // qy (y) or rz (z), n values
+  T alphaD;
+  // interior constants (row n/2 of the tables)
+  T d1[5], dr[5], d4[5], drt[5], d4t[5], wr, w4c, Hc;
+};
+
+template <typename T>
+struct DevOp {
+  int Ny, Nz, order;
+  int doY, doZ;                  // laplace type "yz" / "y" / "z"
+  int bcR, bcT, bcL, bcB;
+  const T* mu;                   // N
+  DevDir<T> y, z;
+};
+
+// ----------------------------------------------------------- context -----
+struct DevTables {                 // device copies of one direction (one precision)
+  void* base = nullptr;
+  size_t bytes = 0;
+};
+
+struct Ctx {
+  sc_grid_desc d{};
+  int N = 0;
+  hipStream_t stream = nullptr;
+  Dir1D ty, tz;
+  std::vector<double> yq, zr, qy, rz;  // 1-D metric (Ny / Nz)
+  std::vector<double> mu_h;            // host copy
+  // device memory
+  double* mu64 = nullptr;
+  float* mu32 = nullptr;
+  void* tab64 = nullptr;
+  void* tab32 = nullptr;
+  DevOp<double> op64{};
+  DevOp<float> op32{};
+  double* diag64 = nullptr;            // diag(A)
+  double* dinv64 = nullptr;            // 1/diag(A)
+  float* dinv32 = nullptr;
+  // solver workspace (lazily allocated)
+  double *r64 = nullptr, *p64 = nullptr, *q64 = nullptr;
+  float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *d32 = nullptr;
+  double* partials = nullptr;          // reduction scratch
+  void* scal = nullptr;                // CgScalars (device)
+  void* scal_h = nullptr;              // pinned host mirror
+  int nPartial = 0;
+};
+
+int ctx_upload_tables(Ctx& c);
+int ctx_upload_mu(Ctx& c);
+int ctx_compute_diag(Ctx& c);
+
+// ------------------------------------------------------ kernel launchers --
+int launch_apply64(const Ctx& c, const double* in, double* out, hipStream_t s);
+int launch_apply32(const Ctx& c, const float* in, float* out, hipStream_t s);
+int launch_diag(const Ctx& c, double* out, hipStream_t s);
+int launch_set_rhs(const Ctx& c, double* rhs, const double* bcL, const double* bcR,
+                   const double* bcT, const double* bcB, hipStream_t s);
+// which: 0 Dy, 1 muxDy, 2 Dz, 3 muxDz, 4 H, 5 Hinv
+int launch_deriv(const Ctx& c, int which, const double* in, double* out, hipStream_t s);
+// sxy on the fault line j = 0 only: out[k] = mu(0,k)*qy[0]*(D1y u)(0,k)
+int launch_fault_sxy(const Ctx& c, const double* u, double* out, hipStream_t s);
+
+int solve(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st);
+int fault_qd_rates(hipStream_t s, const sc_fault_qd_args& a);
+
+}  // namespace sc
+
+struct sc_ctx {
+  sc::Ctx c;
+};

@@ -1,0 +1,80 @@
+"""Python mirror of StrikeSlip_LinearElastic_qd (strikeSlip_linearElastic_qd.cpp)
+driven through the C ABI: d_dt and the RK32/RK43 integrate loop run in
+libscycle_hip.so."""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import GridDesc, QdCounters, QdDesc, QdRun, SolveOpts, check, dptr, load
+
+SOLVE_MODES = {"cg64": 0, "ir32": 1}
+
+
+class QdMediator:
+    def __init__(self, cfg, solve="cg64", rtol=1e-12, inner_rtol=1e-5, jac_mode=1,
+                 check_every=64, max_iters=400000):
+        """cfg: a scycle_amd.configs.QdConfig."""
+        self.cfg = cfg
+        g = GridDesc(cfg.Ny, cfg.Nz, cfg.Ly, cfg.Lz, cfg.order, int(cfg.var_grid), 0, 0,
+                     *cfg.mat_bc)
+        so = SolveOpts(SOLVE_MODES[solve], rtol, inner_rtol, max_iters, 50, check_every)
+        d = QdDesc(g, int(cfg.remote_loading_R), cfg.fault_type_scale, cfg.vL, cfg.f0, cfg.v0,
+                   cfg.root_tol, 10000, cfg.state_law, jac_mode, so)
+        self._keep = [np.ascontiguousarray(x, dtype=np.float64) if x is not None else None
+                      for x in (cfg.mu, cfg.y, cfg.z, cfg.a, cfg.b, cfg.Dc, cfg.sN, cfg.eta,
+                                cfg.locked, cfg.prestress, cfg.bcRShift)]
+        h = C.c_void_p()
+        check(load().sc_qd_create(C.byref(d), *[dptr(x) for x in self._keep], C.byref(h)))
+        self.h = h
+        self.Nz, self.N = cfg.Nz, cfg.Ny * cfg.Nz
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            load().sc_qd_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, psi, slip, V=None):
+        p = np.ascontiguousarray(psi, dtype=np.float64)
+        s = np.ascontiguousarray(slip, dtype=np.float64)
+        v = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
+        check(load().sc_qd_set_state(self.h, dptr(p), dptr(s), dptr(v)))
+
+    def get_state(self):
+        psi, slip, V, tau = (np.zeros(self.Nz) for _ in range(4))
+        check(load().sc_qd_get_state(self.h, dptr(psi), dptr(slip), dptr(V), dptr(tau)))
+        return dict(psi=psi, slip=slip, V=V, tau=tau)
+
+    def get_u(self):
+        u = np.zeros(self.N)
+        check(load().sc_qd_get_u(self.h, dptr(u)))
+        return u
+
+    def d_dt(self, t):
+        dpsi, dslip = np.zeros(self.Nz), np.zeros(self.Nz)
+        check(load().sc_qd_ddt(self.h, float(t), dptr(dpsi), dptr(dslip)))
+        return dpsi, dslip
+
+    def integrate(self, max_steps, integrator="RK32", tol=1e-8, init_dt=1e-3, min_dt=0.0,
+                  max_dt=1e10, init_time=0.0, max_time=1e15, control="PID",
+                  norm="L2_absolute", replay_dt=None):
+        rep = None if replay_dt is None else np.ascontiguousarray(replay_dt, dtype=np.float64)
+        r = QdRun(0 if integrator == "RK32" else 1, 0 if control == "P" else 1,
+                  0 if norm == "L2_absolute" else 1, int(max_steps), init_time, max_time,
+                  init_dt, min_dt, max_dt, tol, 0 if rep is None else len(rep), dptr(rep))
+        tEnd = C.c_double()
+        steps = C.c_int()
+        dts = np.zeros(max(int(max_steps), 1))
+        check(load().sc_qd_integrate(self.h, C.byref(r), C.byref(tEnd), C.byref(steps),
+                                     dptr(dts), len(dts)))
+        return tEnd.value, steps.value, dts[:steps.value].copy()
+
+    def counters(self):
+        c = QdCounters()
+        check(load().sc_qd_get_counters(self.h, C.byref(c)))
+        return {k: getattr(c, k) for k, _ in QdCounters._fields_}

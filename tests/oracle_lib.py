@@ -1,0 +1,199 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: used by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker.  The product library never
+imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(os.path.dirname(_HERE), "oracle", "liboracle.so")
+
+_dp = C.POINTER(C.c_double)
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle not built: {ORACLE_SO} (run `make -C oracle`)")
+        L = C.CDLL(ORACLE_SO)
+        L.orc_sbp_new.restype = C.c_void_p
+        L.orc_sbp_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                  C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  _dp, _dp, _dp]
+        L.orc_sbp_free.argtypes = [C.c_void_p]
+        L.orc_sbp_change_bc.argtypes = [C.c_void_p] + [C.c_int] * 4
+        L.orc_sbp_nnz.restype = C.c_long
+        L.orc_sbp_nnz.argtypes = [C.c_void_p, C.c_int]
+        L.orc_sbp_shape.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_sbp_csr.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_int), _dp]
+        for name in ("orc_sbp_apply", "orc_sbp_muxdy", "orc_sbp_muxdz", "orc_sbp_dy",
+                     "orc_sbp_dz", "orc_sbp_h", "orc_sbp_hinv"):
+            getattr(L, name).argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_sbp_set_rhs.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.orc_sbp_diag.argtypes = [C.c_void_p, _dp]
+        L.orc_sbp_metric.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_sbp_scalars.argtypes = [C.c_void_p, _dp]
+        L.orc_sbp1d.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int] + [_dp] * 10
+        L.orc_pcg.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int, _dp]
+        L.orc_chol_solve.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_strength.restype = C.c_double
+        L.orc_strength.argtypes = [C.c_double] * 5
+        L.orc_aging.restype = C.c_double
+        L.orc_aging.argtypes = [C.c_double] * 6
+        L.orc_sliplaw.restype = C.c_double
+        L.orc_sliplaw.argtypes = [C.c_double] * 7
+        L.orc_vel_qd.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                                 C.c_double, C.c_int, C.c_int, _dp, C.POINTER(C.c_long)]
+        L.orc_qd_new.restype = C.c_void_p
+        L.orc_qd_new.argtypes = [C.c_void_p] + [_dp] * 11
+        L.orc_qd_free.argtypes = [C.c_void_p]
+        L.orc_qd_ddt.argtypes = [C.c_void_p, C.c_double] + [_dp] * 7
+        L.orc_qd_integrate.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                       C.c_int, _dp]
+        L.orc_qd_counter.restype = C.c_long
+        L.orc_qd_counter.argtypes = [C.c_void_p, C.c_int]
+        L.orc_fd_propagate.argtypes = [C.c_void_p, C.c_double] + [_dp] * 6
+        L.orc_fd_ay.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+        L.orc_fdfault_new.restype = C.c_void_p
+        L.orc_fdfault_new.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, C.c_int] + [_dp] * 8
+        L.orc_fdfault_free.argtypes = [C.c_void_p]
+        L.orc_fdfault_step.argtypes = [C.c_void_p, C.c_double] + [_dp] * 9
+        L.orc_powerlaw.argtypes = [C.c_int] + [_dp] * 8 + [C.c_double] + [_dp] * 3
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+DIRICHLET, NEUMANN = 0, 1
+
+
+class OracleSbp:
+    """Assembled SBP-SAT operator (SbpOps_m_varGrid / SbpOps_m_constGrid)."""
+
+    def __init__(self, order, Ny, Nz, Ly, Lz, mu, y=None, z=None, var_grid=True,
+                 bc=(DIRICHLET, NEUMANN, DIRICHLET, NEUMANN), compat=0, laplace=0):
+        self.Ny, self.Nz, self.N = Ny, Nz, Ny * Nz
+        self._mu = np.ascontiguousarray(mu, dtype=np.float64)
+        self._y = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
+        self._z = None if z is None else np.ascontiguousarray(z, dtype=np.float64)
+        self.h = lib().orc_sbp_new(order, Ny, Nz, Ly, Lz, int(var_grid), compat, laplace,
+                                   bc[0], bc[1], bc[2], bc[3], ptr(self._mu), ptr(self._y),
+                                   ptr(self._z))
+        if not self.h:
+            raise RuntimeError("orc_sbp_new failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_sbp_free(self.h)
+            self.h = None
+
+    def change_bc(self, bcR, bcT, bcL, bcB):
+        lib().orc_sbp_change_bc(self.h, bcR, bcT, bcL, bcB)
+
+    def csr(self, which=0):
+        import scipy.sparse as sp
+        nr, nc = C.c_int(), C.c_int()
+        lib().orc_sbp_shape(self.h, which, C.byref(nr), C.byref(nc))
+        nnz = lib().orc_sbp_nnz(self.h, which)
+        rp = np.zeros(nr.value + 1, dtype=np.int64)
+        ci = np.zeros(nnz, dtype=np.int32)
+        v = np.zeros(nnz, dtype=np.float64)
+        lib().orc_sbp_csr(self.h, which, rp.ctypes.data_as(C.POINTER(C.c_long)),
+                          ci.ctypes.data_as(C.POINTER(C.c_int)), ptr(v))
+        return sp.csr_matrix((v, ci, rp), shape=(nr.value, nc.value))
+
+    def _unary(self, fn, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(self.N)
+        fn(self.h, ptr(x), ptr(out))
+        return out
+
+    def apply(self, u):
+        return self._unary(lib().orc_sbp_apply, u)
+
+    def muxdy(self, u):
+        return self._unary(lib().orc_sbp_muxdy, u)
+
+    def muxdz(self, u):
+        return self._unary(lib().orc_sbp_muxdz, u)
+
+    def dy(self, u):
+        return self._unary(lib().orc_sbp_dy, u)
+
+    def dz(self, u):
+        return self._unary(lib().orc_sbp_dz, u)
+
+    def H(self, u):
+        return self._unary(lib().orc_sbp_h, u)
+
+    def Hinv(self, u):
+        return self._unary(lib().orc_sbp_hinv, u)
+
+    def set_rhs(self, bcL, bcR, bcT, bcB):
+        out = np.zeros(self.N)
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (bcL, bcR, bcT, bcB)]
+        lib().orc_sbp_set_rhs(self.h, ptr(out), *[ptr(x) for x in a])
+        return out
+
+    def diag(self):
+        out = np.zeros(self.N)
+        lib().orc_sbp_diag(self.h, ptr(out))
+        return out
+
+    def metric(self):
+        yq, zr = np.zeros(self.N), np.zeros(self.N)
+        lib().orc_sbp_metric(self.h, ptr(yq), ptr(zr))
+        return yq, zr
+
+    def scalars(self):
+        out = np.zeros(7)
+        lib().orc_sbp_scalars(self.h, ptr(out))
+        return dict(zip(("dy", "dz", "alphaDy", "alphaDz", "h11y", "h11z", "alphaT"), out))
+
+    def pcg(self, b, u0=None, rtol=1e-12, maxit=100000):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        u = np.zeros(self.N) if u0 is None else np.array(u0, dtype=np.float64)
+        rr = C.c_double()
+        it = lib().orc_pcg(self.h, ptr(b), ptr(u), rtol, maxit, C.byref(rr))
+        return u, it, rr.value
+
+    def chol_solve(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        u = np.zeros(self.N)
+        if lib().orc_chol_solve(self.h, ptr(b), ptr(u)):
+            raise RuntimeError("banded Cholesky failed")
+        return u
+
+
+def sbp1d(order, N, scale, compat=0):
+    mats = {k: np.zeros((N, N)) for k in ("H", "Hinv", "D1", "BS", "D2", "C2", "D3", "D4", "C3", "C4")}
+    e = lib().orc_sbp1d(order, N, scale, compat, *[ptr(mats[k]) for k in
+                        ("H", "Hinv", "D1", "BS", "D2", "C2", "D3", "D4", "C3", "C4")])
+    if e:
+        raise RuntimeError(f"orc_sbp1d failed ({e})")
+    return mats
+
+
+def vel_qd(eta, tauQS, sN, psi, a, v0, vL=1e-9, tol=1e-12, maxits=10000, jac_mode=0,
+           V0=None, locked=None):
+    n = len(eta)
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (eta, tauQS, sN, psi, a)]
+    lk = None if locked is None else np.ascontiguousarray(locked, dtype=np.float64)
+    V = np.zeros(n) if V0 is None else np.array(V0, dtype=np.float64)
+    its = C.c_long(0)
+    bad = lib().orc_vel_qd(n, *[ptr(x) for x in arrs], ptr(lk), v0, vL, tol, maxits, jac_mode,
+                           ptr(V), C.byref(its))
+    return V, bad, its.value

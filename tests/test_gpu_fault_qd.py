@@ -1,0 +1,167 @@
+"""GPU rate-and-state kernel and quasi-dynamic step against the oracle.
+
+Tolerances: slip velocity from both root solves satisfies |F(V)| < rootTol =
+1e-12 MPa, so V agrees to ~1e-11 relative; state rates and stresses follow.
+One d_dt agrees to 1e-9 relative (the linear solve is CG to 1e-12 on the GPU
+and a direct Cholesky solve in the oracle).  Integrated fault slip after the
+RK steps agrees to 1e-8 relative, the project's parity bar for slip.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle_lib import lib as olib, ptr as optr, vel_qd
+
+pytestmark = pytest.mark.gpu
+
+
+def _fault_inputs(n, seed, creeping=True):
+    rng = np.random.default_rng(seed)
+    a = 0.010 + 0.015 * rng.random(n)
+    b = np.full(n, 0.02)
+    sN = 50.0 * np.ones(n)
+    eta = np.full(n, 0.5 * np.sqrt(36 * 3.0))
+    Dc = np.full(n, 8e-3)
+    psi = 0.6 + 0.2 * rng.random(n)
+    # tauQS spanning creep (V ~ 1e-20..1e-9) to seismic slip (V ~ 1 m/s)
+    V = 10.0 ** rng.uniform(-20, 1, n)
+    tau = a * sN * np.arcsinh(V / 2e-6 * np.exp(psi / a)) + eta * V
+    return dict(a=a, b=b, sN=sN, eta=eta, Dc=Dc, psi=psi, tauQS=tau, Vtrue=V)
+
+
+def _gpu_fault(inp, jac_mode, state_law=0, V0=None):
+    from scycle_amd import DeviceArray, load
+    from scycle_amd._lib import FaultQdArgs, check
+    n = len(inp["a"])
+    d = {k: DeviceArray(n, host=inp[k]) for k in ("a", "b", "sN", "eta", "Dc", "psi", "tauQS")}
+    V = DeviceArray(n, host=np.zeros(n) if V0 is None else V0)
+    out = {k: DeviceArray(n) for k in ("dpsi", "tau", "strength")}
+    st = DeviceArray(2, np.int32)
+    # device-side context for the stream
+    from scycle_amd import SbpOps
+    ops = SbpOps(2, 9, 9, 1.0, 1.0, np.ones(81), var_grid=False)
+    args = FaultQdArgs(n, 1e-6, 1e-9, 0.6, 1e-12, 10000, state_law, jac_mode,
+                       d["eta"].ptr, d["sN"].ptr, d["a"].ptr, d["b"].ptr, d["Dc"].ptr, None, None,
+                       d["psi"].ptr, d["tauQS"].ptr, V.ptr, out["dpsi"].ptr, out["tau"].ptr,
+                       out["strength"].ptr, st.ptr)
+    check(load().sc_fault_qd_rates(ops.h, C.byref(args)))
+    ops.sync()
+    return dict(V=V.to_host(), dpsi=out["dpsi"].to_host(), tau=out["tau"].to_host(),
+                strength=out["strength"].to_host(), status=int(st.to_host()[0]),
+                tauQS=d["tauQS"].to_host())
+
+
+@pytest.mark.parametrize("jac_mode", [0, 1])
+@pytest.mark.parametrize("state_law", [0, 1])
+def test_fault_kernel_parity(jac_mode, state_law):
+    inp = _fault_inputs(1001, 42)
+    g = _gpu_fault(inp, jac_mode, state_law)
+    assert g["status"] == 0
+    Vo, bad, _ = vel_qd(inp["eta"], inp["tauQS"], inp["sN"], inp["psi"], inp["a"], 1e-6,
+                        jac_mode=jac_mode)
+    assert bad == 0
+    rel = np.abs(g["V"] - Vo) / np.maximum(np.abs(Vo), 1e-300)
+    assert rel.max() < 1e-10
+    # both satisfy the residual tolerance of the reference root finder
+    F = inp["a"] * inp["sN"] * np.arcsinh(g["V"] / 2e-6 * np.exp(inp["psi"] / inp["a"])) \
+        - (inp["tauQS"] - inp["eta"] * g["V"])
+    assert np.abs(F).max() < 1e-11
+    L = olib()
+    dref = np.array([L.orc_aging(p, v, b, 0.6, 1e-6, dc) if state_law == 0 else
+                     L.orc_sliplaw(p, v, a, b, 0.6, 1e-6, dc)
+                     for p, v, a, b, dc in zip(inp["psi"], g["V"], inp["a"], inp["b"], inp["Dc"])])
+    assert np.allclose(g["dpsi"], dref, rtol=1e-12, atol=1e-300)
+    assert np.allclose(g["tau"], inp["tauQS"] - inp["eta"] * g["V"], rtol=1e-12)
+
+
+def _small_cfg(Ny, Nz, order, var_grid):
+    from scycle_amd.configs import strike_slip
+    return strike_slip(Ny, Nz, order=order, var_grid=var_grid,
+                       zstretch=3.0 if var_grid else None, layered_mu=var_grid)
+
+
+def _oracle_qd(cfg, solver=0):
+    class Cfg(C.Structure):
+        _fields_ = [("order", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int), ("var_grid", C.c_int),
+                    ("Ly", C.c_double), ("Lz", C.c_double), ("bcR", C.c_int), ("bcT", C.c_int),
+                    ("bcL", C.c_int), ("bcB", C.c_int), ("remoteLoadingR", C.c_int),
+                    ("faultTypeScale", C.c_double), ("vL", C.c_double), ("f0", C.c_double),
+                    ("v0", C.c_double), ("rootTol", C.c_double), ("stateLaw", C.c_int),
+                    ("jac_mode", C.c_int), ("solver", C.c_int), ("cg_rtol", C.c_double)]
+    c = Cfg(cfg.order, cfg.Ny, cfg.Nz, int(cfg.var_grid), cfg.Ly, cfg.Lz, *cfg.mat_bc, 1,
+            cfg.fault_type_scale, cfg.vL, cfg.f0, cfg.v0, cfg.root_tol, cfg.state_law, 0,
+            solver, 1e-12)
+    keep = [np.ascontiguousarray(x) for x in (cfg.mu, cfg.y, cfg.z, cfg.a, cfg.b, cfg.Dc,
+                                              cfg.sN, cfg.eta)]
+    h = olib().orc_qd_new(C.byref(c), *[optr(x) for x in keep], None, None, None)
+    assert h
+    return h, keep, c
+
+
+@pytest.mark.parametrize("order,var_grid,N", [(2, False, 21), (4, True, 31)])
+def test_qd_ddt_parity(order, var_grid, N):
+    from scycle_amd import QdMediator
+    cfg = _small_cfg(N, N, order, var_grid)
+    g = QdMediator(cfg, solve="cg64", rtol=1e-13, jac_mode=0)
+    h, keep, _ = _oracle_qd(cfg)
+    L = olib()
+    slip = 1e-3 * np.linspace(1, 0, N)
+    for t in (0.0, 3.7e7):
+        g.set_state(cfg.psi0, slip, np.full(N, 1e-9))
+        dpsi, dslip = g.d_dt(t)
+        Vo = np.full(N, 1e-9)
+        dp_o, ds_o, tau_o = np.zeros(N), np.zeros(N), np.zeros(N)
+        u_o = np.zeros(N * N)
+        assert L.orc_qd_ddt(h, t, optr(cfg.psi0), optr(slip), optr(Vo), optr(dp_o), optr(ds_o),
+                            optr(tau_o), optr(u_o)) == 0
+        st = g.get_state()
+        assert np.abs(dslip - ds_o).max() <= 1e-9 * np.abs(ds_o).max()
+        assert np.abs(dpsi - dp_o).max() <= 1e-9 * np.abs(dp_o).max()
+        assert np.abs(st["tau"] - tau_o).max() <= 1e-9 * np.abs(tau_o).max()
+        u = g.get_u()
+        assert np.abs(u - u_o).max() <= 1e-9 * np.abs(u_o).max()
+    L.orc_qd_free(h)
+
+
+@pytest.mark.parametrize("integrator", ["RK32", "RK43"])
+@pytest.mark.parametrize("solve", ["cg64", "ir32"])
+def test_qd_integrate_parity(integrator, solve):
+    """Adaptive RK steps from the same state: step sizes, slip, state, slip
+    velocity, shear stress and off-fault displacement against the oracle."""
+    from scycle_amd import QdMediator
+    N = 25
+    cfg = _small_cfg(N, N, 4, True)
+    steps = 40
+    g = QdMediator(cfg, solve=solve, rtol=1e-13, jac_mode=0)
+    g.set_state(cfg.psi0, cfg.slip0, np.zeros(N))
+    tEnd, nst, dts = g.integrate(steps, integrator=integrator, tol=1e-8, init_dt=1e-3,
+                                 max_dt=1e10)
+    gs = g.get_state()
+    h, keep, _ = _oracle_qd(cfg)
+
+    class Run(C.Structure):
+        _fields_ = [("integrator", C.c_int), ("control", C.c_int), ("normType", C.c_int),
+                    ("maxSteps", C.c_int), ("initTime", C.c_double), ("maxTime", C.c_double),
+                    ("initDt", C.c_double), ("minDt", C.c_double), ("maxDt", C.c_double),
+                    ("tol", C.c_double), ("nReplay", C.c_int), ("replayDt", C.c_void_p)]
+    r = Run(0 if integrator == "RK32" else 1, 1, 0, steps, 0.0, 1e15, 1e-3, 0.0, 1e10, 1e-8, 0,
+            None)
+    psi, slip, V, tau = cfg.psi0.copy(), cfg.slip0.copy(), np.zeros(N), np.zeros(N)
+    u = np.zeros(N * N)
+    tE = C.c_double()
+    odts = np.zeros(steps)
+    ns = olib().orc_qd_integrate(h, C.byref(r), optr(psi), optr(slip), optr(V), optr(tau),
+                                 optr(u), C.byref(tE), optr(odts), steps, None)
+    assert ns == nst == steps
+    print(integrator, solve, "tEnd", tEnd, tE.value, flush=True)
+    assert abs(tEnd - tE.value) <= 1e-9 * tE.value
+    assert np.allclose(dts, odts[:steps], rtol=1e-8)
+    for name, a, b in (("slip", gs["slip"], slip), ("psi", gs["psi"], psi), ("V", gs["V"], V),
+                       ("tau", gs["tau"], tau)):
+        err = np.abs(a - b).max() / np.abs(b).max()
+        print(name, err, flush=True)
+        assert err <= 1e-8, name
+    ug = g.get_u()
+    assert np.abs(ug - u).max() <= 1e-8 * np.abs(u).max()
+    olib().orc_qd_free(h)

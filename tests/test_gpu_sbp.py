@@ -1,0 +1,146 @@
+"""GPU parity of the matrix-free SBP-SAT operator against the assembled
+CPU oracle (same seeded inputs), plus size-independent properties at full size.
+
+Tolerance: FP64 results agree with the oracle to 1e-13 relative in the
+max norm, scaled by max|A|*max|u| (the two sides sum the same terms in a
+different order); FP32 apply agrees to 2e-6 of the same scale.
+"""
+import numpy as np
+import pytest
+
+from oracle_lib import OracleSbp
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-13
+TOL32 = 2e-6
+
+
+def _grid(Ny, Nz, Ly, Lz, by=0.0, bz=0.0):
+    q = np.repeat(np.arange(Ny) / (Ny - 1.0), Nz)
+    r = np.tile(np.arange(Nz) / (Nz - 1.0), Ny)
+    y = Ly * (np.sinh(by * q) / np.sinh(by) if by else q)
+    z = Lz * (np.sinh(bz * r) / np.sinh(bz) if bz else r)
+    return y, z
+
+
+def _case(order, var_grid, Ny, Nz, bc, seed, laplace=0):
+    rng = np.random.default_rng(seed)
+    Ly, Lz = 30.0, 24.0
+    mu = 30 + 6 * rng.random(Ny * Nz)
+    y, z = _grid(Ny, Nz, Ly, Lz, 1.2, 3.0) if var_grid else (None, None)
+    return dict(order=order, Ny=Ny, Nz=Nz, Ly=Ly, Lz=Lz, mu=mu, y=y, z=z, var_grid=var_grid,
+                bc=bc, laplace=laplace, rng=rng)
+
+
+def _pair(c):
+    from scycle_amd import SbpOps
+    lap = ["yz", "y", "z"][c["laplace"]]
+    g = SbpOps(c["order"], c["Ny"], c["Nz"], c["Ly"], c["Lz"], c["mu"], c["y"], c["z"],
+               var_grid=c["var_grid"], bc=c["bc"], laplace=lap)
+    o = OracleSbp(c["order"], c["Ny"], c["Nz"], c["Ly"], c["Lz"], c["mu"], c["y"], c["z"],
+                  var_grid=c["var_grid"], bc=c["bc"], laplace=c["laplace"])
+    return g, o
+
+
+CASES = [(o, v, Ny, Nz, bc) for o in (2, 4) for v in (False, True)
+         for (Ny, Nz) in ((21, 25), (41, 37), (9, 12), (18, 130))
+         for bc in ((0, 1, 0, 1), (0, 0, 0, 0), (1, 1, 1, 1), (1, 0, 0, 1))]
+
+
+@pytest.mark.parametrize("order,var_grid,Ny,Nz,bc", CASES)
+def test_apply_parity(order, var_grid, Ny, Nz, bc):
+    c = _case(order, var_grid, Ny, Nz, bc, seed=Ny * 7 + Nz + order)
+    g, o = _pair(c)
+    u = c["rng"].standard_normal(Ny * Nz)
+    ref = o.apply(u)
+    got = g.apply(u)
+    A = o.csr()
+    scale = abs(A).max() * np.abs(u).max()
+    err = np.abs(got - ref).max() / scale
+    print(f"apply order={order} var={var_grid} {Ny}x{Nz} bc={bc} err={err:.2e}", flush=True)
+    assert err < TOL64
+    got32 = g.apply(u.astype(np.float32), prec="fp32").astype(np.float64)
+    err32 = np.abs(got32 - ref).max() / scale
+    assert err32 < TOL32
+
+
+@pytest.mark.parametrize("laplace", [1, 2])
+def test_apply_laplace_types(laplace):
+    c = _case(4, True, 33, 29, (0, 1, 0, 1), seed=11, laplace=laplace)
+    g, o = _pair(c)
+    u = c["rng"].standard_normal(33 * 29)
+    ref, got = o.apply(u), g.apply(u)
+    assert np.abs(got - ref).max() <= TOL64 * abs(o.csr()).max() * np.abs(u).max()
+
+
+@pytest.mark.parametrize("order,var_grid", [(2, False), (4, True), (4, False)])
+def test_rhs_derivs_diag_parity(order, var_grid):
+    Ny, Nz = 37, 29
+    for bc in ((0, 1, 0, 1), (1, 0, 1, 0), (0, 0, 0, 0)):
+        c = _case(order, var_grid, Ny, Nz, bc, seed=5)
+        g, o = _pair(c)
+        rng = c["rng"]
+        bcL, bcR = rng.standard_normal(Nz), rng.standard_normal(Nz)
+        bcT, bcB = rng.standard_normal(Ny), rng.standard_normal(Ny)
+        ref = o.set_rhs(bcL, bcR, bcT, bcB)
+        got = g.setRhs(bcL, bcR, bcT, bcB)
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+        dref, dgot = o.diag(), g.diag()
+        assert np.abs(dgot - dref).max() <= 1e-13 * np.abs(dref).max()
+    u = rng.standard_normal(Ny * Nz)
+    for name in ("Dy", "muxDy", "Dz", "muxDz", "H", "Hinv"):
+        ref = getattr(o, {"Dy": "dy", "muxDy": "muxdy", "Dz": "dz", "muxDz": "muxdz",
+                          "H": "H", "Hinv": "Hinv"}[name])(u)
+        got = getattr(g, name)(u)
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max(), name
+
+
+def test_change_bc_and_update_coeff():
+    c = _case(4, True, 29, 31, (0, 1, 0, 1), seed=9)
+    g, o = _pair(c)
+    u = c["rng"].standard_normal(29 * 31)
+    g.changeBCTypes(0, 1, 1, 1)
+    o.change_bc(0, 1, 1, 1)
+    ref, got = o.apply(u), g.apply(u)
+    assert np.abs(got - ref).max() <= TOL64 * abs(o.csr()).max() * np.abs(u).max()
+    mu2 = 20 + c["rng"].random(29 * 31)
+    g.updateVarCoeff(mu2)
+    o2 = OracleSbp(4, 29, 31, c["Ly"], c["Lz"], mu2, c["y"], c["z"], var_grid=True,
+                   bc=(0, 1, 1, 1))
+    ref, got = o2.apply(u), g.apply(u)
+    assert np.abs(got - ref).max() <= TOL64 * abs(o2.csr()).max() * np.abs(u).max()
+
+
+def test_metric_and_h11():
+    c = _case(4, True, 41, 33, (0, 1, 0, 1), seed=3)
+    g, o = _pair(c)
+    yq, zr = g.metric()
+    oyq, ozr = o.metric()
+    assert np.allclose(yq, oyq.reshape(41, 33)[:, 0], rtol=1e-14, atol=0)
+    assert np.allclose(zr, ozr.reshape(41, 33)[0, :], rtol=1e-14, atol=0)
+    s = o.scalars()
+    h11y, h11z = g.geth11()
+    assert h11y == s["h11y"] and h11z == s["h11z"]
+
+
+@pytest.mark.parametrize("order,var_grid,N", [(4, True, 1001), (2, False, 1001), (4, True, 2001)])
+def test_full_size_properties(order, var_grid, N):
+    """At full size (no oracle): A symmetric (<v,Au> = <u,Av>), the all-Neumann
+    operator annihilates constants, and FP32 apply tracks FP64."""
+    from scycle_amd import SbpOps
+    rng = np.random.default_rng(N)
+    y, z = _grid(N, N, 30.0, 30.0, 0.0, 5.0) if var_grid else (None, None)
+    Z = z if z is not None else np.tile(np.arange(N) * 30.0 / (N - 1), N)
+    t = np.clip(Z / 4.0, 0, 1)
+    mu = np.where(Z >= 4.0, 36.0, 30.0 + 6.0 * (3 * t ** 2 - 2 * t ** 3))
+    g = SbpOps(order, N, N, 30.0, 30.0, mu, y, z, var_grid=var_grid, bc=(0, 1, 0, 1))
+    u, v = rng.standard_normal(N * N), rng.standard_normal(N * N)
+    Au, Av = g.apply(u), g.apply(v)
+    s1, s2 = v @ Au, u @ Av
+    assert abs(s1 - s2) <= 1e-12 * (np.abs(v) @ np.abs(Au))
+    f32 = g.apply(u.astype(np.float32), prec="fp32")
+    assert np.abs(f32 - Au).max() <= 5e-6 * np.abs(Au).max()
+    gn = SbpOps(order, N, N, 30.0, 30.0, mu, y, z, var_grid=var_grid, bc=(1, 1, 1, 1))
+    A1 = gn.apply(np.ones(N * N))
+    assert np.abs(A1).max() <= 1e-11 * np.abs(Au).max()
