@@ -36,9 +36,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_path(cfg, mode, warmup, steps):
+PATHS = {
+    # label: (solve mode, preconditioner)
+    "mixed": ("cg64", "fdiag32"),      # FP64 CG, FP32 fast-diagonalisation preconditioner
+    "mixed_bf16": ("cg64", "fdiag_bf16"),
+    "fp64_fdiag": ("cg64", "fdiag64"),
+    "fp64": ("cg64", "jacobi"),        # plain FP64 Jacobi-PCG
+    "ir32": ("ir32", "jacobi"),        # FP32 Jacobi-PCG inside FP64 refinement
+}
+
+
+def run_path(cfg, label, warmup, steps, reuse=0):
     from scycle_amd import QdMediator
-    q = QdMediator(cfg, solve=mode, rtol=1e-12, inner_rtol=1e-5, jac_mode=1)
+    mode, prec = PATHS[label]
+    tic = time.perf_counter()
+    q = QdMediator(cfg, solve=mode, rtol=1e-12, inner_rtol=1e-5, jac_mode=1, precond=prec,
+                   reuse=reuse)
+    setup = time.perf_counter() - tic
     q.set_state(cfg.psi0, cfg.slip0, np.zeros(cfg.Nz))
     t0 = 0.0
     dt0 = 1e-3
@@ -53,7 +67,8 @@ def run_path(cfg, mode, warmup, steps):
     c1 = q.counters()
     st = q.get_state()
     d = {k: c1[k] - c0[k] for k in ("d_dt_calls", "solves", "cg_iters", "outer_sweeps")}
-    return q, dict(seconds=sec, steps=n, t_end=t1, counters=d, state=st)
+    return q, dict(seconds=sec, steps=n, t_start=t0, t_end=t1, counters=d, state=st,
+                   setup_s=setup, dt_min=float(dts.min()), dt_max=float(dts.max()))
 
 
 def roofline(q):
@@ -106,11 +121,15 @@ def cpu_baseline(cfg, iters_per_step, budget_s=20.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1001)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--paths", default="ir32,cg64")
+    ap.add_argument("--paths", default="mixed,fp64,fp64_fdiag,mixed_bf16")
+    ap.add_argument("--reuse", type=int, default=0,
+                    help="previous solutions projected for the CG initial guess (FD paths)")
+    ap.add_argument("--slow-steps", type=int, default=3,
+                    help="timed steps for the Jacobi-preconditioned paths (same window start)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus > 1 is not supported "
@@ -120,49 +139,66 @@ def main():
     from scycle_amd.configs import config_c2
     cfg = config_c2(args.n, args.n)
     res = {}
-    q_mixed = None
-    for mode in args.paths.split(","):
-        log(f"[bench] path {mode}: warmup {args.warmup} steps, timed {args.steps} steps, "
+    keep = None
+    labels = args.paths.split(",")
+    for label in labels:
+        k = args.steps if PATHS[label][1] != "jacobi" else min(args.steps, args.slow_steps)
+        log(f"[bench] path {label} {PATHS[label]}: warmup {args.warmup} steps, timed {k} steps, "
             f"{cfg.Ny}x{cfg.Nz}")
-        q, r = run_path(cfg, mode, args.warmup, args.steps)
-        log(f"[bench] {mode}: {r['steps']} steps in {r['seconds']:.3f} s, counters {r['counters']}")
-        res[mode] = r
-        if mode == "ir32":
-            q_mixed = q
+        q, r = run_path(cfg, label, args.warmup, k,
+                        reuse=args.reuse if PATHS[label][1] != "jacobi" else 0)
+        log(f"[bench] {label}: {r['steps']} steps in {r['seconds']:.3f} s (setup {r['setup_s']:.1f} s), "
+            f"counters {r['counters']}")
+        res[label] = r
+        if label == labels[0]:
+            keep = q
         else:
             q.close()
-    main_mode = "ir32" if "ir32" in res else list(res)[0]
+    main_mode = labels[0]
     r = res[main_mode]
     value = r["steps"] / r["seconds"]
     ms_step = 1e3 * r["seconds"] / r["steps"]
-    extra = {}
-    if "ir32" in res and "cg64" in res:
-        a, b = res["ir32"]["state"]["slip"], res["cg64"]["state"]["slip"]
-        extra["fp64_steps_per_s"] = res["cg64"]["steps"] / res["cg64"]["seconds"]
+    extra = {"paths": {}}
+    for lab, rr in res.items():
+        extra["paths"][lab] = dict(
+            solve=PATHS[lab][0], precond=PATHS[lab][1], steps=rr["steps"],
+            steps_per_s=rr["steps"] / rr["seconds"],
+            cg_iters_per_solve=rr["counters"]["cg_iters"] / max(rr["counters"]["solves"], 1),
+            solves_per_step=rr["counters"]["solves"] / rr["steps"], setup_s=rr["setup_s"],
+            t_start=rr["t_start"], t_end=rr["t_end"], dt_min=rr["dt_min"], dt_max=rr["dt_max"])
+    if "fp64" in res:
+        extra["fp64_steps_per_s"] = extra["paths"]["fp64"]["steps_per_s"]
         extra["mixed_over_fp64"] = value / extra["fp64_steps_per_s"]
-        extra["slip_rel_err_mixed_vs_fp64"] = float(np.abs(a - b).max() / np.abs(b).max())
-        extra["V_rel_err_mixed_vs_fp64"] = float(
-            np.abs(res["ir32"]["state"]["V"] - res["cg64"]["state"]["V"]).max()
-            / np.abs(res["cg64"]["state"]["V"]).max())
-        extra["cg_iters_per_step"] = {m: res[m]["counters"]["cg_iters"] / res[m]["steps"]
-                                      for m in res}
-    rl = roofline(q_mixed if q_mixed is not None else q)
+    if "fp64_fdiag" in res:
+        extra["mixed_over_fp64_same_preconditioner"] = value / extra["paths"]["fp64_fdiag"]["steps_per_s"]
+    ref = "fp64_fdiag" if "fp64_fdiag" in res else ("fp64" if "fp64" in res else None)
+    if ref and ref != main_mode:
+        # compare at the same step count: the reference path ran the same window
+        n = min(res[ref]["steps"], r["steps"])
+        if n == r["steps"] == res[ref]["steps"]:
+            a, b = r["state"]["slip"], res[ref]["state"]["slip"]
+            extra["slip_rel_err_vs_fp64"] = float(np.abs(a - b).max() / np.abs(b).max())
+            extra["V_rel_err_vs_fp64"] = float(np.abs(r["state"]["V"] - res[ref]["state"]["V"]).max()
+                                               / np.abs(res[ref]["state"]["V"]).max())
+            extra["slip_err_reference_path"] = ref
+    rl = roofline(keep)
     log(f"[bench] roofline {rl}")
     roof = dict(bound="hbm", achieved=rl["achieved"] / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
                 frac=rl["achieved"] / HBM_PEAK, traffic=None, kernel=rl["kernel"],
                 kernel_ms=rl["ms"], bytes_per_launch=rl["bytes"],
                 fp64_interior_GBps=rl["fp64_GBps"])
     cpu = None
-    if not args.no_cpu:
-        ips = (res["cg64"]["counters"]["cg_iters"] / res["cg64"]["steps"]) if "cg64" in res else \
-            r["counters"]["cg_iters"] / r["steps"]
+    if not args.no_cpu and "fp64" in res:
+        ips = res["fp64"]["counters"]["cg_iters"] / res["fp64"]["steps"]
         log(f"[bench] cpu baseline (oracle), {ips:.0f} iterations/step")
         cpu = cpu_baseline(cfg, ips)
     line = {
         "metric": "QD steps/s at 1001^2 mixed vs FP64 GPU (slip rel.err <=1e-8) + HBM GB/s",
         "value": value, "unit": "QD steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "fp32-cg+fp64-ir" if main_mode == "ir32" else "f64",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": {"mixed": "f64 CG + f32 GEMM preconditioner", "mixed_bf16": "f64 CG + bf16 GEMM preconditioner",
+                  "fp64_fdiag": "f64", "fp64": "f64", "ir32": "f32 CG + f64 refinement"}[main_mode],
         "data": "synthetic",
         "config": {"workload": f"QD strike-slip {cfg.Ny}x{cfg.Nz} order-4 sinh-z(5) layered-mu "
                                f"aging-law RK32 tol 1e-8, steps {args.warmup + 1}-"

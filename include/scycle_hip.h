@@ -104,7 +104,9 @@ int sc_sbp_hinv(sc_ctx* ctx, const double* in_d, double* out_d);
 
 /* Measurement: mean duration (ms) of `reps` back-to-back launches timed with
  * HIP events on the context stream, and the nodes they write.  which:
- * 0 FP64 apply, 1 FP32 apply, 2 FP64 interior kernel, 3 FP32 interior kernel.
+ * 0 FP64 apply, 1 FP32 apply, 2 FP64 interior kernel, 3 FP32 interior kernel,
+ * 4 / 5 the interior kernel with the plain (not XCD-grouped) tile order,
+ * 6 FP64 frame kernel, 7 FP32 frame kernel.
  * No counterpart in the reference (its timers are MPI_Wtime accumulators,
  * e.g. linearElastic.cpp:474-477). */
 int sc_bench_apply(sc_ctx* ctx, int which, int reps, double* ms_per_launch, double* nodes);
@@ -120,7 +122,25 @@ typedef struct sc_solve_opts {
   int max_iters;       /* cap on (inner) CG iterations in total           */
   int max_outer;       /* IR32: cap on refinement sweeps                  */
   int check_every;     /* iterations between host convergence checks      */
+  int precond;         /* SC_PREC_*                                         */
+  int reuse;           /* 0: start from u as given.  1..SC_REUSE_MAX: keep
+                          the last `reuse` solutions of this context as an
+                          A-orthogonal basis and start from the projection
+                          of the new solution onto it (Fischer's method for
+                          successive right-hand sides); the basis is reset
+                          when the operator or its BC types change.        */
 } sc_solve_opts;
+
+#define SC_REUSE_MAX 8
+
+/* Preconditioners.  JACOBI: 1/diag(A).  FDIAG*: exact inverse of the
+ * separable part of A (the operator with a depth-only shear modulus and
+ * without the reference's two-point mu average), applied as four dense
+ * GEMMs in FP64, FP32 or bf16 (FP32 accumulation); only with SC_SOLVE_CG64. */
+#define SC_PREC_JACOBI 0
+#define SC_PREC_FDIAG64 1
+#define SC_PREC_FDIAG32 2
+#define SC_PREC_FDIAG_BF16 3
 
 typedef struct sc_solve_stats {
   int iters;           /* total CG iterations (FP64 or FP32)              */
@@ -157,6 +177,79 @@ typedef struct sc_fault_qd_args {
  * rootFinder.cpp:174-257), aging/slip state rate (fault.cpp:1984-2088),
  * tau = tauQS - eta*V, strength (fault.cpp:2246-2283). */
 int sc_fault_qd_rates(sc_ctx* ctx, const sc_fault_qd_args* args);
+
+typedef struct sc_fault_fd_args {
+  int N;                     /* fault nodes (Nz)                          */
+  double dt;                 /* time step                                 */
+  double v0, vL, f0, rootTol, faultTypeScale;
+  int maxNumIts;
+  int stateLaw;              /* SC_AGING_LAW / SC_SLIP_LAW               */
+  const double *a_d, *b_d, *Dc_d, *sNEff_d, *rho_d, *alphay_d, *tau0_d, *slip0_d;
+  const double* locked_d;    /* may be NULL                               */
+  const double* d2u_d;       /* D2u on the fault from sc_wave_step        */
+  const double *psi_d, *psiPrev_d, *slip_d, *slipPrev_d;
+  double* V_d;               /* in: previous |V| (Newton start); out: V   */
+  double *psiNext_d, *slipNext_d, *strength_d;
+  double* uFault_d;          /* out: fault displacement (may be NULL)     */
+  int* status_d;             /* device int: set non-zero on failure (may be NULL) */
+} sc_fault_fd_args;
+
+/* Fault_fd::d_dt (fault.cpp:1436-1523): u, uPrev from slip, setPhi
+ * (fault.cpp:1527-1575), slip-velocity root solve (ComputeVel_fd,
+ * fault.cpp:1588-1685), fault displacement update, implicit state update
+ * (ComputeAging_fd / ComputeSlipLaw_fd, fault.cpp:1697-1879), slip and
+ * strength.  One thread per fault node, FP64. */
+int sc_fault_fd_step(sc_ctx* ctx, const sc_fault_fd_args* args);
+
+/* ------------------------------------------------------------ wave step */
+/* strikeSlip_linearElastic_fd::propagateWaves (strikeSlip_linearElastic_fd.cpp:
+ * 445-516): D2u = Jinv*Hinv*(A u), uNext = (dt^2/rho*D2u + 2u +
+ * (dt*ay-1)*uPrev)/(dt*ay+1), fused into the matrix-free apply.  prec selects
+ * the arrays' type (SC_FP64: double*, SC_FP32: float*); d2uFault receives
+ * D2u on the fault column (Nz values, same type).  ay must vanish off the
+ * domain boundary (it does: strikeSlip_linearElastic_fd.cpp:645-665). */
+int sc_wave_step(sc_ctx* ctx, int prec, double dt, const void* u_d, const void* uPrev_d,
+                 void* uNext_d, const void* rho_d, const void* ay_d, void* d2uFault_d);
+
+/* -------------------------------------------------------- FD integration */
+typedef struct sc_fd sc_fd;
+
+typedef struct sc_fd_desc {
+  sc_grid_desc grid;          /* material SAT types: Neumann on all four sides
+                                 (strikeSlip_linearElastic_fd.cpp:22)        */
+  int outR, outT, outB;       /* outgoing-characteristics sides (a_y terms) */
+  int faultL;                 /* a_y term on the fault side (symmFault /
+                                 rigidFault / outgoing)                     */
+  double faultTypeScale, vL, f0, v0, rootTol;
+  int maxNumIts, stateLaw;
+  int bulkPrec;               /* SC_FP64, or SC_FP32: body arrays and update in
+                                 FP32, fault state and root solves in FP64  */
+} sc_fd_desc;
+
+/* StrikeSlip_LinearElastic_fd with its Fault_fd, one constant-dt step per
+ * OdeSolver_WaveEq::integrate iteration (odeSolver_WaveEq.cpp:147-186).
+ * Host arrays: mu, rho, cs (N, z-fast), y, z (N, may be NULL for a constant
+ * grid), fault arrays a, b, Dc, sNEff, tau0, prestress, slip0 (Nz), locked
+ * (Nz, may be NULL).  The fault density is rho on the fault column and
+ * alpha_y = h11y*yq (computePenaltyVectors, :608-632). */
+int sc_fd_create(const sc_fd_desc* desc, const double* mu, const double* rho, const double* cs,
+                 const double* y, const double* z, const double* a, const double* b,
+                 const double* Dc, const double* sNEff, const double* tau0,
+                 const double* prestress, const double* slip0, const double* locked,
+                 sc_fd** out);
+int sc_fd_destroy(sc_fd* fd);
+sc_ctx* sc_fd_ctx(sc_fd* fd);
+/* state at the previous and current time levels (host arrays; u: N, the
+ * rest Nz; V may be NULL: zero) */
+int sc_fd_set_state(sc_fd* fd, const double* uPrev, const double* u, const double* psiPrev,
+                    const double* psi, const double* slipPrev, const double* slip,
+                    const double* V);
+/* nSteps steps of size dt from time t0 (d_dt, then prev <- cur <- next) */
+int sc_fd_steps(sc_fd* fd, double t0, double dt, int nSteps);
+/* current level (host arrays, any may be NULL): u (N), psi, slip, V, tau,
+ * strength (Nz); tau = sxy on the fault + prestress + tau0 (d_dt, :421-434) */
+int sc_fd_get_state(sc_fd* fd, double* u, double* psi, double* slip, double* V, double* tau,
+                    double* strength);
 
 /* ------------------------------------------------------- QD integration */
 typedef struct sc_qd sc_qd;

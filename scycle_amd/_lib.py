@@ -27,7 +27,8 @@ class GridDesc(C.Structure):
 
 class SolveOpts(C.Structure):
     _fields_ = [("mode", C.c_int), ("rtol", C.c_double), ("inner_rtol", C.c_double),
-                ("max_iters", C.c_int), ("max_outer", C.c_int), ("check_every", C.c_int)]
+                ("max_iters", C.c_int), ("max_outer", C.c_int), ("check_every", C.c_int),
+                ("precond", C.c_int), ("reuse", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -43,6 +44,24 @@ class FaultQdArgs(C.Structure):
                 ("locked_d", _vp), ("prestress_d", _vp), ("psi_d", _vp), ("tauQS_d", _vp),
                 ("V_d", _vp), ("dpsi_d", _vp), ("tau_d", _vp), ("strength_d", _vp),
                 ("status_d", _vp)]
+
+
+class FaultFdArgs(C.Structure):
+    _fields_ = [("N", C.c_int), ("dt", C.c_double), ("v0", C.c_double), ("vL", C.c_double),
+                ("f0", C.c_double), ("rootTol", C.c_double), ("faultTypeScale", C.c_double),
+                ("maxNumIts", C.c_int), ("stateLaw", C.c_int),
+                ("a_d", _vp), ("b_d", _vp), ("Dc_d", _vp), ("sNEff_d", _vp), ("rho_d", _vp),
+                ("alphay_d", _vp), ("tau0_d", _vp), ("slip0_d", _vp), ("locked_d", _vp),
+                ("d2u_d", _vp), ("psi_d", _vp), ("psiPrev_d", _vp), ("slip_d", _vp),
+                ("slipPrev_d", _vp), ("V_d", _vp), ("psiNext_d", _vp), ("slipNext_d", _vp),
+                ("strength_d", _vp), ("uFault_d", _vp), ("status_d", _vp)]
+
+
+class FdDesc(C.Structure):
+    _fields_ = [("grid", GridDesc), ("outR", C.c_int), ("outT", C.c_int), ("outB", C.c_int),
+                ("faultL", C.c_int), ("faultTypeScale", C.c_double), ("vL", C.c_double),
+                ("f0", C.c_double), ("v0", C.c_double), ("rootTol", C.c_double),
+                ("maxNumIts", C.c_int), ("stateLaw", C.c_int), ("bulkPrec", C.c_int)]
 
 
 class QdDesc(C.Structure):
@@ -94,6 +113,14 @@ SIGNATURES = {
     "sc_bench_apply": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
     "sc_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "sc_fault_qd_rates": (C.c_int, [_vp, C.POINTER(FaultQdArgs)]),
+    "sc_fault_fd_step": (C.c_int, [_vp, C.POINTER(FaultFdArgs)]),
+    "sc_wave_step": (C.c_int, [_vp, C.c_int, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sc_fd_create": (C.c_int, [C.POINTER(FdDesc)] + [_dp] * 13 + [C.POINTER(_vp)]),
+    "sc_fd_destroy": (C.c_int, [_vp]),
+    "sc_fd_ctx": (_vp, [_vp]),
+    "sc_fd_set_state": (C.c_int, [_vp] + [_dp] * 7),
+    "sc_fd_steps": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int]),
+    "sc_fd_get_state": (C.c_int, [_vp] + [_dp] * 6),
     "sc_qd_create": (C.c_int, [C.POINTER(QdDesc)] + [_dp] * 11 + [C.POINTER(_vp)]),
     "sc_qd_create_from_file": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
     "sc_qd_destroy": (C.c_int, [_vp]),

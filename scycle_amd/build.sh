@@ -11,16 +11,26 @@ mkdir -p "$objdir"
 flags=(-O3 -std=c++17 -fPIC --offload-arch="$ARCH" -Wall -Wno-unused-function
        -Xclang -target-feature -Xclang -packed-fp32-ops -I"$here/../include")
 objs=()
+pids=()
 for f in "$src"/*.hip "$src"/*.cpp; do
   o="$objdir/$(basename "$f").o"
+  rm -f "$o"  # never link a stale object after a failed compile
   if [[ "$f" == *.hip ]]; then
     "$HIPCC" "${flags[@]}" -x hip -c "$f" -o "$o" &
   else
     "$HIPCC" "${flags[@]}" -c "$f" -o "$o" &
   fi
   objs+=("$o")
+  pids+=("$!")
 done
-wait
-"$HIPCC" -shared --offload-arch="$ARCH" -fPIC -o "$out.tmp" "${objs[@]}"
+fail=0
+for p in "${pids[@]}"; do
+  wait "$p" || fail=1
+done
+if [[ $fail -ne 0 ]]; then
+  echo "build failed" >&2
+  exit 1
+fi
+"$HIPCC" -shared --offload-arch="$ARCH" -fPIC -o "$out.tmp" "${objs[@]}" -L/opt/rocm/lib -lrocblas -Wl,-rpath,/opt/rocm/lib
 mv "$out.tmp" "$out"
 echo "built $out"

@@ -129,3 +129,78 @@ def config_c2(Ny=1001, Nz=1001):
     """1001^2, order 4, sinh-stretched z (b = 5), layered mu 30 -> 36 GPa over 4 km."""
     return strike_slip(Ny, Nz, order=4, var_grid=True, zstretch=5.0, layered_mu=True,
                        name=f"C2-{Ny}x{Nz}", steps=500)
+
+
+@dataclass
+class FdConfig:
+    name: str
+    Ny: int
+    Nz: int
+    Ly: float
+    Lz: float
+    order: int
+    var_grid: bool
+    mu: np.ndarray
+    rho: np.ndarray
+    cs: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    a: np.ndarray
+    b: np.ndarray
+    Dc: np.ndarray
+    sNEff: np.ndarray
+    tau0: np.ndarray
+    prestress: np.ndarray
+    slip0: np.ndarray
+    psi0: np.ndarray
+    dt: float
+    locked: np.ndarray = None
+    outR: bool = True                       # outGoingCharacteristics (strikeSlip_linearElastic_fd.cpp:21)
+    outT: bool = False                      # freeSurface
+    outB: bool = True
+    faultL: bool = True                     # symmFault
+    fault_type_scale: float = 2.0
+    vL: float = 1e-9
+    f0: float = 0.6
+    v0: float = 1e-6
+    root_tol: float = 1e-12
+    state_law: int = 1                      # slip law
+    steps: int = 2000
+
+
+# Time step of the FD runs: dt = CFL * g_cfl * min(dy, dz) / cs with
+# g_cfl = 0.7071/sqrt(1.4498) for the 4th-order operator.  The reference's
+# computeTimeStep (strikeSlip_linearElastic_fd.cpp:521-575) multiplies g_cfl in
+# twice (max_deltaT already holds it); this keeps it once.
+G_CFL4 = 0.7071 / np.sqrt(1.4498)
+
+
+def fd_rupture(Ny, Nz, order=4, Ly=30.0, Lz=30.0, cfl=0.5, name="custom", steps=2000):
+    """The C4 physics: constant grid, mu = 36 GPa, rho = 3 (cs = 3.4641 km/s),
+    slip law with f0 = 0.6, v0 = 1e-6, b = 0.02, Dc = 8 mm, sNEff = 50 MPa;
+    a = 0.015 on the nucleation patch z in [10, 14] km (a - b = -0.005) and
+    0.025 elsewhere; tau0 = 30 + 10 exp(-(z - 12)^2 / 2) MPa (the Heaviside
+    form of Fault_fd::updatePrestress, fault.cpp:1418-1421); the fault starts
+    locked-in at steady state for V = 1e-9 m/s with u = slip = 0."""
+    y, z = _grid(Ny, Nz, Ly, Lz)
+    zf = z[:Nz]
+    mu = np.full(Ny * Nz, 36.0)
+    rho = np.full(Ny * Nz, 3.0)
+    cs = np.sqrt(mu / rho)
+    a = np.where((zf >= 10.0) & (zf <= 14.0), 0.015, 0.025)
+    b = np.full(Nz, 0.02)
+    Dc = np.full(Nz, 8e-3)
+    sNEff = np.full(Nz, 50.0)
+    tau0 = 30.0 + 10.0 * np.exp(-((zf - 12.0) ** 2) / 2.0)
+    f0, v0 = 0.6, 1e-6
+    psi0 = f0 - b * np.log(1e-9 / v0)
+    dy, dz = Ly / (Ny - 1.0), Lz / (Nz - 1.0)
+    dt = cfl * G_CFL4 * min(dy, dz) / cs.max()
+    return FdConfig(name=name, Ny=Ny, Nz=Nz, Ly=Ly, Lz=Lz, order=order, var_grid=False, mu=mu,
+                    rho=rho, cs=cs, y=None, z=None, a=a, b=b, Dc=Dc, sNEff=sNEff, tau0=tau0,
+                    prestress=np.zeros(Nz), slip0=np.zeros(Nz), psi0=psi0, dt=dt, steps=steps)
+
+
+def config_c4(Ny=2001, Nz=2001):
+    """2001^2 fully dynamic rupture, order 4, constant grid, CFL 0.5, 2000 steps."""
+    return fd_rupture(Ny, Nz, name=f"C4-{Ny}x{Nz}", steps=2000)

@@ -11,10 +11,12 @@
 // every kernel of an iteration returns at once when the flag is set, so the
 // host launches iterations in batches and only reads the flag between
 // batches.  Reductions are two-stage and deterministic: per-block partial
-// sums, then the last block to finish sums the partials in block order.
+// sums, then a one-block kernel sums the partials in block order and updates
+// the scalars.
 #include <cmath>
 #include <cstring>
 
+#include "fdiag.hpp"
 #include "sc_internal.hpp"
 
 namespace sc {
@@ -26,7 +28,7 @@ struct CgScal {
 };
 
 constexpr int kRedThreads = 256;
-constexpr int kRedBlocks = 1024;  // grid-stride reduction kernels
+constexpr int kRedBlocks = 2048;  // grid-stride reduction kernels (max blocks)
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -34,13 +36,12 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Block-reduce NV values, write partials, and let the last block combine them.
-// Returns true in exactly one block (the last), with tot[] filled, on thread 0.
+// Block-reduce NV values and write this block's partial sums.  The partials
+// are combined by k_finish, a one-block kernel launched right after: the
+// kernel boundary orders the two, so no device-scope fence is needed.
 template <int NV>
-__device__ bool block_reduce_last(double (&v)[NV], double* partials, unsigned int* cnt,
-                                  double (&tot)[NV]) {
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ partials) {
   __shared__ double sh[NV][kRedThreads / 64];
-  __shared__ bool amLast;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
@@ -48,31 +49,78 @@ __device__ bool block_reduce_last(double (&v)[NV], double* partials, unsigned in
     if (lane == 0) sh[i][wid] = s;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < NV) {
+    double s = 0;
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      double s = 0;
-      for (int w = 0; w < kRedThreads / 64; ++w) s += sh[i][w];
-      partials[(size_t)blockIdx.x * NV + i] = s;
-    }
-    __threadfence();
-    unsigned int prev = atomicAdd(cnt, 1u);
-    amLast = (prev == gridDim.x - 1);
+    for (int w = 0; w < kRedThreads / 64; ++w) s += sh[threadIdx.x][w];
+    partials[(size_t)blockIdx.x * NV + threadIdx.x] = s;
+  }
+}
+
+enum FinishOp { FIN_INIT, FIN_PQ, FIN_UPDATE, FIN_IRRES, FIN_UPDATE_R, FIN_RZ };
+
+// Sum the partials of nb blocks in a fixed order (deterministic for a given
+// grid) and update the CG scalars.  One block of kRedThreads threads.
+template <int OP, int NV>
+__global__ __launch_bounds__(kRedThreads) void k_finish(CgScal* sc, const double* __restrict__ partials,
+                                                        int nb, int first) {
+  if ((OP == FIN_PQ || OP == FIN_UPDATE || OP == FIN_UPDATE_R || OP == FIN_RZ) && sc->done) return;
+  __shared__ double sh[NV][kRedThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double w[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) w[i] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kRedThreads) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) w[i] += partials[(size_t)b * NV + i];
+  }
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = wave_sum(w[i]);
+    if (lane == 0) sh[i][wid] = s;
   }
   __syncthreads();
-  if (!amLast) return false;
-  // last block: sum partials in block order (deterministic)
-  __threadfence();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x != 0) return;
+  double t[NV];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      double s = 0;
-      for (unsigned b = 0; b < gridDim.x; ++b) s += ((volatile double*)partials)[(size_t)b * NV + i];
-      tot[i] = s;
-    }
-    *cnt = 0;
+  for (int i = 0; i < NV; ++i) {
+    double s = 0;
+    for (int ww = 0; ww < kRedThreads / 64; ++ww) s += sh[i][ww];
+    t[i] = s;
   }
-  return true;
+  if constexpr (OP == FIN_INIT) {  // rz, rr, bb
+    sc->rz = t[0];
+    sc->rr = t[1];
+    sc->bb = t[2];
+    sc->iters = 0;
+    sc->done = (t[1] <= sc->tol2 * t[2] || t[2] == 0.0) ? 1 : 0;
+  } else if constexpr (OP == FIN_PQ) {  // pq
+    sc->pq = t[0];
+    sc->alpha = sc->rz / t[0];
+    if (!(t[0] != 0.0) || !isfinite(sc->alpha)) sc->done = 2;  // breakdown
+  } else if constexpr (OP == FIN_UPDATE) {  // rz', rr
+    sc->beta = t[0] / sc->rz;
+    sc->rz = t[0];
+    sc->rr = t[1];
+    sc->iters += 1;
+    if (t[1] <= sc->tol2 * sc->bb) sc->done = 1;
+    else if (sc->iters >= sc->maxit) sc->done = 3;
+    else if (!isfinite(t[1])) sc->done = 4;
+  } else if constexpr (OP == FIN_IRRES) {  // rr, bb
+    sc->rr = t[0];
+    sc->bb = t[1];
+    sc->done = (t[0] <= sc->tol2 * t[1] || t[1] == 0.0) ? 1 : 0;
+  } else if constexpr (OP == FIN_UPDATE_R) {  // rr
+    sc->rr = t[0];
+    sc->iters += 1;
+    if (t[0] <= sc->tol2 * sc->bb) sc->done = 1;
+    else if (sc->iters >= sc->maxit) sc->done = 3;
+    else if (!isfinite(t[0])) sc->done = 4;
+  } else {  // FIN_RZ: r.z, r.z_old
+    sc->beta = first ? 0.0 : (t[0] - t[1]) / sc->rz;
+    sc->rz = t[0];
+    if (!isfinite(t[0])) sc->done = 4;
+  }
 }
 
 // r = b - q ; z = dinv*r ; p = z ; rz = r.z ; rr = r.r ; bb = b.b
@@ -81,7 +129,7 @@ __global__ __launch_bounds__(kRedThreads) void k_cg_init(size_t N, const T* __re
                                                          const T* __restrict__ q,
                                                          const T* __restrict__ dinv,
                                                          T* __restrict__ r, T* __restrict__ p,
-                                                         CgScal* sc, double* partials) {
+                                                         double* partials) {
   double v[3] = {0, 0, 0};
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
     const T bi = b[i];
@@ -93,15 +141,7 @@ __global__ __launch_bounds__(kRedThreads) void k_cg_init(size_t N, const T* __re
     v[1] += (double)ri * (double)ri;
     v[2] += (double)bi * (double)bi;
   }
-  double tot[3];
-  if (block_reduce_last<3>(v, partials, &sc->cnt, tot) && threadIdx.x == 0) {
-    sc->rz = tot[0];
-    sc->rr = tot[1];
-    sc->bb = tot[2];
-    sc->iters = 0;
-    sc->done = (tot[1] <= sc->tol2 * tot[2]) ? 1 : 0;
-    if (tot[2] == 0.0) sc->done = 1;
-  }
+  block_partials(v, partials);
 }
 
 // pq = p.q ; alpha = rz/pq
@@ -113,12 +153,7 @@ __global__ __launch_bounds__(kRedThreads) void k_cg_dot_pq(size_t N, const T* __
   double v[1] = {0};
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x)
     v[0] += (double)p[i] * (double)q[i];
-  double tot[1];
-  if (block_reduce_last<1>(v, partials, &sc->cnt, tot) && threadIdx.x == 0) {
-    sc->pq = tot[0];
-    sc->alpha = sc->rz / tot[0];
-    if (!(tot[0] != 0.0) || !isfinite(sc->alpha)) sc->done = 2;  // breakdown
-  }
+  block_partials(v, partials);
 }
 
 // x += alpha p ; r -= alpha q ; rz' = r.(dinv r) ; rr = r.r ; beta = rz'/rz
@@ -140,16 +175,7 @@ __global__ __launch_bounds__(kRedThreads) void k_cg_update(size_t N, T* __restri
     v[0] += (double)ri * (double)zi;
     v[1] += (double)ri * (double)ri;
   }
-  double tot[2];
-  if (block_reduce_last<2>(v, partials, &sc->cnt, tot) && threadIdx.x == 0) {
-    sc->beta = tot[0] / sc->rz;
-    sc->rz = tot[0];
-    sc->rr = tot[1];
-    sc->iters += 1;
-    if (tot[1] <= sc->tol2 * sc->bb) sc->done = 1;
-    else if (sc->iters >= sc->maxit) sc->done = 3;
-    else if (!isfinite(tot[1])) sc->done = 4;
-  }
+  block_partials(v, partials);
 }
 
 // p = dinv r + beta p
@@ -167,7 +193,7 @@ __global__ __launch_bounds__(256) void k_cg_pvec(size_t N, const T* __restrict__
 // r32 = (float)(r64 * s) is done separately once the norm is known.
 __global__ __launch_bounds__(kRedThreads) void k_ir_residual(size_t N, const double* __restrict__ b,
                                                              const double* __restrict__ q,
-                                                             double* __restrict__ r, CgScal* sc,
+                                                             double* __restrict__ r,
                                                              double* partials) {
   double v[2] = {0, 0};
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
@@ -177,12 +203,7 @@ __global__ __launch_bounds__(kRedThreads) void k_ir_residual(size_t N, const dou
     v[0] += ri * ri;
     v[1] += bi * bi;
   }
-  double tot[2];
-  if (block_reduce_last<2>(v, partials, &sc->cnt, tot) && threadIdx.x == 0) {
-    sc->rr = tot[0];
-    sc->bb = tot[1];
-    sc->done = (tot[0] <= sc->tol2 * tot[1] || tot[1] == 0.0) ? 1 : 0;
-  }
+  block_partials(v, partials);
 }
 
 __global__ __launch_bounds__(256) void k_ir_to32(size_t N, const double* __restrict__ r,
@@ -201,6 +222,179 @@ __global__ __launch_bounds__(256) void k_ir_correct(size_t N, const float* __res
   const double s = sqrt(sc->rr);
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x)
     u[i] += (double)d32[i] * s;
+}
+
+// x += alpha p ; r -= alpha q ; rr = r.r   (preconditioner applied separately)
+__global__ __launch_bounds__(kRedThreads) void k_cg_update_r(size_t N, double* __restrict__ x,
+                                                             double* __restrict__ r,
+                                                             const double* __restrict__ p,
+                                                             const double* __restrict__ q,
+                                                             CgScal* sc, double* partials) {
+  if (sc->done) return;
+  const double alpha = sc->alpha;
+  double v[1] = {0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    v[0] += ri * ri;
+  }
+  block_partials(v, partials);
+}
+
+// Flexible (Polak-Ribiere) beta, robust to a preconditioner that is only
+// approximately symmetric (FP32 / bf16 GEMMs):
+//   beta = r.(z - z_old) / rz_old ; rz = r.z ; z_old <- z
+// (first call: beta = 0, p = z)
+__global__ __launch_bounds__(kRedThreads) void k_cg_rz(size_t N, const double* __restrict__ r,
+                                                       const double* __restrict__ z,
+                                                       double* __restrict__ zold, int first,
+                                                       CgScal* sc, double* partials) {
+  if (sc->done) return;
+  double v[2] = {0, 0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+    const double ri = r[i], zi = z[i];
+    v[0] += ri * zi;
+    if (!first) v[1] += ri * zold[i];
+    zold[i] = zi;
+  }
+  block_partials(v, partials);
+}
+
+__global__ __launch_bounds__(256) void k_cg_pz(size_t N, const double* __restrict__ z,
+                                               double* __restrict__ p, const CgScal* sc, int first) {
+  if (sc->done) return;
+  const double beta = sc->beta;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = first ? z[i] : z[i] + beta * p[i];
+}
+
+// ----------------------------------------------- initial guess from reuse --
+// Fischer's projection for successive right-hand sides: the basis y_i is
+// A-orthogonal and w_i = 1 / (y_i^T (-A) y_i) (A is negative definite), so
+// the (-A)-norm best approximation of A^-1 b in span{y_i} is
+//   x0 = sum_i w_i (y_i^T (-A) x*) y_i = sum_i (-w_i y_i^T b) y_i.
+// After a solve with solution x (A x = b - r) the new basis vector is
+//   y = x - sum_i beta_i y_i,  beta_i = -w_i y_i^T (b - r),
+// and y^T (-A) y = -y^T (b - r) since y is A-orthogonal to every y_i.
+struct VecList {
+  const double* v[SC_REUSE_MAX];
+};
+
+// partial sums of y_m . (a - b)   (b may be null)
+template <int M>
+__global__ __launch_bounds__(kRedThreads) void k_multidot(size_t N, const double* __restrict__ a,
+                                                          const double* __restrict__ b, VecList Y,
+                                                          double* partials) {
+  double acc[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) acc[m] = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+    const double v = b ? a[i] - b[i] : a[i];
+#pragma unroll
+    for (int m = 0; m < M; ++m) acc[m] += Y.v[m][i] * v;
+  }
+  block_partials(acc, partials);
+}
+
+// coef[m] = -w[m] * sum of the partials of column m (fixed order)
+template <int M>
+__global__ __launch_bounds__(kRedThreads) void k_proj_coef(const double* __restrict__ partials, int nb,
+                                                           double* wc) {
+  __shared__ double sh[M][kRedThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double acc[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) acc[m] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kRedThreads) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) acc[m] += partials[(size_t)b * M + m];
+  }
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const double t = wave_sum(acc[m]);
+    if (lane == 0) sh[m][wid] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < M) {
+    double t = 0;
+    for (int w = 0; w < kRedThreads / 64; ++w) t += sh[threadIdx.x][w];
+    wc[SC_REUSE_MAX + threadIdx.x] = -wc[threadIdx.x] * t;
+  }
+}
+
+// x = sum_m coef[m] y_m
+template <int M>
+__global__ __launch_bounds__(256) void k_proj_combine(size_t N, VecList Y, const double* __restrict__ wc,
+                                                      double* __restrict__ x) {
+  double c[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) c[m] = wc[SC_REUSE_MAX + m];
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+    double v = 0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) v += c[m] * Y.v[m][i];
+    x[i] = v;
+  }
+}
+
+// y = x - sum_m coef[m] y_m, partial sums of y . (b - r) and x . (b - r).  y may be one of the
+// y_m (the slot being replaced): every element is read before it is written
+// by the same thread.
+template <int M>
+__global__ __launch_bounds__(kRedThreads) void k_proj_update(size_t N, VecList Y, const double* __restrict__ wc,
+                                                             const double* __restrict__ x,
+                                                             const double* __restrict__ b,
+                                                             const double* __restrict__ r, double* y,
+                                                             double* partials) {
+  double c[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) c[m] = wc[SC_REUSE_MAX + m];
+  double acc[2] = {0.0, 0.0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+    const double xi = x[i];
+    double v = xi;
+#pragma unroll
+    for (int m = 0; m < M; ++m) v -= c[m] * Y.v[m][i];
+    y[i] = v;
+    const double ax = b[i] - r[i];
+    acc[0] += v * ax;
+    acc[1] += xi * ax;
+  }
+  block_partials(acc, partials);
+}
+
+// w[slot] = 1 / (y^T (-A) y), or 0 when y vanishes against x^T (-A) x (x was
+// already in the span to rounding): the slot is then ignored.  Partials hold
+// y.(b - r) and, for NV = 2, x.(b - r) (NV = 1: y = x).
+template <int NV>
+__global__ __launch_bounds__(kRedThreads) void k_proj_weight(const double* __restrict__ partials, int nb,
+                                                             double* wc, int slot) {
+  __shared__ double sh[NV][kRedThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double acc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kRedThreads) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] += partials[(size_t)b * NV + i];
+  }
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    const double t = wave_sum(acc[i]);
+    if (lane == 0) sh[i][wid] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      t[i] = 0;
+      for (int w = 0; w < kRedThreads / 64; ++w) t[i] += sh[i][w];
+    }
+    const double nu2 = -t[0], mu2 = -t[NV - 1];
+    wc[slot] = (nu2 > 1e-20 * mu2 && isfinite(nu2) && mu2 > 0.0) ? 1.0 / nu2 : 0.0;
+  }
 }
 
 template <typename T>
@@ -223,7 +417,7 @@ int ensure_workspace(Ctx& c, bool need32) {
     SC_HIP(hipMalloc(&c.partials, (size_t)kRedBlocks * 4 * sizeof(double)));
     SC_HIP(hipMalloc(&c.scal, 2 * sizeof(CgScal)));
     SC_HIP(hipHostMalloc(&c.scal_h, 2 * sizeof(CgScal), hipHostMallocDefault));
-    SC_HIP(hipMemset(c.scal, 0, 2 * sizeof(CgScal)));
+    SC_HIP(hipMemsetAsync(c.scal, 0, 2 * sizeof(CgScal), c.stream));
   }
   if (need32 && !c.r32) {
     SC_HIP(hipMalloc(&c.r32, N * sizeof(float)));
@@ -235,7 +429,7 @@ int ensure_workspace(Ctx& c, bool need32) {
 }
 
 unsigned red_blocks(size_t N) {
-  size_t b = (N + kRedThreads - 1) / kRedThreads;
+  size_t b = (N + 4 * kRedThreads - 1) / (4 * kRedThreads);  // about 4 entries per thread
   if (b > (size_t)kRedBlocks) b = kRedBlocks;
   return (unsigned)(b ? b : 1);
 }
@@ -243,6 +437,12 @@ unsigned ew_blocks(size_t N) {
   size_t b = (N + 255) / 256;
   if (b > 4096) b = 4096;
   return (unsigned)(b ? b : 1);
+}
+
+template <int OP, int NV>
+void finish(hipStream_t s, CgScal* sc, const Ctx& c, size_t N, int first = 0) {
+  hipLaunchKernelGGL((k_finish<OP, NV>), dim3(1), dim3(kRedThreads), 0, s, sc, c.partials,
+                     (int)red_blocks(N), first);
 }
 
 int read_scal(Ctx& c, CgScal* out, int which) {
@@ -267,7 +467,8 @@ int pcg(Ctx& c, const T* b, T* x, T* r, T* p, T* q, const T* dinv, double rtol, 
   int e = apply(x, q);
   if (e) return e;
   hipLaunchKernelGGL(k_cg_init<T>, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, b, q, dinv, r,
-                     p, sc, c.partials);
+                     p, c.partials);
+  finish<FIN_INIT, 3>(s, sc, c, N);
   SC_HIP(hipGetLastError());
   CgScal h;
   int batch = 4;
@@ -279,8 +480,10 @@ int pcg(Ctx& c, const T* b, T* x, T* r, T* p, T* q, const T* dinv, double rtol, 
       if (e) return e;
       hipLaunchKernelGGL(k_cg_dot_pq<T>, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, p, q,
                          sc, c.partials);
+      finish<FIN_PQ, 1>(s, sc, c, N);
       hipLaunchKernelGGL(k_cg_update<T>, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, x, r, p,
                          q, dinv, sc, c.partials);
+      finish<FIN_UPDATE, 2>(s, sc, c, N);
       hipLaunchKernelGGL(k_cg_pvec<T>, dim3(ew_blocks(N)), dim3(256), 0, s, N, r, dinv, p, sc);
     }
     SC_HIP(hipGetLastError());
@@ -296,15 +499,188 @@ int pcg(Ctx& c, const T* b, T* x, T* r, T* p, T* q, const T* dinv, double rtol, 
   return SC_OK;
 }
 
+// FP64 CG preconditioned with the fast-diagonalisation inverse of the
+// separable operator; `fprec` selects the GEMM precision.  The host reads the
+// device scalars once per iteration (iteration counts are small).
+int pcg_fdiag(Ctx& c, const double* b, double* x, int fprec, double rtol, int maxit,
+              int* iters_out, double* relres_out, int* status_out) {
+  const size_t N = (size_t)c.N;
+  hipStream_t s = c.stream;
+  CgScal* sc = (CgScal*)c.scal;
+  double* r = c.r64;
+  double* p = c.p64;
+  double* q = c.q64;
+  double* z = c.z64;
+  hipLaunchKernelGGL(k_set_scal<double>, dim3(1), dim3(1), 0, s, sc, rtol * rtol, maxit);
+  int e = launch_apply64(c, x, q, s);
+  if (e) return e;
+  hipLaunchKernelGGL(k_ir_residual, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, b, q, r,
+                     c.partials);
+  finish<FIN_IRRES, 2>(s, sc, c, N);
+  SC_HIP(hipGetLastError());
+  CgScal h;
+  e = read_scal(c, &h, 0);
+  if (e) return e;
+  int first = 1;
+  while (!h.done) {
+    e = fdiag_apply(c, *c.fd, fprec, r, z);
+    if (e) return e;
+    hipLaunchKernelGGL(k_cg_rz, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, r, z, c.zold64,
+                       first, sc, c.partials);
+    finish<FIN_RZ, 2>(s, sc, c, N, first);
+    hipLaunchKernelGGL(k_cg_pz, dim3(ew_blocks(N)), dim3(256), 0, s, N, z, p, sc, first);
+    first = 0;
+    e = launch_apply64(c, p, q, s);
+    if (e) return e;
+    hipLaunchKernelGGL(k_cg_dot_pq<double>, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, p, q,
+                       sc, c.partials);
+    finish<FIN_PQ, 1>(s, sc, c, N);
+    hipLaunchKernelGGL(k_cg_update_r, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, x, r, p, q,
+                       sc, c.partials);
+    finish<FIN_UPDATE_R, 1>(s, sc, c, N);
+    SC_HIP(hipGetLastError());
+    e = read_scal(c, &h, 0);
+    if (e) return e;
+  }
+  *iters_out = h.iters;
+  *relres_out = h.bb > 0 ? std::sqrt(h.rr / h.bb) : 0.0;
+  *status_out = h.done;
+  return SC_OK;
+}
+
+template <int M>
+void proj_guess_m(Ctx& c, const VecList& Y, const double* b, double* x) {
+  const size_t N = (size_t)c.N;
+  hipStream_t s = c.stream;
+  const unsigned nb = red_blocks(N);
+  hipLaunchKernelGGL(k_multidot<M>, dim3(nb), dim3(kRedThreads), 0, s, N, b, (const double*)nullptr, Y,
+                     c.partials);
+  hipLaunchKernelGGL(k_proj_coef<M>, dim3(1), dim3(kRedThreads), 0, s, c.partials, (int)nb, c.projW);
+  hipLaunchKernelGGL(k_proj_combine<M>, dim3(ew_blocks(N)), dim3(256), 0, s, N, Y, c.projW, x);
+}
+
+template <int M>
+void proj_add_m(Ctx& c, const VecList& Y, const double* b, const double* x, const double* r, double* y) {
+  const size_t N = (size_t)c.N;
+  hipStream_t s = c.stream;
+  const unsigned nb = red_blocks(N);
+  hipLaunchKernelGGL(k_multidot<M>, dim3(nb), dim3(kRedThreads), 0, s, N, b, r, Y, c.partials);
+  hipLaunchKernelGGL(k_proj_coef<M>, dim3(1), dim3(kRedThreads), 0, s, c.partials, (int)nb, c.projW);
+  hipLaunchKernelGGL(k_proj_update<M>, dim3(nb), dim3(kRedThreads), 0, s, N, Y, c.projW, x, b, r, y,
+                     c.partials);
+}
+
+template <int M>
+void proj_dispatch(bool guess, Ctx& c, const VecList& Y, const double* b, double* x, const double* r,
+                   double* y) {
+  if (guess) proj_guess_m<M>(c, Y, b, x);
+  else proj_add_m<M>(c, Y, b, x, r, y);
+}
+
+void proj_run(bool guess, int M, Ctx& c, const VecList& Y, const double* b, double* x,
+              const double* r, double* y) {
+  switch (M) {
+    case 1: proj_dispatch<1>(guess, c, Y, b, x, r, y); break;
+    case 2: proj_dispatch<2>(guess, c, Y, b, x, r, y); break;
+    case 3: proj_dispatch<3>(guess, c, Y, b, x, r, y); break;
+    case 4: proj_dispatch<4>(guess, c, Y, b, x, r, y); break;
+    case 5: proj_dispatch<5>(guess, c, Y, b, x, r, y); break;
+    case 6: proj_dispatch<6>(guess, c, Y, b, x, r, y); break;
+    case 7: proj_dispatch<7>(guess, c, Y, b, x, r, y); break;
+    default: proj_dispatch<8>(guess, c, Y, b, x, r, y); break;
+  }
+}
+
+int proj_prepare(Ctx& c, int cap) {
+  if (c.projCap == cap) return SC_OK;
+  for (double*& p : c.projY)
+    if (p) {
+      SC_HIP(hipFree(p));
+      p = nullptr;
+    }
+  for (int i = 0; i < cap; ++i) SC_HIP(hipMalloc(&c.projY[i], (size_t)c.N * sizeof(double)));
+  if (!c.projW) SC_HIP(hipMalloc(&c.projW, 2 * SC_REUSE_MAX * sizeof(double)));
+  c.projCap = cap;
+  c.projCount = c.projNext = 0;
+  return SC_OK;
+}
+
+VecList proj_list(const Ctx& c) {
+  VecList Y{};
+  for (int i = 0; i < c.projCount; ++i) Y.v[i] = c.projY[i];
+  return Y;
+}
+
 }  // namespace
 
+int solve_core(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st);
+
 int solve(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st) {
+  if (o.reuse < 0 || o.reuse > SC_REUSE_MAX) return fail(SC_ERR_ARG, "reuse must be in [0, SC_REUSE_MAX]");
+  if (o.reuse == 0) return solve_core(c, rhs, u, o, st);
+  int e = ensure_workspace(c, o.mode == SC_SOLVE_IR32);
+  if (e) return e;
+  e = proj_prepare(c, o.reuse);
+  if (e) return e;
+  if (c.projCount > 0) proj_run(true, c.projCount, c, proj_list(c), rhs, u, nullptr, nullptr);
+  e = solve_core(c, rhs, u, o, st);
+  if (e) return e;
+  // r64 holds b - A u of the converged solve
+  const int slot = c.projNext;
+  const VecList Y = proj_list(c);
+  double* y = c.projY[slot];
+  const int nb = (int)red_blocks(c.N);
+  if (c.projCount > 0) {
+    proj_run(false, c.projCount, c, Y, rhs, u, c.r64, y);
+    hipLaunchKernelGGL(k_proj_weight<2>, dim3(1), dim3(kRedThreads), 0, c.stream, c.partials, nb,
+                       c.projW, slot);
+  } else {
+    // first vector: y = u
+    SC_HIP(hipMemcpyAsync(y, u, (size_t)c.N * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    hipLaunchKernelGGL(k_multidot<1>, dim3(nb), dim3(kRedThreads), 0, c.stream, (size_t)c.N, rhs,
+                       (const double*)c.r64, VecList{{u}}, c.partials);
+    hipLaunchKernelGGL(k_proj_weight<1>, dim3(1), dim3(kRedThreads), 0, c.stream, c.partials, nb,
+                       c.projW, slot);
+  }
+  SC_HIP(hipGetLastError());
+  c.projNext = (slot + 1) % c.projCap;
+  if (c.projCount < c.projCap) ++c.projCount;
+  return SC_OK;
+}
+
+int solve_core(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st) {
   std::memset(&st, 0, sizeof(st));
   const bool ir = o.mode == SC_SOLVE_IR32;
   if (o.mode != SC_SOLVE_CG64 && !ir) return fail(SC_ERR_ARG, "unknown solve mode");
   if (!(o.rtol > 0)) return fail(SC_ERR_ARG, "rtol must be > 0");
+  if (o.precond < SC_PREC_JACOBI || o.precond > SC_PREC_FDIAG_BF16)
+    return fail(SC_ERR_ARG, "unknown preconditioner");
   int e = ensure_workspace(c, ir);
   if (e) return e;
+  if (o.precond != SC_PREC_JACOBI) {
+    if (ir) return fail(SC_ERR_ARG, "the fast-diagonalisation preconditioner runs inside SC_SOLVE_CG64");
+    const int key = c.d.bcR | c.d.bcT << 1 | c.d.bcL << 2 | c.d.bcB << 3;
+    if (!c.fd) c.fd = new FdDiag;
+    if (c.fd->bcKey != key || c.fd_stale) {
+      e = fdiag_build(c, *c.fd);
+      if (e) return e;
+      c.fd_stale = false;
+    }
+    if (!c.z64) SC_HIP(hipMalloc(&c.z64, (size_t)c.N * sizeof(double)));
+    if (!c.zold64) SC_HIP(hipMalloc(&c.zold64, (size_t)c.N * sizeof(double)));
+    const int fprec = o.precond == SC_PREC_FDIAG64 ? FD_F64 : (o.precond == SC_PREC_FDIAG32 ? FD_F32 : FD_BF16);
+    int it = 0, status = 0;
+    double rr = 0;
+    e = pcg_fdiag(c, rhs, u, fprec, o.rtol, o.max_iters > 0 ? o.max_iters : 10000, &it, &rr, &status);
+    if (e) return e;
+    st.iters = it;
+    st.relres = rr;
+    st.converged = status == 1;
+    if (!st.converged)
+      return fail(SC_ERR_NOCONV, "FD-preconditioned CG did not converge (status " +
+                                     std::to_string(status) + ", relres " + std::to_string(rr) + ")");
+    return SC_OK;
+  }
   const int maxit = o.max_iters > 0 ? o.max_iters : 100000;
   const size_t N = (size_t)c.N;
   hipStream_t s = c.stream;
@@ -333,7 +709,8 @@ int solve(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve
     e = launch_apply64(c, u, c.q64, s);
     if (e) return e;
     hipLaunchKernelGGL(k_ir_residual, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, rhs, c.q64,
-                       c.r64, sc0, c.partials);
+                       c.r64, c.partials);
+    finish<FIN_IRRES, 2>(s, sc0, c, N);
     SC_HIP(hipGetLastError());
     CgScal h;
     e = read_scal(c, &h, 0);

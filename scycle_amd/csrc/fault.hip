@@ -1,6 +1,7 @@
 // Fault-side kernels of the quasi-dynamic step: rate-and-state slip-velocity
 // root solve, state evolution, shear stress and strength (Fault_qd::d_dt,
-// fault.cpp:861-918), plus the small vector kernels of the explicit
+// fault.cpp:861-918), of the fully dynamic step (Fault_fd::d_dt,
+// fault.cpp:1436-1523), plus the small vector kernels of the explicit
 // Runge-Kutta driver (odeSolver.cpp) and the boundary-data update of
 // StrikeSlip_LinearElastic_qd::d_dt (strikeSlip_linearElastic_qd.cpp:763-770).
 // All of it is FP64: the argument of asinh reaches ~1e30.
@@ -53,6 +54,103 @@ int fault_qd_rates(hipStream_t s, const sc_fault_qd_args& a) {
   hipLaunchKernelGGL(k_fault_qd, dim3((a.N + 255) / 256), dim3(256), 0, s, a);
   SC_HIP(hipGetLastError());
   return SC_OK;
+}
+
+// ------------------------------------------------------------ FD fault --
+// Fault_fd::d_dt for one node (fault.cpp:1436-1523, setPhi :1527-1575,
+// ComputeVel_fd :1588-1685, ComputeAging_fd / ComputeSlipLaw_fd :1697-1879).
+// uBody (may be null) receives the new fault displacement in the body array's
+// type: the fault column of uNext (strikeSlip_linearElastic_fd.cpp:421-423).
+template <typename TB, typename TD>
+__global__ __launch_bounds__(256) void k_fault_fd(sc_fault_fd_args a, const TD* __restrict__ d2u,
+                                                  TB* __restrict__ uBody) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.N) return;
+  const double dt = a.dt, fts = a.faultTypeScale, v0 = a.v0;
+  const double psiC = a.psi_d[k], psiP = a.psiPrev_d[k];
+  const double aa = a.a_d[k], bb = a.b_d[k], Dc = a.Dc_d[k], sN = a.sNEff_d[k];
+  const double rho = a.rho_d[k], alphay = a.alphay_d[k], slip0 = a.slip0_d[k];
+  int bad = 0;
+  // u, uPrev from slip: (slip - slip0) / faultTypeScale
+  const double uPrev = (-1.0 * slip0 + a.slipPrev_d[k]) * (1.0 / fts);
+  double u = (-1.0 * slip0 + a.slip_d[k]) * (1.0 / fts);
+  // setPhi
+  const double an = (double)d2u[k] + a.tau0_d[k] / alphay;
+  const double Phi = 2.0 / dt * (u - uPrev) + dt * an / rho;
+  const double fricPen = dt / alphay / rho;
+  // slip velocity: fricPen*strength(V) = |Phi| - V on [0, |Phi|]
+  const double lk = a.locked_d ? a.locked_d[k] : 0.0;
+  double V;
+  if (lk > 0.5) {
+    V = 0.;
+  } else if (lk < -0.5) {
+    V = a.vL;
+  } else {
+    const double left = 0., right = fabs(Phi);
+    if (fabs(left - right) < 1e-14) {
+      V = left;
+    } else {
+      const double P = Phi < 0 ? -Phi : Phi;
+      const double A = aa * sN, B = exp(psiC / aa) / (2. * v0);
+      auto resid = [&](double vel, double* f, double* J) {
+        *f = fricPen * strength_psi(sN, psiC, vel, aa, v0) - (P - vel);
+        *J = 1 + fricPen * A * B / sqrt(1. + B * B * vel * vel);
+      };
+      int its;
+      if (bracketed_newton(resid, left, right, fabs(a.V_d[k]), a.rootTol, a.maxNumIts, &V, &its))
+        bad = SC_ERR_NOCONV;
+    }
+  }
+  // fault displacement with the frictional damping of the slip velocity
+  if (!(V < 1e-14)) {
+    const double fric = strength_psi(sN, psiC, V, aa, v0);
+    const double alpha = 1.0 / (rho * alphay) * fric / V;
+    const double Aden = 1.0 + alpha * dt;
+    V = Phi / (1. + dt * alpha);
+    u = (2. * u + (an * dt * dt / rho) + (dt * alpha - 1.) * uPrev) / Aden;
+  }
+  // state: psiNext - psiPrev = 2 dt G((psiPrev + psiNext)/2, V) on [-2, 2]
+  double psiN;
+  {
+    auto resid = [&](double state, double* f, double* J) {
+      const double mid = (psiP + state) / 2.0;
+      if (a.stateLaw == SC_AGING_LAW) {
+        const double G = aging_law_psi(mid, V, bb, a.f0, v0, Dc);
+        *f = -2 * dt * G + (state - psiP);
+        *J = 1 + dt * v0 / Dc * exp((a.f0 - mid) / bb);
+      } else {
+        const double G = slip_law_psi(mid, V, aa, bb, a.f0, v0, Dc);
+        *f = -2 * dt * G + (state - psiP);
+        const double A = fabs(V) / 2. / v0 * exp((psiP + state) / 2.0 / aa);
+        *J = 1 + dt * fabs(V) / Dc * A / sqrt(1 + A * A);
+      }
+    };
+    int its;
+    if (bracketed_newton(resid, -2., 2., psiC, a.rootTol, a.maxNumIts, &psiN, &its)) bad = SC_ERR_NOCONV;
+  }
+  const double slipN = fts * u + slip0;
+  a.V_d[k] = V;
+  a.psiNext_d[k] = psiN;
+  a.slipNext_d[k] = slipN;
+  if (a.strength_d) a.strength_d[k] = strength_psi(sN, psiN, V, aa, v0);
+  if (a.uFault_d) a.uFault_d[k] = u;
+  if (uBody) uBody[k] = (TB)u;
+  if (!isfinite(V) || !isfinite(psiN) || !isfinite(u)) bad = SC_ERR_NONFINITE;
+  if (bad && a.status_d) atomicOr(a.status_d, bad);
+}
+
+template <typename TB, typename TD>
+int fault_fd_step_t(hipStream_t s, const sc_fault_fd_args& a, const TD* d2u, TB* uBody) {
+  if (a.N <= 0) return SC_OK;
+  hipLaunchKernelGGL((k_fault_fd<TB, TD>), dim3((a.N + 255) / 256), dim3(256), 0, s, a, d2u, uBody);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+int fault_fd_step64(hipStream_t s, const sc_fault_fd_args& a, double* uBody) {
+  return fault_fd_step_t<double, double>(s, a, a.d2u_d, uBody);
+}
+int fault_fd_step32(hipStream_t s, const sc_fault_fd_args& a, const float* d2u32, float* uBody) {
+  return fault_fd_step_t<float, float>(s, a, d2u32, uBody);
 }
 
 // ------------------------------------------------------ QD driver helpers
@@ -139,6 +237,19 @@ int launch_err(hipStream_t s, int n, const double* lo, const double* hi, double*
 }
 
 }  // namespace sc
+
+extern "C" int sc_fault_fd_step(sc_ctx* h, const sc_fault_fd_args* args) {
+  if (!h || !args) return sc::fail(SC_ERR_ARG, "sc_fault_fd_step: null argument");
+  const sc_fault_fd_args& a = *args;
+  if (!a.a_d || !a.b_d || !a.Dc_d || !a.sNEff_d || !a.rho_d || !a.alphay_d || !a.tau0_d ||
+      !a.slip0_d || !a.d2u_d || !a.psi_d || !a.psiPrev_d || !a.slip_d || !a.slipPrev_d || !a.V_d ||
+      !a.psiNext_d || !a.slipNext_d)
+    return sc::fail(SC_ERR_ARG, "sc_fault_fd_step: null array");
+  if (a.stateLaw != SC_AGING_LAW && a.stateLaw != SC_SLIP_LAW)
+    return sc::fail(SC_ERR_ARG, "sc_fault_fd_step: stateLaw must be aging or slip");
+  if (!(a.dt > 0)) return sc::fail(SC_ERR_ARG, "sc_fault_fd_step: dt must be > 0");
+  return sc::fault_fd_step64(h->c.stream, a, nullptr);
+}
 
 extern "C" int sc_fault_qd_rates(sc_ctx* h, const sc_fault_qd_args* args) {
   if (!h || !args) return sc::fail(SC_ERR_ARG, "sc_fault_qd_rates: null argument");

@@ -1,0 +1,41 @@
+// Fast-diagonalisation preconditioner: the separable part of the SBP-SAT
+// operator, A_sep = Ay (x) Dz + Dy (x) Az, inverted exactly through the
+// generalised eigen-decompositions Ay V = Dy V Lambda, Az W = Dz W Theta:
+//   A_sep^-1 = (V (x) W) diag(1/(lambda_j + theta_k)) (V^T (x) W^T).
+// For a shear modulus that varies with depth only the two operators differ
+// only by the reference's mu-averaging term, so CG converges in a handful of
+// iterations; each application is four dense GEMMs on the matrix cores.
+#pragma once
+#include <vector>
+
+#include "sc_internal.hpp"
+
+namespace sc {
+
+int sym_eig(int n, std::vector<double> a, std::vector<double>& w, std::vector<double>& vt);
+
+enum FdPrec { FD_F64 = 0, FD_F32 = 1, FD_BF16 = 2 };
+
+struct FdDiag {
+  int Ny = 0, Nz = 0;
+  int bcKey = -1;                // BC signature the factors were built for
+  bool muSeparable = true;       // mu depends on depth only
+  double maxAsym = 0;            // max |L - L^T| / max |L| of the 1-D operators
+  std::vector<double> lam, theta;
+  // device factors (column-major V: Ny x Ny, W: Nz x Nz; invden: Nz x Ny)
+  double *V64 = nullptr, *W64 = nullptr, *VT64 = nullptr, *WT64 = nullptr, *inv64 = nullptr;
+  float *V32 = nullptr, *W32 = nullptr, *VT32 = nullptr, *WT32 = nullptr, *inv32 = nullptr;
+  void *Vb = nullptr, *Wb = nullptr, *VTb = nullptr, *WTb = nullptr;   // bf16
+  // workspaces
+  double *g64 = nullptr, *t64 = nullptr;
+  float *r32 = nullptr, *g32 = nullptr, *t32 = nullptr;
+  void *rb = nullptr, *gb = nullptr;
+  void* blas = nullptr;           // rocblas_handle
+  ~FdDiag();
+};
+
+int fdiag_build(Ctx& c, FdDiag& f);
+// z = A_sep^-1 r (FP64 in/out) with the GEMMs in the given precision.
+int fdiag_apply(Ctx& c, FdDiag& f, int prec, const double* r, double* z);
+
+}  // namespace sc

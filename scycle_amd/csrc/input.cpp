@@ -316,6 +316,8 @@ int build_from_deck(const Deck& d, QdInputs& in) {
   q.solve.max_iters = 1000000;
   q.solve.max_outer = 50;
   q.solve.check_every = 64;
+  q.solve.precond = SC_PREC_JACOBI;
+  q.solve.reuse = 0;
   // run settings (odeSolver set-up in StrikeSlip_LinearElastic_qd::integrate)
   sc_qd_run& r = in.run;
   std::memset(&r, 0, sizeof(r));

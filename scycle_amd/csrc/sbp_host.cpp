@@ -6,6 +6,7 @@
 #include <cstring>
 #include <mutex>
 
+#include "fdiag.hpp"
 #include "sc_internal.hpp"
 
 namespace sc {
@@ -235,6 +236,45 @@ void pack_dir(const Dir1D& d, const std::vector<double>& met, const std::vector<
 
 }  // namespace
 
+namespace {
+template <typename T>
+void fill_closure(const Dir1D& d, int hi, ClosureTab<T>& t) {
+  const int n = d.n;
+  std::memset(&t, 0, sizeof(t));
+  if (n <= 2 * kFrame) return;  // the frame kernel for such lines does not use the tables
+  const int mb = hi ? n - 12 : 0, jb = hi ? n - kFrame : 0;
+  for (int i = 0; i < 12 * kBand; ++i) {
+    t.fD1[i] = (T)d.D1[(size_t)mb * kBand + i];
+    t.fDR[i] = (T)d.DR[(size_t)mb * kBand + i];
+    t.fD4[i] = (T)d.D4[(size_t)mb * kBand + i];
+  }
+  for (int i = 0; i < kFrame * kBand; ++i) {
+    t.oD1[i] = (T)d.D1[(size_t)jb * kBand + i];
+    t.oDRT[i] = (T)d.DRT[(size_t)jb * kBand + i];
+    t.oD4T[i] = (T)d.D4T[(size_t)jb * kBand + i];
+  }
+  for (int i = 0; i < 12; ++i) {
+    t.wR[i] = (T)d.wR[mb + i];
+    t.w4[i] = (T)d.w4[mb + i];
+  }
+  for (int i = 0; i < kFrame; ++i) t.H[i] = (T)d.H[jb + i];
+  for (int i = 0; i < kBand; ++i) t.BS[i] = (T)(hi ? d.BSN[i] : d.BS0[i]);
+}
+
+template <typename T>
+int upload_closure(const Ctx& c, void*& dev, DevOp<T>& op) {
+  ClosureTab<T> h[4];
+  fill_closure(c.ty, 0, h[0]);
+  fill_closure(c.ty, 1, h[1]);
+  fill_closure(c.tz, 0, h[2]);
+  fill_closure(c.tz, 1, h[3]);
+  if (!dev) SC_HIP(hipMalloc(&dev, sizeof(h)));
+  SC_HIP(hipMemcpy(dev, h, sizeof(h), hipMemcpyHostToDevice));
+  op.ctab = (const ClosureTab<T>*)dev;
+  return SC_OK;
+}
+}  // namespace
+
 int ctx_upload_tables(Ctx& c) {
   const size_t ey = dir_elems(c.d.Ny), ez = dir_elems(c.d.Nz);
   if (!c.tab64) SC_HIP(hipMalloc(&c.tab64, (ey + ez) * sizeof(double)));
@@ -243,6 +283,9 @@ int ctx_upload_tables(Ctx& c) {
   pack_dir<double>(c.tz, c.zr, c.rz, (double*)c.tab64 + ey, c.op64.z);
   pack_dir<float>(c.ty, c.yq, c.qy, (float*)c.tab32, c.op32.y);
   pack_dir<float>(c.tz, c.zr, c.rz, (float*)c.tab32 + ey, c.op32.z);
+  int e = upload_closure<double>(c, c.ctab64, c.op64);
+  if (!e) e = upload_closure<float>(c, c.ctab32, c.op32);
+  if (e) return e;
   SC_HIP(hipGetLastError());
   for (auto* op : {&c.op64.Ny, &c.op32.Ny}) *op = c.d.Ny;
   c.op64.Nz = c.op32.Nz = c.d.Nz;
@@ -318,6 +361,13 @@ extern "C" int sc_ctx_create(const sc_grid_desc* desc, const double* y_h, const 
   }
   int e = sc::build_dir(d.order, Ny, hy, d.compat != 0, c.ty);
   if (!e) e = sc::build_dir(d.order, Nz, hz, d.compat != 0, c.tz);
+  // the kernels apply SAT terms on the kFrame nodes next to each boundary only
+  for (const sc::Dir1D* t : {&c.ty, &c.tz}) {
+    if (e) break;
+    for (int i = 0; i < sc::kBand; ++i)
+      if ((i >= sc::kFrame && t->BS0[i] != 0.0) || (i < sc::kBand - sc::kFrame && t->BSN[i] != 0.0))
+        e = fail(SC_ERR_ARG, "BS has entries beyond the frame width");
+  }
   if (e) {
     delete h;
     return e;
@@ -375,7 +425,9 @@ extern "C" int sc_ctx_create(const sc_grid_desc* desc, const double* y_h, const 
       delete h;
       return fail(SC_ERR_ARG, "z coordinate must increase with k");
     }
-  hipError_t he = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+  // A blocking stream: host-side hipMemcpy / hipMemset on the null stream are
+  // ordered with the work queued here, so uploads and clears never race it.
+  hipError_t he = hipStreamCreateWithFlags(&c.stream, hipStreamDefault);
   if (he != hipSuccess) {
     delete h;
     return fail(SC_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
@@ -395,10 +447,15 @@ extern "C" int sc_ctx_destroy(sc_ctx* h) {
   if (!h) return SC_OK;
   sc::Ctx& c = h->c;
   if (c.stream) (void)hipStreamSynchronize(c.stream);
-  void* bufs[] = {c.mu64, c.mu32, c.tab64, c.tab32, c.diag64, c.dinv64, c.dinv32, c.r64,
-                  c.p64, c.q64, c.r32, c.p32, c.q32, c.d32, c.partials, c.scal};
+  void* bufs[] = {c.mu64, c.mu32, c.tab64, c.tab32, c.ctab64, c.ctab32, c.diag64, c.dinv64, c.dinv32, c.r64,
+                  c.p64, c.q64, c.r32, c.p32, c.q32, c.d32, c.partials, c.scal, c.z64,
+                  c.zold64};
+  delete c.fd;
   for (void* p : bufs)
     if (p) (void)hipFree(p);
+  for (double* p : c.projY)
+    if (p) (void)hipFree(p);
+  if (c.projW) (void)hipFree(c.projW);
   if (c.scal_h) (void)hipHostFree(c.scal_h);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete h;
@@ -422,6 +479,8 @@ extern "C" int sc_update_var_coeff(sc_ctx* h, const double* mu_h) {
   sc::Ctx& c = h->c;
   SC_HIP(hipStreamSynchronize(c.stream));
   c.mu_h.assign(mu_h, mu_h + c.N);
+  c.fd_stale = true;
+  c.projCount = c.projNext = 0;
   int e = sc::ctx_upload_mu(c);
   if (!e) e = sc::ctx_compute_diag(c);
   return e;
@@ -441,6 +500,7 @@ extern "C" int sc_change_bc_types(sc_ctx* h, int bcR, int bcT, int bcL, int bcB)
   c.op64.bcT = c.op32.bcT = bcT;
   c.op64.bcL = c.op32.bcL = bcL;
   c.op64.bcB = c.op32.bcB = bcB;
+  c.projCount = c.projNext = 0;
   return sc::ctx_compute_diag(c);
 }
 

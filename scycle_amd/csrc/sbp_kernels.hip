@@ -12,8 +12,10 @@
 //
 // Two kernels share the work: an interior kernel (rows/columns at least
 // kFrame from every boundary, where all stencils are the constant interior
-// ones) and a generic table-driven kernel for the frame, which also carries
-// the SAT boundary terms.
+// ones) and a frame kernel, one thread per node, which also carries the SAT
+// boundary terms and evaluates the boundary closures from the 1-D tables
+// (wave-uniform along the boundary normal).  Grids with a side of at most
+// 2*kFrame nodes use a per-node generic kernel instead.
 #include "sc_internal.hpp"
 
 namespace sc {
@@ -33,7 +35,7 @@ struct DeltaAcc {  // u = e_I, for extracting diag(A)
 
 // Generic 1-D operator along y at (j, k) (no H_z*zr prefactor).
 template <typename T, class U>
-__device__ T line_y(const DevOp<T>& p, const U& u, int j, int k) {
+__device__ __forceinline__ T line_y(const DevOp<T>& p, const U& u, int j, int k) {
   const DevDir<T>& d = p.y;
   const int Nz = p.Nz;
   const size_t N = (size_t)p.Ny * Nz;
@@ -89,7 +91,7 @@ __device__ T line_y(const DevOp<T>& p, const U& u, int j, int k) {
 
 // Generic 1-D operator along z at (j, k) (no H_y*yq prefactor).
 template <typename T, class U>
-__device__ T line_z(const DevOp<T>& p, const U& u, int j, int k) {
+__device__ __forceinline__ T line_z(const DevOp<T>& p, const U& u, int j, int k) {
   const DevDir<T>& d = p.z;
   const int Nz = p.Nz;
   const size_t N = (size_t)p.Ny * Nz;
@@ -157,7 +159,7 @@ __device__ __forceinline__ T d1_row(const DevDir<T>& d, int i, const F& f) {
 
 // SAT terms at (j, k) (sbpOps_m_varGrid.cpp:601-772 constructBC_*).
 template <typename T, class U>
-__device__ T sat_terms(const DevOp<T>& p, const U& u, int j, int k) {
+__device__ __forceinline__ T sat_terms(const DevOp<T>& p, const U& u, int j, int k) {
   T s = 0;
   const int Ny = p.Ny, Nz = p.Nz;
   if (p.doY) {
@@ -213,6 +215,47 @@ __device__ __forceinline__ T eval_node(const DevOp<T>& p, const U& u, int j, int
   return v + sat_terms(p, u, j, k);
 }
 
+// ------------------------------------------------------------ epilogues --
+// What the apply kernels do with (A u) at node (j, k), index I: `uc` is u at
+// the node, `edge` tells whether the node may lie on the domain boundary
+// (frame kernels) or certainly does not (interior kernel).
+template <typename T>
+struct StoreAu {  // out = A u
+  T* __restrict__ out;
+  __device__ __forceinline__ void operator()(int, int, size_t I, T Au, T, bool) const { out[I] = Au; }
+};
+
+// propagateWaves (strikeSlip_linearElastic_fd.cpp:445-516):
+//   D2u = Jinv * Hinv * (A u),  Jinv = qy*rz, Hinv = Hy^-1 (x) Hz^-1
+//   uNext = (dt^2/rho * D2u + 2u + (dt*ay - 1)*uPrev) / (dt*ay + 1)
+// a_y is zero off the domain boundary, where the update reduces exactly to
+// dt^2/rho*D2u + 2u - uPrev.  D2u on the fault (j = 0) is kept for the
+// fault update.
+template <typename T>
+struct WaveUpdate {
+  T* __restrict__ uNext;
+  const T* __restrict__ uPrev;
+  const T* __restrict__ rho;
+  const T* __restrict__ ay;
+  const T *hinvY, *hinvZ, *qy, *rz;
+  T* __restrict__ d2uFault;
+  T dt;
+  int varGrid;
+  __device__ __forceinline__ void operator()(int j, int k, size_t I, T Au, T uc, bool edge) const {
+    T d2u = (hinvY[j] * hinvZ[k]) * Au;
+    if (varGrid) d2u = (qy[j] * rz[k]) * d2u;
+    if (j == 0) d2uFault[k] = d2u;
+    const T c1 = dt * dt / rho[I];
+    if (edge) {
+      const T a = ay[I];
+      const T c2 = dt * a - T(1), c3 = dt * a + T(1);
+      uNext[I] = (c1 * d2u + T(2) * uc + c2 * uPrev[I]) / c3;
+    } else {
+      uNext[I] = c1 * d2u + T(2) * uc + T(-1) * uPrev[I];
+    }
+  }
+};
+
 // Frame node t -> (j, k).  Frame = rows [0,F) U [Ny-F,Ny) (all k) and, for
 // the middle rows, columns [0,F) U [Nz-F,Nz).
 struct FrameMap {
@@ -245,15 +288,16 @@ struct FrameMap {
   }
 };
 
-template <typename T>
+template <typename T, class EP>
 __global__ __launch_bounds__(256) void k_apply_frame(DevOp<T> p, FrameMap fm,
-                                                     const T* __restrict__ u, T* __restrict__ out) {
+                                                     const T* __restrict__ u, EP ep) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= fm.count()) return;
   int j, k;
   fm.node(t, j, k);
   FieldAcc<T> acc{u, p.Nz};
-  out[(size_t)j * p.Nz + k] = eval_node(p, acc, j, k);
+  const size_t I = (size_t)j * p.Nz + k;
+  ep(j, k, I, eval_node(p, acc, j, k), u[I], true);
 }
 
 // ------------------------------------------------------ interior kernel ---
@@ -263,17 +307,33 @@ __global__ __launch_bounds__(256) void k_apply_frame(DevOp<T> p, FrameMap fm,
 // registers (rows j0-4..j0+JC+3), y-direction fluxes are computed once per
 // row and reused by the transposed stencils, z-direction neighbours come from
 // cross-lane shuffles.
+//
+// Tiles are numbered down the column blocks (tile t: column block t / nby,
+// row block t % nby) so that vertically adjacent tiles, which share eight halo
+// rows of u, are neighbours.  With XCD = 1 the linear block id is remapped
+// so that each of the eight XCDs (which receive blocks round-robin) owns a
+// contiguous range of tiles and the halo rows stay in its own L2.
 constexpr int kJC = 8;
 constexpr int kLanesOut = 56;
+constexpr int kXcd = 8;
 
-template <typename T, int ORDER>
-__global__ __launch_bounds__(64) void k_apply_interior(DevOp<T> p, const T* __restrict__ u,
-                                                       T* __restrict__ out) {
+__device__ __forceinline__ int tile_of_block(int b, int total, int xcd) {
+  if (!xcd) return b;
+  const int q = total / kXcd, r = total % kXcd;
+  const int x = b % kXcd, slot = b / kXcd;
+  return x * q + min(x, r) + slot;
+}
+
+template <typename T, int ORDER, class EP, int XCD = 1>
+__global__ __launch_bounds__(64) void k_apply_interior(DevOp<T> p, const T* __restrict__ u, EP ep,
+                                                       int nbx, int nby) {
   constexpr int JC = kJC;
   const int Ny = p.Ny, Nz = p.Nz;
   const int lane = threadIdx.x;
-  const int k = kFrame + blockIdx.x * kLanesOut - 4 + lane;
-  const int j0 = kFrame + blockIdx.y * JC;
+  const int tile = tile_of_block(blockIdx.x, nbx * nby, XCD);
+  const int bx = tile / nby, by = tile - bx * nby;
+  const int k = kFrame + bx * kLanesOut - 4 + lane;
+  const int j0 = kFrame + by * JC;
   const int kc = k < Nz ? k : Nz - 1;  // k >= kFrame-4 >= 0 always
   T uw[JC + 8];
 #pragma unroll
@@ -358,36 +418,344 @@ __global__ __launch_bounds__(64) void k_apply_interior(DevOp<T> p, const T* __re
                    __shfl_down(w4, 2, 64);
       v += (dy.H[j < Ny ? j : Ny - 1] * dy.met[j < Ny ? j : Ny - 1]) * Lz;
     }
-    if (writer && j < Ny - kFrame) out[(size_t)j * Nz + k] = v;
+    if (writer && j < Ny - kFrame) ep(j, k, (size_t)j * Nz + k, v, uw[ro + 4], false);
   }
 }
 
+// ----------------------------------------------------------- frame nodes --
+// Interior (constant-stencil) 1-D operator at a node at least kFrame from both
+// ends of its line, evaluated straight from memory (no prefactor).  Used for
+// the direction tangential to the boundary in the frame kernel.
+template <typename T, int ORDER>
+__device__ __forceinline__ T y_interior(const DevOp<T>& p, const T* __restrict__ u, int j, int k) {
+  const DevDir<T>& d = p.y;
+  const int Nz = p.Nz;
+  T uu[9];
+#pragma unroll
+  for (int r = 0; r < 9; ++r) uu[r] = u[(size_t)(j - 4 + r) * Nz + k];
+  T g1[5], g3[5], g4[5];
+#pragma unroll
+  for (int s = 0; s < 5; ++s) {
+    const int m = j - 2 + s;
+    const T qm = d.imet[m];
+    const T cm = p.mu[(size_t)m * Nz + k] * qm;
+    g1[s] = cm * (d.d1[0] * uu[s] + d.d1[1] * uu[s + 1] + d.d1[3] * uu[s + 3] + d.d1[4] * uu[s + 4]);
+    if (ORDER == 4) {
+      // two-point average with global index I+1 (wraps to the next y-row at k = Nz-1)
+      const bool inRow = k + 1 < Nz;
+      const T mn = p.mu[inRow ? (size_t)m * Nz + k + 1 : (size_t)(m + 1) * Nz];
+      const T qn = d.imet[inRow ? m : m + 1];
+      const T c3 = T(0.5) * (cm + mn * qn);
+      g3[s] = d.wr * c3 * (d.dr[1] * uu[s + 1] + d.dr[2] * uu[s + 2] + d.dr[3] * uu[s + 3] + d.dr[4] * uu[s + 4]);
+      g4[s] = d.w4c * cm * (d.d4[0] * uu[s] + d.d4[1] * uu[s + 1] + d.d4[2] * uu[s + 2] +
+                            d.d4[3] * uu[s + 3] + d.d4[4] * uu[s + 4]);
+    } else {
+      g3[s] = d.wr * cm * (d.dr[1] * uu[s + 1] + d.dr[2] * uu[s + 2] + d.dr[3] * uu[s + 3]);
+      g4[s] = T(0);
+    }
+  }
+  T L = d.Hc * (d.d1[0] * g1[0] + d.d1[1] * g1[1] + d.d1[3] * g1[3] + d.d1[4] * g1[4]);
+  L -= (d.drt[0] * g3[0] + d.drt[1] * g3[1] + d.drt[2] * g3[2] + d.drt[3] * g3[3] + d.drt[4] * g3[4]) +
+       (d.d4t[0] * g4[0] + d.d4t[1] * g4[1] + d.d4t[2] * g4[2] + d.d4t[3] * g4[3] + d.d4t[4] * g4[4]);
+  return L;
+}
+
+template <typename T, int ORDER>
+__device__ __forceinline__ T z_interior(const DevOp<T>& p, const T* __restrict__ u, int j, int k) {
+  const DevDir<T>& d = p.z;
+  const size_t row = (size_t)j * p.Nz;
+  T uu[9];
+#pragma unroll
+  for (int r = 0; r < 9; ++r) uu[r] = u[row + k - 4 + r];
+  T g1[5], g3[5], g4[5];
+#pragma unroll
+  for (int s = 0; s < 5; ++s) {
+    const int m = k - 2 + s;
+    const T cm = p.mu[row + m] * d.imet[m];
+    g1[s] = cm * (d.d1[0] * uu[s] + d.d1[1] * uu[s + 1] + d.d1[3] * uu[s + 3] + d.d1[4] * uu[s + 4]);
+    if (ORDER == 4) {
+      const T c3 = T(0.5) * (cm + p.mu[row + m + 1] * d.imet[m + 1]);
+      g3[s] = d.wr * c3 * (d.dr[1] * uu[s + 1] + d.dr[2] * uu[s + 2] + d.dr[3] * uu[s + 3] + d.dr[4] * uu[s + 4]);
+      g4[s] = d.w4c * cm * (d.d4[0] * uu[s] + d.d4[1] * uu[s + 1] + d.d4[2] * uu[s + 2] +
+                            d.d4[3] * uu[s + 3] + d.d4[4] * uu[s + 4]);
+    } else {
+      g3[s] = d.wr * cm * (d.dr[1] * uu[s + 1] + d.dr[2] * uu[s + 2] + d.dr[3] * uu[s + 3]);
+      g4[s] = T(0);
+    }
+  }
+  T L = d.Hc * (d.d1[0] * g1[0] + d.d1[1] * g1[1] + d.d1[3] * g1[3] + d.d1[4] * g1[4]);
+  L -= (d.drt[0] * g3[0] + d.drt[1] * g3[1] + d.drt[2] * g3[2] + d.drt[3] * g3[3] + d.drt[4] * g3[4]) +
+       (d.d4t[0] * g4[0] + d.d4t[1] * g4[1] + d.d4t[2] * g4[2] + d.d4t[3] * g4[3] + d.d4t[4] * g4[4]);
+  return L;
+}
+
+// Copies the four packed closure tables (ClosureTab, built at context
+// creation) into LDS: every thread issues all of its 16-byte loads before its
+// first LDS store, so the copy costs one memory latency.
+constexpr int kFrameThreads = 128;
+
 template <typename T>
-static int launch_apply_t(const Ctx& c, const DevOp<T>& op, const T* in, T* out, hipStream_t s) {
+__device__ __forceinline__ void stage_closures(ClosureTab<T> (&tab)[4], const ClosureTab<T>* src) {
+  constexpr int n16 = (int)(4 * sizeof(ClosureTab<T>) / 16);
+  constexpr int per = (n16 + kFrameThreads - 1) / kFrameThreads;
+  const uint4* s = reinterpret_cast<const uint4*>(src);
+  uint4* d = reinterpret_cast<uint4*>(&tab[0]);
+  uint4 v[per];
+#pragma unroll
+  for (int i = 0; i < per; ++i) {  // unconditional (clamped) loads, all in flight together
+    const int e = threadIdx.x + i * kFrameThreads;
+    v[i] = s[e < n16 ? e : n16 - 1];
+  }
+#pragma unroll
+  for (int i = 0; i < per; ++i) {
+    const int e = threadIdx.x + i * kFrameThreads;
+    if (e < n16) d[e] = v[i];
+  }
+}
+
+// 1-D operator at line index i (within kFrame of one end of a line of length
+// n), in flux form with the boundary closures, from the window w[r] = u at
+// line index i-8+r (zero outside the line) and the coefficients at the nine
+// flux rows m = i-4+o: cw[o] = c and c3w[o] = its two-point average.  All
+// inputs are in registers before the loop, so the loads overlap.
+template <typename T, int ORDER>
+__device__ __forceinline__ T closure_line(const ClosureTab<T>& t, int n, int i,
+                                          const T (&w)[17], const T (&cw)[kBand],
+                                          const T (&c3w)[kBand]) {
+  const bool hi = i >= kFrame;  // n > 2*kFrame here
+  const int mb = hi ? n - 12 : 0, jb = hi ? n - kFrame : 0;
+  const int io = i - jb;
+  T a1 = 0, aR = 0, a4 = 0;
+#pragma unroll
+  for (int o = 0; o < kBand; ++o) {
+    const int m = i + o - 4;
+    if (m < 0 || m >= n) continue;  // outside the line: zero coefficients
+    const int fm = m - mb;
+    T s1 = 0, sR = 0, s4 = 0;
+#pragma unroll
+    for (int q = 0; q < kBand; ++q) {
+      const T x = w[o + q];
+      s1 += t.fD1[fm * kBand + q] * x;
+      sR += t.fDR[fm * kBand + q] * x;
+      if (ORDER == 4) s4 += t.fD4[fm * kBand + q] * x;
+    }
+    const T cm = cw[o];
+    a1 += t.oD1[io * kBand + o] * (cm * s1);
+    const T wr = t.wR[fm];
+    if (wr != T(0)) aR += t.oDRT[io * kBand + o] * (wr * (ORDER == 4 ? c3w[o] : cm) * sR);
+    if (ORDER == 4) {
+      const T w4 = t.w4[fm];
+      if (w4 != T(0)) a4 += t.oD4T[io * kBand + o] * (w4 * cm * s4);
+    }
+  }
+  return t.H[io] * a1 - (aR + a4);
+}
+
+// y-direction operator at (j, k), j in the row frame.
+template <typename T, int ORDER>
+__device__ __forceinline__ T y_closure(const DevOp<T>& p, const ClosureTab<T>& t,
+                                       const T* __restrict__ u, int j, int k) {
+  const int Ny = p.Ny, Nz = p.Nz;
+  const T* mu = p.mu;
+  const T* im = p.y.imet;
+  // every load is unconditional (clamped index, then a select): conditional
+  // loads would each cost a full memory latency
+  T w[17];
+#pragma unroll
+  for (int r = 0; r < 17; ++r) {
+    const int jj = j - 8 + r;
+    const T x = u[(size_t)min(max(jj, 0), Ny - 1) * Nz + k];
+    w[r] = (jj >= 0 && jj < Ny) ? x : T(0);
+  }
+  // c = mu*qy at (m, k); c3 = average with global index I+1: (m, k+1), or the
+  // first node of row m+1 when k is the last column, or I itself at the last
+  // node (0.5*(c+c) = c exactly)
+  T cw[kBand], c3w[kBand];
+#pragma unroll
+  for (int o = 0; o < kBand; ++o) {
+    const int m = min(max(j + o - 4, 0), Ny - 1);
+    const size_t I = (size_t)m * Nz + k;
+    const bool inRow = k + 1 < Nz, lastRow = m + 1 >= Ny;
+    const size_t In = inRow ? I + 1 : (lastRow ? I : (size_t)(m + 1) * Nz);
+    const int mn = (inRow || lastRow) ? m : m + 1;
+    const T cm = mu[I] * im[m];
+    cw[o] = cm;
+    c3w[o] = T(0.5) * (cm + mu[In] * im[mn]);
+  }
+  return closure_line<T, ORDER>(t, Ny, j, w, cw, c3w);
+}
+
+// z-direction operator at (j, k), k in the column frame.
+template <typename T, int ORDER>
+__device__ __forceinline__ T z_closure(const DevOp<T>& p, const ClosureTab<T>& t,
+                                       const T* __restrict__ u, int j, int k) {
+  const int Ny = p.Ny, Nz = p.Nz;
+  const size_t row = (size_t)j * Nz;
+  const T* mu = p.mu;
+  const T* im = p.z.imet;
+  T w[17];
+#pragma unroll
+  for (int r = 0; r < 17; ++r) {
+    const int kk = k - 8 + r;
+    const T x = u[row + min(max(kk, 0), Nz - 1)];
+    w[r] = (kk >= 0 && kk < Nz) ? x : T(0);
+  }
+  // c = mu*rz at flux row m; c3 = average with global index I+1: the next
+  // column, or the first node of the next row after the last column, or I
+  // itself at the last node (unconditional loads, as in y_closure)
+  const bool lastRow = j + 1 >= Ny;
+  T cw[kBand], c3w[kBand];
+#pragma unroll
+  for (int o = 0; o < kBand; ++o) {
+    const int m = min(max(k + o - 4, 0), Nz - 1);
+    const bool inRow = m + 1 < Nz;
+    const size_t In = inRow ? row + m + 1 : (lastRow ? row + m : row + Nz);
+    const int kn = inRow ? m + 1 : (lastRow ? m : 0);
+    const T cm = mu[row + m] * im[m];
+    cw[o] = cm;
+    c3w[o] = T(0.5) * (cm + mu[In] * im[kn]);
+  }
+  return closure_line<T, ORDER>(t, Nz, k, w, cw, c3w);
+}
+
+// SAT terms at a frame node (constructBC_* of sbpOps_m_varGrid.cpp:601-772),
+// with BS and the boundary rows of D1 read from the staged tables.  BS has no
+// entries beyond kFrame, so only frame nodes carry SAT terms.
+template <typename T>
+__device__ __forceinline__ T sat_frame(const DevOp<T>& p, const ClosureTab<T> (&tab)[4],
+                                       const T* __restrict__ u, int j, int k) {
+  T s = 0;
+  const int Ny = p.Ny, Nz = p.Nz;
+  if (p.doY) {
+    const DevDir<T>& d = p.y;
+    const T pre = p.z.H[k] * p.z.met[k];
+    if (j < kFrame) {
+      if (p.bcL == SC_DIRICHLET) {
+        const T bs = tab[0].BS[j] + (j == 0 ? d.alphaD : T(0));
+        if (bs != T(0)) s += pre * (bs * (p.mu[k] * d.imet[0])) * u[k];
+      } else if (j == 0) {
+        T dv = 0;
+        for (int q = 4; q < kBand; ++q) dv += tab[0].fD1[q] * u[(size_t)(q - 4) * Nz + k];
+        s += pre * (p.mu[k] * (d.imet[0] * dv));
+      }
+    }
+    if (j >= Ny - kFrame) {
+      const size_t IN = (size_t)(Ny - 1) * Nz + k;
+      if (p.bcR == SC_DIRICHLET) {
+        const T bs = tab[1].BS[j - (Ny - kBand)] + (j == Ny - 1 ? d.alphaD : T(0));
+        if (bs != T(0)) s += pre * (bs * (p.mu[IN] * d.imet[Ny - 1])) * u[IN];
+      } else if (j == Ny - 1) {
+        T dv = 0;
+        for (int q = 0; q < 5; ++q) dv += tab[1].fD1[11 * kBand + q] * u[(size_t)(Ny - 5 + q) * Nz + k];
+        s -= pre * (p.mu[IN] * (d.imet[Ny - 1] * dv));
+      }
+    }
+  }
+  if (p.doZ) {
+    const DevDir<T>& d = p.z;
+    const T pre = p.y.H[j] * p.y.met[j];
+    const size_t row = (size_t)j * Nz;
+    if (k < kFrame) {
+      if (p.bcT == SC_DIRICHLET) {
+        const T bs = tab[2].BS[k] + (k == 0 ? d.alphaD : T(0));
+        if (bs != T(0)) s += pre * (bs * (p.mu[row] * d.imet[0])) * u[row];
+      } else if (k == 0) {
+        T dv = 0;
+        for (int q = 4; q < kBand; ++q) dv += tab[2].fD1[q] * u[row + q - 4];
+        s += pre * (p.mu[row] * (d.imet[0] * dv));
+      }
+    }
+    if (k >= Nz - kFrame) {
+      if (p.bcB == SC_DIRICHLET) {
+        const T bs = tab[3].BS[k - (Nz - kBand)] + (k == Nz - 1 ? d.alphaD : T(0));
+        if (bs != T(0)) s += pre * (bs * (p.mu[row + Nz - 1] * d.imet[Nz - 1])) * u[row + Nz - 1];
+      } else if (k == Nz - 1) {
+        T dv = 0;
+        for (int q = 0; q < 5; ++q) dv += tab[3].fD1[11 * kBand + q] * u[row + Nz - 5 + q];
+        s -= pre * (p.mu[row + Nz - 1] * (d.imet[Nz - 1] * dv));
+      }
+    }
+  }
+  return s;
+}
+
+// Frame kernel for grids with both sides above 2*kFrame nodes: one thread
+// per frame node (FrameMap order), closure tables of both directions and
+// both ends staged in LDS once per block.
+
+template <typename T, int ORDER, class EP>
+__global__ __launch_bounds__(kFrameThreads) void k_apply_frame_lds(DevOp<T> p, FrameMap fm,
+                                                                   const T* __restrict__ u, EP ep) {
+  __shared__ __align__(16) ClosureTab<T> tab[4];  // y low, y high, z low, z high
+  stage_closures(tab, p.ctab);
+  __syncthreads();
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= fm.count()) return;
+  int j, k;
+  fm.node(t, j, k);
+  const int Ny = p.Ny, Nz = p.Nz;
+  T v = 0;
+  if (p.doY) {
+    const bool yb = j < kFrame || j >= Ny - kFrame;
+    const T Ly = yb ? y_closure<T, ORDER>(p, tab[j >= kFrame ? 1 : 0], u, j, k)
+                    : y_interior<T, ORDER>(p, u, j, k);
+    v += (p.z.H[k] * p.z.met[k]) * Ly;
+  }
+  if (p.doZ) {
+    const bool zb = k < kFrame || k >= Nz - kFrame;
+    const T Lz = zb ? z_closure<T, ORDER>(p, tab[k >= kFrame ? 3 : 2], u, j, k)
+                    : z_interior<T, ORDER>(p, u, j, k);
+    v += (p.y.H[j] * p.y.met[j]) * Lz;
+  }
+  const size_t I = (size_t)j * Nz + k;
+  ep(j, k, I, v + sat_frame(p, tab, u, j, k), u[I], true);
+}
+
+template <typename T, class EP>
+static int launch_apply_t(const Ctx& c, const DevOp<T>& op, const T* in, const EP& ep, hipStream_t s) {
   const int Ny = c.d.Ny, Nz = c.d.Nz;
   FrameMap fm = FrameMap::make(Ny, Nz);
   const long long nf = fm.count();
-  if (nf > 0) {
+  if (Ny > 2 * kFrame && Nz > 2 * kFrame) {
+    const unsigned nb = (unsigned)((nf + kFrameThreads - 1) / kFrameThreads);
+    if (c.d.order == 4)
+      hipLaunchKernelGGL((k_apply_frame_lds<T, 4, EP>), dim3(nb), dim3(kFrameThreads), 0, s, op, fm, in, ep);
+    else
+      hipLaunchKernelGGL((k_apply_frame_lds<T, 2, EP>), dim3(nb), dim3(kFrameThreads), 0, s, op, fm, in, ep);
+  } else if (nf > 0) {
     const unsigned nb = (unsigned)((nf + 255) / 256);
-    hipLaunchKernelGGL(k_apply_frame<T>, dim3(nb), dim3(256), 0, s, op, fm, in, out);
+    hipLaunchKernelGGL((k_apply_frame<T, EP>), dim3(nb), dim3(256), 0, s, op, fm, in, ep);
   }
   const int niy = Ny - 2 * kFrame, niz = Nz - 2 * kFrame;
   if (niy > 0 && niz > 0) {
-    dim3 grid((niz + kLanesOut - 1) / kLanesOut, (niy + kJC - 1) / kJC);
+    const int nbx = (niz + kLanesOut - 1) / kLanesOut, nby = (niy + kJC - 1) / kJC;
     if (c.d.order == 4)
-      hipLaunchKernelGGL((k_apply_interior<T, 4>), grid, dim3(64), 0, s, op, in, out);
+      hipLaunchKernelGGL((k_apply_interior<T, 4, EP>), dim3(nbx * nby), dim3(64), 0, s, op, in, ep, nbx, nby);
     else
-      hipLaunchKernelGGL((k_apply_interior<T, 2>), grid, dim3(64), 0, s, op, in, out);
+      hipLaunchKernelGGL((k_apply_interior<T, 2, EP>), dim3(nbx * nby), dim3(64), 0, s, op, in, ep, nbx, nby);
   }
   SC_HIP(hipGetLastError());
   return SC_OK;
 }
 
 int launch_apply64(const Ctx& c, const double* in, double* out, hipStream_t s) {
-  return launch_apply_t<double>(c, c.op64, in, out, s);
+  return launch_apply_t(c, c.op64, in, StoreAu<double>{out}, s);
 }
 int launch_apply32(const Ctx& c, const float* in, float* out, hipStream_t s) {
-  return launch_apply_t<float>(c, c.op32, in, out, s);
+  return launch_apply_t(c, c.op32, in, StoreAu<float>{out}, s);
+}
+
+template <typename T>
+static int launch_wave_t(const Ctx& c, const DevOp<T>& op, const WaveArgs<T>& a, hipStream_t s) {
+  WaveUpdate<T> ep{a.uNext, a.uPrev, a.rho, a.ay, op.y.Hinv, op.z.Hinv, op.y.imet, op.z.imet,
+                   a.d2uFault, a.dt, c.d.var_grid};
+  return launch_apply_t(c, op, a.u, ep, s);
+}
+int launch_wave64(const Ctx& c, const WaveArgs<double>& a, hipStream_t s) {
+  return launch_wave_t(c, c.op64, a, s);
+}
+int launch_wave32(const Ctx& c, const WaveArgs<float>& a, hipStream_t s) {
+  return launch_wave_t(c, c.op32, a, s);
 }
 
 // ----------------------------------------------------------------- diag ---
@@ -564,21 +932,71 @@ extern "C" int sc_sbp_mux_dz(sc_ctx* h, const double* in, double* out) { return 
 extern "C" int sc_sbp_h(sc_ctx* h, const double* in, double* out) { return deriv(h, 4, in, out); }
 extern "C" int sc_sbp_hinv(sc_ctx* h, const double* in, double* out) { return deriv(h, 5, in, out); }
 
+extern "C" int sc_wave_step(sc_ctx* h, int prec, double dt, const void* u, const void* uPrev,
+                            void* uNext, const void* rho, const void* ay, void* d2uFault) {
+  if (!h || !u || !uPrev || !uNext || !rho || !ay || !d2uFault)
+    return fail(SC_ERR_ARG, "sc_wave_step: null argument");
+  if (u == uNext || uPrev == uNext) return fail(SC_ERR_ARG, "sc_wave_step: uNext must be a separate buffer");
+  if (!(dt > 0)) return fail(SC_ERR_ARG, "sc_wave_step: dt must be > 0");
+  const sc::Ctx& c = h->c;
+  if (c.d.laplace != 0) return fail(SC_ERR_ARG, "sc_wave_step: needs laplace type yz");
+  if (prec == SC_FP64)
+    return sc::launch_wave64(c, {(const double*)u, (const double*)uPrev, (double*)uNext, (const double*)rho,
+                                 (const double*)ay, (double*)d2uFault, dt}, c.stream);
+  if (prec == SC_FP32)
+    return sc::launch_wave32(c, {(const float*)u, (const float*)uPrev, (float*)uNext, (const float*)rho,
+                                 (const float*)ay, (float*)d2uFault, (float)dt}, c.stream);
+  return fail(SC_ERR_ARG, "sc_wave_step: prec must be SC_FP64 or SC_FP32");
+}
+
+namespace sc {
+// single-kernel launches for timing: interior kernel with a given tile map, or
+// the frame kernel alone
+template <typename T, int XCD>
+static void bench_interior(const Ctx& c, const DevOp<T>& op, const T* a, T* b) {
+  const int niy = c.d.Ny - 2 * kFrame, niz = c.d.Nz - 2 * kFrame;
+  const int nbx = (niz + kLanesOut - 1) / kLanesOut, nby = (niy + kJC - 1) / kJC;
+  if (c.d.order == 4)
+    hipLaunchKernelGGL((k_apply_interior<T, 4, StoreAu<T>, XCD>), dim3(nbx * nby), dim3(64), 0, c.stream,
+                       op, a, StoreAu<T>{b}, nbx, nby);
+  else
+    hipLaunchKernelGGL((k_apply_interior<T, 2, StoreAu<T>, XCD>), dim3(nbx * nby), dim3(64), 0, c.stream,
+                       op, a, StoreAu<T>{b}, nbx, nby);
+}
+template <typename T>
+static void bench_frame(const Ctx& c, const DevOp<T>& op, const T* a, T* b) {
+  FrameMap fm = FrameMap::make(c.d.Ny, c.d.Nz);
+  const unsigned nb = (unsigned)((fm.count() + kFrameThreads - 1) / kFrameThreads);
+  if (c.d.order == 4)
+    hipLaunchKernelGGL((k_apply_frame_lds<T, 4, StoreAu<T>>), dim3(nb), dim3(kFrameThreads), 0, c.stream,
+                       op, fm, a, StoreAu<T>{b});
+  else
+    hipLaunchKernelGGL((k_apply_frame_lds<T, 2, StoreAu<T>>), dim3(nb), dim3(kFrameThreads), 0, c.stream,
+                       op, fm, a, StoreAu<T>{b});
+}
+}  // namespace sc
+
 // Times `reps` back-to-back launches on the context stream with HIP events
 // (the stream the kernels run on); returns the mean time per launch in ms and
 // the number of nodes the timed kernel(s) write.  which: 0 = FP64 apply,
-// 1 = FP32 apply, 2 = FP64 interior kernel only, 3 = FP32 interior kernel only.
+// 1 = FP32 apply, 2 / 3 = FP64 / FP32 interior kernel only, 4 / 5 = the same
+// with the plain (not XCD-grouped) tile order, 6 / 7 = FP64 / FP32 frame
+// kernel only.
 extern "C" int sc_bench_apply(sc_ctx* h, int which, int reps, double* ms_per_launch,
                               double* nodes) {
-  if (!h || reps <= 0 || !ms_per_launch || which < 0 || which > 3)
+  if (!h || reps <= 0 || !ms_per_launch || which < 0 || which > 7)
     return fail(SC_ERR_ARG, "sc_bench_apply: bad argument");
   sc::Ctx& c = h->c;
-  const bool f64 = which == 0 || which == 2;
+  const bool f64 = which % 2 == 0;
   const size_t N = (size_t)c.N, esz = f64 ? 8 : 4;
   const int Ny = c.d.Ny, Nz = c.d.Nz;
   const int niy = Ny - 2 * sc::kFrame, niz = Nz - 2 * sc::kFrame;
   if (which >= 2 && (niy <= 0 || niz <= 0)) return fail(SC_ERR_ARG, "grid has no interior block");
-  if (nodes) *nodes = which >= 2 ? (double)niy * niz : (double)N;
+  if (nodes) {
+    if (which < 2) *nodes = (double)N;
+    else if (which < 6) *nodes = (double)niy * niz;
+    else *nodes = (double)N - (double)niy * niz;
+  }
   void *a = nullptr, *b = nullptr;
   SC_HIP(hipMalloc(&a, N * esz));
   SC_HIP(hipMalloc(&b, N * esz));
@@ -586,19 +1004,18 @@ extern "C" int sc_bench_apply(sc_ctx* h, int which, int reps, double* ms_per_lau
   hipEvent_t e0, e1;
   SC_HIP(hipEventCreate(&e0));
   SC_HIP(hipEventCreate(&e1));
-  dim3 grid((niz + sc::kLanesOut - 1) / sc::kLanesOut, (niy + sc::kJC - 1) / sc::kJC);
   auto launch = [&]() -> int {
+    const double *a64 = (const double*)a;
+    const float* a32 = (const float*)a;
     switch (which) {
-      case 0: return sc::launch_apply64(c, (double*)a, (double*)b, c.stream);
-      case 1: return sc::launch_apply32(c, (float*)a, (float*)b, c.stream);
-      case 2:
-        if (c.d.order == 4) hipLaunchKernelGGL((sc::k_apply_interior<double, 4>), grid, dim3(64), 0, c.stream, c.op64, (const double*)a, (double*)b);
-        else hipLaunchKernelGGL((sc::k_apply_interior<double, 2>), grid, dim3(64), 0, c.stream, c.op64, (const double*)a, (double*)b);
-        return 0;
-      default:
-        if (c.d.order == 4) hipLaunchKernelGGL((sc::k_apply_interior<float, 4>), grid, dim3(64), 0, c.stream, c.op32, (const float*)a, (float*)b);
-        else hipLaunchKernelGGL((sc::k_apply_interior<float, 2>), grid, dim3(64), 0, c.stream, c.op32, (const float*)a, (float*)b);
-        return 0;
+      case 0: return sc::launch_apply64(c, a64, (double*)b, c.stream);
+      case 1: return sc::launch_apply32(c, a32, (float*)b, c.stream);
+      case 2: sc::bench_interior<double, 1>(c, c.op64, a64, (double*)b); return 0;
+      case 3: sc::bench_interior<float, 1>(c, c.op32, a32, (float*)b); return 0;
+      case 4: sc::bench_interior<double, 0>(c, c.op64, a64, (double*)b); return 0;
+      case 5: sc::bench_interior<float, 0>(c, c.op32, a32, (float*)b); return 0;
+      case 6: sc::bench_frame<double>(c, c.op64, a64, (double*)b); return 0;
+      default: sc::bench_frame<float>(c, c.op32, a32, (float*)b); return 0;
     }
   };
   int err = 0;

@@ -9,6 +9,7 @@
 
 namespace sc {
 
+struct FdDiag;
 constexpr int kBand = 9;     // band width of every 1-D table (offsets -4..+4)
 constexpr int kFrame = 8;    // frame width handled by the generic kernel
 
@@ -51,6 +52,18 @@ struct DevDir {
   T d1[5], dr[5], d4[5], drt[5], d4t[5], wr, w4c, Hc;
 };
 
+// Boundary-closure tables of one direction and one end, in the layout the
+// frame kernel stages into LDS: flux rows mb..mb+11 (D1, DR, D4 and their
+// weights) and output rows jb..jb+7 (D1, DR^T, D4^T, H), with mb = jb = 0 at
+// the low end and mb = n-12, jb = n-8 at the high end, plus the SAT row of BS.
+template <typename T>
+struct ClosureTab {
+  T fD1[12 * kBand], fDR[12 * kBand], fD4[12 * kBand];
+  T oD1[kFrame * kBand], oDRT[kFrame * kBand], oD4T[kFrame * kBand];
+  T wR[12], w4[12], H[kFrame], BS[kBand];
+};
+static_assert((4 * sizeof(ClosureTab<float>)) % 16 == 0, "closure tables copy as 16-byte words");
+
 template <typename T>
 struct DevOp {
   int Ny, Nz, order;
@@ -58,6 +71,7 @@ struct DevOp {
   int bcR, bcT, bcL, bcB;
   const T* mu;                   // N
   DevDir<T> y, z;
+  const ClosureTab<T>* ctab;     // [y low, y high, z low, z high] (n > 2*kFrame only)
 };
 
 // ----------------------------------------------------------- context -----
@@ -78,6 +92,8 @@ struct Ctx {
   float* mu32 = nullptr;
   void* tab64 = nullptr;
   void* tab32 = nullptr;
+  void* ctab64 = nullptr;              // packed closure tables (ClosureTab x4)
+  void* ctab32 = nullptr;
   DevOp<double> op64{};
   DevOp<float> op32{};
   double* diag64 = nullptr;            // diag(A)
@@ -86,7 +102,16 @@ struct Ctx {
   // solver workspace (lazily allocated)
   double *r64 = nullptr, *p64 = nullptr, *q64 = nullptr;
   float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *d32 = nullptr;
+  double* z64 = nullptr;               // preconditioned residual
+  double* zold64 = nullptr;            // previous z (flexible CG)
+  struct FdDiag* fd = nullptr;         // fast-diagonalisation factors (lazy)
+  bool fd_stale = false;
   double* partials = nullptr;          // reduction scratch
+  // previous solutions kept for the initial guess (ring buffer of A-orthogonal
+  // vectors y_i with weights w_i = 1 / (y_i^T (-A) y_i) on the device)
+  double* projY[SC_REUSE_MAX] = {};
+  double* projW = nullptr;             // w[SC_REUSE_MAX], coef[SC_REUSE_MAX]
+  int projCap = 0, projCount = 0, projNext = 0;
   void* scal = nullptr;                // CgScalars (device)
   void* scal_h = nullptr;              // pinned host mirror
   int nPartial = 0;
@@ -100,6 +125,20 @@ int ctx_compute_diag(Ctx& c);
 int launch_apply64(const Ctx& c, const double* in, double* out, hipStream_t s);
 int launch_apply32(const Ctx& c, const float* in, float* out, hipStream_t s);
 int launch_diag(const Ctx& c, double* out, hipStream_t s);
+// one explicit wave-equation step of the body (propagateWaves): uNext from u,
+// uPrev; D2u on the fault column into d2uFault (Nz)
+template <typename T>
+struct WaveArgs {
+  const T* u;
+  const T* uPrev;
+  T* uNext;
+  const T* rho;
+  const T* ay;
+  T* d2uFault;
+  T dt;
+};
+int launch_wave64(const Ctx& c, const WaveArgs<double>& a, hipStream_t s);
+int launch_wave32(const Ctx& c, const WaveArgs<float>& a, hipStream_t s);
 int launch_set_rhs(const Ctx& c, double* rhs, const double* bcL, const double* bcR,
                    const double* bcT, const double* bcB, hipStream_t s);
 // which: 0 Dy, 1 muxDy, 2 Dz, 3 muxDz, 4 H, 5 Hinv
@@ -109,6 +148,9 @@ int launch_fault_sxy(const Ctx& c, const double* u, double* out, hipStream_t s);
 
 int solve(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st);
 int fault_qd_rates(hipStream_t s, const sc_fault_qd_args& a);
+// FD fault step; uBody (may be null) receives the fault displacement
+int fault_fd_step64(hipStream_t s, const sc_fault_fd_args& a, double* uBody);
+int fault_fd_step32(hipStream_t s, const sc_fault_fd_args& a, const float* d2u32, float* uBody);
 
 }  // namespace sc
 

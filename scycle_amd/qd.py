@@ -12,12 +12,16 @@ SOLVE_MODES = {"cg64": 0, "ir32": 1}
 
 class QdMediator:
     def __init__(self, cfg, solve="cg64", rtol=1e-12, inner_rtol=1e-5, jac_mode=1,
-                 check_every=64, max_iters=400000):
-        """cfg: a scycle_amd.configs.QdConfig."""
+                 check_every=64, max_iters=400000, precond="jacobi", reuse=0):
+        """cfg: a scycle_amd.configs.QdConfig; solve "cg64" or "ir32";
+        precond "jacobi", "fdiag64", "fdiag32" or "fdiag_bf16" (cg64 only);
+        reuse: number of previous solutions projected for the initial guess."""
+        from .sbp_ops import PRECONDS
         self.cfg = cfg
         g = GridDesc(cfg.Ny, cfg.Nz, cfg.Ly, cfg.Lz, cfg.order, int(cfg.var_grid), 0, 0,
                      *cfg.mat_bc)
-        so = SolveOpts(SOLVE_MODES[solve], rtol, inner_rtol, max_iters, 50, check_every)
+        so = SolveOpts(SOLVE_MODES[solve], rtol, inner_rtol, max_iters, 50, check_every,
+                       PRECONDS[precond], reuse)
         d = QdDesc(g, int(cfg.remote_loading_R), cfg.fault_type_scale, cfg.vL, cfg.f0, cfg.v0,
                    cfg.root_tol, 10000, cfg.state_law, jac_mode, so)
         self._keep = [np.ascontiguousarray(x, dtype=np.float64) if x is not None else None

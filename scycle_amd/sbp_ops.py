@@ -11,6 +11,7 @@ from .device import DeviceArray
 
 DIRICHLET, NEUMANN = 0, 1
 _LAPLACE = {"yz": 0, "y": 1, "z": 2}
+PRECONDS = {"jacobi": 0, "fdiag64": 1, "fdiag32": 2, "fdiag_bf16": 3}
 
 
 class SbpOps:
@@ -110,11 +111,15 @@ class SbpOps:
         return yq, zr
 
     def solve(self, rhs, u0=None, mode="cg64", rtol=1e-12, inner_rtol=1e-5, max_iters=200000,
-              check_every=64):
-        """KSPSolve(A, rhs, u) with u0 as the initial guess; returns (u, stats)."""
+              check_every=64, precond="jacobi", reuse=0):
+        """KSPSolve(A, rhs, u) with u0 as the initial guess; returns (u, stats).
+        precond: "jacobi", "fdiag64", "fdiag32" or "fdiag_bf16"; reuse: number
+        of previous solutions of this operator kept to project the initial
+        guess (0: start from u0)."""
         b = DeviceArray(self.N, host=rhs)
         u = DeviceArray(self.N, host=np.zeros(self.N) if u0 is None else u0)
-        o = SolveOpts(0 if mode == "cg64" else 1, rtol, inner_rtol, max_iters, 50, check_every)
+        o = SolveOpts(0 if mode == "cg64" else 1, rtol, inner_rtol, max_iters, 50, check_every,
+                      PRECONDS[precond], reuse)
         st = SolveStats()
         check(load().sc_solve(self.h, b.ptr, u.ptr, C.byref(o), C.byref(st)))
         self.sync()

@@ -197,3 +197,45 @@ def vel_qd(eta, tauQS, sN, psi, a, v0, vL=1e-9, tol=1e-12, maxits=10000, jac_mod
     bad = lib().orc_vel_qd(n, *[ptr(x) for x in arrs], ptr(lk), v0, vL, tol, maxits, jac_mode,
                            ptr(V), C.byref(its))
     return V, bad, its.value
+
+
+def fd_propagate(o, dt, u, u_prev, rho, ay):
+    """propagateWaves on the oracle operator `o`: (uNext, D2u)."""
+    N = len(u)
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (u, u_prev, rho, ay)]
+    u_next, d2u = np.zeros(N), np.zeros(N)
+    lib().orc_fd_propagate(o.h, dt, *[ptr(x) for x in arrs], ptr(u_next), ptr(d2u))
+    return u_next, d2u
+
+
+def fd_ay(o, cs, outR, outT, outB, faultL):
+    ay = np.zeros(len(cs))
+    lib().orc_fd_ay(o.h, ptr(np.ascontiguousarray(cs, dtype=np.float64)), int(outR), int(outT),
+                    int(outB), int(faultL), ptr(ay))
+    return ay
+
+
+class FdFaultOracle:
+    """Fault_fd::d_dt of the oracle (one step at a time)."""
+
+    def __init__(self, N, v0, f0, vL, rootTol, fts, stateLaw, a, b, Dc, sNEff, rho, alphay,
+                 tau0, slip0):
+        self._keep = [np.ascontiguousarray(x, dtype=np.float64) for x in
+                      (a, b, Dc, sNEff, rho, alphay, tau0, slip0)]
+        self.N = N
+        self.h = lib().orc_fdfault_new(N, v0, f0, vL, rootTol, fts, stateLaw,
+                                       *[ptr(x) for x in self._keep])
+
+    def __del__(self):
+        try:
+            lib().orc_fdfault_free(self.h)
+        except Exception:
+            pass
+
+    def step(self, dt, d2u, V, psi, psi_prev, slip, slip_prev):
+        """Returns (psiNext, slipNext, uFault, V, bad); V is updated in place."""
+        arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (d2u, psi, psi_prev, slip, slip_prev)]
+        psi_n, slip_n, uf = np.zeros(self.N), np.zeros(self.N), np.zeros(self.N)
+        bad = lib().orc_fdfault_step(self.h, dt, ptr(arrs[0]), ptr(V), ptr(arrs[1]), ptr(arrs[2]),
+                                     ptr(arrs[3]), ptr(arrs[4]), ptr(psi_n), ptr(slip_n), ptr(uf))
+        return psi_n, slip_n, uf, V, bad

@@ -71,7 +71,18 @@ def test_fault_kernel_parity(jac_mode, state_law):
     dref = np.array([L.orc_aging(p, v, b, 0.6, 1e-6, dc) if state_law == 0 else
                      L.orc_sliplaw(p, v, a, b, 0.6, 1e-6, dc)
                      for p, v, a, b, dc in zip(inp["psi"], g["V"], inp["a"], inp["b"], inp["Dc"])])
-    assert np.allclose(g["dpsi"], dref, rtol=1e-12, atol=1e-300)
+    # Both laws are differences of nearly equal terms near steady state, so the
+    # bound is relative to the magnitude of the terms (libm ulps differ between
+    # the device and host exp/log/asinh): aging (b v0/Dc)(|e^((f0-psi)/b)| + V/v0),
+    # slip |V|/Dc (|f| + |fss|).
+    V, a, b, Dc, psi = g["V"], inp["a"], inp["b"], inp["Dc"], inp["psi"]
+    if state_law == 0:
+        scale = (b * 1e-6 / Dc) * (np.exp((0.6 - psi) / b) + V / 1e-6)
+    else:
+        f = a * np.arcsinh(V / 2e-6 * np.exp(psi / a))
+        fss = 0.6 + (a - b) * np.log(np.maximum(V, 1e-300) / 1e-6)
+        scale = V / Dc * (np.abs(f) + np.abs(fss))
+    assert np.all(np.abs(g["dpsi"] - dref) <= 1e-13 * scale + 1e-300)
     assert np.allclose(g["tau"], inp["tauQS"] - inp["eta"] * g["V"], rtol=1e-12)
 
 
@@ -124,44 +135,64 @@ def test_qd_ddt_parity(order, var_grid, N):
     L.orc_qd_free(h)
 
 
+class _Run(C.Structure):
+    _fields_ = [("integrator", C.c_int), ("control", C.c_int), ("normType", C.c_int),
+                ("maxSteps", C.c_int), ("initTime", C.c_double), ("maxTime", C.c_double),
+                ("initDt", C.c_double), ("minDt", C.c_double), ("maxDt", C.c_double),
+                ("tol", C.c_double), ("nReplay", C.c_int), ("replayDt", C.c_void_p)]
+
+
+def _fields_err(gs, ug, ref):
+    out = {}
+    for name in ("slip", "psi", "V", "tau"):
+        out[name] = np.abs(gs[name] - ref[name]).max() / np.abs(ref[name]).max()
+    out["u"] = np.abs(ug - ref["u"]).max() / np.abs(ref["u"]).max()
+    return out
+
+
 @pytest.mark.parametrize("integrator", ["RK32", "RK43"])
 @pytest.mark.parametrize("solve", ["cg64", "ir32"])
 def test_qd_integrate_parity(integrator, solve):
-    """Adaptive RK steps from the same state: step sizes, slip, state, slip
-    velocity, shear stress and off-fault displacement against the oracle."""
+    """40 adaptive RK steps from the same state against the oracle (direct
+    Cholesky solves).  Replay mode forces the oracle's step sizes on the GPU
+    run: fault slip, state, slip velocity, shear stress and off-fault
+    displacement then agree to 1e-9 relative.  In a free adaptive run the PID
+    controller maps rounding-level differences of the error estimate
+    (itself a difference of two nearly equal RK solutions) into the step
+    sizes; they agree to 1e-6 and the fields to 1e-7."""
     from scycle_amd import QdMediator
     N = 25
     cfg = _small_cfg(N, N, 4, True)
     steps = 40
+    h, keep, _ = _oracle_qd(cfg)
+    r = _Run(0 if integrator == "RK32" else 1, 1, 0, steps, 0.0, 1e15, 1e-3, 0.0, 1e10, 1e-8, 0,
+             None)
+    ref = dict(psi=cfg.psi0.copy(), slip=cfg.slip0.copy(), V=np.zeros(N), tau=np.zeros(N),
+               u=np.zeros(N * N))
+    tE = C.c_double()
+    odts = np.zeros(steps)
+    ns = olib().orc_qd_integrate(h, C.byref(r), optr(ref["psi"]), optr(ref["slip"]),
+                                 optr(ref["V"]), optr(ref["tau"]), optr(ref["u"]), C.byref(tE),
+                                 optr(odts), steps, None)
+    olib().orc_qd_free(h)
+    assert ns == steps
+    # replay: identical step sizes
     g = QdMediator(cfg, solve=solve, rtol=1e-13, jac_mode=0)
     g.set_state(cfg.psi0, cfg.slip0, np.zeros(N))
     tEnd, nst, dts = g.integrate(steps, integrator=integrator, tol=1e-8, init_dt=1e-3,
-                                 max_dt=1e10)
-    gs = g.get_state()
-    h, keep, _ = _oracle_qd(cfg)
-
-    class Run(C.Structure):
-        _fields_ = [("integrator", C.c_int), ("control", C.c_int), ("normType", C.c_int),
-                    ("maxSteps", C.c_int), ("initTime", C.c_double), ("maxTime", C.c_double),
-                    ("initDt", C.c_double), ("minDt", C.c_double), ("maxDt", C.c_double),
-                    ("tol", C.c_double), ("nReplay", C.c_int), ("replayDt", C.c_void_p)]
-    r = Run(0 if integrator == "RK32" else 1, 1, 0, steps, 0.0, 1e15, 1e-3, 0.0, 1e10, 1e-8, 0,
-            None)
-    psi, slip, V, tau = cfg.psi0.copy(), cfg.slip0.copy(), np.zeros(N), np.zeros(N)
-    u = np.zeros(N * N)
-    tE = C.c_double()
-    odts = np.zeros(steps)
-    ns = olib().orc_qd_integrate(h, C.byref(r), optr(psi), optr(slip), optr(V), optr(tau),
-                                 optr(u), C.byref(tE), optr(odts), steps, None)
-    assert ns == nst == steps
-    print(integrator, solve, "tEnd", tEnd, tE.value, flush=True)
-    assert abs(tEnd - tE.value) <= 1e-9 * tE.value
-    assert np.allclose(dts, odts[:steps], rtol=1e-8)
-    for name, a, b in (("slip", gs["slip"], slip), ("psi", gs["psi"], psi), ("V", gs["V"], V),
-                       ("tau", gs["tau"], tau)):
-        err = np.abs(a - b).max() / np.abs(b).max()
-        print(name, err, flush=True)
-        assert err <= 1e-8, name
-    ug = g.get_u()
-    assert np.abs(ug - u).max() <= 1e-8 * np.abs(u).max()
-    olib().orc_qd_free(h)
+                                 max_dt=1e10, replay_dt=odts)
+    assert nst == steps and abs(tEnd - tE.value) <= 1e-14 * tE.value
+    e = _fields_err(g.get_state(), g.get_u(), ref)
+    print(integrator, solve, "replay", e, flush=True)
+    assert max(e.values()) <= 1e-9, e
+    # free adaptive run
+    g2 = QdMediator(cfg, solve=solve, rtol=1e-13, jac_mode=0)
+    g2.set_state(cfg.psi0, cfg.slip0, np.zeros(N))
+    tEnd2, nst2, dts2 = g2.integrate(steps, integrator=integrator, tol=1e-8, init_dt=1e-3,
+                                     max_dt=1e10)
+    assert nst2 == steps
+    print(integrator, solve, "free dt err", np.abs(dts2 / odts - 1).max(), flush=True)
+    assert np.allclose(dts2, odts, rtol=1e-6) and abs(tEnd2 / tE.value - 1) <= 1e-6
+    e2 = _fields_err(g2.get_state(), g2.get_u(), ref)
+    print(integrator, solve, "free", e2, flush=True)
+    assert max(e2.values()) <= 1e-7, e2

@@ -44,7 +44,7 @@ def _pair(c):
 
 
 CASES = [(o, v, Ny, Nz, bc) for o in (2, 4) for v in (False, True)
-         for (Ny, Nz) in ((21, 25), (41, 37), (9, 12), (18, 130))
+         for (Ny, Nz) in ((21, 25), (41, 37), (9, 12), (18, 130), (17, 33))
          for bc in ((0, 1, 0, 1), (0, 0, 0, 0), (1, 1, 1, 1), (1, 0, 0, 1))]
 
 

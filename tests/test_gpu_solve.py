@@ -41,6 +41,34 @@ def test_solve_matches_direct(mode, order, var_grid, Ny, Nz):
     assert st2["iters"] == 0
 
 
+@pytest.mark.parametrize("precond", ["fdiag64", "fdiag32", "fdiag_bf16"])
+@pytest.mark.parametrize("layered", [True, False])
+def test_fdiag_preconditioned_solve(precond, layered):
+    """CG preconditioned with the fast-diagonalisation inverse of the separable
+    operator: same solution as the direct solve, in a few iterations when mu
+    depends on depth only (layered) and still converging for general mu."""
+    from scycle_amd import SbpOps
+    Ny, Nz = 81, 73
+    y, z = _grid(Ny, Nz, 30.0, 30.0, 5.0)
+    rng = np.random.default_rng(3)
+    if layered:
+        t = np.clip(z / 4.0, 0, 1)
+        mu = np.where(z >= 4.0, 36.0, 30.0 + 6.0 * (3 * t ** 2 - 2 * t ** 3))
+    else:
+        mu = 30 + 6 * rng.random(Ny * Nz)
+    bc = (0, 1, 0, 1)
+    o = OracleSbp(4, Ny, Nz, 30.0, 30.0, mu, y, z, var_grid=True, bc=bc)
+    g = SbpOps(4, Ny, Nz, 30.0, 30.0, mu, y, z, var_grid=True, bc=bc)
+    rhs = o.set_rhs(rng.standard_normal(Nz), np.full(Nz, 0.3), np.zeros(Ny), np.zeros(Ny))
+    uref = o.chol_solve(rhs)
+    u, st = g.solve(rhs, rtol=1e-12, precond=precond, max_iters=2000)
+    print(precond, layered, st, flush=True)
+    assert st["converged"] and st["relres"] <= 1e-12
+    assert np.abs(u - uref).max() <= 1e-9 * np.abs(uref).max()
+    if layered:
+        assert st["iters"] <= (12 if precond != "fdiag_bf16" else 30)
+
+
 def test_solve_full_size_residual():
     """1001^2 C2 operator: the FP64 residual of both solvers meets 1e-12 and
     the two solutions agree to the conditioning-limited 1e-8."""
@@ -57,3 +85,41 @@ def test_solve_full_size_residual():
     print("cg64", s1, "ir32", s2, flush=True)
     assert s1["converged"] and s2["converged"]
     assert np.abs(u1 - u2).max() <= 1e-7 * np.abs(u1).max()
+
+
+@pytest.mark.parametrize("mode,precond", [("cg64", "jacobi"), ("cg64", "fdiag64"), ("cg64", "fdiag32"),
+                                          ("ir32", "jacobi")])
+def test_reuse_projection_guess(mode, precond):
+    """Initial guess projected on the previous solutions of the same operator
+    (Fischer's method): every solution still matches the direct solve, a
+    right-hand side in the span of earlier ones needs no iteration, and a BC
+    type change resets the basis."""
+    from scycle_amd import SbpOps
+    Ny, Nz = 61, 57
+    y, z = _grid(Ny, Nz, 30.0, 30.0, 5.0)
+    rng = np.random.default_rng(11)
+    mu = 30 + 6 * rng.random(Ny * Nz)
+    bc = (0, 1, 0, 1)
+    o = OracleSbp(4, Ny, Nz, 30.0, 30.0, mu, y, z, var_grid=True, bc=bc)
+    g = SbpOps(4, Ny, Nz, 30.0, 30.0, mu, y, z, var_grid=True, bc=bc)
+    rhss = [o.set_rhs(rng.standard_normal(Nz), rng.standard_normal(Nz), np.zeros(Ny), np.zeros(Ny))
+            for _ in range(3)]
+    rhss.append(0.7 * rhss[0] - 1.3 * rhss[1] + 0.2 * rhss[2])
+    its = []
+    # the basis solutions to 1e-13, so that the combination of them meets 1e-10
+    # before any iteration
+    for b, rtol in zip(rhss, (1e-13, 1e-13, 1e-13, 1e-10)):
+        uref = o.chol_solve(b)
+        u, st = g.solve(b, mode=mode, rtol=rtol, precond=precond, reuse=3, max_iters=20000)
+        assert st["converged"] and st["relres"] <= rtol
+        assert np.abs(u - uref).max() <= 1e-8 * np.abs(uref).max()
+        its.append(st["iters"])
+    print(mode, precond, "iterations", its, flush=True)
+    assert its[3] == 0
+    # the basis belongs to the operator: after a BC type change it starts afresh
+    g.changeBCTypes(0, 1, 1, 1)
+    o2 = OracleSbp(4, Ny, Nz, 30.0, 30.0, mu, y, z, var_grid=True, bc=(0, 1, 1, 1))
+    b = o2.set_rhs(rng.standard_normal(Nz), rng.standard_normal(Nz), np.zeros(Ny), np.zeros(Ny))
+    u, st = g.solve(b, mode=mode, rtol=1e-11, precond=precond, reuse=3, max_iters=20000)
+    uref = o2.chol_solve(b)
+    assert np.abs(u - uref).max() <= 1e-8 * np.abs(uref).max()
