@@ -312,6 +312,47 @@ int orc_fdfault_step(void* h, double dt, const double* d2u, double* V_inout, con
   return e;
 }
 
+// Integrates y' = lam*y (two components: lam and lam2) with the RK32 / RK43
+// solver over n fixed steps of size dt (replay mode: no step-size control) or,
+// with n < 0, adaptively to finalT with tolerance tol.  Test hook for the
+// stability polynomials and the controller.  Returns the step count.
+namespace {
+struct LinOde : OdeCtx {
+  double lam, lam2;
+  long calls = 0;
+  int d_dt(double, const VarMap& var, VarMap& dvar) override {
+    ++calls;
+    const auto& y = var.at("y");
+    auto& d = dvar["y"];
+    d.assign(y.size(), 0.0);
+    d[0] = lam * y[0];
+    if (y.size() > 1) d[1] = lam2 * y[1];
+    return 0;
+  }
+};
+}  // namespace
+
+int orc_rk_linear(int ord, double lam, double lam2, double dt, int n, double finalT, double tol,
+                  double y0, double* y_out, double* t_out, long* calls) {
+  LinOde ode;
+  ode.lam = lam;
+  ode.lam2 = lam2;
+  RkSolver rk(ord, n >= 0 ? n : 1000000, n >= 0 ? dt * n * 2.0 : finalT, dt, "PID");
+  rk.totTol = tol;
+  rk.var["y"] = {y0, y0};
+  rk.errInds = {"y"};
+  if (n >= 0) {
+    rk.force_dt = 1;
+    rk.replayDt.assign((size_t)n, dt);
+  }
+  int e = rk.integrate(&ode);
+  y_out[0] = rk.var["y"][0];
+  y_out[1] = rk.var["y"][1];
+  *t_out = rk.currT;
+  *calls = ode.calls;
+  return e ? -1 : rk.stepCount;
+}
+
 void orc_powerlaw(int N, const double* sxy, const double* sxz, const double* T, const double* A,
                   const double* B, const double* n, const double* mu, const double* sat,
                   double cap, double* effVisc, double* gxy_t, double* gxz_t) {

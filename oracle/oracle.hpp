@@ -13,6 +13,25 @@
 //
 // Layout: index I = j*Nz + k (z fastest), fault is j = 0 (domain.cpp:379-383,
 // 449-457).  Units follow the reference (km, GPa, g/cm^3, MPa, m, m/s, s).
+//
+// How each part is pinned (the reference itself cannot be built here: it
+// needs PETSc, MPI and Boost):
+//   1-D SBP matrices    SBP identity, exactness per order, remainder structure
+//                       (tests/test_oracle_sbp.py)
+//   2-D operator        symmetry/definiteness, Neumann null space, closed-form
+//                       order-2 interior stencil, var-grid == const-grid on a
+//                       uniform grid, MMS convergence rates, the mu average
+//                       with global index I+1 (tests/test_oracle_sbp.py)
+//   friction, roots     strength limits, steady states of both state laws,
+//                       closed-form and bisection roots (test_oracle_physics)
+//   RK32 / RK43         stability polynomial / fifth-order local error,
+//                       adaptive runs meeting their tolerance (same file)
+//   wave step, FD fault constant state at rest, leapfrog energy, a_y sides,
+//                       fault equations solved independently (same file)
+//   power law           s^n and Arrhenius scaling, cap, SAT term (same file)
+//   PETSc Vec / .in I/O the reference's own test fixture (tests/golden/)
+// Parity unpinned: MUMPS's and PETSc's floating-point summation order (the
+// assembled CSR matches the reference's algebra, not its bits).
 #pragma once
 #include <cstdint>
 #include <map>

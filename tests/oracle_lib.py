@@ -66,6 +66,9 @@ def lib():
         L.orc_fdfault_free.argtypes = [C.c_void_p]
         L.orc_fdfault_step.argtypes = [C.c_void_p, C.c_double] + [_dp] * 9
         L.orc_powerlaw.argtypes = [C.c_int] + [_dp] * 8 + [C.c_double] + [_dp] * 3
+        L.orc_rk_linear.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                    C.c_double, C.c_double, C.c_double, _dp, _dp,
+                                    C.POINTER(C.c_long)]
         _lib = L
     return _lib
 
@@ -239,3 +242,13 @@ class FdFaultOracle:
         bad = lib().orc_fdfault_step(self.h, dt, ptr(arrs[0]), ptr(V), ptr(arrs[1]), ptr(arrs[2]),
                                      ptr(arrs[3]), ptr(arrs[4]), ptr(psi_n), ptr(slip_n), ptr(uf))
         return psi_n, slip_n, uf, V, bad
+
+
+def rk_linear(order, lam, dt, n, lam2=None, final_t=0.0, tol=1e-8, y0=1.0):
+    """y' = lam*y integrated by the oracle's RK32 (order 3) / RK43 (order 4):
+    n fixed steps of dt, or adaptive to final_t when n < 0.
+    Returns (y[2], t, steps, d_dt calls)."""
+    y, t, calls = np.zeros(2), C.c_double(), C.c_long()
+    steps = lib().orc_rk_linear(order, lam, lam if lam2 is None else lam2, dt, n, final_t, tol,
+                                y0, ptr(y), C.byref(t), C.byref(calls))
+    return y, t.value, steps, calls.value

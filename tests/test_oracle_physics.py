@@ -1,0 +1,284 @@
+"""Pins the oracle's fault, integrator, wave-step and power-law functions
+against what the mathematics fixes (CPU only): closed forms, invariants,
+limits, independent brute-force solves and order conditions.  A plausible
+restatement slip (a wrong sign, a dropped factor, a transposed argument)
+fails at least one of these."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from oracle_lib import lib, ptr
+
+V0, F0 = 1e-6, 0.6
+
+
+# ------------------------------------------------------------- friction --
+def test_strength_limits():
+    """a sN asinh(V/(2v0) e^(psi/a)): linear in V for small V, and
+    sN (psi + a ln(V/v0)) (the standard rate-and-state form) for large V."""
+    a, sN, psi = 0.015, 50.0, 0.7
+    Vs = 1e-40
+    lin = a * sN * Vs / (2 * V0) * math.exp(psi / a)
+    assert lib().orc_strength(sN, psi, Vs, a, V0) == pytest.approx(lin, rel=1e-12)
+    Vl = 1.0
+    log_form = sN * (psi + a * math.log(Vl / V0))
+    assert lib().orc_strength(sN, psi, Vl, a, V0) == pytest.approx(log_form, rel=1e-12)
+    # odd in V
+    assert lib().orc_strength(sN, psi, -1e-3, a, V0) == -lib().orc_strength(sN, psi, 1e-3, a, V0)
+
+
+@pytest.mark.parametrize("V", [1e-12, 1e-9, 1e-3, 1.0])
+def test_aging_law_steady_state(V):
+    b, Dc = 0.02, 8e-3
+    psi_ss = F0 - b * math.log(V / V0)
+    rate = b * V0 / Dc * math.exp((F0 - psi_ss) / b)  # scale of each term
+    assert abs(lib().orc_aging(psi_ss, V, b, F0, V0, Dc)) <= 1e-12 * rate
+    # sign: state grows below steady state (healing), decays above
+    assert lib().orc_aging(psi_ss - 0.01, V, b, F0, V0, Dc) > 0
+    assert lib().orc_aging(psi_ss + 0.01, V, b, F0, V0, Dc) < 0
+    # b <= 1e-3 switches the law off; an overflowing exponential gives 0
+    assert lib().orc_aging(psi_ss, V, 1e-3, F0, V0, Dc) == 0.0
+    assert lib().orc_aging(-1e3, V, b, F0, V0, Dc) == 0.0
+
+
+@pytest.mark.parametrize("V", [1e-12, 1e-9, 1e-3, 1.0])
+def test_slip_law_steady_state(V):
+    """slip law vanishes where f = a asinh(V/(2v0) e^(psi/a)) equals
+    f_ss = f0 + (a - b) ln(V/v0)."""
+    a, b, Dc = 0.015, 0.02, 8e-3
+    fss = F0 + (a - b) * math.log(V / V0)
+    psi = a * math.log(2 * V0 / V * math.sinh(fss / a))
+    scale = V / Dc * fss
+    assert abs(lib().orc_sliplaw(psi, V, a, b, F0, V0, Dc)) <= 1e-12 * scale
+    assert lib().orc_sliplaw(psi + 0.01, V, a, b, F0, V0, Dc) < 0
+    assert lib().orc_sliplaw(psi, 0.0, a, b, F0, V0, Dc) == 0.0
+
+
+def _bisect(f, lo, hi, n=200):
+    flo = f(lo)
+    for _ in range(n):
+        mid = 0.5 * (lo + hi)
+        fm = f(mid)
+        if (fm < 0) == (flo < 0):
+            lo, flo = mid, fm
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def test_vel_qd_closed_form_without_damping():
+    """eta -> 0: V = 2 v0 e^(-psi/a) sinh(tau/(a sN))."""
+    n = 64
+    rng = np.random.default_rng(0)
+    a = 0.01 + 0.015 * rng.random(n)
+    sN = np.full(n, 50.0)
+    psi = 0.6 + 0.2 * rng.random(n)
+    tau = 0.5 * a * sN * rng.random(n) * 40
+    exact = 2 * V0 * np.exp(-psi / a) * np.sinh(tau / (a * sN))
+    eta = np.full(n, 1e-30)
+    V, bad, _ = ol.vel_qd(eta, tau, sN, psi, a, V0, tol=1e-14, jac_mode=1)
+    assert bad == 0
+    assert np.all(np.abs(V - exact) <= 1e-9 * exact + 1e-30)
+
+
+@pytest.mark.parametrize("jac_mode", [0, 1])
+def test_vel_qd_matches_bisection(jac_mode):
+    """eta > 0: the BracketedNewton root equals a 200-step bisection of
+    F(V) = a sN asinh(V e^(psi/a)/(2v0)) + eta V - tau on [0, tau/eta], for
+    both slope expressions (the reference's A*V and the exact A*B)."""
+    n = 40
+    rng = np.random.default_rng(1)
+    a = 0.01 + 0.015 * rng.random(n)
+    sN = np.full(n, 50.0)
+    eta = np.full(n, 0.5 * math.sqrt(36 * 3.0))
+    psi = 0.6 + 0.2 * rng.random(n)
+    Vt = 10.0 ** rng.uniform(-15, 0.5, n)
+    tau = a * sN * np.arcsinh(Vt / (2 * V0) * np.exp(psi / a)) + eta * Vt
+    V, bad, _ = ol.vel_qd(eta, tau, sN, psi, a, V0, tol=1e-12, jac_mode=jac_mode)
+    assert bad == 0
+    for k in range(n):
+        f = lambda v: a[k] * sN[k] * math.asinh(v * math.exp(psi[k] / a[k]) / (2 * V0)) + eta[k] * v - tau[k]
+        vb = _bisect(f, 0.0, tau[k] / eta[k])
+        slope = a[k] * sN[k] * math.exp(psi[k] / a[k]) / (2 * V0) / math.sqrt(
+            1 + (vb * math.exp(psi[k] / a[k]) / (2 * V0)) ** 2) + eta[k]
+        # |F| < 1e-12 on both sides: the roots differ by at most 2e-12/slope
+        assert abs(V[k] - vb) <= 2e-12 / slope + 1e-15 * vb
+
+
+def test_vel_qd_locked_nodes():
+    n = 3
+    V, bad, _ = ol.vel_qd(np.full(n, 1.0), np.full(n, 30.0), np.full(n, 50.0), np.full(n, 0.7),
+                          np.full(n, 0.015), V0, vL=2e-9, locked=np.array([1.0, -1.0, 0.0]))
+    assert bad == 0 and V[0] == 0.0 and V[1] == 2e-9 and V[2] > 0
+
+
+# ------------------------------------------------------------ integrators --
+def test_rk32_stability_polynomial():
+    """A 3-stage 3rd-order explicit RK applied to y' = lam y multiplies y by
+    1 + z + z^2/2 + z^3/6 exactly (z = lam dt)."""
+    for lam, dt in ((-1.0, 0.1), (-3.0, 0.2), (0.5, 0.3)):
+        y, t, steps, calls = ol.rk_linear(3, lam, dt, 1)
+        z = lam * dt
+        assert y[0] == pytest.approx(1 + z + z * z / 2 + z ** 3 / 6, rel=1e-14)
+        assert steps == 1 and t == pytest.approx(dt)
+        assert calls == 4  # d_dt at the start, two stages, and at the accepted state
+
+
+def test_rk43_fourth_order():
+    """RK43 (Kennedy-Carpenter ARK4(3)6L explicit part): one-step error
+    against e^z scales as dt^5 and matches the Taylor series to z^4."""
+    errs = []
+    for dt in (0.2, 0.1, 0.05):
+        y, _, _, calls = ol.rk_linear(4, -1.0, dt, 1)
+        errs.append(abs(y[0] - math.exp(-dt)))
+    assert calls == 7  # d_dt at the start, five more stages, and at the accepted state
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert 28 < r1 < 36 and 28 < r2 < 36
+    z = -0.05
+    taylor4 = 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24
+    y, *_ = ol.rk_linear(4, -1.0, 0.05, 1)
+    # R(z) - (1 + z + ... + z^4/24) = O(z^5) with a coefficient near 1/120
+    assert abs(y[0] - taylor4) < 0.02 * abs(z) ** 5
+
+
+@pytest.mark.parametrize("order", [3, 4])
+def test_adaptive_run_meets_tolerance(order):
+    """Adaptive run to t = 5 on y' = -y (and a stiffer component): the error
+    against e^-t stays of the order of the tolerance times the step count and
+    shrinks with the tolerance."""
+    out = []
+    for tol in (1e-6, 1e-9):
+        y, t, steps, _ = ol.rk_linear(order, -1.0, 1e-3, -1, lam2=-4.0, final_t=5.0, tol=tol)
+        assert t == pytest.approx(5.0)
+        out.append((abs(y[0] - math.exp(-5.0)), steps))
+    (e1, s1), (e2, s2) = out
+    assert e2 < e1 and s2 > s1
+    assert e1 < 1e-6 * s1 and e2 < 1e-9 * s2
+
+
+# ----------------------------------------------------------- wave step ---
+def _neumann_ops(order, n):
+    rng = np.random.default_rng(order + n)
+    mu = 30 + 6 * rng.random(n * n)
+    y, z = np.repeat(np.arange(n) / (n - 1.0), n) * 20, np.tile(np.arange(n) / (n - 1.0), n) ** 1.3 * 20
+    return ol.OracleSbp(order, n, n, 20.0, 20.0, mu, y, z, var_grid=True, bc=(1, 1, 1, 1)), mu
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_fd_propagate_constant_state_and_energy(order):
+    """All-Neumann body without absorption: a constant displacement at rest
+    stays put (A 1 = 0), and the leapfrog energy
+    (du)^T diag(rho J H) (du)/dt^2 - u_{n+1}^T A u_n is conserved."""
+    n = 21
+    o, mu = _neumann_ops(order, n)
+    N = n * n
+    rho = np.full(N, 3.0)
+    ay = np.zeros(N)
+    c = np.full(N, 0.37)
+    un, _ = ol.fd_propagate(o, 1e-3, c, c, rho, ay)
+    assert np.abs(un - c).max() < 1e-12
+    yq, zr = o.metric()
+    W = rho * o.H(np.ones(N)) * yq * zr
+    A = o.csr()
+    u0 = np.exp(-((np.arange(N) % n - 10) ** 2 + (np.arange(N) // n - 8) ** 2) / 6.0)
+    dt = 2e-4
+    prev, cur = u0, u0.copy()
+    es = []
+    for _ in range(30):
+        nxt, _ = ol.fd_propagate(o, dt, cur, prev, rho, ay)
+        es.append(((nxt - cur) ** 2 * W).sum() / dt ** 2 - nxt @ (A @ cur))
+        prev, cur = cur, nxt
+    es = np.array(es)
+    assert es[0] > 0 and np.abs(es - es[0]).max() <= 1e-10 * es[0]
+
+
+def test_fd_ay_sides():
+    n = 9
+    o, _ = _neumann_ops(4, n)
+    cs = np.full(n * n, 2.0)
+    s = o.scalars()
+    ay = ol.fd_ay(o, cs, 1, 0, 1, 1).reshape(n, n)  # [j, k]
+    assert ay[n // 2, n // 2] == 0.0 and ay[0, 0] == 0.0 + 2 * 0.5 / s["h11y"]
+    assert ay[n - 1, n - 1] == pytest.approx(2 * (0.5 / s["h11y"] + 0.5 / s["h11z"]))
+    assert ay[3, n - 1] == pytest.approx(2 * 0.5 / s["h11z"]) and ay[3, 0] == 0.0
+
+
+@pytest.mark.parametrize("state_law", [0, 1])
+def test_fd_fault_step_solves_its_equations(state_law):
+    """Fault_fd step: V solves fricPen*strength(V) = |Phi| - V (independent
+    bisection), the state solves the implicit midpoint rule, and the
+    displacement update is the damped leapfrog of the fault node."""
+    N, dt = 24, 2e-3
+    rng = np.random.default_rng(2 + state_law)
+    a = np.where(np.arange(N) < 12, 0.015, 0.025)
+    b, Dc, sN = np.full(N, 0.02), np.full(N, 8e-3), np.full(N, 50.0)
+    rho, alphay = np.full(N, 3.0), np.full(N, 0.0083)
+    tau0 = 30 + 10 * rng.random(N)
+    slip0 = np.zeros(N)
+    f = ol.FdFaultOracle(N, V0, F0, 1e-9, 1e-12, 2.0, state_law, a, b, Dc, sN, rho, alphay, tau0, slip0)
+    psi = 0.6 + 0.15 * rng.random(N)
+    psi_prev = psi + 1e-4 * rng.standard_normal(N)
+    slip = 1e-3 * rng.random(N)
+    slip_prev = slip - 1e-4 * rng.random(N)
+    d2u = 0.1 * rng.standard_normal(N)
+    V = np.zeros(N)
+    psi_n, slip_n, uf, V, bad = f.step(dt, d2u, V, psi, psi_prev, slip, slip_prev)
+    assert bad == 0
+    for j in range(N):
+        u, up = slip[j] / 2, slip_prev[j] / 2
+        an = d2u[j] + tau0[j] / alphay[j]
+        Phi = 2.0 / dt * (u - up) + dt * an / rho[j]
+        fp = dt / alphay[j] / rho[j]
+        g = lambda v: fp * a[j] * sN[j] * math.asinh(v / (2 * V0) * math.exp(psi[j] / a[j])) - (abs(Phi) - v)
+        vb = _bisect(g, 0.0, abs(Phi))
+        fric = a[j] * sN[j] * math.asinh(vb / (2 * V0) * math.exp(psi[j] / a[j]))
+        alpha = fric / vb / (rho[j] * alphay[j])
+        assert V[j] == pytest.approx(Phi / (1 + dt * alpha), rel=1e-9)
+        u_new = (2 * u + an * dt * dt / rho[j] + (dt * alpha - 1) * up) / (1 + alpha * dt)
+        assert uf[j] == pytest.approx(u_new, rel=1e-9, abs=1e-15)
+        assert slip_n[j] == pytest.approx(2 * uf[j], rel=1e-14)
+        mid = 0.5 * (psi_prev[j] + psi_n[j])
+        if state_law == 0:
+            G = b[j] * V0 / Dc[j] * (math.exp((F0 - mid) / b[j]) - V[j] / V0)
+        else:
+            fss = F0 + (a[j] - b[j]) * math.log(abs(V[j]) / V0)
+            G = -abs(V[j]) / Dc[j] * (a[j] * math.asinh(abs(V[j]) / (2 * V0) * math.exp(mid / a[j])) - fss)
+        assert abs(psi_n[j] - psi_prev[j] - 2 * dt * G) < 1e-11
+
+
+# ------------------------------------------------------------- power law --
+def _powerlaw(sxy, sxz, T, A, B, n, mu, sat=None, cap=1e30):
+    N = len(sxy)
+    ev, gxy, gxz = np.zeros(N), np.zeros(N), np.zeros(N)
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (sxy, sxz, T, A, B, n, mu)]
+    lib().orc_powerlaw(N, *[ptr(x) for x in arrs], ptr(None if sat is None else np.asarray(sat, float)),
+                       cap, ptr(ev), ptr(gxy), ptr(gxz))
+    return ev, gxy, gxz
+
+
+def test_powerlaw_scaling_laws():
+    """Dislocation creep 1/eta = 1e3 A s^(n-1) e^(-B/T): strain rate scales as
+    s^n with the stress magnitude, as e^(-B/T) with temperature, shares the
+    direction of the stress, and the cap bounds the viscosity."""
+    N = 5
+    rng = np.random.default_rng(3)
+    sxy, sxz = 1 + rng.random(N), 0.5 * rng.random(N)
+    T = 600 + 400 * rng.random(N)
+    A, B, n = np.full(N, 1e-3), np.full(N, 15635.2), np.full(N, 3.5)
+    mu = np.full(N, 30.0)
+    ev1, gxy1, gxz1 = _powerlaw(sxy, sxz, T, A, B, n, mu)
+    ev2, gxy2, gxz2 = _powerlaw(2 * sxy, 2 * sxz, T, A, B, n, mu)
+    assert np.allclose(gxy2 / gxy1, 2 ** 3.5, rtol=1e-12)
+    assert np.allclose(gxz1 / gxy1, sxz / sxy, rtol=1e-12)
+    ev3, gxy3, _ = _powerlaw(sxy, sxz, T + 50, A, B, n, mu)
+    assert np.allclose(gxy3 / gxy1, np.exp(-B / (T + 50) + B / T), rtol=1e-10)
+    s = np.sqrt(sxy ** 2 + sxz ** 2)
+    assert np.allclose(1 / ev1, 1e-30 + 1e3 * A * s ** 2.5 * np.exp(-B / T), rtol=1e-12)
+    ev4, _, _ = _powerlaw(sxy, sxz, T, np.full(N, 1e-300), B, n, mu, cap=1e22)
+    assert np.allclose(ev4, 1e22, rtol=1e-12)
+    # the SAT term adds mu*sat to the stress driving gxy
+    sat = rng.random(N)
+    _, gxy5, _ = _powerlaw(sxy, sxz, T, A, B, n, mu, sat=sat)
+    assert np.allclose(gxy5 * ev1, sxy + mu * sat, rtol=1e-12)
