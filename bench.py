@@ -5,17 +5,30 @@ Ny = Nz = 1001, Ly = Lz = 30 km, 4th-order SBP-SAT, sinh-stretched z grid
 (stretch 5), layered shear modulus 30 -> 36 GPa over the top 4 km, aging-law
 rate-and-state friction (a ramps 0.015 -> 0.025 between 12 and 17 km),
 adaptive RK32 with PID control (timeStepTol 1e-8), synthetic inputs with a
-seeded +-1% state perturbation.  A "step" is one accepted RK32 step (three
+seeded +-1% state perturbation.  A "step" is one accepted RK32 step: three
 d_dt evaluations, each a full elasticity solve to relative residual 1e-12,
-plus rejected attempts if any).
+plus rejected attempts if any.
 
-Two GPU paths run the same steps from the same state: the mixed-precision
-solve (FP32 Jacobi-PCG inside FP64 iterative refinement) -- `value` -- and the
-plain FP64 Jacobi-PCG.  The line also carries the slip relative error of the
-mixed path against the FP64 path, the HBM roofline fraction of the dominant
-kernel (the FP32 interior SBP apply, timed with HIP events on its stream) and
-a CPU baseline (the FP64 oracle's Jacobi-PCG on the assembled operator, timed
-on a bounded sample and scaled by the iterations per step of the FP64 run).
+Solver paths, all from the same state over the same step window:
+  fp64_fdiag  FP64 CG preconditioned with the FP64 fast-diagonalisation
+              inverse of the separable operator (four DGEMMs)
+  mixed       the same CG with the preconditioner's GEMMs in FP32
+  mixed_bf16  ... in bf16 with FP32 accumulation
+  fp64        plain FP64 Jacobi-PCG (the straightforward GPU port)
+  ir32        FP32 Jacobi-PCG inside FP64 iterative refinement
+The FD-preconditioned paths start each solve from the projection of the new
+solution onto the last --reuse solutions (0 = previous solution only).
+`value` is the fastest path whose fault slip stays within 1e-8 (relative) of
+the fp64_fdiag path over the window; the line names it (config.solve) and
+lists every path.
+
+Also on the line: the HBM roofline of the FP64 SBP-SAT apply (interior and
+frame kernels, 24 algorithmic bytes per node: read u, read mu, write A u),
+timed with HIP events on the context stream; `traffic` from the committed
+rocprofv3 FETCH_SIZE / WRITE_SIZE summary when present; the fully dynamic
+2001^2 rupture (C4) in FP64 and FP32-bulk steps/s; and a CPU baseline (the
+FP64 oracle's Jacobi-PCG on the assembled operator, timed on a bounded sample
+and scaled by the FP64 Jacobi path's iterations per step).
 """
 import argparse
 import ctypes as C
@@ -30,6 +43,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # B/s, MI355X
+SLIP_TOL = 1e-8
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "apply_fp64_1001x1001_traffic.json")
 
 
 def log(*a):
@@ -37,18 +52,18 @@ def log(*a):
 
 
 PATHS = {
-    # label: (solve mode, preconditioner)
-    "mixed": ("cg64", "fdiag32"),      # FP64 CG, FP32 fast-diagonalisation preconditioner
-    "mixed_bf16": ("cg64", "fdiag_bf16"),
-    "fp64_fdiag": ("cg64", "fdiag64"),
-    "fp64": ("cg64", "jacobi"),        # plain FP64 Jacobi-PCG
-    "ir32": ("ir32", "jacobi"),        # FP32 Jacobi-PCG inside FP64 refinement
+    # label: (solve mode, preconditioner, dtype string)
+    "fp64_fdiag": ("cg64", "fdiag64", "f64"),
+    "mixed": ("cg64", "fdiag32", "f64 CG + f32 GEMM preconditioner"),
+    "mixed_bf16": ("cg64", "fdiag_bf16", "f64 CG + bf16 GEMM preconditioner"),
+    "fp64": ("cg64", "jacobi", "f64"),
+    "ir32": ("ir32", "jacobi", "f32 CG + f64 refinement"),
 }
 
 
 def run_path(cfg, label, warmup, steps, reuse=0):
     from scycle_amd import QdMediator
-    mode, prec = PATHS[label]
+    mode, prec, _ = PATHS[label]
     tic = time.perf_counter()
     q = QdMediator(cfg, solve=mode, rtol=1e-12, inner_rtol=1e-5, jac_mode=1, precond=prec,
                    reuse=reuse)
@@ -68,23 +83,33 @@ def run_path(cfg, label, warmup, steps, reuse=0):
     st = q.get_state()
     d = {k: c1[k] - c0[k] for k in ("d_dt_calls", "solves", "cg_iters", "outer_sweeps")}
     return q, dict(seconds=sec, steps=n, t_start=t0, t_end=t1, counters=d, state=st,
-                   setup_s=setup, dt_min=float(dts.min()), dt_max=float(dts.max()))
+                   setup_s=setup, dt_min=float(dts.min()), dt_max=float(dts.max()), reuse=reuse)
 
 
-def roofline(q):
-    """FP32 interior apply kernel: 12 algorithmic bytes per node (read u, read
-    mu, write A*u), timed with HIP events on the context stream."""
+def roofline(ctx):
+    """FP64 apply (frame + interior kernels): 24 algorithmic bytes per node,
+    mean of 200 launches timed with HIP events on the context stream."""
     from scycle_amd._lib import check, load
-    ctx = load().sc_qd_ctx(q.h)
-    ms, nodes = C.c_double(), C.c_double()
-    check(load().sc_bench_apply(ctx, 3, 200, C.byref(ms), C.byref(nodes)))
-    b = 12.0 * nodes.value
-    ach = b / (ms.value * 1e-3)
-    out = dict(kernel="k_apply_interior<float,4>", ms=ms.value, bytes=b, achieved=ach)
-    check(load().sc_bench_apply(ctx, 2, 200, C.byref(ms), C.byref(nodes)))
-    out["fp64_ms"] = ms.value
-    out["fp64_GBps"] = 24.0 * nodes.value / (ms.value * 1e-3) / 1e9
+    out = {}
+    for which, key, bpn in ((0, "apply64", 24.0), (1, "apply32", 12.0), (2, "interior64", 24.0),
+                            (6, "frame64", 24.0)):
+        ms, nodes = C.c_double(), C.c_double()
+        check(load().sc_bench_apply(ctx, which, 200, C.byref(ms), C.byref(nodes)))
+        out[key] = dict(ms=ms.value, nodes=nodes.value, bytes=bpn * nodes.value,
+                        GBps=bpn * nodes.value / (ms.value * 1e-3) / 1e9)
     return out
+
+
+def traffic_per_launch():
+    """HBM bytes per FP64 apply from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE of the two kernels, scaled by the copy-kernel
+    calibration recorded with it), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            d = json.load(f)
+        return float(d["hbm_bytes_per_apply"])
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def cpu_baseline(cfg, iters_per_step, budget_s=20.0):
@@ -115,7 +140,59 @@ def cpu_baseline(cfg, iters_per_step, budget_s=20.0):
                 sample=(f"{done2} Jacobi-PCG iterations of the FP64 oracle on the assembled "
                         f"{cfg.Ny}x{cfg.Nz} order-{cfg.order} operator ({t_iter*1e3:.1f} ms/iter, "
                         f"assembly {t_build:.1f} s untimed), scaled by "
-                        f"{iters_per_step:.0f} CG iterations per RK32 step of the FP64 GPU run"))
+                        f"{iters_per_step:.0f} CG iterations per RK32 step of the FP64 Jacobi "
+                        f"GPU path (same algorithm)"))
+
+
+def fd_c4(n, steps):
+    """Fully dynamic 2001^2 rupture: constant-dt wave + fault steps."""
+    from scycle_amd import FdMediator
+    from scycle_amd.configs import config_c4
+    cfg = config_c4(n, n)
+    N, Nz = cfg.Ny * cfg.Nz, cfg.Nz
+    out = {}
+    for bulk in ("fp64", "fp32"):
+        m = FdMediator(cfg, bulk=bulk)
+        z0 = np.zeros(N)
+        psi = np.full(Nz, 1.0) * cfg.psi0
+        m.set_state(z0, z0, psi, psi, np.zeros(Nz), np.zeros(Nz))
+        m.steps(10, cfg.dt)  # warm-up
+        tic = time.perf_counter()
+        m.steps(steps, cfg.dt, t0=10 * cfg.dt)
+        sec = time.perf_counter() - tic
+        st = m.get_state()
+        out[bulk] = dict(steps_per_s=steps / sec, ms_per_step=1e3 * sec / steps,
+                         max_slip=float(st["slip"].max()), max_V=float(st["V"].max()))
+        m.close()
+    out["slip_rel_err_fp32_vs_fp64"] = abs(out["fp32"]["max_slip"] - out["fp64"]["max_slip"]) / \
+        max(abs(out["fp64"]["max_slip"]), 1e-300)
+    out["workload"] = (f"FD rupture {n}x{n} order-4 constant grid, cs 3.464 km/s, CFL 0.5, slip law, "
+                       f"nucleation patch; {steps} steps after 10 warm-up steps")
+    return out
+
+
+def kernels_only(n, reps):
+    """Launches the FP64 apply `reps` times and an 8-byte-per-element device
+    copy of the same vector length `reps` times (the calibration kernel for
+    the FETCH_SIZE / WRITE_SIZE counters: it moves exactly 2 x 8 N bytes),
+    for counter runs under rocprofv3."""
+    import torch
+    from scycle_amd import SbpOps
+    from scycle_amd._lib import check, load
+    from scycle_amd.configs import config_c2
+    cfg = config_c2(n, n)
+    g = SbpOps(cfg.order, cfg.Ny, cfg.Nz, cfg.Ly, cfg.Lz, cfg.mu, cfg.y, cfg.z,
+               var_grid=cfg.var_grid, bc=cfg.mat_bc)
+    ms, nodes = C.c_double(), C.c_double()
+    check(load().sc_bench_apply(g.h, 0, reps, C.byref(ms), C.byref(nodes)))
+    N = cfg.Ny * cfg.Nz
+    x = torch.ones(N, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"apply64_ms": ms.value, "nodes": nodes.value, "copy_bytes": 16 * N,
+                      "reps": reps}))
 
 
 def main():
@@ -125,68 +202,90 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1001)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--paths", default="mixed,fp64,fp64_fdiag,mixed_bf16")
-    ap.add_argument("--reuse", type=int, default=0,
+    ap.add_argument("--no-fd", action="store_true")
+    ap.add_argument("--fd-n", type=int, default=2001)
+    ap.add_argument("--fd-steps", type=int, default=2000)
+    ap.add_argument("--paths", default="fp64_fdiag,mixed,mixed_bf16,fp64")
+    ap.add_argument("--reuse", type=int, default=4,
                     help="previous solutions projected for the CG initial guess (FD paths)")
     ap.add_argument("--slow-steps", type=int, default=3,
                     help="timed steps for the Jacobi-preconditioned paths (same window start)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only launch the FP64 apply and a calibration copy (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus > 1 is not supported "
               "(the multi-GPU decomposition is a design note, not a measured path)",
               file=sys.stderr)
         sys.exit(2)
+    if args.kernels_only:
+        kernels_only(args.n, 50)
+        return
     from scycle_amd.configs import config_c2
     cfg = config_c2(args.n, args.n)
     res = {}
-    keep = None
     labels = args.paths.split(",")
+    ref = "fp64_fdiag" if "fp64_fdiag" in labels else labels[0]
+    ctx_q = None
     for label in labels:
-        k = args.steps if PATHS[label][1] != "jacobi" else min(args.steps, args.slow_steps)
-        log(f"[bench] path {label} {PATHS[label]}: warmup {args.warmup} steps, timed {k} steps, "
+        jac = PATHS[label][1] == "jacobi"
+        k = min(args.steps, args.slow_steps) if jac else args.steps
+        log(f"[bench] path {label} {PATHS[label][:2]}: warmup {args.warmup} steps, timed {k} steps, "
             f"{cfg.Ny}x{cfg.Nz}")
-        q, r = run_path(cfg, label, args.warmup, k,
-                        reuse=args.reuse if PATHS[label][1] != "jacobi" else 0)
+        q, r = run_path(cfg, label, args.warmup, k, reuse=0 if jac else args.reuse)
         log(f"[bench] {label}: {r['steps']} steps in {r['seconds']:.3f} s (setup {r['setup_s']:.1f} s), "
             f"counters {r['counters']}")
         res[label] = r
-        if label == labels[0]:
-            keep = q
+        if label == ref:
+            ctx_q = q
         else:
             q.close()
-    main_mode = labels[0]
-    r = res[main_mode]
-    value = r["steps"] / r["seconds"]
-    ms_step = 1e3 * r["seconds"] / r["steps"]
-    extra = {"paths": {}}
+    # accuracy of each path against the FP64 FD-preconditioned path over the same window
+    paths = {}
     for lab, rr in res.items():
-        extra["paths"][lab] = dict(
-            solve=PATHS[lab][0], precond=PATHS[lab][1], steps=rr["steps"],
-            steps_per_s=rr["steps"] / rr["seconds"],
-            cg_iters_per_solve=rr["counters"]["cg_iters"] / max(rr["counters"]["solves"], 1),
-            solves_per_step=rr["counters"]["solves"] / rr["steps"], setup_s=rr["setup_s"],
-            t_start=rr["t_start"], t_end=rr["t_end"], dt_min=rr["dt_min"], dt_max=rr["dt_max"])
-    if "fp64" in res:
-        extra["fp64_steps_per_s"] = extra["paths"]["fp64"]["steps_per_s"]
-        extra["mixed_over_fp64"] = value / extra["fp64_steps_per_s"]
-    if "fp64_fdiag" in res:
-        extra["mixed_over_fp64_same_preconditioner"] = value / extra["paths"]["fp64_fdiag"]["steps_per_s"]
-    ref = "fp64_fdiag" if "fp64_fdiag" in res else ("fp64" if "fp64" in res else None)
-    if ref and ref != main_mode:
-        # compare at the same step count: the reference path ran the same window
-        n = min(res[ref]["steps"], r["steps"])
-        if n == r["steps"] == res[ref]["steps"]:
-            a, b = r["state"]["slip"], res[ref]["state"]["slip"]
-            extra["slip_rel_err_vs_fp64"] = float(np.abs(a - b).max() / np.abs(b).max())
-            extra["V_rel_err_vs_fp64"] = float(np.abs(r["state"]["V"] - res[ref]["state"]["V"]).max()
-                                               / np.abs(res[ref]["state"]["V"]).max())
-            extra["slip_err_reference_path"] = ref
-    rl = roofline(keep)
+        p = dict(solve=PATHS[lab][0], precond=PATHS[lab][1], reuse=rr["reuse"], steps=rr["steps"],
+                 steps_per_s=rr["steps"] / rr["seconds"], ms_per_step=1e3 * rr["seconds"] / rr["steps"],
+                 cg_iters_per_solve=rr["counters"]["cg_iters"] / max(rr["counters"]["solves"], 1),
+                 solves_per_step=rr["counters"]["solves"] / rr["steps"], setup_s=rr["setup_s"],
+                 t_start=rr["t_start"], t_end=rr["t_end"], dt_min=rr["dt_min"], dt_max=rr["dt_max"])
+        if lab != ref and rr["steps"] == res[ref]["steps"]:
+            a, b = rr["state"]["slip"], res[ref]["state"]["slip"]
+            p["slip_rel_err_vs_fp64"] = float(np.abs(a - b).max() / np.abs(b).max())
+            p["V_rel_err_vs_fp64"] = float(np.abs(rr["state"]["V"] - res[ref]["state"]["V"]).max()
+                                           / np.abs(res[ref]["state"]["V"]).max())
+        elif lab == ref:
+            p["slip_rel_err_vs_fp64"] = 0.0
+        paths[lab] = p
+    ok = [lab for lab, p in paths.items()
+          if p.get("slip_rel_err_vs_fp64") is not None and p["slip_rel_err_vs_fp64"] <= SLIP_TOL
+          and p["steps"] == paths[ref]["steps"]]
+    best = max(ok, key=lambda lab: paths[lab]["steps_per_s"])
+    value = paths[best]["steps_per_s"]
+    extra = {"paths": paths, "slip_err_reference_path": ref}
+    if "fp64" in paths:
+        extra["fp64_jacobi_steps_per_s"] = paths["fp64"]["steps_per_s"]
+        if "mixed" in paths:
+            extra["mixed_over_fp64"] = paths["mixed"]["steps_per_s"] / paths["fp64"]["steps_per_s"]
+        extra["value_over_fp64_jacobi"] = value / paths["fp64"]["steps_per_s"]
+    if "mixed" in paths and "fp64_fdiag" in paths:
+        extra["mixed_over_fp64_same_preconditioner"] = (paths["mixed"]["steps_per_s"]
+                                                        / paths["fp64_fdiag"]["steps_per_s"])
+    from scycle_amd._lib import load
+    rl = roofline(load().sc_qd_ctx(ctx_q.h))
     log(f"[bench] roofline {rl}")
-    roof = dict(bound="hbm", achieved=rl["achieved"] / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
-                frac=rl["achieved"] / HBM_PEAK, traffic=None, kernel=rl["kernel"],
-                kernel_ms=rl["ms"], bytes_per_launch=rl["bytes"],
-                fp64_interior_GBps=rl["fp64_GBps"])
+    a64 = rl["apply64"]
+    traffic = traffic_per_launch()
+    roof = dict(bound="hbm", achieved=a64["GBps"], peak=HBM_PEAK / 1e9, unit="GB/s",
+                frac=a64["GBps"] * 1e9 / HBM_PEAK, traffic=traffic,
+                kernel="SBP-SAT apply FP64 (k_apply_frame_lds + k_apply_interior)",
+                ms_per_launch=a64["ms"], algorithmic_bytes_per_launch=a64["bytes"],
+                parts={k: v for k, v in rl.items() if k != "apply64"})
+    ctx_q.close()
+    fd = None
+    if not args.no_fd:
+        log(f"[bench] fully dynamic C4 {args.fd_n}^2, {args.fd_steps} steps")
+        fd = fd_c4(args.fd_n, args.fd_steps)
+        log(f"[bench] fd {fd}")
     cpu = None
     if not args.no_cpu and "fp64" in res:
         ips = res["fp64"]["counters"]["cg_iters"] / res["fp64"]["steps"]
@@ -195,16 +294,14 @@ def main():
     line = {
         "metric": "QD steps/s at 1001^2 mixed vs FP64 GPU (slip rel.err <=1e-8) + HBM GB/s",
         "value": value, "unit": "QD steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None,
-        "dtype": {"mixed": "f64 CG + f32 GEMM preconditioner", "mixed_bf16": "f64 CG + bf16 GEMM preconditioner",
-                  "fp64_fdiag": "f64", "fp64": "f64", "ir32": "f32 CG + f64 refinement"}[main_mode],
-        "data": "synthetic",
+        "warmup": args.warmup, "ms_per_step": paths[best]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": PATHS[best][2], "data": "synthetic",
         "config": {"workload": f"QD strike-slip {cfg.Ny}x{cfg.Nz} order-4 sinh-z(5) layered-mu "
                                f"aging-law RK32 tol 1e-8, steps {args.warmup + 1}-"
                                f"{args.warmup + args.steps} from t=0",
-                   "solve": main_mode, "rtol": 1e-12},
-        "roofline": roof, "cpu_baseline": cpu,
+                   "solve": best, "precond": PATHS[best][1], "reuse": paths[best]["reuse"],
+                   "rtol": 1e-12},
+        "roofline": roof, "cpu_baseline": cpu, "fd_c4": fd,
     }
     line.update(extra)
     print(json.dumps(line))
