@@ -201,6 +201,25 @@ typedef struct sc_fault_fd_args {
  * strength.  One thread per fault node, FP64. */
 int sc_fault_fd_step(sc_ctx* ctx, const sc_fault_fd_args* args);
 
+/* ------------------------------------------------------------ power law */
+typedef struct sc_powerlaw_args {
+  int N;
+  double effViscCap;         /* viscosity cap (PowerLaw _effViscCap, default 1e30) */
+  const void *sxy_d, *sxz_d; /* stresses (sxz may be NULL: 0)              */
+  const void *T_d, *A_d, *B_d, *n_d;  /* temperature (K), A, Q/R, n           */
+  const void *mu_d;
+  const void *sat_d;         /* SAT term of the strain rate (may be NULL: 0) */
+  void *effVisc_d;           /* out (may be NULL)                           */
+  void *gxy_t_d, *gxz_t_d;   /* out: viscous strain rates (gxz may be NULL) */
+} sc_powerlaw_args;
+
+/* DislocationCreep::computeInvEffVisc + PowerLaw::computeViscosity +
+ * computeViscStrainRates, pointwise (powerLaw.cpp:438-474, 1674-1743):
+ * 1/eta = 1/cap + 1e3 A s^(n-1) exp(-B/T), s = |(sxy, sxz)|,
+ * d/dt gxy = (mu*sat + sxy)/eta, d/dt gxz = sxz/eta.  prec SC_FP64: double
+ * arrays; SC_FP32: float arrays (the law itself is evaluated in FP64). */
+int sc_powerlaw_rates(sc_ctx* ctx, int prec, const sc_powerlaw_args* args);
+
 /* ------------------------------------------------------------ wave step */
 /* strikeSlip_linearElastic_fd::propagateWaves (strikeSlip_linearElastic_fd.cpp:
  * 445-516): D2u = Jinv*Hinv*(A u), uNext = (dt^2/rho*D2u + 2u +
@@ -314,6 +333,25 @@ int sc_qd_file_run(sc_qd* q, sc_qd_run* run_out);
  * slip from the solution, back to the configured BC types.  psi_h in/out,
  * slip_h out (Nz); also sets the driver state (psi, slip, V = vL). */
 int sc_qd_solve_ss(sc_qd* q, double* psi_h, int have_state_vals, double* slip_h);
+
+/* Power-law viscoelastic QD: StrikeSlip_PowerLaw_qd (strikeSlip_powerLaw_qd.cpp
+ * :633-686, 787-823) with PowerLaw in its dislocation-creep form.  Same
+ * arguments as sc_qd_create plus T, A, B (= Q/R), n (N each, z-fast) and the
+ * viscosity cap; the integrated variables are psi, slip (Nz) and the viscous
+ * strains gVxy, gVxz (N, zero initially).  errMask picks the fields of the
+ * RK error estimate (timeIntInds): 1 psi, 2 slip, 4 gVxy, 8 gVxz. */
+int sc_qd_create_powerlaw(const sc_qd_desc* desc, const double* mu_h, const double* y_h,
+                          const double* z_h, const double* a_h, const double* b_h,
+                          const double* Dc_h, const double* sN_h, const double* eta_h,
+                          const double* locked_h, const double* prestress_h,
+                          const double* bcRShift_h, const double* T_h, const double* A_h,
+                          const double* B_h, const double* n_h, double effViscCap, int errMask,
+                          sc_qd** out);
+/* viscous strains in/out (host arrays, N each; any may be NULL) */
+int sc_qd_set_visc(sc_qd* q, const double* gVxy_h, const double* gVxz_h);
+/* state of the last d_dt: strains, their rates, effective viscosity, sxy */
+int sc_qd_get_visc(sc_qd* q, double* gVxy_h, double* gVxz_h, double* dgVxy_h, double* dgVxz_h,
+                   double* effVisc_h, double* sxy_h);
 
 /* --------------------------------------------- host-only input utilities */
 /* Value of `key` in a .in deck, cut at its first space like every

@@ -170,10 +170,13 @@ struct OrcQdRun {
   const double* replayDt;
 };
 
-void* orc_qd_new(const OrcQdCfg* c, const double* mu, const double* y, const double* z,
-                 const double* a, const double* b, const double* Dc, const double* sN,
-                 const double* eta, const double* locked, const double* prestress,
-                 const double* bcRShift) {
+}  // extern "C"
+
+namespace {
+QdModel* setup_qd(QdModel* m, const OrcQdCfg* c, const double* mu, const double* y,
+                  const double* z, const double* a, const double* b, const double* Dc,
+                  const double* sN, const double* eta, const double* locked,
+                  const double* prestress, const double* bcRShift) {
   QdConfig q;
   q.sbp.order = c->order;
   q.sbp.Ny = c->Ny;
@@ -195,7 +198,6 @@ void* orc_qd_new(const OrcQdCfg* c, const double* mu, const double* y, const dou
   q.jac_mode = c->jac_mode;
   q.solver = c->solver;
   q.cg_rtol = c->cg_rtol;
-  auto* m = new QdModel;
   int Nz = c->Nz;
   m->a.assign(a, a + Nz);
   m->b.assign(b, b + Nz);
@@ -211,7 +213,101 @@ void* orc_qd_new(const OrcQdCfg* c, const double* mu, const double* y, const dou
   if (bcRShift) m->bcRShift.assign(bcRShift, bcRShift + Nz);
   return m;
 }
+}  // namespace
+
+extern "C" {
+
+void* orc_qd_new(const OrcQdCfg* c, const double* mu, const double* y, const double* z,
+                 const double* a, const double* b, const double* Dc, const double* sN,
+                 const double* eta, const double* locked, const double* prestress,
+                 const double* bcRShift) {
+  return setup_qd(new QdModel, c, mu, y, z, a, b, Dc, sN, eta, locked, prestress, bcRShift);
+}
 void orc_qd_free(void* h) { delete (QdModel*)h; }
+
+// ------------------------------------------------------ power-law QD -----
+void* orc_pl_new(const OrcQdCfg* c, const double* mu, const double* y, const double* z,
+                 const double* a, const double* b, const double* Dc, const double* sN,
+                 const double* eta, const double* locked, const double* prestress,
+                 const double* bcRShift, const double* T, const double* A, const double* B,
+                 const double* n, double cap) {
+  auto* m = (PowerLawModel*)setup_qd(new PowerLawModel, c, mu, y, z, a, b, Dc, sN, eta, locked,
+                                     prestress, bcRShift);
+  if (m && m->initPL(T, A, B, n, cap)) {
+    delete m;
+    return nullptr;
+  }
+  return m;
+}
+void orc_pl_free(void* h) { delete (PowerLawModel*)h; }
+void orc_pl_source(void* h, const double* gxy, const double* gxz, double* out) {
+  powerlaw_source(*(PowerLawModel*)h, gxy, gxz, out);
+}
+void orc_pl_sat(void* h, const double* u, const double* gL, const double* gR, double* out) {
+  auto* m = (PowerLawModel*)h;
+  powerlaw_sat(m->sbp, m->sbp.d.bcL == 0, m->sbp.d.bcR == 0, u, gL, gR, out);
+}
+// One d_dt; outputs (any may be null): rates, fault tau, u, sxy, effVisc.
+int orc_pl_ddt(void* h, double t, const double* psi, const double* slip, const double* gxy,
+               const double* gxz, double* V_inout, double* dpsi, double* dslip, double* dgxy,
+               double* dgxz, double* u, double* sxy, double* effVisc) {
+  auto* m = (PowerLawModel*)h;
+  const int Nz = m->Nz, N = m->N;
+  if (V_inout) m->V.assign(V_inout, V_inout + Nz);
+  VarMap var, dvar;
+  var["psi"].assign(psi, psi + Nz);
+  var["slip"].assign(slip, slip + Nz);
+  var["gVxy"].assign(gxy, gxy + N);
+  var["gVxz"].assign(gxz, gxz + N);
+  int e = m->d_dt(t, var, dvar);
+  if (e) return e;
+  if (V_inout) std::memcpy(V_inout, m->V.data(), sizeof(double) * Nz);
+  if (dpsi) std::memcpy(dpsi, dvar["psi"].data(), sizeof(double) * Nz);
+  if (dslip) std::memcpy(dslip, dvar["slip"].data(), sizeof(double) * Nz);
+  if (dgxy) std::memcpy(dgxy, dvar["gVxy"].data(), sizeof(double) * N);
+  if (dgxz) std::memcpy(dgxz, dvar["gVxz"].data(), sizeof(double) * N);
+  if (u) std::memcpy(u, m->u.data(), sizeof(double) * N);
+  if (sxy) std::memcpy(sxy, m->sxy.data(), sizeof(double) * N);
+  if (effVisc) std::memcpy(effVisc, m->effVisc.data(), sizeof(double) * N);
+  return 0;
+}
+// RK integration of (psi, slip, gVxy, gVxz) with the given error indices
+// (errMask bit 0 psi, 1 slip, 2 gVxy, 3 gVxz); state arrays in/out.
+int orc_pl_integrate(void* h, const OrcQdRun* r, int errMask, double* psi, double* slip,
+                     double* gxy, double* gxz, double* V, double* tEnd, double* dts, int dtsCap) {
+  auto* m = (PowerLawModel*)h;
+  const int Nz = m->Nz, N = m->N;
+  RkSolver rk(r->integrator == 0 ? 3 : 4, r->maxSteps, r->maxTime, r->initDt,
+              r->control == 0 ? "P" : "PID");
+  rk.initT = rk.currT = r->initTime;
+  rk.totTol = r->tol;
+  rk.minDeltaT = r->minDt;
+  rk.maxDeltaT = r->maxDt;
+  rk.normType = r->normType == 0 ? "L2_absolute" : "L2_relative";
+  const char* names[4] = {"psi", "slip", "gVxy", "gVxz"};
+  for (int i = 0; i < 4; ++i)
+    if (errMask & (1 << i)) rk.errInds.push_back(names[i]);
+  rk.var["psi"].assign(psi, psi + Nz);
+  rk.var["slip"].assign(slip, slip + Nz);
+  rk.var["gVxy"].assign(gxy, gxy + N);
+  rk.var["gVxz"].assign(gxz, gxz + N);
+  if (r->nReplay > 0) {
+    rk.force_dt = 1;
+    rk.replayDt.assign(r->replayDt, r->replayDt + r->nReplay);
+    if (rk.maxNumSteps > r->nReplay) rk.maxNumSteps = r->nReplay;
+  }
+  if (V) m->V.assign(V, V + Nz);
+  int e = rk.integrate(m);
+  if (e) return -e;
+  std::memcpy(psi, rk.var["psi"].data(), sizeof(double) * Nz);
+  std::memcpy(slip, rk.var["slip"].data(), sizeof(double) * Nz);
+  std::memcpy(gxy, rk.var["gVxy"].data(), sizeof(double) * N);
+  std::memcpy(gxz, rk.var["gVxz"].data(), sizeof(double) * N);
+  if (V) std::memcpy(V, m->V.data(), sizeof(double) * Nz);
+  if (tEnd) *tEnd = rk.currT;
+  for (int i = 0; i < (int)rk.dtHistory.size() && i < dtsCap; ++i) dts[i] = rk.dtHistory[i];
+  return rk.stepCount;
+}
 
 // One d_dt evaluation (the reference's StrikeSlip_LinearElastic_qd::d_dt).
 // V is the root-solve warm start on entry (persisting fault state).

@@ -455,6 +455,124 @@ int FdFault::step(double dt, const double* psiCur, const double* psiPrev, const 
   return bad;
 }
 
+// ---------------------------------------------------- power-law QD -----
+int PowerLawModel::initPL(const double* T_, const double* A_, const double* B_, const double* n_,
+                          double cap) {
+  T.assign(T_, T_ + N);
+  Ad.assign(A_, A_ + N);
+  Bd.assign(B_, B_ + N);
+  nd.assign(n_, n_ + N);
+  effViscCap = cap;
+  visc.assign(N, 0.0);
+  gTxy.assign(N, 0.0); gTxz.assign(N, 0.0); sxz.assign(N, 0.0);
+  effVisc.assign(N, 0.0); sat.assign(N, 0.0);
+  const Sbp2D& s = sbp;
+  // helpful factors yq*zr*H, yq*Hy, zr*Hz (powerLaw.cpp:1256-1268)
+  Csr yqxzrxH, yqxHy, zrxHz;
+  if (s.d.var_grid) {
+    yqxzrxH = matmul3(s.yq, s.zr, s.H);
+    yqxHy = matmul(s.yq, s.Hy_Iz);
+    zrxHz = matmul(s.zr, s.Iy_Hz);
+  } else {
+    yqxHy = matmul(s.H, s.Iy_Hzinv);
+    zrxHz = matmul(s.H, s.Hyinv_Iz);
+    yqxzrxH = s.H;
+  }
+  // B = yq*zr*H * Dy * mu (+ Neumann SATs on L/R), C likewise with Dz (T/B)
+  Bm = matmul3(yqxzrxH, s.Dy_Iz, s.muM);
+  if (s.d.bcL == 1) Bm = axpy(Bm, 1.0, matmul3(zrxHz, s.muM, s.E0y_Iz));
+  if (s.d.bcR == 1) Bm = axpy(Bm, -1.0, matmul3(zrxHz, s.muM, s.ENy_Iz));
+  Cm = matmul3(yqxzrxH, s.Iy_Dz, s.muM);
+  if (s.d.bcT == 1) Cm = axpy(Cm, 1.0, matmul3(yqxHy, s.muM, s.Iy_E0z));
+  if (s.d.bcB == 1) Cm = axpy(Cm, -1.0, matmul3(yqxHy, s.muM, s.Iy_ENz));
+  return 0;
+}
+
+void powerlaw_source(const PowerLawModel& m, const double* gxy, const double* gxz, double* out) {
+  spmv(m.Bm, gxy, out);
+  spmv_add(m.Cm, gxz, out);
+}
+
+void powerlaw_sat(const Sbp2D& s, int bcLDirichlet, int bcRDirichlet, const double* u,
+                  const double* gL, const double* gR, double* out) {
+  const int N = s.N();
+  std::vector<double> temp1(N), G(N);
+  std::fill(out, out + N, 0.0);
+  if (bcLDirichlet) {
+    // out += Hyinv E0y u - Hyinv e0y gL
+    std::vector<double> t(N);
+    spmv(s.E0y_Iz, u, t.data());
+    spmv(s.Hyinv_Iz, t.data(), temp1.data());
+    spmv(s.e0y_Iz, gL, t.data());
+    spmv(s.Hyinv_Iz, t.data(), G.data());
+    for (int i = 0; i < N; ++i) out[i] = out[i] + 1.0 * temp1[i];
+    for (int i = 0; i < N; ++i) out[i] = out[i] + (-1.0) * G[i];
+  }
+  if (bcRDirichlet) {
+    std::vector<double> t(N);
+    spmv(s.ENy_Iz, u, t.data());
+    spmv(s.Hyinv_Iz, t.data(), temp1.data());
+    spmv(s.eNy_Iz, gR, t.data());
+    spmv(s.Hyinv_Iz, t.data(), G.data());
+    for (int i = 0; i < N; ++i) out[i] = out[i] + (-1.0) * temp1[i];
+    for (int i = 0; i < N; ++i) out[i] = out[i] + 1.0 * G[i];
+  }
+  if (s.d.var_grid) {
+    std::vector<double> t(out, out + N);
+    spmv(s.qy, t.data(), out);
+  }
+}
+
+// strikeSlip_powerLaw_qd.cpp:633-686 and solveMomentumBalance :787-823
+int PowerLawModel::d_dt(double t, const VarMap& var, VarMap& dvar) {
+  const auto& vslip = var.at("slip");
+  for (int k = 0; k < Nz; ++k) bcL[k] = vslip[k] * (1.0 / c.faultTypeScale);
+  if (c.remoteLoadingR)
+    for (int k = 0; k < Nz; ++k) bcR[k] = c.vL * t / c.faultTypeScale + 1.0 * bcRShift[k];
+  psi = var.at("psi");
+  slip = vslip;
+  const auto& gVxy = var.at("gVxy");
+  const auto& gVxz = var.at("gVxz");
+  powerlaw_source(*this, gVxy.data(), gVxz.data(), visc.data());
+  sbp2d_set_rhs(sbp, rhs.data(), bcL.data(), bcR.data(), bcT.data(), bcB.data());
+  for (int i = 0; i < N; ++i) rhs[i] = rhs[i] + 1.0 * visc[i];
+  if (c.solver == 0) {
+    chol.solve(rhs.data(), u.data());
+  } else {
+    PcgStats st = pcg(sbp.A, rhs.data(), u.data(), c.cg_rtol, c.cg_maxit);
+    cgIters += st.iters;
+    if (st.relres > c.cg_rtol) return 20;
+  }
+  nSolves++;
+  // total strains, stresses, viscosity, strain rates (powerLaw.cpp:1707-1887)
+  sbp2d_Dy(sbp, u.data(), gTxy.data());
+  sbp2d_Dz(sbp, u.data(), gTxz.data());
+  for (int i = 0; i < N; ++i) sxy[i] = (gTxy[i] + (-1.0) * gVxy[i]) * sbp.mu[i];
+  for (int i = 0; i < N; ++i) sxz[i] = (gTxz[i] + (-1.0) * gVxz[i]) * sbp.mu[i];
+  powerlaw_sat(sbp, sbp.d.bcL == 0, sbp.d.bcR == 0, u.data(), bcL.data(), bcR.data(), sat.data());
+  auto& dgxy = dvar["gVxy"];
+  auto& dgxz = dvar["gVxz"];
+  dgxy.resize(N);
+  dgxz.resize(N);
+  powerlaw_rates(N, sxy.data(), sxz.data(), T.data(), Ad.data(), Bd.data(), nd.data(), sbp.mu.data(),
+                 sat.data(), effViscCap, effVisc.data(), dgxy.data(), dgxz.data());
+  // fault (Fault_qd::d_dt) on sxy at the fault
+  for (int k = 0; k < Nz; ++k) tauQS[k] = sxy[k];
+  for (int k = 0; k < Nz; ++k) tauQS[k] = tauQS[k] + 1.0 * prestress[k];
+  if (computeVel_qd(Nz, eta.data(), tauQS.data(), sN.data(), psi.data(), a.data(), locked.data(),
+                    c.v0, c.vL, c.rootTol, c.maxNumIts, c.jac_mode, V.data(), &rootIts))
+    return 30;
+  dvar["slip"] = V;
+  auto& dpsi = dvar["psi"];
+  dpsi.resize(Nz);
+  for (int k = 0; k < Nz; ++k)
+    dpsi[k] = c.stateLaw == 0 ? agingLaw_psi(psi[k], V[k], b[k], c.f0, c.v0, Dc[k])
+                              : slipLaw_psi(psi[k], V[k], a[k], b[k], c.f0, c.v0, Dc[k]);
+  for (int k = 0; k < Nz; ++k) tau[k] = -1.0 * (eta[k] * V[k]) + tauQS[k];
+  for (int k = 0; k < Nz; ++k) strength[k] = strength_psi(sN[k], psi[k], V[k], a[k], c.v0);
+  return 0;
+}
+
 // ---------------------------------------------------------- power law ----
 void powerlaw_rates(int N, const double* sxy, const double* sxz, const double* T,
                     const double* A, const double* B, const double* n, const double* mu,

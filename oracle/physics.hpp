@@ -108,6 +108,24 @@ struct FdFault {
            const double* slipPrev, double* psiNext, double* slipNext);
 };
 
+// ------------------------------------------------------ power-law QD -----
+// StrikeSlip_PowerLaw_qd::d_dt (strikeSlip_powerLaw_qd.cpp:633-686, 787-823)
+// with PowerLaw (powerLaw.cpp) in its dislocation-creep form: integrated
+// variables psi, slip (Nz) and the viscous strains gVxy, gVxz (N).
+struct PowerLawModel : QdModel {
+  Csr Bm, Cm;                          // PowerLaw::initializeMomBalMats (powerLaw.cpp:1239-1318)
+  std::vector<double> T, Ad, Bd, nd;   // temperature, A, Q/R, n (N each)
+  double effViscCap = 1e30;
+  std::vector<double> visc, gTxy, gTxz, sxz, effVisc, sat;
+  int initPL(const double* T, const double* A, const double* B, const double* n, double cap);
+  int d_dt(double t, const VarMap& var, VarMap& dvar) override;
+};
+// B gxy + C gxz  (PowerLaw::computeViscStrainSourceTerms, powerLaw.cpp:1651-1672)
+void powerlaw_source(const PowerLawModel& m, const double* gxy, const double* gxz, double* out);
+// PowerLaw::computeViscousStrainRateSAT (powerLaw.cpp:1745-1800)
+void powerlaw_sat(const Sbp2D& s, int bcLDirichlet, int bcRDirichlet, const double* u,
+                  const double* gL, const double* gR, double* out);
+
 // -------------------------------------------------------- power law -----
 // DislocationCreep::computeInvEffVisc + PowerLaw::computeViscosity +
 // computeViscStrainRates (powerLaw.cpp:438-474, 1674-1799).

@@ -8,8 +8,8 @@ mediator interfaces for Python callers and tests.
 from ._lib import LIB_PATH, ScError, load  # noqa: F401
 from .device import DeviceArray  # noqa: F401
 from .sbp_ops import SbpOps, DIRICHLET, NEUMANN  # noqa: F401
-from .qd import QdMediator  # noqa: F401
+from .qd import PowerLawMediator, QdMediator  # noqa: F401
 from .fd import FdMediator  # noqa: F401
 
 __all__ = ["LIB_PATH", "ScError", "load", "DeviceArray", "SbpOps", "DIRICHLET", "NEUMANN",
-           "QdMediator", "FdMediator"]
+           "QdMediator", "PowerLawMediator", "FdMediator"]

@@ -64,6 +64,12 @@ class FdDesc(C.Structure):
                 ("maxNumIts", C.c_int), ("stateLaw", C.c_int), ("bulkPrec", C.c_int)]
 
 
+class PowerLawArgs(C.Structure):
+    _fields_ = [("N", C.c_int), ("effViscCap", C.c_double), ("sxy_d", _vp), ("sxz_d", _vp),
+                ("T_d", _vp), ("A_d", _vp), ("B_d", _vp), ("n_d", _vp), ("mu_d", _vp),
+                ("sat_d", _vp), ("effVisc_d", _vp), ("gxy_t_d", _vp), ("gxz_t_d", _vp)]
+
+
 class QdDesc(C.Structure):
     _fields_ = [("grid", GridDesc), ("remoteLoadingR", C.c_int), ("faultTypeScale", C.c_double),
                 ("vL", C.c_double), ("f0", C.c_double), ("v0", C.c_double),
@@ -115,6 +121,11 @@ SIGNATURES = {
     "sc_fault_qd_rates": (C.c_int, [_vp, C.POINTER(FaultQdArgs)]),
     "sc_fault_fd_step": (C.c_int, [_vp, C.POINTER(FaultFdArgs)]),
     "sc_wave_step": (C.c_int, [_vp, C.c_int, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sc_powerlaw_rates": (C.c_int, [_vp, C.c_int, C.POINTER(PowerLawArgs)]),
+    "sc_qd_create_powerlaw": (C.c_int, [C.POINTER(QdDesc)] + [_dp] * 15 + [C.c_double, C.c_int,
+                                                                        C.POINTER(_vp)]),
+    "sc_qd_set_visc": (C.c_int, [_vp, _dp, _dp]),
+    "sc_qd_get_visc": (C.c_int, [_vp] + [_dp] * 6),
     "sc_fd_create": (C.c_int, [C.POINTER(FdDesc)] + [_dp] * 13 + [C.POINTER(_vp)]),
     "sc_fd_destroy": (C.c_int, [_vp]),
     "sc_fd_ctx": (_vp, [_vp]),

@@ -204,3 +204,31 @@ def fd_rupture(Ny, Nz, order=4, Ly=30.0, Lz=30.0, cfl=0.5, name="custom", steps=
 def config_c4(Ny=2001, Nz=2001):
     """2001^2 fully dynamic rupture, order 4, constant grid, CFL 0.5, 2000 steps."""
     return fd_rupture(Ny, Nz, name=f"C4-{Ny}x{Nz}", steps=2000)
+
+
+@dataclass
+class PowerLawConfig(QdConfig):
+    T: np.ndarray = None        # temperature (K), N
+    A: np.ndarray = None        # dislocation-creep prefactor, N
+    B: np.ndarray = None        # Q/R (K), N
+    n: np.ndarray = None        # stress exponent, N
+    visc_cap: float = 1e30      # PowerLaw _effViscCap default (powerLaw.cpp:757)
+
+
+def power_law(Ny, Nz, order=4, Ly=30.0, Lz=30.0, name="custom", steps=300):
+    """The C3 physics: the C1 friction on a constant grid with dislocation
+    creep, A = 1e-3, n = 3.5, B = Q/R = 130 kJ/mol / 8.314462618 J/(mol K)
+    (disl_AVals/nVals/BVals), geotherm T = 293 + 25 z K (TVals = [293 1043],
+    TDepths = [0 30]); viscous strains start at zero."""
+    base = strike_slip(Ny, Nz, order=order, var_grid=False, Ly=Ly, Lz=Lz, name=name, steps=steps)
+    z = base.z
+    N = Ny * Nz
+    T = set_vec(z, [293.0, 1043.0], [0.0, 30.0])
+    kw = {f: getattr(base, f) for f in QdConfig.__dataclass_fields__}
+    return PowerLawConfig(**kw, T=T, A=np.full(N, 1e-3), B=np.full(N, 130000.0 / 8.314462618),
+                          n=np.full(N, 3.5))
+
+
+def config_c3(Ny=801, Nz=801):
+    """801^2 power-law viscoelastic QD, order 4, constant grid, 300 steps."""
+    return power_law(Ny, Nz, name=f"C3-{Ny}x{Nz}", steps=300)

@@ -230,8 +230,63 @@ int launch_lincomb(hipStream_t s, int n, double* out, const double* base, int nt
   return SC_OK;
 }
 
-int launch_err(hipStream_t s, int n, const double* lo, const double* hi, double* out3) {
-  hipLaunchKernelGGL(k_err, dim3(1), dim3(256), 0, s, n, lo, hi, out3);
+// Long vectors (the viscous strains, N entries): per-block partial sums, then
+// one block combines them in block order (deterministic).
+constexpr int kErrBlocksMax = 512;
+
+__global__ __launch_bounds__(256) void k_err_part(int n, const double* __restrict__ lo,
+                                                  const double* __restrict__ hi,
+                                                  double* __restrict__ part) {
+  __shared__ double s0[256], s1[256], s2[256];
+  double a = 0, b = 0, m = -INFINITY;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double e = -1.0 * hi[i] + lo[i];
+    a += e * e;
+    b += hi[i] * hi[i];
+    m = fmax(m, fabs(e) / hi[i]);
+  }
+  s0[threadIdx.x] = a;
+  s1[threadIdx.x] = b;
+  s2[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+      s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = s0[0];
+    part[3 * blockIdx.x + 1] = s1[0];
+    part[3 * blockIdx.x + 2] = s2[0];
+  }
+}
+
+__global__ void k_err_fin(int nb, const double* __restrict__ part, double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double a = 0, b = 0, m = -INFINITY;
+  for (int i = 0; i < nb; ++i) {
+    a += part[3 * i];
+    b += part[3 * i + 1];
+    m = fmax(m, part[3 * i + 2]);
+  }
+  out[0] = a;
+  out[1] = b;
+  out[2] = m;
+}
+
+int launch_err(hipStream_t s, int n, const double* lo, const double* hi, double* out3,
+               double* partials) {
+  if (n <= 16384 || !partials) {
+    hipLaunchKernelGGL(k_err, dim3(1), dim3(256), 0, s, n, lo, hi, out3);
+  } else {
+    int nb = (n + 1023) / 1024;
+    if (nb > kErrBlocksMax) nb = kErrBlocksMax;
+    hipLaunchKernelGGL(k_err_part, dim3(nb), dim3(256), 0, s, n, lo, hi, partials);
+    hipLaunchKernelGGL(k_err_fin, dim3(1), dim3(64), 0, s, nb, partials, out3);
+  }
   SC_HIP(hipGetLastError());
   return SC_OK;
 }

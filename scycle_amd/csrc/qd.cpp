@@ -23,7 +23,8 @@ int launch_qd_bc(hipStream_t s, int Nz, const double* slip, const double* shift,
                  double vL, double fts, int remote, double* bcL, double* bcR);
 int launch_lincomb(hipStream_t s, int n, double* out, const double* base, int nt,
                    const double* const* f, const double* c);
-int launch_err(hipStream_t s, int n, const double* lo, const double* hi, double* out3);
+int launch_err(hipStream_t s, int n, const double* lo, const double* hi, double* out3,
+               double* partials);
 
 namespace {
 double now() {
@@ -77,23 +78,15 @@ int QdDriver::init(const sc_qd_desc& d, const double* mu, const double* y, const
   tau_d = dalloc(Nz, nullptr);
   strength_d = dalloc(Nz, nullptr);
   V_d = dalloc(Nz, nullptr);
-  err_d = dalloc(8, nullptr);
+  err_d = dalloc(16, nullptr);
   int* st = nullptr;
   if (hipMalloc(&st, sizeof(int)) != hipSuccess) return fail(SC_ERR_HIP, "hipMalloc status");
   owned.push_back(st);
   (void)hipMemset(st, 0, sizeof(int));
   status_d = st;
-  for (const char* key : {"psi", "slip"}) {
-    VecSet& v = vars[key];
-    v.var = dalloc(Nz, nullptr);
-    v.dvar = dalloc(Nz, nullptr);
-    for (int i = 0; i < 7; ++i) {
-      v.k[i] = dalloc(Nz, nullptr);
-      v.f[i] = dalloc(Nz, nullptr);
-    }
-    v.lo = dalloc(Nz, nullptr);
-    v.hi = dalloc(Nz, nullptr);
-  }
+  for (const char* key : {"psi", "slip"})
+    if ((e = add_var(key, Nz))) return e;
+  err_part_d = dalloc((size_t)3 * kErrBlocks, nullptr);
   if (hipHostMalloc(&host_scratch, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess)
     return fail(SC_ERR_HIP, "hipHostMalloc");
   for (void* p : owned)
@@ -101,14 +94,60 @@ int QdDriver::init(const sc_qd_desc& d, const double* mu, const double* y, const
   return SC_OK;
 }
 
-// StrikeSlip_LinearElastic_qd::d_dt (strikeSlip_linearElastic_qd.cpp:756-798)
-int QdDriver::d_dt(double t, const double* psi, const double* slip, double* dpsi, double* dslip) {
+int QdDriver::add_var(const char* key, int n) {
+  VecSet& v = vars[key];
+  v.n = n;
+  v.var = dalloc(n, nullptr);
+  v.dvar = dalloc(n, nullptr);
+  for (int i = 0; i < 7; ++i) {
+    v.k[i] = dalloc(n, nullptr);
+    v.f[i] = dalloc(n, nullptr);
+  }
+  v.lo = dalloc(n, nullptr);
+  v.hi = dalloc(n, nullptr);
+  for (void* p : owned)
+    if (!p) return fail(SC_ERR_HIP, "device allocation failed");
+  return SC_OK;
+}
+
+int QdDriver::init_powerlaw(const double* T, const double* A, const double* B, const double* n,
+                            double cap, int mask) {
+  pl.reset(new PowerLawState);
+  pl->T = dalloc(N, T);
+  pl->A = dalloc(N, A);
+  pl->B = dalloc(N, B);
+  pl->n = dalloc(N, n);
+  pl->effVisc = dalloc(N, nullptr);
+  pl->sxy = dalloc(N, nullptr);
+  pl->cap = cap;
+  errMask = mask;
+  int e = add_var("gVxy", N);
+  if (!e) e = add_var("gVxz", N);
+  SC_HIP(hipStreamSynchronize(ctx->c.stream));
+  return e;
+}
+
+int QdDriver::d_dt_vars(double t) {
+  VecSet& P = vars["psi"];
+  VecSet& S = vars["slip"];
+  if (!pl) return d_dt(t, P.var, S.var, P.dvar, S.dvar);
+  VecSet& X = vars["gVxy"];
+  VecSet& Z = vars["gVxz"];
+  return d_dt(t, P.var, S.var, P.dvar, S.dvar, X.var, Z.var, X.dvar, Z.dvar);
+}
+
+// StrikeSlip_LinearElastic_qd::d_dt (strikeSlip_linearElastic_qd.cpp:756-798);
+// with the power law, StrikeSlip_PowerLaw_qd::d_dt (strikeSlip_powerLaw_qd.cpp
+// :633-686) and solveMomentumBalance (:787-823)
+int QdDriver::d_dt(double t, const double* psi, const double* slip, double* dpsi, double* dslip,
+                   const double* gxy, const double* gxz, double* dgxy, double* dgxz) {
   hipStream_t s = ctx->c.stream;
   const double t0 = now();
   ++counters.d_dt_calls;
   int e = launch_qd_bc(s, Nz, slip, shift_d, t, desc.vL, desc.faultTypeScale, desc.remoteLoadingR,
                        bcL_d, bcR_d);
   if (!e) e = launch_set_rhs(ctx->c, rhs_d, bcL_d, bcR_d, bcT_d, bcB_d, s);
+  if (!e && pl) e = launch_pl_source(ctx->c, gxy, gxz, rhs_d, s);
   if (e) return e;
   const double ts = now();
   sc_solve_stats st;
@@ -118,7 +157,11 @@ int QdDriver::d_dt(double t, const double* psi, const double* slip, double* dpsi
   counters.cg_iters += st.iters;
   counters.outer_sweeps += st.outer;
   if (e) return e;
-  e = launch_fault_sxy(ctx->c, u_d, tauQS_d, s);
+  if (pl)
+    e = launch_pl_strain(ctx->c, u_d, gxy, gxz, pl->T, pl->A, pl->B, pl->n, bcL_d, bcR_d, pl->cap,
+                         dgxy, dgxz, tauQS_d, pl->effVisc, pl->sxy, s);
+  else
+    e = launch_fault_sxy(ctx->c, u_d, tauQS_d, s);
   if (e) return e;
   sc_fault_qd_args fa{};
   fa.N = Nz;
@@ -180,19 +223,27 @@ double QdDriver::step_size(const RkState& r, double totErr) const {
   return dt;
 }
 
-// computeError (odeSolver.cpp:449-501, 902-969) over errInds = [psi slip]
+// computeError (odeSolver.cpp:449-501, 902-969) over errInds (timeIntInds):
+// psi and slip by default, the viscous strains when selected
 int QdDriver::error(const RkState& r, double* totErr) {
   hipStream_t s = ctx->c.stream;
-  int e = launch_err(s, Nz, vars["psi"].lo, vars["psi"].hi, err_d);
-  if (!e) e = launch_err(s, Nz, vars["slip"].lo, vars["slip"].hi, err_d + 3);
-  if (e) return e;
-  SC_HIP(hipMemcpyAsync(host_scratch, err_d, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
+  const char* names[4] = {"psi", "slip", "gVxy", "gVxz"};
+  int sel[4], ns = 0;
+  for (int i = 0; i < 4; ++i)
+    if ((errMask & (1 << i)) && vars.count(names[i])) sel[ns++] = i;
+  for (int q = 0; q < ns; ++q) {
+    VecSet& v = vars[names[sel[q]]];
+    int e = launch_err(s, v.n, v.lo, v.hi, err_d + 3 * q, err_part_d);
+    if (e) return e;
+  }
+  SC_HIP(hipMemcpyAsync(host_scratch, err_d, 3 * ns * sizeof(double), hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
   const double* h = (const double*)host_scratch;
   double tot = 0;
-  for (int i = 0; i < 2; ++i) {
-    const double* v = h + 3 * i;
-    if (r.normType == 0) tot += std::sqrt(v[0]) / (std::sqrt((double)Nz) * 1.0);
+  for (int q = 0; q < ns; ++q) {
+    const double* v = h + 3 * q;
+    const double n = (double)vars[names[sel[q]]].n;
+    if (r.normType == 0) tot += std::sqrt(v[0]) / (std::sqrt(n) * 1.0);
     else if (r.ord == 3) tot += std::sqrt(v[0]) / (std::sqrt(v[1]) * 1.0);
     else tot += v[2] / 1.0;
   }
@@ -210,7 +261,7 @@ int QdDriver::lincomb_all(const char* out_sel, int stage, int nt, const int* fid
     else out = v.hi;
     const double* f[6];
     for (int i = 0; i < nt; ++i) f[i] = fidx[i] == 0 ? v.dvar : v.f[fidx[i]];
-    int e = launch_lincomb(s, Nz, out, v.var, nt, f, c);
+    int e = launch_lincomb(s, v.n, out, v.var, nt, f, c);
     if (e) return e;
   }
   return SC_OK;
@@ -219,7 +270,10 @@ int QdDriver::lincomb_all(const char* out_sel, int stage, int nt, const int* fid
 int QdDriver::stage(int st, double t) {
   VecSet& p = vars["psi"];
   VecSet& q = vars["slip"];
-  return d_dt(t, p.k[st], q.k[st], p.f[st], q.f[st]);
+  if (!pl) return d_dt(t, p.k[st], q.k[st], p.f[st], q.f[st]);
+  VecSet& x = vars["gVxy"];
+  VecSet& z = vars["gVxz"];
+  return d_dt(t, p.k[st], q.k[st], p.f[st], q.f[st], x.k[st], z.k[st], x.f[st], z.f[st]);
 }
 
 int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* dts, int dtsCap) {
@@ -237,8 +291,6 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
   const bool replay = run.nReplay > 0 && run.replayDt_h;
   if (replay && maxSteps > run.nReplay) maxSteps = run.nReplay;
   hipStream_t s = ctx->c.stream;
-  VecSet& P = vars["psi"];
-  VecSet& S = vars["slip"];
   int e;
   if (finalT == r.currT || maxSteps == 0) {
     *tEnd = r.currT;
@@ -246,7 +298,7 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
     return SC_OK;
   }
   if (r.deltaT == 0) r.deltaT = (finalT - r.currT) / maxSteps;
-  e = d_dt(r.currT, P.var, S.var, P.dvar, S.dvar);
+  e = d_dt_vars(r.currT);
   if (e) return e;
   double totErr = 0, newDt = r.deltaT;
   int nOut = 0;
@@ -302,11 +354,10 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
       if (r.minDt == r.deltaT) break;
     }
     r.currT = r.currT + r.deltaT;
-    for (auto& kv : vars) {
-      SC_HIP(hipMemcpyAsync(kv.second.var, kv.second.hi, Nz * sizeof(double), hipMemcpyDeviceToDevice, s));
-      SC_HIP(hipMemsetAsync(kv.second.dvar, 0, Nz * sizeof(double), s));
-    }
-    if ((e = d_dt(r.currT, P.var, S.var, P.dvar, S.dvar))) return e;
+    for (auto& kv : vars)
+      SC_HIP(hipMemcpyAsync(kv.second.var, kv.second.hi, (size_t)kv.second.n * sizeof(double),
+                            hipMemcpyDeviceToDevice, s));
+    if ((e = d_dt_vars(r.currT))) return e;
     if (totErr != 0.0) newDt = step_size(r, totErr);
     r.errA[1] = r.errA[0];
     r.errA[0] = totErr;
@@ -344,6 +395,60 @@ extern "C" int sc_qd_create(const sc_qd_desc* desc, const double* mu_h, const do
     return e;
   }
   *out = h;
+  return SC_OK;
+}
+
+extern "C" int sc_qd_create_powerlaw(const sc_qd_desc* desc, const double* mu_h, const double* y_h,
+                                     const double* z_h, const double* a_h, const double* b_h,
+                                     const double* Dc_h, const double* sN_h, const double* eta_h,
+                                     const double* locked_h, const double* prestress_h,
+                                     const double* bcRShift_h, const double* T_h,
+                                     const double* A_h, const double* B_h, const double* n_h,
+                                     double effViscCap, int errMask, sc_qd** out) {
+  if (!T_h || !A_h || !B_h || !n_h || !out) return fail(SC_ERR_ARG, "sc_qd_create_powerlaw: null argument");
+  if (!(effViscCap > 0)) return fail(SC_ERR_ARG, "effViscCap must be > 0");
+  if (errMask <= 0 || errMask > 15) return fail(SC_ERR_ARG, "errMask must select 1..4 fields");
+  if (desc && desc->grid.laplace != 0) return fail(SC_ERR_ARG, "power law needs laplace type yz");
+  sc_qd* h = nullptr;
+  int e = sc_qd_create(desc, mu_h, y_h, z_h, a_h, b_h, Dc_h, sN_h, eta_h, locked_h, prestress_h,
+                       bcRShift_h, &h);
+  if (e) return e;
+  e = h->q->init_powerlaw(T_h, A_h, B_h, n_h, effViscCap, errMask);
+  if (e) {
+    delete h;
+    return e;
+  }
+  *out = h;
+  return SC_OK;
+}
+
+extern "C" int sc_qd_set_visc(sc_qd* h, const double* gxy_h, const double* gxz_h) {
+  if (!h) return fail(SC_ERR_ARG, "null handle");
+  auto& q = *h->q;
+  if (!q.pl) return fail(SC_ERR_ARG, "not a power-law driver");
+  hipStream_t s = q.ctx->c.stream;
+  const size_t bytes = (size_t)q.N * sizeof(double);
+  if (gxy_h) SC_HIP(hipMemcpyAsync(q.vars["gVxy"].var, gxy_h, bytes, hipMemcpyHostToDevice, s));
+  if (gxz_h) SC_HIP(hipMemcpyAsync(q.vars["gVxz"].var, gxz_h, bytes, hipMemcpyHostToDevice, s));
+  SC_HIP(hipStreamSynchronize(s));
+  return SC_OK;
+}
+
+extern "C" int sc_qd_get_visc(sc_qd* h, double* gxy_h, double* gxz_h, double* dgxy_h,
+                              double* dgxz_h, double* effVisc_h, double* sxy_h) {
+  if (!h) return fail(SC_ERR_ARG, "null handle");
+  auto& q = *h->q;
+  if (!q.pl) return fail(SC_ERR_ARG, "not a power-law driver");
+  hipStream_t s = q.ctx->c.stream;
+  const size_t bytes = (size_t)q.N * sizeof(double);
+  SC_HIP(hipStreamSynchronize(s));
+  if (gxy_h) SC_HIP(hipMemcpyAsync(gxy_h, q.vars["gVxy"].var, bytes, hipMemcpyDeviceToHost, s));
+  if (gxz_h) SC_HIP(hipMemcpyAsync(gxz_h, q.vars["gVxz"].var, bytes, hipMemcpyDeviceToHost, s));
+  if (dgxy_h) SC_HIP(hipMemcpyAsync(dgxy_h, q.vars["gVxy"].dvar, bytes, hipMemcpyDeviceToHost, s));
+  if (dgxz_h) SC_HIP(hipMemcpyAsync(dgxz_h, q.vars["gVxz"].dvar, bytes, hipMemcpyDeviceToHost, s));
+  if (effVisc_h) SC_HIP(hipMemcpyAsync(effVisc_h, q.pl->effVisc, bytes, hipMemcpyDeviceToHost, s));
+  if (sxy_h) SC_HIP(hipMemcpyAsync(sxy_h, q.pl->sxy, bytes, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipStreamSynchronize(s));
   return SC_OK;
 }
 
@@ -391,7 +496,7 @@ extern "C" int sc_qd_ddt(sc_qd* h, double t, double* dpsi_h, double* dslip_h) {
   auto& q = *h->q;
   auto& P = q.vars["psi"];
   auto& S = q.vars["slip"];
-  int e = q.d_dt(t, P.var, S.var, P.dvar, S.dvar);
+  int e = q.d_dt_vars(t);
   if (e) return e;
   SC_HIP(hipStreamSynchronize(q.ctx->c.stream));
   if ((e = q.check_status())) return e;

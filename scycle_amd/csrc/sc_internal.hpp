@@ -147,6 +147,12 @@ int launch_deriv(const Ctx& c, int which, const double* in, double* out, hipStre
 int launch_fault_sxy(const Ctx& c, const double* u, double* out, hipStream_t s);
 
 int solve(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st);
+// power law: rhs += B gxy + C gxz; strains -> stresses -> viscosity -> rates
+int launch_pl_source(const Ctx& c, const double* gxy, const double* gxz, double* rhs, hipStream_t s);
+int launch_pl_strain(const Ctx& c, const double* u, const double* gVxy, const double* gVxz,
+                     const double* T, const double* A, const double* B, const double* n,
+                     const double* bcL, const double* bcR, double cap, double* dgxy, double* dgxz,
+                     double* tauQS, double* effVisc, double* sxyOut, hipStream_t s);
 int fault_qd_rates(hipStream_t s, const sc_fault_qd_args& a);
 // FD fault step; uBody (may be null) receives the fault displacement
 int fault_fd_step64(hipStream_t s, const sc_fault_fd_args& a, double* uBody);

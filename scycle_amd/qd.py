@@ -28,9 +28,12 @@ class QdMediator:
                       for x in (cfg.mu, cfg.y, cfg.z, cfg.a, cfg.b, cfg.Dc, cfg.sN, cfg.eta,
                                 cfg.locked, cfg.prestress, cfg.bcRShift)]
         h = C.c_void_p()
-        check(load().sc_qd_create(C.byref(d), *[dptr(x) for x in self._keep], C.byref(h)))
+        self._create(d, h)
         self.h = h
         self.Nz, self.N = cfg.Nz, cfg.Ny * cfg.Nz
+
+    def _create(self, d, h):
+        check(load().sc_qd_create(C.byref(d), *[dptr(x) for x in self._keep], C.byref(h)))
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -82,3 +85,33 @@ class QdMediator:
         c = QdCounters()
         check(load().sc_qd_get_counters(self.h, C.byref(c)))
         return {k: getattr(c, k) for k, _ in QdCounters._fields_}
+
+
+ERR_FIELDS = {"psi": 1, "slip": 2, "gVxy": 4, "gVxz": 8}
+
+
+class PowerLawMediator(QdMediator):
+    """StrikeSlip_PowerLaw_qd (strikeSlip_powerLaw_qd.cpp) with dislocation
+    creep: the integrated variables are psi, slip and the viscous strains
+    gVxy, gVxz.  cfg is a scycle_amd.configs.PowerLawConfig; err_inds are the
+    RK error fields (timeIntInds)."""
+
+    def __init__(self, cfg, err_inds=("psi", "slip"), **kw):
+        self._pl = [np.ascontiguousarray(x, dtype=np.float64) for x in (cfg.T, cfg.A, cfg.B, cfg.n)]
+        self._mask = sum(ERR_FIELDS[k] for k in err_inds)
+        super().__init__(cfg, **kw)
+
+    def _create(self, d, h):
+        check(load().sc_qd_create_powerlaw(C.byref(d), *[dptr(x) for x in self._keep],
+                                           *[dptr(x) for x in self._pl], self.cfg.visc_cap,
+                                           self._mask, C.byref(h)))
+
+    def set_visc(self, gVxy=None, gVxz=None):
+        a = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (gVxy, gVxz)]
+        check(load().sc_qd_set_visc(self.h, dptr(a[0]), dptr(a[1])))
+
+    def get_visc(self):
+        out = {k: np.zeros(self.N) for k in ("gVxy", "gVxz", "dgVxy", "dgVxz", "effVisc", "sxy")}
+        check(load().sc_qd_get_visc(self.h, *[dptr(out[k]) for k in
+                                             ("gVxy", "gVxz", "dgVxy", "dgVxz", "effVisc", "sxy")]))
+        return out

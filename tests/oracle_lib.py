@@ -66,6 +66,13 @@ def lib():
         L.orc_fdfault_free.argtypes = [C.c_void_p]
         L.orc_fdfault_step.argtypes = [C.c_void_p, C.c_double] + [_dp] * 9
         L.orc_powerlaw.argtypes = [C.c_int] + [_dp] * 8 + [C.c_double] + [_dp] * 3
+        L.orc_pl_new.restype = C.c_void_p
+        L.orc_pl_new.argtypes = [C.c_void_p] + [_dp] * 15 + [C.c_double]
+        L.orc_pl_free.argtypes = [C.c_void_p]
+        L.orc_pl_source.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.orc_pl_sat.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_pl_ddt.argtypes = [C.c_void_p, C.c_double] + [_dp] * 12
+        L.orc_pl_integrate.argtypes = [C.c_void_p, C.c_void_p, C.c_int] + [_dp] * 7 + [C.c_int]
         L.orc_rk_linear.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                     C.c_double, C.c_double, C.c_double, _dp, _dp,
                                     C.POINTER(C.c_long)]
@@ -252,3 +259,83 @@ def rk_linear(order, lam, dt, n, lam2=None, final_t=0.0, tol=1e-8, y0=1.0):
     steps = lib().orc_rk_linear(order, lam, lam if lam2 is None else lam2, dt, n, final_t, tol,
                                 y0, ptr(y), C.byref(t), C.byref(calls))
     return y, t.value, steps, calls.value
+
+
+class OrcQdCfg(C.Structure):
+    _fields_ = [("order", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int), ("var_grid", C.c_int),
+                ("Ly", C.c_double), ("Lz", C.c_double), ("bcR", C.c_int), ("bcT", C.c_int),
+                ("bcL", C.c_int), ("bcB", C.c_int), ("remoteLoadingR", C.c_int),
+                ("faultTypeScale", C.c_double), ("vL", C.c_double), ("f0", C.c_double),
+                ("v0", C.c_double), ("rootTol", C.c_double), ("stateLaw", C.c_int),
+                ("jac_mode", C.c_int), ("solver", C.c_int), ("cg_rtol", C.c_double)]
+
+
+class OrcQdRun(C.Structure):
+    _fields_ = [("integrator", C.c_int), ("control", C.c_int), ("normType", C.c_int),
+                ("maxSteps", C.c_int), ("initTime", C.c_double), ("maxTime", C.c_double),
+                ("initDt", C.c_double), ("minDt", C.c_double), ("maxDt", C.c_double),
+                ("tol", C.c_double), ("nReplay", C.c_int), ("replayDt", _dp)]
+
+
+class PowerLawOracle:
+    """StrikeSlip_PowerLaw_qd of the oracle (dislocation creep).  cfg is a
+    scycle_amd.configs.PowerLawConfig."""
+
+    def __init__(self, cfg, solver=0, jac_mode=1):
+        c = OrcQdCfg(cfg.order, cfg.Ny, cfg.Nz, int(cfg.var_grid), cfg.Ly, cfg.Lz, *cfg.mat_bc,
+                     int(cfg.remote_loading_R), cfg.fault_type_scale, cfg.vL, cfg.f0, cfg.v0,
+                     cfg.root_tol, cfg.state_law, jac_mode, solver, 1e-12)
+        self._keep = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in
+                      (cfg.mu, cfg.y, cfg.z, cfg.a, cfg.b, cfg.Dc, cfg.sN, cfg.eta, cfg.locked,
+                       cfg.prestress, cfg.bcRShift, cfg.T, cfg.A, cfg.B, cfg.n)]
+        self.N, self.Nz = cfg.Ny * cfg.Nz, cfg.Nz
+        self.h = lib().orc_pl_new(C.byref(c), *[ptr(x) for x in self._keep], cfg.visc_cap)
+        if not self.h:
+            raise RuntimeError("orc_pl_new failed")
+
+    def __del__(self):
+        try:
+            lib().orc_pl_free(self.h)
+        except Exception:
+            pass
+
+    def source(self, gxy, gxz):
+        out = np.zeros(self.N)
+        lib().orc_pl_source(self.h, ptr(np.ascontiguousarray(gxy, float)),
+                            ptr(np.ascontiguousarray(gxz, float)), ptr(out))
+        return out
+
+    def sat(self, u, gL, gR):
+        out = np.zeros(self.N)
+        lib().orc_pl_sat(self.h, *[ptr(np.ascontiguousarray(x, float)) for x in (u, gL, gR)], ptr(out))
+        return out
+
+    def d_dt(self, t, psi, slip, gxy, gxz, V=None):
+        N, Nz = self.N, self.Nz
+        V = np.zeros(Nz) if V is None else np.array(V, dtype=np.float64)
+        out = dict(dpsi=np.zeros(Nz), dslip=np.zeros(Nz), dgxy=np.zeros(N), dgxz=np.zeros(N),
+                   u=np.zeros(N), sxy=np.zeros(N), effVisc=np.zeros(N))
+        e = lib().orc_pl_ddt(self.h, t, *[ptr(np.ascontiguousarray(x, float)) for x in (psi, slip, gxy, gxz)],
+                             ptr(V), *[ptr(out[k]) for k in ("dpsi", "dslip", "dgxy", "dgxz", "u", "sxy",
+                                                              "effVisc")])
+        if e:
+            raise RuntimeError(f"orc_pl_ddt failed ({e})")
+        out["V"] = V
+        return out
+
+    def integrate(self, steps, psi, slip, gxy, gxz, integrator=0, tol=1e-8, init_dt=1e-3,
+                  init_time=0.0, max_time=1e30, max_dt=1e30, replay=None, err_mask=3, V=None):
+        N, Nz = self.N, self.Nz
+        st = [np.array(x, dtype=np.float64) for x in (psi, slip, gxy, gxz)]
+        Vv = np.zeros(Nz) if V is None else np.array(V, dtype=np.float64)
+        rp = None if replay is None else np.ascontiguousarray(replay, dtype=np.float64)
+        r = OrcQdRun(integrator, 1, 0, steps, init_time, max_time, init_dt, 0.0, max_dt, tol,
+                     0 if rp is None else len(rp), ptr(rp))
+        tEnd = C.c_double()
+        dts = np.zeros(max(steps, 1))
+        n = lib().orc_pl_integrate(self.h, C.byref(r), err_mask, *[ptr(x) for x in st], ptr(Vv),
+                                   C.byref(tEnd), ptr(dts), len(dts))
+        if n < 0:
+            raise RuntimeError(f"orc_pl_integrate failed ({n})")
+        return dict(psi=st[0], slip=st[1], gVxy=st[2], gVxz=st[3], V=Vv, t=tEnd.value, steps=n,
+                    dts=dts[:n])

@@ -282,3 +282,85 @@ def test_powerlaw_scaling_laws():
     sat = rng.random(N)
     _, gxy5, _ = _powerlaw(sxy, sxz, T, A, B, n, mu, sat=sat)
     assert np.allclose(gxy5 * ev1, sxy + mu * sat, rtol=1e-12)
+
+
+# ------------------------------------------------------- power-law QD ----
+def _pl_case(Ny=17, Nz=19, order=4, mat_bc=(0, 1, 0, 1)):
+    from scycle_amd.configs import power_law
+    c = power_law(Ny, Nz, order=order, Ly=20.0, Lz=24.0)
+    c.mat_bc = mat_bc
+    rng = np.random.default_rng(Ny + Nz + order)
+    c.mu = 30 + 6 * rng.random(Ny * Nz)
+    return c, rng
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_powerlaw_source_summation_by_parts(order):
+    """B g = yq zr H Dy (mu g) (+ Neumann SATs): by the SBP property
+    1^T H D1 w = w_N - w_0, the sum of B g over the domain is the boundary
+    difference of mu g on Dirichlet y-sides and vanishes on Neumann ones;
+    likewise for C in z."""
+    for bc in ((0, 1, 0, 1), (1, 1, 1, 1)):
+        c, rng = _pl_case(order=order, mat_bc=bc)
+        o = ol.PowerLawOracle(c, solver=1)  # no factorisation: all-Neumann A is singular
+        Ny, Nz, N = c.Ny, c.Nz, c.Ny * c.Nz
+        g = rng.standard_normal(N)
+        zero = np.zeros(N)
+        Bg, Cg = o.source(g, zero), o.source(zero, g)
+        w = (c.mu * g).reshape(Ny, Nz)
+        Hz = np.diag(ol.sbp1d(order, Nz, (Nz - 1) / c.Lz)["H"])
+        Hy = np.diag(ol.sbp1d(order, Ny, (Ny - 1) / c.Ly)["H"])
+        scale = np.abs(Bg).sum()
+        expect_B = 0.0 if bc[2] == 1 else float((Hz * (w[-1] - w[0])).sum())
+        assert abs(Bg.sum() - expect_B) < 1e-12 * scale
+        expect_C = 0.0 if bc[1] == 1 else float((Hy * (w[:, -1] - w[:, 0])).sum())
+        assert abs(Cg.sum() - expect_C) < 1e-12 * np.abs(Cg).sum()
+
+
+def test_powerlaw_sat_rows():
+    """The strain-rate SAT lives on the Dirichlet y-sides only:
+    Hy^-1[0] (u - gL) at j = 0 and -Hy^-1[N] (u - gR) at j = Ny-1."""
+    c, rng = _pl_case()
+    o = ol.PowerLawOracle(c)
+    Ny, Nz, N = c.Ny, c.Nz, c.Ny * c.Nz
+    u, gL, gR = rng.standard_normal(N), rng.standard_normal(Nz), rng.standard_normal(Nz)
+    sat = o.sat(u, gL, gR).reshape(Ny, Nz)
+    Hinv = np.diag(ol.sbp1d(4, Ny, (Ny - 1) / c.Ly)["Hinv"])
+    U = u.reshape(Ny, Nz)
+    assert np.allclose(sat[0], Hinv[0] * (U[0] - gL), rtol=1e-14, atol=0)
+    assert np.allclose(sat[-1], -Hinv[-1] * (U[-1] - gR), rtol=1e-14, atol=0)
+    assert np.all(sat[1:-1] == 0.0)
+
+
+def test_powerlaw_without_creep_is_linear_elastic():
+    """With vanishing creep (A = 0) and zero viscous strain the power-law
+    mediator's fault rates equal the linear-elastic mediator's, and the
+    viscous strain rates are sxy / cap."""
+    c, rng = _pl_case()
+    c.A = np.zeros(c.Ny * c.Nz)
+    o = ol.PowerLawOracle(c)
+    N, Nz = c.Ny * c.Nz, c.Nz
+    slip = 1e-3 * rng.random(Nz)
+    psi = c.psi0
+    r = o.d_dt(3e7, psi, slip, np.zeros(N), np.zeros(N))
+    cfg = ol.OrcQdCfg(c.order, c.Ny, c.Nz, 0, c.Ly, c.Lz, *c.mat_bc, 1, 2.0, c.vL, c.f0, c.v0,
+                      c.root_tol, c.state_law, 1, 0, 1e-12)
+    keep = [np.ascontiguousarray(x, dtype=np.float64) for x in (c.mu, c.a, c.b, c.Dc, c.sN, c.eta)]
+    h = lib().orc_qd_new(C_byref(cfg), ptr(keep[0]), None, None, ptr(keep[1]), ptr(keep[2]),
+                         ptr(keep[3]), ptr(keep[4]), ptr(keep[5]), None, None, None)
+    dpsi, dslip, V = np.zeros(Nz), np.zeros(Nz), np.zeros(Nz)
+    assert lib().orc_qd_ddt(h, 3e7, ptr(np.ascontiguousarray(psi)), ptr(slip), ptr(V), ptr(dpsi),
+                            ptr(dslip), None, None) == 0
+    lib().orc_qd_free(h)
+    assert np.allclose(r["dslip"], dslip, rtol=1e-12, atol=0)
+    assert np.allclose(r["dpsi"], dpsi, rtol=1e-12, atol=1e-300)
+    # eta = cap: d/dt gxy = (sxy + mu*SAT)/cap with the SAT of the boundary data
+    t = 3e7
+    sat = o.sat(r["u"], slip / 2.0, np.full(Nz, c.vL * t / 2.0))
+    assert np.allclose(r["effVisc"], 1e30, rtol=1e-15)
+    assert np.allclose(r["dgxy"] * 1e30, r["sxy"] + c.mu * sat, rtol=1e-12, atol=1e-12 * np.abs(r["sxy"]).max())
+
+
+def C_byref(x):
+    import ctypes
+    return ctypes.byref(x)
