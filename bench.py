@@ -16,11 +16,12 @@ Solver paths, all from the same state over the same step window:
   mixed_bf16  ... in bf16 with FP32 accumulation
   fp64        plain FP64 Jacobi-PCG (the straightforward GPU port)
   ir32        FP32 Jacobi-PCG inside FP64 iterative refinement
-The FD-preconditioned paths start each solve from the projection of the new
-solution onto the last --reuse solutions (0 = previous solution only).
-`value` is the fastest path whose fault slip stays within 1e-8 (relative) of
-the fp64_fdiag path over the window; the line names it (config.solve) and
-lists every path.
+  fp64_ref    fp64_fdiag started from the previous solution only: the
+              accuracy reference
+The other FD-preconditioned paths start each solve from the projection of the
+new solution onto the last --reuse solutions.  `value` is the fastest path
+whose fault slip stays within 1e-8 (relative) of fp64_ref over the window; the
+line names it (config.solve) and lists every path.
 
 Also on the line: the HBM roofline of the FP64 SBP-SAT apply (interior and
 frame kernels, 24 algorithmic bytes per node: read u, read mu, write A u),
@@ -54,6 +55,7 @@ def log(*a):
 PATHS = {
     # label: (solve mode, preconditioner, dtype string)
     "fp64_fdiag": ("cg64", "fdiag64", "f64"),
+    "fp64_ref": ("cg64", "fdiag64", "f64"),
     "mixed": ("cg64", "fdiag32", "f64 CG + f32 GEMM preconditioner"),
     "mixed_bf16": ("cg64", "fdiag_bf16", "f64 CG + bf16 GEMM preconditioner"),
     "fp64": ("cg64", "jacobi", "f64"),
@@ -81,7 +83,8 @@ def run_path(cfg, label, warmup, steps, reuse=0):
     sec = time.perf_counter() - tic
     c1 = q.counters()
     st = q.get_state()
-    d = {k: c1[k] - c0[k] for k in ("d_dt_calls", "solves", "cg_iters", "outer_sweeps")}
+    d = {k: c1[k] - c0[k] for k in ("d_dt_calls", "solves", "cg_iters", "outer_sweeps",
+                                    "deferred_checks", "missed_guesses", "rejected_steps")}
     return q, dict(seconds=sec, steps=n, t_start=t0, t_end=t1, counters=d, state=st,
                    setup_s=setup, dt_min=float(dts.min()), dt_max=float(dts.max()), reuse=reuse)
 
@@ -205,7 +208,7 @@ def main():
     ap.add_argument("--no-fd", action="store_true")
     ap.add_argument("--fd-n", type=int, default=2001)
     ap.add_argument("--fd-steps", type=int, default=2000)
-    ap.add_argument("--paths", default="fp64_fdiag,mixed,mixed_bf16,fp64")
+    ap.add_argument("--paths", default="fp64_ref,fp64_fdiag,mixed,mixed_bf16,fp64")
     ap.add_argument("--reuse", type=int, default=4,
                     help="previous solutions projected for the CG initial guess (FD paths)")
     ap.add_argument("--slow-steps", type=int, default=3,
@@ -225,14 +228,15 @@ def main():
     cfg = config_c2(args.n, args.n)
     res = {}
     labels = args.paths.split(",")
-    ref = "fp64_fdiag" if "fp64_fdiag" in labels else labels[0]
+    ref = "fp64_ref" if "fp64_ref" in labels else labels[0]
     ctx_q = None
     for label in labels:
         jac = PATHS[label][1] == "jacobi"
         k = min(args.steps, args.slow_steps) if jac else args.steps
         log(f"[bench] path {label} {PATHS[label][:2]}: warmup {args.warmup} steps, timed {k} steps, "
             f"{cfg.Ny}x{cfg.Nz}")
-        q, r = run_path(cfg, label, args.warmup, k, reuse=0 if jac else args.reuse)
+        q, r = run_path(cfg, label, args.warmup, k,
+                        reuse=0 if (jac or label == "fp64_ref") else args.reuse)
         log(f"[bench] {label}: {r['steps']} steps in {r['seconds']:.3f} s (setup {r['setup_s']:.1f} s), "
             f"counters {r['counters']}")
         res[label] = r
@@ -240,13 +244,16 @@ def main():
             ctx_q = q
         else:
             q.close()
-    # accuracy of each path against the FP64 FD-preconditioned path over the same window
+    # accuracy of each path against the FP64 reference path over the same window
     paths = {}
     for lab, rr in res.items():
         p = dict(solve=PATHS[lab][0], precond=PATHS[lab][1], reuse=rr["reuse"], steps=rr["steps"],
                  steps_per_s=rr["steps"] / rr["seconds"], ms_per_step=1e3 * rr["seconds"] / rr["steps"],
                  cg_iters_per_solve=rr["counters"]["cg_iters"] / max(rr["counters"]["solves"], 1),
                  solves_per_step=rr["counters"]["solves"] / rr["steps"], setup_s=rr["setup_s"],
+                 deferred_checks=rr["counters"]["deferred_checks"],
+                 missed_guesses=rr["counters"]["missed_guesses"],
+                 rejected_steps=rr["counters"]["rejected_steps"],
                  t_start=rr["t_start"], t_end=rr["t_end"], dt_min=rr["dt_min"], dt_max=rr["dt_max"])
         if lab != ref and rr["steps"] == res[ref]["steps"]:
             a, b = rr["state"]["slip"], res[ref]["state"]["slip"]

@@ -295,6 +295,9 @@ typedef struct sc_qd_run {
 typedef struct sc_qd_counters {
   long long d_dt_calls, solves, cg_iters, outer_sweeps, rejected_steps;
   double solve_seconds, total_seconds;
+  /* solves inside sc_qd_integrate whose projected guess was accepted on the
+   * device, and the attempts repeated because such a guess missed */
+  long long deferred_checks, missed_guesses;
 } sc_qd_counters;
 
 /* StrikeSlip_LinearElastic_qd (strikeSlip_linearElastic_qd.cpp:7-78) with

@@ -88,7 +88,8 @@ class QdCounters(C.Structure):
     _fields_ = [("d_dt_calls", C.c_longlong), ("solves", C.c_longlong),
                 ("cg_iters", C.c_longlong), ("outer_sweeps", C.c_longlong),
                 ("rejected_steps", C.c_longlong), ("solve_seconds", C.c_double),
-                ("total_seconds", C.c_double)]
+                ("total_seconds", C.c_double), ("deferred_checks", C.c_longlong),
+                ("missed_guesses", C.c_longlong)]
 
 
 # Every symbol include/scycle_hip.h declares, with (restype, argtypes).

@@ -17,6 +17,8 @@
 #include <cstring>
 
 #include "fdiag.hpp"
+#include "reduce_dev.hpp"
+#include "sbp_dev.hpp"
 #include "sc_internal.hpp"
 
 namespace sc {
@@ -27,43 +29,13 @@ struct CgScal {
   unsigned int cnt;
 };
 
-constexpr int kRedThreads = 256;
-constexpr int kRedBlocks = 2048;  // grid-stride reduction kernels (max blocks)
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// Block-reduce NV values and write this block's partial sums.  The partials
-// are combined by k_finish, a one-block kernel launched right after: the
-// kernel boundary orders the two, so no device-scope fence is needed.
-template <int NV>
-__device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ partials) {
-  __shared__ double sh[NV][kRedThreads / 64];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    double s = wave_sum(v[i]);
-    if (lane == 0) sh[i][wid] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x < NV) {
-    double s = 0;
-#pragma unroll
-    for (int w = 0; w < kRedThreads / 64; ++w) s += sh[threadIdx.x][w];
-    partials[(size_t)blockIdx.x * NV + threadIdx.x] = s;
-  }
-}
-
 enum FinishOp { FIN_INIT, FIN_PQ, FIN_UPDATE, FIN_IRRES, FIN_UPDATE_R, FIN_RZ };
 
 // Sum the partials of nb blocks in a fixed order (deterministic for a given
 // grid) and update the CG scalars.  One block of kRedThreads threads.
 template <int OP, int NV>
 __global__ __launch_bounds__(kRedThreads) void k_finish(CgScal* sc, const double* __restrict__ partials,
-                                                        int nb, int first) {
+                                                        int nb, int first, int* flag) {
   if ((OP == FIN_PQ || OP == FIN_UPDATE || OP == FIN_UPDATE_R || OP == FIN_RZ) && sc->done) return;
   __shared__ double sh[NV][kRedThreads / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -110,6 +82,7 @@ __global__ __launch_bounds__(kRedThreads) void k_finish(CgScal* sc, const double
     sc->rr = t[0];
     sc->bb = t[1];
     sc->done = (t[0] <= sc->tol2 * t[1] || t[1] == 0.0) ? 1 : 0;
+    if (flag && !sc->done) *flag = 1;  // a deferred check records the miss
   } else if constexpr (OP == FIN_UPDATE_R) {  // rr
     sc->rr = t[0];
     sc->iters += 1;
@@ -277,9 +250,6 @@ __global__ __launch_bounds__(256) void k_cg_pz(size_t N, const double* __restric
 // After a solve with solution x (A x = b - r) the new basis vector is
 //   y = x - sum_i beta_i y_i,  beta_i = -w_i y_i^T (b - r),
 // and y^T (-A) y = -y^T (b - r) since y is A-orthogonal to every y_i.
-struct VecList {
-  const double* v[SC_REUSE_MAX];
-};
 
 // partial sums of y_m . (a - b)   (b may be null)
 template <int M>
@@ -323,43 +293,116 @@ __global__ __launch_bounds__(kRedThreads) void k_proj_coef(const double* __restr
   }
 }
 
-// x = sum_m coef[m] y_m
+// partial sums of y_m . b over the boundary frame only, for a b that is zero
+// elsewhere
 template <int M>
-__global__ __launch_bounds__(256) void k_proj_combine(size_t N, VecList Y, const double* __restrict__ wc,
-                                                      double* __restrict__ x) {
-  double c[M];
+__global__ __launch_bounds__(kRedThreads) void k_multidot_frame(FrameMap fm, int Nz, const double* __restrict__ b,
+                                                                VecList Y, double* partials) {
+  double acc[M];
 #pragma unroll
-  for (int m = 0; m < M; ++m) c[m] = wc[SC_REUSE_MAX + m];
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
-    double v = 0;
+  for (int m = 0; m < M; ++m) acc[m] = 0.0;
+  const long long n = fm.count();
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    int j, k;
+    fm.node(t, j, k);
+    const size_t i = (size_t)j * Nz + k;
+    const double v = b[i];
 #pragma unroll
-    for (int m = 0; m < M; ++m) v += c[m] * Y.v[m][i];
-    x[i] = v;
+    for (int m = 0; m < M; ++m) acc[m] += Y.v[m][i] * v;
   }
+  block_partials(acc, partials);
 }
 
-// y = x - sum_m coef[m] y_m, partial sums of y . (b - r) and x . (b - r).  y may be one of the
-// y_m (the slot being replaced): every element is read before it is written
+// The projected guess and its residual in one pass.  Every block first sums
+// the mnb multidot partials y_m . b in the same fixed order, so all blocks
+// hold the same coef[m] = -w[m] y_m . b; then x = sum_m coef[m] y_m and
+// r = b - sum_m coef[m] z_m with z_m = A y_m kept beside the basis, partial
+// sums of r.r and b.b (r itself is not stored: a solve that goes on from x
+// recomputes it).  Block 0 also starts the residual check (tolerance tol2).
+template <int M>
+__global__ __launch_bounds__(kRedThreads) void k_proj_guess_res(size_t N, VecList Y, VecList Z,
+                                                                const double* __restrict__ mpart, int mnb,
+                                                                const double* __restrict__ w, CgScal* sc,
+                                                                double tol2, const double* __restrict__ b,
+                                                                double* __restrict__ x, double* partials) {
+  __shared__ double shc[M][kRedThreads / 64];
+  __shared__ double coef[M];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  {
+    double a[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) a[m] = 0.0;
+    for (int q = threadIdx.x; q < mnb; q += kRedThreads) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) a[m] += mpart[(size_t)q * M + m];
+    }
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const double t = wave_sum(a[m]);
+      if (lane == 0) shc[m][wid] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < M) {
+      double t = 0;
+      for (int ww = 0; ww < kRedThreads / 64; ++ww) t += shc[threadIdx.x][ww];
+      coef[threadIdx.x] = -w[threadIdx.x] * t;
+    }
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->tol2 = tol2;
+    sc->maxit = 0;
+    sc->done = 0;
+    sc->iters = 0;
+    sc->cnt = 0;
+  }
+  double c[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) c[m] = coef[m];
+  double acc[2] = {0.0, 0.0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+    double v = 0.0, az = 0.0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      v += c[m] * Y.v[m][i];
+      az += c[m] * Z.v[m][i];
+    }
+    const double bi = b[i];
+    const double ri = bi - az;
+    x[i] = v;
+    acc[0] += ri * ri;
+    acc[1] += bi * bi;
+  }
+  block_partials(acc, partials);
+}
+
+// New basis vector, A-orthogonal to the others: y = x - sum_m coef[m] y_m and
+// z = A x - sum_m coef[m] z_m (= A y), partial sums of y.z and x.(A x).  y, z
+// may be the slot being replaced: every element is read before it is written
 // by the same thread.
 template <int M>
-__global__ __launch_bounds__(kRedThreads) void k_proj_update(size_t N, VecList Y, const double* __restrict__ wc,
-                                                             const double* __restrict__ x,
-                                                             const double* __restrict__ b,
-                                                             const double* __restrict__ r, double* y,
-                                                             double* partials) {
+__global__ __launch_bounds__(kRedThreads) void k_proj_update_az(size_t N, VecList Y, VecList Z,
+                                                                const double* __restrict__ wc,
+                                                                const double* __restrict__ x,
+                                                                const double* __restrict__ ax,
+                                                                double* y, double* z, double* partials) {
   double c[M];
 #pragma unroll
   for (int m = 0; m < M; ++m) c[m] = wc[SC_REUSE_MAX + m];
   double acc[2] = {0.0, 0.0};
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
-    const double xi = x[i];
-    double v = xi;
+    const double xi = x[i], axi = ax[i];
+    double v = xi, w = axi;
 #pragma unroll
-    for (int m = 0; m < M; ++m) v -= c[m] * Y.v[m][i];
+    for (int m = 0; m < M; ++m) {
+      v -= c[m] * Y.v[m][i];
+      w -= c[m] * Z.v[m][i];
+    }
     y[i] = v;
-    const double ax = b[i] - r[i];
-    acc[0] += v * ax;
-    acc[1] += xi * ax;
+    z[i] = w;
+    acc[0] += v * w;
+    acc[1] += xi * axi;
   }
   block_partials(acc, partials);
 }
@@ -440,9 +483,9 @@ unsigned ew_blocks(size_t N) {
 }
 
 template <int OP, int NV>
-void finish(hipStream_t s, CgScal* sc, const Ctx& c, size_t N, int first = 0) {
+void finish(hipStream_t s, CgScal* sc, const Ctx& c, size_t N, int first = 0, int* flag = nullptr) {
   hipLaunchKernelGGL((k_finish<OP, NV>), dim3(1), dim3(kRedThreads), 0, s, sc, c.partials,
-                     (int)red_blocks(N), first);
+                     (int)red_blocks(N), first, flag);
 }
 
 int read_scal(Ctx& c, CgScal* out, int which) {
@@ -548,99 +591,191 @@ int pcg_fdiag(Ctx& c, const double* b, double* x, int fprec, double rtol, int ma
   return SC_OK;
 }
 
+// multidot partials of the guess sit above the partials of the guess kernel
+constexpr int kMdOffset = 2 * kRedBlocks;
+constexpr int kMdDoubles = 2 * kRedBlocks;
+constexpr int kMdBlocksMax = 256;
+
 template <int M>
-void proj_guess_m(Ctx& c, const VecList& Y, const double* b, double* x) {
+void proj_guess_m(Ctx& c, const VecList& Y, const VecList& Z, const double* b, double* x, double tol2,
+                  bool rhs_frame, int pre_mnb) {
+  const size_t N = (size_t)c.N;
+  hipStream_t s = c.stream;
+  double* mpart = c.partials + kMdOffset;
+  static_assert(kMdBlocksMax * SC_REUSE_MAX <= kMdDoubles, "multidot partials overflow");
+  unsigned mnb;
+  if (pre_mnb > 0) {
+    mnb = (unsigned)pre_mnb;  // the caller's kernel wrote them
+  } else if (rhs_frame) {
+    const FrameMap fm = FrameMap::make(c.d.Ny, c.d.Nz);
+    mnb = (unsigned)std::min<long long>((fm.count() + kRedThreads - 1) / kRedThreads, kMdBlocksMax);
+    hipLaunchKernelGGL(k_multidot_frame<M>, dim3(mnb), dim3(kRedThreads), 0, s, fm, c.d.Nz, b, Y, mpart);
+  } else {
+    mnb = std::min<unsigned>(red_blocks(N), kMdBlocksMax);
+    hipLaunchKernelGGL(k_multidot<M>, dim3(mnb), dim3(kRedThreads), 0, s, N, b, (const double*)nullptr, Y,
+                       mpart);
+  }
+  hipLaunchKernelGGL(k_proj_guess_res<M>, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, Y, Z,
+                     (const double*)mpart, (int)mnb, (const double*)c.projW, (CgScal*)c.scal, tol2, b, x,
+                     c.partials);
+}
+
+template <int M>
+void proj_add_m(Ctx& c, const VecList& Y, const VecList& Z, const double* x, const double* ax, double* y,
+                double* z) {
   const size_t N = (size_t)c.N;
   hipStream_t s = c.stream;
   const unsigned nb = red_blocks(N);
-  hipLaunchKernelGGL(k_multidot<M>, dim3(nb), dim3(kRedThreads), 0, s, N, b, (const double*)nullptr, Y,
+  hipLaunchKernelGGL(k_multidot<M>, dim3(nb), dim3(kRedThreads), 0, s, N, ax, (const double*)nullptr, Y,
                      c.partials);
   hipLaunchKernelGGL(k_proj_coef<M>, dim3(1), dim3(kRedThreads), 0, s, c.partials, (int)nb, c.projW);
-  hipLaunchKernelGGL(k_proj_combine<M>, dim3(ew_blocks(N)), dim3(256), 0, s, N, Y, c.projW, x);
-}
-
-template <int M>
-void proj_add_m(Ctx& c, const VecList& Y, const double* b, const double* x, const double* r, double* y) {
-  const size_t N = (size_t)c.N;
-  hipStream_t s = c.stream;
-  const unsigned nb = red_blocks(N);
-  hipLaunchKernelGGL(k_multidot<M>, dim3(nb), dim3(kRedThreads), 0, s, N, b, r, Y, c.partials);
-  hipLaunchKernelGGL(k_proj_coef<M>, dim3(1), dim3(kRedThreads), 0, s, c.partials, (int)nb, c.projW);
-  hipLaunchKernelGGL(k_proj_update<M>, dim3(nb), dim3(kRedThreads), 0, s, N, Y, c.projW, x, b, r, y,
+  hipLaunchKernelGGL(k_proj_update_az<M>, dim3(nb), dim3(kRedThreads), 0, s, N, Y, Z, c.projW, x, ax, y, z,
                      c.partials);
 }
 
+// guess: x (p1) = projection of the solution of A x = b (p0) and a residual
+// check of tolerance tol2; add: new basis pair from x (p0) and A x (p1) into
+// y (p2), z (p3)
+struct ProjArgs {
+  const double* p0;
+  double *p1, *p2, *p3;
+  double tol2;
+  bool rhs_frame;
+  int pre_mnb;
+};
+
 template <int M>
-void proj_dispatch(bool guess, Ctx& c, const VecList& Y, const double* b, double* x, const double* r,
-                   double* y) {
-  if (guess) proj_guess_m<M>(c, Y, b, x);
-  else proj_add_m<M>(c, Y, b, x, r, y);
+void proj_dispatch(bool guess, Ctx& c, const VecList& Y, const VecList& Z, const ProjArgs& a) {
+  if (guess) proj_guess_m<M>(c, Y, Z, a.p0, a.p1, a.tol2, a.rhs_frame, a.pre_mnb);
+  else proj_add_m<M>(c, Y, Z, a.p0, a.p1, a.p2, a.p3);
 }
 
-void proj_run(bool guess, int M, Ctx& c, const VecList& Y, const double* b, double* x,
-              const double* r, double* y) {
+void proj_run(bool guess, int M, Ctx& c, const VecList& Y, const VecList& Z, const ProjArgs& a) {
   switch (M) {
-    case 1: proj_dispatch<1>(guess, c, Y, b, x, r, y); break;
-    case 2: proj_dispatch<2>(guess, c, Y, b, x, r, y); break;
-    case 3: proj_dispatch<3>(guess, c, Y, b, x, r, y); break;
-    case 4: proj_dispatch<4>(guess, c, Y, b, x, r, y); break;
-    case 5: proj_dispatch<5>(guess, c, Y, b, x, r, y); break;
-    case 6: proj_dispatch<6>(guess, c, Y, b, x, r, y); break;
-    case 7: proj_dispatch<7>(guess, c, Y, b, x, r, y); break;
-    default: proj_dispatch<8>(guess, c, Y, b, x, r, y); break;
+    case 1: proj_dispatch<1>(guess, c, Y, Z, a); break;
+    case 2: proj_dispatch<2>(guess, c, Y, Z, a); break;
+    case 3: proj_dispatch<3>(guess, c, Y, Z, a); break;
+    case 4: proj_dispatch<4>(guess, c, Y, Z, a); break;
+    case 5: proj_dispatch<5>(guess, c, Y, Z, a); break;
+    case 6: proj_dispatch<6>(guess, c, Y, Z, a); break;
+    case 7: proj_dispatch<7>(guess, c, Y, Z, a); break;
+    default: proj_dispatch<8>(guess, c, Y, Z, a); break;
   }
 }
 
 int proj_prepare(Ctx& c, int cap) {
   if (c.projCap == cap) return SC_OK;
-  for (double*& p : c.projY)
-    if (p) {
-      SC_HIP(hipFree(p));
-      p = nullptr;
-    }
-  for (int i = 0; i < cap; ++i) SC_HIP(hipMalloc(&c.projY[i], (size_t)c.N * sizeof(double)));
+  for (int i = 0; i < SC_REUSE_MAX; ++i) {
+    if (c.projY[i]) SC_HIP(hipFree(c.projY[i]));
+    if (c.projZ[i]) SC_HIP(hipFree(c.projZ[i]));
+    c.projY[i] = c.projZ[i] = nullptr;
+  }
+  for (int i = 0; i < cap; ++i) {
+    SC_HIP(hipMalloc(&c.projY[i], (size_t)c.N * sizeof(double)));
+    SC_HIP(hipMalloc(&c.projZ[i], (size_t)c.N * sizeof(double)));
+  }
   if (!c.projW) SC_HIP(hipMalloc(&c.projW, 2 * SC_REUSE_MAX * sizeof(double)));
   c.projCap = cap;
   c.projCount = c.projNext = 0;
   return SC_OK;
 }
 
-VecList proj_list(const Ctx& c) {
+VecList proj_list(const Ctx& c, bool az) {
   VecList Y{};
-  for (int i = 0; i < c.projCount; ++i) Y.v[i] = c.projY[i];
+  for (int i = 0; i < c.projCount; ++i) Y.v[i] = az ? c.projZ[i] : c.projY[i];
   return Y;
+}
+
+int check_opts(const sc_solve_opts& o) {
+  if (o.mode != SC_SOLVE_CG64 && o.mode != SC_SOLVE_IR32) return fail(SC_ERR_ARG, "unknown solve mode");
+  if (!(o.rtol > 0)) return fail(SC_ERR_ARG, "rtol must be > 0");
+  if (o.precond < SC_PREC_JACOBI || o.precond > SC_PREC_FDIAG_BF16)
+    return fail(SC_ERR_ARG, "unknown preconditioner");
+  if (o.reuse < 0 || o.reuse > SC_REUSE_MAX) return fail(SC_ERR_ARG, "reuse must be in [0, SC_REUSE_MAX]");
+  return SC_OK;
 }
 
 }  // namespace
 
 int solve_core(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st);
 
-int solve(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st) {
-  if (o.reuse < 0 || o.reuse > SC_REUSE_MAX) return fail(SC_ERR_ARG, "reuse must be in [0, SC_REUSE_MAX]");
+// With o.reuse > 0 the solve starts from the A-orthogonal projection of the
+// new solution onto up to o.reuse earlier ones (Fischer's projection), with
+// A y kept beside each basis vector y so that the guess's residual costs no
+// operator apply.  A guess that already meets the tolerance is the answer.
+// With miss_flag set and a basis present the check is deferred: u gets the
+// guess, *miss_flag is set to 1 on the device if it misses the tolerance, and
+// *deferred tells the caller, who must test the flag before trusting u.
+int proj_frame_plan(Ctx& c, const sc_solve_opts& o, ProjFramePlan* plan) {
+  plan->M = 0;
+  int e = check_opts(o);
+  if (e || o.reuse == 0) return e;
+  e = ensure_workspace(c, o.mode == SC_SOLVE_IR32);
+  if (e) return e;
+  e = proj_prepare(c, o.reuse);
+  if (e || c.projCount == 0) return e;
+  plan->M = c.projCount;
+  plan->Y = proj_list(c, false);
+  plan->mpart = c.partials + kMdOffset;
+  plan->maxBlocks = kMdBlocksMax;
+  return SC_OK;
+}
+
+int solve(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st,
+          int* miss_flag, bool* deferred, bool rhs_frame, int pre_mnb) {
+  if (deferred) *deferred = false;
+  int e = check_opts(o);
+  if (e) return e;
   if (o.reuse == 0) return solve_core(c, rhs, u, o, st);
-  int e = ensure_workspace(c, o.mode == SC_SOLVE_IR32);
+  e = ensure_workspace(c, o.mode == SC_SOLVE_IR32);
   if (e) return e;
   e = proj_prepare(c, o.reuse);
   if (e) return e;
-  if (c.projCount > 0) proj_run(true, c.projCount, c, proj_list(c), rhs, u, nullptr, nullptr);
+  hipStream_t s = c.stream;
+  const size_t N = (size_t)c.N;
+  if (c.projCount > 0) {
+    CgScal* sc = (CgScal*)c.scal;
+    proj_run(true, c.projCount, c, proj_list(c, false), proj_list(c, true),
+             ProjArgs{rhs, u, nullptr, nullptr, o.rtol * o.rtol, rhs_frame, pre_mnb});
+    finish<FIN_IRRES, 2>(s, sc, c, N, 0, miss_flag);
+    SC_HIP(hipGetLastError());
+    if (miss_flag) {
+      std::memset(&st, 0, sizeof(st));
+      st.converged = 1;
+      st.relres = -1.0;  // not read back
+      if (deferred) *deferred = true;
+      return SC_OK;
+    }
+    CgScal h;
+    e = read_scal(c, &h, 0);
+    if (e) return e;
+    if (h.done) {
+      std::memset(&st, 0, sizeof(st));
+      st.converged = 1;
+      st.relres = h.bb > 0 ? std::sqrt(h.rr / h.bb) : 0.0;
+      // the answer lies in the span of the basis, which therefore stays as it is
+      return SC_OK;
+    }
+  }
   e = solve_core(c, rhs, u, o, st);
   if (e) return e;
-  // r64 holds b - A u of the converged solve
+  // add the solution to the basis: q64 = A u
   const int slot = c.projNext;
-  const VecList Y = proj_list(c);
   double* y = c.projY[slot];
-  const int nb = (int)red_blocks(c.N);
+  double* z = c.projZ[slot];
+  const int nb = (int)red_blocks(N);
+  e = launch_apply64(c, u, c.q64, s);
+  if (e) return e;
   if (c.projCount > 0) {
-    proj_run(false, c.projCount, c, Y, rhs, u, c.r64, y);
-    hipLaunchKernelGGL(k_proj_weight<2>, dim3(1), dim3(kRedThreads), 0, c.stream, c.partials, nb,
-                       c.projW, slot);
+    proj_run(false, c.projCount, c, proj_list(c, false), proj_list(c, true),
+             ProjArgs{u, c.q64, y, z, 0.0, false, 0});
+    hipLaunchKernelGGL(k_proj_weight<2>, dim3(1), dim3(kRedThreads), 0, s, c.partials, nb, c.projW, slot);
   } else {
-    // first vector: y = u
-    SC_HIP(hipMemcpyAsync(y, u, (size_t)c.N * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-    hipLaunchKernelGGL(k_multidot<1>, dim3(nb), dim3(kRedThreads), 0, c.stream, (size_t)c.N, rhs,
-                       (const double*)c.r64, VecList{{u}}, c.partials);
-    hipLaunchKernelGGL(k_proj_weight<1>, dim3(1), dim3(kRedThreads), 0, c.stream, c.partials, nb,
-                       c.projW, slot);
+    SC_HIP(hipMemcpyAsync(y, u, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    SC_HIP(hipMemcpyAsync(z, c.q64, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_multidot<1>, dim3(nb), dim3(kRedThreads), 0, s, N, (const double*)c.q64,
+                       (const double*)nullptr, VecList{{u}}, c.partials);
+    hipLaunchKernelGGL(k_proj_weight<1>, dim3(1), dim3(kRedThreads), 0, s, c.partials, nb, c.projW, slot);
   }
   SC_HIP(hipGetLastError());
   c.projNext = (slot + 1) % c.projCap;
@@ -651,10 +786,6 @@ int solve(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve
 int solve_core(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st) {
   std::memset(&st, 0, sizeof(st));
   const bool ir = o.mode == SC_SOLVE_IR32;
-  if (o.mode != SC_SOLVE_CG64 && !ir) return fail(SC_ERR_ARG, "unknown solve mode");
-  if (!(o.rtol > 0)) return fail(SC_ERR_ARG, "rtol must be > 0");
-  if (o.precond < SC_PREC_JACOBI || o.precond > SC_PREC_FDIAG_BF16)
-    return fail(SC_ERR_ARG, "unknown preconditioner");
   int e = ensure_workspace(c, ir);
   if (e) return e;
   if (o.precond != SC_PREC_JACOBI) {

@@ -15,38 +15,7 @@ namespace sc {
 __global__ __launch_bounds__(256) void k_fault_qd(sc_fault_qd_args a) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.N) return;
-  const double tq = a.tauQS_d[k] + 1.0 * (a.prestress_d ? a.prestress_d[k] : 0.0);
-  a.tauQS_d[k] = tq;
-  const double psi = a.psi_d[k], aa = a.a_d[k], sN = a.sN_d[k], eta = a.eta_d[k];
-  const double lk = a.locked_d ? a.locked_d[k] : 0.0;
-  double V = a.V_d[k];
-  int bad = 0;
-  if (lk > 0.5) {
-    V = 0.;
-  } else if (lk < -0.5) {
-    V = a.vL;
-  } else {
-    const double left = 0., right = tq / eta;
-    if (isnan(right)) {
-      bad = SC_ERR_NONFINITE;
-    } else if (fabs(left - right) < 1e-14) {
-      V = left;
-    } else {
-      QdResid r{psi, aa, sN, eta, tq, a.v0, a.jac_mode};
-      double out;
-      int its;
-      if (bracketed_newton(r, left, right, V, a.rootTol, a.maxNumIts, &out, &its)) bad = SC_ERR_NOCONV;
-      V = out;
-    }
-  }
-  a.V_d[k] = V;
-  double dpsi = a.stateLaw == SC_AGING_LAW ? aging_law_psi(psi, V, a.b_d[k], a.f0, a.v0, a.Dc_d[k])
-                                           : slip_law_psi(psi, V, aa, a.b_d[k], a.f0, a.v0, a.Dc_d[k]);
-  if (!isfinite(dpsi)) bad = SC_ERR_NONFINITE;
-  a.dpsi_d[k] = dpsi;
-  a.tau_d[k] = -1.0 * (eta * V) + tq;
-  a.strength_d[k] = strength_psi(sN, psi, V, aa, a.v0);
-  if (bad && a.status_d) atomicOr(a.status_d, bad);
+  fault_qd_node(a, k, a.tauQS_d[k]);
 }
 
 int fault_qd_rates(hipStream_t s, const sc_fault_qd_args& a) {
@@ -164,21 +133,6 @@ __global__ void k_qd_bc(int Nz, const double* __restrict__ slip, const double* _
   if (remote) bcR[k] = vL * t / fts + 1.0 * shift[k];
 }
 
-// out = base + sum_i c[i]*f[i]   (VecWAXPY followed by VecAXPYs, odeSolver.cpp)
-struct LinComb {
-  const double* f[6];
-  double c[6];
-  int nt;
-};
-__global__ void k_lincomb(int n, double* __restrict__ out, const double* __restrict__ base,
-                          LinComb lc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double v = lc.c[0] * lc.f[0][i] + base[i];
-  for (int t = 1; t < lc.nt; ++t) v = v + lc.c[t] * lc.f[t][i];
-  out[i] = v;
-}
-
 // Error measures over one vector pair (lo, hi): [sum (lo-hi)^2, sum hi^2, max |lo-hi|/hi]
 __global__ __launch_bounds__(256) void k_err(int n, const double* __restrict__ lo,
                                              const double* __restrict__ hi, double* __restrict__ out) {
@@ -213,19 +167,6 @@ int launch_qd_bc(hipStream_t s, int Nz, const double* slip, const double* shift,
                  double vL, double fts, int remote, double* bcL, double* bcR) {
   hipLaunchKernelGGL(k_qd_bc, dim3((Nz + 255) / 256), dim3(256), 0, s, Nz, slip, shift, t, vL, fts,
                      remote, bcL, bcR);
-  SC_HIP(hipGetLastError());
-  return SC_OK;
-}
-
-int launch_lincomb(hipStream_t s, int n, double* out, const double* base, int nt,
-                   const double* const* f, const double* c) {
-  LinComb lc{};
-  lc.nt = nt;
-  for (int i = 0; i < nt; ++i) {
-    lc.f[i] = f[i];
-    lc.c[i] = c[i];
-  }
-  hipLaunchKernelGGL(k_lincomb, dim3((n + 255) / 256), dim3(256), 0, s, n, out, base, lc);
   SC_HIP(hipGetLastError());
   return SC_OK;
 }

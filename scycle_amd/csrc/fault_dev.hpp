@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "scycle_hip.h"
+
 namespace sc {
 
 // a*sN*asinh(V/(2 v0) * exp(psi/a))  (fault.cpp:2246-2250)
@@ -73,15 +75,57 @@ __device__ int bracketed_newton(const R& resid, double left, double right, doubl
 struct QdResid {
   double psi, a, sN, eta, tauQS, v0;
   int jac_mode;
+  double ex;  // exp(psi / a), fixed over the root solve
+  __device__ QdResid(double psi_, double a_, double sN_, double eta_, double tauQS_, double v0_, int jac)
+      : psi(psi_), a(a_), sN(sN_), eta(eta_), tauQS(tauQS_), v0(v0_), jac_mode(jac), ex(exp(psi_ / a_)) {}
   __device__ void operator()(double V, double* f, double* J) const {
-    const double strength = strength_psi(sN, psi, V, a, v0);
+    const double strength = a * sN * asinh((0.5 * V / v0) * ex);  // strength_psi
     const double stress = tauQS - eta * V;
     *f = strength - stress;
     const double A = a * sN;
-    const double B = exp(psi / a) / (2. * v0);
+    const double B = ex / (2. * v0);
     const double den = sqrt(B * B * V * V + 1.);
     *J = (jac_mode == 0 ? A * V / den : A * B / den) + eta;
   }
 };
+
+// Fault_qd::d_dt at fault node k (fault.cpp:861-918) from the quasi-static
+// shear stress tq0 (before the prestress): slip velocity from the root solve,
+// state rate, shear stress and strength; returns V.
+__device__ __forceinline__ double fault_qd_node(const sc_fault_qd_args& a, int k, double tq0) {
+  const double tq = tq0 + 1.0 * (a.prestress_d ? a.prestress_d[k] : 0.0);
+  a.tauQS_d[k] = tq;
+  const double psi = a.psi_d[k], aa = a.a_d[k], sN = a.sN_d[k], eta = a.eta_d[k];
+  const double lk = a.locked_d ? a.locked_d[k] : 0.0;
+  double V = a.V_d[k];
+  int bad = 0;
+  if (lk > 0.5) {
+    V = 0.;
+  } else if (lk < -0.5) {
+    V = a.vL;
+  } else {
+    const double left = 0., right = tq / eta;
+    if (isnan(right)) {
+      bad = SC_ERR_NONFINITE;
+    } else if (fabs(left - right) < 1e-14) {
+      V = left;
+    } else {
+      QdResid r{psi, aa, sN, eta, tq, a.v0, a.jac_mode};
+      double out;
+      int its;
+      if (bracketed_newton(r, left, right, V, a.rootTol, a.maxNumIts, &out, &its)) bad = SC_ERR_NOCONV;
+      V = out;
+    }
+  }
+  a.V_d[k] = V;
+  double dpsi = a.stateLaw == SC_AGING_LAW ? aging_law_psi(psi, V, a.b_d[k], a.f0, a.v0, a.Dc_d[k])
+                                           : slip_law_psi(psi, V, aa, a.b_d[k], a.f0, a.v0, a.Dc_d[k]);
+  if (!isfinite(dpsi)) bad = SC_ERR_NONFINITE;
+  a.dpsi_d[k] = dpsi;
+  a.tau_d[k] = -1.0 * (eta * V) + tq;
+  a.strength_d[k] = strength_psi(sN, psi, V, aa, a.v0);
+  if (bad && a.status_d) atomicOr(a.status_d, bad);
+  return V;
+}
 
 }  // namespace sc

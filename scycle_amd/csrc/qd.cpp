@@ -4,9 +4,12 @@
 // with its LinearElastic and Fault_qd members; integrate() is the host loop of
 // OdeSolver RK32/RK43 (odeSolver.cpp:505-627, 973-1175) with every vector on
 // the device.  Per d_dt: boundary data -> setRhs -> solve (warm start from the
-// previous displacement, as KSPSetInitialGuessNonzero) -> sigma_xy on the
-// fault line -> rate-and-state kernel.  The host reads back one scalar block
-// per RK attempt (the error estimate) and the CG convergence flags.
+// previous displacement, as KSPSetInitialGuessNonzero, or from the projection
+// onto earlier solutions) -> sigma_xy on the fault line -> rate-and-state
+// kernel.  Inside integrate() a solve whose projected guess is accepted on
+// the device needs no host round trip: the host reads back one block per RK
+// attempt (error sums, fault status, missed-guess flag) and repeats the
+// attempt with checked solves in the rare case that a guess missed.
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -21,8 +24,6 @@ namespace sc {
 
 int launch_qd_bc(hipStream_t s, int Nz, const double* slip, const double* shift, double t,
                  double vL, double fts, int remote, double* bcL, double* bcR);
-int launch_lincomb(hipStream_t s, int n, double* out, const double* base, int nt,
-                   const double* const* f, const double* c);
 int launch_err(hipStream_t s, int n, const double* lo, const double* hi, double* out3,
                double* partials);
 
@@ -78,12 +79,12 @@ int QdDriver::init(const sc_qd_desc& d, const double* mu, const double* y, const
   tau_d = dalloc(Nz, nullptr);
   strength_d = dalloc(Nz, nullptr);
   V_d = dalloc(Nz, nullptr);
-  err_d = dalloc(16, nullptr);
-  int* st = nullptr;
-  if (hipMalloc(&st, sizeof(int)) != hipSuccess) return fail(SC_ERR_HIP, "hipMalloc status");
-  owned.push_back(st);
-  (void)hipMemset(st, 0, sizeof(int));
-  status_d = st;
+  // one read-back block: 12 error sums, then the fault status and the
+  // missed-guess flag
+  err_d = dalloc(kChkDoubles, nullptr);
+  if (!err_d) return fail(SC_ERR_HIP, "device allocation failed");
+  status_d = (int*)(err_d + kErrSlots);
+  miss_d = status_d + 1;
   for (const char* key : {"psi", "slip"})
     if ((e = add_var(key, Nz))) return e;
   err_part_d = dalloc((size_t)3 * kErrBlocks, nullptr);
@@ -144,25 +145,39 @@ int QdDriver::d_dt(double t, const double* psi, const double* slip, double* dpsi
   hipStream_t s = ctx->c.stream;
   const double t0 = now();
   ++counters.d_dt_calls;
-  int e = launch_qd_bc(s, Nz, slip, shift_d, t, desc.vL, desc.faultTypeScale, desc.remoteLoadingR,
-                       bcL_d, bcR_d);
-  if (!e) e = launch_set_rhs(ctx->c, rhs_d, bcL_d, bcR_d, bcT_d, bcB_d, s);
-  if (!e && pl) e = launch_pl_source(ctx->c, gxy, gxz, rhs_d, s);
+  int e;
+  int mnb = 0;
+  if (!pl) {
+    // the rhs is zero off the boundary frame (rhs_d starts zeroed and only
+    // this kernel writes it); with a solution basis the kernel also forms
+    // the projections that start the solve
+    ProjFramePlan plan;
+    e = proj_frame_plan(ctx->c, desc.solve, &plan);
+    if (!e)
+      e = launch_qd_rhs_frame(ctx->c, rhs_d, slip, shift_d, t, desc.vL, desc.faultTypeScale,
+                              desc.remoteLoadingR, bcL_d, bcR_d, bcT_d, bcB_d, s, &plan, &mnb);
+  } else {
+    e = launch_qd_bc(s, Nz, slip, shift_d, t, desc.vL, desc.faultTypeScale, desc.remoteLoadingR, bcL_d,
+                     bcR_d);
+    if (!e) e = launch_set_rhs(ctx->c, rhs_d, bcL_d, bcR_d, bcT_d, bcB_d, s);
+    if (!e) e = launch_pl_source(ctx->c, gxy, gxz, rhs_d, s);
+  }
   if (e) return e;
   const double ts = now();
   sc_solve_stats st;
-  e = solve(ctx->c, rhs_d, u_d, desc.solve, st);
+  bool deferred = false;
+  e = solve(ctx->c, rhs_d, u_d, desc.solve, st, defer ? miss_d : nullptr, &deferred, !pl, mnb);
+  counters.deferred_checks += deferred ? 1 : 0;
   counters.solve_seconds += now() - ts;
   counters.solves += 1;
   counters.cg_iters += st.iters;
   counters.outer_sweeps += st.outer;
   if (e) return e;
-  if (pl)
+  if (pl) {
     e = launch_pl_strain(ctx->c, u_d, gxy, gxz, pl->T, pl->A, pl->B, pl->n, bcL_d, bcR_d, pl->cap,
                          dgxy, dgxz, tauQS_d, pl->effVisc, pl->sxy, s);
-  else
-    e = launch_fault_sxy(ctx->c, u_d, tauQS_d, s);
-  if (e) return e;
+    if (e) return e;
+  }
   sc_fault_qd_args fa{};
   fa.N = Nz;
   fa.v0 = desc.v0;
@@ -186,9 +201,9 @@ int QdDriver::d_dt(double t, const double* psi, const double* slip, double* dpsi
   fa.tau_d = tau_d;
   fa.strength_d = strength_d;
   fa.status_d = status_d;
-  e = fault_qd_rates(s, fa);
+  // sigma_xy on the fault from u (the power law's strain kernel wrote it already)
+  e = launch_qd_fault(ctx->c, pl ? nullptr : u_d, fa, dslip, s);
   if (e) return e;
-  SC_HIP(hipMemcpyAsync(dslip, V_d, Nz * sizeof(double), hipMemcpyDeviceToDevice, s));
   counters.total_seconds += now() - t0;
   return SC_OK;
 }
@@ -223,48 +238,113 @@ double QdDriver::step_size(const RkState& r, double totErr) const {
   return dt;
 }
 
-// computeError (odeSolver.cpp:449-501, 902-969) over errInds (timeIntInds):
-// psi and slip by default, the viscous strains when selected
-int QdDriver::error(const RkState& r, double* totErr) {
+// y_lo, y_hi of the embedded pair and computeError (odeSolver.cpp:449-501,
+// 902-969) over errInds (timeIntInds): psi and slip by default, the viscous
+// strains when selected.  Short fields take one fused kernel, long ones
+// (the viscous strains) a combination and a block-partial error pass.  The
+// same read-back brings the fault status and the missed-guess flag: *missed
+// says that a deferred solve since the last check missed its tolerance (both
+// flags are then cleared and the error is meaningless).
+int QdDriver::lohi_error(const RkState& r, int ntlo, const int* filo, const double* clo, int nthi,
+                         const int* fihi, const double* chi, double* totErr, bool* missed) {
   hipStream_t s = ctx->c.stream;
   const char* names[4] = {"psi", "slip", "gVxy", "gVxz"};
-  int sel[4], ns = 0;
-  for (int i = 0; i < 4; ++i)
-    if ((errMask & (1 << i)) && vars.count(names[i])) sel[ns++] = i;
-  for (int q = 0; q < ns; ++q) {
-    VecSet& v = vars[names[sel[q]]];
-    int e = launch_err(s, v.n, v.lo, v.hi, err_d + 3 * q, err_part_d);
-    if (e) return e;
+  int slot[4], ns = 0;  // error slot of each field, -1 when not selected
+  for (int i = 0; i < 4; ++i) slot[i] = ((errMask & (1 << i)) && vars.count(names[i])) ? ns++ : -1;
+  VarsLoHi S;
+  S.ntlo = ntlo;
+  S.nthi = nthi;
+  for (int i = 0; i < ntlo; ++i) S.clo[i] = clo[i];
+  for (int i = 0; i < nthi; ++i) S.chi[i] = chi[i];
+  VarsLinComb Llo, Lhi;
+  Llo.nt = ntlo;
+  Lhi.nt = nthi;
+  for (int i = 0; i < ntlo; ++i) Llo.c[i] = clo[i];
+  for (int i = 0; i < nthi; ++i) Lhi.c[i] = chi[i];
+  std::vector<int> longSel;
+  for (int fi = 0; fi < 4; ++fi) {
+    if (!vars.count(names[fi])) continue;
+    VecSet& v = vars[names[fi]];
+    auto fsel = [&](int id) { return id == 0 ? (const double*)v.dvar : (const double*)v.f[id]; };
+    if (v.n <= kShortField) {
+      const int q = S.nv++;
+      S.n[q] = v.n;
+      S.base[q] = v.var;
+      S.lo[q] = v.lo;
+      S.hi[q] = v.hi;
+      for (int i = 0; i < ntlo; ++i) S.flo[q][i] = fsel(filo[i]);
+      for (int i = 0; i < nthi; ++i) S.fhi[q][i] = fsel(fihi[i]);
+      S.out[q] = slot[fi] >= 0 ? err_d + 3 * slot[fi] : err_d + 12;
+    } else {
+      for (VarsLinComb* L : {&Llo, &Lhi}) {
+        const int q = L->nv++;
+        L->n[q] = v.n;
+        L->out[q] = L == &Llo ? v.lo : v.hi;
+        L->base[q] = v.var;
+        const int nt = L == &Llo ? ntlo : nthi;
+        const int* fidx = L == &Llo ? filo : fihi;
+        for (int i = 0; i < nt; ++i) L->f[q][i] = fsel(fidx[i]);
+      }
+      if (slot[fi] >= 0) longSel.push_back(fi);
+    }
   }
-  SC_HIP(hipMemcpyAsync(host_scratch, err_d, 3 * ns * sizeof(double), hipMemcpyDeviceToHost, s));
+  int e = launch_vars_lohi_err(s, S);
+  if (!e && Llo.nv) e = launch_vars_lincomb(s, Llo);
+  if (!e && Lhi.nv) e = launch_vars_lincomb(s, Lhi);
+  for (int fi : longSel) {
+    if (e) break;
+    VecSet& v = vars[names[fi]];
+    e = launch_err(s, v.n, v.lo, v.hi, err_d + 3 * slot[fi], err_part_d);
+  }
+  if (e) return e;
+  SC_HIP(hipMemcpyAsync(host_scratch, err_d, kChkDoubles * sizeof(double), hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
   const double* h = (const double*)host_scratch;
+  const int status = ((const int*)(h + kErrSlots))[0];
+  const int miss = ((const int*)(h + kErrSlots))[1];
+  *missed = miss != 0;
+  if (miss) {
+    counters.missed_guesses++;
+    SC_HIP(hipMemsetAsync(status_d, 0, 2 * sizeof(int), s));
+    *totErr = 0;
+    return SC_OK;
+  }
+  if (status) {
+    (void)hipMemsetAsync(status_d, 0, sizeof(int), s);
+    return fail(status & SC_ERR_NONFINITE ? SC_ERR_NONFINITE : SC_ERR_NOCONV,
+                "fault root solve failed (status " + std::to_string(status) + ")");
+  }
   double tot = 0;
-  for (int q = 0; q < ns; ++q) {
-    const double* v = h + 3 * q;
-    const double n = (double)vars[names[sel[q]]].n;
+  for (int fi = 0; fi < 4; ++fi) {
+    if (slot[fi] < 0) continue;
+    const double* v = h + 3 * slot[fi];
+    const double n = (double)vars[names[fi]].n;
     if (r.normType == 0) tot += std::sqrt(v[0]) / (std::sqrt(n) * 1.0);
     else if (r.ord == 3) tot += std::sqrt(v[0]) / (std::sqrt(v[1]) * 1.0);
     else tot += v[2] / 1.0;
   }
   *totErr = tot;
-  return check_status();
+  return SC_OK;
 }
 
+// out = var + sum_i c[i] f_i over every integrated field, one launch
 int QdDriver::lincomb_all(const char* out_sel, int stage, int nt, const int* fidx, const double* c) {
-  hipStream_t s = ctx->c.stream;
+  VarsLinComb L;
+  L.nt = nt;
+  for (int i = 0; i < nt; ++i) L.c[i] = c[i];
   for (auto& kv : vars) {
     VecSet& v = kv.second;
     double* out = nullptr;
     if (!std::strcmp(out_sel, "k")) out = v.k[stage];
     else if (!std::strcmp(out_sel, "lo")) out = v.lo;
     else out = v.hi;
-    const double* f[6];
-    for (int i = 0; i < nt; ++i) f[i] = fidx[i] == 0 ? v.dvar : v.f[fidx[i]];
-    int e = launch_lincomb(s, v.n, out, v.var, nt, f, c);
-    if (e) return e;
+    L.n[L.nv] = v.n;
+    L.out[L.nv] = out;
+    L.base[L.nv] = v.var;
+    for (int i = 0; i < nt; ++i) L.f[L.nv][i] = fidx[i] == 0 ? v.dvar : v.f[fidx[i]];
+    ++L.nv;
   }
-  return SC_OK;
+  return launch_vars_lincomb(ctx->c.stream, L);
 }
 
 int QdDriver::stage(int st, double t) {
@@ -298,6 +378,14 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
     return SC_OK;
   }
   if (r.deltaT == 0) r.deltaT = (finalT - r.currT) / maxSteps;
+  // solves with a projected guess are checked on the device; the flag is read
+  // with the error estimate of each attempt
+  const bool may_defer = desc.solve.reuse > 0;
+  struct DeferGuard {
+    bool& d;
+    ~DeferGuard() { d = false; }
+  } guard{defer};
+  defer = false;
   e = d_dt_vars(r.currT);
   if (e) return e;
   double totErr = 0, newDt = r.deltaT;
@@ -305,20 +393,22 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
   while (r.stepCount < maxSteps && r.currT < finalT) {
     r.stepCount++;
     int attempt = 0;
+    bool checked = false;  // this step's solves are checked on the host
     while (attempt < 100) {
       attempt++;
+      defer = may_defer && !checked;
       if (replay) r.deltaT = run.replayDt_h[r.stepCount - 1];
       if (r.currT + r.deltaT > finalT) r.deltaT = finalT - r.currT;
       const double h = r.deltaT;
+      bool missed = false;
       if (r.ord == 3) {
         { const int fi[1] = {0}; const double c[1] = {0.5 * h}; if ((e = lincomb_all("k", 1, 1, fi, c))) return e; }
         if ((e = stage(1, r.currT + 0.5 * h))) return e;
         { const int fi[2] = {0, 1}; const double c[2] = {-h, 2 * h}; if ((e = lincomb_all("k", 2, 2, fi, c))) return e; }
         if ((e = stage(2, r.currT + h))) return e;
-        { const int fi[2] = {0, 2}; const double c[2] = {0.5 * h, 0.5 * h}; if ((e = lincomb_all("lo", 0, 2, fi, c))) return e; }
-        { const int fi[3] = {0, 1, 2}; const double c[3] = {h / 6.0, 2 * h / 3.0, h / 6.0}; if ((e = lincomb_all("hi", 0, 3, fi, c))) return e; }
-        if ((e = error(r, &totErr))) return e;
-        if (replay || totErr <= r.tol) break;
+        const int flo[2] = {0, 2}, fhi[3] = {0, 1, 2};
+        const double clo[2] = {0.5 * h, 0.5 * h}, chi[3] = {h / 6.0, 2 * h / 3.0, h / 6.0};
+        if ((e = lohi_error(r, 2, flo, clo, 3, fhi, chi, &totErr, &missed))) return e;
       } else {
         const double c2 = 1. / 2., c3 = 83. / 250., c4 = 31. / 50., c5 = 17. / 20., c6 = 1.;
         const double b1 = 82889. / 524892., b3 = 15625. / 83664., b4 = 69875. / 102672.,
@@ -344,19 +434,29 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
         if ((e = stage(5, r.currT + c5 * h))) return e;
         { const int fi[5] = {0, 2, 3, 4, 5}; const double c[5] = {a61 * h, a62 * h, a63 * h, a64 * h, a65 * h}; if ((e = lincomb_all("k", 6, 5, fi, c))) return e; }
         if ((e = stage(6, r.currT + c6 * h))) return e;
-        { const int fi[5] = {0, 3, 4, 5, 6}; const double c[5] = {hb1 * h, hb3 * h, hb4 * h, hb5 * h, hb6 * h}; if ((e = lincomb_all("lo", 0, 5, fi, c))) return e; }
-        { const int fi[5] = {0, 3, 4, 5, 6}; const double c[5] = {b1 * h, b3 * h, b4 * h, b5 * h, b6 * h}; if ((e = lincomb_all("hi", 0, 5, fi, c))) return e; }
-        if ((e = error(r, &totErr))) return e;
-        if (replay || totErr < r.tol) break;
+        const int fi[5] = {0, 3, 4, 5, 6};
+        const double clo[5] = {hb1 * h, hb3 * h, hb4 * h, hb5 * h, hb6 * h};
+        const double chi[5] = {b1 * h, b3 * h, b4 * h, b5 * h, b6 * h};
+        if ((e = lohi_error(r, 5, fi, clo, 5, fi, chi, &totErr, &missed))) return e;
       }
+      if (missed) {
+        // a projected guess since the last check (this attempt's stages or the
+        // rates at the step start) missed: redo both with checked solves
+        defer = false;
+        checked = true;
+        if ((e = d_dt_vars(r.currT))) return e;
+        attempt--;
+        continue;
+      }
+      if (replay || (r.ord == 3 ? totErr <= r.tol : totErr < r.tol)) break;
       r.deltaT = step_size(r, totErr);
       counters.rejected_steps++;
       if (r.minDt == r.deltaT) break;
     }
     r.currT = r.currT + r.deltaT;
-    for (auto& kv : vars)
-      SC_HIP(hipMemcpyAsync(kv.second.var, kv.second.hi, (size_t)kv.second.n * sizeof(double),
-                            hipMemcpyDeviceToDevice, s));
+    // accept: var <- hi (the buffers trade places)
+    for (auto& kv : vars) std::swap(kv.second.var, kv.second.hi);
+    defer = may_defer;
     if ((e = d_dt_vars(r.currT))) return e;
     if (totErr != 0.0) newDt = step_size(r, totErr);
     r.errA[1] = r.errA[0];
@@ -364,7 +464,16 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
     if (dts && nOut < dtsCap) dts[nOut++] = r.deltaT;
     r.deltaT = newDt;
   }
+  defer = false;
+  // the rates at the final state: check their guesses too
+  SC_HIP(hipMemcpyAsync(host_scratch, status_d, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
+  if (((const int*)host_scratch)[1]) {
+    counters.missed_guesses++;
+    SC_HIP(hipMemsetAsync(status_d, 0, 2 * sizeof(int), s));
+    if ((e = d_dt_vars(r.currT))) return e;
+    SC_HIP(hipStreamSynchronize(s));
+  }
   if ((e = check_status())) return e;
   *tEnd = r.currT;
   *steps = r.stepCount;

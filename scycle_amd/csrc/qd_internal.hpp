@@ -21,6 +21,37 @@ struct VecSet {
   double* hi = nullptr;
 };
 
+// Kernel arguments of the fused field kernels (qd_kernels.hip)
+constexpr int kMaxVars = 4;
+struct VarsLinComb {
+  int nv = 0, nt = 0;
+  int n[kMaxVars] = {};
+  double* out[kMaxVars] = {};
+  const double* base[kMaxVars] = {};
+  const double* f[kMaxVars][6] = {};
+  double c[6] = {};
+};
+struct VarsLoHi {
+  int nv = 0, ntlo = 0, nthi = 0;
+  int n[kMaxVars] = {};
+  const double* base[kMaxVars] = {};
+  double* lo[kMaxVars] = {};
+  double* hi[kMaxVars] = {};
+  const double* flo[kMaxVars][6] = {};
+  const double* fhi[kMaxVars][6] = {};
+  double clo[6] = {}, chi[6] = {};
+  double* out[kMaxVars] = {};
+};
+int launch_vars_lohi_err(hipStream_t s, const VarsLoHi& L);
+// plan (optional, from proj_frame_plan): also write the projections of the
+// rhs on the basis; *mnb gets the number of partial blocks (0 without)
+int launch_qd_rhs_frame(const Ctx& c, double* rhs, const double* slip, const double* shift, double t,
+                        double vL, double fts, int remote, double* bcL, double* bcR, const double* bcT,
+                        const double* bcB, hipStream_t s, const ProjFramePlan* plan = nullptr,
+                        int* mnb = nullptr);
+int launch_qd_fault(const Ctx& c, const double* u, const sc_fault_qd_args& a, double* dslip, hipStream_t s);
+int launch_vars_lincomb(hipStream_t s, const VarsLinComb& L);
+
 struct RkState {
   int ord = 3, control = 1, normType = 0, stepCount = 0;
   double tol = 1e-8, kappa = 0.9, minDt = 0, maxDt = 1e30, deltaT = 0, currT = 0;
@@ -44,10 +75,15 @@ struct QdDriver {
   double *bcL_d = nullptr, *bcR_d = nullptr, *bcT_d = nullptr, *bcB_d = nullptr;
   double *rhs_d = nullptr, *u_d = nullptr;
   double *tauQS_d = nullptr, *tau_d = nullptr, *strength_d = nullptr, *V_d = nullptr;
-  double* err_d = nullptr;              // 3 error sums per selected field
+  static constexpr int kErrSlots = 16;  // 3 error sums per selected field (<= 4), then 3 scratch
+  static constexpr int kChkDoubles = 17;  // + fault status and missed-guess flag (2 ints)
+  static constexpr int kShortField = 16384;  // fields up to this length: one block each
+  double* err_d = nullptr;              // the read-back block
   double* err_part_d = nullptr;         // block partials of the error sums
   static constexpr int kErrBlocks = 512;
-  int* status_d = nullptr;
+  int* status_d = nullptr;             // inside the read-back block
+  int* miss_d = nullptr;               // set by a deferred guess check that missed
+  bool defer = false;                  // solves may defer their check (integrate only)
   void* host_scratch = nullptr;
   std::map<std::string, VecSet> vars;
   int errMask = 3;                      // RK error fields: 1 psi, 2 slip, 4 gVxy, 8 gVxz
@@ -72,7 +108,9 @@ struct QdDriver {
   int d_dt_vars(double t);              // on var -> dvar of every integrated variable
   int check_status();
   double step_size(const RkState& r, double totErr) const;
-  int error(const RkState& r, double* totErr);
+  // lo and hi of the embedded pair (lo = var + sum clo f, hi likewise) and the error
+  int lohi_error(const RkState& r, int ntlo, const int* filo, const double* clo, int nthi,
+                 const int* fihi, const double* chi, double* totErr, bool* missed);
   int lincomb_all(const char* out_sel, int stage, int nt, const int* fidx, const double* c);
   int stage(int st, double t);
   int integrate(const sc_qd_run& run, double* tEnd, int* steps, double* dts, int dtsCap);

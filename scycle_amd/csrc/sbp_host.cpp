@@ -455,6 +455,8 @@ extern "C" int sc_ctx_destroy(sc_ctx* h) {
     if (p) (void)hipFree(p);
   for (double* p : c.projY)
     if (p) (void)hipFree(p);
+  for (double* p : c.projZ)
+    if (p) (void)hipFree(p);
   if (c.projW) (void)hipFree(c.projW);
   if (c.scal_h) (void)hipHostFree(c.scal_h);
   if (c.stream) (void)hipStreamDestroy(c.stream);

@@ -16,7 +16,7 @@
 // boundary terms and evaluates the boundary closures from the 1-D tables
 // (wave-uniform along the boundary normal).  Grids with a side of at most
 // 2*kFrame nodes use a per-node generic kernel instead.
-#include "sc_internal.hpp"
+#include "sbp_dev.hpp"
 
 namespace sc {
 
@@ -146,17 +146,6 @@ __device__ __forceinline__ T line_z(const DevOp<T>& p, const U& u, int j, int k)
   return d.H[k] * acc1 - (accR + acc4);
 }
 
-// D1 row i of a direction applied along that direction.
-template <typename T, class F>
-__device__ __forceinline__ T d1_row(const DevDir<T>& d, int i, const F& f) {
-  T s = 0;
-  for (int q = 0; q < kBand; ++q) {
-    const T w = d.D1[i * kBand + q];
-    if (w != T(0)) s += w * f(i + q - 4);
-  }
-  return s;
-}
-
 // SAT terms at (j, k) (sbpOps_m_varGrid.cpp:601-772 constructBC_*).
 template <typename T, class U>
 __device__ __forceinline__ T sat_terms(const DevOp<T>& p, const U& u, int j, int k) {
@@ -258,35 +247,6 @@ struct WaveUpdate {
 
 // Frame node t -> (j, k).  Frame = rows [0,F) U [Ny-F,Ny) (all k) and, for
 // the middle rows, columns [0,F) U [Nz-F,Nz).
-struct FrameMap {
-  int Ny, Nz, nTop, nBot, nL, nR, nMid;
-  __host__ __device__ static FrameMap make(int Ny, int Nz) {
-    FrameMap f;
-    f.Ny = Ny;
-    f.Nz = Nz;
-    f.nTop = Ny < kFrame ? Ny : kFrame;
-    f.nBot = Ny > kFrame ? (Ny - kFrame < kFrame ? Ny - kFrame : kFrame) : 0;
-    f.nMid = Ny - f.nTop - f.nBot;
-    f.nL = Nz < kFrame ? Nz : kFrame;
-    f.nR = Nz > kFrame ? (Nz - kFrame < kFrame ? Nz - kFrame : kFrame) : 0;
-    return f;
-  }
-  __host__ __device__ long long count() const {
-    return (long long)(nTop + nBot) * Nz + (long long)nMid * (nL + nR);
-  }
-  __device__ void node(long long t, int& j, int& k) const {
-    const long long a = (long long)nTop * Nz;
-    if (t < a) { j = (int)(t / Nz); k = (int)(t % Nz); return; }
-    t -= a;
-    const long long b = (long long)nBot * Nz;
-    if (t < b) { j = Ny - nBot + (int)(t / Nz); k = (int)(t % Nz); return; }
-    t -= b;
-    const int w = nL + nR;
-    j = nTop + (int)(t / w);
-    const int c = (int)(t % w);
-    k = c < nL ? c : Nz - nR + (c - nL);
-  }
-};
 
 template <typename T, class EP>
 __global__ __launch_bounds__(256) void k_apply_frame(DevOp<T> p, FrameMap fm,
@@ -781,65 +741,46 @@ int launch_diag(const Ctx& c, double* out, hipStream_t s) {
 }
 
 // --------------------------------------------------------------- setRhs ---
+
+// whole vector: row j = blockIdx.y
 __global__ __launch_bounds__(256) void k_set_rhs(DevOp<double> p, double* __restrict__ rhs,
                                                  const double* __restrict__ bcL,
                                                  const double* __restrict__ bcR,
                                                  const double* __restrict__ bcT,
                                                  const double* __restrict__ bcB) {
-  const size_t I = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int Ny = p.Ny, Nz = p.Nz;
-  if (I >= (size_t)Ny * Nz) return;
-  const int j = (int)(I / Nz), k = (int)(I % Nz);
-  double v = 0;
-  if (p.doY) {
-    const DevDir<double>& d = p.y;
-    const double pre = p.z.H[k] * p.z.met[k];
-    if (j < kBand) {
-      if (p.bcL == SC_DIRICHLET) {
-        const double bs = d.BS0[j] + (j == 0 ? d.alphaD : 0.0);
-        if (bs != 0.0) v += pre * (bs * (p.mu[k] * d.imet[0])) * bcL[k];
-      } else if (j == 0) {
-        v += pre * bcL[k];
-      }
-    }
-    if (j >= Ny - kBand) {
-      if (p.bcR == SC_DIRICHLET) {
-        const double bs = d.BSN[j - (Ny - kBand)] + (j == Ny - 1 ? d.alphaD : 0.0);
-        if (bs != 0.0) v += pre * (bs * (p.mu[(size_t)(Ny - 1) * Nz + k] * d.imet[Ny - 1])) * bcR[k];
-      } else if (j == Ny - 1) {
-        v -= pre * bcR[k];
-      }
-    }
-  }
-  if (p.doZ) {
-    const DevDir<double>& d = p.z;
-    const double pre = p.y.H[j] * p.y.met[j];
-    const size_t row = (size_t)j * Nz;
-    if (k < kBand) {
-      if (p.bcT == SC_DIRICHLET) {
-        const double bs = d.BS0[k] + (k == 0 ? d.alphaD : 0.0);
-        if (bs != 0.0) v += pre * (bs * (p.mu[row] * d.imet[0])) * bcT[j];
-      } else if (k == 0) {
-        v += pre * bcT[j];
-      }
-    }
-    if (k >= Nz - kBand) {
-      if (p.bcB == SC_DIRICHLET) {
-        const double bs = d.BSN[k - (Nz - kBand)] + (k == Nz - 1 ? d.alphaD : 0.0);
-        if (bs != 0.0) v += pre * (bs * (p.mu[row + Nz - 1] * d.imet[Nz - 1])) * bcB[j];
-      } else if (k == Nz - 1) {
-        v -= pre * bcB[j];
-      }
-    }
-  }
-  rhs[I] = v;
+  const int j = blockIdx.y, k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= p.Nz) return;
+  rhs[(size_t)j * p.Nz + k] = rhs_node(p, j, k, [&](int side, int i) {
+    return side == 0 ? bcL[i] : side == 1 ? bcR[i] : side == 2 ? bcT[i] : bcB[i];
+  });
+}
+
+// the boundary frame only (the rest of rhs is known to be zero)
+__global__ __launch_bounds__(256) void k_set_rhs_frame(DevOp<double> p, FrameMap fm, double* __restrict__ rhs,
+                                                       const double* __restrict__ bcL,
+                                                       const double* __restrict__ bcR,
+                                                       const double* __restrict__ bcT,
+                                                       const double* __restrict__ bcB) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= fm.count()) return;
+  int j, k;
+  fm.node(t, j, k);
+  rhs[(size_t)j * p.Nz + k] = rhs_node(p, j, k, [&](int side, int i) {
+    return side == 0 ? bcL[i] : side == 1 ? bcR[i] : side == 2 ? bcT[i] : bcB[i];
+  });
 }
 
 int launch_set_rhs(const Ctx& c, double* rhs, const double* bcL, const double* bcR,
-                   const double* bcT, const double* bcB, hipStream_t s) {
-  const size_t N = (size_t)c.N;
-  hipLaunchKernelGGL(k_set_rhs, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, c.op64, rhs,
-                     bcL, bcR, bcT, bcB);
+                   const double* bcT, const double* bcB, hipStream_t s, bool frame_only) {
+  if (frame_only) {
+    const FrameMap fm = FrameMap::make(c.d.Ny, c.d.Nz);
+    const long long n = fm.count();
+    hipLaunchKernelGGL(k_set_rhs_frame, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c.op64, fm,
+                       rhs, bcL, bcR, bcT, bcB);
+  } else {
+    hipLaunchKernelGGL(k_set_rhs, dim3((unsigned)((c.d.Nz + 255) / 256), (unsigned)c.d.Ny), dim3(256), 0,
+                       s, c.op64, rhs, bcL, bcR, bcT, bcB);
+  }
   SC_HIP(hipGetLastError());
   return SC_OK;
 }
@@ -885,8 +826,7 @@ __global__ __launch_bounds__(256) void k_fault_sxy(DevOp<double> p, const double
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int Nz = p.Nz;
   if (k >= Nz) return;
-  const double d = d1_row(p.y, 0, [&](int m) { return u[(size_t)m * Nz + k]; });
-  out[k] = p.mu[k] * (p.y.imet[0] * d);
+  out[k] = p.mu[k] * dy_row0(p, u, k);
 }
 
 int launch_fault_sxy(const Ctx& c, const double* u, double* out, hipStream_t s) {

@@ -109,7 +109,8 @@ struct Ctx {
   double* partials = nullptr;          // reduction scratch
   // previous solutions kept for the initial guess (ring buffer of A-orthogonal
   // vectors y_i with weights w_i = 1 / (y_i^T (-A) y_i) on the device)
-  double* projY[SC_REUSE_MAX] = {};
+  double* projY[SC_REUSE_MAX] = {};    // A-orthogonal basis of earlier solutions
+  double* projZ[SC_REUSE_MAX] = {};    // A y for each basis vector
   double* projW = nullptr;             // w[SC_REUSE_MAX], coef[SC_REUSE_MAX]
   int projCap = 0, projCount = 0, projNext = 0;
   void* scal = nullptr;                // CgScalars (device)
@@ -139,14 +140,37 @@ struct WaveArgs {
 };
 int launch_wave64(const Ctx& c, const WaveArgs<double>& a, hipStream_t s);
 int launch_wave32(const Ctx& c, const WaveArgs<float>& a, hipStream_t s);
+// frame_only: write only the nodes within kFrame of a boundary, for a
+// caller whose rhs is zero elsewhere
 int launch_set_rhs(const Ctx& c, double* rhs, const double* bcL, const double* bcR,
-                   const double* bcT, const double* bcB, hipStream_t s);
+                   const double* bcT, const double* bcB, hipStream_t s, bool frame_only = false);
 // which: 0 Dy, 1 muxDy, 2 Dz, 3 muxDz, 4 H, 5 Hinv
 int launch_deriv(const Ctx& c, int which, const double* in, double* out, hipStream_t s);
 // sxy on the fault line j = 0 only: out[k] = mu(0,k)*qy[0]*(D1y u)(0,k)
 int launch_fault_sxy(const Ctx& c, const double* u, double* out, hipStream_t s);
 
-int solve(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st);
+// miss_flag/deferred: see cg.hip (deferred residual check of a projected
+// guess); rhs_frame: rhs is zero off the nodes within kFrame of a boundary;
+// pre_mnb > 0: the caller already wrote the projections y_m . rhs as pre_mnb
+// block partials (see proj_frame_plan)
+int solve(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st,
+          int* miss_flag = nullptr, bool* deferred = nullptr, bool rhs_frame = false, int pre_mnb = 0);
+
+// For a caller that forms a frame-supported rhs and its projections on the
+// basis in one kernel: M basis vectors Y (M = 0: no projected guess for this
+// solve), where to write the block partials (M per block, block order) and
+// the most blocks that may write them.
+// Up to SC_REUSE_MAX device vectors (kernel argument)
+struct VecList {
+  const double* v[SC_REUSE_MAX];
+};
+struct ProjFramePlan {
+  int M = 0;
+  VecList Y{};
+  double* mpart = nullptr;
+  int maxBlocks = 0;
+};
+int proj_frame_plan(Ctx& c, const sc_solve_opts& o, ProjFramePlan* plan);
 // power law: rhs += B gxy + C gxz; strains -> stresses -> viscosity -> rates
 int launch_pl_source(const Ctx& c, const double* gxy, const double* gxz, double* rhs, hipStream_t s);
 int launch_pl_strain(const Ctx& c, const double* u, const double* gVxy, const double* gVxz,

@@ -196,3 +196,44 @@ def test_qd_integrate_parity(integrator, solve):
     e2 = _fields_err(g2.get_state(), g2.get_u(), ref)
     print(integrator, solve, "free", e2, flush=True)
     assert max(e2.values()) <= 1e-7, e2
+
+
+@pytest.mark.parametrize("reuse", [1, 4])
+@pytest.mark.parametrize("integrator", ["RK32", "RK43"])
+def test_qd_integrate_projected_guess(integrator, reuse):
+    """The projected initial guess inside sc_qd_integrate: its residual check
+    runs on the device and is read back with each attempt's error estimate;
+    an attempt in which any guess missed the tolerance is repeated with
+    checked solves.  reuse 1 (the guess is a multiple of the last solution)
+    misses often and exercises the repeat; reuse 4 mostly hits.  In replay
+    mode the fields match the oracle's direct solves to 1e-9 either way."""
+    from scycle_amd import QdMediator
+    N = 25
+    cfg = _small_cfg(N, N, 4, True)
+    steps = 40
+    h, keep, _ = _oracle_qd(cfg)
+    r = _Run(0 if integrator == "RK32" else 1, 1, 0, steps, 0.0, 1e15, 1e-3, 0.0, 1e10, 1e-8, 0,
+             None)
+    ref = dict(psi=cfg.psi0.copy(), slip=cfg.slip0.copy(), V=np.zeros(N), tau=np.zeros(N),
+               u=np.zeros(N * N))
+    tE = C.c_double()
+    odts = np.zeros(steps)
+    ns = olib().orc_qd_integrate(h, C.byref(r), optr(ref["psi"]), optr(ref["slip"]),
+                                 optr(ref["V"]), optr(ref["tau"]), optr(ref["u"]), C.byref(tE),
+                                 optr(odts), steps, None)
+    olib().orc_qd_free(h)
+    assert ns == steps
+    g = QdMediator(cfg, solve="cg64", rtol=1e-13, jac_mode=0, precond="fdiag64", reuse=reuse)
+    g.set_state(cfg.psi0, cfg.slip0, np.zeros(N))
+    tEnd, nst, dts = g.integrate(steps, integrator=integrator, tol=1e-8, init_dt=1e-3,
+                                 max_dt=1e10, replay_dt=odts)
+    c = g.counters()
+    assert nst == steps and abs(tEnd - tE.value) <= 1e-14 * tE.value
+    e = _fields_err(g.get_state(), g.get_u(), ref)
+    print(integrator, "reuse", reuse, "replay", e, "deferred", c["deferred_checks"], "missed",
+          c["missed_guesses"], "solves", c["solves"], "iters", c["cg_iters"], flush=True)
+    assert max(e.values()) <= 1e-9, e
+    assert c["deferred_checks"] > 0
+    if reuse == 1:
+        assert c["missed_guesses"] > 0
+    g.close()

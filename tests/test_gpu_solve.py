@@ -116,6 +116,18 @@ def test_reuse_projection_guess(mode, precond):
         its.append(st["iters"])
     print(mode, precond, "iterations", its, flush=True)
     assert its[3] == 0
+    # two more solutions replace the two oldest basis vectors (capacity 3);
+    # the guesses from the mixed basis still lead to the direct solution
+    for _ in range(2):
+        rhss.append(o.set_rhs(rng.standard_normal(Nz), rng.standard_normal(Nz), np.zeros(Ny), np.zeros(Ny)))
+        u, st = g.solve(rhss[-1], mode=mode, rtol=1e-13, precond=precond, reuse=3, max_iters=20000)
+        assert st["converged"] and st["iters"] > 0
+    for b in (0.4 * rhss[2] + 1.1 * rhss[4] - 0.6 * rhss[5], 0.5 * rhss[0] + 0.3 * rhss[5], rhss[5]):
+        u, st = g.solve(b, mode=mode, rtol=1e-10, precond=precond, reuse=3, max_iters=20000)
+        uref = o.chol_solve(b)
+        assert st["converged"] and st["relres"] <= 1e-10
+        assert np.abs(u - uref).max() <= 1e-8 * np.abs(uref).max()
+        print(mode, precond, "after eviction, iterations", st["iters"], flush=True)
     # the basis belongs to the operator: after a BC type change it starts afresh
     g.changeBCTypes(0, 1, 1, 1)
     o2 = OracleSbp(4, Ny, Nz, 30.0, 30.0, mu, y, z, var_grid=True, bc=(0, 1, 1, 1))
