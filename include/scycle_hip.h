@@ -290,6 +290,11 @@ typedef struct sc_qd_run {
   double initTime, maxTime, initDt, minDt, maxDt, tol;
   int nReplay;                /* >0: force the step sizes in replayDt       */
   const double* replayDt_h;
+  /* With sc_solve_opts.reuse > 0, when a projected initial guess may be
+   * accepted on the device and checked with the next error estimate:
+   * 0 after a run of guesses that met the tolerance, 1 never (every solve
+   * is checked on the host), 2 always. */
+  int deferChecks;
 } sc_qd_run;
 
 typedef struct sc_qd_counters {
@@ -355,6 +360,45 @@ int sc_qd_set_visc(sc_qd* q, const double* gVxy_h, const double* gVxz_h);
 /* state of the last d_dt: strains, their rates, effective viscosity, sxy */
 int sc_qd_get_visc(sc_qd* q, double* gVxy_h, double* gVxz_h, double* dgVxy_h, double* dgVxz_h,
                    double* effVisc_h, double* sxy_h);
+
+/* ------------------------------------- discrete adjoint of the QD step --
+ * No counterpart in the reference: the gradient of a surface-displacement
+ * misfit with respect to the fault's friction parameters a, b, Dc (per fault
+ * node), through nSteps RK32 steps of the linear-elastic, aging-law QD
+ * mediator with the step sizes held fixed (as sc_qd_run.replayDt).
+ *   J = 1/2 sum_s (u(j_s, k = 0, t_N) - d_s)^2,
+ * u the off-fault displacement of the last d_dt (at the final state and
+ * time).  The forward starts from the mediator's current psi, slip, V and
+ * both calls leave them there.  The reverse pass recomputes the forward from
+ * checkpoints taken every ckptEvery steps; the elasticity solves of the
+ * adjoint use the same symmetric operator apply (options adjSolve, their own
+ * projection basis), the root solve is differentiated through the implicit
+ * function theorem. */
+typedef struct sc_adj_desc {
+  int nSteps;
+  const double* dts_h;        /* nSteps step sizes                          */
+  double t0;                  /* time of the initial state                  */
+  int ckptEvery;              /* steps between checkpoints (>= 1)           */
+  int nStations;
+  const int* stationJ_h;      /* station y-indices j_s (node j_s*Nz, z = 0) */
+  const double* data_h;       /* d_s (NULL: zero)                           */
+  sc_solve_opts adjSolve;
+} sc_adj_desc;
+
+typedef struct sc_adj_stats {
+  double J;
+  double forward_seconds;     /* checkpointed forward                        */
+  double reverse_seconds;     /* recomputation + adjoint steps               */
+  long long fwd_solves, adj_solves, adj_cg_iters, recomputed_steps;
+} sc_adj_stats;
+
+/* Replace the friction parameters a, b, Dc (host arrays, Nz; NULL keeps one). */
+int sc_qd_set_friction(sc_qd* q, const double* a_h, const double* b_h, const double* Dc_h);
+/* Forward only: J and the station displacements u(j_s, 0, t_N) (may be NULL). */
+int sc_qd_forward_misfit(sc_qd* q, const sc_adj_desc* d, double* J, double* uStations_h);
+/* J and dJ/da, dJ/db, dJ/dDc per fault node (host arrays, Nz each). */
+int sc_qd_adjoint(sc_qd* q, const sc_adj_desc* d, double* ga_h, double* gb_h, double* gDc_h,
+                  sc_adj_stats* stats);
 
 /* --------------------------------------------- host-only input utilities */
 /* Value of `key` in a .in deck, cut at its first space like every

@@ -81,7 +81,8 @@ class QdRun(C.Structure):
     _fields_ = [("integrator", C.c_int), ("control", C.c_int), ("normType", C.c_int),
                 ("maxSteps", C.c_int), ("initTime", C.c_double), ("maxTime", C.c_double),
                 ("initDt", C.c_double), ("minDt", C.c_double), ("maxDt", C.c_double),
-                ("tol", C.c_double), ("nReplay", C.c_int), ("replayDt_h", _dp)]
+                ("tol", C.c_double), ("nReplay", C.c_int), ("replayDt_h", _dp),
+                ("deferChecks", C.c_int)]
 
 
 class QdCounters(C.Structure):
@@ -90,6 +91,19 @@ class QdCounters(C.Structure):
                 ("rejected_steps", C.c_longlong), ("solve_seconds", C.c_double),
                 ("total_seconds", C.c_double), ("deferred_checks", C.c_longlong),
                 ("missed_guesses", C.c_longlong)]
+
+
+class AdjDesc(C.Structure):
+    _fields_ = [("nSteps", C.c_int), ("dts", C.c_void_p), ("t0", C.c_double),
+                ("ckptEvery", C.c_int), ("nStations", C.c_int), ("stationJ", C.c_void_p),
+                ("data", C.c_void_p), ("adjSolve", SolveOpts)]
+
+
+class AdjStats(C.Structure):
+    _fields_ = [("J", C.c_double), ("forward_seconds", C.c_double),
+                ("reverse_seconds", C.c_double), ("fwd_solves", C.c_longlong),
+                ("adj_solves", C.c_longlong), ("adj_cg_iters", C.c_longlong),
+                ("recomputed_steps", C.c_longlong)]
 
 
 # Every symbol include/scycle_hip.h declares, with (restype, argtypes).
@@ -126,6 +140,9 @@ SIGNATURES = {
     "sc_qd_create_powerlaw": (C.c_int, [C.POINTER(QdDesc)] + [_dp] * 15 + [C.c_double, C.c_int,
                                                                         C.POINTER(_vp)]),
     "sc_qd_set_visc": (C.c_int, [_vp, _dp, _dp]),
+    "sc_qd_set_friction": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "sc_qd_forward_misfit": (C.c_int, [_vp, C.POINTER(AdjDesc), _dp, _dp]),
+    "sc_qd_adjoint": (C.c_int, [_vp, C.POINTER(AdjDesc), _dp, _dp, _dp, C.POINTER(AdjStats)]),
     "sc_qd_get_visc": (C.c_int, [_vp] + [_dp] * 6),
     "sc_fd_create": (C.c_int, [C.POINTER(FdDesc)] + [_dp] * 13 + [C.POINTER(_vp)]),
     "sc_fd_destroy": (C.c_int, [_vp]),

@@ -232,3 +232,65 @@ def power_law(Ny, Nz, order=4, Ly=30.0, Lz=30.0, name="custom", steps=300):
 def config_c3(Ny=801, Nz=801):
     """801^2 power-law viscoelastic QD, order 4, constant grid, 300 steps."""
     return power_law(Ny, Nz, name=f"C3-{Ny}x{Nz}", steps=300)
+
+
+def steady_prestress(cfg):
+    """Set the prestress to the steady sliding stress at vL for the initial
+    state, tau0 = a sN asinh(vL/(2 v0) e^{psi0/a}) + eta vL, so that the
+    whole fault creeps at about vL from t = 0 instead of being loaded for
+    centuries first."""
+    cfg.prestress = (cfg.a * cfg.sN * np.arcsinh(cfg.vL / (2 * cfg.v0) * np.exp(cfg.psi0 / cfg.a))
+                     + cfg.eta * cfg.vL)
+    return cfg
+
+
+def config_c5(Ny=401, Nz=401):
+    """401^2, the C2 material and grid (order 4, sinh-stretched z, layered mu)
+    with the C1/C2 friction and a steady prestress; 500 RK32 steps for the
+    adjoint."""
+    return steady_prestress(strike_slip(Ny, Nz, order=4, var_grid=True, zstretch=5.0,
+                                        layered_mu=True, name=f"C5-{Ny}x{Nz}", steps=500))
+
+
+def hat_matrix(z, n_nodes, L):
+    """P[k, i] = hat function i of n_nodes equispaced nodes on [0, L] at z[k]:
+    a nodal field f = P theta is the piecewise-linear profile through theta."""
+    nodes = np.linspace(0.0, L, n_nodes)
+    P = np.zeros((len(z), n_nodes))
+    h = nodes[1] - nodes[0]
+    for i, zi in enumerate(nodes):
+        P[:, i] = np.clip(1.0 - np.abs(z - zi) / h, 0.0, None)
+    return P, nodes
+
+
+def splitmix_choice(seed, n, lo, hi):
+    """n distinct integers from [lo, hi) by a partial Fisher-Yates shuffle
+    driven by SplitMix64 uniforms."""
+    pool = list(range(lo, hi))
+    u = splitmix64(seed, n)
+    out = []
+    for i in range(n):
+        j = i + int(u[i] * (len(pool) - i))
+        pool[i], pool[j] = pool[j], pool[i]
+        out.append(pool[i])
+    return np.array(out, dtype=np.int32)
+
+
+@dataclass
+class AdjointProblem:
+    P: np.ndarray            # Nz x n_nodes hat functions
+    nodes: np.ndarray        # node depths (km)
+    theta: dict              # nodal profiles "a", "b", "Dc" (n_nodes each)
+    stations: np.ndarray     # station y-indices (surface nodes j * Nz)
+
+
+def adjoint_problem(cfg, n_nodes=64, n_stations=50, seed=0xAD70):
+    """C5's inverse problem on a QD config: a, b, Dc as n_nodes-node
+    piecewise-linear depth profiles (through the config's values at the
+    nodes) and n_stations surface stations drawn without replacement from
+    [1, Ny-1)."""
+    zf = cfg.z[:cfg.Nz]
+    P, nodes = hat_matrix(zf, n_nodes, cfg.Lz)
+    theta = {k: np.interp(nodes, zf, getattr(cfg, k)) for k in ("a", "b", "Dc")}
+    st = splitmix_choice(seed, n_stations, 1, cfg.Ny - 1)
+    return AdjointProblem(P=P, nodes=nodes, theta=theta, stations=st)

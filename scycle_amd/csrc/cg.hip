@@ -666,14 +666,14 @@ void proj_run(bool guess, int M, Ctx& c, const VecList& Y, const VecList& Z, con
 int proj_prepare(Ctx& c, int cap) {
   if (c.projCap == cap) return SC_OK;
   for (int i = 0; i < SC_REUSE_MAX; ++i) {
-    if (c.projY[i]) SC_HIP(hipFree(c.projY[i]));
-    if (c.projZ[i]) SC_HIP(hipFree(c.projZ[i]));
-    c.projY[i] = c.projZ[i] = nullptr;
+    for (double** v : {&c.projY[i], &c.projZ[i], &c.projU[i], &c.projZU[i]}) {
+      if (*v) SC_HIP(hipFree(*v));
+      *v = nullptr;
+    }
   }
-  for (int i = 0; i < cap; ++i) {
-    SC_HIP(hipMalloc(&c.projY[i], (size_t)c.N * sizeof(double)));
-    SC_HIP(hipMalloc(&c.projZ[i], (size_t)c.N * sizeof(double)));
-  }
+  for (int i = 0; i < cap; ++i)
+    for (double** v : {&c.projY[i], &c.projZ[i], &c.projU[i], &c.projZU[i]})
+      SC_HIP(hipMalloc(v, (size_t)c.N * sizeof(double)));
   if (!c.projW) SC_HIP(hipMalloc(&c.projW, 2 * SC_REUSE_MAX * sizeof(double)));
   c.projCap = cap;
   c.projCount = c.projNext = 0;
@@ -698,6 +698,56 @@ int check_opts(const sc_solve_opts& o) {
 }  // namespace
 
 int solve_core(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st);
+
+// Basis vector k from the raw pair (x, A x), A-orthogonal to basis vectors
+// 0..k-1, and its weight.
+int proj_basis_vector(Ctx& c, int k, const double* x, const double* ax) {
+  const size_t N = (size_t)c.N;
+  hipStream_t s = c.stream;
+  const int nb = (int)red_blocks(N);
+  if (k == 0) {
+    SC_HIP(hipMemcpyAsync(c.projY[0], x, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    SC_HIP(hipMemcpyAsync(c.projZ[0], ax, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_multidot<1>, dim3(nb), dim3(kRedThreads), 0, s, N, ax, (const double*)nullptr,
+                       VecList{{x}}, c.partials);
+    hipLaunchKernelGGL(k_proj_weight<1>, dim3(1), dim3(kRedThreads), 0, s, c.partials, nb, c.projW, 0);
+  } else {
+    VecList Y{}, Z{};
+    for (int i = 0; i < k; ++i) {
+      Y.v[i] = c.projY[i];
+      Z.v[i] = c.projZ[i];
+    }
+    proj_run(false, k, c, Y, Z, ProjArgs{x, const_cast<double*>(ax), c.projY[k], c.projZ[k], 0.0, false, 0});
+    hipLaunchKernelGGL(k_proj_weight<2>, dim3(1), dim3(kRedThreads), 0, s, c.partials, nb, c.projW, k);
+  }
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
+// Put u (and A u) into the ring of recent solutions.  While the ring fills,
+// u becomes the next basis vector; once it is full, u replaces the oldest
+// solution and the basis is rebuilt from the ring oldest first, so that its
+// span stays that of the last projCap solutions (replacing one basis vector
+// in place would drop the evicted solution's components from the others).
+int proj_add_solution(Ctx& c, const double* u) {
+  const size_t N = (size_t)c.N;
+  hipStream_t s = c.stream;
+  const int slot = c.projNext;
+  SC_HIP(hipMemcpyAsync(c.projU[slot], u, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+  int e = launch_apply64(c, u, c.projZU[slot], s);
+  if (e) return e;
+  c.projNext = (slot + 1) % c.projCap;
+  if (c.projCount < c.projCap) {
+    e = proj_basis_vector(c, c.projCount, c.projU[slot], c.projZU[slot]);
+    ++c.projCount;
+    return e;
+  }
+  for (int k = 0; k < c.projCap && !e; ++k) {
+    const int r = (c.projNext + k) % c.projCap;  // oldest first
+    e = proj_basis_vector(c, k, c.projU[r], c.projZU[r]);
+  }
+  return e;
+}
 
 // With o.reuse > 0 the solve starts from the A-orthogonal projection of the
 // new solution onto up to o.reuse earlier ones (Fischer's projection), with
@@ -759,28 +809,7 @@ int solve(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve
   }
   e = solve_core(c, rhs, u, o, st);
   if (e) return e;
-  // add the solution to the basis: q64 = A u
-  const int slot = c.projNext;
-  double* y = c.projY[slot];
-  double* z = c.projZ[slot];
-  const int nb = (int)red_blocks(N);
-  e = launch_apply64(c, u, c.q64, s);
-  if (e) return e;
-  if (c.projCount > 0) {
-    proj_run(false, c.projCount, c, proj_list(c, false), proj_list(c, true),
-             ProjArgs{u, c.q64, y, z, 0.0, false, 0});
-    hipLaunchKernelGGL(k_proj_weight<2>, dim3(1), dim3(kRedThreads), 0, s, c.partials, nb, c.projW, slot);
-  } else {
-    SC_HIP(hipMemcpyAsync(y, u, N * sizeof(double), hipMemcpyDeviceToDevice, s));
-    SC_HIP(hipMemcpyAsync(z, c.q64, N * sizeof(double), hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(k_multidot<1>, dim3(nb), dim3(kRedThreads), 0, s, N, (const double*)c.q64,
-                       (const double*)nullptr, VecList{{u}}, c.partials);
-    hipLaunchKernelGGL(k_proj_weight<1>, dim3(1), dim3(kRedThreads), 0, s, c.partials, nb, c.projW, slot);
-  }
-  SC_HIP(hipGetLastError());
-  c.projNext = (slot + 1) % c.projCap;
-  if (c.projCount < c.projCap) ++c.projCount;
-  return SC_OK;
+  return proj_add_solution(c, u);
 }
 
 int solve_core(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st) {

@@ -113,6 +113,10 @@ __device__ __forceinline__ double fault_qd_node(const sc_fault_qd_args& a, int k
       QdResid r{psi, aa, sN, eta, tq, a.v0, a.jac_mode};
       double out;
       int its;
+      // the previous V starts the search; a non-finite one (left by rates of
+      // a discarded attempt) would stall Newton, so such a start moves to the
+      // middle of the bracket
+      if (!isfinite(V)) V = 0.5 * (left + right);
       if (bracketed_newton(r, left, right, V, a.rootTol, a.maxNumIts, &out, &its)) bad = SC_ERR_NOCONV;
       V = out;
     }

@@ -61,6 +61,9 @@ int QdDriver::init(const sc_qd_desc& d, const double* mu, const double* y, const
   Ny = d.grid.Ny;
   Nz = d.grid.Nz;
   N = Ny * Nz;
+  mu_host.assign(mu, mu + N);
+  if (y) y_host.assign(y, y + N);
+  if (z) z_host.assign(z, z + N);
   a_d = dalloc(Nz, a);
   b_d = dalloc(Nz, b);
   Dc_d = dalloc(Nz, Dc);
@@ -137,6 +140,15 @@ int QdDriver::d_dt_vars(double t) {
   return d_dt(t, P.var, S.var, P.dvar, S.dvar, X.var, Z.var, X.dvar, Z.dvar);
 }
 
+int QdDriver::read_flags(int* status, int* miss) {
+  hipStream_t s = ctx->c.stream;
+  SC_HIP(hipMemcpyAsync(host_scratch, status_d, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  SC_HIP(hipStreamSynchronize(s));
+  *status = ((const int*)host_scratch)[0];
+  *miss = ((const int*)host_scratch)[1];
+  return SC_OK;
+}
+
 // StrikeSlip_LinearElastic_qd::d_dt (strikeSlip_linearElastic_qd.cpp:756-798);
 // with the power law, StrikeSlip_PowerLaw_qd::d_dt (strikeSlip_powerLaw_qd.cpp
 // :633-686) and solveMomentumBalance (:787-823)
@@ -168,6 +180,8 @@ int QdDriver::d_dt(double t, const double* psi, const double* slip, double* dpsi
   bool deferred = false;
   e = solve(ctx->c, rhs_d, u_d, desc.solve, st, defer ? miss_d : nullptr, &deferred, !pl, mnb);
   counters.deferred_checks += deferred ? 1 : 0;
+  if (deferred) pendingDeferred++;
+  else if (!e) hits = (st.iters == 0 && desc.solve.reuse > 0) ? hits + 1 : 0;
   counters.solve_seconds += now() - ts;
   counters.solves += 1;
   counters.cg_iters += st.iters;
@@ -303,6 +317,7 @@ int QdDriver::lohi_error(const RkState& r, int ntlo, const int* filo, const doub
   const int status = ((const int*)(h + kErrSlots))[0];
   const int miss = ((const int*)(h + kErrSlots))[1];
   *missed = miss != 0;
+  note_checked(*missed);
   if (miss) {
     counters.missed_guesses++;
     SC_HIP(hipMemsetAsync(status_d, 0, 2 * sizeof(int), s));
@@ -380,7 +395,8 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
   if (r.deltaT == 0) r.deltaT = (finalT - r.currT) / maxSteps;
   // solves with a projected guess are checked on the device; the flag is read
   // with the error estimate of each attempt
-  const bool may_defer = desc.solve.reuse > 0;
+  const bool may_defer = desc.solve.reuse > 0 && run.deferChecks != 1;
+  const bool always = run.deferChecks == 2;
   struct DeferGuard {
     bool& d;
     ~DeferGuard() { d = false; }
@@ -396,7 +412,7 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
     bool checked = false;  // this step's solves are checked on the host
     while (attempt < 100) {
       attempt++;
-      defer = may_defer && !checked;
+      defer = may_defer && !checked && (always || defer_ok());
       if (replay) r.deltaT = run.replayDt_h[r.stepCount - 1];
       if (r.currT + r.deltaT > finalT) r.deltaT = finalT - r.currT;
       const double h = r.deltaT;
@@ -456,7 +472,7 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
     r.currT = r.currT + r.deltaT;
     // accept: var <- hi (the buffers trade places)
     for (auto& kv : vars) std::swap(kv.second.var, kv.second.hi);
-    defer = may_defer;
+    defer = may_defer && (always || defer_ok());
     if ((e = d_dt_vars(r.currT))) return e;
     if (totErr != 0.0) newDt = step_size(r, totErr);
     r.errA[1] = r.errA[0];
@@ -468,6 +484,7 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
   // the rates at the final state: check their guesses too
   SC_HIP(hipMemcpyAsync(host_scratch, status_d, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
+  note_checked(((const int*)host_scratch)[1] != 0);
   if (((const int*)host_scratch)[1]) {
     counters.missed_guesses++;
     SC_HIP(hipMemsetAsync(status_d, 0, 2 * sizeof(int), s));
@@ -620,6 +637,7 @@ extern "C" int sc_qd_integrate(sc_qd* h, const sc_qd_run* run, double* tEnd_h, i
   if (!h || !run) return fail(SC_ERR_ARG, "null argument");
   if (run->integrator != 0 && run->integrator != 1) return fail(SC_ERR_ARG, "integrator must be RK32 (0) or RK43 (1)");
   if (!(run->tol >= 1e-14)) return fail(SC_ERR_ARG, "timeStepTol must be >= 1e-14");
+  if (run->deferChecks < 0 || run->deferChecks > 2) return fail(SC_ERR_ARG, "deferChecks must be 0, 1 or 2");
   double tEnd = 0;
   int steps = 0;
   int e = h->q->integrate(*run, &tEnd, &steps, dts_h, dtsCap);

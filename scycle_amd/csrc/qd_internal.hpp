@@ -84,11 +84,23 @@ struct QdDriver {
   int* status_d = nullptr;             // inside the read-back block
   int* miss_d = nullptr;               // set by a deferred guess check that missed
   bool defer = false;                  // solves may defer their check (integrate only)
+  // Deferring pays only while the projected guesses keep meeting the
+  // tolerance: after a miss the driver checks every solve again until
+  // kDeferAfter guesses in a row have hit.
+  static constexpr int kDeferAfter = 8;
+  int hits = 0;                        // consecutive guesses that met the tolerance
+  int pendingDeferred = 0;             // deferred solves since the last read-back
+  bool defer_ok() const { return desc.solve.reuse > 0 && hits >= kDeferAfter; }
+  void note_checked(bool missed) {     // a read-back of the missed-guess flag
+    hits = missed ? 0 : hits + pendingDeferred;
+    pendingDeferred = 0;
+  }
   void* host_scratch = nullptr;
   std::map<std::string, VecSet> vars;
   int errMask = 3;                      // RK error fields: 1 psi, 2 slip, 4 gVxy, 8 gVxz
   std::unique_ptr<PowerLawState> pl;    // set for the power-law mediator
   sc_qd_counters counters{};
+  std::vector<double> mu_host, y_host, z_host;  // grid and modulus, for a second context
   sc_qd_run file_run{};
   bool have_file_run = false;
 
@@ -107,6 +119,8 @@ struct QdDriver {
            double* dgxz = nullptr);
   int d_dt_vars(double t);              // on var -> dvar of every integrated variable
   int check_status();
+  // fault status and missed-guess flag in one read-back (the flags are left set)
+  int read_flags(int* status, int* miss);
   double step_size(const RkState& r, double totErr) const;
   // lo and hi of the embedded pair (lo = var + sum clo f, hi likewise) and the error
   int lohi_error(const RkState& r, int ntlo, const int* filo, const double* clo, int nthi,

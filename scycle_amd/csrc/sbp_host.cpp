@@ -457,6 +457,10 @@ extern "C" int sc_ctx_destroy(sc_ctx* h) {
     if (p) (void)hipFree(p);
   for (double* p : c.projZ)
     if (p) (void)hipFree(p);
+  for (double* p : c.projU)
+    if (p) (void)hipFree(p);
+  for (double* p : c.projZU)
+    if (p) (void)hipFree(p);
   if (c.projW) (void)hipFree(c.projW);
   if (c.scal_h) (void)hipHostFree(c.scal_h);
   if (c.stream) (void)hipStreamDestroy(c.stream);

@@ -109,7 +109,9 @@ struct Ctx {
   double* partials = nullptr;          // reduction scratch
   // previous solutions kept for the initial guess (ring buffer of A-orthogonal
   // vectors y_i with weights w_i = 1 / (y_i^T (-A) y_i) on the device)
-  double* projY[SC_REUSE_MAX] = {};    // A-orthogonal basis of earlier solutions
+  double* projU[SC_REUSE_MAX] = {};    // ring of the last projCap solutions that iterated
+  double* projZU[SC_REUSE_MAX] = {};   // A u for each of them
+  double* projY[SC_REUSE_MAX] = {};    // A-orthogonal basis of the ring's span
   double* projZ[SC_REUSE_MAX] = {};    // A y for each basis vector
   double* projW = nullptr;             // w[SC_REUSE_MAX], coef[SC_REUSE_MAX]
   int projCap = 0, projCount = 0, projNext = 0;

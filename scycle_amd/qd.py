@@ -5,7 +5,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import GridDesc, QdCounters, QdDesc, QdRun, SolveOpts, check, dptr, load
+from ._lib import (AdjDesc, AdjStats, GridDesc, QdCounters, QdDesc, QdRun, SolveOpts, check, dptr,
+                   load)
 
 SOLVE_MODES = {"cg64": 0, "ir32": 1}
 
@@ -69,17 +70,56 @@ class QdMediator:
 
     def integrate(self, max_steps, integrator="RK32", tol=1e-8, init_dt=1e-3, min_dt=0.0,
                   max_dt=1e10, init_time=0.0, max_time=1e15, control="PID",
-                  norm="L2_absolute", replay_dt=None):
+                  norm="L2_absolute", replay_dt=None, defer="auto"):
+        """defer: "auto", "never" or "always" (sc_qd_run.deferChecks)."""
         rep = None if replay_dt is None else np.ascontiguousarray(replay_dt, dtype=np.float64)
         r = QdRun(0 if integrator == "RK32" else 1, 0 if control == "P" else 1,
                   0 if norm == "L2_absolute" else 1, int(max_steps), init_time, max_time,
-                  init_dt, min_dt, max_dt, tol, 0 if rep is None else len(rep), dptr(rep))
+                  init_dt, min_dt, max_dt, tol, 0 if rep is None else len(rep), dptr(rep),
+                  {"auto": 0, "never": 1, "always": 2}[defer])
         tEnd = C.c_double()
         steps = C.c_int()
         dts = np.zeros(max(int(max_steps), 1))
         check(load().sc_qd_integrate(self.h, C.byref(r), C.byref(tEnd), C.byref(steps),
                                      dptr(dts), len(dts)))
         return tEnd.value, steps.value, dts[:steps.value].copy()
+
+    # -------------------------------------------------- discrete adjoint --
+    def set_friction(self, a=None, b=None, Dc=None):
+        """Replace the fault's a, b, Dc (Nz each; None keeps one)."""
+        arrs = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (a, b, Dc)]
+        check(load().sc_qd_set_friction(self.h, *[dptr(x) for x in arrs]))
+
+    def _adj_desc(self, dts, t0, stations, data, ckpt_every, solve, rtol, precond, reuse):
+        from .sbp_ops import PRECONDS
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        js = np.ascontiguousarray(stations, dtype=np.int32)
+        dd = None if data is None else np.ascontiguousarray(data, dtype=np.float64)
+        so = SolveOpts(SOLVE_MODES[solve], rtol, 1e-5, 400000, 50, 64, PRECONDS[precond], reuse)
+        d = AdjDesc(len(dts), dts.ctypes.data, float(t0), int(ckpt_every), len(js), js.ctypes.data,
+                    None if dd is None else dd.ctypes.data, so)
+        return d, (dts, js, dd)
+
+    def forward_misfit(self, dts, stations, data=None, t0=0.0, ckpt_every=50):
+        """J = 1/2 sum_s (u(j_s, z=0, t_N) - d_s)^2 after len(dts) RK32 steps with
+        these step sizes from the current state (left unchanged); returns
+        (J, station displacements)."""
+        d, keep = self._adj_desc(dts, t0, stations, data, ckpt_every, "cg64", 1e-12, "jacobi", 0)
+        J = C.c_double()
+        us = np.zeros(max(len(keep[1]), 1))
+        check(load().sc_qd_forward_misfit(self.h, C.byref(d), C.byref(J), dptr(us)))
+        return J.value, us[:len(keep[1])]
+
+    def adjoint(self, dts, stations, data=None, t0=0.0, ckpt_every=50, solve="cg64",
+                rtol=1e-12, precond="fdiag64", reuse=4):
+        """dJ/da, dJ/db, dJ/dDc per fault node by the discrete adjoint; the
+        adjoint's elasticity solves use (solve, rtol, precond, reuse).  Returns
+        (ga, gb, gDc, stats)."""
+        d, keep = self._adj_desc(dts, t0, stations, data, ckpt_every, solve, rtol, precond, reuse)
+        g = [np.zeros(self.Nz) for _ in range(3)]
+        st = AdjStats()
+        check(load().sc_qd_adjoint(self.h, C.byref(d), *[dptr(x) for x in g], C.byref(st)))
+        return g[0], g[1], g[2], {k: getattr(st, k) for k, _ in AdjStats._fields_}
 
     def counters(self):
         c = QdCounters()

@@ -339,3 +339,44 @@ class PowerLawOracle:
             raise RuntimeError(f"orc_pl_integrate failed ({n})")
         return dict(psi=st[0], slip=st[1], gVxy=st[2], gVxz=st[3], V=Vv, t=tEnd.value, steps=n,
                     dts=dts[:n])
+
+
+class OracleQd:
+    """StrikeSlip_LinearElastic_qd of the oracle (direct Cholesky solves by
+    default) for a scycle_amd.configs.QdConfig, with a, b, Dc replaceable."""
+
+    def __init__(self, cfg, a=None, b=None, Dc=None, solver=0, jac_mode=1):
+        c = OrcQdCfg(cfg.order, cfg.Ny, cfg.Nz, int(cfg.var_grid), cfg.Ly, cfg.Lz, *cfg.mat_bc,
+                     int(cfg.remote_loading_R), cfg.fault_type_scale, cfg.vL, cfg.f0, cfg.v0,
+                     cfg.root_tol, cfg.state_law, jac_mode, solver, 1e-12)
+        fr = [cfg.a if a is None else a, cfg.b if b is None else b, cfg.Dc if Dc is None else Dc]
+        self._keep = [np.ascontiguousarray(x, dtype=np.float64) for x in
+                      (cfg.mu, cfg.y, cfg.z, *fr, cfg.sN, cfg.eta)]
+        self.N, self.Nz = cfg.Ny * cfg.Nz, cfg.Nz
+        self._pre = None if cfg.prestress is None else np.ascontiguousarray(cfg.prestress, dtype=np.float64)
+        self.h = lib().orc_qd_new(C.byref(c), *[ptr(x) for x in self._keep], None, ptr(self._pre), None)
+        if not self.h:
+            raise RuntimeError("orc_qd_new failed")
+
+    def __del__(self):
+        try:
+            lib().orc_qd_free(self.h)
+        except Exception:
+            pass
+
+    def replay(self, dts, psi, slip, V=None, t0=0.0):
+        """RK32 with the step sizes dts; returns the final psi, slip, V and the
+        displacement of the last d_dt (at the final state)."""
+        Nz = self.Nz
+        st = [np.array(x, dtype=np.float64) for x in (psi, slip)]
+        Vv = np.zeros(Nz) if V is None else np.array(V, dtype=np.float64)
+        rp = np.ascontiguousarray(dts, dtype=np.float64)
+        r = OrcQdRun(0, 1, 0, len(rp), t0, 1e30, rp[0], 0.0, 1e30, 1e-8, len(rp), ptr(rp))
+        tau, u = np.zeros(Nz), np.zeros(self.N)
+        tEnd = C.c_double()
+        out_dts = np.zeros(len(rp))
+        n = lib().orc_qd_integrate(self.h, C.byref(r), ptr(st[0]), ptr(st[1]), ptr(Vv), ptr(tau),
+                                   ptr(u), C.byref(tEnd), ptr(out_dts), len(out_dts), None)
+        if n != len(rp):
+            raise RuntimeError(f"orc_qd_integrate failed ({n})")
+        return dict(psi=st[0], slip=st[1], V=Vv, u=u, t=tEnd.value)

@@ -198,15 +198,16 @@ def test_qd_integrate_parity(integrator, solve):
     assert max(e2.values()) <= 1e-7, e2
 
 
-@pytest.mark.parametrize("reuse", [1, 4])
+@pytest.mark.parametrize("reuse,defer", [(1, "always"), (4, "always"), (4, "auto"), (4, "never")])
 @pytest.mark.parametrize("integrator", ["RK32", "RK43"])
-def test_qd_integrate_projected_guess(integrator, reuse):
+def test_qd_integrate_projected_guess(integrator, reuse, defer):
     """The projected initial guess inside sc_qd_integrate: its residual check
     runs on the device and is read back with each attempt's error estimate;
     an attempt in which any guess missed the tolerance is repeated with
-    checked solves.  reuse 1 (the guess is a multiple of the last solution)
-    misses often and exercises the repeat; reuse 4 mostly hits.  In replay
-    mode the fields match the oracle's direct solves to 1e-9 either way."""
+    checked solves.  defer "always" defers every solve with a basis (and so
+    meets misses and the repeat), "auto" only after a run of hits, "never"
+    checks every solve.  In replay mode the fields match the oracle's direct
+    solves to 1e-9 in every case."""
     from scycle_amd import QdMediator
     N = 25
     cfg = _small_cfg(N, N, 4, True)
@@ -226,14 +227,17 @@ def test_qd_integrate_projected_guess(integrator, reuse):
     g = QdMediator(cfg, solve="cg64", rtol=1e-13, jac_mode=0, precond="fdiag64", reuse=reuse)
     g.set_state(cfg.psi0, cfg.slip0, np.zeros(N))
     tEnd, nst, dts = g.integrate(steps, integrator=integrator, tol=1e-8, init_dt=1e-3,
-                                 max_dt=1e10, replay_dt=odts)
+                                 max_dt=1e10, replay_dt=odts, defer=defer)
     c = g.counters()
     assert nst == steps and abs(tEnd - tE.value) <= 1e-14 * tE.value
     e = _fields_err(g.get_state(), g.get_u(), ref)
-    print(integrator, "reuse", reuse, "replay", e, "deferred", c["deferred_checks"], "missed",
+    print(integrator, "reuse", reuse, defer, "replay", e, "deferred", c["deferred_checks"], "missed",
           c["missed_guesses"], "solves", c["solves"], "iters", c["cg_iters"], flush=True)
     assert max(e.values()) <= 1e-9, e
-    assert c["deferred_checks"] > 0
-    if reuse == 1:
+    if defer == "never":
+        assert c["deferred_checks"] == 0
+    else:
+        assert c["deferred_checks"] > 0
+    if defer == "always":
         assert c["missed_guesses"] > 0
     g.close()
