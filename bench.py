@@ -27,7 +27,9 @@ Also on the line: the HBM roofline of the FP64 SBP-SAT apply (interior and
 frame kernels, 24 algorithmic bytes per node: read u, read mu, write A u),
 timed with HIP events on the context stream; `traffic` from the committed
 rocprofv3 FETCH_SIZE / WRITE_SIZE summary when present; the fully dynamic
-2001^2 rupture (C4) in FP64 and FP32-bulk steps/s; and a CPU baseline (the
+2001^2 rupture (C4) in FP64 and FP32-bulk steps/s; the discrete adjoint of
+C5 (cost over one forward, gradient error against a central difference);
+and a CPU baseline (the
 FP64 oracle's Jacobi-PCG on the assembled operator, timed on a bounded sample
 and scaled by the FP64 Jacobi path's iterations per step).
 """
@@ -46,6 +48,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12  # B/s, MI355X
 SLIP_TOL = 1e-8
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "apply_fp64_1001x1001_traffic.json")
+GUESS_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "proj_guess_fp64_1001x1001_traffic.json")
 
 
 def log(*a):
@@ -89,6 +92,17 @@ def run_path(cfg, label, warmup, steps, reuse=0):
                    setup_s=setup, dt_min=float(dts.min()), dt_max=float(dts.max()), reuse=reuse)
 
 
+def roofline_guess(ctx, reps=200):
+    """The projected-guess kernel (k_proj_guess_res<M>) on the basis the QD
+    run left: it reads b, the M basis vectors and their M operator products
+    and writes x, 8 (2M + 2) bytes per node; mean of `reps` launches timed
+    with HIP events on the context stream."""
+    from scycle_amd._lib import check, load
+    ms, words = C.c_double(), C.c_double()
+    check(load().sc_bench_apply(ctx, 8, reps, C.byref(ms), C.byref(words)))
+    return dict(ms=ms.value, bytes=8.0 * words.value, GBps=8.0 * words.value / (ms.value * 1e-3) / 1e9)
+
+
 def roofline(ctx):
     """FP64 apply (frame + interior kernels): 24 algorithmic bytes per node,
     mean of 200 launches timed with HIP events on the context stream."""
@@ -103,14 +117,14 @@ def roofline(ctx):
     return out
 
 
-def traffic_per_launch():
-    """HBM bytes per FP64 apply from the committed rocprofv3 counter summary
-    (FETCH_SIZE + WRITE_SIZE of the two kernels, scaled by the copy-kernel
-    calibration recorded with it), or None."""
+def traffic_per_launch(path=TRAFFIC_FILE, key="hbm_bytes_per_apply"):
+    """HBM bytes per launch from a committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration recorded
+    with it), or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             d = json.load(f)
-        return float(d["hbm_bytes_per_apply"])
+        return float(d[key])
     except (OSError, ValueError, KeyError):
         return None
 
@@ -174,7 +188,77 @@ def fd_c4(n, steps):
     return out
 
 
-def kernels_only(n, reps):
+def adjoint_c5(n, steps, ckpt):
+    """C5: gradient of the surface misfit after `steps` RK32 steps at n^2
+    with respect to 64-node a, b, Dc depth profiles, by the discrete adjoint
+    (FP64 and mixed-precision adjoint solves), timed against one forward run
+    of the same steps; checked by a central difference (h = 1e-6 relative)
+    along a seeded direction."""
+    from scycle_amd import QdMediator
+    from scycle_amd.configs import adjoint_problem, config_c5
+    cfg = config_c5(n, n)
+    prob = adjoint_problem(cfg)
+    th = prob.theta
+    P = prob.P
+    m = QdMediator(cfg, solve="cg64", rtol=1e-12, precond="fdiag64", reuse=4, jac_mode=1)
+
+    def setf(t):
+        m.set_friction(P @ t["a"], P @ t["b"], P @ t["Dc"])
+
+    setf(th)
+    m.set_state(cfg.psi0, cfg.slip0, np.zeros(cfg.Nz))
+    _, k, dts = m.integrate(steps, tol=1e-8, init_dt=1e-3, max_dt=1e10)
+    m.set_state(cfg.psi0, cfg.slip0, np.zeros(cfg.Nz))
+    # data from a model with a 10% weaker velocity-weakening patch
+    bump = 1.0 - 0.1 * np.exp(-((prob.nodes - 0.45 * cfg.Lz) / (0.15 * cfg.Lz)) ** 2)
+    setf(dict(th, a=th["a"] * bump))
+    _, d = m.forward_misfit(dts, prob.stations)
+    setf(th)
+    m.forward_misfit(dts, prob.stations, d, ckpt_every=ckpt)  # warm-up
+    tic = time.perf_counter()
+    J, _ = m.forward_misfit(dts, prob.stations, d, ckpt_every=ckpt)
+    t_fwd = time.perf_counter() - tic
+    out = dict(workload=f"C5 QD {n}x{n} order-4 sinh-z(5) layered-mu aging-law steady prestress, "
+                        f"{steps} RK32 steps (fixed dt from an adaptive run, window "
+                        f"{dts.sum() / 3.156e7:.3f} yr), {len(prob.stations)} surface stations, "
+                        f"64-node a/b/Dc profiles, checkpoints every {ckpt} steps",
+               J=J, forward_seconds=t_fwd)
+    grads = {}
+    for label, pc, rtol in (("fp64", "fdiag64", 1e-12), ("mixed", "fdiag32", 1e-10)):
+        tic = time.perf_counter()
+        ga, gb, gdc, st = m.adjoint(dts, prob.stations, d, ckpt_every=ckpt, rtol=rtol, precond=pc, reuse=4)
+        sec = time.perf_counter() - tic
+        grads[label] = {q: P.T @ g for q, g in zip(("a", "b", "Dc"), (ga, gb, gdc))}
+        out[label] = dict(adjoint_seconds=sec, cost_over_forward=sec / t_fwd, adj_rtol=rtol, precond=pc,
+                          forward_part_s=st["forward_seconds"], reverse_part_s=st["reverse_seconds"],
+                          adj_solves=st["adj_solves"],
+                          adj_cg_iters_per_solve=st["adj_cg_iters"] / max(st["adj_solves"], 1),
+                          recomputed_steps=st["recomputed_steps"])
+    # central difference along a seeded direction, h = 1e-6 relative: the
+    # forward's 1e-12 solve tolerance leaves ~1e-10 relative noise on J, so
+    # the quotient itself carries ~1e-5 (much larger h is not usable: the
+    # replayed step sizes sit at the explicit scheme's stability limit, and
+    # a 1e-4 perturbation of the friction profiles can push a step past it)
+    rng = np.random.default_rng(7)
+    dirn = {q: np.abs(th[q]) * rng.standard_normal(len(th[q])) for q in th}
+    for eps in (1e-6,):
+        setf({q: th[q] + eps * dirn[q] for q in th})
+        Jp, _ = m.forward_misfit(dts, prob.stations, d, ckpt_every=ckpt)
+        setf({q: th[q] - eps * dirn[q] for q in th})
+        Jm, _ = m.forward_misfit(dts, prob.stations, d, ckpt_every=ckpt)
+        fd = (Jp - Jm) / (2 * eps)
+        for label in ("fp64", "mixed"):
+            ad = sum(float(grads[label][q] @ dirn[q]) for q in th)
+            out[label][f"directional_rel_err_vs_fd_h{eps:g}"] = abs(ad - fd) / abs(fd)
+    setf(th)
+    g64 = np.concatenate([grads["fp64"][q] for q in th])
+    gmx = np.concatenate([grads["mixed"][q] for q in th])
+    out["mixed_vs_fp64_grad_rel_err"] = float(np.abs(gmx - g64).max() / np.abs(g64).max())
+    m.close()
+    return out
+
+
+def kernels_only(n, reps, reuse=4):
     """Launches the FP64 apply `reps` times and an 8-byte-per-element device
     copy of the same vector length `reps` times (the calibration kernel for
     the FETCH_SIZE / WRITE_SIZE counters: it moves exactly 2 x 8 N bytes),
@@ -194,8 +278,13 @@ def kernels_only(n, reps):
     for _ in range(reps):
         y.copy_(x)
     torch.cuda.synchronize()
+    # the QD window of the benchmark on the fp64_fdiag path (so that the basis
+    # ends with the same number of vectors), then the guess kernel alone
+    q, _ = run_path(cfg, "fp64_fdiag", 10, 500, reuse=reuse)
+    gq = roofline_guess(load().sc_qd_ctx(q.h), reps)
+    q.close()
     print(json.dumps({"apply64_ms": ms.value, "nodes": nodes.value, "copy_bytes": 16 * N,
-                      "reps": reps}))
+                      "guess": gq, "reps": reps}))
 
 
 def main():
@@ -206,10 +295,14 @@ def main():
     ap.add_argument("--n", type=int, default=1001)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fd", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adj-n", type=int, default=401)
+    ap.add_argument("--adj-steps", type=int, default=500)
+    ap.add_argument("--adj-ckpt", type=int, default=50)
     ap.add_argument("--fd-n", type=int, default=2001)
     ap.add_argument("--fd-steps", type=int, default=2000)
     ap.add_argument("--paths", default="fp64_ref,fp64_fdiag,mixed,mixed_bf16,fp64")
-    ap.add_argument("--reuse", type=int, default=4,
+    ap.add_argument("--reuse", type=int, default=2,
                     help="previous solutions projected for the CG initial guess (FD paths)")
     ap.add_argument("--slow-steps", type=int, default=3,
                     help="timed steps for the Jacobi-preconditioned paths (same window start)")
@@ -222,14 +315,14 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.kernels_only:
-        kernels_only(args.n, 50)
+        kernels_only(args.n, 50, args.reuse)
         return
     from scycle_amd.configs import config_c2
     cfg = config_c2(args.n, args.n)
     res = {}
     labels = args.paths.split(",")
     ref = "fp64_ref" if "fp64_ref" in labels else labels[0]
-    ctx_q = None
+    mediators = {}
     for label in labels:
         jac = PATHS[label][1] == "jacobi"
         k = min(args.steps, args.slow_steps) if jac else args.steps
@@ -240,10 +333,7 @@ def main():
         log(f"[bench] {label}: {r['steps']} steps in {r['seconds']:.3f} s (setup {r['setup_s']:.1f} s), "
             f"counters {r['counters']}")
         res[label] = r
-        if label == ref:
-            ctx_q = q
-        else:
-            q.close()
+        mediators[label] = q
     # accuracy of each path against the FP64 reference path over the same window
     paths = {}
     for lab, rr in res.items():
@@ -278,21 +368,38 @@ def main():
         extra["mixed_over_fp64_same_preconditioner"] = (paths["mixed"]["steps_per_s"]
                                                         / paths["fp64_fdiag"]["steps_per_s"])
     from scycle_amd._lib import load
-    rl = roofline(load().sc_qd_ctx(ctx_q.h))
-    log(f"[bench] roofline {rl}")
+    # the dominant kernel of the timed step: the projected guess (the steps'
+    # solves mostly stop at it); the SBP apply alongside
+    gq = roofline_guess(load().sc_qd_ctx(mediators[best].h))
+    log(f"[bench] projected-guess kernel {gq}")
+    rl = roofline(load().sc_qd_ctx(mediators[best].h))
+    log(f"[bench] apply {rl}")
+    # the counter summary holds the kernel with the basis size this window ends with
+    gtraffic = traffic_per_launch(GUESS_TRAFFIC_FILE, "hbm_bytes_per_launch")
+    if traffic_per_launch(GUESS_TRAFFIC_FILE, "algorithmic_bytes_per_launch") != gq["bytes"]:
+        gtraffic = None
+    roof = dict(bound="hbm", achieved=gq["GBps"], peak=HBM_PEAK / 1e9, unit="GB/s",
+                frac=gq["GBps"] * 1e9 / HBM_PEAK, traffic=gtraffic,
+                kernel="k_proj_guess_res<M> (projected initial guess + residual of each QD solve)",
+                ms_per_launch=gq["ms"], algorithmic_bytes_per_launch=gq["bytes"])
     a64 = rl["apply64"]
-    traffic = traffic_per_launch()
-    roof = dict(bound="hbm", achieved=a64["GBps"], peak=HBM_PEAK / 1e9, unit="GB/s",
-                frac=a64["GBps"] * 1e9 / HBM_PEAK, traffic=traffic,
-                kernel="SBP-SAT apply FP64 (k_apply_frame_lds + k_apply_interior)",
-                ms_per_launch=a64["ms"], algorithmic_bytes_per_launch=a64["bytes"],
-                parts={k: v for k, v in rl.items() if k != "apply64"})
-    ctx_q.close()
+    roof_apply = dict(bound="hbm", achieved=a64["GBps"], peak=HBM_PEAK / 1e9, unit="GB/s",
+                      frac=a64["GBps"] * 1e9 / HBM_PEAK, traffic=traffic_per_launch(),
+                      kernel="SBP-SAT apply FP64 (k_apply_frame_lds + k_apply_interior)",
+                      ms_per_launch=a64["ms"], algorithmic_bytes_per_launch=a64["bytes"],
+                      parts={k: v for k, v in rl.items() if k != "apply64"})
+    for q in mediators.values():
+        q.close()
     fd = None
     if not args.no_fd:
         log(f"[bench] fully dynamic C4 {args.fd_n}^2, {args.fd_steps} steps")
         fd = fd_c4(args.fd_n, args.fd_steps)
         log(f"[bench] fd {fd}")
+    adj = None
+    if not args.no_adjoint:
+        log(f"[bench] adjoint C5 {args.adj_n}^2, {args.adj_steps} steps")
+        adj = adjoint_c5(args.adj_n, args.adj_steps, args.adj_ckpt)
+        log(f"[bench] adjoint {adj}")
     cpu = None
     if not args.no_cpu and "fp64" in res:
         ips = res["fp64"]["counters"]["cg_iters"] / res["fp64"]["steps"]
@@ -308,7 +415,8 @@ def main():
                                f"{args.warmup + args.steps} from t=0",
                    "solve": best, "precond": PATHS[best][1], "reuse": paths[best]["reuse"],
                    "rtol": 1e-12},
-        "roofline": roof, "cpu_baseline": cpu, "fd_c4": fd,
+        "roofline": roof, "roofline_apply": roof_apply, "cpu_baseline": cpu, "fd_c4": fd,
+        "adjoint_c5": adj,
     }
     line.update(extra)
     print(json.dumps(line))

@@ -106,7 +106,9 @@ int sc_sbp_hinv(sc_ctx* ctx, const double* in_d, double* out_d);
  * HIP events on the context stream, and the nodes they write.  which:
  * 0 FP64 apply, 1 FP32 apply, 2 FP64 interior kernel, 3 FP32 interior kernel,
  * 4 / 5 the interior kernel with the plain (not XCD-grouped) tile order,
- * 6 FP64 frame kernel, 7 FP32 frame kernel.
+ * 6 FP64 frame kernel, 7 FP32 frame kernel, 8 the projected-guess kernel of
+ * sc_solve on the context's current basis (needs a solve with reuse > 0;
+ * `nodes` then returns the 8-byte words it moves per launch).
  * No counterpart in the reference (its timers are MPI_Wtime accumulators,
  * e.g. linearElastic.cpp:474-477). */
 int sc_bench_apply(sc_ctx* ctx, int which, int reps, double* ms_per_launch, double* nodes);

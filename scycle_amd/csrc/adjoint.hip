@@ -219,6 +219,8 @@ AdjDriver::~AdjDriver() {
 
 int AdjDriver::setup(bool reverse) {
   Nz_ = q_.Nz;
+  // a run that ended in an error may have left the device flags set
+  SC_HIP(hipMemsetAsync(q_.status_d, 0, 2 * sizeof(int), q_.ctx->c.stream));
   const int C = d_.ckptEvery;
   nseg_ = (d_.nSteps + C - 1) / C;
   ckpt_ = dal((size_t)nseg_ * 3 * Nz_);

@@ -756,6 +756,48 @@ int proj_add_solution(Ctx& c, const double* u) {
 // With miss_flag set and a basis present the check is deferred: u gets the
 // guess, *miss_flag is set to 1 on the device if it misses the tolerance, and
 // *deferred tells the caller, who must test the flag before trusting u.
+// Timing of the projected-guess kernel alone on the current basis (rhs and
+// multidot partials as the last solve left them): mean ms per launch over
+// reps launches, HIP events on the context stream; *words = 8-byte words the
+// kernel moves per launch (b, M basis vectors, M products, x).
+int bench_proj_guess(Ctx& c, int reps, double* ms, double* words) {
+  if (c.projCount == 0) return fail(SC_ERR_ARG, "no projection basis: run a solve with reuse > 0 first");
+  const size_t N = (size_t)c.N;
+  hipStream_t s = c.stream;
+  const FrameMap fm = FrameMap::make(c.d.Ny, c.d.Nz);
+  const int mnb = (int)std::min<long long>((fm.count() + kRedThreads - 1) / kRedThreads, kMdBlocksMax);
+  const int M = c.projCount;
+  const VecList Y = proj_list(c, false), Z = proj_list(c, true);
+  auto launch = [&]() {
+    const dim3 g(red_blocks(N)), b(kRedThreads);
+    double* mp = c.partials + kMdOffset;
+    CgScal* sc = (CgScal*)c.scal;
+    switch (M) {
+#define SC_GUESS(m) \
+  case m: hipLaunchKernelGGL(k_proj_guess_res<m>, g, b, 0, s, N, Y, Z, (const double*)mp, mnb, \
+                             (const double*)c.projW, sc, 0.0, (const double*)c.r64, c.p64, c.partials); break;
+      SC_GUESS(1) SC_GUESS(2) SC_GUESS(3) SC_GUESS(4) SC_GUESS(5) SC_GUESS(6) SC_GUESS(7) SC_GUESS(8)
+#undef SC_GUESS
+    }
+  };
+  launch();  // warm-up
+  hipEvent_t e0, e1;
+  SC_HIP(hipEventCreate(&e0));
+  SC_HIP(hipEventCreate(&e1));
+  SC_HIP(hipEventRecord(e0, s));
+  for (int i = 0; i < reps; ++i) launch();
+  SC_HIP(hipEventRecord(e1, s));
+  SC_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  SC_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  SC_HIP(hipGetLastError());
+  *ms = t / reps;
+  *words = (double)N * (2.0 * M + 2.0);
+  return SC_OK;
+}
+
 int proj_frame_plan(Ctx& c, const sc_solve_opts& o, ProjFramePlan* plan) {
   plan->M = 0;
   int e = check_opts(o);

@@ -393,6 +393,8 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
     return SC_OK;
   }
   if (r.deltaT == 0) r.deltaT = (finalT - r.currT) / maxSteps;
+  // a run that ended in an error may have left the device flags set
+  SC_HIP(hipMemsetAsync(status_d, 0, 2 * sizeof(int), s));
   // solves with a projected guess are checked on the device; the flag is read
   // with the error estimate of each attempt
   const bool may_defer = desc.solve.reuse > 0 && run.deferChecks != 1;

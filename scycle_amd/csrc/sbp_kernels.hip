@@ -924,8 +924,14 @@ static void bench_frame(const Ctx& c, const DevOp<T>& op, const T* a, T* b) {
 // kernel only.
 extern "C" int sc_bench_apply(sc_ctx* h, int which, int reps, double* ms_per_launch,
                               double* nodes) {
-  if (!h || reps <= 0 || !ms_per_launch || which < 0 || which > 7)
+  if (!h || reps <= 0 || !ms_per_launch || which < 0 || which > 8)
     return fail(SC_ERR_ARG, "sc_bench_apply: bad argument");
+  if (which == 8) {
+    double words = 0;
+    int e = sc::bench_proj_guess(h->c, reps, ms_per_launch, &words);
+    if (nodes) *nodes = words;
+    return e;
+  }
   sc::Ctx& c = h->c;
   const bool f64 = which % 2 == 0;
   const size_t N = (size_t)c.N, esz = f64 ? 8 : 4;

@@ -173,6 +173,7 @@ struct ProjFramePlan {
   int maxBlocks = 0;
 };
 int proj_frame_plan(Ctx& c, const sc_solve_opts& o, ProjFramePlan* plan);
+int bench_proj_guess(Ctx& c, int reps, double* ms, double* words);
 // power law: rhs += B gxy + C gxz; strains -> stresses -> viscosity -> rates
 int launch_pl_source(const Ctx& c, const double* gxy, const double* gxz, double* rhs, hipStream_t s);
 int launch_pl_strain(const Ctx& c, const double* u, const double* gVxy, const double* gVxz,
