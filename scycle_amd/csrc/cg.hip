@@ -17,6 +17,7 @@
 #include <cstring>
 
 #include "fdiag.hpp"
+#include "fault_dev.hpp"
 #include "reduce_dev.hpp"
 #include "sbp_dev.hpp"
 #include "sc_internal.hpp"
@@ -319,13 +320,22 @@ __global__ __launch_bounds__(kRedThreads) void k_multidot_frame(FrameMap fm, int
 // hold the same coef[m] = -w[m] y_m . b; then x = sum_m coef[m] y_m and
 // r = b - sum_m coef[m] z_m with z_m = A y_m kept beside the basis, partial
 // sums of r.r and b.b (r itself is not stored: a solve that goes on from x
-// recomputes it).  Block 0 also starts the residual check (tolerance tol2).
-template <int M>
+// recomputes it).  The first streaming block also starts the residual check
+// (tolerance tol2).  FAULT: the first nfb blocks instead compute the fault
+// rates of the QD step from tau = sum_m coef[m] T_m (QdFuse).
+struct GuessFault {
+  QdFuse f;
+  VecList T;
+  int nfb;
+};
+
+template <int M, bool FAULT>
 __global__ __launch_bounds__(kRedThreads) void k_proj_guess_res(size_t N, VecList Y, VecList Z,
                                                                 const double* __restrict__ mpart, int mnb,
                                                                 const double* __restrict__ w, CgScal* sc,
                                                                 double tol2, const double* __restrict__ b,
-                                                                double* __restrict__ x, double* partials) {
+                                                                double* __restrict__ x, double* partials,
+                                                                GuessFault gf) {
   __shared__ double shc[M][kRedThreads / 64];
   __shared__ double coef[M];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -350,18 +360,33 @@ __global__ __launch_bounds__(kRedThreads) void k_proj_guess_res(size_t N, VecLis
     }
     __syncthreads();
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  double c[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) c[m] = coef[m];
+  int blk = blockIdx.x;
+  if constexpr (FAULT) {
+    if (blk < gf.nfb) {
+      const int k = blk * kRedThreads + threadIdx.x;
+      if (k < gf.f.a.N) {
+        double tq = 0.0;
+#pragma unroll
+        for (int m = 0; m < M; ++m) tq += c[m] * gf.T.v[m][k];
+        gf.f.dslip[k] = fault_qd_node(gf.f.a, k, tq);
+      }
+      return;
+    }
+    blk -= gf.nfb;
+  }
+  if (blk == 0 && threadIdx.x == 0) {
     sc->tol2 = tol2;
     sc->maxit = 0;
     sc->done = 0;
     sc->iters = 0;
     sc->cnt = 0;
   }
-  double c[M];
-#pragma unroll
-  for (int m = 0; m < M; ++m) c[m] = coef[m];
+  const unsigned nstream = gridDim.x - (FAULT ? gf.nfb : 0);
   double acc[2] = {0.0, 0.0};
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+  for (size_t i = (size_t)blk * blockDim.x + threadIdx.x; i < N; i += (size_t)nstream * blockDim.x) {
     double v = 0.0, az = 0.0;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
@@ -374,7 +399,15 @@ __global__ __launch_bounds__(kRedThreads) void k_proj_guess_res(size_t N, VecLis
     acc[0] += ri * ri;
     acc[1] += bi * bi;
   }
-  block_partials(acc, partials);
+  block_partials_at(acc, partials, blk);
+}
+
+// fault traction of basis vector k: T = mu Dy y on row 0 (the QD step's S y)
+__global__ __launch_bounds__(256) void k_basis_traction(DevOp<double> p, const double* __restrict__ y,
+                                                        double* __restrict__ T) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= p.Nz) return;
+  T[k] = p.mu[k] * dy_row0(p, y, k);
 }
 
 // New basis vector, A-orthogonal to the others: y = x - sum_m coef[m] y_m and
@@ -598,7 +631,7 @@ constexpr int kMdBlocksMax = 256;
 
 template <int M>
 void proj_guess_m(Ctx& c, const VecList& Y, const VecList& Z, const double* b, double* x, double tol2,
-                  bool rhs_frame, int pre_mnb) {
+                  bool rhs_frame, int pre_mnb, const QdFuse* fuse) {
   const size_t N = (size_t)c.N;
   hipStream_t s = c.stream;
   double* mpart = c.partials + kMdOffset;
@@ -615,9 +648,17 @@ void proj_guess_m(Ctx& c, const VecList& Y, const VecList& Z, const double* b, d
     hipLaunchKernelGGL(k_multidot<M>, dim3(mnb), dim3(kRedThreads), 0, s, N, b, (const double*)nullptr, Y,
                        mpart);
   }
-  hipLaunchKernelGGL(k_proj_guess_res<M>, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, Y, Z,
-                     (const double*)mpart, (int)mnb, (const double*)c.projW, (CgScal*)c.scal, tol2, b, x,
-                     c.partials);
+  if (fuse) {
+    GuessFault gf{*fuse, {}, (fuse->a.N + kRedThreads - 1) / kRedThreads};
+    for (int m = 0; m < M; ++m) gf.T.v[m] = c.projT[m];
+    hipLaunchKernelGGL((k_proj_guess_res<M, true>), dim3(red_blocks(N) + gf.nfb), dim3(kRedThreads), 0, s, N,
+                       Y, Z, (const double*)mpart, (int)mnb, (const double*)c.projW, (CgScal*)c.scal, tol2, b,
+                       x, c.partials, gf);
+  } else {
+    hipLaunchKernelGGL((k_proj_guess_res<M, false>), dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, Y, Z,
+                       (const double*)mpart, (int)mnb, (const double*)c.projW, (CgScal*)c.scal, tol2, b, x,
+                       c.partials, GuessFault{});
+  }
 }
 
 template <int M>
@@ -642,11 +683,12 @@ struct ProjArgs {
   double tol2;
   bool rhs_frame;
   int pre_mnb;
+  const QdFuse* fuse;
 };
 
 template <int M>
 void proj_dispatch(bool guess, Ctx& c, const VecList& Y, const VecList& Z, const ProjArgs& a) {
-  if (guess) proj_guess_m<M>(c, Y, Z, a.p0, a.p1, a.tol2, a.rhs_frame, a.pre_mnb);
+  if (guess) proj_guess_m<M>(c, Y, Z, a.p0, a.p1, a.tol2, a.rhs_frame, a.pre_mnb, a.fuse);
   else proj_add_m<M>(c, Y, Z, a.p0, a.p1, a.p2, a.p3);
 }
 
@@ -666,14 +708,16 @@ void proj_run(bool guess, int M, Ctx& c, const VecList& Y, const VecList& Z, con
 int proj_prepare(Ctx& c, int cap) {
   if (c.projCap == cap) return SC_OK;
   for (int i = 0; i < SC_REUSE_MAX; ++i) {
-    for (double** v : {&c.projY[i], &c.projZ[i], &c.projU[i], &c.projZU[i]}) {
+    for (double** v : {&c.projY[i], &c.projZ[i], &c.projU[i], &c.projZU[i], &c.projT[i]}) {
       if (*v) SC_HIP(hipFree(*v));
       *v = nullptr;
     }
   }
-  for (int i = 0; i < cap; ++i)
+  for (int i = 0; i < cap; ++i) {
     for (double** v : {&c.projY[i], &c.projZ[i], &c.projU[i], &c.projZU[i]})
       SC_HIP(hipMalloc(v, (size_t)c.N * sizeof(double)));
+    SC_HIP(hipMalloc(&c.projT[i], (size_t)c.d.Nz * sizeof(double)));
+  }
   if (!c.projW) SC_HIP(hipMalloc(&c.projW, 2 * SC_REUSE_MAX * sizeof(double)));
   c.projCap = cap;
   c.projCount = c.projNext = 0;
@@ -717,9 +761,11 @@ int proj_basis_vector(Ctx& c, int k, const double* x, const double* ax) {
       Y.v[i] = c.projY[i];
       Z.v[i] = c.projZ[i];
     }
-    proj_run(false, k, c, Y, Z, ProjArgs{x, const_cast<double*>(ax), c.projY[k], c.projZ[k], 0.0, false, 0});
+    proj_run(false, k, c, Y, Z, ProjArgs{x, const_cast<double*>(ax), c.projY[k], c.projZ[k], 0.0, false, 0, nullptr});
     hipLaunchKernelGGL(k_proj_weight<2>, dim3(1), dim3(kRedThreads), 0, s, c.partials, nb, c.projW, k);
   }
+  hipLaunchKernelGGL(k_basis_traction, dim3((c.d.Nz + 255) / 256), dim3(256), 0, s, c.op64,
+                     (const double*)c.projY[k], c.projT[k]);
   SC_HIP(hipGetLastError());
   return SC_OK;
 }
@@ -774,8 +820,9 @@ int bench_proj_guess(Ctx& c, int reps, double* ms, double* words) {
     CgScal* sc = (CgScal*)c.scal;
     switch (M) {
 #define SC_GUESS(m) \
-  case m: hipLaunchKernelGGL(k_proj_guess_res<m>, g, b, 0, s, N, Y, Z, (const double*)mp, mnb, \
-                             (const double*)c.projW, sc, 0.0, (const double*)c.r64, c.p64, c.partials); break;
+  case m: hipLaunchKernelGGL((k_proj_guess_res<m, false>), g, b, 0, s, N, Y, Z, (const double*)mp, mnb, \
+                             (const double*)c.projW, sc, 0.0, (const double*)c.r64, c.p64, c.partials, \
+                             GuessFault{}); break;
       SC_GUESS(1) SC_GUESS(2) SC_GUESS(3) SC_GUESS(4) SC_GUESS(5) SC_GUESS(6) SC_GUESS(7) SC_GUESS(8)
 #undef SC_GUESS
     }
@@ -814,8 +861,9 @@ int proj_frame_plan(Ctx& c, const sc_solve_opts& o, ProjFramePlan* plan) {
 }
 
 int solve(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st,
-          int* miss_flag, bool* deferred, bool rhs_frame, int pre_mnb) {
+          int* miss_flag, bool* deferred, bool rhs_frame, int pre_mnb, const QdFuse* fuse, bool* fused) {
   if (deferred) *deferred = false;
+  if (fused) *fused = false;
   int e = check_opts(o);
   if (e) return e;
   if (o.reuse == 0) return solve_core(c, rhs, u, o, st);
@@ -827,8 +875,10 @@ int solve(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve
   const size_t N = (size_t)c.N;
   if (c.projCount > 0) {
     CgScal* sc = (CgScal*)c.scal;
+    const QdFuse* f = miss_flag ? fuse : nullptr;  // only a deferred guess is taken as the answer
     proj_run(true, c.projCount, c, proj_list(c, false), proj_list(c, true),
-             ProjArgs{rhs, u, nullptr, nullptr, o.rtol * o.rtol, rhs_frame, pre_mnb});
+             ProjArgs{rhs, u, nullptr, nullptr, o.rtol * o.rtol, rhs_frame, pre_mnb, f});
+    if (f && fused) *fused = true;
     finish<FIN_IRRES, 2>(s, sc, c, N, 0, miss_flag);
     SC_HIP(hipGetLastError());
     if (miss_flag) {

@@ -175,23 +175,6 @@ int QdDriver::d_dt(double t, const double* psi, const double* slip, double* dpsi
     if (!e) e = launch_pl_source(ctx->c, gxy, gxz, rhs_d, s);
   }
   if (e) return e;
-  const double ts = now();
-  sc_solve_stats st;
-  bool deferred = false;
-  e = solve(ctx->c, rhs_d, u_d, desc.solve, st, defer ? miss_d : nullptr, &deferred, !pl, mnb);
-  counters.deferred_checks += deferred ? 1 : 0;
-  if (deferred) pendingDeferred++;
-  else if (!e) hits = (st.iters == 0 && desc.solve.reuse > 0) ? hits + 1 : 0;
-  counters.solve_seconds += now() - ts;
-  counters.solves += 1;
-  counters.cg_iters += st.iters;
-  counters.outer_sweeps += st.outer;
-  if (e) return e;
-  if (pl) {
-    e = launch_pl_strain(ctx->c, u_d, gxy, gxz, pl->T, pl->A, pl->B, pl->n, bcL_d, bcR_d, pl->cap,
-                         dgxy, dgxz, tauQS_d, pl->effVisc, pl->sxy, s);
-    if (e) return e;
-  }
   sc_fault_qd_args fa{};
   fa.N = Nz;
   fa.v0 = desc.v0;
@@ -215,9 +198,31 @@ int QdDriver::d_dt(double t, const double* psi, const double* slip, double* dpsi
   fa.tau_d = tau_d;
   fa.strength_d = strength_d;
   fa.status_d = status_d;
-  // sigma_xy on the fault from u (the power law's strain kernel wrote it already)
-  e = launch_qd_fault(ctx->c, pl ? nullptr : u_d, fa, dslip, s);
+  const double ts = now();
+  sc_solve_stats st;
+  bool deferred = false, fused = false;
+  // a deferred guess of the linear-elastic step also computes the fault rates
+  const QdFuse fuse{fa, dslip};
+  e = solve(ctx->c, rhs_d, u_d, desc.solve, st, defer ? miss_d : nullptr, &deferred, !pl, mnb,
+            pl ? nullptr : &fuse, &fused);
+  counters.deferred_checks += deferred ? 1 : 0;
+  if (deferred) pendingDeferred++;
+  else if (!e) hits = (st.iters == 0 && desc.solve.reuse > 0) ? hits + 1 : 0;
+  counters.solve_seconds += now() - ts;
+  counters.solves += 1;
+  counters.cg_iters += st.iters;
+  counters.outer_sweeps += st.outer;
   if (e) return e;
+  if (pl) {
+    e = launch_pl_strain(ctx->c, u_d, gxy, gxz, pl->T, pl->A, pl->B, pl->n, bcL_d, bcR_d, pl->cap,
+                         dgxy, dgxz, tauQS_d, pl->effVisc, pl->sxy, s);
+    if (e) return e;
+  }
+  // sigma_xy on the fault from u (the power law's strain kernel wrote it already)
+  if (!fused) {
+    e = launch_qd_fault(ctx->c, pl ? nullptr : u_d, fa, dslip, s);
+    if (e) return e;
+  }
   counters.total_seconds += now() - t0;
   return SC_OK;
 }

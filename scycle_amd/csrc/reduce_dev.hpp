@@ -18,7 +18,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // kernel of the projection): the kernel boundary orders the two, so no
 // device-scope fence is needed.
 template <int NV>
-__device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ partials) {
+__device__ __forceinline__ void block_partials_at(double (&v)[NV], double* __restrict__ partials, int blk) {
   __shared__ double sh[NV][kRedThreads / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -31,8 +31,13 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double* __restri
     double s = 0;
 #pragma unroll
     for (int w = 0; w < kRedThreads / 64; ++w) s += sh[threadIdx.x][w];
-    partials[(size_t)blockIdx.x * NV + threadIdx.x] = s;
+    partials[(size_t)blk * NV + threadIdx.x] = s;
   }
+}
+
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ partials) {
+  block_partials_at(v, partials, blockIdx.x);
 }
 
 }  // namespace sc

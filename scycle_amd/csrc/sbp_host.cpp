@@ -459,6 +459,8 @@ extern "C" int sc_ctx_destroy(sc_ctx* h) {
     if (p) (void)hipFree(p);
   for (double* p : c.projU)
     if (p) (void)hipFree(p);
+  for (double* p : c.projT)
+    if (p) (void)hipFree(p);
   for (double* p : c.projZU)
     if (p) (void)hipFree(p);
   if (c.projW) (void)hipFree(c.projW);

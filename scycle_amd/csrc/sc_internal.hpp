@@ -111,6 +111,7 @@ struct Ctx {
   // vectors y_i with weights w_i = 1 / (y_i^T (-A) y_i) on the device)
   double* projU[SC_REUSE_MAX] = {};    // ring of the last projCap solutions that iterated
   double* projZU[SC_REUSE_MAX] = {};   // A u for each of them
+  double* projT[SC_REUSE_MAX] = {};    // fault traction mu Dy y on row j = 0 (Nz each)
   double* projY[SC_REUSE_MAX] = {};    // A-orthogonal basis of the ring's span
   double* projZ[SC_REUSE_MAX] = {};    // A y for each basis vector
   double* projW = nullptr;             // w[SC_REUSE_MAX], coef[SC_REUSE_MAX]
@@ -151,12 +152,23 @@ int launch_deriv(const Ctx& c, int which, const double* in, double* out, hipStre
 // sxy on the fault line j = 0 only: out[k] = mu(0,k)*qy[0]*(D1y u)(0,k)
 int launch_fault_sxy(const Ctx& c, const double* u, double* out, hipStream_t s);
 
+// Fault rates fused into a deferred projected guess (the QD step): with the
+// guess accepted, the fault traction is sum_m c_m (mu Dy y_m) on row 0, so
+// the root solves need only the guess coefficients and run in blocks of
+// the guess kernel beside the streaming ones.  dslip gets V.
+struct QdFuse {
+  sc_fault_qd_args a;
+  double* dslip;
+};
+
 // miss_flag/deferred: see cg.hip (deferred residual check of a projected
 // guess); rhs_frame: rhs is zero off the nodes within kFrame of a boundary;
 // pre_mnb > 0: the caller already wrote the projections y_m . rhs as pre_mnb
-// block partials (see proj_frame_plan)
+// block partials (see proj_frame_plan); fuse (deferred solves only): also
+// compute the fault rates from the guess, *fused tells whether it did.
 int solve(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st,
-          int* miss_flag = nullptr, bool* deferred = nullptr, bool rhs_frame = false, int pre_mnb = 0);
+          int* miss_flag = nullptr, bool* deferred = nullptr, bool rhs_frame = false, int pre_mnb = 0,
+          const QdFuse* fuse = nullptr, bool* fused = nullptr);
 
 // For a caller that forms a frame-supported rhs and its projections on the
 // basis in one kernel: M basis vectors Y (M = 0: no projected guess for this
