@@ -43,33 +43,32 @@ __global__ __launch_bounds__(256) void k_qd_rhs_frame(DevOp<double> p, FrameMap 
   if (j == p.Ny - 1 && remote) bcR[k] = r;
 }
 
-// The same over a grid-stride loop, plus block partials of y_m . rhs for the
-// M basis vectors (the sums k_multidot_frame forms from the stored rhs).
+// With the top and bottom data zero (the QD driver never sets them) the rhs
+// is nonzero only in the kBand rows next to each y-side: this kernel covers
+// those rows (the rest of the frame stays zero), over a grid-stride loop, and
+// forms the block partials of y_m . rhs for the M basis vectors.
 template <int M>
-__global__ __launch_bounds__(kRedThreads) void k_qd_rhs_frame_md(DevOp<double> p, FrameMap fm,
+__global__ __launch_bounds__(kRedThreads) void k_qd_rhs_frame_md(DevOp<double> p, int nLow, int nHigh,
                                                                  double* __restrict__ rhs,
                                                                  const double* __restrict__ slip,
                                                                  const double* __restrict__ shift, double t,
                                                                  double vL, double fts, int remote,
                                                                  double* __restrict__ bcL,
-                                                                 double* __restrict__ bcR,
-                                                                 const double* __restrict__ bcT,
-                                                                 const double* __restrict__ bcB, VecList Y,
+                                                                 double* __restrict__ bcR, VecList Y,
                                                                  double* mpart) {
   double acc[M];
 #pragma unroll
   for (int m = 0; m < M; ++m) acc[m] = 0.0;
-  const long long n = fm.count();
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
-       q += (long long)gridDim.x * blockDim.x) {
-    int j, k;
-    fm.node(q, j, k);
+  const unsigned Nz = (unsigned)p.Nz;
+  const unsigned n = (unsigned)(nLow + nHigh) * Nz;
+  for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const unsigned row = q / Nz;
+    const int k = (int)(q - row * Nz);
+    const int j = (int)row < nLow ? (int)row : p.Ny - nHigh + ((int)row - nLow);
     const double l = slip[k] * (1.0 / fts);
     const double r = remote ? vL * t / fts + 1.0 * shift[k] : bcR[k];
-    const size_t i = (size_t)j * p.Nz + k;
-    const double v = rhs_node(p, j, k, [&](int side, int ii) {
-      return side == 0 ? l : side == 1 ? r : side == 2 ? bcT[ii] : bcB[ii];
-    });
+    const size_t i = (size_t)j * Nz + k;
+    const double v = rhs_node(p, j, k, [&](int side, int) { return side == 0 ? l : side == 1 ? r : 0.0; });
     rhs[i] = v;
     if (j == 0) bcL[k] = l;
     if (j == p.Ny - 1 && remote) bcR[k] = r;
@@ -80,11 +79,11 @@ __global__ __launch_bounds__(kRedThreads) void k_qd_rhs_frame_md(DevOp<double> p
 }
 
 template <int M>
-void launch_rhs_md(const Ctx& c, const FrameMap& fm, unsigned nb, double* rhs, const double* slip,
+void launch_rhs_md(const Ctx& c, int nLow, int nHigh, unsigned nb, double* rhs, const double* slip,
                    const double* shift, double t, double vL, double fts, int remote, double* bcL, double* bcR,
-                   const double* bcT, const double* bcB, const ProjFramePlan& plan, hipStream_t s) {
-  hipLaunchKernelGGL(k_qd_rhs_frame_md<M>, dim3(nb), dim3(kRedThreads), 0, s, c.op64, fm, rhs, slip, shift, t,
-                     vL, fts, remote, bcL, bcR, bcT, bcB, plan.Y, plan.mpart);
+                   const ProjFramePlan& plan, hipStream_t s) {
+  hipLaunchKernelGGL(k_qd_rhs_frame_md<M>, dim3(nb), dim3(kRedThreads), 0, s, c.op64, nLow, nHigh, rhs, slip,
+                     shift, t, vL, fts, remote, bcL, bcR, plan.Y, plan.mpart);
 }
 
 int launch_qd_rhs_frame(const Ctx& c, double* rhs, const double* slip, const double* shift, double t,
@@ -99,17 +98,23 @@ int launch_qd_rhs_frame(const Ctx& c, double* rhs, const double* slip, const dou
     SC_HIP(hipGetLastError());
     return SC_OK;
   }
+  // the y-band rows only: the QD driver never sets the top/bottom data, so
+  // the rest of the frame is zero (as the plan-less launch above leaves it)
+  const int Ny = c.d.Ny;
+  const int nLow = std::min(kBand, Ny);
+  const int nHigh = Ny > kBand ? std::min(kBand, Ny - kBand) : 0;
+  const long long nb_rows = (long long)(nLow + nHigh) * c.d.Nz;
   const unsigned nb = (unsigned)std::max<long long>(
-      1, std::min<long long>((n + kRedThreads - 1) / kRedThreads, plan->maxBlocks));
+      1, std::min<long long>((nb_rows + kRedThreads - 1) / kRedThreads, plan->maxBlocks));
   switch (plan->M) {
-    case 1: launch_rhs_md<1>(c, fm, nb, rhs, slip, shift, t, vL, fts, remote, bcL, bcR, bcT, bcB, *plan, s); break;
-    case 2: launch_rhs_md<2>(c, fm, nb, rhs, slip, shift, t, vL, fts, remote, bcL, bcR, bcT, bcB, *plan, s); break;
-    case 3: launch_rhs_md<3>(c, fm, nb, rhs, slip, shift, t, vL, fts, remote, bcL, bcR, bcT, bcB, *plan, s); break;
-    case 4: launch_rhs_md<4>(c, fm, nb, rhs, slip, shift, t, vL, fts, remote, bcL, bcR, bcT, bcB, *plan, s); break;
-    case 5: launch_rhs_md<5>(c, fm, nb, rhs, slip, shift, t, vL, fts, remote, bcL, bcR, bcT, bcB, *plan, s); break;
-    case 6: launch_rhs_md<6>(c, fm, nb, rhs, slip, shift, t, vL, fts, remote, bcL, bcR, bcT, bcB, *plan, s); break;
-    case 7: launch_rhs_md<7>(c, fm, nb, rhs, slip, shift, t, vL, fts, remote, bcL, bcR, bcT, bcB, *plan, s); break;
-    default: launch_rhs_md<8>(c, fm, nb, rhs, slip, shift, t, vL, fts, remote, bcL, bcR, bcT, bcB, *plan, s); break;
+    case 1: launch_rhs_md<1>(c, nLow, nHigh, nb, rhs, slip, shift, t, vL, fts, remote, bcL, bcR, *plan, s); break;
+    case 2: launch_rhs_md<2>(c, nLow, nHigh, nb, rhs, slip, shift, t, vL, fts, remote, bcL, bcR, *plan, s); break;
+    case 3: launch_rhs_md<3>(c, nLow, nHigh, nb, rhs, slip, shift, t, vL, fts, remote, bcL, bcR, *plan, s); break;
+    case 4: launch_rhs_md<4>(c, nLow, nHigh, nb, rhs, slip, shift, t, vL, fts, remote, bcL, bcR, *plan, s); break;
+    case 5: launch_rhs_md<5>(c, nLow, nHigh, nb, rhs, slip, shift, t, vL, fts, remote, bcL, bcR, *plan, s); break;
+    case 6: launch_rhs_md<6>(c, nLow, nHigh, nb, rhs, slip, shift, t, vL, fts, remote, bcL, bcR, *plan, s); break;
+    case 7: launch_rhs_md<7>(c, nLow, nHigh, nb, rhs, slip, shift, t, vL, fts, remote, bcL, bcR, *plan, s); break;
+    default: launch_rhs_md<8>(c, nLow, nHigh, nb, rhs, slip, shift, t, vL, fts, remote, bcL, bcR, *plan, s); break;
   }
   SC_HIP(hipGetLastError());
   if (mnb) *mnb = (int)nb;
