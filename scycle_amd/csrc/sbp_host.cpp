@@ -318,6 +318,7 @@ int ctx_compute_diag(Ctx& c) {
   if (!c.dinv64) SC_HIP(hipMalloc(&c.dinv64, N * sizeof(double)));
   if (!c.dinv32) SC_HIP(hipMalloc(&c.dinv32, N * sizeof(float)));
   int e = launch_diag(c, c.diag64, c.stream);
+  if (!e) e = ctx_build_frame(c);
   if (e) return e;
   SC_HIP(hipStreamSynchronize(c.stream));
   return SC_OK;
@@ -447,7 +448,8 @@ extern "C" int sc_ctx_destroy(sc_ctx* h) {
   if (!h) return SC_OK;
   sc::Ctx& c = h->c;
   if (c.stream) (void)hipStreamSynchronize(c.stream);
-  void* bufs[] = {c.mu64, c.mu32, c.tab64, c.tab32, c.ctab64, c.ctab32, c.diag64, c.dinv64, c.dinv32, c.r64,
+  void* bufs[] = {c.mu64, c.mu32, c.tab64, c.tab32, c.ctab64, c.ctab32, c.fcoef64, c.fcoef32, c.diag64,
+                  c.dinv64, c.dinv32, c.r64,
                   c.p64, c.q64, c.r32, c.p32, c.q32, c.d32, c.partials, c.scal, c.z64,
                   c.zold64};
   delete c.fd;

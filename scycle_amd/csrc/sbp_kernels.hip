@@ -10,12 +10,13 @@
 // with c = mu*qy (y) or mu*rz (z) and c3 the two-point average the reference
 // builds with global index I+1 (sbpOps_m_varGrid.cpp:1285-1311, 1428-1448).
 //
-// Two kernels share the work: an interior kernel (rows/columns at least
-// kFrame from every boundary, where all stencils are the constant interior
-// ones) and a frame kernel, one thread per node, which also carries the SAT
-// boundary terms and evaluates the boundary closures from the 1-D tables
-// (wave-uniform along the boundary normal).  Grids with a side of at most
-// 2*kFrame nodes use a per-node generic kernel instead.
+// One launch does the work: interior blocks (rows/columns at least kFrame
+// from every boundary, where all stencils are the constant interior ones)
+// and frame blocks, one thread per node within kFrame of a boundary, which
+// apply the frame operator assembled once per operator (34 coefficients a
+// node: the boundary closures, prefactors and SAT terms folded in).  Grids
+// with a side of at most 2*kFrame nodes use a per-node generic kernel
+// instead.
 #include "sbp_dev.hpp"
 
 namespace sc {
@@ -260,6 +261,31 @@ __global__ __launch_bounds__(256) void k_apply_frame(DevOp<T> p, FrameMap fm,
   ep(j, k, I, eval_node(p, acc, j, k), u[I], true);
 }
 
+// Whole-wave lane shifts by one (DPP wave_shr:1 / wave_shl:1 of the GFX9
+// family): lane i receives lane i-1 (up) or lane i+1 (down), as
+// __shfl_up/__shfl_down(x, 1, 64) but as a VALU operand modifier rather than
+// an LDS permute.  The end lane keeps its own value; the kernels never use
+// the values shifted in at the wave's edges.
+constexpr int kDppWaveShr1 = 0x138;
+constexpr int kDppWaveShl1 = 0x130;
+template <int CTRL>
+__device__ __forceinline__ float dpp_shift(float x) {
+  const int v = __float_as_int(x);
+  return __int_as_float(__builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
+  const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
+  const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi2 << 32) | (unsigned int)lo2);
+}
+template <typename T>
+__device__ __forceinline__ T lane_up1(T x) { return dpp_shift<kDppWaveShr1>(x); }
+template <typename T>
+__device__ __forceinline__ T lane_down1(T x) { return dpp_shift<kDppWaveShl1>(x); }
+
 // ------------------------------------------------------ interior kernel ---
 // One wave per block.  Lanes cover columns [k0-4, k0+60); lanes 4..59 write
 // outputs (56 columns), the outer 4+4 lanes only feed the z-direction
@@ -273,6 +299,43 @@ __global__ __launch_bounds__(256) void k_apply_frame(DevOp<T> p, FrameMap fm,
 // rows of u, are neighbours.  With XCD = 1 the linear block id is remapped
 // so that each of the eight XCDs (which receive blocks round-robin) owns a
 // contiguous range of tiles and the halo rows stay in its own L2.
+// ------------------------------------------- assembled frame operator ----
+// At a frame node the operator (both directions, their prefactors and the
+// SAT terms) is a cross of two 17-point stencils: cy[r] on u(j-8+r, k) and
+// cz[r] on u(j, k-8+r), zero outside the grid.  Its coefficients depend only
+// on mu, the grid and the BC types, so they are formed once per operator
+// (k_frame_coefs, the same algebra as the closure and SAT terms with the sums
+// taken per u entry) and the frame apply is a 34-term product per node,
+// coefficients stored coef[r * nFrame + t] for frame node t (FrameMap order).
+constexpr int kCross = 17;
+constexpr int kFrameCoefs = 2 * kCross;
+
+template <typename T, class EP>
+__device__ __forceinline__ void frame_cf_node(const DevOp<T>& p, const FrameMap& fm, const T* __restrict__ coef,
+                                              const T* __restrict__ u, const EP& ep, long long t) {
+  const long long nf = fm.count();
+  if (t >= nf) return;
+  int j, k;
+  fm.node(t, j, k);
+  const int Ny = p.Ny, Nz = p.Nz;
+  T v = 0;
+#pragma unroll
+  for (int r = 0; r < kCross; ++r) {  // clamped loads, then a select: all in flight together
+    const int jj = j - 8 + r;
+    const T x = u[(size_t)min(max(jj, 0), Ny - 1) * Nz + k];
+    v += coef[(size_t)r * nf + t] * ((jj >= 0 && jj < Ny) ? x : T(0));
+  }
+  const size_t row = (size_t)j * Nz;
+#pragma unroll
+  for (int r = 0; r < kCross; ++r) {
+    const int kk = k - 8 + r;
+    const T x = u[row + min(max(kk, 0), Nz - 1)];
+    v += coef[(size_t)(kCross + r) * nf + t] * ((kk >= 0 && kk < Nz) ? x : T(0));
+  }
+  const size_t I = row + k;
+  ep(j, k, I, v, u[I], true);
+}
+
 constexpr int kJC = 8;
 constexpr int kLanesOut = 56;
 constexpr int kXcd = 8;
@@ -284,13 +347,23 @@ __device__ __forceinline__ int tile_of_block(int b, int total, int xcd) {
   return x * q + min(x, r) + slot;
 }
 
+//
+// The first nfb blocks (a multiple of kXcd, so the remap above is unchanged
+// for the rest) apply the assembled frame operator to 64 frame nodes each:
+// one launch covers the whole grid, and the frame's latency-bound loads
+// overlap the interior's streaming.
 template <typename T, int ORDER, class EP, int XCD = 1>
 __global__ __launch_bounds__(64) void k_apply_interior(DevOp<T> p, const T* __restrict__ u, EP ep,
-                                                       int nbx, int nby) {
+                                                       int nbx, int nby, FrameMap fm,
+                                                       const T* __restrict__ fcoef, int nfb) {
   constexpr int JC = kJC;
   const int Ny = p.Ny, Nz = p.Nz;
   const int lane = threadIdx.x;
-  const int tile = tile_of_block(blockIdx.x, nbx * nby, XCD);
+  if ((int)blockIdx.x < nfb) {
+    frame_cf_node(p, fm, fcoef, u, ep, (long long)blockIdx.x * 64 + lane);
+    return;
+  }
+  const int tile = tile_of_block((int)blockIdx.x - nfb, nbx * nby, XCD);
   const int bx = tile / nby, by = tile - bx * nby;
   const int k = kFrame + bx * kLanesOut - 4 + lane;
   const int j0 = kFrame + by * JC;
@@ -319,7 +392,7 @@ __global__ __launch_bounds__(64) void k_apply_interior(DevOp<T> p, const T* __re
     const T du = dy.d1[0] * uw[r] + dy.d1[1] * uw[r + 1] + dy.d1[3] * uw[r + 3] + dy.d1[4] * uw[r + 4];
     g1[r] = cm * du;
     if (ORDER == 4) {
-      const T mnext = __shfl_down(mw[r], 1, 64);
+      const T mnext = lane_down1(mw[r]);
       const T c3 = T(0.5) * (cm + mnext * qm);
       const T d3u = dy.dr[1] * uw[r + 1] + dy.dr[2] * uw[r + 2] + dy.dr[3] * uw[r + 3] + dy.dr[4] * uw[r + 4];
       g3[r] = dy.wr * c3 * d3u;
@@ -354,13 +427,13 @@ __global__ __launch_bounds__(64) void k_apply_interior(DevOp<T> p, const T* __re
     }
     if (p.doZ) {
       const T uc = uw[ro + 4];
-      const T um1 = __shfl_up(uc, 1, 64), um2 = __shfl_up(uc, 2, 64);
-      const T up1 = __shfl_down(uc, 1, 64), up2 = __shfl_down(uc, 2, 64);
+      const T um1 = lane_up1(uc), um2 = lane_up1(um1);
+      const T up1 = lane_down1(uc), up2 = lane_down1(up1);
       const T cmz = mw[ro + 2] * rzk;
       const T g1z = cmz * (dz.d1[0] * um2 + dz.d1[1] * um1 + dz.d1[3] * up1 + dz.d1[4] * up2);
       T g3z, g4z;
       if (ORDER == 4) {
-        const T mz1 = __shfl_down(mw[ro + 2], 1, 64);
+        const T mz1 = lane_down1(mw[ro + 2]);
         const T c3z = T(0.5) * (cmz + mz1 * rzk1);
         g3z = dz.wr * c3z * (dz.dr[1] * um1 + dz.dr[2] * uc + dz.dr[3] * up1 + dz.dr[4] * up2);
         g4z = dz.w4c * cmz * (dz.d4[0] * um2 + dz.d4[1] * um1 + dz.d4[2] * uc + dz.d4[3] * up1 + dz.d4[4] * up2);
@@ -374,79 +447,11 @@ __global__ __launch_bounds__(64) void k_apply_interior(DevOp<T> p, const T* __re
       const T w2 = -(dz.drt[2] * g3z + dz.d4t[2] * g4z);
       const T w3 = dz.Hc * dz.d1[3] * g1z - (dz.drt[3] * g3z + dz.d4t[3] * g4z);
       const T w4 = dz.Hc * dz.d1[4] * g1z - (dz.drt[4] * g3z + dz.d4t[4] * g4z);
-      const T Lz = w2 + __shfl_up(w1, 1, 64) + __shfl_up(w0, 2, 64) + __shfl_down(w3, 1, 64) +
-                   __shfl_down(w4, 2, 64);
+      const T Lz = w2 + lane_up1(w1) + lane_up1(lane_up1(w0)) + lane_down1(w3) + lane_down1(lane_down1(w4));
       v += (dy.H[j < Ny ? j : Ny - 1] * dy.met[j < Ny ? j : Ny - 1]) * Lz;
     }
     if (writer && j < Ny - kFrame) ep(j, k, (size_t)j * Nz + k, v, uw[ro + 4], false);
   }
-}
-
-// ----------------------------------------------------------- frame nodes --
-// Interior (constant-stencil) 1-D operator at a node at least kFrame from both
-// ends of its line, evaluated straight from memory (no prefactor).  Used for
-// the direction tangential to the boundary in the frame kernel.
-template <typename T, int ORDER>
-__device__ __forceinline__ T y_interior(const DevOp<T>& p, const T* __restrict__ u, int j, int k) {
-  const DevDir<T>& d = p.y;
-  const int Nz = p.Nz;
-  T uu[9];
-#pragma unroll
-  for (int r = 0; r < 9; ++r) uu[r] = u[(size_t)(j - 4 + r) * Nz + k];
-  T g1[5], g3[5], g4[5];
-#pragma unroll
-  for (int s = 0; s < 5; ++s) {
-    const int m = j - 2 + s;
-    const T qm = d.imet[m];
-    const T cm = p.mu[(size_t)m * Nz + k] * qm;
-    g1[s] = cm * (d.d1[0] * uu[s] + d.d1[1] * uu[s + 1] + d.d1[3] * uu[s + 3] + d.d1[4] * uu[s + 4]);
-    if (ORDER == 4) {
-      // two-point average with global index I+1 (wraps to the next y-row at k = Nz-1)
-      const bool inRow = k + 1 < Nz;
-      const T mn = p.mu[inRow ? (size_t)m * Nz + k + 1 : (size_t)(m + 1) * Nz];
-      const T qn = d.imet[inRow ? m : m + 1];
-      const T c3 = T(0.5) * (cm + mn * qn);
-      g3[s] = d.wr * c3 * (d.dr[1] * uu[s + 1] + d.dr[2] * uu[s + 2] + d.dr[3] * uu[s + 3] + d.dr[4] * uu[s + 4]);
-      g4[s] = d.w4c * cm * (d.d4[0] * uu[s] + d.d4[1] * uu[s + 1] + d.d4[2] * uu[s + 2] +
-                            d.d4[3] * uu[s + 3] + d.d4[4] * uu[s + 4]);
-    } else {
-      g3[s] = d.wr * cm * (d.dr[1] * uu[s + 1] + d.dr[2] * uu[s + 2] + d.dr[3] * uu[s + 3]);
-      g4[s] = T(0);
-    }
-  }
-  T L = d.Hc * (d.d1[0] * g1[0] + d.d1[1] * g1[1] + d.d1[3] * g1[3] + d.d1[4] * g1[4]);
-  L -= (d.drt[0] * g3[0] + d.drt[1] * g3[1] + d.drt[2] * g3[2] + d.drt[3] * g3[3] + d.drt[4] * g3[4]) +
-       (d.d4t[0] * g4[0] + d.d4t[1] * g4[1] + d.d4t[2] * g4[2] + d.d4t[3] * g4[3] + d.d4t[4] * g4[4]);
-  return L;
-}
-
-template <typename T, int ORDER>
-__device__ __forceinline__ T z_interior(const DevOp<T>& p, const T* __restrict__ u, int j, int k) {
-  const DevDir<T>& d = p.z;
-  const size_t row = (size_t)j * p.Nz;
-  T uu[9];
-#pragma unroll
-  for (int r = 0; r < 9; ++r) uu[r] = u[row + k - 4 + r];
-  T g1[5], g3[5], g4[5];
-#pragma unroll
-  for (int s = 0; s < 5; ++s) {
-    const int m = k - 2 + s;
-    const T cm = p.mu[row + m] * d.imet[m];
-    g1[s] = cm * (d.d1[0] * uu[s] + d.d1[1] * uu[s + 1] + d.d1[3] * uu[s + 3] + d.d1[4] * uu[s + 4]);
-    if (ORDER == 4) {
-      const T c3 = T(0.5) * (cm + p.mu[row + m + 1] * d.imet[m + 1]);
-      g3[s] = d.wr * c3 * (d.dr[1] * uu[s + 1] + d.dr[2] * uu[s + 2] + d.dr[3] * uu[s + 3] + d.dr[4] * uu[s + 4]);
-      g4[s] = d.w4c * cm * (d.d4[0] * uu[s] + d.d4[1] * uu[s + 1] + d.d4[2] * uu[s + 2] +
-                            d.d4[3] * uu[s + 3] + d.d4[4] * uu[s + 4]);
-    } else {
-      g3[s] = d.wr * cm * (d.dr[1] * uu[s + 1] + d.dr[2] * uu[s + 2] + d.dr[3] * uu[s + 3]);
-      g4[s] = T(0);
-    }
-  }
-  T L = d.Hc * (d.d1[0] * g1[0] + d.d1[1] * g1[1] + d.d1[3] * g1[3] + d.d1[4] * g1[4]);
-  L -= (d.drt[0] * g3[0] + d.drt[1] * g3[1] + d.drt[2] * g3[2] + d.drt[3] * g3[3] + d.drt[4] * g3[4]) +
-       (d.d4t[0] * g4[0] + d.d4t[1] * g4[1] + d.d4t[2] * g4[2] + d.d4t[3] * g4[3] + d.d4t[4] * g4[4]);
-  return L;
 }
 
 // Copies the four packed closure tables (ClosureTab, built at context
@@ -473,202 +478,226 @@ __device__ __forceinline__ void stage_closures(ClosureTab<T> (&tab)[4], const Cl
   }
 }
 
-// 1-D operator at line index i (within kFrame of one end of a line of length
-// n), in flux form with the boundary closures, from the window w[r] = u at
-// line index i-8+r (zero outside the line) and the coefficients at the nine
-// flux rows m = i-4+o: cw[o] = c and c3w[o] = its two-point average.  All
-// inputs are in registers before the loop, so the loads overlap.
+// ---------------------------------------- frame coefficient assembly ----
+// 1-D operator at line index i within kFrame of one end of a line of length n
+// (n > 2*kFrame), in flux form with the boundary closures of the staged table
+// t, as coefficients on u at line index i-8+r (cf[r], scaled by scale):
+//   L(u)[i] = H[i] sum_o oD1[i][o] c_m (fD1 u)_m - oDRT[i][o] wR_m c3_m (fDR u)_m
+//             - oD4T[i][o] w4_m c_m (fD4 u)_m,   m = i-4+o
+// with cw[o] = c and c3w[o] = its two-point average at flux row m.
 template <typename T, int ORDER>
-__device__ __forceinline__ T closure_line(const ClosureTab<T>& t, int n, int i,
-                                          const T (&w)[17], const T (&cw)[kBand],
-                                          const T (&c3w)[kBand]) {
-  const bool hi = i >= kFrame;  // n > 2*kFrame here
+__device__ __forceinline__ void closure_coefs(const ClosureTab<T>& t, int n, int i, const T (&cw)[kBand],
+                                              const T (&c3w)[kBand], T scale, T (&cf)[kCross]) {
+  const bool hi = i >= kFrame;
   const int mb = hi ? n - 12 : 0, jb = hi ? n - kFrame : 0;
   const int io = i - jb;
-  T a1 = 0, aR = 0, a4 = 0;
-#pragma unroll
+  const T h = t.H[io];
   for (int o = 0; o < kBand; ++o) {
     const int m = i + o - 4;
-    if (m < 0 || m >= n) continue;  // outside the line: zero coefficients
+    if (m < 0 || m >= n) continue;
     const int fm = m - mb;
-    T s1 = 0, sR = 0, s4 = 0;
-#pragma unroll
-    for (int q = 0; q < kBand; ++q) {
-      const T x = w[o + q];
-      s1 += t.fD1[fm * kBand + q] * x;
-      sR += t.fDR[fm * kBand + q] * x;
-      if (ORDER == 4) s4 += t.fD4[fm * kBand + q] * x;
-    }
     const T cm = cw[o];
-    a1 += t.oD1[io * kBand + o] * (cm * s1);
+    const T a1 = scale * (h * (t.oD1[io * kBand + o] * cm));
     const T wr = t.wR[fm];
-    if (wr != T(0)) aR += t.oDRT[io * kBand + o] * (wr * (ORDER == 4 ? c3w[o] : cm) * sR);
-    if (ORDER == 4) {
-      const T w4 = t.w4[fm];
-      if (w4 != T(0)) a4 += t.oD4T[io * kBand + o] * (w4 * cm * s4);
+    const T aR = wr != T(0) ? scale * (t.oDRT[io * kBand + o] * (wr * (ORDER == 4 ? c3w[o] : cm))) : T(0);
+    const T w4 = ORDER == 4 ? t.w4[fm] : T(0);
+    const T a4 = w4 != T(0) ? scale * (t.oD4T[io * kBand + o] * (w4 * cm)) : T(0);
+    for (int q = 0; q < kBand; ++q) {
+      T v = a1 * t.fD1[fm * kBand + q] - aR * t.fDR[fm * kBand + q];
+      if (ORDER == 4) v -= a4 * t.fD4[fm * kBand + q];
+      cf[o + q] += v;
     }
   }
-  return t.H[io] * a1 - (aR + a4);
 }
 
-// y-direction operator at (j, k), j in the row frame.
+// The constant interior 1-D operator (a node at least kFrame from both ends
+// of its line) as coefficients on u at offsets -4..4 (cf[4..12]); cs[s],
+// c3s[s] are c and its two-point average at the five flux rows offset s-2.
 template <typename T, int ORDER>
-__device__ __forceinline__ T y_closure(const DevOp<T>& p, const ClosureTab<T>& t,
-                                       const T* __restrict__ u, int j, int k) {
+__device__ __forceinline__ void interior_coefs(const DevDir<T>& d, const T (&cs)[5], const T (&c3s)[5], T scale,
+                                               T (&cf)[kCross]) {
+  for (int sf = 0; sf < 5; ++sf) {
+    const T a1 = (sf == 2) ? T(0) : scale * (d.Hc * d.d1[sf] * cs[sf]);
+    const T aR = scale * (d.drt[sf] * d.wr * (ORDER == 4 ? c3s[sf] : cs[sf]));
+    const T a4 = ORDER == 4 ? scale * (d.d4t[sf] * d.w4c * cs[sf]) : T(0);
+    for (int q = 0; q < 5; ++q) {
+      T v = (q == 2 ? T(0) : a1 * d.d1[q]);
+      if (ORDER == 4) {
+        if (q >= 1) v -= aR * d.dr[q];
+        v -= a4 * d.d4[q];
+      } else if (q >= 1 && q <= 3) {
+        v -= aR * d.dr[q];
+      }
+      cf[4 + sf + q] += v;
+    }
+  }
+}
+
+template <typename T, int ORDER>
+__global__ __launch_bounds__(kFrameThreads) void k_frame_coefs(DevOp<T> p, FrameMap fm, T* __restrict__ coef) {
+  __shared__ __align__(16) ClosureTab<T> tab[4];
+  stage_closures(tab, p.ctab);
+  __syncthreads();
+  const long long nf = fm.count();
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nf) return;
+  int j, k;
+  fm.node(t, j, k);
   const int Ny = p.Ny, Nz = p.Nz;
   const T* mu = p.mu;
-  const T* im = p.y.imet;
-  // every load is unconditional (clamped index, then a select): conditional
-  // loads would each cost a full memory latency
-  T w[17];
-#pragma unroll
-  for (int r = 0; r < 17; ++r) {
-    const int jj = j - 8 + r;
-    const T x = u[(size_t)min(max(jj, 0), Ny - 1) * Nz + k];
-    w[r] = (jj >= 0 && jj < Ny) ? x : T(0);
-  }
-  // c = mu*qy at (m, k); c3 = average with global index I+1: (m, k+1), or the
-  // first node of row m+1 when k is the last column, or I itself at the last
-  // node (0.5*(c+c) = c exactly)
-  T cw[kBand], c3w[kBand];
-#pragma unroll
-  for (int o = 0; o < kBand; ++o) {
-    const int m = min(max(j + o - 4, 0), Ny - 1);
-    const size_t I = (size_t)m * Nz + k;
-    const bool inRow = k + 1 < Nz, lastRow = m + 1 >= Ny;
-    const size_t In = inRow ? I + 1 : (lastRow ? I : (size_t)(m + 1) * Nz);
-    const int mn = (inRow || lastRow) ? m : m + 1;
-    const T cm = mu[I] * im[m];
-    cw[o] = cm;
-    c3w[o] = T(0.5) * (cm + mu[In] * im[mn]);
-  }
-  return closure_line<T, ORDER>(t, Ny, j, w, cw, c3w);
-}
-
-// z-direction operator at (j, k), k in the column frame.
-template <typename T, int ORDER>
-__device__ __forceinline__ T z_closure(const DevOp<T>& p, const ClosureTab<T>& t,
-                                       const T* __restrict__ u, int j, int k) {
-  const int Ny = p.Ny, Nz = p.Nz;
-  const size_t row = (size_t)j * Nz;
-  const T* mu = p.mu;
-  const T* im = p.z.imet;
-  T w[17];
-#pragma unroll
-  for (int r = 0; r < 17; ++r) {
-    const int kk = k - 8 + r;
-    const T x = u[row + min(max(kk, 0), Nz - 1)];
-    w[r] = (kk >= 0 && kk < Nz) ? x : T(0);
-  }
-  // c = mu*rz at flux row m; c3 = average with global index I+1: the next
-  // column, or the first node of the next row after the last column, or I
-  // itself at the last node (unconditional loads, as in y_closure)
-  const bool lastRow = j + 1 >= Ny;
-  T cw[kBand], c3w[kBand];
-#pragma unroll
-  for (int o = 0; o < kBand; ++o) {
-    const int m = min(max(k + o - 4, 0), Nz - 1);
-    const bool inRow = m + 1 < Nz;
-    const size_t In = inRow ? row + m + 1 : (lastRow ? row + m : row + Nz);
-    const int kn = inRow ? m + 1 : (lastRow ? m : 0);
-    const T cm = mu[row + m] * im[m];
-    cw[o] = cm;
-    c3w[o] = T(0.5) * (cm + mu[In] * im[kn]);
-  }
-  return closure_line<T, ORDER>(t, Nz, k, w, cw, c3w);
-}
-
-// SAT terms at a frame node (constructBC_* of sbpOps_m_varGrid.cpp:601-772),
-// with BS and the boundary rows of D1 read from the staged tables.  BS has no
-// entries beyond kFrame, so only frame nodes carry SAT terms.
-template <typename T>
-__device__ __forceinline__ T sat_frame(const DevOp<T>& p, const ClosureTab<T> (&tab)[4],
-                                       const T* __restrict__ u, int j, int k) {
-  T s = 0;
-  const int Ny = p.Ny, Nz = p.Nz;
+  T cy[kCross], cz[kCross];
+  for (int r = 0; r < kCross; ++r) cy[r] = cz[r] = T(0);
   if (p.doY) {
-    const DevDir<T>& d = p.y;
     const T pre = p.z.H[k] * p.z.met[k];
+    const T* im = p.y.imet;
+    if (j < kFrame || j >= Ny - kFrame) {
+      T cw[kBand], c3w[kBand];
+      // c = mu*qy at the nine flux rows m = j-4+o (clamped: rows outside the
+      // line get zero coefficients); c3 = average with global index I+1: (m, k+1),
+      // the first node of row m+1 after the last column, or I at the last node
+      for (int o = 0; o < kBand; ++o) {
+        const int m = min(max(j + o - 4, 0), Ny - 1);
+        const size_t I = (size_t)m * Nz + k;
+        const bool inRow = k + 1 < Nz, lastRow = m + 1 >= Ny;
+        const size_t In = inRow ? I + 1 : (lastRow ? I : (size_t)(m + 1) * Nz);
+        const int mn = (inRow || lastRow) ? m : m + 1;
+        const T cm = mu[I] * im[m];
+        cw[o] = cm;
+        c3w[o] = T(0.5) * (cm + mu[In] * im[mn]);
+      }
+      closure_coefs<T, ORDER>(tab[j >= kFrame ? 1 : 0], Ny, j, cw, c3w, pre, cy);
+    } else {
+      T cs[5], c3s[5];
+      for (int sf = 0; sf < 5; ++sf) {  // flux rows j-2..j+2
+        const int m = j - 2 + sf;
+        const T cm = mu[(size_t)m * Nz + k] * im[m];
+        const bool inRow = k + 1 < Nz;
+        const T mn = mu[inRow ? (size_t)m * Nz + k + 1 : (size_t)(m + 1) * Nz];
+        const T qn = im[inRow ? m : m + 1];
+        cs[sf] = cm;
+        c3s[sf] = T(0.5) * (cm + mn * qn);
+      }
+      interior_coefs<T, ORDER>(p.y, cs, c3s, pre, cy);
+    }
+    // SAT terms of the y-sides (constructBC_*, sbpOps_m_varGrid.cpp:601-772): BS
+    // rows on the boundary value, or the boundary row of D1 (Neumann)
+    const DevDir<T>& d = p.y;
     if (j < kFrame) {
       if (p.bcL == SC_DIRICHLET) {
         const T bs = tab[0].BS[j] + (j == 0 ? d.alphaD : T(0));
-        if (bs != T(0)) s += pre * (bs * (p.mu[k] * d.imet[0])) * u[k];
+        if (bs != T(0)) cy[8 - j] += pre * (bs * (mu[k] * d.imet[0]));
       } else if (j == 0) {
-        T dv = 0;
-        for (int q = 4; q < kBand; ++q) dv += tab[0].fD1[q] * u[(size_t)(q - 4) * Nz + k];
-        s += pre * (p.mu[k] * (d.imet[0] * dv));
+        for (int q = 4; q < kBand; ++q) cy[8 + q - 4] += pre * (mu[k] * (d.imet[0] * tab[0].fD1[q]));
       }
     }
     if (j >= Ny - kFrame) {
       const size_t IN = (size_t)(Ny - 1) * Nz + k;
       if (p.bcR == SC_DIRICHLET) {
         const T bs = tab[1].BS[j - (Ny - kBand)] + (j == Ny - 1 ? d.alphaD : T(0));
-        if (bs != T(0)) s += pre * (bs * (p.mu[IN] * d.imet[Ny - 1])) * u[IN];
+        if (bs != T(0)) cy[8 + (Ny - 1 - j)] += pre * (bs * (mu[IN] * d.imet[Ny - 1]));
       } else if (j == Ny - 1) {
-        T dv = 0;
-        for (int q = 0; q < 5; ++q) dv += tab[1].fD1[11 * kBand + q] * u[(size_t)(Ny - 5 + q) * Nz + k];
-        s -= pre * (p.mu[IN] * (d.imet[Ny - 1] * dv));
+        for (int q = 0; q < 5; ++q)
+          cy[8 + q - 4] -= pre * (mu[IN] * (d.imet[Ny - 1] * tab[1].fD1[11 * kBand + q]));
       }
     }
   }
   if (p.doZ) {
-    const DevDir<T>& d = p.z;
     const T pre = p.y.H[j] * p.y.met[j];
+    const T* im = p.z.imet;
     const size_t row = (size_t)j * Nz;
+    if (k < kFrame || k >= Nz - kFrame) {
+      const bool lastRow = j + 1 >= Ny;
+      T cw[kBand], c3w[kBand];
+      for (int o = 0; o < kBand; ++o) {  // as along y, within row j
+        const int m = min(max(k + o - 4, 0), Nz - 1);
+        const bool inRow = m + 1 < Nz;
+        const size_t In = inRow ? row + m + 1 : (lastRow ? row + m : row + Nz);
+        const int kn = inRow ? m + 1 : (lastRow ? m : 0);
+        const T cm = mu[row + m] * im[m];
+        cw[o] = cm;
+        c3w[o] = T(0.5) * (cm + mu[In] * im[kn]);
+      }
+      closure_coefs<T, ORDER>(tab[k >= kFrame ? 3 : 2], Nz, k, cw, c3w, pre, cz);
+    } else {
+      T cs[5], c3s[5];
+      for (int sf = 0; sf < 5; ++sf) {  // flux columns k-2..k+2
+        const int m = k - 2 + sf;
+        const T cm = mu[row + m] * im[m];
+        cs[sf] = cm;
+        c3s[sf] = T(0.5) * (cm + mu[row + m + 1] * im[m + 1]);
+      }
+      interior_coefs<T, ORDER>(p.z, cs, c3s, pre, cz);
+    }
+    const DevDir<T>& d = p.z;
     if (k < kFrame) {
       if (p.bcT == SC_DIRICHLET) {
         const T bs = tab[2].BS[k] + (k == 0 ? d.alphaD : T(0));
-        if (bs != T(0)) s += pre * (bs * (p.mu[row] * d.imet[0])) * u[row];
+        if (bs != T(0)) cz[8 - k] += pre * (bs * (mu[row] * d.imet[0]));
       } else if (k == 0) {
-        T dv = 0;
-        for (int q = 4; q < kBand; ++q) dv += tab[2].fD1[q] * u[row + q - 4];
-        s += pre * (p.mu[row] * (d.imet[0] * dv));
+        for (int q = 4; q < kBand; ++q) cz[8 + q - 4] += pre * (mu[row] * (d.imet[0] * tab[2].fD1[q]));
       }
     }
     if (k >= Nz - kFrame) {
       if (p.bcB == SC_DIRICHLET) {
         const T bs = tab[3].BS[k - (Nz - kBand)] + (k == Nz - 1 ? d.alphaD : T(0));
-        if (bs != T(0)) s += pre * (bs * (p.mu[row + Nz - 1] * d.imet[Nz - 1])) * u[row + Nz - 1];
+        if (bs != T(0)) cz[8 + (Nz - 1 - k)] += pre * (bs * (mu[row + Nz - 1] * d.imet[Nz - 1]));
       } else if (k == Nz - 1) {
-        T dv = 0;
-        for (int q = 0; q < 5; ++q) dv += tab[3].fD1[11 * kBand + q] * u[row + Nz - 5 + q];
-        s -= pre * (p.mu[row + Nz - 1] * (d.imet[Nz - 1] * dv));
+        for (int q = 0; q < 5; ++q)
+          cz[8 + q - 4] -= pre * (mu[row + Nz - 1] * (d.imet[Nz - 1] * tab[3].fD1[11 * kBand + q]));
       }
     }
   }
-  return s;
+  for (int r = 0; r < kCross; ++r) {
+    coef[(size_t)r * nf + t] = cy[r];
+    coef[(size_t)(kCross + r) * nf + t] = cz[r];
+  }
 }
 
-// Frame kernel for grids with both sides above 2*kFrame nodes: one thread
-// per frame node (FrameMap order), closure tables of both directions and
-// both ends staged in LDS once per block.
+template <typename T, class EP>
+__global__ __launch_bounds__(256) void k_apply_frame_cf(DevOp<T> p, FrameMap fm, const T* __restrict__ coef,
+                                                       const T* __restrict__ u, EP ep) {
+  frame_cf_node(p, fm, coef, u, ep, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
 
-template <typename T, int ORDER, class EP>
-__global__ __launch_bounds__(kFrameThreads) void k_apply_frame_lds(DevOp<T> p, FrameMap fm,
-                                                                   const T* __restrict__ u, EP ep) {
-  __shared__ __align__(16) ClosureTab<T> tab[4];  // y low, y high, z low, z high
-  stage_closures(tab, p.ctab);
-  __syncthreads();
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= fm.count()) return;
-  int j, k;
-  fm.node(t, j, k);
-  const int Ny = p.Ny, Nz = p.Nz;
-  T v = 0;
-  if (p.doY) {
-    const bool yb = j < kFrame || j >= Ny - kFrame;
-    const T Ly = yb ? y_closure<T, ORDER>(p, tab[j >= kFrame ? 1 : 0], u, j, k)
-                    : y_interior<T, ORDER>(p, u, j, k);
-    v += (p.z.H[k] * p.z.met[k]) * Ly;
-  }
-  if (p.doZ) {
-    const bool zb = k < kFrame || k >= Nz - kFrame;
-    const T Lz = zb ? z_closure<T, ORDER>(p, tab[k >= kFrame ? 3 : 2], u, j, k)
-                    : z_interior<T, ORDER>(p, u, j, k);
-    v += (p.y.H[j] * p.y.met[j]) * Lz;
-  }
-  const size_t I = (size_t)j * Nz + k;
-  ep(j, k, I, v + sat_frame(p, tab, u, j, k), u[I], true);
+template <typename T>
+static int build_frame_coefs(const Ctx& c, const DevOp<T>& op, void*& buf) {
+  const int Ny = c.d.Ny, Nz = c.d.Nz;
+  if (!(Ny > 2 * kFrame && Nz > 2 * kFrame)) return SC_OK;
+  const FrameMap fm = FrameMap::make(Ny, Nz);
+  const long long nf = fm.count();
+  if (!buf) SC_HIP(hipMalloc(&buf, (size_t)kFrameCoefs * nf * sizeof(T)));
+  const unsigned nb = (unsigned)((nf + kFrameThreads - 1) / kFrameThreads);
+  if (c.d.order == 4)
+    hipLaunchKernelGGL((k_frame_coefs<T, 4>), dim3(nb), dim3(kFrameThreads), 0, c.stream, op, fm, (T*)buf);
+  else
+    hipLaunchKernelGGL((k_frame_coefs<T, 2>), dim3(nb), dim3(kFrameThreads), 0, c.stream, op, fm, (T*)buf);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
+__global__ __launch_bounds__(256) void k_round_coefs(const double* __restrict__ a, float* __restrict__ b,
+                                                     long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+// The FP32 coefficients are the FP64 ones rounded (closer to the operator
+// than the same algebra in FP32 would be).
+int ctx_build_frame(Ctx& c) {
+  int e = build_frame_coefs<double>(c, c.op64, c.fcoef64);
+  if (e || !c.fcoef64) return e;
+  const long long n = (long long)kFrameCoefs * FrameMap::make(c.d.Ny, c.d.Nz).count();
+  if (!c.fcoef32) SC_HIP(hipMalloc(&c.fcoef32, (size_t)n * sizeof(float)));
+  hipLaunchKernelGGL(k_round_coefs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream,
+                     (const double*)c.fcoef64, (float*)c.fcoef32, n);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
+template <typename T>
+static const T* frame_coefs(const Ctx& c) {
+  if constexpr (sizeof(T) == 8) return (const T*)c.fcoef64;
+  else return (const T*)c.fcoef32;
 }
 
 template <typename T, class EP>
@@ -676,23 +705,18 @@ static int launch_apply_t(const Ctx& c, const DevOp<T>& op, const T* in, const E
   const int Ny = c.d.Ny, Nz = c.d.Nz;
   FrameMap fm = FrameMap::make(Ny, Nz);
   const long long nf = fm.count();
-  if (Ny > 2 * kFrame && Nz > 2 * kFrame) {
-    const unsigned nb = (unsigned)((nf + kFrameThreads - 1) / kFrameThreads);
+  if (Ny > 2 * kFrame && Nz > 2 * kFrame) {  // frame and interior in one launch
+    const int nbx = (Nz - 2 * kFrame + kLanesOut - 1) / kLanesOut, nby = (Ny - 2 * kFrame + kJC - 1) / kJC;
+    const int nfb = (int)((nf + 64 * kXcd - 1) / (64 * kXcd)) * kXcd;
+    const T* cf = frame_coefs<T>(c);
+    const dim3 g((unsigned)(nfb + nbx * nby));
     if (c.d.order == 4)
-      hipLaunchKernelGGL((k_apply_frame_lds<T, 4, EP>), dim3(nb), dim3(kFrameThreads), 0, s, op, fm, in, ep);
+      hipLaunchKernelGGL((k_apply_interior<T, 4, EP>), g, dim3(64), 0, s, op, in, ep, nbx, nby, fm, cf, nfb);
     else
-      hipLaunchKernelGGL((k_apply_frame_lds<T, 2, EP>), dim3(nb), dim3(kFrameThreads), 0, s, op, fm, in, ep);
-  } else if (nf > 0) {
+      hipLaunchKernelGGL((k_apply_interior<T, 2, EP>), g, dim3(64), 0, s, op, in, ep, nbx, nby, fm, cf, nfb);
+  } else if (nf > 0) {  // no interior block: every node is a frame node
     const unsigned nb = (unsigned)((nf + 255) / 256);
     hipLaunchKernelGGL((k_apply_frame<T, EP>), dim3(nb), dim3(256), 0, s, op, fm, in, ep);
-  }
-  const int niy = Ny - 2 * kFrame, niz = Nz - 2 * kFrame;
-  if (niy > 0 && niz > 0) {
-    const int nbx = (niz + kLanesOut - 1) / kLanesOut, nby = (niy + kJC - 1) / kJC;
-    if (c.d.order == 4)
-      hipLaunchKernelGGL((k_apply_interior<T, 4, EP>), dim3(nbx * nby), dim3(64), 0, s, op, in, ep, nbx, nby);
-    else
-      hipLaunchKernelGGL((k_apply_interior<T, 2, EP>), dim3(nbx * nby), dim3(64), 0, s, op, in, ep, nbx, nby);
   }
   SC_HIP(hipGetLastError());
   return SC_OK;
@@ -898,21 +922,17 @@ static void bench_interior(const Ctx& c, const DevOp<T>& op, const T* a, T* b) {
   const int nbx = (niz + kLanesOut - 1) / kLanesOut, nby = (niy + kJC - 1) / kJC;
   if (c.d.order == 4)
     hipLaunchKernelGGL((k_apply_interior<T, 4, StoreAu<T>, XCD>), dim3(nbx * nby), dim3(64), 0, c.stream,
-                       op, a, StoreAu<T>{b}, nbx, nby);
+                       op, a, StoreAu<T>{b}, nbx, nby, FrameMap{}, (const T*)nullptr, 0);
   else
     hipLaunchKernelGGL((k_apply_interior<T, 2, StoreAu<T>, XCD>), dim3(nbx * nby), dim3(64), 0, c.stream,
-                       op, a, StoreAu<T>{b}, nbx, nby);
+                       op, a, StoreAu<T>{b}, nbx, nby, FrameMap{}, (const T*)nullptr, 0);
 }
 template <typename T>
 static void bench_frame(const Ctx& c, const DevOp<T>& op, const T* a, T* b) {
   FrameMap fm = FrameMap::make(c.d.Ny, c.d.Nz);
-  const unsigned nb = (unsigned)((fm.count() + kFrameThreads - 1) / kFrameThreads);
-  if (c.d.order == 4)
-    hipLaunchKernelGGL((k_apply_frame_lds<T, 4, StoreAu<T>>), dim3(nb), dim3(kFrameThreads), 0, c.stream,
-                       op, fm, a, StoreAu<T>{b});
-  else
-    hipLaunchKernelGGL((k_apply_frame_lds<T, 2, StoreAu<T>>), dim3(nb), dim3(kFrameThreads), 0, c.stream,
-                       op, fm, a, StoreAu<T>{b});
+  const unsigned nb = (unsigned)((fm.count() + 255) / 256);
+  hipLaunchKernelGGL((k_apply_frame_cf<T, StoreAu<T>>), dim3(nb), dim3(256), 0, c.stream, op, fm,
+                     frame_coefs<T>(c), a, StoreAu<T>{b});
 }
 }  // namespace sc
 

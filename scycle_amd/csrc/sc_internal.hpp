@@ -53,7 +53,7 @@ struct DevDir {
 };
 
 // Boundary-closure tables of one direction and one end, in the layout the
-// frame kernel stages into LDS: flux rows mb..mb+11 (D1, DR, D4 and their
+// frame-coefficient kernel stages into LDS: flux rows mb..mb+11 (D1, DR, D4 and their
 // weights) and output rows jb..jb+7 (D1, DR^T, D4^T, H), with mb = jb = 0 at
 // the low end and mb = n-12, jb = n-8 at the high end, plus the SAT row of BS.
 template <typename T>
@@ -92,6 +92,8 @@ struct Ctx {
   float* mu32 = nullptr;
   void* tab64 = nullptr;
   void* tab32 = nullptr;
+  void* fcoef64 = nullptr;             // assembled frame operator, 34 x frame nodes
+  void* fcoef32 = nullptr;
   void* ctab64 = nullptr;              // packed closure tables (ClosureTab x4)
   void* ctab32 = nullptr;
   DevOp<double> op64{};
@@ -145,6 +147,8 @@ int launch_wave64(const Ctx& c, const WaveArgs<double>& a, hipStream_t s);
 int launch_wave32(const Ctx& c, const WaveArgs<float>& a, hipStream_t s);
 // frame_only: write only the nodes within kFrame of a boundary, for a
 // caller whose rhs is zero elsewhere
+// forms the frame coefficients of the current operator (after any change of mu or BC types)
+int ctx_build_frame(Ctx& c);
 int launch_set_rhs(const Ctx& c, double* rhs, const double* bcL, const double* bcR,
                    const double* bcT, const double* bcB, hipStream_t s, bool frame_only = false);
 // which: 0 Dy, 1 muxDy, 2 Dz, 3 muxDz, 4 H, 5 Hinv
