@@ -23,8 +23,9 @@ new solution onto the last --reuse solutions.  `value` is the fastest path
 whose fault slip stays within 1e-8 (relative) of fp64_ref over the window; the
 line names it (config.solve) and lists every path.
 
-Also on the line: the HBM roofline of the FP64 SBP-SAT apply (interior and
-frame kernels, 24 algorithmic bytes per node: read u, read mu, write A u),
+Also on the line: the HBM roofline of the FP64 SBP-SAT apply (one launch of
+interior tiles and frame blocks, 24 algorithmic bytes per node: read u, read
+mu, write A u),
 timed with HIP events on the context stream; `traffic` from the committed
 rocprofv3 FETCH_SIZE / WRITE_SIZE summary when present; the fully dynamic
 2001^2 rupture (C4) in FP64 and FP32-bulk steps/s; the discrete adjoint of
@@ -385,7 +386,7 @@ def main():
     a64 = rl["apply64"]
     roof_apply = dict(bound="hbm", achieved=a64["GBps"], peak=HBM_PEAK / 1e9, unit="GB/s",
                       frac=a64["GBps"] * 1e9 / HBM_PEAK, traffic=traffic_per_launch(),
-                      kernel="SBP-SAT apply FP64 (k_apply_frame_lds + k_apply_interior)",
+                      kernel="SBP-SAT apply FP64 (k_apply_interior: interior tiles + assembled-frame blocks, one launch)",
                       ms_per_launch=a64["ms"], algorithmic_bytes_per_launch=a64["bytes"],
                       parts={k: v for k, v in rl.items() if k != "apply64"})
     for q in mediators.values():
