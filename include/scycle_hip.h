@@ -297,6 +297,11 @@ typedef struct sc_qd_run {
    * 0 after a run of guesses that met the tolerance, 1 never (every solve
    * is checked on the host), 2 always. */
   int deferChecks;
+  /* PID error history at the start, errA[0] and errA[1] of the reference's
+   * OdeSolver (odeSolver.cpp:231-234, 610): 0 and 0 for a fresh start, the
+   * checkpointed chkpt_currErr / chkpt_prevErr on a restart
+   * (strikeSlip_linearElastic_qd.cpp:734-737). */
+  double currErr, prevErr;
 } sc_qd_run;
 
 typedef struct sc_qd_counters {
@@ -316,7 +321,12 @@ int sc_qd_create(const sc_qd_desc* desc, const double* mu_h, const double* y_h,
                  const double* sN_h, const double* eta_h, const double* locked_h,
                  const double* prestress_h, const double* bcRShift_h, sc_qd** out);
 /* Reads a reference .in deck (Domain/LinearElastic/Fault/mediator keys,
- * binary Vec overrides from inputDir) and builds the same mediator. */
+ * binary Vec overrides from inputDir) and builds the same mediator.  A deck
+ * that names outputDir also sets the output (sc_qd_set_output with its
+ * stride1D, stride2D, enableCheckpointing, interval; reference defaults 1, 1,
+ * 0, 10000); when that directory holds a checkpoint the mediator starts from
+ * it (sc_qd_load_checkpoint into the deck's run, no solveSS), as the
+ * reference's constructor does. */
 int sc_qd_create_from_file(const char* in_path, sc_qd** out);
 int sc_qd_destroy(sc_qd* q);
 sc_ctx* sc_qd_ctx(sc_qd* q);
@@ -343,6 +353,55 @@ int sc_qd_file_run(sc_qd* q, sc_qd_run* run_out);
  * slip from the solution, back to the configured BC types.  psi_h in/out,
  * slip_h out (Nz); also sets the driver state (psi, slip, V = vL). */
 int sc_qd_solve_ss(sc_qd* q, double* psi_h, int have_state_vals, double* slip_h);
+
+/* Output series and checkpoints of the linear-elastic QD mediator
+ * (StrikeSlip_LinearElastic_qd::timeMonitor, strikeSlip_linearElastic_qd.cpp
+ * :417-463).  With an output set, sc_qd_integrate calls the monitor before
+ * its first step (step 0, with the fields as they are) and after every
+ * accepted step:
+ *   stride1D > 0, step % stride1D == 0 or t == maxTime: one record appended
+ *     to each PETSc-binary series outputDir+name: slip, slipVel, tauP, tauQSP,
+ *     strength, psi, sNEff (Fault::writeStep fault.cpp:512-557), surfDisp,
+ *     momBal_bcR/T/L/B and momBal_bcRShift (LinearElastic::writeStep1D
+ *     linearElastic.cpp:623-652; as there, the bcRShift series holds bcRShift
+ *     in its first record and bcR after it), and a line "%.15e" to
+ *     med_time1D.txt and med_dt1D.txt (:466-495);
+ *   stride2D > 0, likewise: momBal_u, momBal_sxy (N, z-fast;
+ *     linearElastic.cpp:656-686), med_time2D.txt, med_dt2D.txt;
+ *   enableCheckpointing > 0, step % interval == 0, step >= maxSteps or
+ *     t >= maxTime: ckptNumber += 1 and the checkpoint files (ASCII
+ *     ckptNumber, chkpt_currT, chkpt_stepCount, chkpt_prevErr, chkpt_currErr,
+ *     chkpt_deltaT; PETSc-binary chkpt_slip, chkpt_slipVel, chkpt_psi,
+ *     chkpt_sNEff, chkpt_sN, chkpt_tau, chkpt_tauQS, chkpt_prestress,
+ *     chkpt_momBal_bcR, chkpt_momBal_bcRShift, chkpt_momBal_bcT,
+ *     chkpt_momBal_bcL, chkpt_momBal_bcB; :527-557, fault.cpp:561-584,
+ *     linearElastic.cpp:689-705), after which the integration stops (not at
+ *     step 0, whose stop request the reference ignores).
+ * Each record is of the rates at the accepted state, with any deferred guess
+ * check settled first.  Series start truncated, or appended when
+ * enableCheckpointing > 0 and outputDir+ckptNumber holds a number > 0
+ * (Domain, domain.cpp:26-28).  o = NULL switches output off.  Not available
+ * for the power-law mediator (SC_ERR_ARG). */
+typedef struct sc_qd_output {
+  const char* outputDir;      /* prefix of every file name                   */
+  int stride1D, stride2D;     /* <= 0: that series is not written            */
+  int enableCheckpointing;    /* > 0: checkpoints every `interval` steps     */
+  int interval;               /* >= 1 with checkpoints                       */
+} sc_qd_output;
+int sc_qd_set_output(sc_qd* q, const sc_qd_output* o);
+/* The restart half of the checkpoint (the mediator's constructor,
+ * strikeSlip_linearElastic_qd.cpp:32-39, Fault::loadCheckpoint
+ * fault.cpp:144-176, LinearElastic::loadCheckpoint linearElastic.cpp:396-420):
+ * with an output set whose ckptNumber is > 0, loads psi, slip = bcL *
+ * faultTypeScale (initiateIntegrand takes slip from bcL, :391-395), sN(Eff),
+ * prestress, tauQS, tau, bcRShift and bcR/T/L/B, and sets run->initTime,
+ * initDt, currErr, prevErr from the ASCII files.  The saved slip velocity is
+ * taken as the first root-solve start (the reference restarts that from its
+ * set-up value; either start gives the root to rootTol).  The step count
+ * restarts at 0: the reference's step-0 monitor call overwrites the loaded
+ * count before it reaches the solver (:732-737).  *ckptNumber gets the number
+ * (0: nothing loaded). */
+int sc_qd_load_checkpoint(sc_qd* q, sc_qd_run* run, int* ckptNumber);
 
 /* Power-law viscoelastic QD: StrikeSlip_PowerLaw_qd (strikeSlip_powerLaw_qd.cpp
  * :633-686, 787-823) with PowerLaw in its dislocation-creep form.  Same

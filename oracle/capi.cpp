@@ -168,6 +168,7 @@ struct OrcQdRun {
   double initTime, maxTime, initDt, minDt, maxDt, tol;
   int nReplay;
   const double* replayDt;
+  double currErr, prevErr;  // PID error history at the start (odeSolver errA[0], errA[1])
 };
 
 }  // extern "C"
@@ -280,6 +281,8 @@ int orc_pl_integrate(void* h, const OrcQdRun* r, int errMask, double* psi, doubl
   RkSolver rk(r->integrator == 0 ? 3 : 4, r->maxSteps, r->maxTime, r->initDt,
               r->control == 0 ? "P" : "PID");
   rk.initT = rk.currT = r->initTime;
+  rk.errA[0] = r->currErr;
+  rk.errA[1] = r->prevErr;
   rk.totTol = r->tol;
   rk.minDeltaT = r->minDt;
   rk.maxDeltaT = r->maxDt;
@@ -339,6 +342,8 @@ int orc_qd_integrate(void* h, const OrcQdRun* r, double* psi, double* slip, doub
   RkSolver rk(r->integrator == 0 ? 3 : 4, r->maxSteps, r->maxTime, r->initDt,
               r->control == 0 ? "P" : "PID");
   rk.initT = rk.currT = r->initTime;
+  rk.errA[0] = r->currErr;
+  rk.errA[1] = r->prevErr;
   rk.totTol = r->tol;
   rk.minDeltaT = r->minDt;
   rk.maxDeltaT = r->maxDt;

@@ -82,7 +82,12 @@ class QdRun(C.Structure):
                 ("maxSteps", C.c_int), ("initTime", C.c_double), ("maxTime", C.c_double),
                 ("initDt", C.c_double), ("minDt", C.c_double), ("maxDt", C.c_double),
                 ("tol", C.c_double), ("nReplay", C.c_int), ("replayDt_h", _dp),
-                ("deferChecks", C.c_int)]
+                ("deferChecks", C.c_int), ("currErr", C.c_double), ("prevErr", C.c_double)]
+
+
+class QdOutput(C.Structure):
+    _fields_ = [("outputDir", C.c_char_p), ("stride1D", C.c_int), ("stride2D", C.c_int),
+                ("enableCheckpointing", C.c_int), ("interval", C.c_int)]
 
 
 class QdCounters(C.Structure):
@@ -162,6 +167,8 @@ SIGNATURES = {
     "sc_qd_get_counters": (C.c_int, [_vp, C.POINTER(QdCounters)]),
     "sc_qd_file_run": (C.c_int, [_vp, C.POINTER(QdRun)]),
     "sc_qd_solve_ss": (C.c_int, [_vp, _dp, C.c_int, _dp]),
+    "sc_qd_set_output": (C.c_int, [_vp, C.POINTER(QdOutput)]),
+    "sc_qd_load_checkpoint": (C.c_int, [_vp, C.POINTER(QdRun), C.POINTER(C.c_int)]),
     "sc_in_get": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]),
     "sc_in_get_vector": (C.c_int, [C.c_char_p, C.c_char_p, _dp, C.c_int, C.POINTER(C.c_int)]),
     "sc_vec_count": (C.c_int, [C.c_char_p, C.POINTER(C.c_int)]),

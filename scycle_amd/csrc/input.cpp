@@ -163,7 +163,11 @@ int load_deck(const std::string& path, Deck& d) {
     else if (v == "Ly") d.Ly = std::atof(r.c_str());
     else if (v == "Lz") d.Lz = std::atof(r.c_str());
     else if (v == "inputDir") d.inputDir = r;
-    else if (v == "outputDir") d.outputDir = r;
+    else if (v == "outputDir") { d.outputDir = r; d.haveOutputDir = true; }
+    else if (v == "enableCheckpointing") d.ckpt = std::atoi(r.c_str());
+    else if (v == "interval") d.interval = (int)std::atof(r.c_str());
+    else if (v == "stride1D") d.stride1D = (int)std::atof(r.c_str());
+    else if (v == "stride2D") d.stride2D = (int)std::atof(r.c_str());
     else if (v == "sbpCompatibilityType") d.compat = r;
     else if (v == "gridSpacingType") d.gridSpacingType = r;
     else if (v == "bulkDeformationType") d.bulk = r;
@@ -435,6 +439,21 @@ extern "C" int sc_qd_create_from_file(const char* in_path, sc_qd** out) {
   sc::QdDriver& q = *(*out)->q;
   q.file_run = in.run;
   q.have_file_run = true;
+  if (d.haveOutputDir) {
+    // the deck's output series and checkpoints; a checkpoint in the output
+    // directory replaces the initial conditions (and solveSS) with its state
+    const sc_qd_output o{d.outputDir.c_str(), d.stride1D, d.stride2D, d.ckpt, d.interval};
+    int n = 0;
+    e = q.set_output(&o);
+    if (!e) e = q.load_checkpoint(&q.file_run, &n);
+    if (e || n > 0) {
+      if (e) {
+        sc_qd_destroy(*out);
+        *out = nullptr;
+      }
+      return e;
+    }
+  }
   std::vector<double> V(d.Nz, 0.0);
   if (in.guessSS) {
     e = sc::solve_ss(q, in.psi, in.haveStateVals, in.slip, V);

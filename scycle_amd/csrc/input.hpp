@@ -41,6 +41,10 @@ struct Deck {
   std::string bcR = "remoteLoading", bcT = "freeSurface", bcL = "symmFault", bcB = "freeSurface";
   std::string linSolver = "MUMPSCHOLESKY";
   double kspTol = 1e-10;
+  // output and checkpoints (strikeSlip_linearElastic_qd.cpp:14,167-168;
+  // domain.cpp:17,187-188)
+  bool haveOutputDir = false;
+  int stride1D = 1, stride2D = 1, ckpt = 0, interval = 10000;
 };
 
 struct QdInputs {

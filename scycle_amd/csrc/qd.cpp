@@ -386,12 +386,19 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
   r.maxDt = run.maxDt;
   r.deltaT = run.initDt;
   r.currT = run.initTime;
+  r.errA[0] = run.currErr;
+  r.errA[1] = run.prevErr;
   const double finalT = run.maxTime;
   int maxSteps = run.maxSteps;
   const bool replay = run.nReplay > 0 && run.replayDt_h;
   if (replay && maxSteps > run.nReplay) maxSteps = run.nReplay;
   hipStream_t s = ctx->c.stream;
   int e;
+  if (out) {  // timeMonitor(initTime, initDeltaT, 0) before the solver starts (:729-732)
+    const double zero[2] = {0.0, 0.0};
+    int stop = 0;  // ignored, as there
+    if ((e = monitor(r.currT, r.deltaT, 0, zero, r.deltaT, run.maxSteps, finalT, true, &stop))) return e;
+  }
   if (finalT == r.currT || maxSteps == 0) {
     *tEnd = r.currT;
     *steps = 0;
@@ -485,22 +492,39 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
     r.errA[1] = r.errA[0];
     r.errA[0] = totErr;
     if (dts && nOut < dtsCap) dts[nOut++] = r.deltaT;
+    if (out) {
+      int stop = 0;
+      if ((e = monitor(r.currT, r.deltaT, r.stepCount, r.errA, newDt, run.maxSteps, finalT, false, &stop)))
+        return e;
+      if (stop) break;
+    }
     r.deltaT = newDt;
   }
   defer = false;
   // the rates at the final state: check their guesses too
-  SC_HIP(hipMemcpyAsync(host_scratch, status_d, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-  SC_HIP(hipStreamSynchronize(s));
-  note_checked(((const int*)host_scratch)[1] != 0);
-  if (((const int*)host_scratch)[1]) {
-    counters.missed_guesses++;
-    SC_HIP(hipMemsetAsync(status_d, 0, 2 * sizeof(int), s));
-    if ((e = d_dt_vars(r.currT))) return e;
-    SC_HIP(hipStreamSynchronize(s));
-  }
+  if ((e = settle(r.currT))) return e;
   if ((e = check_status())) return e;
   *tEnd = r.currT;
   *steps = r.stepCount;
+  return SC_OK;
+}
+
+int QdDriver::settle(double t) {
+  hipStream_t s = ctx->c.stream;
+  SC_HIP(hipMemcpyAsync(host_scratch, status_d, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  SC_HIP(hipStreamSynchronize(s));
+  const bool missed = ((const int*)host_scratch)[1] != 0;
+  note_checked(missed);
+  if (missed) {
+    counters.missed_guesses++;
+    SC_HIP(hipMemsetAsync(status_d, 0, 2 * sizeof(int), s));
+    const bool d = defer;
+    defer = false;
+    const int e = d_dt_vars(t);
+    defer = d;
+    if (e) return e;
+    SC_HIP(hipStreamSynchronize(s));
+  }
   return SC_OK;
 }
 

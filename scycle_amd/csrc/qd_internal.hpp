@@ -65,6 +65,16 @@ struct PowerLawState {
   double *effVisc = nullptr, *sxy = nullptr;                       // last d_dt
 };
 
+// Output series and checkpoints (output.cpp)
+struct QdOutput {
+  std::string dir;
+  int stride1D = 0, stride2D = 0, ckpt = 0, interval = 1;
+  int ckptNumber = 0;
+  bool append = false;                  // series start appended (a restart), else truncated
+  bool started1D = false, started2D = false;
+  double* sxy_d = nullptr;              // muxDy u for the 2-D series (N)
+};
+
 struct QdDriver {
   sc_qd_desc desc{};
   sc_ctx* ctx = nullptr;
@@ -103,6 +113,7 @@ struct QdDriver {
   std::vector<double> mu_host, y_host, z_host;  // grid and modulus, for a second context
   sc_qd_run file_run{};
   bool have_file_run = false;
+  std::unique_ptr<QdOutput> out;       // set: integrate calls monitor()
 
   ~QdDriver();
   double* dalloc(size_t n, const double* init_h);
@@ -128,6 +139,15 @@ struct QdDriver {
   int lincomb_all(const char* out_sel, int stage, int nt, const int* fidx, const double* c);
   int stage(int st, double t);
   int integrate(const sc_qd_run& run, double* tEnd, int* steps, double* dts, int dtsCap);
+  // the rates at time t were computed with deferred guess checks: read the
+  // flags and redo them with checked solves if a guess missed
+  int settle(double t);
+  // timeMonitor (output.cpp): writes what is due at this step; *stop set by
+  // a checkpoint.  initial: the call before the first step (no rates to settle)
+  int monitor(double t, double dt, int step, const double* errA, double newDt, int maxSteps,
+              double maxTime, bool initial, int* stop);
+  int set_output(const sc_qd_output* o);
+  int load_checkpoint(sc_qd_run* run, int* ckptNumber);
 };
 
 }  // namespace sc

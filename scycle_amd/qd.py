@@ -5,8 +5,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (AdjDesc, AdjStats, GridDesc, QdCounters, QdDesc, QdRun, SolveOpts, check, dptr,
-                   load)
+from ._lib import (AdjDesc, AdjStats, GridDesc, QdCounters, QdDesc, QdOutput, QdRun, SolveOpts, check,
+                   dptr, load)
 
 SOLVE_MODES = {"cg64": 0, "ir32": 1}
 
@@ -70,19 +70,45 @@ class QdMediator:
 
     def integrate(self, max_steps, integrator="RK32", tol=1e-8, init_dt=1e-3, min_dt=0.0,
                   max_dt=1e10, init_time=0.0, max_time=1e15, control="PID",
-                  norm="L2_absolute", replay_dt=None, defer="auto"):
-        """defer: "auto", "never" or "always" (sc_qd_run.deferChecks)."""
+                  norm="L2_absolute", replay_dt=None, defer="auto", curr_err=0.0, prev_err=0.0):
+        """defer: "auto", "never" or "always" (sc_qd_run.deferChecks);
+        curr_err, prev_err: the PID error history at the start (a restart's
+        chkpt_currErr / chkpt_prevErr, see load_checkpoint)."""
         rep = None if replay_dt is None else np.ascontiguousarray(replay_dt, dtype=np.float64)
         r = QdRun(0 if integrator == "RK32" else 1, 0 if control == "P" else 1,
                   0 if norm == "L2_absolute" else 1, int(max_steps), init_time, max_time,
                   init_dt, min_dt, max_dt, tol, 0 if rep is None else len(rep), dptr(rep),
-                  {"auto": 0, "never": 1, "always": 2}[defer])
+                  {"auto": 0, "never": 1, "always": 2}[defer], curr_err, prev_err)
         tEnd = C.c_double()
         steps = C.c_int()
         dts = np.zeros(max(int(max_steps), 1))
         check(load().sc_qd_integrate(self.h, C.byref(r), C.byref(tEnd), C.byref(steps),
                                      dptr(dts), len(dts)))
         return tEnd.value, steps.value, dts[:steps.value].copy()
+
+    # ------------------------------------------- output and checkpoints --
+    def set_output(self, output_dir, stride1D=1, stride2D=0, checkpoint=0, interval=10000):
+        """Output series and checkpoints written by integrate (timeMonitor of
+        strikeSlip_linearElastic_qd.cpp:417-463); output_dir is the prefix of
+        every file name.  None switches output off."""
+        if output_dir is None:
+            check(load().sc_qd_set_output(self.h, None))
+            return
+        self._out_dir = str(output_dir).encode()
+        o = QdOutput(self._out_dir, int(stride1D), int(stride2D), int(checkpoint), int(interval))
+        check(load().sc_qd_set_output(self.h, C.byref(o)))
+
+    def load_checkpoint(self):
+        """Restore the state of the last checkpoint under the output prefix;
+        returns None when there is none, else the integrate() arguments it
+        sets: init_time, init_dt, curr_err, prev_err (and ckpt_number)."""
+        r = QdRun()
+        n = C.c_int()
+        check(load().sc_qd_load_checkpoint(self.h, C.byref(r), C.byref(n)))
+        if n.value == 0:
+            return None
+        return dict(init_time=r.initTime, init_dt=r.initDt, curr_err=r.currErr, prev_err=r.prevErr,
+                    ckpt_number=n.value)
 
     # -------------------------------------------------- discrete adjoint --
     def set_friction(self, a=None, b=None, Dc=None):

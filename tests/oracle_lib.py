@@ -274,7 +274,8 @@ class OrcQdRun(C.Structure):
     _fields_ = [("integrator", C.c_int), ("control", C.c_int), ("normType", C.c_int),
                 ("maxSteps", C.c_int), ("initTime", C.c_double), ("maxTime", C.c_double),
                 ("initDt", C.c_double), ("minDt", C.c_double), ("maxDt", C.c_double),
-                ("tol", C.c_double), ("nReplay", C.c_int), ("replayDt", _dp)]
+                ("tol", C.c_double), ("nReplay", C.c_int), ("replayDt", _dp),
+                ("currErr", C.c_double), ("prevErr", C.c_double)]
 
 
 class PowerLawOracle:

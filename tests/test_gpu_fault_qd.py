@@ -139,7 +139,8 @@ class _Run(C.Structure):
     _fields_ = [("integrator", C.c_int), ("control", C.c_int), ("normType", C.c_int),
                 ("maxSteps", C.c_int), ("initTime", C.c_double), ("maxTime", C.c_double),
                 ("initDt", C.c_double), ("minDt", C.c_double), ("maxDt", C.c_double),
-                ("tol", C.c_double), ("nReplay", C.c_int), ("replayDt", C.c_void_p)]
+                ("tol", C.c_double), ("nReplay", C.c_int), ("replayDt", C.c_void_p),
+                ("currErr", C.c_double), ("prevErr", C.c_double)]
 
 
 def _fields_err(gs, ug, ref):
