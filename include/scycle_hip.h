@@ -403,6 +403,80 @@ int sc_qd_set_output(sc_qd* q, const sc_qd_output* o);
  * (0: nothing loaded). */
 int sc_qd_load_checkpoint(sc_qd* q, sc_qd_run* run, int* ckptNumber);
 
+/* ------------------------------------------------ QD <-> FD switching */
+/* strikeSlip_linearElastic_qd_fd (strikeSlip_linearElastic_qd_fd.cpp): QD
+ * phases (the QD mediator's d_dt, RK32/RK43) and FD phases (propagateWaves +
+ * Fault_fd::d_dt at the constant deltaT_fd) in turn, switched on
+ * R = max_k eta V / tauQS (checkSwitchRegime, :590-640): a QD phase may end
+ * once R has been below limit_qd and then exceeds trigger_qd2fd; an FD phase
+ * once R has been above limit_fd and then falls below trigger_fd2qd.
+ * Switching to FD takes one QD step of deltaT_fd (prepare_qd2fd, :807-889);
+ * the FD phase then integrates the displacement change du from the QD
+ * displacement u0 at that step's start, starting from rest (uPrev = u, as
+ * OdeSolver_WaveEq::setInitialConds(var) sets it), with slip0 = slip and
+ * tau0 = strength + prestress; switching back hands psi, slip and V to the
+ * QD phase (prepare_fd2qd, :747-805).  Each QD phase starts at dt =
+ * deltaT_fd; each phase may take maxStepCount more steps
+ * (setInitialStepCount).  Two reference behaviours are kept by default:
+ *   fdStop = 0: OdeSolver_WaveEq::integrate never reads the stop request of
+ *     the monitor (odeSolver_WaveEq.cpp:166-182), so an FD phase runs to
+ *     maxTime or maxStepCount; fdStop = 1 ends it on the switching criterion
+ *     (full cycles) and starts each QD phase's step-size controller afresh
+ *     (P control for its first four steps, as in a first phase): the
+ *     reference's new RK object would keep the global step count with an
+ *     empty error history, for which the PID ratio is NaN and a rejected
+ *     step grows fivefold (odeSolver.cpp:415-433);
+ *   ayCompat = 1: computePenaltyVectors of this mediator (:486-500) writes
+ *     a_y through an index that advances only on the fault row, so a_y holds
+ *     the top/bottom terms of the fault row's nodes, the last node's value at
+ *     index Nz, and zeros elsewhere (and no fault-side term); ayCompat = 0
+ *     uses the FD mediator's a_y (strikeSlip_linearElastic_fd.cpp:640-665).
+ * deltaT_fd: computeTimeStep (:356-443) from deltaT_fd and CFL (<= 0:
+ * unset; with CFL alone dt = CFL g_cfl^2 min(d/cs)). */
+typedef struct sc_qdfd sc_qdfd;
+typedef struct sc_qdfd_desc {
+  sc_qd_desc qd;              /* grid with the QD-phase material SAT types, friction, solve */
+  int fdBcR, fdBcT, fdBcL, fdBcB;  /* material SAT types of the FD phase (reference: Neumann) */
+  int outR, outT, outB, faultL;    /* a_y sides (momBal_bc*_fd outGoingCharacteristics; fault) */
+  int ayCompat;               /* 1: the reference's a_y of this mediator (see above) */
+  int bulkPrec;               /* SC_FP64 / SC_FP32 body in FD phases     */
+  double deltaT_fd, CFL;      /* <= 0: unset                              */
+  double trigger_qd2fd, trigger_fd2qd, limit_qd, limit_fd;  /* reference defaults 1e-3, 1e-3, 10 vL, 0.1 */
+  int maxNumCycles;
+  int fdStop;                 /* 0 as the reference, 1: FD phases end on trigger_fd2qd */
+} sc_qdfd_desc;
+typedef struct sc_qdfd_run {
+  int integrator, control, normType;  /* QD phases, as sc_qd_run */
+  int maxStepCount;           /* the mediator's maxStepCount             */
+  double initTime, maxTime, minDt, maxDt, tol;
+  int deferChecks;
+} sc_qdfd_run;
+/* one phase of a run: regime 0 QD, 1 FD, 2 the QD step of deltaT_fd that
+ * starts an FD phase; R the switching ratio at its last step */
+typedef struct sc_qdfd_phase {
+  int regime, stepStart, stepEnd;
+  double tStart, tEnd, R;
+} sc_qdfd_phase;
+/* host arrays: mu, rho, cs, y, z (N; y, z may be NULL for a constant grid),
+ * a, b, Dc, sN, eta (radiation damping), locked, prestress, bcRShift (Nz;
+ * the last three may be NULL = zero) */
+int sc_qdfd_create(const sc_qdfd_desc* desc, const double* mu_h, const double* rho_h,
+                   const double* cs_h, const double* y_h, const double* z_h, const double* a_h,
+                   const double* b_h, const double* Dc_h, const double* sN_h, const double* eta_h,
+                   const double* locked_h, const double* prestress_h, const double* bcRShift_h,
+                   sc_qdfd** out);
+int sc_qdfd_destroy(sc_qdfd* h);
+int sc_qdfd_time_step(sc_qdfd* h, double* deltaT_fd);
+/* the initial state (a QD state): psi, slip, V (Nz; V may be NULL = 0) */
+int sc_qdfd_set_state(sc_qdfd* h, const double* psi_h, const double* slip_h, const double* V_h);
+/* the current state of the current regime: psi, slip, V (Nz), the total
+ * displacement u (N) and the regime (0 QD, 1 FD); any may be NULL */
+int sc_qdfd_get_state(sc_qdfd* h, double* psi_h, double* slip_h, double* V_h, double* u_h, int* regime);
+/* strikeSlip_linearElastic_qd_fd::integrate (:511-580) from run->initTime;
+ * the phases go to log[0..logCap) and their number to *nPhases */
+int sc_qdfd_integrate(sc_qdfd* h, const sc_qdfd_run* run, double* tEnd, int* steps, int* cycles,
+                      sc_qdfd_phase* log, int logCap, int* nPhases);
+
 /* Power-law viscoelastic QD: StrikeSlip_PowerLaw_qd (strikeSlip_powerLaw_qd.cpp
  * :633-686, 787-823) with PowerLaw in its dislocation-creep form.  Same
  * arguments as sc_qd_create plus T, A, B (= Q/R), n (N each, z-fast) and the

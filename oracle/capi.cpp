@@ -413,6 +413,75 @@ int orc_fdfault_step(void* h, double dt, const double* d2u, double* V_inout, con
   return e;
 }
 
+// ------------------------------------------------------- QD <-> FD -----
+struct OrcQdFdCfg {
+  int fdBcR, fdBcT, fdBcL, fdBcB, outR, outT, outB, ayCompat;
+  double deltaT_fd, CFL, trigger_qd2fd, trigger_fd2qd, limit_qd, limit_fd;
+  int maxNumCycles, fdStop;
+};
+struct OrcQdFdRun {
+  int integrator, control, normType, maxStepCount;
+  double initTime, maxTime, minDt, maxDt, tol;
+};
+// qd: a model from orc_qd_new (not owned; freed by the caller after this one)
+void* orc_qdfd_new(void* qd, const OrcQdFdCfg* c, const double* mu, const double* y, const double* z,
+                   const double* rho, const double* cs) {
+  auto* m = new QdFdModel;
+  QdFdConfig cfg;
+  cfg.fdBcR = c->fdBcR; cfg.fdBcT = c->fdBcT; cfg.fdBcL = c->fdBcL; cfg.fdBcB = c->fdBcB;
+  cfg.outR = c->outR; cfg.outT = c->outT; cfg.outB = c->outB; cfg.ayCompat = c->ayCompat;
+  cfg.deltaT_fd = c->deltaT_fd; cfg.CFL = c->CFL;
+  cfg.trigger_qd2fd = c->trigger_qd2fd; cfg.trigger_fd2qd = c->trigger_fd2qd;
+  cfg.limit_qd = c->limit_qd; cfg.limit_fd = c->limit_fd;
+  cfg.maxNumCycles = c->maxNumCycles; cfg.fdStop = c->fdStop;
+  if (m->init((QdModel*)qd, cfg, mu, y, z, rho, cs)) {
+    delete m;
+    return nullptr;
+  }
+  return m;
+}
+void orc_qdfd_free(void* h) { delete (QdFdModel*)h; }
+double orc_qdfd_dt(void* h) { return ((QdFdModel*)h)->dtFd; }
+void orc_qdfd_ay(void* h, double* ay) {
+  auto* m = (QdFdModel*)h;
+  std::memcpy(ay, m->ay.data(), sizeof(double) * m->N);
+}
+// psi, slip, V: initial QD state in, final state out; u (N): total
+// displacement out; phases to log[0..cap) as (regime, stepStart, stepEnd,
+// tStart, tEnd, R).  Returns the number of phases or < 0.
+int orc_qdfd_integrate(void* h, const OrcQdFdRun* r, double* psi, double* slip, double* V, double* u,
+                       double* tEnd, int* steps, int* cycles, int* regime, QdFdPhase* log, int cap) {
+  auto* m = (QdFdModel*)h;
+  const int Nz = m->Nz, N = m->N;
+  m->qvar["psi"].assign(psi, psi + Nz);
+  m->qvar["slip"].assign(slip, slip + Nz);
+  m->qd->V.assign(V, V + Nz);
+  QdFdRunCfg rc;
+  rc.integrator = r->integrator; rc.control = r->control; rc.normType = r->normType;
+  rc.maxStepCount = r->maxStepCount; rc.initTime = r->initTime; rc.maxTime = r->maxTime;
+  rc.minDt = r->minDt; rc.maxDt = r->maxDt; rc.tol = r->tol;
+  const int e = m->integrate(rc);
+  if (e) return -e;
+  if (m->inDynamic) {
+    std::memcpy(psi, m->psiF[1].data(), sizeof(double) * Nz);
+    std::memcpy(slip, m->slipF[1].data(), sizeof(double) * Nz);
+    std::memcpy(V, m->ff.V.data(), sizeof(double) * Nz);
+    for (int i = 0; i < N; ++i) u[i] = m->du[i] + 1.0 * m->u0[i];
+  } else {
+    std::memcpy(psi, m->qvar["psi"].data(), sizeof(double) * Nz);
+    std::memcpy(slip, m->qvar["slip"].data(), sizeof(double) * Nz);
+    std::memcpy(V, m->qd->V.data(), sizeof(double) * Nz);
+    std::memcpy(u, m->qd->u.data(), sizeof(double) * N);
+  }
+  *tEnd = m->currT;
+  *steps = m->stepCount;
+  *cycles = m->cycleCount;
+  *regime = m->inDynamic ? 1 : 0;
+  const int n = (int)m->log.size();
+  for (int i = 0; i < n && i < cap; ++i) log[i] = m->log[i];
+  return n;
+}
+
 // Integrates y' = lam*y (two components: lam and lam2) with the RK32 / RK43
 // solver over n fixed steps of size dt (replay mode: no step-size control) or,
 // with n < 0, adaptively to finalT with tolerance tol.  Test hook for the

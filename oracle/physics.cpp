@@ -118,7 +118,7 @@ double RkSolver::computeStepSize(double totErr) {
     stepRatio = kappa * std::pow(totTol / totErr, 1. / (1. + ord));
   } else {
     double alpha = 0.49 / ord, beta = 0.34 / ord, gamma = 0.1 / ord;
-    if (stepCount < 4) stepRatio = kappa * std::pow(totTol / totErr, 1. / (1. + ord));
+    if (stepCount - pidStep0 < 4) stepRatio = kappa * std::pow(totTol / totErr, 1. / (1. + ord));
     else
       stepRatio = kappa * std::pow(totTol / totErr, alpha) * std::pow(errA[0] / totTol, beta) *
                   std::pow(totTol / errA[1], gamma);
@@ -283,6 +283,7 @@ int RkSolver::integrate(OdeCtx* obj) {
     errA[0] = totErr;
     dtHistory.push_back(deltaT);
     obj->timeMonitor(currT, deltaT, stepCount, stop);
+    if (stop > 0) break;  // odeSolver.cpp:614, 1161
     deltaT = newDeltaT;
   }
   return 0;
@@ -453,6 +454,263 @@ int FdFault::step(double dt, const double* psiCur, const double* psiPrev, const 
     strength[j] = strength_psi(sNEff[j], psi[j], V[j], a[j], v0);
   }
   return bad;
+}
+
+// ----------------------------------------------------- QD <-> FD -----
+static double csr_diag(const Csr& m, int i) {
+  for (int64_t p = m.rp[i]; p < m.rp[i + 1]; ++p)
+    if (m.ci[p] == i) return m.v[p];
+  return 0.0;
+}
+
+// constructor + computePenaltyVectors (:446-509) + computeTimeStep (:356-443)
+int QdFdModel::init(QdModel* q, const QdFdConfig& cfg, const double* mu, const double* y,
+                    const double* z, const double* rho_, const double* cs_) {
+  qd = q;
+  c = cfg;
+  Ny = q->Ny;
+  Nz = q->Nz;
+  N = q->N;
+  SbpDesc d = q->c.sbp;
+  d.bcR = c.fdBcR;
+  d.bcT = c.fdBcT;
+  d.bcL = c.fdBcL;
+  d.bcB = c.fdBcB;
+  int e = sbp2d_build(fs, d, mu, y, z);
+  if (e) return e;
+  rho.assign(rho_, rho_ + N);
+  cs.assign(cs_, cs_ + N);
+  ay.assign(N, 0.0);
+  if (c.ayCompat) {
+    // the loop as it runs: Jj advances only on the fault row (the if-statement
+    // of the fault side has Jj++ as its body)
+    int Jj = 0;
+    for (int Ii = 0; Ii < N; ++Ii) {
+      ay[Jj] = 0;
+      if ((Ii / Nz == Ny - 1) && c.outR) ay[Jj] += 0.5 / fs.h11y;
+      if ((Ii % Nz == 0) && c.outT) ay[Jj] += 0.5 / fs.h11z;
+      if (((Ii + 1) % Nz == 0) && c.outB) ay[Jj] += 0.5 / fs.h11z;
+      if (Ii / Nz == 0) Jj++;
+    }
+    for (int i = 0; i < N; ++i) ay[i] = ay[i] * cs[i];
+  } else {
+    fd_ay(fs, cs.data(), c.outR, c.outT, c.outB, 1, ay.data());
+  }
+  // FD fault: the QD fault's parameters; alphay = h11y (times yq on a
+  // variable grid) on the fault; rho of the fault column
+  ff.N = Nz;
+  ff.v0 = q->c.v0; ff.f0 = q->c.f0; ff.vL = q->c.vL; ff.rootTol = q->c.rootTol;
+  ff.faultTypeScale = q->c.faultTypeScale; ff.maxNumIts = q->c.maxNumIts; ff.stateLaw = q->c.stateLaw;
+  ff.a = q->a; ff.b = q->b; ff.Dc = q->Dc; ff.sNEff = q->sN; ff.locked = q->locked;
+  ff.rho.assign(rho.begin(), rho.begin() + Nz);
+  ff.alphay.assign(Nz, fs.h11y);
+  if (d.var_grid)
+    for (int k = 0; k < Nz; ++k) ff.alphay[k] = csr_diag(fs.yq, k) * fs.h11y;
+  ff.tau0.assign(Nz, 0.0); ff.slip0.assign(Nz, 0.0);
+  for (auto* v : {&ff.psi, &ff.slip, &ff.V, &ff.u, &ff.uPrev, &ff.d2u, &ff.an, &ff.Phi, &ff.fricPen,
+                  &ff.strength})
+    v->assign(Nz, 0.0);
+  for (int l = 0; l < 3; ++l) { psiF[l].assign(Nz, 0.0); slipF[l].assign(Nz, 0.0); }
+  u0.assign(N, 0.0); du.assign(N, 0.0); duPrev.assign(N, 0.0); duNext.assign(N, 0.0); d2u.assign(N, 0.0);
+  // computeTimeStep
+  const double gcfl = d.order == 4 ? 0.7071 / std::sqrt(1.4498) : 0.7071;
+  double minY = INFINITY, minZ = INFINITY;
+  for (int i = 0; i < N; ++i) {
+    const double dy = d.var_grid ? csr_diag(fs.yq, i) / (Ny - 1) : d.Ly / (Ny - 1.0);
+    const double dz = d.var_grid ? csr_diag(fs.zr, i) / (Nz - 1) : d.Lz / (Nz - 1.0);
+    minY = std::min(minY, dy / cs[i]);
+    minZ = std::min(minZ, dz / cs[i]);
+  }
+  const double maxDt = gcfl * std::min(std::fabs(minY), std::fabs(minZ));
+  const double cflDt = c.CFL * gcfl * maxDt;
+  const double req = c.deltaT_fd;
+  dtFd = maxDt;
+  if (req <= 0. && cflDt <= 0.) dtFd = maxDt;
+  else if (req > 0. && cflDt <= 0.) dtFd = req;
+  else if (req <= 0. && cflDt > 0.) dtFd = cflDt;
+  else dtFd = req;
+  // checkInput (:282-293)
+  if (c.limit_fd < c.trigger_qd2fd) c.limit_fd = 10 * c.trigger_qd2fd;
+  if (c.limit_qd > c.trigger_fd2qd) c.limit_qd = c.trigger_qd2fd / 10.0;
+  qvar["psi"].assign(Nz, 0.0);
+  qvar["slip"].assign(Nz, 0.0);
+  return 0;
+}
+
+// checkSwitchRegime (:585-640)
+bool QdFdModel::checkSwitch(double R) {
+  bool must = false;
+  if (inDynamic && !allowed && R > c.limit_fd) allowed = true;
+  if (!inDynamic && !allowed && R < c.limit_qd) allowed = true;
+  if (inDynamic && allowed && R < c.trigger_fd2qd) must = true;
+  if (!inDynamic && allowed && R > c.trigger_qd2fd) must = true;
+  return must;
+}
+
+// R = eta V / tauQSP, VecMax
+double QdFdModel::qdRatio() const {
+  double m = -INFINITY;
+  for (int k = 0; k < Nz; ++k) m = std::fmax(m, (qd->eta[k] * qd->V[k]) / qd->tauQS[k]);
+  return m;
+}
+
+// the FD phase's tauQSP = sxy(u + u0) on the fault + prestress (:1696-1708)
+double QdFdModel::fdRatio() const {
+  std::vector<double> tot(N), sxy(N);
+  for (int i = 0; i < N; ++i) tot[i] = du[i] + 1.0 * u0[i];
+  sbp2d_muxDy(fs, tot.data(), sxy.data());
+  double m = -INFINITY;
+  for (int k = 0; k < Nz; ++k) {
+    const double tq = sxy[k] + 1.0 * qd->prestress[k];
+    m = std::fmax(m, (qd->eta[k] * ff.V[k]) / tq);
+  }
+  return m;
+}
+
+namespace {
+struct QdPhaseCtx : OdeCtx {
+  QdFdModel* m = nullptr;
+  int d_dt(double t, const VarMap& var, VarMap& dvar) override { return m->qd->d_dt(t, var, dvar); }
+  int timeMonitor(double, double, int, int& stop) override {
+    m->lastR = m->qdRatio();
+    if (m->checkSwitch(m->lastR)) stop = 1;
+    return 0;
+  }
+};
+}  // namespace
+
+// integrate_qd (:1436-1509) / integrate_singleQDTimeStep (:891-957)
+int QdFdModel::qdPhase(const QdFdRunCfg& r, int regime, int maxSteps, double finalT, double minDt,
+                       double maxDt) {
+  QdPhaseCtx cx;
+  cx.m = this;
+  RkSolver rk(r.integrator == 0 ? 3 : 4, maxSteps, finalT, dtFd, r.control == 0 ? "P" : "PID");
+  rk.initT = rk.currT = currT;
+  rk.totTol = r.tol;
+  rk.minDeltaT = minDt;
+  rk.maxDeltaT = maxDt;
+  rk.normType = r.normType == 0 ? "L2_absolute" : "L2_relative";
+  rk.errInds = {"psi", "slip"};
+  rk.stepCount = stepCount;             // setInitialStepCount
+  rk.maxNumSteps = stepCount + maxSteps;
+  if (c.fdStop) rk.pidStep0 = stepCount; // a fresh controller for each QD phase
+  rk.var = qvar;
+  const int s0 = stepCount;
+  const double t0 = currT;
+  int e = rk.integrate(&cx);
+  if (e) return e;
+  qvar = rk.var;
+  currT = rk.currT;
+  stepCount = rk.stepCount;
+  log.push_back(QdFdPhase{regime, s0, stepCount, t0, currT, lastR});
+  return 0;
+}
+
+// prepare_fd2qd (:747-805)
+int QdFdModel::prepareFd2Qd() {
+  if (!started) { started = true; return 0; }
+  qvar["psi"] = psiF[1];
+  qvar["slip"] = slipF[1];
+  qd->V = ff.V;
+  for (int i = 0; i < N; ++i) qd->u[i] = du[i] + 1.0 * u0[i];
+  return 0;
+}
+
+// prepare_qd2fd (:807-889)
+int QdFdModel::prepareQd2Fd(const QdFdRunCfg& r) {
+  const std::vector<double> uStart = qd->u;
+  inDynamic = false;
+  int e = qdPhase(r, 2, 1, currT + dtFd, dtFd, dtFd);
+  inDynamic = true;
+  if (e) return e;
+  for (int i = 0; i < N; ++i) {
+    du[i] = qd->u[i] + -1.0 * uStart[i];
+    duPrev[i] = du[i];
+  }
+  u0 = uStart;
+  for (int l : {0, 1}) {
+    psiF[l] = qvar["psi"];
+    slipF[l] = qvar["slip"];
+  }
+  ff.V = qd->V;
+  ff.slip0 = qvar["slip"];
+  for (int k = 0; k < Nz; ++k) ff.tau0[k] = qd->strength[k] + 1.0 * qd->prestress[k];
+  return 0;
+}
+
+// d_dt of the FD phase (:1676-1759): propagateWaves, Fault_fd::d_dt, the
+// fault's u into the body; levels 0 prev, 1 cur, 2 next
+int QdFdModel::fdStep() {
+  fd_propagate(fs, dtFd, du.data(), duPrev.data(), rho.data(), ay.data(), duNext.data(), d2u.data());
+  std::copy(d2u.begin(), d2u.begin() + Nz, ff.d2u.begin());
+  int e = ff.step(dtFd, psiF[1].data(), psiF[0].data(), slipF[1].data(), slipF[0].data(), psiF[2].data(),
+                  slipF[2].data());
+  if (e) return 40;
+  std::copy(ff.u.begin(), ff.u.end(), duNext.begin());
+  duPrev.swap(du);
+  du.swap(duNext);
+  psiF[0].swap(psiF[1]);
+  psiF[1].swap(psiF[2]);
+  slipF[0].swap(slipF[1]);
+  slipF[1].swap(slipF[2]);
+  return 0;
+}
+
+// integrate_fd (:1511-1546), OdeSolver_WaveEq::integrate (odeSolver_WaveEq.cpp:147-186)
+int QdFdModel::fdPhase(const QdFdRunCfg& r) {
+  const double finalT = r.maxTime;
+  const int maxNum = stepCount + r.maxStepCount;
+  const int s0 = stepCount;
+  const double t0 = currT;
+  if (finalT != currT) {
+    while (stepCount < maxNum && currT < finalT) {
+      currT = currT + dtFd;
+      if (currT > finalT) currT = finalT;
+      stepCount++;
+      int e = fdStep();
+      if (e) return e;
+      if (c.fdStop) {
+        lastR = fdRatio();
+        if (checkSwitch(lastR)) break;
+      }
+    }
+    if (!c.fdStop) lastR = fdRatio();
+  }
+  log.push_back(QdFdPhase{1, s0, stepCount, t0, currT, lastR});
+  return 0;
+}
+
+// integrate (:511-580)
+int QdFdModel::integrate(const QdFdRunCfg& r) {
+  log.clear();
+  currT = r.initTime;
+  stepCount = 0;
+  cycleCount = 0;
+  started = false;
+  auto qdp = [&]() {
+    allowed = false;
+    inDynamic = false;
+    prepareFd2Qd();
+    return qdPhase(r, 0, r.maxStepCount, r.maxTime, r.minDt, r.maxDt);
+  };
+  auto fdp = [&]() {
+    allowed = false;
+    inDynamic = true;
+    int e = prepareQd2Fd(r);
+    return e ? e : fdPhase(r);
+  };
+  int e = qdp();
+  if (e) return e;
+  if (currT >= r.maxTime || stepCount >= r.maxStepCount) return 0;
+  if ((e = fdp())) return e;
+  if (currT >= r.maxTime || stepCount >= r.maxStepCount || c.maxNumCycles <= 1) return 0;
+  cycleCount++;
+  while (cycleCount < c.maxNumCycles && stepCount <= r.maxStepCount && currT <= r.maxTime) {
+    if ((e = qdp())) return e;
+    if ((e = fdp())) return e;
+    cycleCount++;
+  }
+  return 0;
 }
 
 // ---------------------------------------------------- power-law QD -----

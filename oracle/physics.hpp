@@ -44,6 +44,7 @@ struct RkSolver {
   double initT = 0, finalT = 0, currT = 0, deltaT = 0, newDeltaT = 0;
   double totTol = 1e-9, kappa = 0.9, minDeltaT = 0, maxDeltaT = 0;
   int maxNumSteps = 0, stepCount = 0, numRejected = 0;
+  int pidStep0 = 0;               // PID after four steps past this one
   double errA[2] = {0, 0};       // circular buffer: [0] latest, [1] previous
   std::vector<std::string> errInds;
   std::vector<double> scaleV;
@@ -106,6 +107,53 @@ struct FdFault {
   long rootIts = 0;
   int step(double dt, const double* psiCur, const double* psiPrev, const double* slipCur,
            const double* slipPrev, double* psiNext, double* slipNext);
+};
+
+// ----------------------------------------------------- QD <-> FD -----
+// strikeSlip_linearElastic_qd_fd (strikeSlip_linearElastic_qd_fd.cpp): QD
+// phases with a QdModel (QD material SAT types), FD phases with fd_propagate
+// and FdFault on an operator with the FD material SAT types.
+struct QdFdConfig {
+  int fdBcR = NEUMANN, fdBcT = NEUMANN, fdBcL = NEUMANN, fdBcB = NEUMANN;
+  int outR = 1, outT = 0, outB = 1;
+  int ayCompat = 1;                       // computePenaltyVectors as it runs (:486-500)
+  double deltaT_fd = -1, CFL = -1;
+  double trigger_qd2fd = 1e-3, trigger_fd2qd = 1e-3, limit_qd = 1e-8, limit_fd = 1e-1;
+  int maxNumCycles = 1000;
+  int fdStop = 0;                          // 1: FD phases end on the switching test
+};
+struct QdFdRunCfg {
+  int integrator = 0, control = 1, normType = 0, maxStepCount = 0;
+  double initTime = 0, maxTime = 0, minDt = 1e-3, maxDt = 1e10, tol = 1e-8;
+};
+struct QdFdPhase {
+  int regime, stepStart, stepEnd;
+  double tStart, tEnd, R;
+};
+struct QdFdModel {
+  QdModel* qd = nullptr;                   // not owned
+  QdFdConfig c;
+  Sbp2D fs;                                // the FD phase's operator
+  FdFault ff;
+  int Ny = 0, Nz = 0, N = 0;
+  std::vector<double> rho, cs, ay, u0, du, duPrev, duNext, d2u;
+  std::vector<double> psiF[3], slipF[3];   // FD fault levels: prev, cur, next
+  VarMap qvar;                             // QD state (psi, slip)
+  double dtFd = 0, currT = 0, lastR = 0;
+  int stepCount = 0, cycleCount = 0;
+  bool inDynamic = false, allowed = false, started = false;
+  std::vector<QdFdPhase> log;
+  int init(QdModel* q, const QdFdConfig& cfg, const double* mu, const double* y, const double* z,
+           const double* rho, const double* cs);
+  bool checkSwitch(double R);
+  double qdRatio() const;
+  double fdRatio() const;
+  int qdPhase(const QdFdRunCfg& r, int regime, int maxSteps, double finalT, double minDt, double maxDt);
+  int prepareFd2Qd();
+  int prepareQd2Fd(const QdFdRunCfg& r);
+  int fdPhase(const QdFdRunCfg& r);
+  int fdStep();
+  int integrate(const QdFdRunCfg& r);
 };
 
 // ------------------------------------------------------ power-law QD -----

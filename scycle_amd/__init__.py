@@ -10,6 +10,7 @@ from .device import DeviceArray  # noqa: F401
 from .sbp_ops import SbpOps, DIRICHLET, NEUMANN  # noqa: F401
 from .qd import PowerLawMediator, QdMediator  # noqa: F401
 from .fd import FdMediator  # noqa: F401
+from .qdfd import QdFdMediator  # noqa: F401
 
 __all__ = ["LIB_PATH", "ScError", "load", "DeviceArray", "SbpOps", "DIRICHLET", "NEUMANN",
-           "QdMediator", "PowerLawMediator", "FdMediator"]
+           "QdMediator", "PowerLawMediator", "FdMediator", "QdFdMediator"]

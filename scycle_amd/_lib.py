@@ -85,6 +85,27 @@ class QdRun(C.Structure):
                 ("deferChecks", C.c_int), ("currErr", C.c_double), ("prevErr", C.c_double)]
 
 
+class QdFdDesc(C.Structure):
+    _fields_ = [("qd", QdDesc), ("fdBcR", C.c_int), ("fdBcT", C.c_int), ("fdBcL", C.c_int),
+                ("fdBcB", C.c_int), ("outR", C.c_int), ("outT", C.c_int), ("outB", C.c_int),
+                ("faultL", C.c_int), ("ayCompat", C.c_int), ("bulkPrec", C.c_int),
+                ("deltaT_fd", C.c_double), ("CFL", C.c_double), ("trigger_qd2fd", C.c_double),
+                ("trigger_fd2qd", C.c_double), ("limit_qd", C.c_double), ("limit_fd", C.c_double),
+                ("maxNumCycles", C.c_int), ("fdStop", C.c_int)]
+
+
+class QdFdRun(C.Structure):
+    _fields_ = [("integrator", C.c_int), ("control", C.c_int), ("normType", C.c_int),
+                ("maxStepCount", C.c_int), ("initTime", C.c_double), ("maxTime", C.c_double),
+                ("minDt", C.c_double), ("maxDt", C.c_double), ("tol", C.c_double),
+                ("deferChecks", C.c_int)]
+
+
+class QdFdPhase(C.Structure):
+    _fields_ = [("regime", C.c_int), ("stepStart", C.c_int), ("stepEnd", C.c_int),
+                ("tStart", C.c_double), ("tEnd", C.c_double), ("R", C.c_double)]
+
+
 class QdOutput(C.Structure):
     _fields_ = [("outputDir", C.c_char_p), ("stride1D", C.c_int), ("stride2D", C.c_int),
                 ("enableCheckpointing", C.c_int), ("interval", C.c_int)]
@@ -168,6 +189,14 @@ SIGNATURES = {
     "sc_qd_file_run": (C.c_int, [_vp, C.POINTER(QdRun)]),
     "sc_qd_solve_ss": (C.c_int, [_vp, _dp, C.c_int, _dp]),
     "sc_qd_set_output": (C.c_int, [_vp, C.POINTER(QdOutput)]),
+    "sc_qdfd_create": (C.c_int, [C.POINTER(QdFdDesc)] + [_dp] * 13 + [C.POINTER(_vp)]),
+    "sc_qdfd_destroy": (C.c_int, [_vp]),
+    "sc_qdfd_time_step": (C.c_int, [_vp, _dp]),
+    "sc_qdfd_set_state": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "sc_qdfd_get_state": (C.c_int, [_vp, _dp, _dp, _dp, _dp, C.POINTER(C.c_int)]),
+    "sc_qdfd_integrate": (C.c_int, [_vp, C.POINTER(QdFdRun), _dp, C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int), C.POINTER(QdFdPhase), C.c_int,
+                                    C.POINTER(C.c_int)]),
     "sc_qd_load_checkpoint": (C.c_int, [_vp, C.POINTER(QdRun), C.POINTER(C.c_int)]),
     "sc_in_get": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]),
     "sc_in_get_vector": (C.c_int, [C.c_char_p, C.c_char_p, _dp, C.c_int, C.POINTER(C.c_int)]),

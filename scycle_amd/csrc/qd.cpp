@@ -243,7 +243,7 @@ int QdDriver::check_status() {
 // computeStepSize (odeSolver.cpp:395-445, 842-898)
 double QdDriver::step_size(const RkState& r, double totErr) const {
   double ratio;
-  if (r.control == 0 || r.stepCount < 4) {
+  if (r.control == 0 || r.stepCount - r.pidBase < 4) {
     ratio = r.kappa * std::pow(r.tol / totErr, 1. / (1. + r.ord));
   } else {
     const double al = 0.49 / r.ord, be = 0.34 / r.ord, ga = 0.1 / r.ord;
@@ -392,6 +392,9 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
   int maxSteps = run.maxSteps;
   const bool replay = run.nReplay > 0 && run.replayDt_h;
   if (replay && maxSteps > run.nReplay) maxSteps = run.nReplay;
+  r.stepCount = stepBase;
+  r.pidBase = pidFresh ? stepBase : 0;
+  maxSteps += stepBase;
   hipStream_t s = ctx->c.stream;
   int e;
   if (out) {  // timeMonitor(initTime, initDeltaT, 0) before the solver starts (:729-732)
@@ -399,12 +402,12 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
     int stop = 0;  // ignored, as there
     if ((e = monitor(r.currT, r.deltaT, 0, zero, r.deltaT, run.maxSteps, finalT, true, &stop))) return e;
   }
-  if (finalT == r.currT || maxSteps == 0) {
+  if (finalT == r.currT || run.maxSteps == 0) {
     *tEnd = r.currT;
-    *steps = 0;
+    *steps = r.stepCount;
     return SC_OK;
   }
-  if (r.deltaT == 0) r.deltaT = (finalT - r.currT) / maxSteps;
+  if (r.deltaT == 0) r.deltaT = (finalT - r.currT) / run.maxSteps;
   // a run that ended in an error may have left the device flags set
   SC_HIP(hipMemsetAsync(status_d, 0, 2 * sizeof(int), s));
   // solves with a projected guess are checked on the device; the flag is read
@@ -427,7 +430,7 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
     while (attempt < 100) {
       attempt++;
       defer = may_defer && !checked && (always || defer_ok());
-      if (replay) r.deltaT = run.replayDt_h[r.stepCount - 1];
+      if (replay) r.deltaT = run.replayDt_h[r.stepCount - stepBase - 1];
       if (r.currT + r.deltaT > finalT) r.deltaT = finalT - r.currT;
       const double h = r.deltaT;
       bool missed = false;
@@ -498,6 +501,11 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
         return e;
       if (stop) break;
     }
+    if (hook) {
+      int stop = 0;
+      if ((e = hook->after_step(*this, r.currT, r.deltaT, r.stepCount, &stop))) return e;
+      if (stop) break;
+    }
     r.deltaT = newDt;
   }
   defer = false;
@@ -506,6 +514,18 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
   if ((e = check_status())) return e;
   *tEnd = r.currT;
   *steps = r.stepCount;
+  return SC_OK;
+}
+
+int QdDriver::max_ratio(double t, double* R) {
+  int e = settle(t);
+  if (e) return e;
+  hipStream_t s = ctx->c.stream;
+  double* slot = err_d + kErrSlots - 1;
+  if ((e = launch_max_ratio(s, eta_d, V_d, tauQS_d, Nz, slot))) return e;
+  SC_HIP(hipMemcpyAsync(host_scratch, slot, sizeof(double), hipMemcpyDeviceToHost, s));
+  SC_HIP(hipStreamSynchronize(s));
+  *R = ((const double*)host_scratch)[0];
   return SC_OK;
 }
 

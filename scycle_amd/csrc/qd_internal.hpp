@@ -51,9 +51,14 @@ int launch_qd_rhs_frame(const Ctx& c, double* rhs, const double* slip, const dou
                         int* mnb = nullptr);
 int launch_qd_fault(const Ctx& c, const double* u, const sc_fault_qd_args& a, double* dslip, hipStream_t s);
 int launch_vars_lincomb(hipStream_t s, const VarsLinComb& L);
+// out[0] = max over k of (eta[k] V[k]) / tauQS[k] (the switching ratio of the
+// QD-FD mediator); NaN entries are ignored
+int launch_max_ratio(hipStream_t s, const double* eta, const double* V, const double* tauQS, int n,
+                     double* out);
 
 struct RkState {
   int ord = 3, control = 1, normType = 0, stepCount = 0;
+  int pidBase = 0;  // PID needs four steps after this one (the controller's start)
   double tol = 1e-8, kappa = 0.9, minDt = 0, maxDt = 1e30, deltaT = 0, currT = 0;
   double errA[2] = {0, 0};
 };
@@ -73,6 +78,14 @@ struct QdOutput {
   bool append = false;                  // series start appended (a restart), else truncated
   bool started1D = false, started2D = false;
   double* sxy_d = nullptr;              // muxDy u for the 2-D series (N)
+};
+
+struct QdDriver;
+// Called by integrate after every accepted step, with the rates at the new
+// state computed (the mediator's timeMonitor); *stop ends the integration.
+struct QdStepHook {
+  virtual ~QdStepHook() {}
+  virtual int after_step(QdDriver& q, double t, double dt, int step, int* stop) = 0;
 };
 
 struct QdDriver {
@@ -114,6 +127,11 @@ struct QdDriver {
   sc_qd_run file_run{};
   bool have_file_run = false;
   std::unique_ptr<QdOutput> out;       // set: integrate calls monitor()
+  QdStepHook* hook = nullptr;           // set: integrate calls hook->after_step()
+  // step count integrate starts from; run.maxSteps counts from it
+  // (OdeSolver::setInitialStepCount, odeSolver.cpp:25-29)
+  int stepBase = 0;
+  bool pidFresh = false;  // the PID controller counts its four P steps from stepBase
 
   ~QdDriver();
   double* dalloc(size_t n, const double* init_h);
@@ -142,6 +160,8 @@ struct QdDriver {
   // the rates at time t were computed with deferred guess checks: read the
   // flags and redo them with checked solves if a guess missed
   int settle(double t);
+  // max over the fault of eta V / tauQS of the last d_dt (settled first)
+  int max_ratio(double t, double* R);
   // timeMonitor (output.cpp): writes what is due at this step; *stop set by
   // a checkpoint.  initial: the call before the first step (no rates to settle)
   int monitor(double t, double dt, int step, const double* errA, double newDt, int maxSteps,

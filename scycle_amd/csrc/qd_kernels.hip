@@ -211,5 +211,30 @@ int launch_vars_lohi_err(hipStream_t s, const VarsLoHi& L) {
   return SC_OK;
 }
 
+// strikeSlip_linearElastic_qd_fd::checkSwitchRegime (:598-608): R = eta V,
+// R = R / tauQS, VecMax(R); one block, grid-stride over the fault
+constexpr int kRatioThreads = 1024;
+__global__ __launch_bounds__(kRatioThreads) void k_max_ratio(const double* __restrict__ eta,
+                                                             const double* __restrict__ V,
+                                                             const double* __restrict__ tauQS, int n,
+                                                             double* out) {
+  __shared__ double sh[kRatioThreads];
+  double m = -INFINITY;
+  for (int k = threadIdx.x; k < n; k += kRatioThreads) m = fmax(m, (eta[k] * V[k]) / tauQS[k]);
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kRatioThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+int launch_max_ratio(hipStream_t s, const double* eta, const double* V, const double* tauQS, int n,
+                     double* out) {
+  hipLaunchKernelGGL(k_max_ratio, dim3(1), dim3(kRatioThreads), 0, s, eta, V, tauQS, n, out);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
 
 }  // namespace sc

@@ -73,6 +73,14 @@ def lib():
         L.orc_pl_sat.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_pl_ddt.argtypes = [C.c_void_p, C.c_double] + [_dp] * 12
         L.orc_pl_integrate.argtypes = [C.c_void_p, C.c_void_p, C.c_int] + [_dp] * 7 + [C.c_int]
+        L.orc_qdfd_new.restype = C.c_void_p
+        L.orc_qdfd_new.argtypes = [C.c_void_p, C.c_void_p] + [_dp] * 5
+        L.orc_qdfd_free.argtypes = [C.c_void_p]
+        L.orc_qdfd_dt.restype = C.c_double
+        L.orc_qdfd_dt.argtypes = [C.c_void_p]
+        L.orc_qdfd_ay.argtypes = [C.c_void_p, _dp]
+        L.orc_qdfd_integrate.argtypes = [C.c_void_p, C.c_void_p] + [_dp] * 5 + \
+            [C.POINTER(C.c_int)] * 3 + [C.c_void_p, C.c_int]
         L.orc_rk_linear.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                     C.c_double, C.c_double, C.c_double, _dp, _dp,
                                     C.POINTER(C.c_long)]
@@ -354,8 +362,9 @@ class OracleQd:
         self._keep = [np.ascontiguousarray(x, dtype=np.float64) for x in
                       (cfg.mu, cfg.y, cfg.z, *fr, cfg.sN, cfg.eta)]
         self.N, self.Nz = cfg.Ny * cfg.Nz, cfg.Nz
-        self._pre = None if cfg.prestress is None else np.ascontiguousarray(cfg.prestress, dtype=np.float64)
-        self.h = lib().orc_qd_new(C.byref(c), *[ptr(x) for x in self._keep], None, ptr(self._pre), None)
+        self._opt = [None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+                     for x in (cfg.locked, cfg.prestress, cfg.bcRShift)]
+        self.h = lib().orc_qd_new(C.byref(c), *[ptr(x) for x in self._keep], *[ptr(x) for x in self._opt])
         if not self.h:
             raise RuntimeError("orc_qd_new failed")
 
@@ -381,3 +390,77 @@ class OracleQd:
         if n != len(rp):
             raise RuntimeError(f"orc_qd_integrate failed ({n})")
         return dict(psi=st[0], slip=st[1], V=Vv, u=u, t=tEnd.value)
+
+
+class OrcQdFdCfg(C.Structure):
+    _fields_ = [("fdBcR", C.c_int), ("fdBcT", C.c_int), ("fdBcL", C.c_int), ("fdBcB", C.c_int),
+                ("outR", C.c_int), ("outT", C.c_int), ("outB", C.c_int), ("ayCompat", C.c_int),
+                ("deltaT_fd", C.c_double), ("CFL", C.c_double), ("trigger_qd2fd", C.c_double),
+                ("trigger_fd2qd", C.c_double), ("limit_qd", C.c_double), ("limit_fd", C.c_double),
+                ("maxNumCycles", C.c_int), ("fdStop", C.c_int)]
+
+
+class OrcQdFdRun(C.Structure):
+    _fields_ = [("integrator", C.c_int), ("control", C.c_int), ("normType", C.c_int),
+                ("maxStepCount", C.c_int), ("initTime", C.c_double), ("maxTime", C.c_double),
+                ("minDt", C.c_double), ("maxDt", C.c_double), ("tol", C.c_double)]
+
+
+class OrcQdFdPhase(C.Structure):
+    _fields_ = [("regime", C.c_int), ("stepStart", C.c_int), ("stepEnd", C.c_int),
+                ("tStart", C.c_double), ("tEnd", C.c_double), ("R", C.c_double)]
+
+
+class OracleQdFd:
+    """strikeSlip_linearElastic_qd_fd of the oracle: QD phases with direct
+    Cholesky solves, FD phases with the assembled operator."""
+
+    def __init__(self, cfg, rho, fd_bc=(1, 1, 1, 1), out=(1, 0, 1), ay_compat=True, deltaT_fd=-1.0,
+                 cfl=-1.0, trigger_qd2fd=1e-3, trigger_fd2qd=1e-3, limit_qd=None, limit_fd=1e-1,
+                 max_num_cycles=1000, fd_stop=False, jac_mode=1):
+        self.qd = OracleQd(cfg, jac_mode=jac_mode)
+        N = cfg.Ny * cfg.Nz
+        self.N, self.Nz = N, cfg.Nz
+        rho_a = np.broadcast_to(np.asarray(rho, dtype=np.float64), (N,)).copy()
+        cs = np.sqrt(np.asarray(cfg.mu, dtype=np.float64) / rho_a)
+        c = OrcQdFdCfg(*[int(x) for x in fd_bc], *[int(x) for x in out], int(ay_compat), deltaT_fd, cfl,
+                       trigger_qd2fd, trigger_fd2qd, 10 * cfg.vL if limit_qd is None else limit_qd,
+                       limit_fd, int(max_num_cycles), int(fd_stop))
+        self._keep = [np.ascontiguousarray(x, dtype=np.float64) for x in (cfg.mu, cfg.y, cfg.z, rho_a, cs)]
+        self.h = lib().orc_qdfd_new(self.qd.h, C.byref(c), *[ptr(x) for x in self._keep])
+        if not self.h:
+            raise RuntimeError("orc_qdfd_new failed")
+
+    def __del__(self):
+        try:
+            lib().orc_qdfd_free(self.h)
+        except Exception:
+            pass
+
+    @property
+    def deltaT_fd(self):
+        return lib().orc_qdfd_dt(self.h)
+
+    def ay(self):
+        out = np.zeros(self.N)
+        lib().orc_qdfd_ay(self.h, ptr(out))
+        return out
+
+    def integrate(self, psi, slip, V, max_step_count, max_time=1e15, init_time=0.0, integrator=0,
+                  tol=1e-8, min_dt=1e-3, max_dt=1e10, control=1, log_cap=4096):
+        st = [np.array(x, dtype=np.float64) for x in (psi, slip, V)]
+        u = np.zeros(self.N)
+        r = OrcQdFdRun(integrator, control, 0, int(max_step_count), init_time, max_time, min_dt,
+                       max_dt, tol)
+        log = (OrcQdFdPhase * log_cap)()
+        tEnd = C.c_double()
+        steps, cycles, regime = C.c_int(), C.c_int(), C.c_int()
+        n = lib().orc_qdfd_integrate(self.h, C.byref(r), *[ptr(x) for x in st], ptr(u), C.byref(tEnd),
+                                     C.byref(steps), C.byref(cycles), C.byref(regime), log, log_cap)
+        if n < 0:
+            raise RuntimeError(f"orc_qdfd_integrate failed ({n})")
+        names = {0: "qd", 1: "fd", 2: "qd2fd_step"}
+        phases = [dict(regime=names[p.regime], step_start=p.stepStart, step_end=p.stepEnd,
+                       t_start=p.tStart, t_end=p.tEnd, R=p.R) for p in log[:min(n, log_cap)]]
+        return dict(psi=st[0], slip=st[1], V=st[2], u=u, t=tEnd.value, steps=steps.value,
+                    cycles=cycles.value, regime=names[regime.value], phases=phases)

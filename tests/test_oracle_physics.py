@@ -364,3 +364,78 @@ def test_powerlaw_without_creep_is_linear_elastic():
 def C_byref(x):
     import ctypes
     return ctypes.byref(x)
+
+
+# ------------------------------------------------------------ QD <-> FD --
+def _qdfd_cfg(Ny=9, Nz=11, var_grid=False):
+    from scycle_amd.configs import strike_slip
+    cfg = strike_slip(Ny, Nz, order=4, var_grid=var_grid, zstretch=3.0 if var_grid else None)
+    cfg.bcRShift = np.full(Nz, 25.0)
+    return cfg
+
+
+def test_qdfd_penalty_vector_as_the_reference_runs_it():
+    """strikeSlip_linearElastic_qd_fd::computePenaltyVectors (:486-500) writes
+    a_y through an index Jj that only the fault-side if-statement advances,
+    so only the fault row (indices < Nz) and index Nz are ever written: the
+    fault row's top and bottom corners get their 0.5/h11z terms, index Nz
+    ends with the last node's value (right side and bottom: 0.5/h11y +
+    0.5/h11z), everything else stays zero; each entry is then scaled by cs at
+    its own index.  With ayCompat off a_y equals the FD mediator's."""
+    cfg = _qdfd_cfg()
+    Ny, Nz = cfg.Ny, cfg.Nz
+    cs = np.sqrt(cfg.mu / 3.0)
+    ops = ol.OracleSbp(cfg.order, Ny, Nz, cfg.Ly, cfg.Lz, cfg.mu, var_grid=False, bc=(1, 1, 1, 1))
+    h11y, h11z = ops.scalars()["h11y"], ops.scalars()["h11z"]
+    o = ol.OracleQdFd(cfg, 3.0, out=(1, 1, 1), ay_compat=True)
+    ay = o.ay()
+    nz = np.flatnonzero(ay)
+    assert set(nz) == {0, Nz - 1, Nz}
+    assert ay[0] == pytest.approx(0.5 / h11z * cs[0], rel=1e-15)
+    assert ay[Nz - 1] == pytest.approx(0.5 / h11z * cs[Nz - 1], rel=1e-15)
+    assert ay[Nz] == pytest.approx((0.5 / h11y + 0.5 / h11z) * cs[Nz], rel=1e-15)
+    o2 = ol.OracleQdFd(cfg, 3.0, out=(1, 1, 1), ay_compat=False)
+    ref = np.zeros(Ny * Nz)
+    lib().orc_fd_ay(ops.h, ptr(np.ascontiguousarray(cs)), 1, 1, 1, 1, ptr(ref))
+    assert np.array_equal(o2.ay(), ref)
+
+
+def test_qdfd_time_step():
+    """computeTimeStep (:356-443): the largest CFL step g_cfl min(d/cs) with
+    g_cfl = 0.7071/sqrt(1.4498) at order 4; a requested deltaT_fd wins; CFL
+    alone gives CFL g_cfl (g_cfl min(d/cs)), the factor applied twice."""
+    cfg = _qdfd_cfg()
+    g = 0.7071 / np.sqrt(1.4498)
+    d = min(cfg.Ly / (cfg.Ny - 1), cfg.Lz / (cfg.Nz - 1)) / np.sqrt(36.0 / 3.0)
+    assert ol.OracleQdFd(cfg, 3.0).deltaT_fd == pytest.approx(g * d, rel=1e-14)
+    assert ol.OracleQdFd(cfg, 3.0, deltaT_fd=0.01).deltaT_fd == 0.01
+    assert ol.OracleQdFd(cfg, 3.0, cfl=0.5).deltaT_fd == pytest.approx(0.5 * g * g * d, rel=1e-14)
+    assert ol.OracleQdFd(cfg, 3.0, cfl=0.5, deltaT_fd=0.02).deltaT_fd == 0.02
+
+
+@pytest.mark.parametrize("fd_stop", [False, True])
+def test_qdfd_switching_control_flow(fd_stop):
+    """A 25 x 25 fault near steady sliding nucleates an event: the QD phase ends
+    at the first step with R = max eta V / tauQS above trigger_qd2fd (after R
+    was below limit_qd), one QD step of deltaT_fd follows, then the FD phase;
+    it runs its maxStepCount steps (reference) or ends at the first step with
+    R below trigger_fd2qd after R exceeded limit_fd (fd_stop).  Phases are
+    contiguous in time and step count."""
+    cfg = _qdfd_cfg(25, 25, var_grid=True)
+    kw = dict(trigger_qd2fd=1e-3, trigger_fd2qd=1e-3, limit_qd=1e-6, limit_fd=1e-2)
+    o = ol.OracleQdFd(cfg, 3.0, max_num_cycles=1, fd_stop=fd_stop, **kw)
+    steps = 8000
+    r = o.integrate(cfg.psi0, cfg.slip0, np.full(cfg.Nz, 1e-9), max_step_count=steps, max_time=3e9)
+    ph = r["phases"]
+    assert [p["regime"] for p in ph] == ["qd", "qd2fd_step", "fd"]
+    for a, b in zip(ph, ph[1:]):
+        assert a["step_end"] == b["step_start"] and a["t_end"] == b["t_start"]
+    assert ph[0]["R"] > kw["trigger_qd2fd"]
+    assert ph[1]["step_end"] == ph[1]["step_start"] + 1
+    assert ph[1]["t_end"] - ph[1]["t_start"] == pytest.approx(o.deltaT_fd, rel=1e-6)
+    if fd_stop:
+        assert ph[2]["R"] < kw["trigger_fd2qd"] and ph[2]["step_end"] < ph[2]["step_start"] + steps
+    else:
+        assert ph[2]["step_end"] == ph[2]["step_start"] + steps
+    # the event: a metre-scale slip increment at depth during the FD phase
+    assert r["slip"].max() > 0.1 and r["regime"] == "fd"
