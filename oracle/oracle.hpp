@@ -29,6 +29,11 @@
 //   wave step, FD fault constant state at rest, leapfrog energy, a_y sides,
 //                       fault equations solved independently (same file)
 //   power law           s^n and Arrhenius scaling, cap, SAT term (same file)
+//   QD <-> FD switching a_y as the reference's loop writes it, the time step
+//                       formula's cases, the phase sequence and switching
+//                       thresholds of a nucleating event (same file); the
+//                       phases themselves are the QD model and FD pieces
+//                       pinned above
 //   PETSc Vec / .in I/O the reference's own test fixture (tests/golden/)
 // Parity unpinned: MUMPS's and PETSc's floating-point summation order (the
 // assembled CSR matches the reference's algebra, not its bits).
