@@ -94,10 +94,12 @@ def run_path(cfg, label, warmup, steps, reuse=0):
 
 
 def roofline_guess(ctx, reps=200):
-    """The projected-guess kernel (k_proj_guess_res<M>) on the basis the QD
-    run left: it reads b, the M basis vectors and their M operator products
-    and writes x, 8 (2M + 2) bytes per node; mean of `reps` launches timed
-    with HIP events on the context stream."""
+    """The projected-guess kernel of the QD step (k_proj_guess_res<M>, the
+    deferred form that leaves x = sum c_m y_m to be formed on demand) on the
+    basis the QD run left: it reads the M operator products A y_m over the
+    grid and b on the rhs band rows (b is zero elsewhere), 8 (M N + band
+    nodes) bytes per launch; mean of `reps` launches timed with HIP events on
+    the context stream."""
     from scycle_amd._lib import check, load
     ms, words = C.c_double(), C.c_double()
     check(load().sc_bench_apply(ctx, 8, reps, C.byref(ms), C.byref(words)))

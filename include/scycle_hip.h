@@ -302,6 +302,11 @@ typedef struct sc_qd_run {
    * checkpointed chkpt_currErr / chkpt_prevErr on a restart
    * (strikeSlip_linearElastic_qd.cpp:734-737). */
   double currErr, prevErr;
+  /* 0: with deferred checks (and no replay, output or power law), RK32's
+   * step control runs on the device and the host reads it back once per
+   * batch of attempts; 1: the host controls every step (the reference's
+   * loop as written).  Both take the same steps up to the last bit of pow. */
+  int stepControl;
 } sc_qd_run;
 
 typedef struct sc_qd_counters {
@@ -310,6 +315,8 @@ typedef struct sc_qd_counters {
   /* solves inside sc_qd_integrate whose projected guess was accepted on the
    * device, and the attempts repeated because such a guess missed */
   long long deferred_checks, missed_guesses;
+  /* accepted steps whose step control ran on the device (sc_qd_run.stepControl) */
+  long long device_steps;
 } sc_qd_counters;
 
 /* StrikeSlip_LinearElastic_qd (strikeSlip_linearElastic_qd.cpp:7-78) with

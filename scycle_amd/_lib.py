@@ -82,7 +82,8 @@ class QdRun(C.Structure):
                 ("maxSteps", C.c_int), ("initTime", C.c_double), ("maxTime", C.c_double),
                 ("initDt", C.c_double), ("minDt", C.c_double), ("maxDt", C.c_double),
                 ("tol", C.c_double), ("nReplay", C.c_int), ("replayDt_h", _dp),
-                ("deferChecks", C.c_int), ("currErr", C.c_double), ("prevErr", C.c_double)]
+                ("deferChecks", C.c_int), ("currErr", C.c_double), ("prevErr", C.c_double),
+                ("stepControl", C.c_int)]
 
 
 class QdFdDesc(C.Structure):
@@ -116,7 +117,7 @@ class QdCounters(C.Structure):
                 ("cg_iters", C.c_longlong), ("outer_sweeps", C.c_longlong),
                 ("rejected_steps", C.c_longlong), ("solve_seconds", C.c_double),
                 ("total_seconds", C.c_double), ("deferred_checks", C.c_longlong),
-                ("missed_guesses", C.c_longlong)]
+                ("missed_guesses", C.c_longlong), ("device_steps", C.c_longlong)]
 
 
 class AdjDesc(C.Structure):

@@ -32,5 +32,12 @@ if [[ $fail -ne 0 ]]; then
   exit 1
 fi
 "$HIPCC" -shared --offload-arch="$ARCH" -fPIC -o "$out.tmp" "${objs[@]}" -L/opt/rocm/lib -lrocblas -Wl,-rpath,/opt/rocm/lib
+# every symbol of the library's own namespace must resolve inside it
+if nm -D --undefined-only "$out.tmp" | grep -q '_ZN2sc'; then
+  nm -D --undefined-only "$out.tmp" | grep '_ZN2sc' >&2
+  echo "unresolved library symbols" >&2
+  rm -f "$out.tmp"
+  exit 1
+fi
 mv "$out.tmp" "$out"
 echo "built $out"

@@ -219,7 +219,9 @@ AdjDriver::~AdjDriver() {
 
 int AdjDriver::setup(bool reverse) {
   Nz_ = q_.Nz;
-  // a run that ended in an error may have left the device flags set
+  // a run that ended in an error may have left the device flags set (and a
+  // residual test pending)
+  q_.finPending = false;
   SC_HIP(hipMemsetAsync(q_.status_d, 0, 2 * sizeof(int), q_.ctx->c.stream));
   const int C = d_.ckptEvery;
   nseg_ = (d_.nSteps + C - 1) / C;
@@ -342,6 +344,7 @@ int AdjDriver::forward_misfit(double* J, double* us_h, bool keep_records) {
   }
   // u of the last d_dt is at the final state
   hipStream_t s = q_.ctx->c.stream;
+  if ((e = q_.materialize_u())) return e;
   if (d_.nStations > 0) {
     hipLaunchKernelGGL(k_station_rhs, dim3((d_.nStations + 63) / 64), dim3(64), 0, s, d_.nStations, js_d_, Nz_,
                        (const double*)q_.u_d, (const double*)data_d_, rhsQ_, us_d_);

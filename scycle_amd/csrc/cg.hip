@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "cg_dev.hpp"
 #include "fdiag.hpp"
 #include "fault_dev.hpp"
 #include "reduce_dev.hpp"
@@ -23,12 +24,6 @@
 #include "sc_internal.hpp"
 
 namespace sc {
-
-struct CgScal {
-  double rz, pq, rr, bb, alpha, beta, tol2;
-  int done, iters, maxit, pad;
-  unsigned int cnt;
-};
 
 enum FinishOp { FIN_INIT, FIN_PQ, FIN_UPDATE, FIN_IRRES, FIN_UPDATE_R, FIN_RZ };
 
@@ -329,13 +324,20 @@ struct GuessFault {
   int nfb;
 };
 
-template <int M, bool FAULT>
+// WX = false (a deferred guess whose caller reads the solution only later):
+// x is not written (nor Y read); block 0 leaves the coefficients in coefOut
+// and proj_materialize forms x = sum c_m y_m from them when it is needed.
+// b is known to vanish on [zLo, zHi) (zLo = zHi: nowhere) and is not read
+// there.
+template <int M, bool FAULT, bool WX>
 __global__ __launch_bounds__(kRedThreads) void k_proj_guess_res(size_t N, VecList Y, VecList Z,
                                                                 const double* __restrict__ mpart, int mnb,
                                                                 const double* __restrict__ w, CgScal* sc,
                                                                 double tol2, const double* __restrict__ b,
                                                                 double* __restrict__ x, double* partials,
-                                                                GuessFault gf) {
+                                                                GuessFault gf, StepGate gate, size_t zLo,
+                                                                size_t zHi, double* coefOut) {
+  if (gate_closed(gate)) return;
   __shared__ double shc[M][kRedThreads / 64];
   __shared__ double coef[M];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -384,22 +386,39 @@ __global__ __launch_bounds__(kRedThreads) void k_proj_guess_res(size_t N, VecLis
     sc->iters = 0;
     sc->cnt = 0;
   }
+  if (!WX && blk == 0 && threadIdx.x < M) coefOut[threadIdx.x] = c[threadIdx.x];
   const unsigned nstream = gridDim.x - (FAULT ? gf.nfb : 0);
   double acc[2] = {0.0, 0.0};
   for (size_t i = (size_t)blk * blockDim.x + threadIdx.x; i < N; i += (size_t)nstream * blockDim.x) {
     double v = 0.0, az = 0.0;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-      v += c[m] * Y.v[m][i];
+      if (WX) v += c[m] * Y.v[m][i];
       az += c[m] * Z.v[m][i];
     }
-    const double bi = b[i];
+    const double bi = (i >= zLo && i < zHi) ? 0.0 : b[i];
     const double ri = bi - az;
-    x[i] = v;
+    if (WX) x[i] = v;
     acc[0] += ri * ri;
     acc[1] += bi * bi;
   }
   block_partials_at(acc, partials, blk);
+}
+
+// x = sum_m coef[m] y_m: the x a k_proj_guess_res<M, *, false> launch left
+// unwritten, with its arithmetic (the same value bit for bit)
+template <int M>
+__global__ __launch_bounds__(256) void k_proj_x(size_t N, VecList Y, const double* __restrict__ coef,
+                                                double* __restrict__ x) {
+  double c[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) c[m] = coef[m];
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+    double v = 0.0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) v += c[m] * Y.v[m][i];
+    x[i] = v;
+  }
 }
 
 // fault traction of basis vector k: T = mu Dy y on row 0 (the QD step's S y)
@@ -480,6 +499,15 @@ __global__ void k_set_scal(CgScal* sc, double tol2, int maxit) {
   sc->done = 0;
   sc->iters = 0;
   sc->cnt = 0;
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_pending_finish(PendingFinish f) { finish_irres_block(f); }
+
+int launch_pending_finish(hipStream_t s, const PendingFinish& f) {
+  if (!f.sc) return SC_OK;
+  hipLaunchKernelGGL(k_pending_finish, dim3(1), dim3(kRedThreads), 0, s, f);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
 }
 
 namespace {
@@ -631,7 +659,7 @@ constexpr int kMdBlocksMax = 256;
 
 template <int M>
 void proj_guess_m(Ctx& c, const VecList& Y, const VecList& Z, const double* b, double* x, double tol2,
-                  bool rhs_frame, int pre_mnb, const QdFuse* fuse) {
+                  bool rhs_frame, int pre_mnb, const QdFuse* fuse, StepGate gate, bool lazyX) {
   const size_t N = (size_t)c.N;
   hipStream_t s = c.stream;
   double* mpart = c.partials + kMdOffset;
@@ -648,16 +676,31 @@ void proj_guess_m(Ctx& c, const VecList& Y, const VecList& Z, const double* b, d
     hipLaunchKernelGGL(k_multidot<M>, dim3(mnb), dim3(kRedThreads), 0, s, N, b, (const double*)nullptr, Y,
                        mpart);
   }
+  // the projections came from the QD driver's rhs kernel: b vanishes off
+  // its band rows
+  size_t zLo = 0, zHi = 0;
+  if (pre_mnb > 0) {
+    int nLow, nHigh;
+    qd_band_rows(c.d.Ny, &nLow, &nHigh);
+    zLo = (size_t)nLow * c.d.Nz;
+    zHi = (size_t)(c.d.Ny - nHigh) * c.d.Nz;
+  }
+  double* coefOut = c.projW + 2 * SC_REUSE_MAX;
   if (fuse) {
     GuessFault gf{*fuse, {}, (fuse->a.N + kRedThreads - 1) / kRedThreads};
     for (int m = 0; m < M; ++m) gf.T.v[m] = c.projT[m];
-    hipLaunchKernelGGL((k_proj_guess_res<M, true>), dim3(red_blocks(N) + gf.nfb), dim3(kRedThreads), 0, s, N,
-                       Y, Z, (const double*)mpart, (int)mnb, (const double*)c.projW, (CgScal*)c.scal, tol2, b,
-                       x, c.partials, gf);
+    if (lazyX)
+      hipLaunchKernelGGL((k_proj_guess_res<M, true, false>), dim3(red_blocks(N) + gf.nfb), dim3(kRedThreads), 0, s,
+                         N, Y, Z, (const double*)mpart, (int)mnb, (const double*)c.projW, (CgScal*)c.scal, tol2, b,
+                         x, c.partials, gf, gate, zLo, zHi, coefOut);
+    else
+      hipLaunchKernelGGL((k_proj_guess_res<M, true, true>), dim3(red_blocks(N) + gf.nfb), dim3(kRedThreads), 0, s,
+                         N, Y, Z, (const double*)mpart, (int)mnb, (const double*)c.projW, (CgScal*)c.scal, tol2, b,
+                         x, c.partials, gf, gate, zLo, zHi, coefOut);
   } else {
-    hipLaunchKernelGGL((k_proj_guess_res<M, false>), dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, Y, Z,
+    hipLaunchKernelGGL((k_proj_guess_res<M, false, true>), dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, Y, Z,
                        (const double*)mpart, (int)mnb, (const double*)c.projW, (CgScal*)c.scal, tol2, b, x,
-                       c.partials, GuessFault{});
+                       c.partials, GuessFault{}, gate, zLo, zHi, coefOut);
   }
 }
 
@@ -684,11 +727,13 @@ struct ProjArgs {
   bool rhs_frame;
   int pre_mnb;
   const QdFuse* fuse;
+  StepGate gate;
+  bool lazyX;
 };
 
 template <int M>
 void proj_dispatch(bool guess, Ctx& c, const VecList& Y, const VecList& Z, const ProjArgs& a) {
-  if (guess) proj_guess_m<M>(c, Y, Z, a.p0, a.p1, a.tol2, a.rhs_frame, a.pre_mnb, a.fuse);
+  if (guess) proj_guess_m<M>(c, Y, Z, a.p0, a.p1, a.tol2, a.rhs_frame, a.pre_mnb, a.fuse, a.gate, a.lazyX);
   else proj_add_m<M>(c, Y, Z, a.p0, a.p1, a.p2, a.p3);
 }
 
@@ -718,7 +763,8 @@ int proj_prepare(Ctx& c, int cap) {
       SC_HIP(hipMalloc(v, (size_t)c.N * sizeof(double)));
     SC_HIP(hipMalloc(&c.projT[i], (size_t)c.d.Nz * sizeof(double)));
   }
-  if (!c.projW) SC_HIP(hipMalloc(&c.projW, 2 * SC_REUSE_MAX * sizeof(double)));
+  // w, the basis update's coefficients, the last lazy guess's coefficients
+  if (!c.projW) SC_HIP(hipMalloc(&c.projW, 3 * SC_REUSE_MAX * sizeof(double)));
   c.projCap = cap;
   c.projCount = c.projNext = 0;
   return SC_OK;
@@ -761,7 +807,8 @@ int proj_basis_vector(Ctx& c, int k, const double* x, const double* ax) {
       Y.v[i] = c.projY[i];
       Z.v[i] = c.projZ[i];
     }
-    proj_run(false, k, c, Y, Z, ProjArgs{x, const_cast<double*>(ax), c.projY[k], c.projZ[k], 0.0, false, 0, nullptr});
+    proj_run(false, k, c, Y, Z,
+             ProjArgs{x, const_cast<double*>(ax), c.projY[k], c.projZ[k], 0.0, false, 0, nullptr, StepGate{}, false});
     hipLaunchKernelGGL(k_proj_weight<2>, dim3(1), dim3(kRedThreads), 0, s, c.partials, nb, c.projW, k);
   }
   hipLaunchKernelGGL(k_basis_traction, dim3((c.d.Nz + 255) / 256), dim3(256), 0, s, c.op64,
@@ -806,10 +853,31 @@ int proj_add_solution(Ctx& c, const double* u) {
 // multidot partials as the last solve left them): mean ms per launch over
 // reps launches, HIP events on the context stream; *words = 8-byte words the
 // kernel moves per launch (b, M basis vectors, M products, x).
+int proj_materialize(Ctx& c, double* u) {
+  if (c.projCount == 0) return fail(SC_ERR_ARG, "proj_materialize: no basis");
+  const size_t N = (size_t)c.N;
+  const VecList Y = proj_list(c, false);
+  const double* coef = c.projW + 2 * SC_REUSE_MAX;
+  const dim3 g(ew_blocks(N)), b(256);
+  switch (c.projCount) {
+#define SC_PX(m) \
+  case m: hipLaunchKernelGGL(k_proj_x<m>, g, b, 0, c.stream, N, Y, coef, u); break;
+    SC_PX(1) SC_PX(2) SC_PX(3) SC_PX(4) SC_PX(5) SC_PX(6) SC_PX(7) SC_PX(8)
+#undef SC_PX
+  }
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
+// The QD step's guess (x left to proj_materialize, b read on the rhs band
+// only): M basis products and the band of b per launch
 int bench_proj_guess(Ctx& c, int reps, double* ms, double* words) {
   if (c.projCount == 0) return fail(SC_ERR_ARG, "no projection basis: run a solve with reuse > 0 first");
   const size_t N = (size_t)c.N;
   hipStream_t s = c.stream;
+  int nLow, nHigh;
+  qd_band_rows(c.d.Ny, &nLow, &nHigh);
+  const size_t zLo = (size_t)nLow * c.d.Nz, zHi = (size_t)(c.d.Ny - nHigh) * c.d.Nz;
   const FrameMap fm = FrameMap::make(c.d.Ny, c.d.Nz);
   const int mnb = (int)std::min<long long>((fm.count() + kRedThreads - 1) / kRedThreads, kMdBlocksMax);
   const int M = c.projCount;
@@ -820,9 +888,9 @@ int bench_proj_guess(Ctx& c, int reps, double* ms, double* words) {
     CgScal* sc = (CgScal*)c.scal;
     switch (M) {
 #define SC_GUESS(m) \
-  case m: hipLaunchKernelGGL((k_proj_guess_res<m, false>), g, b, 0, s, N, Y, Z, (const double*)mp, mnb, \
+  case m: hipLaunchKernelGGL((k_proj_guess_res<m, false, false>), g, b, 0, s, N, Y, Z, (const double*)mp, mnb, \
                              (const double*)c.projW, sc, 0.0, (const double*)c.r64, c.p64, c.partials, \
-                             GuessFault{}); break;
+                             GuessFault{}, StepGate{}, zLo, zHi, c.projW + 2 * SC_REUSE_MAX); break;
       SC_GUESS(1) SC_GUESS(2) SC_GUESS(3) SC_GUESS(4) SC_GUESS(5) SC_GUESS(6) SC_GUESS(7) SC_GUESS(8)
 #undef SC_GUESS
     }
@@ -841,7 +909,7 @@ int bench_proj_guess(Ctx& c, int reps, double* ms, double* words) {
   (void)hipEventDestroy(e1);
   SC_HIP(hipGetLastError());
   *ms = t / reps;
-  *words = (double)N * (2.0 * M + 2.0);
+  *words = (double)N * M + (double)(nLow + nHigh) * c.d.Nz;
   return SC_OK;
 }
 
@@ -861,7 +929,10 @@ int proj_frame_plan(Ctx& c, const sc_solve_opts& o, ProjFramePlan* plan) {
 }
 
 int solve(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st,
-          int* miss_flag, bool* deferred, bool rhs_frame, int pre_mnb, const QdFuse* fuse, bool* fused) {
+          int* miss_flag, bool* deferred, bool rhs_frame, int pre_mnb, const QdFuse* fuse, bool* fused,
+          PendingFinish* postpone, StepGate gate, bool* x_lazy) {
+  if (postpone) *postpone = PendingFinish{};
+  if (x_lazy) *x_lazy = false;
   if (deferred) *deferred = false;
   if (fused) *fused = false;
   int e = check_opts(o);
@@ -876,10 +947,17 @@ int solve(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve
   if (c.projCount > 0) {
     CgScal* sc = (CgScal*)c.scal;
     const QdFuse* f = miss_flag ? fuse : nullptr;  // only a deferred guess is taken as the answer
+    // a deferred, fused guess may leave u to proj_materialize
+    const bool lazy = f && x_lazy;
     proj_run(true, c.projCount, c, proj_list(c, false), proj_list(c, true),
-             ProjArgs{rhs, u, nullptr, nullptr, o.rtol * o.rtol, rhs_frame, pre_mnb, f});
+             ProjArgs{rhs, u, nullptr, nullptr, o.rtol * o.rtol, rhs_frame, pre_mnb, f, miss_flag ? gate : StepGate{},
+                      lazy});
+    if (lazy) *x_lazy = true;
     if (f && fused) *fused = true;
-    finish<FIN_IRRES, 2>(s, sc, c, N, 0, miss_flag);
+    if (miss_flag && postpone)
+      *postpone = PendingFinish{sc, c.partials, (int)red_blocks(N), miss_flag};
+    else
+      finish<FIN_IRRES, 2>(s, sc, c, N, 0, miss_flag);
     SC_HIP(hipGetLastError());
     if (miss_flag) {
       std::memset(&st, 0, sizeof(st));

@@ -122,7 +122,7 @@ int QdDriver::monitor(double t, double dt, int step, const double* errA, double 
   const bool ck = o.ckpt > 0 && (step % o.interval == 0 || step >= maxSteps || t >= maxTime);
   if (!(w1 || w2 || ck)) return SC_OK;
   hipStream_t s = ctx->c.stream;
-  int e = initial ? SC_OK : settle(t);
+  int e = initial ? materialize_u() : settle(t);
   if (e) return e;
   SC_HIP(hipStreamSynchronize(s));
   const size_t nz = (size_t)Nz, ny = (size_t)Ny;

@@ -142,6 +142,9 @@ int QdDriver::d_dt_vars(double t) {
 
 int QdDriver::read_flags(int* status, int* miss) {
   hipStream_t s = ctx->c.stream;
+  int e = flush_finish();
+  if (!e) e = materialize_u();
+  if (e) return e;
   SC_HIP(hipMemcpyAsync(host_scratch, status_d, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
   *status = ((const int*)host_scratch)[0];
@@ -152,25 +155,46 @@ int QdDriver::read_flags(int* status, int* miss) {
 // StrikeSlip_LinearElastic_qd::d_dt (strikeSlip_linearElastic_qd.cpp:756-798);
 // with the power law, StrikeSlip_PowerLaw_qd::d_dt (strikeSlip_powerLaw_qd.cpp
 // :633-686) and solveMomentumBalance (:787-823)
+int QdDriver::materialize_u() {
+  if (!uLazy) return SC_OK;
+  uLazy = false;
+  return proj_materialize(ctx->c, u_d);
+}
+
+int QdDriver::flush_finish() {
+  if (!finPending) return SC_OK;
+  finPending = false;
+  return launch_pending_finish(ctx->c.stream, fin);
+}
+
 int QdDriver::d_dt(double t, const double* psi, const double* slip, double* dpsi, double* dslip,
-                   const double* gxy, const double* gxz, double* dgxy, double* dgxz) {
+                   const double* gxy, const double* gxz, double* dgxy, double* dgxz,
+                   const VarsLinComb* pre, int preSlip, StepGate gate, double tfrac) {
   hipStream_t s = ctx->c.stream;
   const double t0 = now();
-  ++counters.d_dt_calls;
+  const bool count = gate.ctl == nullptr;  // the device loop counts what ran
+  if (count) ++counters.d_dt_calls;
   int e;
   int mnb = 0;
   if (!pl) {
     // the rhs is zero off the boundary frame (rhs_d starts zeroed and only
     // this kernel writes it); with a solution basis the kernel also forms
     // the projections that start the solve
+    // (a pending residual test and the stage combination go into the same
+    // launch when there is a plan, before it otherwise)
     ProjFramePlan plan;
     e = proj_frame_plan(ctx->c, desc.solve, &plan);
     if (!e)
       e = launch_qd_rhs_frame(ctx->c, rhs_d, slip, shift_d, t, desc.vL, desc.faultTypeScale,
-                              desc.remoteLoadingR, bcL_d, bcR_d, bcT_d, bcB_d, s, &plan, &mnb);
+                              desc.remoteLoadingR, bcL_d, bcR_d, bcT_d, bcB_d, s, &plan, &mnb, pre, preSlip,
+                              finPending ? &fin : nullptr, gate, tfrac);
+    finPending = false;
   } else {
-    e = launch_qd_bc(s, Nz, slip, shift_d, t, desc.vL, desc.faultTypeScale, desc.remoteLoadingR, bcL_d,
-                     bcR_d);
+    e = flush_finish();
+    if (!e && pre) e = launch_vars_lincomb(s, *pre);
+    if (!e)
+      e = launch_qd_bc(s, Nz, slip, shift_d, t, desc.vL, desc.faultTypeScale, desc.remoteLoadingR, bcL_d,
+                       bcR_d);
     if (!e) e = launch_set_rhs(ctx->c, rhs_d, bcL_d, bcR_d, bcT_d, bcB_d, s);
     if (!e) e = launch_pl_source(ctx->c, gxy, gxz, rhs_d, s);
   }
@@ -198,20 +222,26 @@ int QdDriver::d_dt(double t, const double* psi, const double* slip, double* dpsi
   fa.tau_d = tau_d;
   fa.strength_d = strength_d;
   fa.status_d = status_d;
+  // a checked solve may start CG from u: form the lazy one first
+  if (uLazy && !defer && (e = materialize_u())) return e;
   const double ts = now();
   sc_solve_stats st;
-  bool deferred = false, fused = false;
+  bool deferred = false, fused = false, lazy = false;
   // a deferred guess of the linear-elastic step also computes the fault rates
   const QdFuse fuse{fa, dslip};
   e = solve(ctx->c, rhs_d, u_d, desc.solve, st, defer ? miss_d : nullptr, &deferred, !pl, mnb,
-            pl ? nullptr : &fuse, &fused);
-  counters.deferred_checks += deferred ? 1 : 0;
-  if (deferred) pendingDeferred++;
-  else if (!e) hits = (st.iters == 0 && desc.solve.reuse > 0) ? hits + 1 : 0;
-  counters.solve_seconds += now() - ts;
-  counters.solves += 1;
-  counters.cg_iters += st.iters;
-  counters.outer_sweeps += st.outer;
+            pl ? nullptr : &fuse, &fused, pl ? nullptr : &fin, gate, pl ? nullptr : &lazy);
+  finPending = fin.sc != nullptr;
+  uLazy = lazy;  // a solve that wrote u ends the laziness of an earlier one
+  if (count) {
+    counters.deferred_checks += deferred ? 1 : 0;
+    if (deferred) pendingDeferred++;
+    else if (!e) hits = (st.iters == 0 && desc.solve.reuse > 0) ? hits + 1 : 0;
+    counters.solve_seconds += now() - ts;
+    counters.solves += 1;
+    counters.cg_iters += st.iters;
+    counters.outer_sweeps += st.outer;
+  }
   if (e) return e;
   if (pl) {
     e = launch_pl_strain(ctx->c, u_d, gxy, gxz, pl->T, pl->A, pl->B, pl->n, bcL_d, bcR_d, pl->cap,
@@ -223,7 +253,203 @@ int QdDriver::d_dt(double t, const double* psi, const double* slip, double* dpsi
     e = launch_qd_fault(ctx->c, pl ? nullptr : u_d, fa, dslip, s);
     if (e) return e;
   }
-  counters.total_seconds += now() - t0;
+  if (count) counters.total_seconds += now() - t0;
+  return SC_OK;
+}
+
+int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newDt, double* dts, int dtsCap,
+                           int& nOut, bool* finished, bool* halted) {
+  *finished = *halted = false;
+  hipStream_t s = ctx->c.stream;
+  if (!ctl_d) {
+    SC_HIP(hipMalloc(&ctl_d, sizeof(StepCtl)));
+    SC_HIP(hipHostMalloc(&ctl_h, sizeof(StepCtl), hipHostMallocDefault));
+  }
+  if (dts && dtsCap > ctlDtsCap) {
+    if (ctlDts_d) SC_HIP(hipFree(ctlDts_d));
+    SC_HIP(hipMalloc(&ctlDts_d, (size_t)dtsCap * sizeof(double)));
+    ctlDtsCap = dtsCap;
+  }
+  // the lo/hi pair and the error fields (all fields are short here)
+  const char* names[4] = {"psi", "slip", "gVxy", "gVxz"};
+  StepCtl h{};
+  h.t = r.currT;
+  h.dt = r.deltaT;
+  if (h.t + h.dt > finalT) h.dt = finalT - h.t;
+  h.newDt = newDt;
+  h.tol = r.tol;
+  h.minDt = r.minDt;
+  h.maxDt = r.maxDt;
+  h.finalT = finalT;
+  h.kappa = r.kappa;
+  h.errA0 = r.errA[0];
+  h.errA1 = r.errA[1];
+  h.stepCount = r.stepCount;
+  h.maxSteps = maxSteps;
+  h.pidBase = r.pidBase;
+  h.control = r.control;
+  h.normType = r.normType;
+  h.ord = r.ord;
+  h.nOut = nOut;
+  h.dtsCap = dts ? dtsCap : 0;
+  h.dts = ctlDts_d;
+  VarsLoHi S;
+  S.ntlo = 2;
+  S.nthi = 3;
+  S.clo[0] = 0.5; S.clo[1] = 0.5;                        // 0.5 h, 0.5 h
+  S.chi[0] = 1; S.chi[1] = 2; S.chi[2] = 1;              // h / 6.0, 2 h / 3.0, h / 6.0
+  S.cdhi[0] = 6; S.cdhi[1] = 3; S.cdhi[2] = 6;
+  VarsCopy cp;
+  for (int fi = 0; fi < 4; ++fi) {
+    if (!vars.count(names[fi])) continue;
+    VecSet& v = vars[names[fi]];
+    const int q = S.nv++;
+    S.n[q] = v.n;
+    S.base[q] = v.var;
+    S.lo[q] = v.lo;
+    S.hi[q] = v.hi;
+    S.flo[q][0] = v.dvar; S.flo[q][1] = v.f[2];
+    S.fhi[q][0] = v.dvar; S.fhi[q][1] = v.f[1]; S.fhi[q][2] = v.f[2];
+    const bool sel = (errMask & (1 << fi)) != 0;
+    S.out[q] = sel ? err_d + 3 * h.nsel : err_d + 12;
+    if (sel) h.selN[h.nsel++] = v.n;
+    cp.n[cp.nv] = v.n;
+    cp.dst[cp.nv] = v.var;
+    cp.src[cp.nv] = v.hi;
+    cp.nv++;
+  }
+  *ctl_h = h;
+  SC_HIP(hipMemcpyAsync(ctl_d, ctl_h, sizeof(StepCtl), hipMemcpyHostToDevice, s));
+  // the stage combinations as templates of h: k1 = var + (0.5 h) f0,
+  // k2 = var + (-h) f0 + (2 h) f1
+  auto comb = [&](int st, int nt, const int* fidx, const double* c) {
+    VarsLinComb L;
+    L.nt = nt;
+    for (int i = 0; i < nt; ++i) L.c[i] = c[i];
+    for (auto& kv : vars) {
+      VecSet& v = kv.second;
+      L.n[L.nv] = v.n;
+      L.out[L.nv] = v.k[st];
+      L.base[L.nv] = v.var;
+      for (int i = 0; i < nt; ++i) L.f[L.nv][i] = fidx[i] == 0 ? v.dvar : v.f[fidx[i]];
+      ++L.nv;
+    }
+    return L;
+  };
+  int slipIdx = 0;
+  for (auto& kv : vars) {
+    if (kv.first == "slip") break;
+    ++slipIdx;
+  }
+  const int f1[1] = {0}, f2[2] = {0, 1};
+  const double c1[1] = {0.5}, c2[2] = {-1.0, 2.0};
+  const VarsLinComb L1 = comb(1, 1, f1, c1), L2 = comb(2, 2, f2, c2);
+  VecSet& P = vars["psi"];
+  VecSet& Q = vars["slip"];
+  const StepGate g1{ctl_d, 1}, g2{ctl_d, 2};
+  const int nOut0 = nOut;
+  int prevAcc = 0, prevRej = 0;
+  defer = true;
+  // one attempt: both stages, the lo/hi pair, the control kernel, the rates
+  // after an accepted step
+  auto slot = [&]() -> int {
+    int e = d_dt(0.0, P.k[1], Q.k[1], P.f[1], Q.f[1], nullptr, nullptr, nullptr, nullptr, &L1, slipIdx, g1, 0.5);
+    if (!e) e = d_dt(0.0, P.k[2], Q.k[2], P.f[2], Q.f[2], nullptr, nullptr, nullptr, nullptr, &L2, slipIdx, g1, 1.0);
+    if (!e) e = launch_vars_lohi_err(s, S, finPending ? &fin : nullptr, g1);
+    finPending = false;
+    if (!e) e = launch_step_control(s, ctl_d, err_d, status_d, cp);
+    if (!e) e = d_dt(0.0, P.var, Q.var, P.dvar, Q.dvar, nullptr, nullptr, nullptr, nullptr, nullptr, -1, g2, 0.0);
+    return e;
+  };
+  // The first attempt goes in directly (it also takes any residual test left
+  // from the host's last solve); every later one repeats the same launches
+  // with the same arguments (all that changes lives in the control block),
+  // so one captured attempt is replayed as a graph.  The capture holds no
+  // memset nodes.
+  int e = slot();
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (!e) {
+    SC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    e = slot();
+    hipError_t ce = hipStreamEndCapture(s, &graph);
+    if (!e && ce != hipSuccess) e = fail(SC_ERR_HIP, std::string("stream capture: ") + hipGetErrorString(ce));
+    if (!e && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+      e = fail(SC_ERR_HIP, "graph instantiation failed");
+  }
+  struct GraphGuard {
+    hipGraph_t& g;
+    hipGraphExec_t& x;
+    ~GraphGuard() {
+      if (x) (void)hipGraphExecDestroy(x);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } gguard{graph, exec};
+  bool first = true;
+  for (;;) {
+    if (e) break;
+    const int left = maxSteps - ctl_h->stepCount + 1;
+    const int slots = std::max(1, std::min(kSlotsPerBatch, left + 1)) - (first ? 1 : 0);
+    first = false;
+    for (int k = 0; k < slots; ++k) {
+      if (hipGraphLaunch(exec, s) != hipSuccess) {
+        e = fail(SC_ERR_HIP, "graph launch failed");
+        break;
+      }
+    }
+    finPending = true;  // the last replayed attempt's rates
+    if (e) break;
+    SC_HIP(hipMemcpyAsync(ctl_h, ctl_d, sizeof(StepCtl), hipMemcpyDeviceToHost, s));
+    SC_HIP(hipStreamSynchronize(s));
+    // what ran since the last read-back: two stage solves per attempt and
+    // one more per accepted step
+    const int acc = ctl_h->nOut - nOut0, rej = ctl_h->rejected;
+    const int dAcc = acc - prevAcc, dRej = rej - prevRej;
+    const long long ran = 2LL * (dAcc + dRej + (ctl_h->halt ? 1 : 0)) + dAcc;
+    prevAcc = acc;
+    prevRej = rej;
+    counters.d_dt_calls += ran;
+    counters.solves += ran;
+    counters.deferred_checks += ran;
+    counters.rejected_steps += dRej;
+    counters.device_steps += dAcc;
+    if (ctl_h->halt) break;
+    hits += (int)ran;
+    if (ctl_h->done) break;
+  }
+  defer = false;
+  if (e) return e;
+  const StepCtl& c = *ctl_h;
+  r.currT = c.t;
+  r.deltaT = c.dt;
+  r.errA[0] = c.errA0;
+  r.errA[1] = c.errA1;
+  r.stepCount = c.stepCount;
+  newDt = c.newDt;
+  if (dts && c.nOut > nOut0) {
+    const int hi = std::min(c.nOut, dtsCap);
+    if (hi > nOut0)
+      SC_HIP(hipMemcpy(dts + nOut0, ctlDts_d + nOut0, (size_t)(hi - nOut0) * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  nOut = c.nOut;
+  if (c.halt) {
+    // the attempt in progress read a set flag: a root-solve failure ends the
+    // run, a missed guess sends the step back to the host
+    finPending = false;  // the flags are cleared below; a repeated test would set them again
+    int st = 0, miss = 0;
+    if ((e = read_flags(&st, &miss))) return e;
+    note_checked(true);
+    if (st) {
+      (void)hipMemsetAsync(status_d, 0, 2 * sizeof(int), s);
+      return fail(st & SC_ERR_NONFINITE ? SC_ERR_NONFINITE : SC_ERR_NOCONV,
+                  "fault root solve failed (status " + std::to_string(st) + ")");
+    }
+    counters.missed_guesses++;
+    SC_HIP(hipMemsetAsync(status_d, 0, 2 * sizeof(int), s));
+    *halted = true;
+    return SC_OK;
+  }
+  *finished = true;
   return SC_OK;
 }
 
@@ -307,7 +533,8 @@ int QdDriver::lohi_error(const RkState& r, int ntlo, const int* filo, const doub
       if (slot[fi] >= 0) longSel.push_back(fi);
     }
   }
-  int e = launch_vars_lohi_err(s, S);
+  int e = launch_vars_lohi_err(s, S, finPending ? &fin : nullptr);
+  finPending = false;
   if (!e && Llo.nv) e = launch_vars_lincomb(s, Llo);
   if (!e && Lhi.nv) e = launch_vars_lincomb(s, Lhi);
   for (int fi : longSel) {
@@ -367,6 +594,29 @@ int QdDriver::lincomb_all(const char* out_sel, int stage, int nt, const int* fid
   return launch_vars_lincomb(ctx->c.stream, L);
 }
 
+int QdDriver::stage_comb(int st, double t, int nt, const int* fidx, const double* c) {
+  if (pl) {
+    const int e = lincomb_all("k", st, nt, fidx, c);
+    return e ? e : stage(st, t);
+  }
+  VarsLinComb L;
+  L.nt = nt;
+  for (int i = 0; i < nt; ++i) L.c[i] = c[i];
+  int slipIdx = -1;
+  for (auto& kv : vars) {
+    VecSet& v = kv.second;
+    if (kv.first == "slip") slipIdx = L.nv;
+    L.n[L.nv] = v.n;
+    L.out[L.nv] = v.k[st];
+    L.base[L.nv] = v.var;
+    for (int i = 0; i < nt; ++i) L.f[L.nv][i] = fidx[i] == 0 ? v.dvar : v.f[fidx[i]];
+    ++L.nv;
+  }
+  VecSet& p = vars["psi"];
+  VecSet& q = vars["slip"];
+  return d_dt(t, p.k[st], q.k[st], p.f[st], q.f[st], nullptr, nullptr, nullptr, nullptr, &L, slipIdx);
+}
+
 int QdDriver::stage(int st, double t) {
   VecSet& p = vars["psi"];
   VecSet& q = vars["slip"];
@@ -408,7 +658,9 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
     return SC_OK;
   }
   if (r.deltaT == 0) r.deltaT = (finalT - r.currT) / run.maxSteps;
-  // a run that ended in an error may have left the device flags set
+  // a run that ended in an error may have left the device flags set (and a
+  // residual test pending)
+  finPending = false;
   SC_HIP(hipMemsetAsync(status_d, 0, 2 * sizeof(int), s));
   // solves with a projected guess are checked on the device; the flag is read
   // with the error estimate of each attempt
@@ -423,10 +675,25 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
   if (e) return e;
   double totErr = 0, newDt = r.deltaT;
   int nOut = 0;
+  // RK32 of the linear-elastic mediator with deferred checks runs its step
+  // control on the device (no per-step read-back) unless the caller asks
+  // otherwise or something needs the host at every step
+  const bool deviceLoop = run.stepControl == 0 && !replay && !out && !hook && !pl && r.ord == 3 && may_defer &&
+                          desc.solve.reuse > 0;
   while (r.stepCount < maxSteps && r.currT < finalT) {
     r.stepCount++;
     int attempt = 0;
     bool checked = false;  // this step's solves are checked on the host
+    if (deviceLoop && (always || defer_ok()) && ctx->c.projCount > 0) {
+      bool finished = false, halted = false;
+      if ((e = device_steps(r, finalT, maxSteps, newDt, dts, dtsCap, nOut, &finished, &halted))) return e;
+      if (finished) break;
+      // a missed guess halted it: this step on the host with checked solves,
+      // its starting rates first
+      checked = true;
+      defer = false;
+      if ((e = d_dt_vars(r.currT))) return e;
+    }
     while (attempt < 100) {
       attempt++;
       defer = may_defer && !checked && (always || defer_ok());
@@ -435,10 +702,8 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
       const double h = r.deltaT;
       bool missed = false;
       if (r.ord == 3) {
-        { const int fi[1] = {0}; const double c[1] = {0.5 * h}; if ((e = lincomb_all("k", 1, 1, fi, c))) return e; }
-        if ((e = stage(1, r.currT + 0.5 * h))) return e;
-        { const int fi[2] = {0, 1}; const double c[2] = {-h, 2 * h}; if ((e = lincomb_all("k", 2, 2, fi, c))) return e; }
-        if ((e = stage(2, r.currT + h))) return e;
+        { const int fi[1] = {0}; const double c[1] = {0.5 * h}; if ((e = stage_comb(1, r.currT + 0.5 * h, 1, fi, c))) return e; }
+        { const int fi[2] = {0, 1}; const double c[2] = {-h, 2 * h}; if ((e = stage_comb(2, r.currT + h, 2, fi, c))) return e; }
         const int flo[2] = {0, 2}, fhi[3] = {0, 1, 2};
         const double clo[2] = {0.5 * h, 0.5 * h}, chi[3] = {h / 6.0, 2 * h / 3.0, h / 6.0};
         if ((e = lohi_error(r, 2, flo, clo, 3, fhi, chi, &totErr, &missed))) return e;
@@ -457,16 +722,11 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
         const double a61 = 647845179188. / 3216320057751., a62 = 73281519250. / 8382639484533.,
                      a63 = 552539513391. / 3454668386233., a64 = 3354512671639. / 8306763924573.,
                      a65 = 4040. / 17871.;
-        { const int fi[1] = {0}; const double c[1] = {a21 * h}; if ((e = lincomb_all("k", 2, 1, fi, c))) return e; }
-        if ((e = stage(2, r.currT + c2 * h))) return e;
-        { const int fi[2] = {0, 2}; const double c[2] = {a31 * h, a32 * h}; if ((e = lincomb_all("k", 3, 2, fi, c))) return e; }
-        if ((e = stage(3, r.currT + c3 * h))) return e;
-        { const int fi[3] = {0, 2, 3}; const double c[3] = {a41 * h, a42 * h, a43 * h}; if ((e = lincomb_all("k", 4, 3, fi, c))) return e; }
-        if ((e = stage(4, r.currT + c4 * h))) return e;
-        { const int fi[4] = {0, 2, 3, 4}; const double c[4] = {a51 * h, a52 * h, a53 * h, a54 * h}; if ((e = lincomb_all("k", 5, 4, fi, c))) return e; }
-        if ((e = stage(5, r.currT + c5 * h))) return e;
-        { const int fi[5] = {0, 2, 3, 4, 5}; const double c[5] = {a61 * h, a62 * h, a63 * h, a64 * h, a65 * h}; if ((e = lincomb_all("k", 6, 5, fi, c))) return e; }
-        if ((e = stage(6, r.currT + c6 * h))) return e;
+        { const int fi[1] = {0}; const double c[1] = {a21 * h}; if ((e = stage_comb(2, r.currT + c2 * h, 1, fi, c))) return e; }
+        { const int fi[2] = {0, 2}; const double c[2] = {a31 * h, a32 * h}; if ((e = stage_comb(3, r.currT + c3 * h, 2, fi, c))) return e; }
+        { const int fi[3] = {0, 2, 3}; const double c[3] = {a41 * h, a42 * h, a43 * h}; if ((e = stage_comb(4, r.currT + c4 * h, 3, fi, c))) return e; }
+        { const int fi[4] = {0, 2, 3, 4}; const double c[4] = {a51 * h, a52 * h, a53 * h, a54 * h}; if ((e = stage_comb(5, r.currT + c5 * h, 4, fi, c))) return e; }
+        { const int fi[5] = {0, 2, 3, 4, 5}; const double c[5] = {a61 * h, a62 * h, a63 * h, a64 * h, a65 * h}; if ((e = stage_comb(6, r.currT + c6 * h, 5, fi, c))) return e; }
         const int fi[5] = {0, 3, 4, 5, 6};
         const double clo[5] = {hb1 * h, hb3 * h, hb4 * h, hb5 * h, hb6 * h};
         const double chi[5] = {b1 * h, b3 * h, b4 * h, b5 * h, b6 * h};
@@ -531,6 +791,8 @@ int QdDriver::max_ratio(double t, double* R) {
 
 int QdDriver::settle(double t) {
   hipStream_t s = ctx->c.stream;
+  const int ef = flush_finish();
+  if (ef) return ef;
   SC_HIP(hipMemcpyAsync(host_scratch, status_d, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
   const bool missed = ((const int*)host_scratch)[1] != 0;
@@ -545,7 +807,7 @@ int QdDriver::settle(double t) {
     if (e) return e;
     SC_HIP(hipStreamSynchronize(s));
   }
-  return SC_OK;
+  return materialize_u();
 }
 
 }  // namespace sc
@@ -663,6 +925,8 @@ extern "C" int sc_qd_get_state(sc_qd* h, double* psi_h, double* slip_h, double* 
 extern "C" int sc_qd_get_u(sc_qd* h, double* u_h) {
   if (!h || !u_h) return fail(SC_ERR_ARG, "null argument");
   auto& q = *h->q;
+  const int e = q.materialize_u();
+  if (e) return e;
   SC_HIP(hipStreamSynchronize(q.ctx->c.stream));
   SC_HIP(hipMemcpy(u_h, q.u_d, (size_t)q.N * sizeof(double), hipMemcpyDeviceToHost));
   return SC_OK;
@@ -689,6 +953,7 @@ extern "C" int sc_qd_integrate(sc_qd* h, const sc_qd_run* run, double* tEnd_h, i
   if (run->integrator != 0 && run->integrator != 1) return fail(SC_ERR_ARG, "integrator must be RK32 (0) or RK43 (1)");
   if (!(run->tol >= 1e-14)) return fail(SC_ERR_ARG, "timeStepTol must be >= 1e-14");
   if (run->deferChecks < 0 || run->deferChecks > 2) return fail(SC_ERR_ARG, "deferChecks must be 0, 1 or 2");
+  if (run->stepControl != 0 && run->stepControl != 1) return fail(SC_ERR_ARG, "stepControl must be 0 or 1");
   double tEnd = 0;
   int steps = 0;
   int e = h->q->integrate(*run, &tEnd, &steps, dts_h, dtsCap);

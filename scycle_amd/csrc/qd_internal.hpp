@@ -30,6 +30,9 @@ struct VarsLinComb {
   const double* base[kMaxVars] = {};
   const double* f[kMaxVars][6] = {};
   double c[6] = {};
+  // under a step gate c[i] is a template: the coefficient is (c[i] h) / cd[i]
+  // with h the control block's step (the order of the host's h/6.0, 2*h/3.0)
+  double cd[6] = {1, 1, 1, 1, 1, 1};
 };
 struct VarsLoHi {
   int nv = 0, ntlo = 0, nthi = 0;
@@ -40,15 +43,34 @@ struct VarsLoHi {
   const double* flo[kMaxVars][6] = {};
   const double* fhi[kMaxVars][6] = {};
   double clo[6] = {}, chi[6] = {};
+  double cdlo[6] = {1, 1, 1, 1, 1, 1}, cdhi[6] = {1, 1, 1, 1, 1, 1};  // as VarsLinComb::cd
   double* out[kMaxVars] = {};
 };
-int launch_vars_lohi_err(hipStream_t s, const VarsLoHi& L);
+// dst_v = src_v for every field (the accepted solution into var)
+struct VarsCopy {
+  int nv = 0;
+  int n[kMaxVars] = {};
+  double* dst[kMaxVars] = {};
+  const double* src[kMaxVars] = {};
+};
+// The control kernel of the device RK32 loop: error norm from the lo/hi sums,
+// acceptance, the PID step size (odeSolver.cpp:395-445, 449-501, 505-627),
+// the accepted solution into var; flags = (fault status, missed guess)
+int launch_step_control(hipStream_t s, StepCtl* ctl, const double* err, const int* flags, const VarsCopy& cp);
+// fin (optional): a pending residual test run as one more block
+int launch_vars_lohi_err(hipStream_t s, const VarsLoHi& L, const PendingFinish* fin = nullptr,
+                         StepGate gate = StepGate{});
 // plan (optional, from proj_frame_plan): also write the projections of the
-// rhs on the basis; *mnb gets the number of partial blocks (0 without)
+// rhs on the basis; *mnb gets the number of partial blocks (0 without).
+// pre (optional): the Runge-Kutta combination that forms this stage's state
+// (done in the same launch with a plan, before it without), field preSlip
+// being the slip the rhs is formed from; fin (optional): a pending residual
+// test, run in the same launch with a plan, before it without.
 int launch_qd_rhs_frame(const Ctx& c, double* rhs, const double* slip, const double* shift, double t,
                         double vL, double fts, int remote, double* bcL, double* bcR, const double* bcT,
                         const double* bcB, hipStream_t s, const ProjFramePlan* plan = nullptr,
-                        int* mnb = nullptr);
+                        int* mnb = nullptr, const VarsLinComb* pre = nullptr, int preSlip = -1,
+                        const PendingFinish* fin = nullptr, StepGate gate = StepGate{}, double tfrac = 0.0);
 int launch_qd_fault(const Ctx& c, const double* u, const sc_fault_qd_args& a, double* dslip, hipStream_t s);
 int launch_vars_lincomb(hipStream_t s, const VarsLinComb& L);
 // out[0] = max over k of (eta[k] V[k]) / tauQS[k] (the switching ratio of the
@@ -143,9 +165,12 @@ struct QdDriver {
   int init_powerlaw(const double* T, const double* A, const double* B, const double* n,
                     double cap, int errMask);
   // d_dt of the mediator; gxy.. are the viscous strains and their rates (power law only)
+  // pre (linear-elastic mediator only): the Runge-Kutta combination that
+  // forms psi and slip (field preSlip) of this stage, done inside the rhs launch
   int d_dt(double t, const double* psi, const double* slip, double* dpsi, double* dslip,
            const double* gxy = nullptr, const double* gxz = nullptr, double* dgxy = nullptr,
-           double* dgxz = nullptr);
+           double* dgxz = nullptr, const VarsLinComb* pre = nullptr, int preSlip = -1,
+           StepGate gate = StepGate{}, double tfrac = 0.0);
   int d_dt_vars(double t);              // on var -> dvar of every integrated variable
   int check_status();
   // fault status and missed-guess flag in one read-back (the flags are left set)
@@ -156,6 +181,31 @@ struct QdDriver {
                  const int* fihi, const double* chi, double* totErr, bool* missed);
   int lincomb_all(const char* out_sel, int stage, int nt, const int* fidx, const double* c);
   int stage(int st, double t);
+  // k[st] = var + sum_i c[i] f_i, then the rates there (one launch fewer
+  // than lincomb_all + stage for the linear-elastic mediator)
+  int stage_comb(int st, double t, int nt, const int* fidx, const double* c);
+  // The residual test of the last deferred guess, postponed into the next
+  // rhs or lo/hi launch (PendingFinish); flush_finish launches it on its own
+  // before anything that reads the flags.
+  PendingFinish fin{};
+  bool finPending = false;
+  int flush_finish();
+  // u_d of the last deferred solve is formed only on demand (solve's x_lazy)
+  bool uLazy = false;
+  int materialize_u();
+  // RK32 step control on the device (linear-elastic mediator, deferred
+  // checks, no replay / output / hook): the host enqueues attempts in
+  // batches and reads the control block once per batch.
+  StepCtl* ctl_d = nullptr;
+  StepCtl* ctl_h = nullptr;           // pinned
+  double* ctlDts_d = nullptr;
+  int ctlDtsCap = 0;
+  static constexpr int kSlotsPerBatch = 24;
+  // runs steps from the current one (r.stepCount, started) on the device
+  // until the run ends (*finished) or a missed guess / fault failure halts
+  // it (*halted: the host redoes the current step with checked solves)
+  int device_steps(RkState& r, double finalT, int maxSteps, double& newDt, double* dts, int dtsCap, int& nOut,
+                   bool* finished, bool* halted);
   int integrate(const sc_qd_run& run, double* tEnd, int* steps, double* dts, int dtsCap);
   // the rates at time t were computed with deferred guess checks: read the
   // flags and redo them with checked solves if a guess missed

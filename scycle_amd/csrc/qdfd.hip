@@ -292,6 +292,7 @@ int QdFdDriver::prepare_fd2qd() {
     hipLaunchKernelGGL(k_total_u<double>, dim3(nblk(N)), dim3(256), 0, s, (const double*)f->u[f->cur], u0,
                        q->u_d, (size_t)N);
   SC_HIP(hipGetLastError());
+  q->uLazy = false;  // u_d now holds the total displacement
   SC_HIP(hipStreamSynchronize(s));
   return SC_OK;
 }
@@ -301,6 +302,7 @@ int QdFdDriver::prepare_qd2fd(const sc_qdfd_run& run) {
   hipStream_t sq = q->ctx->c.stream;
   // u at time n-1, then one QD step of deltaT_fd to time n (with the
   // switching monitor in the QD regime, as there)
+  if (int e0 = q->materialize_u()) return e0;
   SC_HIP(hipMemcpyAsync(uStart, q->u_d, (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, sq));
   inDynamic = false;
   sc_qd_run r{};

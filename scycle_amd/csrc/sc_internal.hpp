@@ -165,14 +165,58 @@ struct QdFuse {
   double* dslip;
 };
 
+// Runge-Kutta step control on the device (the QD driver's device loop, see
+// qd.cpp): the state of odeSolver.cpp's RK32 loop between attempts, updated
+// by one control kernel per attempt.  The step kernels read t and dt from it.
+struct StepCtl {
+  double t, dt, newDt, tol, minDt, maxDt, finalT, kappa, errA0, errA1;
+  int stepCount, maxSteps, pidBase, control, normType, ord;
+  int attempts, halt, accepted, done, rejected, nOut, dtsCap, nsel;
+  int selN[4];   // lengths of the selected error fields, in error-slot order
+  double* dts;   // accepted step sizes, indexed by nOut (dtsCap entries)
+};
+// Which step kernels run: mode 0 always; 1 a stage of the current attempt
+// (not once halted or done); 2 the rates after an accepted attempt.
+struct StepGate {
+  const StepCtl* ctl = nullptr;
+  int mode = 0;
+};
+
+// Rows of the QD driver's rhs band next to each y-side (the rhs vanishes
+// elsewhere: its top and bottom data are zero)
+inline void qd_band_rows(int Ny, int* nLow, int* nHigh) {
+  *nLow = Ny < kBand ? Ny : kBand;
+  *nHigh = Ny > kBand ? (Ny - kBand < kBand ? Ny - kBand : kBand) : 0;
+}
+
+// The residual test of a deferred projected guess, left for a later launch
+// (cg_dev.hpp): the partials of nb blocks, the scalars, the missed-guess flag.
+struct CgScal;
+struct PendingFinish {
+  CgScal* sc = nullptr;
+  const double* partials = nullptr;
+  int nb = 0;
+  int* flag = nullptr;
+};
+// launches a pending test on its own (one block)
+int launch_pending_finish(hipStream_t s, const PendingFinish& f);
+
 // miss_flag/deferred: see cg.hip (deferred residual check of a projected
 // guess); rhs_frame: rhs is zero off the nodes within kFrame of a boundary;
 // pre_mnb > 0: the caller already wrote the projections y_m . rhs as pre_mnb
 // block partials (see proj_frame_plan); fuse (deferred solves only): also
-// compute the fault rates from the guess, *fused tells whether it did.
+// compute the fault rates from the guess, *fused tells whether it did;
+// postpone (deferred solves only): the residual test is not launched but
+// described there, for the caller to run in a later launch before anything
+// else writes the reduction partials.
 int solve(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st,
           int* miss_flag = nullptr, bool* deferred = nullptr, bool rhs_frame = false, int pre_mnb = 0,
-          const QdFuse* fuse = nullptr, bool* fused = nullptr);
+          const QdFuse* fuse = nullptr, bool* fused = nullptr, PendingFinish* postpone = nullptr,
+          StepGate gate = StepGate{}, bool* x_lazy = nullptr);
+// x_lazy (deferred, fused solves): u is left unwritten (*x_lazy = true) and
+// proj_materialize writes it from the guess's coefficients when needed;
+// valid until the next solve on the context
+int proj_materialize(Ctx& c, double* u);
 
 // For a caller that forms a frame-supported rhs and its projections on the
 // basis in one kernel: M basis vectors Y (M = 0: no projected guess for this

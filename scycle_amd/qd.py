@@ -70,15 +70,18 @@ class QdMediator:
 
     def integrate(self, max_steps, integrator="RK32", tol=1e-8, init_dt=1e-3, min_dt=0.0,
                   max_dt=1e10, init_time=0.0, max_time=1e15, control="PID",
-                  norm="L2_absolute", replay_dt=None, defer="auto", curr_err=0.0, prev_err=0.0):
+                  norm="L2_absolute", replay_dt=None, defer="auto", curr_err=0.0, prev_err=0.0,
+                  step_control="auto"):
         """defer: "auto", "never" or "always" (sc_qd_run.deferChecks);
         curr_err, prev_err: the PID error history at the start (a restart's
-        chkpt_currErr / chkpt_prevErr, see load_checkpoint)."""
+        chkpt_currErr / chkpt_prevErr, see load_checkpoint); step_control:
+        "auto" (on the device where it can) or "host" (sc_qd_run.stepControl)."""
         rep = None if replay_dt is None else np.ascontiguousarray(replay_dt, dtype=np.float64)
         r = QdRun(0 if integrator == "RK32" else 1, 0 if control == "P" else 1,
                   0 if norm == "L2_absolute" else 1, int(max_steps), init_time, max_time,
                   init_dt, min_dt, max_dt, tol, 0 if rep is None else len(rep), dptr(rep),
-                  {"auto": 0, "never": 1, "always": 2}[defer], curr_err, prev_err)
+                  {"auto": 0, "never": 1, "always": 2}[defer], curr_err, prev_err,
+                  {"auto": 0, "host": 1}[step_control])
         tEnd = C.c_double()
         steps = C.c_int()
         dts = np.zeros(max(int(max_steps), 1))

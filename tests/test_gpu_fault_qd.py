@@ -242,3 +242,50 @@ def test_qd_integrate_projected_guess(integrator, reuse, defer):
     if defer == "always":
         assert c["missed_guesses"] > 0
     g.close()
+
+
+@pytest.mark.parametrize("defer", ["auto", "always"])
+def test_qd_device_step_control(defer):
+    """RK32 with the step control on the device (the host enqueues attempts
+    in batches and reads the control block once per batch) takes the steps
+    the host-controlled loop takes: the controller is the same arithmetic,
+    pow aside, so step sizes agree to rounding and the fields to 1e-10.
+    With defer "always" the first guesses miss, the device loop halts and
+    the host redoes those steps with checked solves, then resumes.  Both
+    runs also match the oracle's free adaptive run as
+    test_qd_integrate_parity does (step sizes 1e-6, fields 1e-7)."""
+    from scycle_amd import QdMediator
+    N = 25
+    cfg = _small_cfg(N, N, 4, True)
+    steps = 80
+    out = {}
+    for mode in ("host", "auto"):
+        g = QdMediator(cfg, solve="cg64", rtol=1e-13, jac_mode=0, precond="fdiag64", reuse=2)
+        g.set_state(cfg.psi0, cfg.slip0, np.zeros(N))
+        tEnd, nst, dts = g.integrate(steps, tol=1e-8, init_dt=1e-3, max_dt=1e10, defer=defer, step_control=mode)
+        out[mode] = dict(t=tEnd, n=nst, dts=dts, st=g.get_state(), u=g.get_u(), c=g.counters())
+        g.close()
+    h, d = out["host"], out["auto"]
+    print(defer, "device steps", d["c"]["device_steps"], "missed", d["c"]["missed_guesses"], "host missed",
+          h["c"]["missed_guesses"], flush=True)
+    assert h["c"]["device_steps"] == 0 and d["c"]["device_steps"] > steps // 2
+    assert h["n"] == d["n"] == steps
+    assert np.allclose(d["dts"], h["dts"], rtol=1e-12, atol=0)
+    assert abs(d["t"] / h["t"] - 1) <= 1e-12
+    for k in ("psi", "slip", "V", "tau"):
+        ref = h["st"][k]
+        assert np.abs(d["st"][k] - ref).max() <= 1e-10 * np.abs(ref).max(), k
+    assert np.abs(d["u"] - h["u"]).max() <= 1e-10 * np.abs(h["u"]).max()
+    # the oracle's free run
+    hq, keep, _ = _oracle_qd(cfg)
+    r = _Run(0, 1, 0, steps, 0.0, 1e15, 1e-3, 0.0, 1e10, 1e-8, 0, None)
+    ref = dict(psi=cfg.psi0.copy(), slip=cfg.slip0.copy(), V=np.zeros(N), tau=np.zeros(N), u=np.zeros(N * N))
+    tE = C.c_double()
+    odts = np.zeros(steps)
+    ns = olib().orc_qd_integrate(hq, C.byref(r), optr(ref["psi"]), optr(ref["slip"]), optr(ref["V"]),
+                                 optr(ref["tau"]), optr(ref["u"]), C.byref(tE), optr(odts), steps, None)
+    olib().orc_qd_free(hq)
+    assert ns == steps
+    assert np.allclose(d["dts"], odts, rtol=1e-6) and abs(d["t"] / tE.value - 1) <= 1e-6
+    e = _fields_err(d["st"], d["u"], ref)
+    assert max(e.values()) <= 1e-7, e
