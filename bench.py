@@ -67,7 +67,7 @@ PATHS = {
 }
 
 
-def run_path(cfg, label, warmup, steps, reuse=0):
+def run_path(cfg, label, warmup, steps, reuse=0, step_control="auto"):
     from scycle_amd import QdMediator
     mode, prec, _ = PATHS[label]
     tic = time.perf_counter()
@@ -78,12 +78,13 @@ def run_path(cfg, label, warmup, steps, reuse=0):
     t0 = 0.0
     dt0 = 1e-3
     if warmup > 0:
-        t0, n, dts = q.integrate(warmup, integrator="RK32", tol=1e-8, init_dt=dt0, max_dt=1e10)
+        t0, n, dts = q.integrate(warmup, integrator="RK32", tol=1e-8, init_dt=dt0, max_dt=1e10,
+                                 step_control=step_control)
         dt0 = float(dts[-1])
     c0 = q.counters()
     tic = time.perf_counter()
     t1, n, dts = q.integrate(steps, integrator="RK32", tol=1e-8, init_dt=dt0, init_time=t0,
-                             max_dt=1e10)
+                             max_dt=1e10, step_control=step_control)
     sec = time.perf_counter() - tic
     c1 = q.counters()
     st = q.get_state()
@@ -282,8 +283,10 @@ def kernels_only(n, reps, reuse=4):
         y.copy_(x)
     torch.cuda.synchronize()
     # the QD window of the benchmark on the fp64_fdiag path (so that the basis
-    # ends with the same number of vectors), then the guess kernel alone
-    q, _ = run_path(cfg, "fp64_fdiag", 10, 500, reuse=reuse)
+    # ends with the same number of vectors), then the guess kernel alone; the
+    # host-side step control keeps every launch a plain dispatch for the
+    # counter runs
+    q, _ = run_path(cfg, "fp64_fdiag", 10, 500, reuse=reuse, step_control="host")
     gq = roofline_guess(load().sc_qd_ctx(q.h), reps)
     q.close()
     print(json.dumps({"apply64_ms": ms.value, "nodes": nodes.value, "copy_bytes": 16 * N,

@@ -34,15 +34,16 @@ __device__ inline void finish_irres_block(const PendingFinish& f) {
     sh[1][wid] = s1;
   }
   __syncthreads();
-  if (tid != 0) return;
-  double t0 = 0, t1 = 0;
-  for (int ww = 0; ww < kRedThreads / 64; ++ww) t0 += sh[0][ww];
-  for (int ww = 0; ww < kRedThreads / 64; ++ww) t1 += sh[1][ww];
-  CgScal* sc = f.sc;
-  sc->rr = t0;
-  sc->bb = t1;
-  sc->done = (t0 <= sc->tol2 * t1 || t1 == 0.0) ? 1 : 0;
-  if (f.flag && !sc->done) *f.flag = 1;
+  if (tid == 0) {
+    double t0 = 0, t1 = 0;
+    for (int ww = 0; ww < kRedThreads / 64; ++ww) t0 += sh[0][ww];
+    for (int ww = 0; ww < kRedThreads / 64; ++ww) t1 += sh[1][ww];
+    CgScal* sc = f.sc;
+    sc->rr = t0;
+    sc->bb = t1;
+    sc->done = (t0 <= sc->tol2 * t1 || t1 == 0.0) ? 1 : 0;
+    if (f.flag && !sc->done) *f.flag = 1;
+  }
 }
 
 // true when a step kernel under this gate has nothing to do (uniform over

@@ -38,6 +38,9 @@ QdDriver::~QdDriver() {
     (void)hipStreamSynchronize(ctx->c.stream);
     for (void* p : owned) (void)hipFree(p);
     if (host_scratch) (void)hipHostFree(host_scratch);
+    if (ctl_d) (void)hipFree(ctl_d);
+    if (ctl_h) (void)hipHostFree(ctl_h);
+    if (ctlDts_d) (void)hipFree(ctlDts_d);
     sc_ctx_destroy(ctx);
   }
 }
@@ -350,28 +353,29 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
   const int nOut0 = nOut;
   int prevAcc = 0, prevRej = 0;
   defer = true;
-  // one attempt: both stages, the lo/hi pair, the control kernel, the rates
-  // after an accepted step
+  // one attempt: both stages, the attempt's end (the last stage's residual
+  // test, the lo/hi pair, the step control: one launch), the rates after an
+  // accepted step
   auto slot = [&]() -> int {
     int e = d_dt(0.0, P.k[1], Q.k[1], P.f[1], Q.f[1], nullptr, nullptr, nullptr, nullptr, &L1, slipIdx, g1, 0.5);
     if (!e) e = d_dt(0.0, P.k[2], Q.k[2], P.f[2], Q.f[2], nullptr, nullptr, nullptr, nullptr, &L2, slipIdx, g1, 1.0);
-    if (!e) e = launch_vars_lohi_err(s, S, finPending ? &fin : nullptr, g1);
+    if (!e) e = launch_step_end(s, StepEnd{finPending ? fin : PendingFinish{}, S, ctl_d, err_d, status_d, cp}, g1);
     finPending = false;
-    if (!e) e = launch_step_control(s, ctl_d, err_d, status_d, cp);
     if (!e) e = d_dt(0.0, P.var, Q.var, P.dvar, Q.dvar, nullptr, nullptr, nullptr, nullptr, nullptr, -1, g2, 0.0);
     return e;
   };
   // The first attempt goes in directly (it also takes any residual test left
   // from the host's last solve); every later one repeats the same launches
   // with the same arguments (all that changes lives in the control block),
-  // so one captured attempt is replayed as a graph.  The capture holds no
-  // memset nodes.
+  // so a captured run of kAttemptsPerGraph attempts is replayed as a graph
+  // (attempts past the run's end find the gate closed).  The capture holds
+  // no memset nodes.
   int e = slot();
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   if (!e) {
     SC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
-    e = slot();
+    for (int a = 0; a < kAttemptsPerGraph && !e; ++a) e = slot();
     hipError_t ce = hipStreamEndCapture(s, &graph);
     if (!e && ce != hipSuccess) e = fail(SC_ERR_HIP, std::string("stream capture: ") + hipGetErrorString(ce));
     if (!e && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
@@ -391,7 +395,7 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
     const int left = maxSteps - ctl_h->stepCount + 1;
     const int slots = std::max(1, std::min(kSlotsPerBatch, left + 1)) - (first ? 1 : 0);
     first = false;
-    for (int k = 0; k < slots; ++k) {
+    for (int k = 0; k < (slots + kAttemptsPerGraph - 1) / kAttemptsPerGraph; ++k) {
       if (hipGraphLaunch(exec, s) != hipSuccess) {
         e = fail(SC_ERR_HIP, "graph launch failed");
         break;

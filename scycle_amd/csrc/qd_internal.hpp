@@ -22,7 +22,6 @@ struct VecSet {
 };
 
 // Kernel arguments of the fused field kernels (qd_kernels.hip)
-constexpr int kMaxVars = 4;
 struct VarsLinComb {
   int nv = 0, nt = 0;
   int n[kMaxVars] = {};
@@ -34,29 +33,20 @@ struct VarsLinComb {
   // with h the control block's step (the order of the host's h/6.0, 2*h/3.0)
   double cd[6] = {1, 1, 1, 1, 1, 1};
 };
-struct VarsLoHi {
-  int nv = 0, ntlo = 0, nthi = 0;
-  int n[kMaxVars] = {};
-  const double* base[kMaxVars] = {};
-  double* lo[kMaxVars] = {};
-  double* hi[kMaxVars] = {};
-  const double* flo[kMaxVars][6] = {};
-  const double* fhi[kMaxVars][6] = {};
-  double clo[6] = {}, chi[6] = {};
-  double cdlo[6] = {1, 1, 1, 1, 1, 1}, cdhi[6] = {1, 1, 1, 1, 1, 1};  // as VarsLinComb::cd
-  double* out[kMaxVars] = {};
+// The end of an RK32 attempt in one launch of one block (k_step_end): the
+// residual test the last stage's deferred guess left (fin), the lo/hi pair
+// and its error sums, the step control: error norm, acceptance, the PID
+// step size (odeSolver.cpp:395-445, 449-501, 505-627), the accepted
+// solution into var; flags = (fault status, missed guess)
+struct StepEnd {
+  PendingFinish fin{};
+  VarsLoHi L{};
+  StepCtl* ctl = nullptr;
+  double* err = nullptr;
+  const int* flags = nullptr;
+  VarsCopy cp{};
 };
-// dst_v = src_v for every field (the accepted solution into var)
-struct VarsCopy {
-  int nv = 0;
-  int n[kMaxVars] = {};
-  double* dst[kMaxVars] = {};
-  const double* src[kMaxVars] = {};
-};
-// The control kernel of the device RK32 loop: error norm from the lo/hi sums,
-// acceptance, the PID step size (odeSolver.cpp:395-445, 449-501, 505-627),
-// the accepted solution into var; flags = (fault status, missed guess)
-int launch_step_control(hipStream_t s, StepCtl* ctl, const double* err, const int* flags, const VarsCopy& cp);
+int launch_step_end(hipStream_t s, const StepEnd& e, StepGate gate);
 // fin (optional): a pending residual test run as one more block
 int launch_vars_lohi_err(hipStream_t s, const VarsLoHi& L, const PendingFinish* fin = nullptr,
                          StepGate gate = StepGate{});
@@ -201,6 +191,9 @@ struct QdDriver {
   double* ctlDts_d = nullptr;
   int ctlDtsCap = 0;
   static constexpr int kSlotsPerBatch = 24;
+  // attempts captured in one graph: a relaunch of a graph waits for its
+  // previous run, so each launch carries several attempts
+  static constexpr int kAttemptsPerGraph = 4;
   // runs steps from the current one (r.stepCount, started) on the device
   // until the run ends (*finished) or a missed guess / fault failure halts
   // it (*halted: the host redoes the current step with checked solves)

@@ -11,6 +11,8 @@
 //   k_vars_lincomb  one Runge-Kutta combination over every integrated field
 //   k_vars_lohi_err the two solutions of an embedded pair and their error
 //                   sums, one block per short field
+//   k_step_end      the end of a device-controlled RK32 attempt: the last
+//                   stage's residual test, the pair, the step control
 #include <algorithm>
 #include <cmath>
 
@@ -19,6 +21,7 @@
 #include "qd_internal.hpp"
 #include "reduce_dev.hpp"
 #include "sbp_dev.hpp"
+#include "step_dev.hpp"
 
 namespace sc {
 
@@ -218,151 +221,110 @@ int launch_vars_lincomb(hipStream_t s, const VarsLinComb& L) {
   return SC_OK;
 }
 
-// Block v: lo = base + sum_t clo[t] flo[t], hi likewise (the order of
-// k_vars_lincomb), then [sum (lo-hi)^2, sum hi^2, max |lo-hi|/hi] over the
-// pair.  kLoHiThreads threads: a field of up to that many entries takes one
-// pass.
-constexpr int kLoHiThreads = 1024;
-__global__ __launch_bounds__(kLoHiThreads) void k_vars_lohi_err(VarsLoHi L, PendingFinish fin, StepGate gate) {
-  if ((int)blockIdx.x >= L.nv) {  // a pending residual test of the last solve
+// Block v: the pair's field v; a block past the fields runs a pending
+// residual test of the last solve
+__global__ __launch_bounds__(kRedThreads) void k_vars_lohi_err(VarsLoHi L, PendingFinish fin, StepGate gate) {
+  if ((int)blockIdx.x >= L.nv) {
     if (!(gate.ctl && gate.ctl->halt)) finish_irres_block(fin);
     return;
   }
   if (gate_closed(gate)) return;
+  __shared__ double s0[kRedThreads], s1[kRedThreads], s2[kRedThreads];
   double clo[6], chi[6];
-#pragma unroll
-  for (int i = 0; i < 6; ++i) {
-    clo[i] = L.clo[i];
-    chi[i] = L.chi[i];
-  }
-  if (gate.ctl) {
-    const double h = gate.ctl->dt;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      clo[i] = (L.clo[i] * h) / L.cdlo[i];
-      chi[i] = (L.chi[i] * h) / L.cdhi[i];
-    }
-  }
-  __shared__ double s0[kLoHiThreads], s1[kLoHiThreads], s2[kLoHiThreads];
-  const int v = blockIdx.x;
-  const int n = L.n[v];
-  double a = 0, b = 0, m = -INFINITY;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const double base = L.base[v][i];
-    double lo = clo[0] * L.flo[v][0][i] + base;
-    for (int t = 1; t < L.ntlo; ++t) lo = lo + clo[t] * L.flo[v][t][i];
-    double hi = chi[0] * L.fhi[v][0][i] + base;
-    for (int t = 1; t < L.nthi; ++t) hi = hi + chi[t] * L.fhi[v][t][i];
-    L.lo[v][i] = lo;
-    L.hi[v][i] = hi;
-    const double e = -1.0 * hi + lo;
-    a += e * e;
-    b += hi * hi;
-    m = fmax(m, fabs(e) / hi);
-  }
-  s0[threadIdx.x] = a;
-  s1[threadIdx.x] = b;
-  s2[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = kLoHiThreads / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      s0[threadIdx.x] += s0[threadIdx.x + w];
-      s1[threadIdx.x] += s1[threadIdx.x + w];
-      s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]);
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    L.out[v][0] = s0[0];
-    L.out[v][1] = s1[0];
-    L.out[v][2] = s2[0];
-  }
+  lohi_coefs(L, gate.ctl, clo, chi);
+  lohi_accum(L, blockIdx.x, clo, chi, threadIdx.x, s0, s1, s2);
+  lohi_tree(s0, s1, s2, threadIdx.x, true, L.out[blockIdx.x]);
 }
 
 int launch_vars_lohi_err(hipStream_t s, const VarsLoHi& L, const PendingFinish* fin, StepGate gate) {
   const PendingFinish F = fin ? *fin : PendingFinish{};
   const int grid = L.nv + (F.sc ? 1 : 0);
   if (grid == 0) return SC_OK;
-  hipLaunchKernelGGL(k_vars_lohi_err, dim3(grid), dim3(kLoHiThreads), 0, s, L, F, gate);
+  hipLaunchKernelGGL(k_vars_lohi_err, dim3(grid), dim3(kRedThreads), 0, s, L, F, gate);
   SC_HIP(hipGetLastError());
   return SC_OK;
 }
 
-// computeStepSize (odeSolver.cpp:395-445) on the control block's state, with
-// the host's std::min / std::max order (a NaN ratio keeps the bound)
-__device__ double step_size_dev(const StepCtl& c, double totErr) {
-  double ratio;
-  if (c.control == 0 || c.stepCount - c.pidBase < 4) {
-    ratio = c.kappa * pow(c.tol / totErr, 1. / (1. + c.ord));
-  } else {
-    const double al = 0.49 / c.ord, be = 0.34 / c.ord, ga = 0.1 / c.ord;
-    ratio = c.kappa * pow(c.tol / totErr, al) * pow(c.errA0 / c.tol, be) * pow(c.tol / c.errA1, ga);
-  }
-  double dt = ratio * c.dt;
-  dt = (dt < c.dt * 5.0) ? dt : c.dt * 5.0;
-  dt = (dt < c.maxDt) ? dt : c.maxDt;
-  dt = (c.minDt < dt) ? dt : c.minDt;
-  return dt;
-}
-
-// One attempt's end (the tail of the RK32 attempt loop and the accepted-step
-// update of odeSolver.cpp:587-627): thread 0 updates the control block, the
-// block copies an accepted solution into var.  A set fault status or
-// missed-guess flag halts the loop with the state of the attempt's start.
-__global__ __launch_bounds__(1024) void k_step_control(StepCtl* c, const double* __restrict__ err,
-                                                       const int* __restrict__ flags, VarsCopy cp) {
-  __shared__ int acc;
-  if (threadIdx.x == 0) {
-    acc = 0;
-    c->accepted = 0;
-    if (!c->halt && !c->done) {
-      if (flags[0] || flags[1]) {
-        c->halt = 1;
-      } else {
-        double tot = 0;
-        for (int s = 0; s < c->nsel; ++s) {
-          const double* v = err + 3 * s;
-          if (c->normType == 0) tot += sqrt(v[0]) / (sqrt((double)c->selN[s]) * 1.0);
-          else if (c->ord == 3) tot += sqrt(v[0]) / (sqrt(v[1]) * 1.0);
-          else tot += v[2] / 1.0;
-        }
-        bool accept = c->ord == 3 ? tot <= c->tol : tot < c->tol;
-        c->attempts++;
-        if (!accept) {
-          c->dt = step_size_dev(*c, tot);
-          c->rejected++;
-          if (c->minDt == c->dt || c->attempts >= 100) accept = true;
-          else if (c->t + c->dt > c->finalT) c->dt = c->finalT - c->t;
-        }
-        if (accept) {
-          c->t = c->t + c->dt;
-          if (tot != 0.0) c->newDt = step_size_dev(*c, tot);
-          c->errA1 = c->errA0;
-          c->errA0 = tot;
-          if (c->nOut < c->dtsCap) c->dts[c->nOut] = c->dt;
-          c->nOut++;
-          c->dt = c->newDt;
-          c->accepted = 1;
-          c->attempts = 0;
-          if (!(c->stepCount < c->maxSteps && c->t < c->finalT)) {
-            c->done = 1;
-          } else {
-            c->stepCount++;
-            if (c->t + c->dt > c->finalT) c->dt = c->finalT - c->t;
-          }
-          acc = 1;
-        }
-      }
-    }
+// One block of kRedThreads (1 + nv) threads: group 0 the pending residual
+// test (not once halted), group 1 + v field v of the pair (not once halted
+// or done), side by side; then thread 0 updates the control block and an
+// accepted solution goes into var.  The sums are those of k_vars_lohi_err
+// and k_finish bit for bit.  The control block and the flags are read once
+// into LDS, the error sums pass through LDS, and var takes hi from the
+// registers that formed it (fields of up to kLoHiChunk kRedThreads entries),
+// so the chain of dependent memory accesses stays short.
+constexpr int kStepEndVars = 1024 / kRedThreads - 1;
+constexpr int kCtlWords = sizeof(StepCtl) / sizeof(unsigned long long);
+static_assert(sizeof(StepCtl) % sizeof(unsigned long long) == 0, "StepCtl is copied as 8-byte words");
+__global__ __launch_bounds__(1024) void k_step_end(StepEnd e) {
+  __shared__ double sh[3][kStepEndVars][kRedThreads];
+  __shared__ unsigned long long cw[kCtlWords];
+  __shared__ double errS[6 * kMaxVars];  // field sums, then the same by error slot
+  __shared__ int flg[2], acc;
+  StepCtl* cs = reinterpret_cast<StepCtl*>(cw);
+  if (threadIdx.x < kCtlWords) cw[threadIdx.x] = reinterpret_cast<const unsigned long long*>(e.ctl)[threadIdx.x];
+  if (threadIdx.x == kCtlWords) {
+    flg[0] = e.flags[0];
+    flg[1] = e.flags[1];
   }
   __syncthreads();
+  const bool halted = cs->halt != 0;
+  const bool closed = halted || cs->done != 0;
+  const int grp = threadIdx.x / kRedThreads, tid = threadIdx.x % kRedThreads;
+  const int v = grp - 1;
+  const bool pair = v >= 0 && v < e.L.nv && !closed;
+  double keep[kLoHiChunk];
+  if (pair) {
+    double clo[6], chi[6];
+    lohi_coefs(e.L, cs, clo, chi);
+    lohi_accum(e.L, v, clo, chi, tid, sh[0][v], sh[1][v], sh[2][v], keep);
+  }
+  // only the first kRedThreads threads accumulate in the test
+  if (e.fin.sc && !halted) finish_irres_block(e.fin);
+  if (threadIdx.x == 0 && e.fin.sc && !halted && !e.fin.sc->done) flg[1] = 1;
+  const int w = pair ? v : 0;
+  double* errOut = pair ? errS + 3 * v : nullptr;
+  lohi_tree(sh[0][w], sh[1][w], sh[2][w], tid, pair, errOut);
+  __syncthreads();
+  if (pair && tid < 3) e.L.out[v][tid] = errS[3 * v + tid];
+  if (threadIdx.x == 0) {
+    // the sums at their error slots: field q's go to out[q] = err + 3 slot
+    // (slots below kMaxVars are the selected fields)
+    double* sel = errS + 3 * kMaxVars;  // LDS: indexed at run time
+    for (int q = 0; q < 3 * kMaxVars; ++q) sel[q] = 0.0;
+    for (int q = 0; q < e.L.nv; ++q) {
+      const long long slot = (e.L.out[q] - e.err) / 3;
+      if (slot >= 0 && slot < kMaxVars) {
+        sel[3 * slot] = errS[3 * q];
+        sel[3 * slot + 1] = errS[3 * q + 1];
+        sel[3 * slot + 2] = errS[3 * q + 2];
+      }
+    }
+    acc = step_control_update(cs, sel, flg[0] || flg[1]) ? 1 : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x < kCtlWords) reinterpret_cast<unsigned long long*>(e.ctl)[threadIdx.x] = cw[threadIdx.x];
   if (!acc) return;
-  for (int v = 0; v < cp.nv; ++v)
-    for (int i = threadIdx.x; i < cp.n[v]; i += blockDim.x) cp.dst[v][i] = cp.src[v][i];
+  if (pair && e.L.n[v] <= kLoHiChunk * kRedThreads && e.cp.dst[v] == e.L.base[v]) {
+#pragma unroll
+    for (int q = 0; q < kLoHiChunk; ++q) {
+      const int i = tid + q * kRedThreads;
+      if (i < e.L.n[v]) e.cp.dst[v][i] = keep[q];
+    }
+    return;
+  }
+  if (grp != 0 || pair) return;  // a field too long for the registers: group 0 copies it
+  for (int q = 0; q < e.cp.nv; ++q) {
+    if (q < e.L.nv && e.L.n[q] <= kLoHiChunk * kRedThreads && e.cp.dst[q] == e.L.base[q]) continue;
+    for (int i = tid; i < e.cp.n[q]; i += kRedThreads) e.cp.dst[q][i] = e.cp.src[q][i];
+  }
 }
 
-int launch_step_control(hipStream_t s, StepCtl* ctl, const double* err, const int* flags, const VarsCopy& cp) {
-  hipLaunchKernelGGL(k_step_control, dim3(1), dim3(1024), 0, s, ctl, err, flags, cp);
+int launch_step_end(hipStream_t s, const StepEnd& e, StepGate gate) {
+  if (!gate.ctl || gate.ctl != e.ctl || gate.mode != 1)
+    return fail(SC_ERR_ARG, "launch_step_end: the gate must be the attempt's gate on the control block");
+  if (e.L.nv > kStepEndVars || e.cp.nv != e.L.nv) return fail(SC_ERR_ARG, "launch_step_end: field lists");
+  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(kRedThreads * (1 + e.L.nv)), 0, s, e);
   SC_HIP(hipGetLastError());
   return SC_OK;
 }

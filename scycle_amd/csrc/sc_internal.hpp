@@ -165,6 +165,29 @@ struct QdFuse {
   double* dslip;
 };
 
+// The fields of an RK pair (kernel arguments of qd_kernels.hip): lo and hi = base + sum_t c[t] f[t], their error sums
+// [sum (lo-hi)^2, sum hi^2, max |lo-hi|/hi] into out[v][0..2]
+constexpr int kMaxVars = 4;
+struct VarsLoHi {
+  int nv = 0, ntlo = 0, nthi = 0;
+  int n[kMaxVars] = {};
+  const double* base[kMaxVars] = {};
+  double* lo[kMaxVars] = {};
+  double* hi[kMaxVars] = {};
+  const double* flo[kMaxVars][6] = {};
+  const double* fhi[kMaxVars][6] = {};
+  double clo[6] = {}, chi[6] = {};
+  double cdlo[6] = {1, 1, 1, 1, 1, 1}, cdhi[6] = {1, 1, 1, 1, 1, 1};  // as VarsLinComb::cd
+  double* out[kMaxVars] = {};
+};
+// dst_v = src_v for every field (the accepted solution into var)
+struct VarsCopy {
+  int nv = 0;
+  int n[kMaxVars] = {};
+  double* dst[kMaxVars] = {};
+  const double* src[kMaxVars] = {};
+};
+
 // Runge-Kutta step control on the device (the QD driver's device loop, see
 // qd.cpp): the state of odeSolver.cpp's RK32 loop between attempts, updated
 // by one control kernel per attempt.  The step kernels read t and dt from it.
