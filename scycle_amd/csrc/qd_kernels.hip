@@ -229,11 +229,11 @@ __global__ __launch_bounds__(kRedThreads) void k_vars_lohi_err(VarsLoHi L, Pendi
     return;
   }
   if (gate_closed(gate)) return;
-  __shared__ double s0[kRedThreads], s1[kRedThreads], s2[kRedThreads];
-  double clo[6], chi[6];
+  __shared__ double sh[3 * (kRedThreads / 64)];
+  double clo[6], chi[6], a, b, m;
   lohi_coefs(L, gate.ctl, clo, chi);
-  lohi_accum(L, blockIdx.x, clo, chi, threadIdx.x, s0, s1, s2);
-  lohi_tree(s0, s1, s2, threadIdx.x, true, L.out[blockIdx.x]);
+  lohi_accum(L, blockIdx.x, clo, chi, threadIdx.x, a, b, m);
+  lohi_reduce(a, b, m, threadIdx.x, true, sh, L.out[blockIdx.x]);
 }
 
 int launch_vars_lohi_err(hipStream_t s, const VarsLoHi& L, const PendingFinish* fin, StepGate gate) {
@@ -257,7 +257,7 @@ constexpr int kStepEndVars = 1024 / kRedThreads - 1;
 constexpr int kCtlWords = sizeof(StepCtl) / sizeof(unsigned long long);
 static_assert(sizeof(StepCtl) % sizeof(unsigned long long) == 0, "StepCtl is copied as 8-byte words");
 __global__ __launch_bounds__(1024) void k_step_end(StepEnd e) {
-  __shared__ double sh[3][kStepEndVars][kRedThreads];
+  __shared__ double sh[kStepEndVars][3 * (kRedThreads / 64)];
   __shared__ unsigned long long cw[kCtlWords];
   __shared__ double errS[6 * kMaxVars];  // field sums, then the same by error slot
   __shared__ int flg[2], acc;
@@ -273,34 +273,32 @@ __global__ __launch_bounds__(1024) void k_step_end(StepEnd e) {
   const int grp = threadIdx.x / kRedThreads, tid = threadIdx.x % kRedThreads;
   const int v = grp - 1;
   const bool pair = v >= 0 && v < e.L.nv && !closed;
-  double keep[kLoHiChunk];
+  double keep[kLoHiChunk], a = 0, b = 0, m = 0;
   if (pair) {
     double clo[6], chi[6];
     lohi_coefs(e.L, cs, clo, chi);
-    lohi_accum(e.L, v, clo, chi, tid, sh[0][v], sh[1][v], sh[2][v], keep);
+    lohi_accum(e.L, v, clo, chi, tid, a, b, m, keep);
   }
   // only the first kRedThreads threads accumulate in the test
   if (e.fin.sc && !halted) finish_irres_block(e.fin);
   if (threadIdx.x == 0 && e.fin.sc && !halted && !e.fin.sc->done) flg[1] = 1;
-  const int w = pair ? v : 0;
-  double* errOut = pair ? errS + 3 * v : nullptr;
-  lohi_tree(sh[0][w], sh[1][w], sh[2][w], tid, pair, errOut);
+  lohi_reduce(a, b, m, tid, pair, sh[pair ? v : 0], pair ? errS + 3 * v : nullptr);
   __syncthreads();
   if (pair && tid < 3) e.L.out[v][tid] = errS[3 * v + tid];
-  if (threadIdx.x == 0) {
-    // the sums at their error slots: field q's go to out[q] = err + 3 slot
-    // (slots below kMaxVars are the selected fields)
-    double* sel = errS + 3 * kMaxVars;  // LDS: indexed at run time
-    for (int q = 0; q < 3 * kMaxVars; ++q) sel[q] = 0.0;
-    for (int q = 0; q < e.L.nv; ++q) {
-      const long long slot = (e.L.out[q] - e.err) / 3;
-      if (slot >= 0 && slot < kMaxVars) {
-        sel[3 * slot] = errS[3 * q];
-        sel[3 * slot + 1] = errS[3 * q + 1];
-        sel[3 * slot + 2] = errS[3 * q + 2];
-      }
-    }
-    acc = step_control_update(cs, sel, flg[0] || flg[1]) ? 1 : 0;
+  // the sums at their error slots: field q's go to out[q] = err + 3 slot
+  // (slots below kMaxVars are the selected fields)
+  double* sel = errS + 3 * kMaxVars;
+  if (threadIdx.x < 3 * kMaxVars) sel[threadIdx.x] = 0.0;
+  __syncthreads();
+  if (threadIdx.x < 3 * e.L.nv) {
+    const int q = threadIdx.x / 3;
+    const long long slot = (e.L.out[q] - e.err) / 3;
+    if (slot >= 0 && slot < kMaxVars) sel[3 * slot + threadIdx.x % 3] = errS[threadIdx.x];
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const bool ok = step_control_wave(cs, sel, flg[0] || flg[1]);
+    if (threadIdx.x == 0) acc = ok ? 1 : 0;
   }
   __syncthreads();
   if (threadIdx.x < kCtlWords) reinterpret_cast<unsigned long long*>(e.ctl)[threadIdx.x] = cw[threadIdx.x];

@@ -13,6 +13,12 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
 // Block-reduce NV values and write this block's partial sums.  The partials
 // are combined by a kernel launched after this one (k_finish, or the guess
 // kernel of the projection): the kernel boundary orders the two, so no

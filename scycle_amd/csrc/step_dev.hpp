@@ -63,11 +63,13 @@ __device__ inline void lohi_accum_t(const VarsLoHi& L, int v, const double* clo,
 
 // Field v of the pair at local thread tid (0..kRedThreads-1): lo = base +
 // sum_t clo[t] flo[t], hi likewise (the order of k_vars_lincomb), written
-// out, and this thread's share of [sum (lo-hi)^2, sum hi^2, max |lo-hi|/hi]
-// into s0[tid], s1[tid], s2[tid]
+// out, and this thread's share (a, b, m) of [sum (lo-hi)^2, sum hi^2,
+// max |lo-hi|/hi]
 __device__ inline void lohi_accum(const VarsLoHi& L, int v, const double* clo, const double* chi, int tid,
-                                  double* s0, double* s1, double* s2, double* keep = nullptr) {
-  double a = 0, b = 0, m = -INFINITY;
+                                  double& a, double& b, double& m, double* keep = nullptr) {
+  a = 0;
+  b = 0;
+  m = -INFINITY;
   if (L.ntlo == 2 && L.nthi == 3) {
     lohi_accum_t<2, 3>(L, v, clo, chi, tid, a, b, m, keep);
   } else {
@@ -86,28 +88,35 @@ __device__ inline void lohi_accum(const VarsLoHi& L, int v, const double* clo, c
       m = fmax(m, fabs(e) / hi);
     }
   }
-  s0[tid] = a;
-  s1[tid] = b;
-  s2[tid] = m;
 }
 
-// The tree over kRedThreads shares of lohi_accum, the sums into out[0..2].
-// Every thread of the block calls it (a barrier per level); threads with
-// act = false only take the barriers.
-__device__ inline void lohi_tree(double* s0, double* s1, double* s2, int tid, bool act, double* out) {
-  __syncthreads();
-  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
-    if (act && tid < w) {
-      s0[tid] += s0[tid + w];
-      s1[tid] += s1[tid + w];
-      s2[tid] = fmax(s2[tid], s2[tid + w]);
-    }
-    __syncthreads();
+// The sums over a group of kRedThreads threads (kRedThreads / 64 waves,
+// group-aligned) of their lohi_accum shares: a butterfly in each wave, then
+// the wave values in wave order, into out[0..2].  sh: kRedThreads / 64 x 3
+// doubles of LDS for the group.  Every thread of the block calls it (one
+// barrier); threads with act = false only take the barrier.
+__device__ inline void lohi_reduce(double a, double b, double m, int tid, bool act, double* sh, double* out) {
+  constexpr int kW = kRedThreads / 64;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  m = wave_max(m);
+  const int w = tid >> 6;
+  if (act && (tid & 63) == 0) {
+    sh[w] = a;
+    sh[kW + w] = b;
+    sh[2 * kW + w] = m;
   }
+  __syncthreads();
   if (act && tid == 0) {
-    out[0] = s0[0];
-    out[1] = s1[0];
-    out[2] = s2[0];
+    double sa = sh[0], sb = sh[kW], sm = sh[2 * kW];
+    for (int q = 1; q < kW; ++q) {
+      sa += sh[q];
+      sb += sh[kW + q];
+      sm = fmax(sm, sh[2 * kW + q]);
+    }
+    out[0] = sa;
+    out[1] = sb;
+    out[2] = sm;
   }
 }
 
@@ -128,65 +137,111 @@ __device__ inline void lohi_coefs(const VarsLoHi& L, const StepCtl* ctl, double*
   }
 }
 
-// computeStepSize (odeSolver.cpp:395-445) on the control block's state, with
-// the host's std::min / std::max order (a NaN ratio keeps the bound)
-__device__ inline double step_size_dev(const StepCtl& c, double totErr) {
-  double ratio;
-  if (c.control == 0 || c.stepCount - c.pidBase < 4) {
-    ratio = c.kappa * pow(c.tol / totErr, 1. / (1. + c.ord));
-  } else {
-    const double al = 0.49 / c.ord, be = 0.34 / c.ord, ga = 0.1 / c.ord;
-    ratio = c.kappa * pow(c.tol / totErr, al) * pow(c.errA0 / c.tol, be) * pow(c.tol / c.errA1, ga);
+// computeStepSize (odeSolver.cpp:395-445) by one wave: the controller's
+// powers run on lanes 0..2 side by side (P control: lane 0 only) and the
+// ratio multiplies them in the host's order; every lane returns the step.
+// The host's std::min / std::max order (a NaN ratio keeps the bound).
+__device__ inline double step_size_wave(bool pCtl, int ord, double kappa, double tol, double errA0, double errA1,
+                                        double dt0, double minDt, double maxDt, double totErr) {
+  const int lane = threadIdx.x & 63;
+  double base = tol / totErr, ex = pCtl ? 1. / (1. + ord) : 0.49 / ord;
+  if (!pCtl && lane == 1) {
+    base = errA0 / tol;
+    ex = 0.34 / ord;
+  } else if (!pCtl && lane == 2) {
+    base = tol / errA1;
+    ex = 0.1 / ord;
   }
-  double dt = ratio * c.dt;
-  dt = (dt < c.dt * 5.0) ? dt : c.dt * 5.0;
-  dt = (dt < c.maxDt) ? dt : c.maxDt;
-  dt = (c.minDt < dt) ? dt : c.minDt;
+  const double pw = pow(base, ex);
+  const double p0 = __shfl(pw, 0, 64), p1 = __shfl(pw, 1, 64), p2 = __shfl(pw, 2, 64);
+  const double ratio = pCtl ? kappa * p0 : kappa * p0 * p1 * p2;
+  double dt = ratio * dt0;
+  dt = (dt < dt0 * 5.0) ? dt : dt0 * 5.0;
+  dt = (dt < maxDt) ? dt : maxDt;
+  dt = (minDt < dt) ? dt : minDt;
   return dt;
 }
 
 // One attempt's end (the tail of the RK32 attempt loop and the accepted-step
-// update of odeSolver.cpp:587-627) on the control block c (one thread): err
-// holds the selected fields' sums, flagged a set fault status or missed
-// guess, which halts the loop with the state of the attempt's start.
-// Returns whether the attempt was accepted (its solution goes into var).
-__device__ inline bool step_control_update(StepCtl* c, const double* err, bool flagged) {
-  c->accepted = 0;
-  if (c->halt || c->done) return false;
-  if (flagged) {
-    c->halt = 1;
+// update of odeSolver.cpp:587-627) on the control block c (in LDS), run by
+// the 64 lanes of one wave on register copies of its fields, lane 0 writing
+// the changed ones back: err holds the selected fields' sums, flagged a set
+// fault status or missed guess, which halts the loop with the state of the
+// attempt's start.  Returns (on every lane) whether the attempt was accepted
+// (its solution goes into var).  The proposed step is the same for a
+// rejected attempt's retry and an accepted step's next size (computeStepSize
+// sees the same state in both), so it is computed once; an attempt accepted
+// at minDt or the attempt limit proposes again from the step it took.
+__device__ inline bool step_control_wave(StepCtl* c, const double* err, bool flagged) {
+  const bool lead = (threadIdx.x & 63) == 0;
+  if (c->halt || c->done) {
+    if (lead) c->accepted = 0;
     return false;
   }
+  if (flagged) {
+    if (lead) {
+      c->accepted = 0;
+      c->halt = 1;
+    }
+    return false;
+  }
+  const int ord = c->ord, nsel = c->nsel, normType = c->normType;
   double tot = 0;
-  for (int s = 0; s < c->nsel; ++s) {
-    const double* v = err + 3 * s;
-    if (c->normType == 0) tot += sqrt(v[0]) / (sqrt((double)c->selN[s]) * 1.0);
-    else if (c->ord == 3) tot += sqrt(v[0]) / (sqrt(v[1]) * 1.0);
+  for (int q = 0; q < nsel; ++q) {
+    const double* v = err + 3 * q;
+    if (normType == 0) tot += sqrt(v[0]) / (sqrt((double)c->selN[q]) * 1.0);
+    else if (ord == 3) tot += sqrt(v[0]) / (sqrt(v[1]) * 1.0);
     else tot += v[2] / 1.0;
   }
-  bool accept = c->ord == 3 ? tot <= c->tol : tot < c->tol;
-  c->attempts++;
+  const double tol = c->tol, minDt = c->minDt, finalT = c->finalT;
+  double t = c->t, dt = c->dt, newDt = c->newDt;
+  int attempts = c->attempts + 1, rejected = c->rejected, stepCount = c->stepCount;
+  const bool pCtl = c->control == 0 || stepCount - c->pidBase < 4;
+  const double kappa = c->kappa, errA0 = c->errA0, errA1 = c->errA1, maxDt = c->maxDt;
+  const double sz = step_size_wave(pCtl, ord, kappa, tol, errA0, errA1, dt, minDt, maxDt, tot);
+  bool accept = ord == 3 ? tot <= tol : tot < tol, forced = false;
   if (!accept) {
-    c->dt = step_size_dev(*c, tot);
-    c->rejected++;
-    if (c->minDt == c->dt || c->attempts >= 100) accept = true;
-    else if (c->t + c->dt > c->finalT) c->dt = c->finalT - c->t;
+    dt = sz;
+    rejected++;
+    if (minDt == dt || attempts >= 100) accept = forced = true;
+    else if (t + dt > finalT) dt = finalT - t;
   }
-  if (!accept) return false;
-  c->t = c->t + c->dt;
-  if (tot != 0.0) c->newDt = step_size_dev(*c, tot);
-  c->errA1 = c->errA0;
-  c->errA0 = tot;
-  if (c->nOut < c->dtsCap) c->dts[c->nOut] = c->dt;
-  c->nOut++;
-  c->dt = c->newDt;
-  c->accepted = 1;
-  c->attempts = 0;
-  if (!(c->stepCount < c->maxSteps && c->t < c->finalT)) {
-    c->done = 1;
+  if (!accept) {
+    if (lead) {
+      c->accepted = 0;
+      c->attempts = attempts;
+      c->dt = dt;
+      c->rejected = rejected;
+    }
+    return false;
+  }
+  t = t + dt;
+  // a forced acceptance proposes from the step it just took
+  if (tot != 0.0) newDt = forced ? step_size_wave(pCtl, ord, kappa, tol, errA0, errA1, dt, minDt, maxDt, tot) : sz;
+  const int nOut = c->nOut;
+  if (lead) {
+    if (nOut < c->dtsCap) c->dts[nOut] = dt;
+    c->errA1 = errA0;
+    c->errA0 = tot;
+  }
+  dt = newDt;
+  int done = 0;
+  if (!(stepCount < c->maxSteps && t < finalT)) {
+    done = 1;
   } else {
-    c->stepCount++;
-    if (c->t + c->dt > c->finalT) c->dt = c->finalT - c->t;
+    stepCount++;
+    if (t + dt > finalT) dt = finalT - t;
+  }
+  if (lead) {
+    c->t = t;
+    c->newDt = newDt;
+    c->nOut = nOut + 1;
+    c->dt = dt;
+    c->accepted = 1;
+    c->attempts = 0;
+    c->rejected = rejected;
+    c->done = done;
+    c->stepCount = stepCount;
   }
   return true;
 }
