@@ -337,11 +337,12 @@ __global__ __launch_bounds__(kRedThreads) void k_proj_guess_res(size_t N, VecLis
                                                                 double* __restrict__ x, double* partials,
                                                                 GuessFault gf, StepGate gate, size_t zLo,
                                                                 size_t zHi, double* coefOut) {
-  if (gate_closed(gate)) return;
   __shared__ double shc[M][kRedThreads / 64];
   __shared__ double coef[M];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   {
+    // the projections' partials load while the gate is read (they are in
+    // bounds either way)
     double a[M];
 #pragma unroll
     for (int m = 0; m < M; ++m) a[m] = 0.0;
@@ -349,6 +350,7 @@ __global__ __launch_bounds__(kRedThreads) void k_proj_guess_res(size_t N, VecLis
 #pragma unroll
       for (int m = 0; m < M; ++m) a[m] += mpart[(size_t)q * M + m];
     }
+    if (gate_closed(gate)) return;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       const double t = wave_sum(a[m]);

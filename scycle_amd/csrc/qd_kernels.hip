@@ -79,6 +79,45 @@ __global__ __launch_bounds__(kRedThreads) void k_qd_rhs_frame_md(DevOp<double> p
     if (!(gate.ctl && gate.ctl->halt)) finish_irres_block(fin);
     return;
   }
+  const unsigned stride = (unsigned)nbWork * blockDim.x;
+  const unsigned gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned Nz = (unsigned)p.Nz;
+  const unsigned n = (unsigned)(nLow + nHigh) * Nz;
+  unsigned total = 0;
+  for (int v = 0; v < L.nv; ++v) total += (unsigned)L.n[v];
+  // The first combination entry's and the first band node's operands load
+  // while the gate and the step arrive with the control block (they are in
+  // bounds either way); later grid-stride passes load as they go.
+  double pf[6] = {}, pb = 0.0;  // combination entry gid
+  int pv = 0;
+  unsigned pi = gid;
+  if (gid < total) {
+    while (pi >= (unsigned)L.n[pv]) pi -= (unsigned)L.n[pv++];
+    pb = L.base[pv][pi];
+#pragma unroll
+    for (int u = 0; u < 6; ++u)
+      if (u < L.nt) pf[u] = L.f[pv][u][pi];
+  }
+  double sf[6] = {}, sb = 0.0, sh0 = 0.0, y0[M];  // band node gid
+#pragma unroll
+  for (int m = 0; m < M; ++m) y0[m] = 0.0;
+  if (gid < n) {
+    const unsigned row = gid / Nz;
+    const int k = (int)(gid - row * Nz);
+    const int j = (int)row < nLow ? (int)row : p.Ny - nHigh + ((int)row - nLow);
+    const size_t i = (size_t)j * Nz + k;
+    if (L.nv) {
+      sb = L.base[preSlip][k];
+#pragma unroll
+      for (int u = 0; u < 6; ++u)
+        if (u < L.nt) sf[u] = L.f[preSlip][u][k];
+    } else {
+      sb = slip[k];
+    }
+    sh0 = remote ? shift[k] : bcR[k];
+#pragma unroll
+    for (int m = 0; m < M; ++m) y0[m] = Y.v[m][i];
+  }
   if (gate_closed(gate)) return;
   double cc[6];
 #pragma unroll
@@ -89,12 +128,20 @@ __global__ __launch_bounds__(kRedThreads) void k_qd_rhs_frame_md(DevOp<double> p
 #pragma unroll
     for (int i = 0; i < 6; ++i) cc[i] = (L.c[i] * h) / L.cd[i];
   }
-  const unsigned stride = (unsigned)nbWork * blockDim.x;
-  const unsigned gid = blockIdx.x * blockDim.x + threadIdx.x;
+  // the combination (k_vars_lincomb's order)
+  auto comb = [&](const double* f, double base) {
+    double x = cc[0] * f[0] + base;
+#pragma unroll
+    for (int u = 1; u < 6; ++u)
+      if (u < L.nt) x = x + cc[u] * f[u];
+    return x;
+  };
   if (L.nv) {
-    unsigned total = 0;
-    for (int v = 0; v < L.nv; ++v) total += (unsigned)L.n[v];
     for (unsigned q = gid; q < total; q += stride) {
+      if (q == gid) {
+        L.out[pv][pi] = comb(pf, pb);
+        continue;
+      }
       int v = 0;
       unsigned i = q;
       while (i >= (unsigned)L.n[v]) i -= (unsigned)L.n[v++];
@@ -104,22 +151,22 @@ __global__ __launch_bounds__(kRedThreads) void k_qd_rhs_frame_md(DevOp<double> p
   double acc[M];
 #pragma unroll
   for (int m = 0; m < M; ++m) acc[m] = 0.0;
-  const unsigned Nz = (unsigned)p.Nz;
-  const unsigned n = (unsigned)(nLow + nHigh) * Nz;
   for (unsigned q = gid; q < n; q += stride) {
     const unsigned row = q / Nz;
     const int k = (int)(q - row * Nz);
     const int j = (int)row < nLow ? (int)row : p.Ny - nHigh + ((int)row - nLow);
-    const double sk = L.nv ? lincomb_at(L, cc, preSlip, k) : slip[k];
+    const bool first = q == gid;
+    const double sk = L.nv ? (first ? comb(sf, sb) : lincomb_at(L, cc, preSlip, k)) : (first ? sb : slip[k]);
     const double l = sk * (1.0 / fts);
-    const double r = remote ? vL * t / fts + 1.0 * shift[k] : bcR[k];
+    const double shk = first ? sh0 : (remote ? shift[k] : bcR[k]);
+    const double r = remote ? vL * t / fts + 1.0 * shk : shk;
     const size_t i = (size_t)j * Nz + k;
     const double v = rhs_node(p, j, k, [&](int side, int) { return side == 0 ? l : side == 1 ? r : 0.0; });
     rhs[i] = v;
     if (j == 0) bcL[k] = l;
     if (j == p.Ny - 1 && remote) bcR[k] = r;
 #pragma unroll
-    for (int m = 0; m < M; ++m) acc[m] += Y.v[m][i] * v;
+    for (int m = 0; m < M; ++m) acc[m] += (first ? y0[m] : Y.v[m][i]) * v;
   }
   block_partials(acc, mpart);
 }
