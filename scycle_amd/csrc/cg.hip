@@ -340,6 +340,31 @@ __global__ __launch_bounds__(kRedThreads) void k_proj_guess_res(size_t N, VecLis
   __shared__ double shc[M][kRedThreads / 64];
   __shared__ double coef[M];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  // Streaming blocks: kU entries per pass with every load of a pass issued
+  // before its arithmetic; the first pass's loads go out before the
+  // projection sums (they do not depend on the coefficients), and the sums
+  // run in entry order as a plain strided loop's.
+  constexpr int kU = M <= 2 ? 4 : (M <= 4 ? 2 : 1);
+  const int sblk = (int)blockIdx.x - (FAULT ? gf.nfb : 0);  // streaming block index, < 0: a fault block
+  const unsigned nstream = gridDim.x - (FAULT ? gf.nfb : 0);
+  const size_t stride = (size_t)nstream * blockDim.x;
+  double zv[kU][M], yv[kU][WX ? M : 1], bv[kU];
+  auto load_pass = [&](size_t i0) {
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const size_t i = i0 + u * stride;
+      if (i < N) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          zv[u][m] = Z.v[m][i];
+          if constexpr (WX) yv[u][m] = Y.v[m][i];
+        }
+        bv[u] = (i >= zLo && i < zHi) ? 0.0 : b[i];
+      }
+    }
+  };
+  const size_t iFirst = sblk >= 0 ? (size_t)sblk * blockDim.x + threadIdx.x : N;
+  load_pass(iFirst);
   {
     // the projections' partials load while the gate is read (they are in
     // bounds either way)
@@ -389,20 +414,26 @@ __global__ __launch_bounds__(kRedThreads) void k_proj_guess_res(size_t N, VecLis
     sc->cnt = 0;
   }
   if (!WX && blk == 0 && threadIdx.x < M) coefOut[threadIdx.x] = c[threadIdx.x];
-  const unsigned nstream = gridDim.x - (FAULT ? gf.nfb : 0);
   double acc[2] = {0.0, 0.0};
-  for (size_t i = (size_t)blk * blockDim.x + threadIdx.x; i < N; i += (size_t)nstream * blockDim.x) {
-    double v = 0.0, az = 0.0;
+  for (size_t i0 = iFirst; i0 < N; i0 += kU * stride) {
+    if (i0 != iFirst) load_pass(i0);
 #pragma unroll
-    for (int m = 0; m < M; ++m) {
-      if (WX) v += c[m] * Y.v[m][i];
-      az += c[m] * Z.v[m][i];
+    for (int u = 0; u < kU; ++u) {
+      const size_t i = i0 + u * stride;
+      if (i < N) {
+        double v = 0.0, az = 0.0;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          if constexpr (WX) v += c[m] * yv[u][m];
+          az += c[m] * zv[u][m];
+        }
+        const double bi = bv[u];
+        const double ri = bi - az;
+        if constexpr (WX) x[i] = v;
+        acc[0] += ri * ri;
+        acc[1] += bi * bi;
+      }
     }
-    const double bi = (i >= zLo && i < zHi) ? 0.0 : b[i];
-    const double ri = bi - az;
-    if (WX) x[i] = v;
-    acc[0] += ri * ri;
-    acc[1] += bi * bi;
   }
   block_partials_at(acc, partials, blk);
 }
