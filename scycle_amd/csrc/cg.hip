@@ -318,6 +318,7 @@ __global__ __launch_bounds__(kRedThreads) void k_multidot_frame(FrameMap fm, int
 // recomputes it).  The first streaming block also starts the residual check
 // (tolerance tol2).  FAULT: the first nfb blocks instead compute the fault
 // rates of the QD step from tau = sum_m coef[m] T_m (QdFuse).
+constexpr int kFaultWave = 64;  // fault nodes per fault block (its first wave)
 struct GuessFault {
   QdFuse f;
   VecList T;
@@ -395,8 +396,10 @@ __global__ __launch_bounds__(kRedThreads) void k_proj_guess_res(size_t N, VecLis
   int blk = blockIdx.x;
   if constexpr (FAULT) {
     if (blk < gf.nfb) {
-      const int k = blk * kRedThreads + threadIdx.x;
-      if (k < gf.f.a.N) {
+      // one wave of fault nodes per block: the root solves are FP64 VALU
+      // work, spread over as many CUs as there are waves of nodes
+      const int k = blk * kFaultWave + threadIdx.x;
+      if (threadIdx.x < kFaultWave && k < gf.f.a.N) {
         double tq = 0.0;
 #pragma unroll
         for (int m = 0; m < M; ++m) tq += c[m] * gf.T.v[m][k];
@@ -720,7 +723,7 @@ void proj_guess_m(Ctx& c, const VecList& Y, const VecList& Z, const double* b, d
   }
   double* coefOut = c.projW + 2 * SC_REUSE_MAX;
   if (fuse) {
-    GuessFault gf{*fuse, {}, (fuse->a.N + kRedThreads - 1) / kRedThreads};
+    GuessFault gf{*fuse, {}, (fuse->a.N + kFaultWave - 1) / kFaultWave};
     for (int m = 0; m < M; ++m) gf.T.v[m] = c.projT[m];
     if (lazyX)
       hipLaunchKernelGGL((k_proj_guess_res<M, true, false>), dim3(red_blocks(N) + gf.nfb), dim3(kRedThreads), 0, s,

@@ -41,6 +41,7 @@ QdDriver::~QdDriver() {
     if (ctl_d) (void)hipFree(ctl_d);
     if (ctl_h) (void)hipHostFree(ctl_h);
     if (ctlDts_d) (void)hipFree(ctlDts_d);
+    stage_bufs_free(stageBufs);
     sc_ctx_destroy(ctx);
   }
 }
@@ -158,6 +159,33 @@ int QdDriver::read_flags(int* status, int* miss) {
 // StrikeSlip_LinearElastic_qd::d_dt (strikeSlip_linearElastic_qd.cpp:756-798);
 // with the power law, StrikeSlip_PowerLaw_qd::d_dt (strikeSlip_powerLaw_qd.cpp
 // :633-686) and solveMomentumBalance (:787-823)
+sc_fault_qd_args QdDriver::fault_args(const double* psi, double* dpsi) const {
+  sc_fault_qd_args fa{};
+  fa.N = Nz;
+  fa.v0 = desc.v0;
+  fa.vL = desc.vL;
+  fa.f0 = desc.f0;
+  fa.rootTol = desc.rootTol;
+  fa.maxNumIts = desc.maxNumIts;
+  fa.stateLaw = desc.stateLaw;
+  fa.jac_mode = desc.jac_mode;
+  fa.eta_d = eta_d;
+  fa.sN_d = sN_d;
+  fa.a_d = a_d;
+  fa.b_d = b_d;
+  fa.Dc_d = Dc_d;
+  fa.locked_d = locked_d;
+  fa.prestress_d = prestress_d;
+  fa.psi_d = psi;
+  fa.tauQS_d = tauQS_d;
+  fa.V_d = V_d;
+  fa.dpsi_d = dpsi;
+  fa.tau_d = tau_d;
+  fa.strength_d = strength_d;
+  fa.status_d = status_d;
+  return fa;
+}
+
 int QdDriver::materialize_u() {
   if (!uLazy) return SC_OK;
   uLazy = false;
@@ -202,29 +230,7 @@ int QdDriver::d_dt(double t, const double* psi, const double* slip, double* dpsi
     if (!e) e = launch_pl_source(ctx->c, gxy, gxz, rhs_d, s);
   }
   if (e) return e;
-  sc_fault_qd_args fa{};
-  fa.N = Nz;
-  fa.v0 = desc.v0;
-  fa.vL = desc.vL;
-  fa.f0 = desc.f0;
-  fa.rootTol = desc.rootTol;
-  fa.maxNumIts = desc.maxNumIts;
-  fa.stateLaw = desc.stateLaw;
-  fa.jac_mode = desc.jac_mode;
-  fa.eta_d = eta_d;
-  fa.sN_d = sN_d;
-  fa.a_d = a_d;
-  fa.b_d = b_d;
-  fa.Dc_d = Dc_d;
-  fa.locked_d = locked_d;
-  fa.prestress_d = prestress_d;
-  fa.psi_d = psi;
-  fa.tauQS_d = tauQS_d;
-  fa.V_d = V_d;
-  fa.dpsi_d = dpsi;
-  fa.tau_d = tau_d;
-  fa.strength_d = strength_d;
-  fa.status_d = status_d;
+  sc_fault_qd_args fa = fault_args(psi, dpsi);
   // a checked solve may start CG from u: form the lazy one first
   if (uLazy && !defer && (e = materialize_u())) return e;
   const double ts = now();
@@ -356,7 +362,57 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
   // one attempt: both stages, the attempt's end (the last stage's residual
   // test, the lo/hi pair, the step control: one launch), the rates after an
   // accepted step
+  // With a basis and Nz within its tables each d_dt is one stage kernel
+  // (no rhs launch): the projection plan of the current basis first (the
+  // basis does not change inside the loop; a missed guess returns to the
+  // host).  The stage kernels of one attempt take positions 0 (stage 2),
+  // 1 (stage 3) and 2 (rates at the accepted state); each tests the previous
+  // position's residual, the step end tests position 1's.
+  const bool oneKernel = stage_fits(ctx->c);
+  if (oneKernel) {
+    int e0 = stage_bufs_ensure(stageBufs, Nz, (size_t)N);
+    if (!e0) e0 = launch_stage_plan(ctx->c, stageBufs, desc.remoteLoadingR ? shift_d : bcR_d, s);
+    if (e0) return e0;
+  }
+  int psiIdx = 0;
+  for (auto& kv : vars) {
+    if (kv.first == "psi") break;
+    ++psiIdx;
+  }
+  auto stage_in = [&](const VarsLinComb* L, int st, StepGate g, double tfrac, PendingFinish f) {
+    QdStageIn in;
+    in.tol2 = desc.solve.rtol * desc.solve.rtol;
+    const double* psi = L ? P.k[st] : P.var;
+    double* dpsi = L ? P.f[st] : P.dvar;
+    in.fuse = QdFuse{fault_args(psi, dpsi), L ? Q.f[st] : Q.dvar};
+    if (L) in.L = *L;
+    in.psiIdx = psiIdx;
+    in.slipIdx = slipIdx;
+    in.slipVar = Q.var;
+    in.fts = desc.faultTypeScale;
+    in.vL = desc.vL;
+    in.remote = desc.remoteLoadingR;
+    in.rfix = desc.remoteLoadingR ? shift_d : bcR_d;
+    in.bcL = bcL_d;
+    in.bcR = bcR_d;
+    in.gate = g;
+    in.tfrac = tfrac;
+    in.fin = f;
+    return in;
+  };
   auto slot = [&]() -> int {
+    if (oneKernel) {
+      const PendingFinish prev = finPending ? fin : PendingFinish{};
+      int e = launch_qd_stage(ctx->c, stageBufs, 0, stage_in(&L1, 1, g1, 0.5, prev), s);
+      if (!e) e = launch_qd_stage(ctx->c, stageBufs, 1, stage_in(&L2, 2, g1, 1.0, stage_finish(stageBufs, 0, miss_d)), s);
+      if (!e) e = launch_step_end(s, StepEnd{stage_finish(stageBufs, 1, miss_d), S, ctl_d, err_d, status_d, cp}, g1);
+      if (!e) e = launch_qd_stage(ctx->c, stageBufs, 2, stage_in(nullptr, 0, g2, 0.0, PendingFinish{}), s);
+      // the rates' residual test is left for the next launch; u for proj_materialize
+      fin = stage_finish(stageBufs, 2, miss_d);
+      finPending = true;
+      uLazy = true;
+      return e;
+    }
     int e = d_dt(0.0, P.k[1], Q.k[1], P.f[1], Q.f[1], nullptr, nullptr, nullptr, nullptr, &L1, slipIdx, g1, 0.5);
     if (!e) e = d_dt(0.0, P.k[2], Q.k[2], P.f[2], Q.f[2], nullptr, nullptr, nullptr, nullptr, &L2, slipIdx, g1, 1.0);
     if (!e) e = launch_step_end(s, StepEnd{finPending ? fin : PendingFinish{}, S, ctl_d, err_d, status_d, cp}, g1);

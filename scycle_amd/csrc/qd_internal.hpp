@@ -47,6 +47,43 @@ struct StepEnd {
   VarsCopy cp{};
 };
 int launch_step_end(hipStream_t s, const StepEnd& e, StepGate gate);
+
+// One kernel per RK stage of the device loop (qd_kernels.hip, k_qd_stage):
+// the stage combination, the projections of the stage's rhs from the
+// per-basis plan (k_stage_plan), the fault rates and the guess's residual,
+// with b formed on the band rows; no rhs launch.  Buffers: the plan (PL, PR,
+// sums), and per stage position (stage 2, stage 3, rates at the accepted
+// state) the residual partials and scalars, so that a launch can test the
+// previous stage's residual while it writes its own.
+struct QdStageBufs {
+  double* plan = nullptr;
+  double* parts = nullptr;
+  void* scal = nullptr;
+  int nz = 0, nstream = 0;
+};
+struct QdStageIn {
+  double tol2 = 0;
+  QdFuse fuse{};
+  VarsLinComb L{};
+  int psiIdx = -1, slipIdx = -1;
+  const double* slipVar = nullptr;
+  double fts = 1, vL = 0;
+  int remote = 0;
+  const double* rfix = nullptr;
+  double* bcL = nullptr;
+  double* bcR = nullptr;
+  StepGate gate{};
+  double tfrac = 0;
+  PendingFinish fin{};
+};
+int stage_bufs_ensure(QdStageBufs& b, int Nz, size_t N);
+void stage_bufs_free(QdStageBufs& b);
+// a basis and Nz within the kernel's LDS tables
+bool stage_fits(const Ctx& c);
+// the residual test a stage launch at position pos leaves
+PendingFinish stage_finish(const QdStageBufs& b, int pos, int* miss);
+int launch_stage_plan(const Ctx& c, QdStageBufs& b, const double* rfix, hipStream_t s);
+int launch_qd_stage(const Ctx& c, const QdStageBufs& b, int pos, const QdStageIn& in, hipStream_t s);
 // fin (optional): a pending residual test run as one more block
 int launch_vars_lohi_err(hipStream_t s, const VarsLoHi& L, const PendingFinish* fin = nullptr,
                          StepGate gate = StepGate{});
@@ -161,7 +198,9 @@ struct QdDriver {
            const double* gxy = nullptr, const double* gxz = nullptr, double* dgxy = nullptr,
            double* dgxz = nullptr, const VarsLinComb* pre = nullptr, int preSlip = -1,
            StepGate gate = StepGate{}, double tfrac = 0.0);
-  int d_dt_vars(double t);              // on var -> dvar of every integrated variable
+  int d_dt_vars(double t);
+  // the fault-rate arguments for a stage's psi and psi rate
+  sc_fault_qd_args fault_args(const double* psi, double* dpsi) const;              // on var -> dvar of every integrated variable
   int check_status();
   // fault status and missed-guess flag in one read-back (the flags are left set)
   int read_flags(int* status, int* miss);
@@ -190,6 +229,7 @@ struct QdDriver {
   StepCtl* ctl_h = nullptr;           // pinned
   double* ctlDts_d = nullptr;
   int ctlDtsCap = 0;
+  QdStageBufs stageBufs;  // the device loop's one-kernel stages
   static constexpr int kSlotsPerBatch = 24;
   // attempts captured in one graph: a relaunch of a graph waits for its
   // previous run, so each launch carries several attempts

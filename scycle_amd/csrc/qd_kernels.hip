@@ -400,4 +400,396 @@ int launch_max_ratio(hipStream_t s, const double* eta, const double* V, const do
   return SC_OK;
 }
 
+
+// ---- One kernel per Runge-Kutta stage (the device-controlled RK32 loop) ----
+//
+// The QD rhs is linear in the boundary data and nonzero only on the band
+// rows, so the projections of a stage's rhs on the basis are
+//   y_m . b = sum_k PL_m(k) l(k) + sum_k PR_m(k) r(k),
+// PL_m(k) / PR_m(k) = sum over the band rows j of y_m(j, k) times the rhs at
+// (j, k) for a unit left / right datum, l = slip / fts, r = vL t / fts +
+// shift (remote loading) or the fixed bcR.  With PL, sum_k PR_m and
+// sum_k PR_m r_fix formed once per basis (k_stage_plan), every block of the
+// stage kernel forms the guess coefficients from Nz entries, and the stage
+// needs no rhs launch: the streaming blocks form b on the band rows where
+// the residual needs it from each band node's rhs for unit data (also formed
+// once per basis), b = aL l + aR r.  b and the coefficients round
+// differently from the rhs kernel's (to the last bit).
+
+constexpr int kStageThreads = 512;
+constexpr int kStageNzMax = 2048;
+constexpr int kStagePos = 3;  // stage 2, stage 3, rates at the accepted state
+constexpr int kStageNT = 2;   // terms of a stage combination (RK32: 1 and 2)
+
+// thread per k: PL and PR over the band rows, and each band node's rhs for
+// unit left and right data (aL, aR: band-row order, rows 0..nLow-1 then
+// Ny-nHigh..Ny-1)
+template <int M>
+__global__ __launch_bounds__(256) void k_stage_plan(DevOp<double> p, int nLow, int nHigh, VecList Y,
+                                                    double* __restrict__ PL, double* __restrict__ PR,
+                                                    double* __restrict__ aL, double* __restrict__ aR) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Nz = p.Nz;
+  if (k >= Nz) return;
+  double al[M], ar[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) al[m] = ar[m] = 0.0;
+  for (int row = 0; row < nLow + nHigh; ++row) {
+    const int j = row < nLow ? row : p.Ny - nHigh + (row - nLow);
+    const double uL = rhs_node(p, j, k, [](int side, int) { return side == 0 ? 1.0 : 0.0; });
+    const double uR = rhs_node(p, j, k, [](int side, int) { return side == 1 ? 1.0 : 0.0; });
+    aL[(size_t)row * Nz + k] = uL;
+    aR[(size_t)row * Nz + k] = uR;
+    const size_t i = (size_t)j * Nz + k;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      al[m] += Y.v[m][i] * uL;
+      ar[m] += Y.v[m][i] * uR;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    PL[(size_t)m * Nz + k] = al[m];
+    PR[(size_t)m * Nz + k] = ar[m];
+  }
+}
+
+// one block: sR[m] = sum_k PR_m(k), sR[M + m] = sum_k PR_m(k) rfix(k)
+template <int M>
+__global__ __launch_bounds__(256) void k_stage_plan_sums(const double* __restrict__ PR, const double* __restrict__ rfix,
+                                                         int Nz, double* __restrict__ sR) {
+  __shared__ double sh[2 * M][4];
+  double a[2 * M];
+#pragma unroll
+  for (int q = 0; q < 2 * M; ++q) a[q] = 0.0;
+  for (int k = threadIdx.x; k < Nz; k += 256) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const double v = PR[(size_t)m * Nz + k];
+      a[m] += v;
+      a[M + m] += v * rfix[k];
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 2 * M; ++q) {
+    const double t = wave_sum(a[q]);
+    if (lane == 0) sh[q][wid] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * M) sR[threadIdx.x] = ((sh[threadIdx.x][0] + sh[threadIdx.x][1]) + sh[threadIdx.x][2]) + sh[threadIdx.x][3];
+}
+
+struct QdStageArgs {
+  int Nz, nfb, nstream;
+  size_t N, zLo, zHi;
+  VecList Z, T;
+  const double* w;   // projection weights
+  const double* PL;  // M x Nz
+  const double* sR;  // 2 M
+  const double* aL;  // band nodes' rhs for a unit left datum
+  const double* aR;  // and a unit right datum
+  double* coefOut;   // the coefficients, for proj_materialize
+  double* parts;     // this stage's residual partials (2 per streaming block)
+  CgScal* sc;        // this stage's scalars
+  double tol2;
+  QdFuse fuse;       // fault rates: psi_d the stage's psi, dpsi_d / dslip its rates
+  VarsLinComb L;     // the stage combination (nv = 0: the state itself)
+  int psiIdx, slipIdx;
+  const double* slipVar;  // the slip when L.nv = 0
+  double fts, vL;
+  int remote;
+  const double* rfix;  // shift (remote loading) or the fixed bcR
+  double* bcL;
+  double* bcR;
+  StepGate gate;
+  double tfrac;
+  PendingFinish fin;  // the previous stage's residual test (the last block)
+};
+
+// Blocks [0, nfb): the stage state and fault rates of 64 fault nodes each
+// (one wave: the root solves are FP64 VALU work, spread one wave per CU);
+// [nfb, nfb + nstream): the residual b - sum c_m (A y_m) over the grid with
+// its r.r, b.b partials, b = aL l + aR r on the band rows; one more block:
+// the previous stage's residual test.
+template <int M>
+__global__ __launch_bounds__(kStageThreads) void k_qd_stage(QdStageArgs a) {
+  const int blk = blockIdx.x;
+  if (blk >= a.nfb + a.nstream) {
+    if (!(a.gate.ctl && a.gate.ctl->halt)) finish_irres_block(a.fin);
+    return;
+  }
+  __shared__ double sS[kStageNzMax], rS[kStageNzMax];
+  __shared__ double red[M > 2 ? M : 2][kStageThreads / 64];
+  __shared__ double coef[M];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int Nz = a.Nz;
+  const int sblk = blk - a.nfb;  // < 0: a fault block
+  // Every load that does not depend on the step goes out before the gate is
+  // read: the streaming blocks' first pass (A y_m, and the unit-datum rhs on
+  // band rows), and each thread's fault nodes' slip operands and PL entries.
+  constexpr int kU = M <= 2 ? 4 : (M <= 4 ? 2 : 1);
+  const size_t stride = (size_t)a.nstream * kStageThreads;
+  double zv[kU][M], av[kU][2];
+  const size_t nBand0 = a.zLo;
+  auto load_pass = [&](size_t i0) {
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const size_t i = i0 + u * stride;
+      if (i < a.N) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) zv[u][m] = a.Z.v[m][i];
+        if (i < a.zLo || i >= a.zHi) {
+          const size_t q = i < a.zLo ? i : nBand0 + (i - a.zHi);
+          av[u][0] = a.aL[q];
+          av[u][1] = a.aR[q];
+        }
+      }
+    }
+  };
+  const size_t iFirst = sblk >= 0 ? (size_t)sblk * kStageThreads + tid : a.N;
+  load_pass(iFirst);
+  constexpr int kQ = kStageNzMax / kStageThreads;
+  double sb[kQ], sf[kQ][kStageNT], pl[kQ][M], rk[kQ];
+#pragma unroll
+  for (int q = 0; q < kQ; ++q) {
+    const int k = tid + q * kStageThreads;
+    if (k < Nz) {
+      if (a.L.nv) {
+        sb[q] = a.L.base[a.slipIdx][k];
+#pragma unroll
+        for (int u = 0; u < kStageNT; ++u)
+          if (u < a.L.nt) sf[q][u] = a.L.f[a.slipIdx][u][k];
+      } else {
+        sb[q] = a.slipVar[k];
+      }
+      rk[q] = a.rfix[k];
+#pragma unroll
+      for (int m = 0; m < M; ++m) pl[q][m] = a.PL[(size_t)m * Nz + k];
+    }
+  }
+  if (gate_closed(a.gate)) return;
+  double cc[kStageNT];
+#pragma unroll
+  for (int i = 0; i < kStageNT; ++i) cc[i] = a.L.c[i];
+  double t = 0.0;
+  if (a.gate.ctl) {  // this attempt's step and stage time
+    const double h = a.gate.ctl->dt;
+    t = a.gate.ctl->t + a.tfrac * h;
+#pragma unroll
+    for (int i = 0; i < kStageNT; ++i) cc[i] = (a.L.c[i] * h) / a.L.cd[i];
+  }
+  // the stage's slip and right datum at every fault node (into LDS), and
+  // this thread's share of sum_k PL_m(k) l(k)
+  double acc[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) acc[m] = 0.0;
+#pragma unroll
+  for (int q = 0; q < kQ; ++q) {
+    const int k = tid + q * kStageThreads;
+    if (k < Nz) {
+      double sk = sb[q];
+      if (a.L.nv) {  // the combination (k_vars_lincomb's order)
+        sk = cc[0] * sf[q][0] + sb[q];
+#pragma unroll
+        for (int u = 1; u < kStageNT; ++u)
+          if (u < a.L.nt) sk = sk + cc[u] * sf[q][u];
+      }
+      const double l = sk * (1.0 / a.fts);
+      const double r = a.remote ? a.vL * t / a.fts + 1.0 * rk[q] : rk[q];
+      sS[k] = sk;
+      rS[k] = r;
+#pragma unroll
+      for (int m = 0; m < M; ++m) acc[m] += pl[q][m] * l;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const double s = wave_sum(acc[m]);
+    if (lane == 0) red[m][wid] = s;
+  }
+  __syncthreads();
+  if (tid < M) {
+    double yb = 0.0;
+    for (int q = 0; q < kStageThreads / 64; ++q) yb += red[tid][q];
+    yb += a.remote ? (a.vL * t / a.fts) * a.sR[tid] + a.sR[M + tid] : a.sR[M + tid];
+    coef[tid] = -a.w[tid] * yb;
+  }
+  __syncthreads();
+  double c[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) c[m] = coef[m];
+  if (sblk < 0) {
+    // fault nodes: the stage state, tau = sum c_m T_m, the rates
+    const int k = blk * 64 + tid;
+    if (tid < 64 && k < Nz) {
+      if (a.L.nv) {  // psi of the stage (k_vars_lincomb's order)
+        double x = cc[0] * a.L.f[a.psiIdx][0][k] + a.L.base[a.psiIdx][k];
+#pragma unroll
+        for (int u = 1; u < kStageNT; ++u)
+          if (u < a.L.nt) x = x + cc[u] * a.L.f[a.psiIdx][u][k];
+        a.L.out[a.psiIdx][k] = x;
+        a.L.out[a.slipIdx][k] = sS[k];
+      }
+      double tq = 0.0;
+#pragma unroll
+      for (int m = 0; m < M; ++m) tq += c[m] * a.T.v[m][k];
+      a.fuse.dslip[k] = fault_qd_node(a.fuse.a, k, tq);
+      a.bcL[k] = sS[k] * (1.0 / a.fts);
+      if (a.remote) a.bcR[k] = rS[k];
+    }
+    return;
+  }
+  if (sblk == 0 && tid == 0) {
+    a.sc->tol2 = a.tol2;
+    a.sc->maxit = 0;
+    a.sc->done = 0;
+    a.sc->iters = 0;
+    a.sc->cnt = 0;
+  }
+  if (sblk == 0 && tid < M) a.coefOut[tid] = c[tid];
+  double r2 = 0.0, b2 = 0.0;
+  for (size_t i0 = iFirst; i0 < a.N; i0 += kU * stride) {
+    if (i0 != iFirst) load_pass(i0);
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const size_t i = i0 + u * stride;
+      if (i < a.N) {
+        double az = 0.0;
+#pragma unroll
+        for (int m = 0; m < M; ++m) az += c[m] * zv[u][m];
+        double bi = 0.0;
+        if (i < a.zLo || i >= a.zHi) {  // a band row
+          const int k = (int)(i % (size_t)Nz);
+          bi = av[u][0] * (sS[k] * (1.0 / a.fts)) + av[u][1] * rS[k];
+        }
+        const double ri = bi - az;
+        r2 += ri * ri;
+        b2 += bi * bi;
+      }
+    }
+  }
+  // block partials: waves in order
+  r2 = wave_sum(r2);
+  b2 = wave_sum(b2);
+  __syncthreads();  // red is reused
+  if (lane == 0) {
+    red[0][wid] = r2;
+    red[1][wid] = b2;
+  }
+  __syncthreads();
+  if (tid < 2) {
+    double v = 0.0;
+    for (int q = 0; q < kStageThreads / 64; ++q) v += red[tid][q];
+    a.parts[(size_t)sblk * 2 + tid] = v;
+  }
+}
+
+int stage_bufs_ensure(QdStageBufs& b, int Nz, size_t N) {
+  const int nstream = (int)((N + 4 * kStageThreads - 1) / (4 * kStageThreads));  // 4 entries per thread
+  if (b.plan && b.nz == Nz && b.nstream == nstream) return SC_OK;
+  stage_bufs_free(b);
+  // PL, PR (SC_REUSE_MAX x Nz each), sums (2 SC_REUSE_MAX), aL, aR (2 kBand x Nz each)
+  SC_HIP(hipMalloc(&b.plan, (2 * (size_t)SC_REUSE_MAX * Nz + 2 * SC_REUSE_MAX + 4 * (size_t)kBand * Nz) *
+                                sizeof(double)));
+  SC_HIP(hipMalloc(&b.parts, (size_t)kStagePos * 2 * nstream * sizeof(double)));
+  SC_HIP(hipMalloc(&b.scal, kStagePos * sizeof(CgScal)));
+  SC_HIP(hipMemset(b.scal, 0, kStagePos * sizeof(CgScal)));
+  b.nz = Nz;
+  b.nstream = nstream;
+  return SC_OK;
+}
+
+void stage_bufs_free(QdStageBufs& b) {
+  if (b.plan) (void)hipFree(b.plan);
+  if (b.parts) (void)hipFree(b.parts);
+  if (b.scal) (void)hipFree(b.scal);
+  b = QdStageBufs{};
+}
+
+bool stage_fits(const Ctx& c) { return c.d.Nz <= kStageNzMax && c.projCount > 0; }
+
+PendingFinish stage_finish(const QdStageBufs& b, int pos, int* miss) {
+  return PendingFinish{(CgScal*)b.scal + pos, b.parts + (size_t)pos * 2 * b.nstream, b.nstream, miss};
+}
+
+int launch_stage_plan(const Ctx& c, QdStageBufs& b, const double* rfix, hipStream_t s) {
+  const int M = c.projCount, Nz = c.d.Nz;
+  if (!stage_fits(c)) return fail(SC_ERR_ARG, "stage kernel: no basis or Nz too large");
+  int nLow, nHigh;
+  qd_band_rows(c.d.Ny, &nLow, &nHigh);
+  VecList Y{};
+  for (int m = 0; m < M; ++m) Y.v[m] = c.projY[m];
+  double* PL = b.plan;
+  double* PR = b.plan + (size_t)SC_REUSE_MAX * Nz;
+  double* sR = b.plan + 2 * (size_t)SC_REUSE_MAX * Nz;
+  double* aL = sR + 2 * SC_REUSE_MAX;
+  double* aR = aL + 2 * (size_t)kBand * Nz;
+  const dim3 g((Nz + 255) / 256), bl(256);
+  switch (M) {
+#define SC_SP(m)                                                                                    \
+  case m:                                                                                           \
+    hipLaunchKernelGGL(k_stage_plan<m>, g, bl, 0, s, c.op64, nLow, nHigh, Y, PL, PR, aL, aR);       \
+    hipLaunchKernelGGL(k_stage_plan_sums<m>, dim3(1), dim3(256), 0, s, (const double*)PR, rfix, Nz, sR); \
+    break;
+    SC_SP(1) SC_SP(2) SC_SP(3) SC_SP(4) SC_SP(5) SC_SP(6) SC_SP(7) SC_SP(8)
+#undef SC_SP
+  }
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
+int launch_qd_stage(const Ctx& c, const QdStageBufs& b, int pos, const QdStageIn& in, hipStream_t s) {
+  const int M = c.projCount, Nz = c.d.Nz;
+  if (!stage_fits(c) || b.nz != Nz || pos < 0 || pos >= kStagePos)
+    return fail(SC_ERR_ARG, "stage kernel: buffers or position");
+  if (in.L.nv && (in.L.nv != 2 || in.L.nt > kStageNT || in.psiIdx < 0 || in.slipIdx < 0 || in.psiIdx == in.slipIdx))
+    return fail(SC_ERR_ARG, "stage kernel: the combination must hold psi and slip with at most 2 terms");
+  QdStageArgs a{};
+  int nLow, nHigh;
+  qd_band_rows(c.d.Ny, &nLow, &nHigh);
+  a.Nz = Nz;
+  a.nfb = (Nz + 63) / 64;
+  a.nstream = b.nstream;
+  a.N = (size_t)c.N;
+  a.zLo = (size_t)nLow * Nz;
+  a.zHi = (size_t)(c.d.Ny - nHigh) * Nz;
+  a.aL = b.plan + 2 * (size_t)SC_REUSE_MAX * Nz + 2 * SC_REUSE_MAX;
+  a.aR = a.aL + 2 * (size_t)kBand * Nz;
+  for (int m = 0; m < M; ++m) {
+    a.Z.v[m] = c.projZ[m];
+    a.T.v[m] = c.projT[m];
+  }
+  a.w = c.projW;
+  a.PL = b.plan;
+  a.sR = b.plan + 2 * (size_t)SC_REUSE_MAX * Nz;
+  a.coefOut = c.projW + 2 * SC_REUSE_MAX;
+  a.parts = b.parts + (size_t)pos * 2 * b.nstream;
+  a.sc = (CgScal*)b.scal + pos;
+  a.tol2 = in.tol2;
+  a.fuse = in.fuse;
+  a.L = in.L;
+  a.psiIdx = in.psiIdx;
+  a.slipIdx = in.slipIdx;
+  a.slipVar = in.slipVar;
+  a.fts = in.fts;
+  a.vL = in.vL;
+  a.remote = in.remote;
+  a.rfix = in.rfix;
+  a.bcL = in.bcL;
+  a.bcR = in.bcR;
+  a.gate = in.gate;
+  a.tfrac = in.tfrac;
+  a.fin = in.fin;
+  const dim3 g(a.nfb + a.nstream + (in.fin.sc ? 1 : 0)), bl(kStageThreads);
+  switch (M) {
+#define SC_ST(m) \
+  case m: hipLaunchKernelGGL(k_qd_stage<m>, g, bl, 0, s, a); break;
+    SC_ST(1) SC_ST(2) SC_ST(3) SC_ST(4) SC_ST(5) SC_ST(6) SC_ST(7) SC_ST(8)
+#undef SC_ST
+  }
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
 }  // namespace sc
