@@ -23,7 +23,12 @@ new solution onto the last --reuse solutions.  `value` is the fastest path
 whose fault slip stays within 1e-8 (relative) of fp64_ref over the window; the
 line names it (config.solve) and lists every path.
 
-Also on the line: the HBM roofline of the FP64 SBP-SAT apply (one launch of
+`roofline` is the dominant kernel of the timed step, the device loop's
+per-stage kernel (k_qd_stage: stage state, projections, fault rates and the
+projected guess's residual in one launch; three per attempt), timed with HIP
+events on the context stream; `roofline_guess` the projected-guess kernel of
+the host-checked solves.  Also on the line: the HBM roofline of the FP64
+SBP-SAT apply (one launch of
 interior tiles and frame blocks, 24 algorithmic bytes per node: read u, read
 mu, write A u),
 timed with HIP events on the context stream; `traffic` from the committed
@@ -50,6 +55,7 @@ HBM_PEAK = 8.0e12  # B/s, MI355X
 SLIP_TOL = 1e-8
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "apply_fp64_1001x1001_traffic.json")
 GUESS_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "proj_guess_fp64_1001x1001_traffic.json")
+STAGE_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "qd_stage_fp64_1001x1001_traffic.json")
 
 
 def log(*a):
@@ -104,6 +110,18 @@ def roofline_guess(ctx, reps=200):
     from scycle_amd._lib import check, load
     ms, words = C.c_double(), C.c_double()
     check(load().sc_bench_apply(ctx, 8, reps, C.byref(ms), C.byref(words)))
+    return dict(ms=ms.value, bytes=8.0 * words.value, GBps=8.0 * words.value / (ms.value * 1e-3) / 1e9)
+
+
+def roofline_stage(q, t, reps=200):
+    """The device loop's stage kernel (k_qd_stage<M>) in its accepted-state
+    form on the basis and state the QD run left: it reads the M operator
+    products over the grid, the band nodes' unit-datum rhs, and per fault node
+    the plan, tractions, friction inputs and outputs; mean of `reps` launches
+    timed with HIP events on the context stream (sc_qd_bench_stage)."""
+    from scycle_amd._lib import check, load
+    ms, words = C.c_double(), C.c_double()
+    check(load().sc_qd_bench_stage(q.h, t, reps, C.byref(ms), C.byref(words)))
     return dict(ms=ms.value, bytes=8.0 * words.value, GBps=8.0 * words.value / (ms.value * 1e-3) / 1e9)
 
 
@@ -286,11 +304,12 @@ def kernels_only(n, reps, reuse=4):
     # ends with the same number of vectors), then the guess kernel alone; the
     # host-side step control keeps every launch a plain dispatch for the
     # counter runs
-    q, _ = run_path(cfg, "fp64_fdiag", 10, 500, reuse=reuse, step_control="host")
+    q, r = run_path(cfg, "fp64_fdiag", 10, 500, reuse=reuse, step_control="host")
     gq = roofline_guess(load().sc_qd_ctx(q.h), reps)
+    sq = roofline_stage(q, r["t_end"], reps)
     q.close()
     print(json.dumps({"apply64_ms": ms.value, "nodes": nodes.value, "copy_bytes": 16 * N,
-                      "guess": gq, "reps": reps}))
+                      "guess": gq, "stage": sq, "reps": reps}))
 
 
 def main():
@@ -374,20 +393,32 @@ def main():
         extra["mixed_over_fp64_same_preconditioner"] = (paths["mixed"]["steps_per_s"]
                                                         / paths["fp64_fdiag"]["steps_per_s"])
     from scycle_amd._lib import load
-    # the dominant kernel of the timed step: the projected guess (the steps'
-    # solves mostly stop at it); the SBP apply alongside
+    # the dominant kernel of the timed step: the device loop's stage kernel;
+    # the host-checked solves' guess kernel and the SBP apply alongside
+    if paths[best]["reuse"] == 0:
+        sys.exit("bench.py: the fastest path runs without a solution basis; the stage and guess "
+                 "kernels need one (run it with --reuse > 0)")
+    sq = roofline_stage(mediators[best], paths[best]["t_end"])
+    log(f"[bench] stage kernel {sq}")
     gq = roofline_guess(load().sc_qd_ctx(mediators[best].h))
     log(f"[bench] projected-guess kernel {gq}")
     rl = roofline(load().sc_qd_ctx(mediators[best].h))
     log(f"[bench] apply {rl}")
-    # the counter summary holds the kernel with the basis size this window ends with
-    gtraffic = traffic_per_launch(GUESS_TRAFFIC_FILE, "hbm_bytes_per_launch")
-    if traffic_per_launch(GUESS_TRAFFIC_FILE, "algorithmic_bytes_per_launch") != gq["bytes"]:
-        gtraffic = None
-    roof = dict(bound="hbm", achieved=gq["GBps"], peak=HBM_PEAK / 1e9, unit="GB/s",
-                frac=gq["GBps"] * 1e9 / HBM_PEAK, traffic=gtraffic,
-                kernel="k_proj_guess_res<M> (projected initial guess + residual of each QD solve)",
-                ms_per_launch=gq["ms"], algorithmic_bytes_per_launch=gq["bytes"])
+
+    def counted(path, alg):
+        # the counter summary holds the kernel with the basis size this window ends with
+        t = traffic_per_launch(path, "hbm_bytes_per_launch")
+        return t if traffic_per_launch(path, "algorithmic_bytes_per_launch") == alg else None
+
+    roof = dict(bound="hbm", achieved=sq["GBps"], peak=HBM_PEAK / 1e9, unit="GB/s",
+                frac=sq["GBps"] * 1e9 / HBM_PEAK, traffic=counted(STAGE_TRAFFIC_FILE, sq["bytes"]),
+                kernel="k_qd_stage<M> (one RK stage of the device loop: stage state, projections, fault rates, "
+                       "projected guess and its residual)",
+                ms_per_launch=sq["ms"], algorithmic_bytes_per_launch=sq["bytes"])
+    roof_guess = dict(bound="hbm", achieved=gq["GBps"], peak=HBM_PEAK / 1e9, unit="GB/s",
+                      frac=gq["GBps"] * 1e9 / HBM_PEAK, traffic=counted(GUESS_TRAFFIC_FILE, gq["bytes"]),
+                      kernel="k_proj_guess_res<M> (projected guess + residual of the host-checked solves)",
+                      ms_per_launch=gq["ms"], algorithmic_bytes_per_launch=gq["bytes"])
     a64 = rl["apply64"]
     roof_apply = dict(bound="hbm", achieved=a64["GBps"], peak=HBM_PEAK / 1e9, unit="GB/s",
                       frac=a64["GBps"] * 1e9 / HBM_PEAK, traffic=traffic_per_launch(),
@@ -421,7 +452,7 @@ def main():
                                f"{args.warmup + args.steps} from t=0",
                    "solve": best, "precond": PATHS[best][1], "reuse": paths[best]["reuse"],
                    "rtol": 1e-12},
-        "roofline": roof, "roofline_apply": roof_apply, "cpu_baseline": cpu, "fd_c4": fd,
+        "roofline": roof, "roofline_guess": roof_guess, "roofline_apply": roof_apply, "cpu_baseline": cpu, "fd_c4": fd,
         "adjoint_c5": adj,
     }
     line.update(extra)

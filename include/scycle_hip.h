@@ -349,6 +349,14 @@ int sc_qd_ddt(sc_qd* q, double t, double* dpsi_h, double* dslip_h);
 int sc_qd_integrate(sc_qd* q, const sc_qd_run* run, double* tEnd_h, int* steps_h,
                     double* dts_h, int dtsCap);
 int sc_qd_get_counters(sc_qd* q, sc_qd_counters* c);
+/* Measurement: mean duration (ms) of `reps` back-to-back launches of the
+ * device loop's stage kernel (k_qd_stage) in its accepted-state form — the
+ * rates at the current state and time t, which it leaves as d_dt would up
+ * to rounding — timed with HIP events on the context stream, and the 8-byte
+ * words it moves per launch.  Needs the linear-elastic mediator, a solution
+ * basis (an integration with reuse > 0) and Nz <= 2048.  No counterpart in
+ * the reference (its timers are MPI_Wtime accumulators). */
+int sc_qd_bench_stage(sc_qd* q, double t, int reps, double* ms_per_launch, double* words);
 /* Time-integration settings read from the deck (timeIntegrator,
  * timeControlType, maxStepCount, init/maxTime, min/max/initDeltaT with
  * computeMinTimeStep, timeStepTol, normType), for sc_qd_create_from_file

@@ -186,6 +186,73 @@ sc_fault_qd_args QdDriver::fault_args(const double* psi, double* dpsi) const {
   return fa;
 }
 
+QdStageIn QdDriver::stage_input(const VarsLinComb* L, int st, StepGate g, double tfrac, PendingFinish f) {
+  VecSet& P = vars["psi"];
+  VecSet& Q = vars["slip"];
+  int psiIdx = 0, slipIdx = 0, i = 0;
+  for (auto& kv : vars) {
+    if (kv.first == "psi") psiIdx = i;
+    if (kv.first == "slip") slipIdx = i;
+    ++i;
+  }
+  QdStageIn in;
+  in.tol2 = desc.solve.rtol * desc.solve.rtol;
+  const double* psi = L ? P.k[st] : P.var;
+  double* dpsi = L ? P.f[st] : P.dvar;
+  in.fuse = QdFuse{fault_args(psi, dpsi), L ? Q.f[st] : Q.dvar};
+  if (L) in.L = *L;
+  in.psiIdx = psiIdx;
+  in.slipIdx = slipIdx;
+  in.slipVar = Q.var;
+  in.fts = desc.faultTypeScale;
+  in.vL = desc.vL;
+  in.remote = desc.remoteLoadingR;
+  in.rfix = desc.remoteLoadingR ? shift_d : bcR_d;
+  in.bcL = bcL_d;
+  in.bcR = bcR_d;
+  in.gate = g;
+  in.tfrac = tfrac;
+  in.fin = f;
+  return in;
+}
+
+int QdDriver::bench_stage(double t, int reps, double* ms, double* words) {
+  if (pl) return fail(SC_ERR_ARG, "the stage kernel is the linear-elastic mediator's");
+  if (reps < 1) return fail(SC_ERR_ARG, "reps must be >= 1");
+  Ctx& c = ctx->c;
+  if (!stage_fits(c)) return fail(SC_ERR_ARG, "the stage kernel needs a solution basis (integrate with reuse > 0) and Nz <= 2048");
+  hipStream_t s = c.stream;
+  int e = flush_finish();
+  if (!e) e = stage_bufs_ensure(stageBufs, Nz, (size_t)N);
+  if (!e) e = launch_stage_plan(c, stageBufs, desc.remoteLoadingR ? shift_d : bcR_d, s);
+  if (e) return e;
+  QdStageIn in = stage_input(nullptr, 0, StepGate{}, 0.0, PendingFinish{});
+  in.t0 = t;
+  if ((e = launch_qd_stage(c, stageBufs, 2, in, s))) return e;  // warm-up
+  hipEvent_t e0, e1;
+  SC_HIP(hipEventCreate(&e0));
+  SC_HIP(hipEventCreate(&e1));
+  SC_HIP(hipEventRecord(e0, s));
+  for (int r = 0; r < reps && !e; ++r) e = launch_qd_stage(c, stageBufs, 2, in, s);
+  SC_HIP(hipEventRecord(e1, s));
+  SC_HIP(hipEventSynchronize(e1));
+  float msTot = 0.f;
+  SC_HIP(hipEventElapsedTime(&msTot, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e) return e;
+  uLazy = true;  // u is the last launch's guess at the current state
+  *ms = (double)msTot / reps;
+  // the M operator products over the grid, the band nodes' unit-datum rhs
+  // (2 per node), and per fault node PL, T (M each), slip, the right datum,
+  // nine friction and state inputs and eight outputs
+  int nLow, nHigh;
+  qd_band_rows(c.d.Ny, &nLow, &nHigh);
+  const double M = c.projCount;
+  *words = M * N + 2.0 * (nLow + nHigh) * Nz + (2.0 * M + 19.0) * Nz;
+  return SC_OK;
+}
+
 int QdDriver::materialize_u() {
   if (!uLazy) return SC_OK;
   uLazy = false;
@@ -374,31 +441,8 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
     if (!e0) e0 = launch_stage_plan(ctx->c, stageBufs, desc.remoteLoadingR ? shift_d : bcR_d, s);
     if (e0) return e0;
   }
-  int psiIdx = 0;
-  for (auto& kv : vars) {
-    if (kv.first == "psi") break;
-    ++psiIdx;
-  }
   auto stage_in = [&](const VarsLinComb* L, int st, StepGate g, double tfrac, PendingFinish f) {
-    QdStageIn in;
-    in.tol2 = desc.solve.rtol * desc.solve.rtol;
-    const double* psi = L ? P.k[st] : P.var;
-    double* dpsi = L ? P.f[st] : P.dvar;
-    in.fuse = QdFuse{fault_args(psi, dpsi), L ? Q.f[st] : Q.dvar};
-    if (L) in.L = *L;
-    in.psiIdx = psiIdx;
-    in.slipIdx = slipIdx;
-    in.slipVar = Q.var;
-    in.fts = desc.faultTypeScale;
-    in.vL = desc.vL;
-    in.remote = desc.remoteLoadingR;
-    in.rfix = desc.remoteLoadingR ? shift_d : bcR_d;
-    in.bcL = bcL_d;
-    in.bcR = bcR_d;
-    in.gate = g;
-    in.tfrac = tfrac;
-    in.fin = f;
-    return in;
+    return stage_input(L, st, g, tfrac, f);
   };
   auto slot = [&]() -> int {
     if (oneKernel) {
@@ -957,6 +1001,11 @@ extern "C" int sc_qd_destroy(sc_qd* q) {
 }
 
 extern "C" sc_ctx* sc_qd_ctx(sc_qd* q) { return q ? q->q->ctx : nullptr; }
+
+extern "C" int sc_qd_bench_stage(sc_qd* h, double t, int reps, double* ms_per_launch, double* words) {
+  if (!h || !ms_per_launch || !words) return fail(SC_ERR_ARG, "null argument");
+  return h->q->bench_stage(t, reps, ms_per_launch, words);
+}
 
 extern "C" int sc_qd_set_state(sc_qd* h, const double* psi_h, const double* slip_h,
                                const double* V_h) {

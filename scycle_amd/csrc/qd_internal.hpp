@@ -74,6 +74,7 @@ struct QdStageIn {
   double* bcR = nullptr;
   StepGate gate{};
   double tfrac = 0;
+  double t0 = 0;  // the time without a gate
   PendingFinish fin{};
 };
 int stage_bufs_ensure(QdStageBufs& b, int Nz, size_t N);
@@ -200,7 +201,12 @@ struct QdDriver {
            StepGate gate = StepGate{}, double tfrac = 0.0);
   int d_dt_vars(double t);
   // the fault-rate arguments for a stage's psi and psi rate
-  sc_fault_qd_args fault_args(const double* psi, double* dpsi) const;              // on var -> dvar of every integrated variable
+  sc_fault_qd_args fault_args(const double* psi, double* dpsi) const;
+  // a stage kernel's inputs: L the combination of stage st (nullptr: the
+  // rates at the current state)
+  QdStageIn stage_input(const VarsLinComb* L, int st, StepGate g, double tfrac, PendingFinish f);
+  // times the stage kernel in its accepted-state form (sc_qd_bench_stage)
+  int bench_stage(double t, int reps, double* ms, double* words);              // on var -> dvar of every integrated variable
   int check_status();
   // fault status and missed-guess flag in one read-back (the flags are left set)
   int read_flags(int* status, int* miss);

@@ -503,7 +503,7 @@ struct QdStageArgs {
   double* bcL;
   double* bcR;
   StepGate gate;
-  double tfrac;
+  double tfrac, t0;   // t0: the time without a gate
   PendingFinish fin;  // the previous stage's residual test (the last block)
 };
 
@@ -572,7 +572,7 @@ __global__ __launch_bounds__(kStageThreads) void k_qd_stage(QdStageArgs a) {
   double cc[kStageNT];
 #pragma unroll
   for (int i = 0; i < kStageNT; ++i) cc[i] = a.L.c[i];
-  double t = 0.0;
+  double t = a.t0;
   if (a.gate.ctl) {  // this attempt's step and stage time
     const double h = a.gate.ctl->dt;
     t = a.gate.ctl->t + a.tfrac * h;
@@ -780,6 +780,7 @@ int launch_qd_stage(const Ctx& c, const QdStageBufs& b, int pos, const QdStageIn
   a.bcR = in.bcR;
   a.gate = in.gate;
   a.tfrac = in.tfrac;
+  a.t0 = in.t0;
   a.fin = in.fin;
   const dim3 g(a.nfb + a.nstream + (in.fin.sc ? 1 : 0)), bl(kStageThreads);
   switch (M) {
