@@ -289,3 +289,36 @@ def test_qd_device_step_control(defer):
     assert np.allclose(d["dts"], odts, rtol=1e-6) and abs(d["t"] / tE.value - 1) <= 1e-6
     e = _fields_err(d["st"], d["u"], ref)
     assert max(e.values()) <= 1e-7, e
+
+
+@pytest.mark.parametrize("N", [25, 33])
+def test_qd_stage_kernel_rates(N):
+    """The device loop's stage kernel (k_qd_stage: stage state, projections
+    from the per-basis plan, fault rates, projected guess) in its
+    accepted-state form against a host-checked d_dt at the same state and
+    time: slip rate, state rate (through a second evaluation) and the
+    displacement formed from the guess.  The guess meets the solve tolerance
+    (1e-13) at a state the basis was built on, so the rates agree to the
+    root solve's and the solve's tolerance."""
+    from scycle_amd import QdMediator
+    from scycle_amd._lib import check, load
+    cfg = _small_cfg(N, N, 4, True)
+    g = QdMediator(cfg, solve="cg64", rtol=1e-13, jac_mode=1, precond="fdiag64", reuse=2)
+    g.set_state(cfg.psi0, cfg.slip0, np.zeros(N))
+    tEnd, nst, _ = g.integrate(40, tol=1e-8, init_dt=1e-3, max_dt=1e10)
+    assert nst == 40
+    ms, words = C.c_double(), C.c_double()
+    check(load().sc_qd_bench_stage(g.h, tEnd, 3, C.byref(ms), C.byref(words)))
+    M = 2
+    assert words.value == M * N * N + 2 * 18 * N + (2 * M + 19) * N
+    st = g.get_state()
+    u_stage = g.get_u()
+    dpsi, dslip = g.d_dt(tEnd)  # host-checked solve at the same state
+    st2 = g.get_state()
+    u_ref = g.get_u()
+    print("V", np.abs(st["V"] - dslip).max() / np.abs(dslip).max(),
+          "u", np.abs(u_stage - u_ref).max() / np.abs(u_ref).max(), flush=True)
+    assert np.abs(st["V"] - dslip).max() <= 1e-9 * np.abs(dslip).max()
+    assert np.abs(st["tau"] - st2["tau"]).max() <= 1e-9 * np.abs(st2["tau"]).max()
+    assert np.abs(u_stage - u_ref).max() <= 1e-9 * np.abs(u_ref).max()
+    g.close()
