@@ -338,7 +338,7 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
   *finished = *halted = false;
   hipStream_t s = ctx->c.stream;
   if (!ctl_d) {
-    SC_HIP(hipMalloc(&ctl_d, sizeof(StepCtl)));
+    SC_HIP(hipMalloc(&ctl_d, 2 * sizeof(StepCtl)));  // two for the stage loop's ping-pong
     SC_HIP(hipHostMalloc(&ctl_h, sizeof(StepCtl), hipHostMallocDefault));
   }
   if (dts && dtsCap > ctlDtsCap) {
@@ -444,13 +444,75 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
   auto stage_in = [&](const VarsLinComb* L, int st, StepGate g, double tfrac, PendingFinish f) {
     return stage_input(L, st, g, tfrac, f);
   };
+  // The stage loop's attempts alternate between two states and two control
+  // blocks (parity p): an attempt's kernels read state p and control block
+  // p, and its last kernel (the attempt's end and the rates at the accepted
+  // state, every block alike) writes state and control block 1 - p, so that
+  // no launch writes what its other blocks read.  The direct attempt has
+  // p = 0 and a graph holds an even number of attempts starting at p = 1:
+  // after every batch the latest state and control block are those of p = 1.
+  static_assert(kAttemptsPerGraph % 2 == 0, "the stage loop's ping-pong needs an even number of attempts per graph");
+  double* varB[2][2] = {{P.var, Q.var}, {nullptr, nullptr}};
+  double* dvarB[2][2] = {{P.dvar, Q.dvar}, {nullptr, nullptr}};
+  if (oneKernel) {
+    double* alt = stageBufs.alt;
+    varB[1][0] = alt;
+    varB[1][1] = alt + Nz;
+    dvarB[1][0] = alt + 2 * (size_t)Nz;
+    dvarB[1][1] = alt + 3 * (size_t)Nz;
+  }
+  int psiIdx = 0, slipIdxL = 0, vi = 0;
+  for (auto& kv : vars) {
+    if (kv.first == "psi") psiIdx = vi;
+    if (kv.first == "slip") slipIdxL = vi;
+    ++vi;
+  }
+  auto at_parity = [&](VarsLinComb L, int p) {  // base and f0 of state p
+    L.base[psiIdx] = varB[p][0];
+    L.base[slipIdxL] = varB[p][1];
+    L.f[psiIdx][0] = dvarB[p][0];
+    L.f[slipIdxL][0] = dvarB[p][1];
+    return L;
+  };
+  int selSlot[2], nsSel = 0;
+  selSlot[0] = (errMask & 1) ? nsSel++ : -1;
+  selSlot[1] = (errMask & 2) ? nsSel++ : -1;
+  int parity = 0;
   auto slot = [&]() -> int {
     if (oneKernel) {
+      const int p = parity;
+      parity = 1 - p;
+      const StepGate gp{ctl_d + p, 1};
       const PendingFinish prev = finPending ? fin : PendingFinish{};
-      int e = launch_qd_stage(ctx->c, stageBufs, 0, stage_in(&L1, 1, g1, 0.5, prev), s);
-      if (!e) e = launch_qd_stage(ctx->c, stageBufs, 1, stage_in(&L2, 2, g1, 1.0, stage_finish(stageBufs, 0, miss_d)), s);
-      if (!e) e = launch_step_end(s, StepEnd{stage_finish(stageBufs, 1, miss_d), S, ctl_d, err_d, status_d, cp}, g1);
-      if (!e) e = launch_qd_stage(ctx->c, stageBufs, 2, stage_in(nullptr, 0, g2, 0.0, PendingFinish{}), s);
+      const VarsLinComb A1 = at_parity(L1, p), A2 = at_parity(L2, p);
+      int e = launch_qd_stage(ctx->c, stageBufs, 0, stage_in(&A1, 1, gp, 0.5, prev), s);
+      if (!e) e = launch_qd_stage(ctx->c, stageBufs, 1, stage_in(&A2, 2, gp, 1.0, stage_finish(stageBufs, 0, miss_d)), s);
+      if (!e) {
+        QdStageIn in = stage_in(nullptr, 0, gp, 0.0, PendingFinish{});
+        in.fuse = QdFuse{fault_args(varB[1 - p][0], dvarB[1 - p][0]), dvarB[1 - p][1]};
+        in.end = true;
+        in.ctlOut = ctl_d + (1 - p);
+        in.flags = status_d;
+        in.fin3 = stage_finish(stageBufs, 1, miss_d);
+        for (int f = 0; f < 2; ++f) {
+          in.pv[f] = varB[p][f];
+          in.pvOut[f] = varB[1 - p][f];
+          in.pf[f][0] = dvarB[p][f];
+          in.pf[f][1] = f == 0 ? P.f[1] : Q.f[1];
+          in.pf[f][2] = f == 0 ? P.f[2] : Q.f[2];
+          in.selSlot[f] = selSlot[f];
+        }
+        // the RK32 pair: lo = y + h/2 (f0 + f2), hi = y + h/6 f0 + 2h/3 f1 + h/6 f2
+        in.clo[0] = 0.5;
+        in.clo[1] = 0.5;
+        in.chi[0] = 1;
+        in.chi[1] = 2;
+        in.chi[2] = 1;
+        in.cdhi[0] = 6;
+        in.cdhi[1] = 3;
+        in.cdhi[2] = 6;
+        e = launch_qd_stage(ctx->c, stageBufs, 2, in, s);
+      }
       // the rates' residual test is left for the next launch; u for proj_materialize
       fin = stage_finish(stageBufs, 2, miss_d);
       finPending = true;
@@ -503,7 +565,7 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
     }
     finPending = true;  // the last replayed attempt's rates
     if (e) break;
-    SC_HIP(hipMemcpyAsync(ctl_h, ctl_d, sizeof(StepCtl), hipMemcpyDeviceToHost, s));
+    SC_HIP(hipMemcpyAsync(ctl_h, ctl_d + (oneKernel ? 1 : 0), sizeof(StepCtl), hipMemcpyDeviceToHost, s));
     SC_HIP(hipStreamSynchronize(s));
     // what ran since the last read-back: two stage solves per attempt and
     // one more per accepted step
@@ -523,6 +585,12 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
   }
   defer = false;
   if (e) return e;
+  if (oneKernel) {  // the latest state is state 1: back into var and dvar
+    for (int f = 0; f < 2; ++f) {
+      SC_HIP(hipMemcpyAsync(varB[0][f], varB[1][f], (size_t)Nz * sizeof(double), hipMemcpyDeviceToDevice, s));
+      SC_HIP(hipMemcpyAsync(dvarB[0][f], dvarB[1][f], (size_t)Nz * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+  }
   const StepCtl& c = *ctl_h;
   r.currT = c.t;
   r.deltaT = c.dt;

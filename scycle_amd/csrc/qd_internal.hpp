@@ -59,6 +59,9 @@ struct QdStageBufs {
   double* plan = nullptr;
   double* parts = nullptr;
   void* scal = nullptr;
+  // the second state of the device loop's ping-pong: psi, slip, their rates
+  // (Nz each)
+  double* alt = nullptr;
   int nz = 0, nstream = 0;
 };
 struct QdStageIn {
@@ -76,6 +79,19 @@ struct QdStageIn {
   double tfrac = 0;
   double t0 = 0;  // the time without a gate
   PendingFinish fin{};
+  // the attempt's end in the same launch (the rates at the accepted state):
+  // gate.ctl is the attempt's control block, ctlOut the next attempt's;
+  // fields 0 psi, 1 slip: var, the three stage rates, the pair's templates
+  // and the error slots (-1: not in the norm)
+  bool end = false;
+  StepCtl* ctlOut = nullptr;
+  int* flags = nullptr;
+  PendingFinish fin3{};
+  const double* pv[2] = {};
+  double* pvOut[2] = {};
+  const double* pf[2][3] = {};
+  double clo[2] = {}, cdlo[2] = {1, 1}, chi[3] = {}, cdhi[3] = {1, 1, 1};
+  int selSlot[2] = {-1, -1};
 };
 int stage_bufs_ensure(QdStageBufs& b, int Nz, size_t N);
 void stage_bufs_free(QdStageBufs& b);

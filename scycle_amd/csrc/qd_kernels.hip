@@ -505,6 +505,16 @@ struct QdStageArgs {
   StepGate gate;
   double tfrac, t0;   // t0: the time without a gate
   PendingFinish fin;  // the previous stage's residual test (the last block)
+  // the attempt's end (END kernels; fields 0 psi, 1 slip)
+  const StepCtl* ctlIn;
+  StepCtl* ctlOut;
+  int* flags;          // fault status, missed guess
+  PendingFinish fin3;  // the last stage's residual test
+  const double* pv[2];      // the attempt's state (read)
+  double* pvOut[2];         // the next attempt's state (written)
+  const double* pf[2][3];  // the rates of the attempt's three stages
+  double clo[2], cdlo[2], chi[3], cdhi[3];
+  int selSlot[2];
 };
 
 // Blocks [0, nfb): the stage state and fault rates of 64 fault nodes each
@@ -512,14 +522,15 @@ struct QdStageArgs {
 // [nfb, nfb + nstream): the residual b - sum c_m (A y_m) over the grid with
 // its r.r, b.b partials, b = aL l + aR r on the band rows; one more block:
 // the previous stage's residual test.
-template <int M>
+template <int M, bool END>
 __global__ __launch_bounds__(kStageThreads) void k_qd_stage(QdStageArgs a) {
   const int blk = blockIdx.x;
-  if (blk >= a.nfb + a.nstream) {
+  if (!END && blk >= a.nfb + a.nstream) {
     if (!(a.gate.ctl && a.gate.ctl->halt)) finish_irres_block(a.fin);
     return;
   }
   __shared__ double sS[kStageNzMax], rS[kStageNzMax];
+  __shared__ double pS[END ? kStageNzMax : 1];
   __shared__ double red[M > 2 ? M : 2][kStageThreads / 64];
   __shared__ double coef[M];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -555,7 +566,9 @@ __global__ __launch_bounds__(kStageThreads) void k_qd_stage(QdStageArgs a) {
   for (int q = 0; q < kQ; ++q) {
     const int k = tid + q * kStageThreads;
     if (k < Nz) {
-      if (a.L.nv) {
+      if (END) {
+        sb[q] = 0.0;  // the state is the attempt's solution, formed below
+      } else if (a.L.nv) {
         sb[q] = a.L.base[a.slipIdx][k];
 #pragma unroll
         for (int u = 0; u < kStageNT; ++u)
@@ -568,16 +581,147 @@ __global__ __launch_bounds__(kStageThreads) void k_qd_stage(QdStageArgs a) {
       for (int m = 0; m < M; ++m) pl[q][m] = a.PL[(size_t)m * Nz + k];
     }
   }
-  if (gate_closed(a.gate)) return;
   double cc[kStageNT];
 #pragma unroll
   for (int i = 0; i < kStageNT; ++i) cc[i] = a.L.c[i];
   double t = a.t0;
-  if (a.gate.ctl) {  // this attempt's step and stage time
-    const double h = a.gate.ctl->dt;
-    t = a.gate.ctl->t + a.tfrac * h;
+  if constexpr (END) {
+    // The attempt's end, formed by every block alike (the same loads and
+    // arithmetic, so every block reaches the same decision): the last
+    // stage's residual test (k_finish's order), the RK32 pair and its error
+    // sums, the step control on a copy of the control block.  Block 0
+    // writes the updated control block to ctlOut (the next attempt reads
+    // it; this attempt's kernels read ctlIn), the test's scalars and flag;
+    // the fault waves write var <- hi.  An accepted attempt then forms the
+    // rates at its solution.
+    constexpr int kCtlW = sizeof(StepCtl) / sizeof(unsigned long long);
+    __shared__ unsigned long long cw[kCtlW];
+    __shared__ double eS[2][3][kStageThreads / 64], fS[2][kRedThreads / 64], sel[3 * kMaxVars];
+    __shared__ int flg[2], accS;
+    StepCtl* cs = reinterpret_cast<StepCtl*>(cw);
+    if (tid < kCtlW) cw[tid] = reinterpret_cast<const unsigned long long*>(a.ctlIn)[tid];
+    if (tid == kCtlW) {
+      flg[0] = a.flags[0];
+      flg[1] = a.flags[1];
+    }
+    double w0 = 0.0, w1 = 0.0;
+    if (tid < kRedThreads)
+      for (int bq = tid; bq < a.fin3.nb; bq += kRedThreads) {
+        w0 += a.fin3.partials[(size_t)bq * 2];
+        w1 += a.fin3.partials[(size_t)bq * 2 + 1];
+      }
+    __syncthreads();
+    // a closed or rejected attempt passes the state and its rates on
+    auto pass_state = [&]() {
+      const int k = blk * 64 + tid;
+      if (blk < a.nfb && tid < 64 && k < Nz) {
+        a.pvOut[0][k] = a.pv[0][k];
+        a.pvOut[1][k] = a.pv[1][k];
+        a.fuse.a.dpsi_d[k] = a.pf[0][0][k];
+        a.fuse.dslip[k] = a.pf[1][0][k];
+      }
+    };
+    if (cs->halt || cs->done) {  // closed: pass the control block on
+      if (blk == 0 && tid == 0) {
+        StepCtl c = *cs;
+        c.accepted = 0;
+        *a.ctlOut = c;
+      }
+      pass_state();
+      return;
+    }
+    w0 = wave_sum(w0);
+    w1 = wave_sum(w1);
+    if (lane == 0 && wid < kRedThreads / 64) {
+      fS[0][wid] = w0;
+      fS[1][wid] = w1;
+    }
+    const double h = cs->dt;
+    double clo[2], chi[3];
 #pragma unroll
-    for (int i = 0; i < kStageNT; ++i) cc[i] = (a.L.c[i] * h) / a.L.cd[i];
+    for (int i = 0; i < 2; ++i) clo[i] = (a.clo[i] * h) / a.cdlo[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) chi[i] = (a.chi[i] * h) / a.cdhi[i];
+    double ea[2] = {0.0, 0.0}, eb[2] = {0.0, 0.0}, em[2] = {-INFINITY, -INFINITY};
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+      const int k = tid + q * kStageThreads;
+      if (k < Nz) {
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+          const double base = a.pv[f][k], f0 = a.pf[f][0][k], f1 = a.pf[f][1][k], f2 = a.pf[f][2][k];
+          double lo = clo[0] * f0 + base;  // y + h/2 (f0 + f2)
+          lo = lo + clo[1] * f2;
+          double hi = chi[0] * f0 + base;  // y + h (f0/6 + 2 f1/3 + f2/6)
+          hi = hi + chi[1] * f1;
+          hi = hi + chi[2] * f2;
+          const double e = -1.0 * hi + lo;
+          ea[f] += e * e;
+          eb[f] += hi * hi;
+          em[f] = fmax(em[f], fabs(e) / hi);
+          (f == 0 ? pS : sS)[k] = hi;
+        }
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      const double x = wave_sum(ea[f]), y = wave_sum(eb[f]), z = wave_max(em[f]);
+      if (lane == 0) {
+        eS[f][0][wid] = x;
+        eS[f][1][wid] = y;
+        eS[f][2][wid] = z;
+      }
+    }
+    __syncthreads();
+    bool miss3 = false;
+    if (tid < 64) {
+      double t0 = 0.0, t1 = 0.0;
+      for (int ww = 0; ww < kRedThreads / 64; ++ww) t0 += fS[0][ww];
+      for (int ww = 0; ww < kRedThreads / 64; ++ww) t1 += fS[1][ww];
+      miss3 = !(t0 <= a.tol2 * t1 || t1 == 0.0);
+      if (tid == 0) {
+        for (int q = 0; q < 3 * kMaxVars; ++q) sel[q] = 0.0;
+        for (int f = 0; f < 2; ++f) {
+          const int sl = a.selSlot[f];
+          if (sl < 0) continue;
+          double x = eS[f][0][0], y = eS[f][1][0], z = eS[f][2][0];
+          for (int ww = 1; ww < kStageThreads / 64; ++ww) {
+            x += eS[f][0][ww];
+            y += eS[f][1][ww];
+            z = fmax(z, eS[f][2][ww]);
+          }
+          sel[3 * sl] = x;
+          sel[3 * sl + 1] = y;
+          sel[3 * sl + 2] = z;
+        }
+        if (blk == 0) {  // the test's scalars and flag, for the host
+          a.fin3.sc->rr = t0;
+          a.fin3.sc->bb = t1;
+          a.fin3.sc->done = miss3 ? 0 : 1;
+          if (miss3 && a.fin3.flag) *a.fin3.flag = 1;
+        }
+      }
+    }
+    __syncthreads();
+    if (tid < 64) {
+      const bool ok = step_control_wave(cs, sel, flg[0] || flg[1] || miss3, blk == 0);
+      if (tid == 0) accS = ok ? 1 : 0;
+    }
+    __syncthreads();
+    if (blk == 0 && tid < kCtlW) reinterpret_cast<unsigned long long*>(a.ctlOut)[tid] = cw[tid];
+    if (!accS) {
+      pass_state();
+      return;
+    }
+    t = cs->t;  // the accepted step's end
+  } else {
+    if (gate_closed(a.gate)) return;
+    if (a.gate.ctl) {  // this attempt's step and stage time
+      const double h = a.gate.ctl->dt;
+      t = a.gate.ctl->t + a.tfrac * h;
+#pragma unroll
+      for (int i = 0; i < kStageNT; ++i) cc[i] = (a.L.c[i] * h) / a.L.cd[i];
+    }
   }
   // the stage's slip and right datum at every fault node (into LDS), and
   // this thread's share of sum_k PL_m(k) l(k)
@@ -588,8 +732,8 @@ __global__ __launch_bounds__(kStageThreads) void k_qd_stage(QdStageArgs a) {
   for (int q = 0; q < kQ; ++q) {
     const int k = tid + q * kStageThreads;
     if (k < Nz) {
-      double sk = sb[q];
-      if (a.L.nv) {  // the combination (k_vars_lincomb's order)
+      double sk = END ? sS[k] : sb[q];
+      if (!END && a.L.nv) {  // the combination (k_vars_lincomb's order)
         sk = cc[0] * sf[q][0] + sb[q];
 #pragma unroll
         for (int u = 1; u < kStageNT; ++u)
@@ -623,7 +767,10 @@ __global__ __launch_bounds__(kStageThreads) void k_qd_stage(QdStageArgs a) {
     // fault nodes: the stage state, tau = sum c_m T_m, the rates
     const int k = blk * 64 + tid;
     if (tid < 64 && k < Nz) {
-      if (a.L.nv) {  // psi of the stage (k_vars_lincomb's order)
+      if (END) {  // the accepted solution: the next attempt's state
+        a.pvOut[0][k] = pS[k];
+        a.pvOut[1][k] = sS[k];
+      } else if (a.L.nv) {  // psi of the stage (k_vars_lincomb's order)
         double x = cc[0] * a.L.f[a.psiIdx][0][k] + a.L.base[a.psiIdx][k];
 #pragma unroll
         for (int u = 1; u < kStageNT; ++u)
@@ -695,6 +842,7 @@ int stage_bufs_ensure(QdStageBufs& b, int Nz, size_t N) {
   SC_HIP(hipMalloc(&b.parts, (size_t)kStagePos * 2 * nstream * sizeof(double)));
   SC_HIP(hipMalloc(&b.scal, kStagePos * sizeof(CgScal)));
   SC_HIP(hipMemset(b.scal, 0, kStagePos * sizeof(CgScal)));
+  SC_HIP(hipMalloc(&b.alt, 4 * (size_t)Nz * sizeof(double)));
   b.nz = Nz;
   b.nstream = nstream;
   return SC_OK;
@@ -704,6 +852,7 @@ void stage_bufs_free(QdStageBufs& b) {
   if (b.plan) (void)hipFree(b.plan);
   if (b.parts) (void)hipFree(b.parts);
   if (b.scal) (void)hipFree(b.scal);
+  if (b.alt) (void)hipFree(b.alt);
   b = QdStageBufs{};
 }
 
@@ -782,10 +931,37 @@ int launch_qd_stage(const Ctx& c, const QdStageBufs& b, int pos, const QdStageIn
   a.tfrac = in.tfrac;
   a.t0 = in.t0;
   a.fin = in.fin;
-  const dim3 g(a.nfb + a.nstream + (in.fin.sc ? 1 : 0)), bl(kStageThreads);
+  if (in.end) {
+    if (!in.ctlOut || !in.gate.ctl || !in.flags || !in.fin3.sc || in.fin.sc || in.L.nv)
+      return fail(SC_ERR_ARG, "stage kernel: an attempt's end needs its control blocks, flags and test");
+    a.ctlIn = in.gate.ctl;
+    a.ctlOut = in.ctlOut;
+    a.flags = in.flags;
+    a.fin3 = in.fin3;
+    for (int f = 0; f < 2; ++f) {
+      a.pv[f] = in.pv[f];
+      a.pvOut[f] = in.pvOut[f];
+      if (!a.pv[f] || !a.pvOut[f] || a.pv[f] == a.pvOut[f])
+        return fail(SC_ERR_ARG, "stage kernel: an attempt's end reads and writes different states");
+      for (int r = 0; r < 3; ++r) a.pf[f][r] = in.pf[f][r];
+      a.selSlot[f] = in.selSlot[f];
+    }
+    for (int i = 0; i < 2; ++i) {
+      a.clo[i] = in.clo[i];
+      a.cdlo[i] = in.cdlo[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+      a.chi[i] = in.chi[i];
+      a.cdhi[i] = in.cdhi[i];
+    }
+  }
+  const dim3 g(a.nfb + a.nstream + (in.fin.sc && !in.end ? 1 : 0)), bl(kStageThreads);
   switch (M) {
-#define SC_ST(m) \
-  case m: hipLaunchKernelGGL(k_qd_stage<m>, g, bl, 0, s, a); break;
+#define SC_ST(m)                                                             \
+  case m:                                                                    \
+    if (in.end) hipLaunchKernelGGL((k_qd_stage<m, true>), g, bl, 0, s, a);   \
+    else hipLaunchKernelGGL((k_qd_stage<m, false>), g, bl, 0, s, a);         \
+    break;
     SC_ST(1) SC_ST(2) SC_ST(3) SC_ST(4) SC_ST(5) SC_ST(6) SC_ST(7) SC_ST(8)
 #undef SC_ST
   }

@@ -172,7 +172,7 @@ __device__ inline double step_size_wave(bool pCtl, int ord, double kappa, double
 // rejected attempt's retry and an accepted step's next size (computeStepSize
 // sees the same state in both), so it is computed once; an attempt accepted
 // at minDt or the attempt limit proposes again from the step it took.
-__device__ inline bool step_control_wave(StepCtl* c, const double* err, bool flagged) {
+__device__ inline bool step_control_wave(StepCtl* c, const double* err, bool flagged, bool writeDts = true) {
   const bool lead = (threadIdx.x & 63) == 0;
   if (c->halt || c->done) {
     if (lead) c->accepted = 0;
@@ -220,7 +220,7 @@ __device__ inline bool step_control_wave(StepCtl* c, const double* err, bool fla
   if (tot != 0.0) newDt = forced ? step_size_wave(pCtl, ord, kappa, tol, errA0, errA1, dt, minDt, maxDt, tot) : sz;
   const int nOut = c->nOut;
   if (lead) {
-    if (nOut < c->dtsCap) c->dts[nOut] = dt;
+    if (writeDts && nOut < c->dtsCap) c->dts[nOut] = dt;
     c->errA1 = errA0;
     c->errA0 = tot;
   }
