@@ -322,3 +322,35 @@ def test_qd_stage_kernel_rates(N):
     assert np.abs(st["tau"] - st2["tau"]).max() <= 1e-9 * np.abs(st2["tau"]).max()
     assert np.abs(u_stage - u_ref).max() <= 1e-9 * np.abs(u_ref).max()
     g.close()
+
+
+def test_qd_device_step_control_wide_fault():
+    """A fault longer than the stage kernel's tables (Nz = 2049 > 2048): the
+    device loop runs the rhs, guess and step-end kernels instead of one
+    kernel per stage, and takes the host loop's steps.  Checks are deferred
+    from the first guess on (the default waits for a run of 8 hits, which 30
+    steps of this start do not reach), and the solves are held to 1e-10 (a
+    basis of 3 solutions does not reach 1e-13 at this shape): the two loops'
+    step sizes and fields then agree to 1e-8 (measured 2e-9 for the step
+    sizes)."""
+    from scycle_amd import QdMediator
+    Ny, Nz = 17, 2049
+    cfg = _small_cfg(Ny, Nz, 4, True)
+    out = {}
+    for mode in ("host", "auto"):
+        g = QdMediator(cfg, solve="cg64", rtol=1e-10, jac_mode=0, precond="fdiag64", reuse=3)
+        g.set_state(cfg.psi0, cfg.slip0, np.zeros(Nz))
+        tEnd, nst, dts = g.integrate(30, tol=1e-8, init_dt=1e-3, max_dt=1e10, defer="always",
+                                     step_control=mode)
+        out[mode] = dict(t=tEnd, n=nst, dts=dts, st=g.get_state(), u=g.get_u(), c=g.counters())
+        g.close()
+    h, d = out["host"], out["auto"]
+    print("device steps", d["c"]["device_steps"], "missed", d["c"]["missed_guesses"], "host missed",
+          h["c"]["missed_guesses"], "deferred", h["c"]["deferred_checks"], flush=True)
+    assert h["c"]["device_steps"] == 0 and d["c"]["device_steps"] > 10
+    assert h["n"] == d["n"] == 30
+    assert np.allclose(d["dts"], h["dts"], rtol=1e-8, atol=0)
+    for k in ("psi", "slip", "V", "tau"):
+        ref = h["st"][k]
+        assert np.abs(d["st"][k] - ref).max() <= 1e-8 * np.abs(ref).max(), k
+    assert np.abs(d["u"] - h["u"]).max() <= 1e-8 * np.abs(h["u"]).max()
