@@ -426,15 +426,16 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
   const int nOut0 = nOut;
   int prevAcc = 0, prevRej = 0;
   defer = true;
-  // one attempt: both stages, the attempt's end (the last stage's residual
-  // test, the lo/hi pair, the step control: one launch), the rates after an
-  // accepted step
-  // With a basis and Nz within its tables each d_dt is one stage kernel
-  // (no rhs launch): the projection plan of the current basis first (the
-  // basis does not change inside the loop; a missed guess returns to the
-  // host).  The stage kernels of one attempt take positions 0 (stage 2),
-  // 1 (stage 3) and 2 (rates at the accepted state); each tests the previous
-  // position's residual, the step end tests position 1's.
+  // One attempt: both stages, the attempt's end (the last stage's residual
+  // test, the lo/hi pair, the step control), the rates after an accepted
+  // step.  With a basis and Nz within its tables each d_dt is one stage
+  // kernel (no rhs launch) and the attempt's end runs in the last one: the
+  // projection plan of the current basis first (the basis does not change
+  // inside the loop; a missed guess returns to the host).  The stage kernels
+  // of one attempt take positions 0 (stage 2), 1 (stage 3) and 2 (the end
+  // and the rates at the accepted state); positions 0 and 1 test the
+  // previous position's residual, position 2 tests position 1's.  Otherwise
+  // each d_dt is the rhs and guess kernels and k_step_end ends the attempt.
   const bool oneKernel = stage_fits(ctx->c);
   if (oneKernel) {
     int e0 = stage_bufs_ensure(stageBufs, Nz, (size_t)N);
