@@ -291,8 +291,8 @@ def test_qd_device_step_control(defer):
     assert max(e.values()) <= 1e-7, e
 
 
-@pytest.mark.parametrize("N", [25, 33])
-def test_qd_stage_kernel_rates(N):
+@pytest.mark.parametrize("N,order,var_grid", [(25, 4, True), (33, 4, True), (29, 2, False)])
+def test_qd_stage_kernel_rates(N, order, var_grid):
     """The device loop's stage kernel (k_qd_stage: stage state, projections
     from the per-basis plan, fault rates, projected guess) in its
     accepted-state form against a host-checked d_dt at the same state and
@@ -302,7 +302,7 @@ def test_qd_stage_kernel_rates(N):
     root solve's and the solve's tolerance."""
     from scycle_amd import QdMediator
     from scycle_amd._lib import check, load
-    cfg = _small_cfg(N, N, 4, True)
+    cfg = _small_cfg(N, N, order, var_grid)
     g = QdMediator(cfg, solve="cg64", rtol=1e-13, jac_mode=1, precond="fdiag64", reuse=2)
     g.set_state(cfg.psi0, cfg.slip0, np.zeros(N))
     tEnd, nst, _ = g.integrate(40, tol=1e-8, init_dt=1e-3, max_dt=1e10)
@@ -310,7 +310,7 @@ def test_qd_stage_kernel_rates(N):
     ms, words = C.c_double(), C.c_double()
     check(load().sc_qd_bench_stage(g.h, tEnd, 3, C.byref(ms), C.byref(words)))
     M = 2
-    assert words.value == M * N * N + 2 * 18 * N + (2 * M + 19) * N
+    assert words.value == M * N * N + 2 * 9 * 2 * N + (2 * M + 19) * N
     st = g.get_state()
     u_stage = g.get_u()
     dpsi, dslip = g.d_dt(tEnd)  # host-checked solve at the same state
