@@ -663,9 +663,15 @@ __global__ __launch_bounds__(kStageThreads) void k_qd_stage(QdStageArgs a) {
         }
       }
     }
+    // only the sums the norm reads (computeError, odeSolver.cpp:449-501):
+    // L2_absolute sum (lo-hi)^2; L2_relative with RK32 also sum hi^2;
+    // otherwise max |lo-hi|/hi (the others stay zero)
+    const bool needB = cs->normType != 0 && cs->ord == 3, needM = cs->normType != 0 && cs->ord != 3;
 #pragma unroll
     for (int f = 0; f < 2; ++f) {
-      const double x = wave_sum(ea[f]), y = wave_sum(eb[f]), z = wave_max(em[f]);
+      const double x = wave_sum(ea[f]);
+      const double y = needB ? wave_sum(eb[f]) : 0.0;
+      const double z = needM ? wave_max(em[f]) : 0.0;
       if (lane == 0) {
         eS[f][0][wid] = x;
         eS[f][1][wid] = y;
