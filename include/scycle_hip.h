@@ -317,6 +317,10 @@ typedef struct sc_qd_counters {
   long long deferred_checks, missed_guesses;
   /* accepted steps whose step control ran on the device (sc_qd_run.stepControl) */
   long long device_steps;
+  /* captures of the device loop's attempt graph (it is kept across
+   * sc_qd_integrate calls and captured again only when its launches'
+   * arguments change, e.g. after the solution basis grew) */
+  long long graph_builds;
 } sc_qd_counters;
 
 /* StrikeSlip_LinearElastic_qd (strikeSlip_linearElastic_qd.cpp:7-78) with

@@ -117,7 +117,8 @@ class QdCounters(C.Structure):
                 ("cg_iters", C.c_longlong), ("outer_sweeps", C.c_longlong),
                 ("rejected_steps", C.c_longlong), ("solve_seconds", C.c_double),
                 ("total_seconds", C.c_double), ("deferred_checks", C.c_longlong),
-                ("missed_guesses", C.c_longlong), ("device_steps", C.c_longlong)]
+                ("missed_guesses", C.c_longlong), ("device_steps", C.c_longlong),
+                ("graph_builds", C.c_longlong)]
 
 
 class AdjDesc(C.Structure):

@@ -31,7 +31,7 @@ if [[ $fail -ne 0 ]]; then
   echo "build failed" >&2
   exit 1
 fi
-"$HIPCC" -shared --offload-arch="$ARCH" -fPIC -o "$out.tmp" "${objs[@]}" -L/opt/rocm/lib -lrocblas -Wl,-rpath,/opt/rocm/lib
+"$HIPCC" -shared --offload-arch="$ARCH" -fPIC -o "$out.tmp" "${objs[@]}" -L/opt/rocm/lib -lrocblas -lrocsolver -Wl,-rpath,/opt/rocm/lib
 # every symbol of the library's own namespace must resolve inside it
 if nm -D --undefined-only "$out.tmp" | grep -q '_ZN2sc'; then
   nm -D --undefined-only "$out.tmp" | grep '_ZN2sc' >&2

@@ -1,7 +1,8 @@
 // Fast-diagonalisation preconditioner: set-up (1-D operators, generalised
-// eigen-decompositions) and application (four GEMMs, rocBLAS).
+// eigen-decompositions with rocSOLVER) and application (four GEMMs, rocBLAS).
 #include <hip/hip_bf16.h>
 #include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
@@ -62,10 +63,53 @@ std::vector<double> dense_line(const Dir1D& d, const std::vector<double>& c,
   return M;
 }
 
-// Generalised symmetric eigenproblem L V = D V Lambda with V^T D V = I.
-// Returns the column-major V (n x n) in vcm and eigenvalues in w.
-int gen_eig(int n, std::vector<double> L, const std::vector<double>& D, std::vector<double>& w,
-            std::vector<double>& vcm, double* asym) {
+// Symmetric eigen-decomposition S = Q diag(w) Q^T on the device with
+// rocSOLVER's divide-and-conquer driver (dsyevd), once per operator and BC
+// set.  On return w is ascending and vt[i*n + k] is component k of
+// eigenvector i (rocSOLVER's column-major Q read row-wise).
+int sym_eig(rocblas_handle hb, hipStream_t s, int n, const std::vector<double>& S,
+            std::vector<double>& w, std::vector<double>& vt) {
+  const size_t nn = (size_t)n * n;
+  double *a_d = nullptr, *d_d = nullptr, *e_d = nullptr;
+  rocblas_int* info_d = nullptr;
+  int rc = SC_OK;
+  rocblas_int info = 0;
+  rocblas_set_stream(hb, s);
+  if (hipMalloc(&a_d, nn * sizeof(double)) != hipSuccess ||
+      hipMalloc(&d_d, (size_t)n * sizeof(double)) != hipSuccess ||
+      hipMalloc(&e_d, (size_t)n * sizeof(double)) != hipSuccess ||
+      hipMalloc(&info_d, sizeof(rocblas_int)) != hipSuccess) {
+    rc = fail(SC_ERR_HIP, "eigensolver workspace allocation failed");
+  } else if (hipMemcpyAsync(a_d, S.data(), nn * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess) {
+    rc = fail(SC_ERR_HIP, "eigensolver upload failed");
+  } else {
+    const rocblas_status st = rocsolver_dsyevd(hb, rocblas_evect_original, rocblas_fill_upper, n, a_d,
+                                               n, d_d, e_d, info_d);
+    if (st != rocblas_status_success) {
+      rc = fail(SC_ERR_HIP, std::string("rocsolver_dsyevd -> ") + rocblas_status_to_string(st));
+    } else {
+      w.resize(n);
+      vt.resize(nn);
+      if (hipMemcpyAsync(w.data(), d_d, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipMemcpyAsync(vt.data(), a_d, nn * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipMemcpyAsync(&info, info_d, sizeof(info), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        rc = fail(SC_ERR_HIP, "eigensolver read-back failed");
+      else if (info != 0)
+        rc = fail(SC_ERR_NOCONV, "rocsolver_dsyevd did not converge (info " + std::to_string(info) + ")");
+    }
+  }
+  for (void* p : {(void*)a_d, (void*)d_d, (void*)e_d, (void*)info_d})
+    if (p) (void)hipFree(p);
+  return rc;
+}
+
+// Generalised symmetric eigenproblem L V = D V Lambda with V^T D V = I
+// (D diagonal, positive), through S = D^-1/2 L D^-1/2 = Q Lambda Q^T and
+// V = D^-1/2 Q.  Returns the column-major V (n x n) in vcm and eigenvalues in w.
+int gen_eig(rocblas_handle hb, hipStream_t s, int n, std::vector<double> L,
+            const std::vector<double>& D, std::vector<double>& w, std::vector<double>& vcm,
+            double* asym) {
   double amax = 0, dmax = 0;
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) {
@@ -80,7 +124,7 @@ int gen_eig(int n, std::vector<double> L, const std::vector<double>& D, std::vec
     for (int j = 0; j < n; ++j)
       S[(size_t)i * n + j] = 0.5 * (L[(size_t)i * n + j] + L[(size_t)j * n + i]) * dis[i] * dis[j];
   std::vector<double> vt;
-  if (sym_eig(n, std::move(S), w, vt)) return 1;
+  if (int e = sym_eig(hb, s, n, S, w, vt)) return e;
   // V = D^{-1/2} Q; column i of V is row i of vt scaled
   vcm.resize((size_t)n * n);
   for (int i = 0; i < n; ++i)
@@ -159,10 +203,15 @@ int fdiag_build(Ctx& c, FdDiag& f) {
   // decompose -L to work with positive eigenvalues and flip the sign back.
   for (auto& x : Ly) x = -x;
   for (auto& x : Lz) x = -x;
+  if (!f.blas) {
+    rocblas_handle hb;
+    if (rocblas_create_handle(&hb) != rocblas_status_success) return fail(SC_ERR_HIP, "rocblas_create_handle failed");
+    f.blas = hb;
+  }
   std::vector<double> V, W;
   double a1 = 0, a2 = 0;
-  if (gen_eig(Ny, std::move(Ly), Dy, f.lam, V, &a1)) return fail(SC_ERR_NOCONV, "eigensolver (y) failed");
-  if (gen_eig(Nz, std::move(Lz), Dz, f.theta, W, &a2)) return fail(SC_ERR_NOCONV, "eigensolver (z) failed");
+  if (int e = gen_eig((rocblas_handle)f.blas, c.stream, Ny, std::move(Ly), Dy, f.lam, V, &a1)) return e;
+  if (int e = gen_eig((rocblas_handle)f.blas, c.stream, Nz, std::move(Lz), Dz, f.theta, W, &a2)) return e;
   f.maxAsym = std::max(a1, a2);
   std::vector<double> inv(N);
   const double lmax = std::max(std::fabs(f.lam.back()), std::fabs(f.theta.back()));
@@ -220,11 +269,6 @@ int fdiag_build(Ctx& c, FdDiag& f) {
     SC_HIP(hipMalloc(&f.t32, N * sizeof(float)));
     SC_HIP(hipMalloc(&f.rb, N * sizeof(__hip_bfloat16)));
     SC_HIP(hipMalloc(&f.gb, N * sizeof(__hip_bfloat16)));
-  }
-  if (!f.blas) {
-    rocblas_handle hb;
-    if (rocblas_create_handle(&hb) != rocblas_status_success) return fail(SC_ERR_HIP, "rocblas_create_handle failed");
-    f.blas = hb;
   }
   rocblas_set_stream((rocblas_handle)f.blas, c.stream);
   f.bcKey = c.d.bcR | c.d.bcT << 1 | c.d.bcL << 2 | c.d.bcB << 3;

@@ -12,8 +12,6 @@
 
 namespace sc {
 
-int sym_eig(int n, std::vector<double> a, std::vector<double>& w, std::vector<double>& vt);
-
 enum FdPrec { FD_F64 = 0, FD_F32 = 1, FD_BF16 = 2 };
 
 struct FdDiag {

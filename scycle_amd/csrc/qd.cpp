@@ -41,9 +41,18 @@ QdDriver::~QdDriver() {
     if (ctl_d) (void)hipFree(ctl_d);
     if (ctl_h) (void)hipHostFree(ctl_h);
     if (ctlDts_d) (void)hipFree(ctlDts_d);
+    drop_stage_graph();
     stage_bufs_free(stageBufs);
     sc_ctx_destroy(ctx);
   }
+}
+
+void QdDriver::drop_stage_graph() {
+  if (stageExec) (void)hipGraphExecDestroy(stageExec);
+  if (stageGraph) (void)hipGraphDestroy(stageGraph);
+  stageExec = nullptr;
+  stageGraph = nullptr;
+  stageKey.clear();
 }
 
 double* QdDriver::dalloc(size_t n, const double* init_h) {
@@ -479,6 +488,7 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
   selSlot[0] = (errMask & 1) ? nsSel++ : -1;
   selSlot[1] = (errMask & 2) ? nsSel++ : -1;
   int parity = 0;
+  std::vector<unsigned char>* rec = nullptr;  // set: slot() records its launches' arguments
   auto slot = [&]() -> int {
     if (oneKernel) {
       const int p = parity;
@@ -486,8 +496,8 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
       const StepGate gp{ctl_d + p, 1};
       const PendingFinish prev = finPending ? fin : PendingFinish{};
       const VarsLinComb A1 = at_parity(L1, p), A2 = at_parity(L2, p);
-      int e = launch_qd_stage(ctx->c, stageBufs, 0, stage_in(&A1, 1, gp, 0.5, prev), s);
-      if (!e) e = launch_qd_stage(ctx->c, stageBufs, 1, stage_in(&A2, 2, gp, 1.0, stage_finish(stageBufs, 0, miss_d)), s);
+      int e = launch_qd_stage(ctx->c, stageBufs, 0, stage_in(&A1, 1, gp, 0.5, prev), s, rec);
+      if (!e) e = launch_qd_stage(ctx->c, stageBufs, 1, stage_in(&A2, 2, gp, 1.0, stage_finish(stageBufs, 0, miss_d)), s, rec);
       if (!e) {
         QdStageIn in = stage_in(nullptr, 0, gp, 0.0, PendingFinish{});
         in.fuse = QdFuse{fault_args(varB[1 - p][0], dvarB[1 - p][0]), dvarB[1 - p][1]};
@@ -512,7 +522,7 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
         in.cdhi[0] = 6;
         in.cdhi[1] = 3;
         in.cdhi[2] = 6;
-        e = launch_qd_stage(ctx->c, stageBufs, 2, in, s);
+        e = launch_qd_stage(ctx->c, stageBufs, 2, in, s, rec);
       }
       // the rates' residual test is left for the next launch; u for proj_materialize
       fin = stage_finish(stageBufs, 2, miss_d);
@@ -532,17 +542,47 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
   // with the same arguments (all that changes lives in the control block),
   // so a captured run of kAttemptsPerGraph attempts is replayed as a graph
   // (attempts past the run's end find the gate closed).  The capture holds
-  // no memset nodes.
+  // no memset nodes.  The one-kernel loop's graph is kept across calls and
+  // captured again only when the arguments of its launches change.
   int e = slot();
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  if (!e) {
+  auto capture = [&](hipGraph_t* gr, hipGraphExec_t* ex) -> int {
     SC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    int ec = 0;
+    for (int a = 0; a < kAttemptsPerGraph && !ec; ++a) ec = slot();
+    hipError_t ce = hipStreamEndCapture(s, gr);
+    if (!ec && ce != hipSuccess) ec = fail(SC_ERR_HIP, std::string("stream capture: ") + hipGetErrorString(ce));
+    if (!ec && hipGraphInstantiate(ex, *gr, nullptr, nullptr, 0) != hipSuccess)
+      ec = fail(SC_ERR_HIP, "graph instantiation failed");
+    return ec;
+  };
+  hipGraphExec_t run = nullptr;
+  if (!e && oneKernel) {
+    const int parity0 = parity;
+    const PendingFinish fin0 = fin;
+    const bool finPending0 = finPending, uLazy0 = uLazy;
+    std::vector<unsigned char> key;
+    rec = &key;
     for (int a = 0; a < kAttemptsPerGraph && !e; ++a) e = slot();
-    hipError_t ce = hipStreamEndCapture(s, &graph);
-    if (!e && ce != hipSuccess) e = fail(SC_ERR_HIP, std::string("stream capture: ") + hipGetErrorString(ce));
-    if (!e && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
-      e = fail(SC_ERR_HIP, "graph instantiation failed");
+    rec = nullptr;
+    if (!e && (!stageExec || key != stageKey)) {
+      drop_stage_graph();
+      parity = parity0;
+      fin = fin0;
+      finPending = finPending0;
+      uLazy = uLazy0;
+      e = capture(&stageGraph, &stageExec);
+      if (e) drop_stage_graph();
+      else {
+        stageKey.swap(key);
+        ++graphBuilds;
+      }
+    }
+    run = stageExec;
+  } else if (!e) {
+    e = capture(&graph, &exec);
+    run = exec;
   }
   struct GraphGuard {
     hipGraph_t& g;
@@ -559,7 +599,7 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
     const int slots = std::max(1, std::min(kSlotsPerBatch, left + 1)) - (first ? 1 : 0);
     first = false;
     for (int k = 0; k < (slots + kAttemptsPerGraph - 1) / kAttemptsPerGraph; ++k) {
-      if (hipGraphLaunch(exec, s) != hipSuccess) {
+      if (hipGraphLaunch(run, s) != hipSuccess) {
         e = fail(SC_ERR_HIP, "graph launch failed");
         break;
       }
@@ -1143,5 +1183,6 @@ extern "C" int sc_qd_integrate(sc_qd* h, const sc_qd_run* run, double* tEnd_h, i
 extern "C" int sc_qd_get_counters(sc_qd* h, sc_qd_counters* c) {
   if (!h || !c) return fail(SC_ERR_ARG, "null argument");
   *c = h->q->counters;
+  c->graph_builds = h->q->graphBuilds;
   return SC_OK;
 }

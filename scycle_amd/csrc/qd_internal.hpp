@@ -100,7 +100,10 @@ bool stage_fits(const Ctx& c);
 // the residual test a stage launch at position pos leaves
 PendingFinish stage_finish(const QdStageBufs& b, int pos, int* miss);
 int launch_stage_plan(const Ctx& c, QdStageBufs& b, const double* rfix, hipStream_t s);
-int launch_qd_stage(const Ctx& c, const QdStageBufs& b, int pos, const QdStageIn& in, hipStream_t s);
+// rec != nullptr: append the launch's arguments (kernel, grid, argument
+// bytes) to *rec instead of launching
+int launch_qd_stage(const Ctx& c, const QdStageBufs& b, int pos, const QdStageIn& in, hipStream_t s,
+                    std::vector<unsigned char>* rec = nullptr);
 // fin (optional): a pending residual test run as one more block
 int launch_vars_lohi_err(hipStream_t s, const VarsLoHi& L, const PendingFinish* fin = nullptr,
                          StepGate gate = StepGate{});
@@ -256,6 +259,15 @@ struct QdDriver {
   // attempts captured in one graph: a relaunch of a graph waits for its
   // previous run, so each launch carries several attempts
   static constexpr int kAttemptsPerGraph = 4;
+  // The instantiated graph of the one-kernel stage loop, kept across
+  // device_steps calls; stageKey holds the arguments of every launch it
+  // contains, and it is captured again only when they change (a new basis
+  // size or new basis vectors).
+  hipGraph_t stageGraph = nullptr;
+  hipGraphExec_t stageExec = nullptr;
+  std::vector<unsigned char> stageKey;
+  long long graphBuilds = 0;
+  void drop_stage_graph();
   // runs steps from the current one (r.stepCount, started) on the device
   // until the run ends (*finished) or a missed guess / fault failure halts
   // it (*halted: the host redoes the current step with checked solves)

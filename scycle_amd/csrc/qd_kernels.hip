@@ -15,6 +15,8 @@
 //                   stage's residual test, the pair, the step control
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <new>
 
 #include "cg_dev.hpp"
 #include "fault_dev.hpp"
@@ -894,13 +896,18 @@ int launch_stage_plan(const Ctx& c, QdStageBufs& b, const double* rfix, hipStrea
   return SC_OK;
 }
 
-int launch_qd_stage(const Ctx& c, const QdStageBufs& b, int pos, const QdStageIn& in, hipStream_t s) {
+int launch_qd_stage(const Ctx& c, const QdStageBufs& b, int pos, const QdStageIn& in, hipStream_t s,
+                    std::vector<unsigned char>* rec) {
   const int M = c.projCount, Nz = c.d.Nz;
   if (!stage_fits(c) || b.nz != Nz || pos < 0 || pos >= kStagePos)
     return fail(SC_ERR_ARG, "stage kernel: buffers or position");
   if (in.L.nv && (in.L.nv != 2 || in.L.nt > kStageNT || in.psiIdx < 0 || in.slipIdx < 0 || in.psiIdx == in.slipIdx))
     return fail(SC_ERR_ARG, "stage kernel: the combination must hold psi and slip with at most 2 terms");
-  QdStageArgs a{};
+  // the arguments live in zeroed storage so that their bytes (padding
+  // included) can serve as a graph key
+  alignas(QdStageArgs) unsigned char abuf[sizeof(QdStageArgs)];
+  std::memset(abuf, 0, sizeof abuf);
+  QdStageArgs& a = *new (abuf) QdStageArgs{};
   int nLow, nHigh;
   qd_band_rows(c.d.Ny, &nLow, &nHigh);
   a.Nz = Nz;
@@ -962,6 +969,12 @@ int launch_qd_stage(const Ctx& c, const QdStageBufs& b, int pos, const QdStageIn
     }
   }
   const dim3 g(a.nfb + a.nstream + (in.fin.sc && !in.end ? 1 : 0)), bl(kStageThreads);
+  if (rec) {
+    const int head[3] = {M, in.end ? 1 : 0, (int)g.x};
+    rec->insert(rec->end(), (const unsigned char*)head, (const unsigned char*)head + sizeof head);
+    rec->insert(rec->end(), abuf, abuf + sizeof abuf);
+    return SC_OK;
+  }
   switch (M) {
 #define SC_ST(m)                                                             \
   case m:                                                                    \
