@@ -523,6 +523,28 @@ int orc_rk_linear(int ord, double lam, double lam2, double dt, int n, double fin
   return e ? -1 : rk.stepCount;
 }
 
+// Adaptive RK32/RK43 run of y' = lam*y (both components alike) from y0 with
+// step dt0 to finalT, control 0 P / 1 PID: for each accepted step its size,
+// its error estimate and the step size chosen after it.  Test hook for the
+// controller's use of the error history.  Returns the accepted step count.
+int orc_rk_linear_trace(int ord, double lam, double dt0, double finalT, double tol, double y0,
+                        int control, double* dts, double* errs, double* newDts, int cap) {
+  LinOde ode;
+  ode.lam = ode.lam2 = lam;
+  RkSolver rk(ord, 1000000, finalT, dt0, control == 0 ? "P" : "PID");
+  rk.totTol = tol;
+  rk.var["y"] = {y0, y0};
+  rk.errInds = {"y"};
+  if (rk.integrate(&ode)) return -1;
+  const int n = (int)rk.dtHistory.size();
+  for (int i = 0; i < n && i < cap; ++i) {
+    dts[i] = rk.dtHistory[i];
+    errs[i] = rk.errHistory[i];
+    newDts[i] = rk.newDtHistory[i];
+  }
+  return n;
+}
+
 void orc_powerlaw(int N, const double* sxy, const double* sxz, const double* T, const double* A,
                   const double* B, const double* n, const double* mu, const double* sat,
                   double cap, double* effVisc, double* gxy_t, double* gxz_t) {

@@ -282,6 +282,8 @@ int RkSolver::integrate(OdeCtx* obj) {
     errA[1] = errA[0];
     errA[0] = totErr;
     dtHistory.push_back(deltaT);
+    errHistory.push_back(totErr);
+    newDtHistory.push_back(newDeltaT);
     obj->timeMonitor(currT, deltaT, stepCount, stop);
     if (stop > 0) break;  // odeSolver.cpp:614, 1161
     deltaT = newDeltaT;

@@ -50,6 +50,7 @@ struct RkSolver {
   std::vector<double> scaleV;
   VarMap var, dvar;
   std::vector<double> dtHistory;  // accepted step sizes
+  std::vector<double> errHistory, newDtHistory;  // their error estimates and the next step chosen
   int force_dt = 0;               // replay mode: use replayDt[stepCount-1], no control
   std::vector<double> replayDt;
   RkSolver(int order, int maxSteps, double finalT_, double dt0, const std::string& ctl)

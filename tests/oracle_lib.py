@@ -259,6 +259,22 @@ class FdFaultOracle:
         return psi_n, slip_n, uf, V, bad
 
 
+def rk_linear_trace(order, lam, dt0, final_t, tol, y0=1.0, control=1, cap=100000):
+    """Adaptive run of y' = lam*y by the oracle's RK solver; per accepted step
+    (size, error estimate, next step size chosen)."""
+    L = lib()
+    f = L.orc_rk_linear_trace
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                  _dp, _dp, _dp, C.c_int]
+    dts, errs, nd = np.zeros(cap), np.zeros(cap), np.zeros(cap)
+    n = f(order, lam, dt0, final_t, tol, y0, control, ptr(dts), ptr(errs), ptr(nd), cap)
+    if n < 0:
+        raise RuntimeError("orc_rk_linear_trace failed")
+    n = min(n, cap)
+    return dts[:n], errs[:n], nd[:n]
+
+
 def rk_linear(order, lam, dt, n, lam2=None, final_t=0.0, tol=1e-8, y0=1.0):
     """y' = lam*y integrated by the oracle's RK32 (order 3) / RK43 (order 4):
     n fixed steps of dt, or adaptive to final_t when n < 0.

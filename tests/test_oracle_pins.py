@@ -1,0 +1,180 @@
+"""Pins of oracle functions that the other oracle tests leave open.
+
+Each check compares the oracle with a value the mathematics or the
+reference's own fixtures fix, chosen so that a plausible transcription slip
+fails it:
+
+* the 4th-order variable-coefficient second derivative (spmat.cpp:494-627,
+  sbpOps_m_constGrid.cpp:1245-1256 / sbpOps_m_varGrid.cpp:1428-1448) reduces,
+  for a constant shear modulus, to the published narrow-stencil 4th-order D2
+  of Mattsson & Nordstroem (J. Comput. Phys. 199, 2004): rows 1-3 and the
+  interior of h^2 H^-1 A are the rational stencils below.  A 1% change of
+  one remainder weight (C3 or C4) moves these rows by ~1e-3;
+* Dy, Dz, muxDy, muxDz (sbpOps_m_varGrid.cpp:1602-1690) on sinh-stretched
+  grids: D1 of the coordinate itself is the metric, so Dy(y) = Dz(z) = 1 and
+  muxDy(y) = muxDz(z) = mu to rounding (a dropped or inverted metric factor
+  gives y_q or its square instead), and Dy, Dz of smooth functions converge
+  to the derivatives at the rate of the closures;
+* the PID controller's use of the error history (odeSolver.cpp:395-445,
+  610): the reference keeps the errors in a boost circular_buffer filled with
+  push_front, and its own test tests/testIO/test_circBuffer.cpp fixes that
+  after push_front(1), push_front(2) the buffer reads [2, 1]: errA[0] is the
+  latest accepted step's error, errA[1] the one before.  For y' = lam*y the
+  RK32 pair's error estimate has the closed form |y| |lam h|^3 / 3 (the two
+  stability polynomials differ by z^3/3), so the test forms every error and
+  every next step from the step sizes alone.
+"""
+from fractions import Fraction as Fr
+
+import numpy as np
+import pytest
+
+from oracle_lib import OracleSbp, rk_linear_trace
+
+ROWS4 = {
+    1: [Fr(1), Fr(-2), Fr(1)],
+    2: [Fr(-4, 43), Fr(59, 43), Fr(-110, 43), Fr(59, 43), Fr(-4, 43)],
+    3: [Fr(-1, 49), Fr(0), Fr(59, 49), Fr(-118, 49), Fr(64, 49), Fr(-4, 49)],
+}
+INTERIOR4 = [Fr(-1, 12), Fr(4, 3), Fr(-5, 2), Fr(4, 3), Fr(-1, 12)]
+
+
+def _d2_rows(direction, var_grid, n=25, m=9, L=3.0, mu0=36.0):
+    """h^2 H^-1 A / mu along one direction (laplace type y or z, all Neumann,
+    constant mu), as a dense n x n matrix taken on one line of the grid."""
+    Ny, Nz = (n, m) if direction == "y" else (m, n)
+    Ly, Lz = (L, 2.0) if direction == "y" else (2.0, L)
+    mu = np.full(Ny * Nz, mu0)
+    y = z = None
+    if var_grid:
+        y = Ly * np.repeat(np.linspace(0, 1, Ny), Nz)
+        z = Lz * np.tile(np.linspace(0, 1, Nz), Ny)
+    o = OracleSbp(4, Ny, Nz, Ly, Lz, mu, y, z, var_grid=var_grid, bc=(1, 1, 1, 1),
+                  laplace=1 if direction == "y" else 2)
+    A = o.csr().toarray()
+    Hd = o.H(np.ones(Ny * Nz))
+    if var_grid:  # H is the logical-coordinate quadrature: H^-1 A = J D2, J = yq zr
+        yq, zr = o.metric()
+        Hd = Hd * yq * zr
+    HA = A / Hd[:, None]
+    h = L / (n - 1)
+    line = m // 2
+    idx = np.arange(n) * Nz + line if direction == "y" else line * Nz + np.arange(n)
+    return HA[np.ix_(idx, idx)] * h * h / mu0
+
+
+@pytest.mark.parametrize("direction", ["y", "z"])
+@pytest.mark.parametrize("var_grid", [False, True])
+def test_order4_constant_mu_is_narrow_d2(direction, var_grid):
+    n = 25
+    D = _d2_rows(direction, var_grid, n=n)
+    for r, row in ROWS4.items():
+        want = np.zeros(n)
+        want[:len(row)] = [float(x) for x in row]
+        assert np.abs(D[r] - want).max() < 1e-12, (r, D[r, :8])
+        # the mirrored closure at the far end
+        assert np.abs(D[n - 1 - r] - want[::-1]).max() < 1e-12, (r, D[n - 1 - r, -8:])
+    for i in range(4, n - 4):
+        want = np.zeros(n)
+        want[i - 2:i + 3] = [float(x) for x in INTERIOR4]
+        assert np.abs(D[i] - want).max() < 1e-12, (i, D[i, i - 3:i + 4])
+
+
+def test_narrow_d2_rows_are_second_order_consistent():
+    """The published rows themselves: sum zero, first moment zero and second
+    moment 2 (they differentiate x^2/2 ... exactly to 2nd order), so the
+    table the test compares against is not a typo either."""
+    for r, row in ROWS4.items():
+        x = [Fr(j - r) for j in range(len(row))]
+        assert sum(row) == 0
+        assert sum(c * xi for c, xi in zip(row, x)) == 0
+        assert sum(c * xi * xi for c, xi in zip(row, x)) == 2
+    x = [Fr(j) for j in range(-2, 3)]
+    for p in range(6):  # the interior stencil is exact for degree <= 5
+        got = sum(c * xi ** p for c, xi in zip(INTERIOR4, x))
+        assert got == (2 if p == 2 else 0), (p, got)
+
+
+def _stretched(Ny, Nz, Ly, Lz, by, bz):
+    q = np.repeat(np.linspace(0, 1, Ny), Nz)
+    r = np.tile(np.linspace(0, 1, Nz), Ny)
+    y = Ly * np.sinh(by * q) / np.sinh(by)
+    z = Lz * np.sinh(bz * r) / np.sinh(bz)
+    return y, z
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_dy_dz_muxdy_muxdz_of_the_coordinates(order):
+    Ny, Nz, Ly, Lz = 31, 37, 30.0, 25.0
+    y, z = _stretched(Ny, Nz, Ly, Lz, 3.0, 5.0)
+    rng = np.random.default_rng(11)
+    mu = 30 + 6 * rng.random(Ny * Nz)
+    o = OracleSbp(order, Ny, Nz, Ly, Lz, mu, y, z, var_grid=True)
+    one = np.ones(Ny * Nz)
+    assert np.abs(o.dy(y) - one).max() < 1e-11
+    assert np.abs(o.dz(z) - one).max() < 1e-11
+    assert np.abs(o.muxdy(y) - mu).max() < 1e-11 * mu.max()
+    assert np.abs(o.muxdz(z) - mu).max() < 1e-11 * mu.max()
+    # a function of z only has no y-derivative and vice versa
+    assert np.abs(o.dy(np.sin(z / 7.0))).max() < 1e-12
+    assert np.abs(o.dz(np.cos(y / 9.0))).max() < 1e-12
+
+
+@pytest.mark.parametrize("order,rate", [(2, 1.35), (4, 2.0)])
+def test_dy_dz_converge_on_stretched_grids(order, rate):
+    """H-norm error of Dy sin(y/5), Dz cos(z/4) on sinh grids falls at the
+    rate the closures allow: order 2 has a first-order boundary row
+    (h^1.5 in the H norm, measured 1.44-1.74), order 4 second-order closure
+    rows (h^2.5, measured 2.14-2.61); a wrong metric gives rate 0."""
+    errs = []
+    for n in (41, 81):
+        y, z = _stretched(n, n + 2, 30.0, 25.0, 2.0, 4.0)
+        mu = np.full(n * (n + 2), 33.0)
+        o = OracleSbp(order, n, n + 2, 30.0, 25.0, mu, y, z, var_grid=True)
+        w = o.H(np.ones(n * (n + 2)))
+        ey = o.dy(np.sin(y / 5.0)) - np.cos(y / 5.0) / 5.0
+        ez = o.dz(np.cos(z / 4.0)) + np.sin(z / 4.0) / 4.0
+        errs.append((np.sqrt(w @ (ey * ey) / w.sum()), np.sqrt(w @ (ez * ez) / w.sum())))
+    for d in range(2):
+        assert np.log2(errs[0][d] / errs[1][d]) > rate, errs
+
+
+@pytest.mark.parametrize("control", [0, 1])
+def test_rk32_controller_error_history(control):
+    lam, tol, y0 = -1.0, 1e-6, 1.0
+    dts, errs, nd = rk_linear_trace(3, lam, 1e-3, 8.0, tol, y0=y0, control=control)
+    assert len(dts) > 50
+    # the state at each step's start from the accepted steps alone, and the
+    # error estimate in closed form (up to the cancellation of y3 - y2)
+    z = lam * dts
+    growth = 1 + z + z * z / 2 + z ** 3 / 6
+    ys = y0 * np.concatenate([[1.0], np.cumprod(growth)[:-1]])
+    e_closed = np.abs(ys) * np.abs(z) ** 3 / 3
+    assert np.all(np.abs(errs - e_closed) <= 1e-9 * e_closed + 4e-16 * np.abs(ys))
+    kappa, ord_ = 0.9, 3
+    for n in range(len(dts) - 1):
+        step = n + 1  # the reference's _stepCount while step n is taken
+        if control == 0 or step < 4:
+            ratio = kappa * (tol / errs[n]) ** (1.0 / (1 + ord_))
+        else:
+            latest, previous = errs[n - 1], errs[n - 2]  # errA[0], errA[1]
+            ratio = (kappa * (tol / errs[n]) ** (0.49 / ord_) * (latest / tol) ** (0.34 / ord_)
+                     * (tol / previous) ** (0.1 / ord_))
+        want = min(5 * dts[n], ratio * dts[n])
+        assert abs(nd[n] - want) <= 1e-12 * want, (n, nd[n], want)
+        if n + 1 < len(dts) - 1:  # the next step starts with it (no rejection here)
+            assert dts[n + 1] == nd[n]
+
+
+def test_circular_buffer_order_matters_for_this_run():
+    """The run above is not blind to the order: with errA[0] and errA[1]
+    swapped the PID steps differ by far more than the tolerance used."""
+    lam, tol = -1.0, 1e-6
+    dts, errs, nd = rk_linear_trace(3, lam, 1e-3, 8.0, tol, control=1)
+    dev = 0.0
+    for n in range(4, len(dts) - 1):
+        a = (tol / errs[n]) ** (0.49 / 3)
+        good = a * (errs[n - 1] / tol) ** (0.34 / 3) * (tol / errs[n - 2]) ** (0.1 / 3)
+        swap = a * (errs[n - 2] / tol) ** (0.34 / 3) * (tol / errs[n - 1]) ** (0.1 / 3)
+        dev = max(dev, abs(swap / good - 1))
+    assert dev > 1e-6, dev
