@@ -371,6 +371,23 @@ int orc_qd_integrate(void* h, const OrcQdRun* r, double* psi, double* slip, doub
   for (int i = 0; i < (int)rk.dtHistory.size() && i < dtsCap; ++i) dts[i] = rk.dtHistory[i];
   return rk.stepCount;
 }
+// solveSS on the model: psi in/out (in: the deck's psi, used when
+// haveStateVals), outputs slip, V, bcRShift (Nz) and the shifted u (N); any
+// output may be null.
+int orc_qd_solve_ss(void* h, double* psi, int haveStateVals, double* slip, double* V,
+                    double* bcRShift, double* u) {
+  auto* m = (QdModel*)h;
+  const int Nz = m->Nz;
+  m->psi.assign(psi, psi + Nz);
+  int e = m->solveSS(haveStateVals != 0);
+  if (e) return e;
+  std::memcpy(psi, m->psi.data(), sizeof(double) * Nz);
+  if (slip) std::memcpy(slip, m->slip.data(), sizeof(double) * Nz);
+  if (V) std::memcpy(V, m->V.data(), sizeof(double) * Nz);
+  if (bcRShift) std::memcpy(bcRShift, m->bcRShift.data(), sizeof(double) * Nz);
+  if (u) std::memcpy(u, m->u.data(), sizeof(double) * m->N);
+  return 0;
+}
 long orc_qd_counter(void* h, int which) {
   auto* m = (QdModel*)h;
   return which == 0 ? m->nSolves : (which == 1 ? m->cgIters : m->rootIts);

@@ -353,6 +353,47 @@ int QdModel::d_dt(double t, const VarMap& var, VarMap& dvar) {
   return 0;
 }
 
+int QdModel::solveSS(bool haveStateVals) {
+  // Fault::guessSS (fault.cpp:633-668): slip velocity vL, the steady state
+  // psi at vL (computePsiSS :672-709) unless the deck gave stateVals, and
+  // the steady-state friction as shear stress
+  V.assign(Nz, c.vL);
+  if (!haveStateVals)
+    for (int k = 0; k < Nz; ++k) psi[k] = c.f0 - b[k] * std::log(std::fabs(c.vL) / c.v0);
+  std::vector<double> tauP(Nz);
+  for (int k = 0; k < Nz; ++k)
+    tauP[k] = sN[k] * a[k] * std::asinh((double)0.5 * c.vL * std::exp(psi[k] / a[k]) / c.v0);
+  // the mediator builds the momentum balance with a Neumann fault when it
+  // guesses steady-state ICs (strikeSlip_linearElastic_qd.cpp:60-62); the
+  // traction is the fault datum bcL (:912-915), bcR as set up (0)
+  const int bR = sbp.d.bcR, bT = sbp.d.bcT, bL = sbp.d.bcL, bB = sbp.d.bcB;
+  if (sbp2d_change_bc(sbp, bR, bT, NEUMANN, bB)) return 40;
+  if (c.solver == 0 && chol.factor(sbp.A)) return 41;
+  bcL = tauP;
+  std::fill(u.begin(), u.end(), 0.0);
+  int e = solveMomentumBalance(0.0);
+  if (e) return e;
+  // solveSSb (:954-1006): shift u so that it has no negative values, the
+  // remote boundary's displacement becomes bcRShift and bcR, the fault's
+  // the slip (times faultTypeScale) and bcL
+  double mn = u[0];
+  for (int i = 1; i < N; ++i) mn = std::min(mn, u[i]);
+  if (mn < 0) {
+    mn = std::fabs(mn);
+    for (int i = 0; i < N; ++i) u[i] = u[i] + 1.0 * mn;
+  }
+  for (int k = 0; k < Nz; ++k) bcRShift[k] = u[(size_t)(Ny - 1) * Nz + k];
+  bcR = bcRShift;
+  for (int k = 0; k < Nz; ++k) {
+    slip[k] = u[k] * c.faultTypeScale;
+    bcL[k] = u[k];
+  }
+  // changeBCTypes back to the mediator's material types (:930)
+  if (sbp2d_change_bc(sbp, bR, bT, bL, bB)) return 42;
+  if (c.solver == 0 && chol.factor(sbp.A)) return 43;
+  return 0;
+}
+
 // ---------------------------------------------------------------- FD -----
 void fd_propagate(const Sbp2D& s, double dt, const double* u, const double* uPrev,
                   const double* rho, const double* ay, double* uNext, double* d2u) {

@@ -90,6 +90,11 @@ struct QdModel : OdeCtx {
   int init(const QdConfig& cfg, const double* mu, const double* y, const double* z);
   int solveMomentumBalance(double t);
   int d_dt(double t, const VarMap& var, VarMap& dvar) override;
+  // solveSS + solveSSb (strikeSlip_linearElastic_qd.cpp:895-1006) with
+  // Fault::guessSS (fault.cpp:633-709): psi (recomputed at vL unless
+  // haveStateVals), V = vL, slip, bcL, bcR, bcRShift and u set as the
+  // reference leaves them
+  int solveSS(bool haveStateVals);
 };
 
 // ------------------------------------------------------------- FD -----

@@ -397,8 +397,9 @@ int solve_ss(QdDriver& q, std::vector<double>& psi, bool haveStateVals, std::vec
   const sc_grid_desc g = q.desc.grid;
   int e = sc_change_bc_types(q.ctx, g.bcR, g.bcT, SC_NEUMANN, g.bcB);
   if (e) return e;
+  // the traction is the fault's datum; bcR is used as it stands (0 at set-up,
+  // linearElastic.cpp:311-312, 383; bcRShift after an earlier solveSS)
   SC_HIP(hipMemcpy(q.bcL_d, tau.data(), Nz * 8, hipMemcpyHostToDevice));
-  SC_HIP(hipMemset(q.bcR_d, 0, Nz * 8));
   e = launch_set_rhs(c, q.rhs_d, q.bcL_d, q.bcR_d, q.bcT_d, q.bcB_d, c.stream);
   if (e) return e;
   sc_solve_stats st;
@@ -412,8 +413,12 @@ int solve_ss(QdDriver& q, std::vector<double>& psi, bool haveStateVals, std::vec
     for (auto& x : u) x += mn;
   }
   SC_HIP(hipMemcpy(q.u_d, u.data(), (size_t)N * 8, hipMemcpyHostToDevice));
+  // solveSSb (:974-998): bcRShift and bcR from the remote boundary's
+  // displacement, slip and bcL from the fault's
   std::vector<double> shift(u.begin() + (size_t)(Ny - 1) * Nz, u.end());
   SC_HIP(hipMemcpy(q.shift_d, shift.data(), Nz * 8, hipMemcpyHostToDevice));
+  SC_HIP(hipMemcpy(q.bcR_d, shift.data(), Nz * 8, hipMemcpyHostToDevice));
+  SC_HIP(hipMemcpy(q.bcL_d, u.data(), Nz * 8, hipMemcpyHostToDevice));
   slip.resize(Nz);
   for (int k = 0; k < Nz; ++k) slip[k] = u[k] * q.desc.faultTypeScale;
   return sc_change_bc_types(q.ctx, g.bcR, g.bcT, g.bcL, g.bcB);

@@ -333,7 +333,7 @@ extern "C" const char* sc_last_error(void) { return sc::last_error(); }
 
 extern "C" int sc_version(int* major, int* minor) {
   if (major) *major = 0;
-  if (minor) *minor = 1;
+  if (minor) *minor = 2;
   return SC_OK;
 }
 

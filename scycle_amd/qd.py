@@ -36,6 +36,39 @@ class QdMediator:
     def _create(self, d, h):
         check(load().sc_qd_create(C.byref(d), *[dptr(x) for x in self._keep], C.byref(h)))
 
+    @classmethod
+    def from_file(cls, in_path):
+        """The mediator a reference .in deck describes (sc_qd_create_from_file:
+        the deck's keys, binary Vec overrides from its inputDir, solveSS when
+        it sets guessSteadyStateICs = 1)."""
+        self = cls.__new__(cls)
+        p = str(in_path).encode()
+        buf = C.create_string_buffer(64)
+        dims = []
+        for key in (b"Ny", b"Nz"):
+            check(load().sc_in_get(p, key, buf, 64))
+            dims.append(int(float(buf.value)))
+        self.cfg = None
+        self.Nz, self.N = dims[1], dims[0] * dims[1]
+        h = C.c_void_p()
+        check(load().sc_qd_create_from_file(p, C.byref(h)))
+        self.h = h
+        return self
+
+    def file_run(self):
+        """The deck's time-integration settings (sc_qd_file_run) as a QdRun."""
+        r = QdRun()
+        check(load().sc_qd_file_run(self.h, C.byref(r)))
+        return r
+
+    def solve_ss(self, psi, have_state_vals=False):
+        """StrikeSlip_LinearElastic_qd::solveSS: returns (psi, slip) and leaves
+        the mediator at the steady-state start (V = vL)."""
+        p = np.ascontiguousarray(psi, dtype=np.float64).copy()
+        slip = np.zeros(self.Nz)
+        check(load().sc_qd_solve_ss(self.h, dptr(p), int(bool(have_state_vals)), dptr(slip)))
+        return p, slip
+
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
             load().sc_qd_destroy(self.h)
@@ -82,9 +115,14 @@ class QdMediator:
                   init_dt, min_dt, max_dt, tol, 0 if rep is None else len(rep), dptr(rep),
                   {"auto": 0, "never": 1, "always": 2}[defer], curr_err, prev_err,
                   {"auto": 0, "host": 1}[step_control])
+        return self.integrate_run(r)
+
+    def integrate_run(self, r):
+        """sc_qd_integrate with a QdRun (e.g. file_run()); returns (tEnd,
+        steps, accepted step sizes)."""
         tEnd = C.c_double()
         steps = C.c_int()
-        dts = np.zeros(max(int(max_steps), 1))
+        dts = np.zeros(max(int(r.maxSteps), 1))
         check(load().sc_qd_integrate(self.h, C.byref(r), C.byref(tEnd), C.byref(steps),
                                      dptr(dts), len(dts)))
         return tEnd.value, steps.value, dts[:steps.value].copy()

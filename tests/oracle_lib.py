@@ -259,6 +259,21 @@ class FdFaultOracle:
         return psi_n, slip_n, uf, V, bad
 
 
+def qd_solve_ss(h, Nz, N, psi, have_state_vals=False):
+    """solveSS on an oracle QD model (orc_qd_new): psi (in: used when
+    have_state_vals), slip, V, bcRShift and the shifted u."""
+    f = lib().orc_qd_solve_ss
+    f.restype = C.c_int
+    f.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp, _dp, _dp]
+    out = dict(psi=np.array(psi, dtype=np.float64), slip=np.zeros(Nz), V=np.zeros(Nz),
+               bcRShift=np.zeros(Nz), u=np.zeros(N))
+    rc = f(h, ptr(out["psi"]), int(bool(have_state_vals)), ptr(out["slip"]), ptr(out["V"]),
+           ptr(out["bcRShift"]), ptr(out["u"]))
+    if rc:
+        raise RuntimeError(f"orc_qd_solve_ss failed ({rc})")
+    return out
+
+
 def rk_linear_trace(order, lam, dt0, final_t, tol, y0=1.0, control=1, cap=100000):
     """Adaptive run of y' = lam*y by the oracle's RK solver; per accepted step
     (size, error estimate, next step size chosen)."""

@@ -178,3 +178,67 @@ def test_circular_buffer_order_matters_for_this_run():
         swap = a * (errs[n - 2] / tol) ** (0.34 / 3) * (tol / errs[n - 1]) ** (0.1 / 3)
         dev = max(dev, abs(swap / good - 1))
     assert dev > 1e-6, dev
+
+
+def _ss_cfg(order, var_grid, N=21, uniform=False, mat_bc=(0, 1, 0, 1), remote=True):
+    from scycle_amd.configs import strike_slip
+    cfg = strike_slip(N, N, order=order, var_grid=var_grid, zstretch=3.0 if var_grid else None,
+                      layered_mu=var_grid and not uniform)
+    if uniform:
+        cfg.a = np.full(N, 0.015)
+    cfg.mat_bc = mat_bc
+    cfg.remote_loading_R = remote
+    return cfg
+
+
+@pytest.mark.parametrize("order,var_grid", [(2, False), (4, True)])
+def test_solve_ss_uniform_friction_closed_form(order, var_grid):
+    """Uniform a, b, sN, mu: the steady-state traction tau_ss is the same at
+    every depth, the Neumann-fault problem's solution is linear in y,
+    u = tau_ss/mu (y - Ly), which SBP-SAT reproduces exactly; solveSSb shifts
+    it by tau_ss Ly/mu, so u = tau_ss y/mu, the slip is 0 and bcRShift =
+    tau_ss Ly / mu (strikeSlip_linearElastic_qd.cpp:895-1006)."""
+    from oracle_lib import OracleQd, qd_solve_ss
+    cfg = _ss_cfg(order, var_grid, uniform=True)
+    o = OracleQd(cfg, jac_mode=0)
+    r = qd_solve_ss(o.h, cfg.Nz, cfg.Ny * cfg.Nz, cfg.psi0, have_state_vals=False)
+    psi_ss = 0.6 - 0.02 * np.log(1e-9 / 1e-6)
+    assert np.abs(r["psi"] - psi_ss).max() < 1e-15
+    assert np.all(r["V"] == 1e-9)
+    tau = 50 * 0.015 * np.arcsinh(0.5 * 1e-9 * np.exp(psi_ss / 0.015) / 1e-6)
+    shift = tau * cfg.Ly / 36.0
+    assert np.abs(r["bcRShift"] - shift).max() < 1e-11 * shift
+    assert np.abs(r["u"] - tau * cfg.y / 36.0).max() < 1e-11 * shift
+    assert np.abs(r["slip"]).max() < 1e-11 * shift
+
+
+@pytest.mark.parametrize("order,var_grid", [(2, False), (4, True)])
+def test_solve_ss_solves_the_neumann_fault_problem(order, var_grid):
+    """Layered mu and the depth ramp of a: u - min(u) solves the operator with
+    a Neumann fault for the datum tau_ss = sN a asinh(vL e^(psi/a) / (2 v0))
+    (guessSS, fault.cpp:633-668) and zero data elsewhere; the slip is
+    faultTypeScale u on the fault and bcRShift u on the remote boundary; with
+    stateVals the given psi is kept (fault.cpp:646-648)."""
+    from oracle_lib import OracleQd, OracleSbp, qd_solve_ss
+    cfg = _ss_cfg(order, var_grid)
+    N, Nz = cfg.Ny * cfg.Nz, cfg.Nz
+    for keep_psi in (False, True):
+        o = OracleQd(cfg, jac_mode=0)
+        r = qd_solve_ss(o.h, Nz, N, cfg.psi0, have_state_vals=keep_psi)
+        psi = cfg.psi0 if keep_psi else 0.6 - cfg.b * np.log(1e-9 / 1e-6)
+        assert np.abs(r["psi"] - psi).max() < 1e-15
+        tau = cfg.sN * cfg.a * np.arcsinh(0.5 * 1e-9 * np.exp(psi / cfg.a) / 1e-6)
+        u = r["u"]
+        assert u.min() == 0.0
+        bcR, bcT, bcL, bcB = cfg.mat_bc
+        s = OracleSbp(order, cfg.Ny, cfg.Nz, cfg.Ly, cfg.Lz, cfg.mu, cfg.y if var_grid else None,
+                      cfg.z if var_grid else None, var_grid=var_grid, bc=(bcR, bcT, 1, bcB))
+        rhs = s.set_rhs(tau, np.zeros(Nz), np.zeros(cfg.Ny), np.zeros(cfg.Ny))
+        # u before the shift is u - c with c the shift; A_N (u - c) = rhs fixes c
+        one = np.ones(N)
+        A1, Au = s.apply(one), s.apply(u)
+        c = np.dot(A1, Au - rhs) / np.dot(A1, A1)
+        resid = Au - c * A1 - rhs
+        assert np.abs(resid).max() < 1e-9 * np.abs(rhs).max(), np.abs(resid).max()
+        assert np.abs(r["slip"] - cfg.fault_type_scale * u[:Nz]).max() == 0.0
+        assert np.abs(r["bcRShift"] - u[(cfg.Ny - 1) * Nz:]).max() == 0.0
