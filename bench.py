@@ -70,6 +70,8 @@ PATHS = {
     "mixed_bf16": ("cg64", "fdiag_bf16", "f64 CG + bf16 GEMM preconditioner"),
     "fp64": ("cg64", "jacobi", "f64"),
     "ir32": ("ir32", "jacobi", "f32 CG + f64 refinement"),
+    "ir32_fdiag32": ("ir32", "fdiag32", "f32 CG (f32 GEMM preconditioner) + f64 refinement"),
+    "ir32_bf16": ("ir32", "fdiag_bf16", "f32 CG (bf16 GEMM preconditioner) + f64 refinement"),
 }
 
 

@@ -115,7 +115,7 @@ int sc_bench_apply(sc_ctx* ctx, int which, int reps, double* ms_per_launch, doub
 
 /* ---------------------------------------------------------------- solve */
 #define SC_SOLVE_CG64 0  /* FP64 Jacobi-PCG                                */
-#define SC_SOLVE_IR32 1  /* FP32 Jacobi-PCG inside FP64 iterative refinement */
+#define SC_SOLVE_IR32 1  /* FP32 PCG inside FP64 iterative refinement       */
 
 typedef struct sc_solve_opts {
   int mode;            /* SC_SOLVE_CG64 / SC_SOLVE_IR32                   */
@@ -138,7 +138,9 @@ typedef struct sc_solve_opts {
 /* Preconditioners.  JACOBI: 1/diag(A).  FDIAG*: exact inverse of the
  * separable part of A (the operator with a depth-only shear modulus and
  * without the reference's two-point mu average), applied as four dense
- * GEMMs in FP64, FP32 or bf16 (FP32 accumulation); only with SC_SOLVE_CG64. */
+ * GEMMs in FP64, FP32 or bf16 (FP32 accumulation).  SC_SOLVE_CG64 takes any
+ * of them (flexible CG); SC_SOLVE_IR32 takes JACOBI, FDIAG32 or FDIAG_BF16
+ * for its FP32 inner solves (flexible PCG in FP32, FP64 refinement). */
 #define SC_PREC_JACOBI 0
 #define SC_PREC_FDIAG64 1
 #define SC_PREC_FDIAG32 2

@@ -3,9 +3,11 @@
 //
 // SC_SOLVE_CG64: Jacobi-preconditioned CG in FP64 on the matrix-free operator.
 // SC_SOLVE_IR32: FP64 iterative refinement; each sweep solves the correction
-//                equation with Jacobi-PCG in FP32 (operator, vectors and
-//                preconditioner in FP32, dot products accumulated in FP64),
-//                the residual b - A u is always formed in FP64.
+//                equation with PCG in FP32 (operator, vectors and
+//                preconditioner in FP32, dot products accumulated in FP64):
+//                Jacobi, or the fast-diagonalisation inverse with FP32 or
+//                bf16 GEMMs (flexible PCG); the residual b - A u is always
+//                formed in FP64.
 //
 // All scalars (alpha, beta, norms, the converged flag) stay on the device;
 // every kernel of an iteration returns at once when the flag is set, so the
@@ -194,19 +196,35 @@ __global__ __launch_bounds__(256) void k_ir_correct(size_t N, const float* __res
 }
 
 // x += alpha p ; r -= alpha q ; rr = r.r   (preconditioner applied separately)
-__global__ __launch_bounds__(kRedThreads) void k_cg_update_r(size_t N, double* __restrict__ x,
-                                                             double* __restrict__ r,
-                                                             const double* __restrict__ p,
-                                                             const double* __restrict__ q,
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_cg_update_r(size_t N, T* __restrict__ x,
+                                                             T* __restrict__ r,
+                                                             const T* __restrict__ p,
+                                                             const T* __restrict__ q,
                                                              CgScal* sc, double* partials) {
   if (sc->done) return;
-  const double alpha = sc->alpha;
+  const T alpha = (T)sc->alpha;
   double v[1] = {0};
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
     x[i] += alpha * p[i];
-    const double ri = r[i] - alpha * q[i];
+    const T ri = r[i] - alpha * q[i];
     r[i] = ri;
-    v[0] += ri * ri;
+    v[0] += (double)ri * (double)ri;
+  }
+  block_partials(v, partials);
+}
+
+// FP32 start of an inner solve from x = 0: r = b ; rr = r.r ; bb = b.b
+__global__ __launch_bounds__(kRedThreads) void k_cg32_start(size_t N, const float* __restrict__ b,
+                                                            float* __restrict__ x, float* __restrict__ r,
+                                                            double* partials) {
+  double v[2] = {0, 0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+    const float bi = b[i];
+    x[i] = 0.0f;
+    r[i] = bi;
+    v[0] += (double)bi * (double)bi;
+    v[1] += (double)bi * (double)bi;
   }
   block_partials(v, partials);
 }
@@ -215,25 +233,27 @@ __global__ __launch_bounds__(kRedThreads) void k_cg_update_r(size_t N, double* _
 // approximately symmetric (FP32 / bf16 GEMMs):
 //   beta = r.(z - z_old) / rz_old ; rz = r.z ; z_old <- z
 // (first call: beta = 0, p = z)
-__global__ __launch_bounds__(kRedThreads) void k_cg_rz(size_t N, const double* __restrict__ r,
-                                                       const double* __restrict__ z,
-                                                       double* __restrict__ zold, int first,
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_cg_rz(size_t N, const T* __restrict__ r,
+                                                       const T* __restrict__ z,
+                                                       T* __restrict__ zold, int first,
                                                        CgScal* sc, double* partials) {
   if (sc->done) return;
   double v[2] = {0, 0};
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
     const double ri = r[i], zi = z[i];
     v[0] += ri * zi;
-    if (!first) v[1] += ri * zold[i];
-    zold[i] = zi;
+    if (!first) v[1] += ri * (double)zold[i];
+    zold[i] = z[i];
   }
   block_partials(v, partials);
 }
 
-__global__ __launch_bounds__(256) void k_cg_pz(size_t N, const double* __restrict__ z,
-                                               double* __restrict__ p, const CgScal* sc, int first) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_cg_pz(size_t N, const T* __restrict__ z,
+                                               T* __restrict__ p, const CgScal* sc, int first) {
   if (sc->done) return;
-  const double beta = sc->beta;
+  const T beta = (T)sc->beta;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x)
     p[i] = first ? z[i] : z[i] + beta * p[i];
 }
@@ -665,21 +685,71 @@ int pcg_fdiag(Ctx& c, const double* b, double* x, int fprec, double rtol, int ma
   while (!h.done) {
     e = fdiag_apply(c, *c.fd, fprec, r, z);
     if (e) return e;
-    hipLaunchKernelGGL(k_cg_rz, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, r, z, c.zold64,
+    hipLaunchKernelGGL(k_cg_rz<double>, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, r, z, c.zold64,
                        first, sc, c.partials);
     finish<FIN_RZ, 2>(s, sc, c, N, first);
-    hipLaunchKernelGGL(k_cg_pz, dim3(ew_blocks(N)), dim3(256), 0, s, N, z, p, sc, first);
+    hipLaunchKernelGGL(k_cg_pz<double>, dim3(ew_blocks(N)), dim3(256), 0, s, N, z, p, sc, first);
     first = 0;
     e = launch_apply64(c, p, q, s);
     if (e) return e;
     hipLaunchKernelGGL(k_cg_dot_pq<double>, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, p, q,
                        sc, c.partials);
     finish<FIN_PQ, 1>(s, sc, c, N);
-    hipLaunchKernelGGL(k_cg_update_r, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, x, r, p, q,
+    hipLaunchKernelGGL(k_cg_update_r<double>, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, x, r, p, q,
                        sc, c.partials);
     finish<FIN_UPDATE_R, 1>(s, sc, c, N);
     SC_HIP(hipGetLastError());
     e = read_scal(c, &h, 0);
+    if (e) return e;
+  }
+  *iters_out = h.iters;
+  *relres_out = h.bb > 0 ? std::sqrt(h.rr / h.bb) : 0.0;
+  *status_out = h.done;
+  return SC_OK;
+}
+
+// The inner solve of iterative refinement with the fast-diagonalisation
+// preconditioner: flexible PCG in FP32 on A x = b from x = 0, the
+// preconditioner's GEMMs in FP32 or bf16, dot products accumulated in FP64,
+// scalars in slot 1.  Vectors: r, p, q, z, zold (FP32, N each).  The host
+// reads the scalars after every `batch` iterations; the vector kernels of
+// an iteration past convergence return at once.
+int pcg32_fdiag(Ctx& c, const float* b, float* x, float* r, float* p, float* q, float* z, float* zold,
+                int fprec, double rtol, int maxit, int batch, int* iters_out, double* relres_out,
+                int* status_out) {
+  const size_t N = (size_t)c.N;
+  hipStream_t s = c.stream;
+  CgScal* sc = (CgScal*)c.scal + 1;
+  hipLaunchKernelGGL(k_set_scal<float>, dim3(1), dim3(1), 0, s, sc, rtol * rtol, maxit);
+  hipLaunchKernelGGL(k_cg32_start, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, b, x, r, c.partials);
+  finish<FIN_IRRES, 2>(s, sc, c, N);
+  SC_HIP(hipGetLastError());
+  CgScal h;
+  int e = read_scal(c, &h, 1);
+  if (e) return e;
+  int first = 1, launched = 0;
+  if (batch < 1) batch = 1;
+  while (!h.done && launched < maxit) {
+    for (int i = 0; i < batch; ++i) {
+      e = fdiag_apply_f32(c, *c.fd, fprec, r, z);
+      if (e) return e;
+      hipLaunchKernelGGL(k_cg_rz<float>, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, r, z, zold, first,
+                         sc, c.partials);
+      finish<FIN_RZ, 2>(s, sc, c, N, first);
+      hipLaunchKernelGGL(k_cg_pz<float>, dim3(ew_blocks(N)), dim3(256), 0, s, N, z, p, sc, first);
+      first = 0;
+      e = launch_apply32(c, p, q, s);
+      if (e) return e;
+      hipLaunchKernelGGL(k_cg_dot_pq<float>, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, p, q, sc,
+                         c.partials);
+      finish<FIN_PQ, 1>(s, sc, c, N);
+      hipLaunchKernelGGL(k_cg_update_r<float>, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, x, r, p, q,
+                         sc, c.partials);
+      finish<FIN_UPDATE_R, 1>(s, sc, c, N);
+    }
+    SC_HIP(hipGetLastError());
+    launched += batch;
+    e = read_scal(c, &h, 1);
     if (e) return e;
   }
   *iters_out = h.iters;
@@ -1018,13 +1088,69 @@ int solve(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve
   return proj_add_solution(c, u);
 }
 
+// FP64 iterative refinement whose sweeps solve the correction equation with
+// the FP32 fast-diagonalisation PCG (pcg32_fdiag): the residual b - A u is
+// formed in FP64 with the FP64 apply, scaled to unit norm and rounded to
+// FP32; the FP32 correction is added in FP64.  FP32 vectors of the inner
+// solve: r32 (its rhs), d32 (its solution), q32, p32 and three more in the
+// FP64 buffers p64, q64 (free while the inner solve runs).
+int solve_ir_fdiag(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, int fprec,
+                   sc_solve_stats& st) {
+  const size_t N = (size_t)c.N;
+  hipStream_t s = c.stream;
+  CgScal* sc0 = (CgScal*)c.scal;
+  const double inner = o.inner_rtol > 0 ? o.inner_rtol : 1e-5;
+  const int maxOuter = o.max_outer > 0 ? o.max_outer : 30;
+  const int maxit = o.max_iters > 0 ? o.max_iters : 10000;
+  float* zin = (float*)c.p64;
+  float* zold = (float*)c.p64 + N;
+  float* qin = (float*)c.q64;
+  int total = 0;
+  for (int outer = 0;; ++outer) {
+    hipLaunchKernelGGL(k_set_scal<double>, dim3(1), dim3(1), 0, s, sc0, o.rtol * o.rtol, maxit);
+    int e = launch_apply64(c, u, c.q64, s);
+    if (e) return e;
+    hipLaunchKernelGGL(k_ir_residual, dim3(red_blocks(N)), dim3(kRedThreads), 0, s, N, rhs, c.q64,
+                       c.r64, c.partials);
+    finish<FIN_IRRES, 2>(s, sc0, c, N);
+    SC_HIP(hipGetLastError());
+    CgScal h;
+    e = read_scal(c, &h, 0);
+    if (e) return e;
+    st.relres = h.bb > 0 ? std::sqrt(h.rr / h.bb) : 0.0;
+    st.outer = outer;
+    if (h.done) {
+      st.converged = 1;
+      break;
+    }
+    if (outer >= maxOuter || total >= maxit) {
+      st.iters = total;
+      return fail(SC_ERR_NOCONV, "IR32 (fast-diagonalisation) did not converge (relres " +
+                                     std::to_string(st.relres) + ")");
+    }
+    hipLaunchKernelGGL(k_ir_to32, dim3(ew_blocks(N)), dim3(256), 0, s, N, c.r64, sc0, c.r32, c.d32);
+    int it = 0, status = 0;
+    double rr = 0;
+    e = pcg32_fdiag(c, c.r32, c.d32, c.q32, c.p32, qin, zin, zold, fprec, inner, maxit - total, 2, &it, &rr,
+                    &status);
+    if (e) return e;
+    total += it;
+    if (status == 2 || status == 4) return fail(SC_ERR_NONFINITE, "IR32 inner CG breakdown");
+    hipLaunchKernelGGL(k_ir_correct, dim3(ew_blocks(N)), dim3(256), 0, s, N, c.d32, sc0, u);
+    SC_HIP(hipGetLastError());
+  }
+  st.iters = total;
+  return SC_OK;
+}
+
 int solve_core(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st) {
   std::memset(&st, 0, sizeof(st));
   const bool ir = o.mode == SC_SOLVE_IR32;
   int e = ensure_workspace(c, ir);
   if (e) return e;
   if (o.precond != SC_PREC_JACOBI) {
-    if (ir) return fail(SC_ERR_ARG, "the fast-diagonalisation preconditioner runs inside SC_SOLVE_CG64");
+    if (ir && o.precond == SC_PREC_FDIAG64)
+      return fail(SC_ERR_ARG, "SC_SOLVE_IR32 takes the FP32 or bf16 fast-diagonalisation preconditioner");
     const int key = c.d.bcR | c.d.bcT << 1 | c.d.bcL << 2 | c.d.bcB << 3;
     if (!c.fd) c.fd = new FdDiag;
     if (c.fd->bcKey != key || c.fd_stale) {
@@ -1035,6 +1161,7 @@ int solve_core(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_
     if (!c.z64) SC_HIP(hipMalloc(&c.z64, (size_t)c.N * sizeof(double)));
     if (!c.zold64) SC_HIP(hipMalloc(&c.zold64, (size_t)c.N * sizeof(double)));
     const int fprec = o.precond == SC_PREC_FDIAG64 ? FD_F64 : (o.precond == SC_PREC_FDIAG32 ? FD_F32 : FD_BF16);
+    if (ir) return solve_ir_fdiag(c, rhs, u, o, fprec, st);
     int it = 0, status = 0;
     double rr = 0;
     e = pcg_fdiag(c, rhs, u, fprec, o.rtol, o.max_iters > 0 ? o.max_iters : 10000, &it, &rr, &status);

@@ -330,4 +330,41 @@ int fdiag_apply(Ctx& c, FdDiag& f, int prec, const double* r, double* z) {
   return SC_OK;
 }
 
+int fdiag_apply_f32(Ctx& c, FdDiag& f, int prec, const float* r, float* z) {
+  const int Ny = f.Ny, Nz = f.Nz;
+  const size_t N = (size_t)Ny * Nz;
+  hipStream_t s = c.stream;
+  rocblas_handle hb = (rocblas_handle)f.blas;
+  rocblas_set_stream(hb, s);
+  const rocblas_operation nN = rocblas_operation_none;
+  const float one = 1.0f, zero = 0.0f;
+  if (prec == FD_F32) {
+    SC_BLAS(rocblas_sgemm(hb, nN, nN, Nz, Ny, Nz, &one, f.WT32, Nz, r, Nz, &zero, f.g32, Nz));
+    SC_BLAS(rocblas_sgemm(hb, nN, nN, Nz, Ny, Ny, &one, f.g32, Nz, f.V32, Ny, &zero, f.t32, Nz));
+    hipLaunchKernelGGL(k_scale<float>, dim3(nblk(N)), dim3(256), 0, s, N, f.t32, f.inv32);
+    SC_BLAS(rocblas_sgemm(hb, nN, nN, Nz, Ny, Nz, &one, f.W32, Nz, f.t32, Nz, &zero, f.g32, Nz));
+    SC_BLAS(rocblas_sgemm(hb, nN, nN, Nz, Ny, Ny, &one, f.g32, Nz, f.VT32, Ny, &zero, z, Nz));
+  } else if (prec == FD_BF16) {
+    const rocblas_datatype bf = rocblas_datatype_bf16_r, f32 = rocblas_datatype_f32_r;
+    __hip_bfloat16* rb = (__hip_bfloat16*)f.rb;
+    __hip_bfloat16* gb = (__hip_bfloat16*)f.gb;
+    hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk(N)), dim3(256), 0, s, N, r, (const float*)nullptr, rb);
+    SC_BLAS(rocblas_gemm_ex(hb, nN, nN, Nz, Ny, Nz, &one, f.WTb, bf, Nz, rb, bf, Nz, &zero, f.g32, f32,
+                            Nz, f.g32, f32, Nz, f32, rocblas_gemm_algo_standard, 0, 0));
+    hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk(N)), dim3(256), 0, s, N, f.g32, (const float*)nullptr, gb);
+    SC_BLAS(rocblas_gemm_ex(hb, nN, nN, Nz, Ny, Ny, &one, gb, bf, Nz, f.Vb, bf, Ny, &zero, f.t32, f32,
+                            Nz, f.t32, f32, Nz, f32, rocblas_gemm_algo_standard, 0, 0));
+    hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk(N)), dim3(256), 0, s, N, f.t32, (const float*)f.inv32, rb);
+    SC_BLAS(rocblas_gemm_ex(hb, nN, nN, Nz, Ny, Nz, &one, f.Wb, bf, Nz, rb, bf, Nz, &zero, f.g32, f32,
+                            Nz, f.g32, f32, Nz, f32, rocblas_gemm_algo_standard, 0, 0));
+    hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk(N)), dim3(256), 0, s, N, f.g32, (const float*)nullptr, gb);
+    SC_BLAS(rocblas_gemm_ex(hb, nN, nN, Nz, Ny, Ny, &one, gb, bf, Nz, f.VTb, bf, Ny, &zero, z, f32,
+                            Nz, z, f32, Nz, f32, rocblas_gemm_algo_standard, 0, 0));
+  } else {
+    return fail(SC_ERR_ARG, "the FP32 preconditioner application takes FP32 or bf16 GEMMs");
+  }
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
 }  // namespace sc

@@ -35,5 +35,8 @@ struct FdDiag {
 int fdiag_build(Ctx& c, FdDiag& f);
 // z = A_sep^-1 r (FP64 in/out) with the GEMMs in the given precision.
 int fdiag_apply(Ctx& c, FdDiag& f, int prec, const double* r, double* z);
+// z = A_sep^-1 r with FP32 vectors (the inner solve of iterative
+// refinement): prec FD_F32 (SGEMMs) or FD_BF16 (bf16 GEMMs, FP32 accumulation).
+int fdiag_apply_f32(Ctx& c, FdDiag& f, int prec, const float* r, float* z);
 
 }  // namespace sc

@@ -87,8 +87,59 @@ def test_solve_full_size_residual():
     assert np.abs(u1 - u2).max() <= 1e-7 * np.abs(u1).max()
 
 
+@pytest.mark.parametrize("precond", ["fdiag32", "fdiag_bf16"])
+@pytest.mark.parametrize("layered", [True, False])
+def test_ir32_fdiag_solve(precond, layered):
+    """FP32 flexible PCG with the fast-diagonalisation preconditioner (FP32 or
+    bf16 GEMMs) inside FP64 iterative refinement: the FP64 residual meets
+    1e-12 and the solution matches the direct solve to 1e-9."""
+    from scycle_amd import SbpOps
+    Ny, Nz = 81, 73
+    y, z = _grid(Ny, Nz, 30.0, 30.0, 5.0)
+    rng = np.random.default_rng(5)
+    if layered:
+        t = np.clip(z / 4.0, 0, 1)
+        mu = np.where(z >= 4.0, 36.0, 30.0 + 6.0 * (3 * t ** 2 - 2 * t ** 3))
+    else:
+        mu = 30 + 6 * rng.random(Ny * Nz)
+    bc = (0, 1, 0, 1)
+    o = OracleSbp(4, Ny, Nz, 30.0, 30.0, mu, y, z, var_grid=True, bc=bc)
+    g = SbpOps(4, Ny, Nz, 30.0, 30.0, mu, y, z, var_grid=True, bc=bc)
+    rhs = o.set_rhs(rng.standard_normal(Nz), np.full(Nz, 0.3), np.zeros(Ny), np.zeros(Ny))
+    uref = o.chol_solve(rhs)
+    u, st = g.solve(rhs, mode="ir32", rtol=1e-12, inner_rtol=1e-5, precond=precond, max_iters=4000)
+    print("ir32", precond, layered, st, flush=True)
+    assert st["converged"] and st["relres"] <= 1e-12
+    assert np.abs(u - uref).max() <= 1e-9 * np.abs(uref).max()
+    if layered:
+        assert st["outer"] <= 4 and st["iters"] <= (16 if precond == "fdiag32" else 40)
+    # warm start from the solution: no sweep
+    u2, st2 = g.solve(rhs, u0=u, mode="ir32", rtol=1e-12, precond=precond)
+    assert st2["iters"] == 0 and st2["outer"] == 0
+
+
+def test_ir32_fdiag_full_size():
+    """1001^2 C2 operator: IR32 with the FP32 and bf16 fast-diagonalisation
+    inner solves meets the FP64 residual 1e-12 and agrees with FP64 CG with
+    the FP64 preconditioner to the conditioning-limited 1e-8."""
+    from scycle_amd import SbpOps
+    N = 1001
+    y, z = _grid(N, N, 30.0, 30.0, 5.0)
+    t = np.clip(z / 4.0, 0, 1)
+    mu = np.where(z >= 4.0, 36.0, 30.0 + 6.0 * (3 * t ** 2 - 2 * t ** 3))
+    g = SbpOps(4, N, N, 30.0, 30.0, mu, y, z, var_grid=True, bc=(0, 1, 0, 1))
+    rng = np.random.default_rng(9)
+    rhs = g.setRhs(1e-3 * rng.random(N), np.full(N, 5e-4), np.zeros(N), np.zeros(N))
+    u0, s0 = g.solve(rhs, mode="cg64", rtol=1e-12, precond="fdiag64")
+    for precond in ("fdiag32", "fdiag_bf16"):
+        u, st = g.solve(rhs, mode="ir32", rtol=1e-12, precond=precond)
+        print("1001^2 ir32", precond, st, "cg64 fdiag64", s0, flush=True)
+        assert st["converged"] and st["relres"] <= 1e-12
+        assert np.abs(u - u0).max() <= 1e-8 * np.abs(u0).max()
+
+
 @pytest.mark.parametrize("mode,precond", [("cg64", "jacobi"), ("cg64", "fdiag64"), ("cg64", "fdiag32"),
-                                          ("ir32", "jacobi")])
+                                          ("ir32", "jacobi"), ("ir32", "fdiag32"), ("ir32", "fdiag_bf16")])
 def test_reuse_projection_guess(mode, precond):
     """Initial guess projected on the previous solutions of the same operator
     (Fischer's method): every solution still matches the direct solve, a
