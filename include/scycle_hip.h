@@ -158,6 +158,20 @@ typedef struct sc_solve_stats {
 int sc_solve(sc_ctx* ctx, const double* rhs_d, double* u_d, const sc_solve_opts* opts,
              sc_solve_stats* stats);
 
+/* sc_solve without any device allocation: FP64 Jacobi-PCG (SC_SOLVE_CG64,
+ * SC_PREC_JACOBI, reuse 0; other options give SC_ERR_ARG) from u as the
+ * initial guess, the same kernels and results as sc_solve.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its scratch (the CG vectors, reduction partials and scalars)
+ * is one static device array of 25.2 MB per GPU, held until the process
+ * ends, which limits it to grids of at most 1048576 nodes (larger grids:
+ * SC_ERR_ARG).  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU race.  The context itself (made by sc_ctx_create) is
+ * the caller's, as for sc_solve. */
+int sc_solve_noalloc(sc_ctx* ctx, const double* rhs_d, double* u_d, const sc_solve_opts* opts,
+                     sc_solve_stats* stats);
+
 /* ---------------------------------------------------------------- fault */
 #define SC_AGING_LAW 0
 #define SC_SLIP_LAW 1

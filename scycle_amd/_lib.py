@@ -162,6 +162,7 @@ SIGNATURES = {
     "sc_bench_apply": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
     "sc_qd_bench_stage": (C.c_int, [_vp, C.c_double, C.c_int, _dp, _dp]),
     "sc_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "sc_solve_noalloc": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "sc_fault_qd_rates": (C.c_int, [_vp, C.POINTER(FaultQdArgs)]),
     "sc_fault_fd_step": (C.c_int, [_vp, C.POINTER(FaultFdArgs)]),
     "sc_wave_step": (C.c_int, [_vp, C.c_int, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp]),

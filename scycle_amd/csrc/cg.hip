@@ -17,6 +17,8 @@
 // the scalars.
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "cg_dev.hpp"
 #include "fdiag.hpp"
@@ -1243,4 +1245,113 @@ extern "C" int sc_solve(sc_ctx* h, const double* rhs_d, double* u_d, const sc_so
   int e = sc::solve(h->c, rhs_d, u_d, *opts, st);
   if (stats) *stats = st;
   return e;
+}
+
+// ------------------------------------------------ allocation-free solve --
+// The scratch of sc_solve_noalloc lives in one file-scope device array: the
+// CG vectors r, p, q for grids of up to kNoAllocMaxNodes nodes, then the
+// reduction partials, then the two scalar blocks -- 25.2 MB that every GPU
+// the library runs on holds until the process ends.  In host code `scratch`
+// only names the symbol: its address on a device comes from
+// hipGetSymbolAddress, resolved once per device with that device current,
+// and that address is what the kernels receive.
+namespace sc {
+namespace {
+constexpr size_t kNoAllocMaxNodes = (size_t)1 << 20;
+constexpr size_t kNoAllocPartials = (size_t)kRedBlocks * 4;
+constexpr size_t kNoAllocScal = 32;  // doubles reserved for two CgScal blocks
+static_assert(2 * sizeof(CgScal) <= kNoAllocScal * sizeof(double), "scalar blocks exceed their slot");
+constexpr size_t kNoAllocDoubles = 3 * kNoAllocMaxNodes + kNoAllocPartials + kNoAllocScal;
+constexpr int kNoAllocDevices = 64;
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kNoAllocDevices] = {};
+
+// the scratch's address on the device of `stream`, resolved on first use
+int noalloc_scratch(hipStream_t stream, double** out) {
+  int dev = 0, cur = 0;
+  SC_HIP(hipStreamGetDevice(stream, &dev));
+  if (dev < 0 || dev >= kNoAllocDevices) return fail(SC_ERR_ARG, "sc_solve_noalloc: device id out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    SC_HIP(hipGetDevice(&cur));
+    if (cur != dev) SC_HIP(hipSetDevice(dev));
+    double* p = nullptr;
+    const hipError_t he = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+    if (cur != dev) SC_HIP(hipSetDevice(cur));
+    if (he != hipSuccess || !p)
+      return fail(SC_ERR_HIP, std::string("sc_solve_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(he));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return SC_OK;
+}
+
+// zeroes the scalar blocks (a fill kernel, not a memset, so that the solve
+// holds no memset node if a caller records it)
+__global__ void k_zero_doubles(double* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+}  // namespace
+}  // namespace sc
+
+extern "C" int sc_solve_noalloc(sc_ctx* h, const double* rhs_d, double* u_d, const sc_solve_opts* opts,
+                                sc_solve_stats* stats) {
+  using namespace sc;
+  if (!h || !rhs_d || !u_d || !opts) return fail(SC_ERR_ARG, "sc_solve_noalloc: null argument");
+  const sc_solve_opts& o = *opts;
+  if (o.mode != SC_SOLVE_CG64 || o.precond != SC_PREC_JACOBI || o.reuse != 0)
+    return fail(SC_ERR_ARG, "sc_solve_noalloc: only FP64 Jacobi-PCG without reuse (SC_SOLVE_CG64, "
+                            "SC_PREC_JACOBI, reuse 0) runs without allocating");
+  if (!(o.rtol > 0)) return fail(SC_ERR_ARG, "sc_solve_noalloc: rtol must be > 0");
+  Ctx& c = h->c;
+  const size_t N = (size_t)c.N;
+  if (N > kNoAllocMaxNodes)
+    return fail(SC_ERR_ARG, "sc_solve_noalloc: the grid has " + std::to_string(N) +
+                                " nodes, more than the allocation-free scratch holds (" +
+                                std::to_string(kNoAllocMaxNodes) + ")");
+  double* base = nullptr;
+  int e = noalloc_scratch(c.stream, &base);
+  if (e) return e;
+  // the context's kernels run on the scratch for this call; its own
+  // workspace (if any) is put back afterwards
+  struct Swap {
+    Ctx& c;
+    double *r, *p, *q, *part;
+    void *scal, *scal_h;
+    ~Swap() {
+      c.r64 = r;
+      c.p64 = p;
+      c.q64 = q;
+      c.partials = part;
+      c.scal = scal;
+      c.scal_h = scal_h;
+    }
+  } keep{c, c.r64, c.p64, c.q64, c.partials, c.scal, c.scal_h};
+  CgScal hostScal[2];
+  c.r64 = base;
+  c.p64 = base + kNoAllocMaxNodes;
+  c.q64 = base + 2 * kNoAllocMaxNodes;
+  c.partials = base + 3 * kNoAllocMaxNodes;
+  c.scal = base + 3 * kNoAllocMaxNodes + kNoAllocPartials;
+  c.scal_h = hostScal;
+  hipLaunchKernelGGL(k_zero_doubles, dim3(1), dim3(64), 0, c.stream, (double*)c.scal, (int)kNoAllocScal);
+  SC_HIP(hipGetLastError());
+  int it = 0, status = 0;
+  double rr = 0;
+  e = pcg<double>(c, rhs_d, u_d, c.r64, c.p64, c.q64, c.dinv64, o.rtol, o.max_iters > 0 ? o.max_iters : 100000,
+                  o.check_every, 0, &it, &rr, &status);
+  if (e) return e;
+  sc_solve_stats st{};
+  st.iters = it;
+  st.relres = rr;
+  st.converged = status == 1;
+  if (stats) *stats = st;
+  if (!st.converged)
+    return fail(SC_ERR_NOCONV, "sc_solve_noalloc: CG64 did not converge (status " + std::to_string(status) +
+                                   ", relres " + std::to_string(rr) + ")");
+  return SC_OK;
 }
