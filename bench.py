@@ -6,38 +6,41 @@ Ny = Nz = 1001, Ly = Lz = 30 km, 4th-order SBP-SAT, sinh-stretched z grid
 rate-and-state friction (a ramps 0.015 -> 0.025 between 12 and 17 km),
 adaptive RK32 with PID control (timeStepTol 1e-8), synthetic inputs with a
 seeded +-1% state perturbation.  A "step" is one accepted RK32 step: three
-d_dt evaluations, each a full elasticity solve to relative residual 1e-12,
-plus rejected attempts if any.
+d_dt evaluations, each an elasticity solve to FP64 relative residual 1e-12,
+plus rejected attempts if any.  `value` is steps/s over steps 11-510 of that
+run with every input resident on the device.
 
-Solver paths, all from the same state over the same step window:
-  fp64_fdiag  FP64 CG preconditioned with the FP64 fast-diagonalisation
-              inverse of the separable operator (four DGEMMs)
-  mixed       the same CG with the preconditioner's GEMMs in FP32
-  mixed_bf16  ... in bf16 with FP32 accumulation
-  fp64        plain FP64 Jacobi-PCG (the straightforward GPU port)
-  ir32        FP32 Jacobi-PCG inside FP64 iterative refinement
-  fp64_ref    fp64_fdiag started from the previous solution only: the
-              accuracy reference
-The other FD-preconditioned paths start each solve from the projection of the
-new solution onto the last --reuse solutions.  `value` is the fastest path
-whose fault slip stays within 1e-8 (relative) of fp64_ref over the window; the
-line names it (config.solve) and lists every path.
+Solver paths (same state, same window):
+  fp64_fdiag    FP64 CG, FP64 fast-diagonalisation preconditioner (four DGEMMs)
+  mixed         FP64 CG, the preconditioner's GEMMs in FP32
+  ir32_fdiag32  FP32 CG with the FP32 preconditioner inside FP64 iterative
+                refinement (the residual b - A u in FP64)
+  ir32_bf16     ... the preconditioner's GEMMs in bf16 (FP32 accumulation)
+  fp64          plain FP64 Jacobi-PCG (the straightforward GPU port)
+Each runs with the projected initial guess (reuse 2: the last solutions as
+an A-orthogonal basis) and without it (reuse 0: every solve starts from the
+previous solution and iterates).  fp64_fdiag at reuse 0 is the accuracy
+reference of the window; `value` is the fastest reuse-2 path whose fault
+slip stays within 1e-8 of it.  `precision` lists both windows with the
+mixed/FP64 ratios of equal algorithm and equal guess.
 
 `roofline` is the dominant kernel of the timed step, the device loop's
-per-stage kernel (k_qd_stage: stage state, projections, fault rates and the
-projected guess's residual in one launch; three per attempt), timed with HIP
-events on the context stream; `roofline_guess` the projected-guess kernel of
-the host-checked solves.  Also on the line: the HBM roofline of the FP64
-SBP-SAT apply (one launch of
-interior tiles and frame blocks, 24 algorithmic bytes per node: read u, read
-mu, write A u),
-timed with HIP events on the context stream; `traffic` from the committed
-rocprofv3 FETCH_SIZE / WRITE_SIZE summary when present; the fully dynamic
-2001^2 rupture (C4) in FP64 and FP32-bulk steps/s; the discrete adjoint of
-C5 (cost over one forward, gradient error against a central difference);
-and a CPU baseline (the
-FP64 oracle's Jacobi-PCG on the assembled operator, timed on a bounded sample
-and scaled by the FP64 Jacobi path's iterations per step).
+per-stage kernel k_qd_stage<M, false> (stage state, projections, fault rates,
+projected guess and its residual; two per RK32 attempt, plus the attempt's
+end), its mean duration measured live in the step: the same window run with
+the attempts launched one by one and HIP events around each stage launch on
+the context stream (sc_qd_integrate_timed).  `traffic` is from the committed
+rocprofv3 FETCH_SIZE / WRITE_SIZE summary of that kernel.
+
+`oracle_c2`: the headline path's first steps from t = 0 replayed by the CPU
+FP64 oracle (Jacobi-PCG, warm-started, relative residual 1e-12) with the same
+step sizes: field errors against the oracle and the oracle's whole-step
+wall clock, which is `cpu_baseline`.  Also on the line: C1 (101^2, order 2,
+2000 RK32 steps; GPU and the oracle's banded-Cholesky run, the GPU replaying
+the oracle's steps for field errors), C3 (801^2 power law, 300 steps, and the
+pointwise creep update in FP64 and FP32), C4 (2001^2 fully dynamic rupture,
+FP64 and FP32 bulk, field errors, the wave-step roofline), C5 (the discrete
+adjoint).
 """
 import argparse
 import ctypes as C
@@ -75,7 +78,10 @@ PATHS = {
 }
 
 
-def run_path(cfg, label, warmup, steps, reuse=0, step_control="auto"):
+def run_path(cfg, label, warmup, steps, reuse=0, step_control="auto", timed=False):
+    """One solver path over the window: `warmup` untimed steps from t = 0,
+    then `steps` timed steps.  timed: the window runs with the device loop's
+    stage launches timed one by one (sc_qd_integrate_timed) instead."""
     from scycle_amd import QdMediator
     mode, prec, _ = PATHS[label]
     tic = time.perf_counter()
@@ -90,16 +96,22 @@ def run_path(cfg, label, warmup, steps, reuse=0, step_control="auto"):
                                  step_control=step_control)
         dt0 = float(dts[-1])
     c0 = q.counters()
+    kw = dict(integrator="RK32", tol=1e-8, init_dt=dt0, init_time=t0, max_dt=1e10, step_control=step_control)
     tic = time.perf_counter()
-    t1, n, dts = q.integrate(steps, integrator="RK32", tol=1e-8, init_dt=dt0, init_time=t0,
-                             max_dt=1e10, step_control=step_control)
+    stage = None
+    if timed:
+        t1, n, dts, ms_s, ms_e, att = q.integrate_timed(steps, **kw)
+        stage = dict(ms_stage=ms_s, ms_end=ms_e, attempts=att)
+    else:
+        t1, n, dts = q.integrate(steps, **kw)
     sec = time.perf_counter() - tic
     c1 = q.counters()
     st = q.get_state()
     d = {k: c1[k] - c0[k] for k in ("d_dt_calls", "solves", "cg_iters", "outer_sweeps",
-                                    "deferred_checks", "missed_guesses", "rejected_steps")}
+                                    "deferred_checks", "missed_guesses", "rejected_steps", "graph_builds")}
     return q, dict(seconds=sec, steps=n, t_start=t0, t_end=t1, counters=d, state=st,
-                   setup_s=setup, dt_min=float(dts.min()), dt_max=float(dts.max()), reuse=reuse)
+                   setup_s=setup, dt_min=float(dts.min()), dt_max=float(dts.max()), reuse=reuse,
+                   dts=dts, stage=stage)
 
 
 def roofline_guess(ctx, reps=200):
@@ -153,62 +165,215 @@ def traffic_per_launch(path=TRAFFIC_FILE, key="hbm_bytes_per_apply"):
         return None
 
 
-def cpu_baseline(cfg, iters_per_step, budget_s=20.0):
-    """FP64 oracle Jacobi-PCG at the same operator: time a bounded number of
-    iterations, scale by the FP64 GPU run's CG iterations per step."""
+def host_info():
+    """Host cores and model of the machine the CPU legs run on."""
+    import subprocess
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return dict(nproc=os.cpu_count(), threads=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                cpu_model=model)
+
+
+def _rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300))
+
+
+def oracle_c2(cfg, q, label, t0, dt0, k):
+    """k more adaptive RK32 steps of the headline path from the end of its
+    window (mediator q at time t0; the creeping fault then carries
+    V ~ 1e-9 m/s and tens of MPa, where the root solve's absolute tolerance
+    is ~1e-12 relative -- at t = 0 the stresses are ~0 and V ~ 1e-20 m/s is set
+    by that tolerance alone), and the same k steps (same step sizes, same
+    start) replayed by the CPU FP64 oracle: Jacobi-PCG on the assembled
+    operator, every solve warm-started from the previous solution (one
+    untimed d_dt at the start state warms it first), relative residual 1e-12.
+    The reference's MUMPS Cholesky is not used at this size: the band of the
+    z-fast operator is 4 Nz.  Returns the field errors of the GPU against the
+    oracle and the oracle's wall clock per whole step."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
+    s0 = q.get_state()
+    t1, n, dts = q.integrate(k, integrator="RK32", tol=1e-8, init_dt=dt0, init_time=t0, max_dt=1e10)
+    g = q.get_state()
+    gu = q.get_u()
     tic = time.perf_counter()
-    o = ol.OracleSbp(cfg.order, cfg.Ny, cfg.Nz, cfg.Ly, cfg.Lz, cfg.mu, cfg.y, cfg.z,
-                     var_grid=cfg.var_grid, bc=cfg.mat_bc)
+    o = ol.OracleQd(cfg, solver=1, jac_mode=0)
     t_build = time.perf_counter() - tic
-    rng = np.random.default_rng(0)
-    rhs = o.set_rhs(1e-3 * rng.random(cfg.Nz), np.full(cfg.Nz, 5e-4), np.zeros(cfg.Ny),
-                    np.zeros(cfg.Ny))
-    it = 20
+    Vw = s0["V"].copy()
+    dp, ds = np.zeros(cfg.Nz), np.zeros(cfg.Nz)
     tic = time.perf_counter()
-    _, done, _ = o.pcg(rhs, rtol=1e-300, maxit=it)
-    t_probe = time.perf_counter() - tic
-    per = t_probe / max(done, 1)
-    it2 = int(max(10, min(2000, (budget_s - t_probe) / per)))
+    rc = ol.lib().orc_qd_ddt(o.h, t0, ol.ptr(s0["psi"]), ol.ptr(s0["slip"]), ol.ptr(Vw), ol.ptr(dp),
+                             ol.ptr(ds), None, None)
+    t_warm = time.perf_counter() - tic
+    if rc:
+        raise RuntimeError(f"oracle d_dt failed ({rc})")
+    it0 = int(ol.lib().orc_qd_counter(o.h, 1))
     tic = time.perf_counter()
-    _, done2, _ = o.pcg(rhs, rtol=1e-300, maxit=it2)
-    t_iter = (time.perf_counter() - tic) / max(done2, 1)
-    step_s = iters_per_step * t_iter
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=1.0 / step_s if step_s > 0 else None, unit="QD steps/s", cores=cores,
-                kind="port",
-                sample=(f"{done2} Jacobi-PCG iterations of the FP64 oracle on the assembled "
-                        f"{cfg.Ny}x{cfg.Nz} order-{cfg.order} operator ({t_iter*1e3:.1f} ms/iter, "
-                        f"assembly {t_build:.1f} s untimed), scaled by "
-                        f"{iters_per_step:.0f} CG iterations per RK32 step of the FP64 Jacobi "
-                        f"GPU path (same algorithm)"))
+    r = o.replay(dts, s0["psi"], s0["slip"], V=s0["V"], t0=t0)
+    sec = time.perf_counter() - tic
+    iters = int(ol.lib().orc_qd_counter(o.h, 1)) - it0
+    solves = int(ol.lib().orc_qd_counter(o.h, 0)) - 1
+    del o
+    errs = dict(slip=_rel(g["slip"], r["slip"]), psi=_rel(g["psi"], r["psi"]), V=_rel(g["V"], r["V"]),
+                u=_rel(gu, r["u"]), t=abs(t1 / r["t"] - 1))
+    return dict(steps=n, t_start=t0, t_end=t1, dts=dts.tolist(), errors_vs_oracle=errs, oracle_seconds=sec,
+                oracle_steps_per_s=n / sec, oracle_solves=solves, oracle_cg_iters=iters,
+                oracle_assembly_s=t_build, oracle_warm_ddt_s=t_warm, path=label)
+
+
+def c1_leg(steps):
+    """C1: 101^2, order 2, constant grid, mu = 36 GPa, 2000 adaptive RK32
+    steps from t = 0.  GPU: the headline path (FP64 CG, FP64 preconditioner,
+    reuse 2).  CPU: the oracle with the banded Cholesky factorisation (the
+    stand-in for the reference's MUMPS Cholesky) on one thread, whole steps.
+    Accuracy: the GPU replays the oracle's step sizes, fields compared at the
+    end."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    from scycle_amd import QdMediator
+    from scycle_amd.configs import config_c1
+    cfg = config_c1()
+    q = QdMediator(cfg, solve="cg64", rtol=1e-12, jac_mode=0, precond="fdiag64", reuse=2)
+    q.set_state(cfg.psi0, cfg.slip0, np.zeros(cfg.Nz))
+    q.integrate(10, tol=1e-8, init_dt=1e-3, max_dt=1e10)  # warm-up (kernel loads, graph)
+    q.set_state(cfg.psi0, cfg.slip0, np.zeros(cfg.Nz))
+    tic = time.perf_counter()
+    tg, ng, dtg = q.integrate(steps, tol=1e-8, init_dt=1e-3, max_dt=1e10)
+    sec_gpu = time.perf_counter() - tic
+    q.close()
+    o = ol.OracleQd(cfg, solver=0, jac_mode=0)
+    st = [cfg.psi0.copy(), cfg.slip0.copy()]
+    V, tau, u = np.zeros(cfg.Nz), np.zeros(cfg.Nz), np.zeros(cfg.Ny * cfg.Nz)
+    r = ol.OrcQdRun(0, 1, 0, steps, 0.0, 1e15, 1e-3, 0.0, 1e10, 1e-8, 0, None)
+    tE = C.c_double()
+    odts = np.zeros(steps)
+    secs = C.c_double()
+    n = ol.lib().orc_qd_integrate(o.h, C.byref(r), ol.ptr(st[0]), ol.ptr(st[1]), ol.ptr(V), ol.ptr(tau),
+                                  ol.ptr(u), C.byref(tE), ol.ptr(odts), steps, C.byref(secs))
+    del o
+    # the GPU with the oracle's step sizes
+    q = QdMediator(cfg, solve="cg64", rtol=1e-12, jac_mode=0, precond="fdiag64", reuse=2)
+    q.set_state(cfg.psi0, cfg.slip0, np.zeros(cfg.Nz))
+    tr, nr, _ = q.integrate(n, tol=1e-8, init_dt=1e-3, max_dt=1e10, replay_dt=odts[:n])
+    g = q.get_state()
+    gu = q.get_u()
+    q.close()
+    return dict(workload=f"C1 QD {cfg.Ny}x{cfg.Nz} order-2 constant grid mu 36 GPa aging-law RK32 tol 1e-8, "
+                         f"{steps} steps from t=0",
+                gpu_steps_per_s=ng / sec_gpu, gpu_steps=ng, gpu_t_end=tg,
+                cpu_steps_per_s=n / secs.value, cpu_steps=n, cpu_seconds=secs.value, cpu_t_end=tE.value,
+                cpu_solver="oracle banded Cholesky (MUMPS-Cholesky stand-in), 1 thread",
+                gpu_over_cpu=(ng / sec_gpu) / (n / secs.value),
+                replay_errors_vs_oracle=dict(slip=_rel(g["slip"], st[1]), psi=_rel(g["psi"], st[0]),
+                                             V=_rel(g["V"], V), u=_rel(gu, u), t=abs(tr / tE.value - 1)),
+                free_run_dt_rel_diff=float(np.abs(dtg[:min(ng, n)] / odts[:min(ng, n)] - 1).max()))
+
+
+def c3_leg(n, steps):
+    """C3: 801^2 power-law viscoelastic QD (dislocation creep), 300 adaptive
+    RK32 steps on the GPU; and the pointwise creep update (strain rates and
+    effective viscosity) with FP64 and FP32 arrays on the mediator's own
+    stresses and temperature, its error in FP32 against FP64."""
+    from scycle_amd import DeviceArray, PowerLawMediator, load
+    from scycle_amd._lib import PowerLawArgs, check
+    from scycle_amd.configs import config_c3
+    cfg = config_c3(n, n)
+    m = PowerLawMediator(cfg, solve="cg64", rtol=1e-12, jac_mode=0, precond="fdiag64", reuse=2)
+    m.set_state(cfg.psi0, cfg.slip0, np.zeros(cfg.Nz))
+    m.integrate(3, tol=1e-8, init_dt=1e-3, max_dt=1e10)
+    m.set_state(cfg.psi0, cfg.slip0, np.zeros(cfg.Nz))
+    m.set_visc(np.zeros(n * n), np.zeros(n * n))
+    c0 = m.counters()
+    tic = time.perf_counter()
+    t1, k, dts = m.integrate(steps, tol=1e-8, init_dt=1e-3, max_dt=1e10)
+    sec = time.perf_counter() - tic
+    c1 = m.counters()
+    v = m.get_visc()
+    N = n * n
+    # pointwise update on the final stresses (sxz from the viscous-strain rate ratio is not
+    # kept by the mediator: take sxz = 0.5 sxy so that both components are exercised)
+    d = dict(sxy=v["sxy"], sxz=0.5 * v["sxy"], T=cfg.T, A=cfg.A, B=cfg.B, n=cfg.n, mu=cfg.mu,
+             sat=np.zeros(N))
+    ctx = load().sc_qd_ctx(m.h)
+    res = {}
+    for prec, dt in ((0, np.float64), (1, np.float32)):
+        dev = {kk: DeviceArray(N, dt, d[kk]) for kk in d}
+        out = {kk: DeviceArray(N, dt) for kk in ("ev", "gxy", "gxz")}
+        a = PowerLawArgs(N, cfg.visc_cap, dev["sxy"].ptr, dev["sxz"].ptr, dev["T"].ptr, dev["A"].ptr,
+                         dev["B"].ptr, dev["n"].ptr, dev["mu"].ptr, dev["sat"].ptr, out["ev"].ptr,
+                         out["gxy"].ptr, out["gxz"].ptr)
+        check(load().sc_powerlaw_rates(ctx, prec, C.byref(a)))
+        check(load().sc_ctx_sync(ctx))
+        reps = 200
+        tic = time.perf_counter()
+        for _ in range(reps):
+            load().sc_powerlaw_rates(ctx, prec, C.byref(a))
+        check(load().sc_ctx_sync(ctx))
+        ms = 1e3 * (time.perf_counter() - tic) / reps
+        nbytes = 11 * N * (8 if prec == 0 else 4)
+        res[dt.__name__] = dict(ms_per_launch=ms, GBps=nbytes / (ms * 1e-3) / 1e9,
+                                out={kk: out[kk].to_host().astype(np.float64) for kk in out})
+    e32 = {kk: _rel(res["float32"]["out"][kk], res["float64"]["out"][kk]) for kk in ("ev", "gxy", "gxz")}
+    m.close()
+    return dict(workload=f"C3 power-law QD {n}x{n} order-4 constant grid, dislocation creep n=3.5 A=1e-3 "
+                         f"Q=130 kJ/mol, T = 293 K + 25 K/km z; {steps} RK32 steps from t=0",
+                steps_per_s=k / sec, ms_per_step=1e3 * sec / k, steps=k, t_end=t1,
+                solves=c1["solves"] - c0["solves"], cg_iters=c1["cg_iters"] - c0["cg_iters"],
+                pointwise={kk: dict(ms_per_launch=vv["ms_per_launch"], GBps=vv["GBps"])
+                           for kk, vv in res.items()},
+                pointwise_bytes_per_node={"float64": 88, "float32": 44},
+                pointwise_timing="wall clock over 200 launches and one synchronisation",
+                fp32_vs_fp64_rel_err={"effVisc": e32["ev"], "gVxy_rate": e32["gxy"], "gVxz_rate": e32["gxz"]})
 
 
 def fd_c4(n, steps):
-    """Fully dynamic 2001^2 rupture: constant-dt wave + fault steps."""
+    """Fully dynamic 2001^2 rupture: constant-dt wave + fault steps, FP64 and
+    FP32 bulk; field errors of the FP32-bulk run against the FP64 run; the
+    wave step's roofline (40 algorithmic bytes per node: read u, u_prev, mu,
+    rho, write u_next; ay and the frame coefficients are boundary-only) timed
+    with HIP events on the context stream."""
     from scycle_amd import FdMediator
+    from scycle_amd._lib import check, load
     from scycle_amd.configs import config_c4
     cfg = config_c4(n, n)
     N, Nz = cfg.Ny * cfg.Nz, cfg.Nz
     out = {}
+    states = {}
     for bulk in ("fp64", "fp32"):
         m = FdMediator(cfg, bulk=bulk)
         z0 = np.zeros(N)
         psi = np.full(Nz, 1.0) * cfg.psi0
         m.set_state(z0, z0, psi, psi, np.zeros(Nz), np.zeros(Nz))
         m.steps(10, cfg.dt)  # warm-up
+        m.set_state(z0, z0, psi, psi, np.zeros(Nz), np.zeros(Nz))
         tic = time.perf_counter()
-        m.steps(steps, cfg.dt, t0=10 * cfg.dt)
+        m.steps(steps, cfg.dt)
         sec = time.perf_counter() - tic
         st = m.get_state()
+        states[bulk] = st
         out[bulk] = dict(steps_per_s=steps / sec, ms_per_step=1e3 * sec / steps,
                          max_slip=float(st["slip"].max()), max_V=float(st["V"].max()))
+        if bulk == "fp64":
+            ctx = load().sc_fd_ctx(m.h)
+            rl = {}
+            for which, key, bpn in ((9, "wave64", 40.0), (10, "wave32", 20.0)):
+                ms, nodes = C.c_double(), C.c_double()
+                check(load().sc_bench_apply(ctx, which, 100, C.byref(ms), C.byref(nodes)))
+                rl[key] = dict(ms_per_launch=ms.value, bytes=bpn * nodes.value,
+                               GBps=bpn * nodes.value / (ms.value * 1e-3) / 1e9,
+                               frac=bpn * nodes.value / (ms.value * 1e-3) / HBM_PEAK)
+            out["roofline_wave"] = rl
         m.close()
-    out["slip_rel_err_fp32_vs_fp64"] = abs(out["fp32"]["max_slip"] - out["fp64"]["max_slip"]) / \
-        max(abs(out["fp64"]["max_slip"]), 1e-300)
+    out["fp32_bulk_vs_fp64_rel_err"] = {k: _rel(states["fp32"][k], states["fp64"][k])
+                                        for k in ("u", "slip", "psi", "V", "tau")}
     out["workload"] = (f"FD rupture {n}x{n} order-4 constant grid, cs 3.464 km/s, CFL 0.5, slip law, "
-                       f"nucleation patch; {steps} steps after 10 warm-up steps")
+                       f"nucleation patch; {steps} steps from rest (after 10 untimed warm-up steps)")
     return out
 
 
@@ -320,19 +485,29 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1001)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle legs (C2 window, C1 CPU run)")
     ap.add_argument("--no-fd", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--adj-n", type=int, default=401)
     ap.add_argument("--adj-steps", type=int, default=500)
     ap.add_argument("--adj-ckpt", type=int, default=50)
     ap.add_argument("--fd-n", type=int, default=2001)
     ap.add_argument("--fd-steps", type=int, default=2000)
-    ap.add_argument("--paths", default="fp64_ref,fp64_fdiag,mixed,mixed_bf16,fp64")
+    ap.add_argument("--c1-steps", type=int, default=2000)
+    ap.add_argument("--c3-n", type=int, default=801)
+    ap.add_argument("--c3-steps", type=int, default=300)
+    ap.add_argument("--oracle-steps", type=int, default=3,
+                    help="C2 steps from t = 0 replayed by the CPU oracle (errors and CPU baseline)")
+    ap.add_argument("--paths", default="fp64_fdiag,mixed,ir32_fdiag32,ir32_bf16",
+                    help="solver paths of the reuse-2 and reuse-0 windows")
     ap.add_argument("--reuse", type=int, default=2,
-                    help="previous solutions projected for the CG initial guess (FD paths)")
-    ap.add_argument("--slow-steps", type=int, default=3,
-                    help="timed steps for the Jacobi-preconditioned paths (same window start)")
+                    help="previous solutions projected for the CG initial guess")
+    ap.add_argument("--bf16-steps", type=int, default=50,
+                    help="timed steps of the reuse-0 window for ir32_bf16 (its solves iterate ~50 times)")
+    ap.add_argument("--jacobi-steps", type=int, default=3,
+                    help="timed steps of the plain FP64 Jacobi-PCG path")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the FP64 apply and a calibration copy (for counter runs)")
     args = ap.parse_args()
@@ -344,79 +519,103 @@ def main():
     if args.kernels_only:
         kernels_only(args.n, 50, args.reuse)
         return
+    from scycle_amd._lib import load
     from scycle_amd.configs import config_c2
     cfg = config_c2(args.n, args.n)
-    res = {}
     labels = args.paths.split(",")
-    ref = "fp64_ref" if "fp64_ref" in labels else labels[0]
+
+    def run(label, reuse, steps, timed=False):
+        log(f"[bench] path {label} {PATHS[label][:2]} reuse {reuse}: warmup {args.warmup} steps, "
+            f"timed {steps} steps, {cfg.Ny}x{cfg.Nz}")
+        q, r = run_path(cfg, label, args.warmup, steps, reuse=reuse, timed=timed)
+        log(f"[bench] {label} reuse {reuse}: {r['steps']} steps in {r['seconds']:.3f} s "
+            f"(setup {r['setup_s']:.1f} s), counters {r['counters']}")
+        return q, r
+
+    def summary(r, ref):
+        p = dict(reuse=r["reuse"], steps=r["steps"], steps_per_s=r["steps"] / r["seconds"],
+                 ms_per_step=1e3 * r["seconds"] / r["steps"],
+                 cg_iters_per_solve=r["counters"]["cg_iters"] / max(r["counters"]["solves"], 1),
+                 outer_sweeps_per_solve=r["counters"]["outer_sweeps"] / max(r["counters"]["solves"], 1),
+                 ms_per_solve=1e3 * r["seconds"] / max(r["counters"]["solves"], 1),
+                 solves_per_step=r["counters"]["solves"] / r["steps"], setup_s=r["setup_s"],
+                 deferred_checks=r["counters"]["deferred_checks"],
+                 missed_guesses=r["counters"]["missed_guesses"],
+                 rejected_steps=r["counters"]["rejected_steps"], graph_builds=r["counters"]["graph_builds"],
+                 t_start=r["t_start"], t_end=r["t_end"], dt_min=r["dt_min"], dt_max=r["dt_max"])
+        if ref is not None and r["steps"] == ref["steps"]:
+            p["slip_rel_err_vs_fp64_ref"] = _rel(r["state"]["slip"], ref["state"]["slip"])
+            p["V_rel_err_vs_fp64_ref"] = _rel(r["state"]["V"], ref["state"]["V"])
+        return p
+
+    # the FP64 reference of the window: FP64 preconditioner, every solve
+    # from the previous solution
+    q, ref = run("fp64_ref", 0, args.steps)
+    q.close()
+    win = {"reuse2": {}, "reuse0": {"fp64_fdiag": summary(ref, ref)}}
     mediators = {}
     for label in labels:
-        jac = PATHS[label][1] == "jacobi"
-        k = min(args.steps, args.slow_steps) if jac else args.steps
-        log(f"[bench] path {label} {PATHS[label][:2]}: warmup {args.warmup} steps, timed {k} steps, "
-            f"{cfg.Ny}x{cfg.Nz}")
-        q, r = run_path(cfg, label, args.warmup, k,
-                        reuse=0 if (jac or label == "fp64_ref") else args.reuse)
-        log(f"[bench] {label}: {r['steps']} steps in {r['seconds']:.3f} s (setup {r['setup_s']:.1f} s), "
-            f"counters {r['counters']}")
-        res[label] = r
-        mediators[label] = q
-    # accuracy of each path against the FP64 reference path over the same window
-    paths = {}
-    for lab, rr in res.items():
-        p = dict(solve=PATHS[lab][0], precond=PATHS[lab][1], reuse=rr["reuse"], steps=rr["steps"],
-                 steps_per_s=rr["steps"] / rr["seconds"], ms_per_step=1e3 * rr["seconds"] / rr["steps"],
-                 cg_iters_per_solve=rr["counters"]["cg_iters"] / max(rr["counters"]["solves"], 1),
-                 solves_per_step=rr["counters"]["solves"] / rr["steps"], setup_s=rr["setup_s"],
-                 deferred_checks=rr["counters"]["deferred_checks"],
-                 missed_guesses=rr["counters"]["missed_guesses"],
-                 rejected_steps=rr["counters"]["rejected_steps"],
-                 t_start=rr["t_start"], t_end=rr["t_end"], dt_min=rr["dt_min"], dt_max=rr["dt_max"])
-        if lab != ref and rr["steps"] == res[ref]["steps"]:
-            a, b = rr["state"]["slip"], res[ref]["state"]["slip"]
-            p["slip_rel_err_vs_fp64"] = float(np.abs(a - b).max() / np.abs(b).max())
-            p["V_rel_err_vs_fp64"] = float(np.abs(rr["state"]["V"] - res[ref]["state"]["V"]).max()
-                                           / np.abs(res[ref]["state"]["V"]).max())
-        elif lab == ref:
-            p["slip_rel_err_vs_fp64"] = 0.0
-        paths[lab] = p
-    ok = [lab for lab, p in paths.items()
-          if p.get("slip_rel_err_vs_fp64") is not None and p["slip_rel_err_vs_fp64"] <= SLIP_TOL
-          and p["steps"] == paths[ref]["steps"]]
-    best = max(ok, key=lambda lab: paths[lab]["steps_per_s"])
-    value = paths[best]["steps_per_s"]
-    extra = {"paths": paths, "slip_err_reference_path": ref}
-    if "fp64" in paths:
-        extra["fp64_jacobi_steps_per_s"] = paths["fp64"]["steps_per_s"]
-        if "mixed" in paths:
-            extra["mixed_over_fp64"] = paths["mixed"]["steps_per_s"] / paths["fp64"]["steps_per_s"]
-        extra["value_over_fp64_jacobi"] = value / paths["fp64"]["steps_per_s"]
-    if "mixed" in paths and "fp64_fdiag" in paths:
-        extra["mixed_over_fp64_same_preconditioner"] = (paths["mixed"]["steps_per_s"]
-                                                        / paths["fp64_fdiag"]["steps_per_s"])
-    from scycle_amd._lib import load
-    # the dominant kernel of the timed step: the device loop's stage kernel;
-    # the host-checked solves' guess kernel and the SBP apply alongside
-    if paths[best]["reuse"] == 0:
-        sys.exit("bench.py: the fastest path runs without a solution basis; the stage and guess "
-                 "kernels need one (run it with --reuse > 0)")
-    sq = roofline_stage(mediators[best], paths[best]["t_end"])
-    log(f"[bench] stage kernel {sq}")
-    gq = roofline_guess(load().sc_qd_ctx(mediators[best].h))
-    log(f"[bench] projected-guess kernel {gq}")
-    rl = roofline(load().sc_qd_ctx(mediators[best].h))
-    log(f"[bench] apply {rl}")
+        q, r = run(label, args.reuse, args.steps)
+        win["reuse2"][label] = summary(r, ref)
+        mediators[label] = (q, r)
+        if label == "fp64_fdiag":
+            continue
+        k = args.bf16_steps if label == "ir32_bf16" else args.steps
+        q0, r0 = run(label, 0, k)
+        win["reuse0"][label] = summary(r0, ref)
+        q0.close()
+    if args.jacobi_steps > 0:
+        q, r = run("fp64", 0, args.jacobi_steps)
+        win["reuse0"]["fp64_jacobi"] = summary(r, None)
+        q.close()
+    ok = [lab for lab, p in win["reuse2"].items()
+          if p.get("slip_rel_err_vs_fp64_ref") is not None and p["slip_rel_err_vs_fp64_ref"] <= SLIP_TOL]
+    if not ok:
+        sys.exit("bench.py: no reuse-2 path met the slip tolerance against the FP64 reference")
+    best = max(ok, key=lambda lab: win["reuse2"][lab]["steps_per_s"])
+    value = win["reuse2"][best]["steps_per_s"]
+    precision = {"windows": win}
+    for w, base in (("reuse2", "fp64_fdiag"), ("reuse0", "fp64_fdiag")):
+        if base in win[w]:
+            precision[f"over_fp64_fdiag_{w}"] = {lab: p["steps_per_s"] / win[w][base]["steps_per_s"]
+                                                 for lab, p in win[w].items() if lab != base}
+    precision["note"] = (
+        "equal algorithm and guess: each ratio divides a path by FP64 CG with the FP64 preconditioner "
+        "and the same reuse.  With reuse 2 almost every solve ends at the projected guess, so the "
+        "preconditioner's precision does not enter.  With reuse 0 the FP64 preconditioner is near-exact "
+        "for this depth-only mu (its forward transform errs by eps*cond(A) ~ 1e-16*1e6), while FP32 GEMMs "
+        "err by ~6e-8*1e6 and bf16 by ~4e-3*1e6 through the inverse eigenvalue scaling, so the reduced "
+        "precisions need more iterations per solve than the GEMM speed-up (FP32 MFMA peak = 2x FP64) "
+        "buys back.")
+    # the dominant kernel in the step: the same window with the stage
+    # launches timed one by one
+    q_best, r_best = mediators[best]
+    sq = roofline_stage(q_best, r_best["t_end"])
+    log(f"[bench] stage kernel back-to-back {sq}")
+    qt, rt = run(best, args.reuse, args.steps, timed=True)
+    st = rt["stage"]
+    qt.close()
+    log(f"[bench] stage kernel in the step {st}")
+    gq = roofline_guess(load().sc_qd_ctx(q_best.h))
+    rl = roofline(load().sc_qd_ctx(q_best.h))
+    log(f"[bench] guess {gq} apply {rl}")
 
     def counted(path, alg):
-        # the counter summary holds the kernel with the basis size this window ends with
         t = traffic_per_launch(path, "hbm_bytes_per_launch")
         return t if traffic_per_launch(path, "algorithmic_bytes_per_launch") == alg else None
 
-    roof = dict(bound="hbm", achieved=sq["GBps"], peak=HBM_PEAK / 1e9, unit="GB/s",
-                frac=sq["GBps"] * 1e9 / HBM_PEAK, traffic=counted(STAGE_TRAFFIC_FILE, sq["bytes"]),
-                kernel="k_qd_stage<M> (one RK stage of the device loop: stage state, projections, fault rates, "
-                       "projected guess and its residual)",
-                ms_per_launch=sq["ms"], algorithmic_bytes_per_launch=sq["bytes"])
+    ms_in = st["ms_stage"] if st["attempts"] > 0 else None
+    roof = dict(bound="hbm", achieved=sq["bytes"] / (ms_in * 1e-3) / 1e9 if ms_in else None,
+                peak=HBM_PEAK / 1e9, unit="GB/s",
+                frac=sq["bytes"] / (ms_in * 1e-3) / HBM_PEAK if ms_in else None,
+                traffic=counted(STAGE_TRAFFIC_FILE, sq["bytes"]),
+                kernel="k_qd_stage<M, false> (one RK stage of the device loop: stage state, projections, "
+                       "fault rates, projected guess and its residual)",
+                ms_per_launch=ms_in, algorithmic_bytes_per_launch=sq["bytes"],
+                timing=(f"mean of the {2 * st['attempts']} stage launches of {st['attempts']} attempts in the "
+                        f"timed window, HIP events around each launch on the context stream"),
+                attempt_end_ms=st["ms_end"], back_to_back_ms=sq["ms"],
+                back_to_back_frac=sq["GBps"] * 1e9 / HBM_PEAK)
     roof_guess = dict(bound="hbm", achieved=gq["GBps"], peak=HBM_PEAK / 1e9, unit="GB/s",
                       frac=gq["GBps"] * 1e9 / HBM_PEAK, traffic=counted(GUESS_TRAFFIC_FILE, gq["bytes"]),
                       kernel="k_proj_guess_res<M> (projected guess + residual of the host-checked solves)",
@@ -427,8 +626,33 @@ def main():
                       kernel="SBP-SAT apply FP64 (k_apply_interior: interior tiles + assembled-frame blocks, one launch)",
                       ms_per_launch=a64["ms"], algorithmic_bytes_per_launch=a64["bytes"],
                       parts={k: v for k, v in rl.items() if k != "apply64"})
-    for q in mediators.values():
-        q.close()
+    host = host_info()
+    oracle = None
+    cpu = None
+    if not args.no_cpu and args.oracle_steps > 0:
+        log(f"[bench] C2 oracle window: {args.oracle_steps} steps after the window on the GPU ({best}) "
+            f"and on the CPU oracle")
+        oracle = oracle_c2(cfg, q_best, best, r_best["t_end"], float(r_best["dts"][-1]), args.oracle_steps)
+        log(f"[bench] oracle window {oracle['errors_vs_oracle']}, oracle {oracle['oracle_seconds']:.1f} s")
+        cpu = dict(value=oracle["oracle_steps_per_s"], unit="QD steps/s", cores=host["threads"], kind="port",
+                   sample=(f"{oracle['steps']} whole RK32 steps of the {cfg.Ny}x{cfg.Nz} run after its timed "
+                           f"window ({oracle['oracle_solves']} solves, {oracle['oracle_cg_iters']} CG "
+                           f"iterations) by the FP64 oracle: Jacobi-PCG on the assembled operator, "
+                           f"warm-started, rtol 1e-12, OpenMP on {host['threads']} threads of {host['nproc']} "
+                           f"({host['cpu_model']}); {oracle['oracle_seconds']:.1f} s, assembly "
+                           f"{oracle['oracle_assembly_s']:.1f} s untimed"))
+    for qq, _ in mediators.values():
+        qq.close()
+    c1 = None
+    if not args.no_c1:
+        log(f"[bench] C1 101^2, {args.c1_steps} steps")
+        c1 = c1_leg(args.c1_steps) if not args.no_cpu else None
+        log(f"[bench] c1 {c1}")
+    c3 = None
+    if not args.no_c3:
+        log(f"[bench] C3 {args.c3_n}^2 power law, {args.c3_steps} steps")
+        c3 = c3_leg(args.c3_n, args.c3_steps)
+        log(f"[bench] c3 {c3}")
     fd = None
     if not args.no_fd:
         log(f"[bench] fully dynamic C4 {args.fd_n}^2, {args.fd_steps} steps")
@@ -439,25 +663,25 @@ def main():
         log(f"[bench] adjoint C5 {args.adj_n}^2, {args.adj_steps} steps")
         adj = adjoint_c5(args.adj_n, args.adj_steps, args.adj_ckpt)
         log(f"[bench] adjoint {adj}")
-    cpu = None
-    if not args.no_cpu and "fp64" in res:
-        ips = res["fp64"]["counters"]["cg_iters"] / res["fp64"]["steps"]
-        log(f"[bench] cpu baseline (oracle), {ips:.0f} iterations/step")
-        cpu = cpu_baseline(cfg, ips)
+    p_best = win["reuse2"][best]
     line = {
         "metric": "QD steps/s at 1001^2 mixed vs FP64 GPU (slip rel.err <=1e-8) + HBM GB/s",
         "value": value, "unit": "QD steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": paths[best]["ms_per_step"], "higher_is_better": True,
+        "warmup": args.warmup, "ms_per_step": p_best["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": PATHS[best][2], "data": "synthetic",
         "config": {"workload": f"QD strike-slip {cfg.Ny}x{cfg.Nz} order-4 sinh-z(5) layered-mu "
                                f"aging-law RK32 tol 1e-8, steps {args.warmup + 1}-"
                                f"{args.warmup + args.steps} from t=0",
-                   "solve": best, "precond": PATHS[best][1], "reuse": paths[best]["reuse"],
-                   "rtol": 1e-12},
-        "roofline": roof, "roofline_guess": roof_guess, "roofline_apply": roof_apply, "cpu_baseline": cpu, "fd_c4": fd,
-        "adjoint_c5": adj,
+                   "solve": best, "precond": PATHS[best][1], "reuse": args.reuse, "rtol": 1e-12},
+        "roofline": roof, "cpu_baseline": cpu,
+        "slip_rel_err_vs_fp64_ref": p_best.get("slip_rel_err_vs_fp64_ref"),
+        "errors_vs_oracle_c2": oracle and oracle["errors_vs_oracle"],
+        "mixed_over_fp64_same_algorithm": {
+            "reuse2": precision.get("over_fp64_fdiag_reuse2", {}).get("ir32_fdiag32"),
+            "reuse0": precision.get("over_fp64_fdiag_reuse0", {}).get("ir32_fdiag32")},
+        "precision": precision, "roofline_guess": roof_guess, "roofline_apply": roof_apply,
+        "oracle_c2": oracle, "c1": c1, "c3": c3, "fd_c4": fd, "adjoint_c5": adj, "host": host,
     }
-    line.update(extra)
     print(json.dumps(line))
 
 

@@ -108,7 +108,8 @@ int sc_sbp_hinv(sc_ctx* ctx, const double* in_d, double* out_d);
  * 4 / 5 the interior kernel with the plain (not XCD-grouped) tile order,
  * 6 FP64 frame kernel, 7 FP32 frame kernel, 8 the projected-guess kernel of
  * sc_solve on the context's current basis (needs a solve with reuse > 0;
- * `nodes` then returns the 8-byte words it moves per launch).
+ * `nodes` then returns the 8-byte words it moves per launch), 9 / 10 the
+ * FP64 / FP32 wave step of sc_wave_step (laplace type yz).
  * No counterpart in the reference (its timers are MPI_Wtime accumulators,
  * e.g. linearElastic.cpp:474-477). */
 int sc_bench_apply(sc_ctx* ctx, int which, int reps, double* ms_per_launch, double* nodes);
@@ -377,6 +378,16 @@ int sc_qd_get_counters(sc_qd* q, sc_qd_counters* c);
  * basis (an integration with reuse > 0) and Nz <= 2048.  No counterpart in
  * the reference (its timers are MPI_Wtime accumulators). */
 int sc_qd_bench_stage(sc_qd* q, double t, int reps, double* ms_per_launch, double* words);
+/* Measurement: sc_qd_integrate with the device loop's attempts launched one
+ * by one (no graph) and HIP events around each stage launch on the context
+ * stream; returns the mean duration (ms) of the plain stage kernels
+ * (k_qd_stage, two per attempt) and of the attempt-end kernel over the
+ * attempts that ran (*attempts; 0 when the run never entered the device
+ * loop).  The steps taken are those of sc_qd_integrate.  No counterpart in
+ * the reference. */
+int sc_qd_integrate_timed(sc_qd* q, const sc_qd_run* run, double* tEnd_h, int* steps_h,
+                          double* dts_h, int dtsCap, double* ms_stage, double* ms_end,
+                          long long* attempts);
 /* Time-integration settings read from the deck (timeIntegrator,
  * timeControlType, maxStepCount, init/maxTime, min/max/initDeltaT with
  * computeMinTimeStep, timeStepTol, normType), for sc_qd_create_from_file

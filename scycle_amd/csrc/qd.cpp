@@ -489,6 +489,31 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
   selSlot[1] = (errMask & 2) ? nsSel++ : -1;
   int parity = 0;
   std::vector<unsigned char>* rec = nullptr;  // set: slot() records its launches' arguments
+  // per-launch timing (sc_qd_integrate_timed): events around the three stage
+  // launches of each attempt, durations per attempt in launch order
+  const bool timed = timing != nullptr && oneKernel;
+  hipEvent_t ev[6] = {};
+  std::vector<float> attemptMs;
+  struct EventGuard {
+    hipEvent_t* e;
+    ~EventGuard() {
+      for (int i = 0; i < 6; ++i)
+        if (e[i]) (void)hipEventDestroy(e[i]);
+    }
+  } evGuard{ev};
+  if (timed)
+    for (int i = 0; i < 6; ++i) SC_HIP(hipEventCreate(&ev[i]));
+  // the events take the kernels' own start and end (no marker packets between launches)
+  auto evt = [&](int i) -> hipEvent_t { return timed && !rec ? ev[i] : nullptr; };
+  auto collect = [&]() -> int {
+    SC_HIP(hipEventSynchronize(ev[5]));
+    for (int k = 0; k < 3; ++k) {
+      float ms = 0.f;
+      SC_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+      attemptMs.push_back(ms);
+    }
+    return SC_OK;
+  };
   auto slot = [&]() -> int {
     if (oneKernel) {
       const int p = parity;
@@ -496,8 +521,10 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
       const StepGate gp{ctl_d + p, 1};
       const PendingFinish prev = finPending ? fin : PendingFinish{};
       const VarsLinComb A1 = at_parity(L1, p), A2 = at_parity(L2, p);
-      int e = launch_qd_stage(ctx->c, stageBufs, 0, stage_in(&A1, 1, gp, 0.5, prev), s, rec);
-      if (!e) e = launch_qd_stage(ctx->c, stageBufs, 1, stage_in(&A2, 2, gp, 1.0, stage_finish(stageBufs, 0, miss_d)), s, rec);
+      int e = launch_qd_stage(ctx->c, stageBufs, 0, stage_in(&A1, 1, gp, 0.5, prev), s, rec, evt(0), evt(1));
+      if (!e)
+        e = launch_qd_stage(ctx->c, stageBufs, 1, stage_in(&A2, 2, gp, 1.0, stage_finish(stageBufs, 0, miss_d)), s,
+                            rec, evt(2), evt(3));
       if (!e) {
         QdStageIn in = stage_in(nullptr, 0, gp, 0.0, PendingFinish{});
         in.fuse = QdFuse{fault_args(varB[1 - p][0], dvarB[1 - p][0]), dvarB[1 - p][1]};
@@ -522,7 +549,7 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
         in.cdhi[0] = 6;
         in.cdhi[1] = 3;
         in.cdhi[2] = 6;
-        e = launch_qd_stage(ctx->c, stageBufs, 2, in, s, rec);
+        e = launch_qd_stage(ctx->c, stageBufs, 2, in, s, rec, evt(4), evt(5));
       }
       // the rates' residual test is left for the next launch; u for proj_materialize
       fin = stage_finish(stageBufs, 2, miss_d);
@@ -545,6 +572,7 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
   // no memset nodes.  The one-kernel loop's graph is kept across calls and
   // captured again only when the arguments of its launches change.
   int e = slot();
+  if (!e && timed) e = collect();
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   auto capture = [&](hipGraph_t* gr, hipGraphExec_t* ex) -> int {
@@ -558,7 +586,9 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
     return ec;
   };
   hipGraphExec_t run = nullptr;
-  if (!e && oneKernel) {
+  if (timed) {
+    // no graph: the attempts go in one by one
+  } else if (!e && oneKernel) {
     const int parity0 = parity;
     const PendingFinish fin0 = fin;
     const bool finPending0 = finPending, uLazy0 = uLazy;
@@ -597,13 +627,21 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
     if (e) break;
     const int left = maxSteps - ctl_h->stepCount + 1;
     const int slots = std::max(1, std::min(kSlotsPerBatch, left + 1)) - (first ? 1 : 0);
-    first = false;
-    for (int k = 0; k < (slots + kAttemptsPerGraph - 1) / kAttemptsPerGraph; ++k) {
-      if (hipGraphLaunch(run, s) != hipSuccess) {
-        e = fail(SC_ERR_HIP, "graph launch failed");
-        break;
+    if (timed) {
+      // two attempts between read-backs keep the parity of the graph batches
+      for (int k = 0; k < (first ? 0 : 2) && !e; ++k) {
+        e = slot();
+        if (!e) e = collect();
+      }
+    } else {
+      for (int k = 0; k < (slots + kAttemptsPerGraph - 1) / kAttemptsPerGraph; ++k) {
+        if (hipGraphLaunch(run, s) != hipSuccess) {
+          e = fail(SC_ERR_HIP, "graph launch failed");
+          break;
+        }
       }
     }
+    first = false;
     finPending = true;  // the last replayed attempt's rates
     if (e) break;
     SC_HIP(hipMemcpyAsync(ctl_h, ctl_d + (oneKernel ? 1 : 0), sizeof(StepCtl), hipMemcpyDeviceToHost, s));
@@ -626,6 +664,13 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
   }
   defer = false;
   if (e) return e;
+  if (timed) {  // the attempts that ran come first; later ones found the gate closed
+    const long long ran = (long long)(ctl_h->nOut - nOut0) + ctl_h->rejected + (ctl_h->halt ? 1 : 0);
+    const long long nAtt = std::min<long long>(ran, (long long)attemptMs.size() / 3);
+    for (long long a = 0; a < nAtt; ++a)
+      for (int k = 0; k < 3; ++k) timing->ms[k] += attemptMs[(size_t)(3 * a + k)];
+    timing->attempts += nAtt;
+  }
   if (oneKernel) {  // the latest state is state 1: back into var and dvar
     for (int f = 0; f < 2; ++f) {
       SC_HIP(hipMemcpyAsync(varB[0][f], varB[1][f], (size_t)Nz * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -1110,6 +1155,21 @@ extern "C" int sc_qd_destroy(sc_qd* q) {
 }
 
 extern "C" sc_ctx* sc_qd_ctx(sc_qd* q) { return q ? q->q->ctx : nullptr; }
+
+extern "C" int sc_qd_integrate_timed(sc_qd* h, const sc_qd_run* run, double* tEnd_h, int* steps_h,
+                                     double* dts_h, int dtsCap, double* ms_stage, double* ms_end,
+                                     long long* attempts) {
+  if (!h || !run || !ms_stage || !ms_end || !attempts) return fail(SC_ERR_ARG, "null argument");
+  sc::QdDriver& q = *h->q;
+  sc::QdDriver::StageTiming t;
+  q.timing = &t;
+  const int e = sc_qd_integrate(h, run, tEnd_h, steps_h, dts_h, dtsCap);
+  q.timing = nullptr;
+  *attempts = t.attempts;
+  *ms_stage = t.attempts > 0 ? (t.ms[0] + t.ms[1]) / (2.0 * t.attempts) : 0.0;
+  *ms_end = t.attempts > 0 ? t.ms[2] / t.attempts : 0.0;
+  return e;
+}
 
 extern "C" int sc_qd_bench_stage(sc_qd* h, double t, int reps, double* ms_per_launch, double* words) {
   if (!h || !ms_per_launch || !words) return fail(SC_ERR_ARG, "null argument");

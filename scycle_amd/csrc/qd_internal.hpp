@@ -102,8 +102,11 @@ PendingFinish stage_finish(const QdStageBufs& b, int pos, int* miss);
 int launch_stage_plan(const Ctx& c, QdStageBufs& b, const double* rfix, hipStream_t s);
 // rec != nullptr: append the launch's arguments (kernel, grid, argument
 // bytes) to *rec instead of launching
+// evStart / evStop (optional): events that take the kernel's own start and
+// end (hipExtLaunchKernelGGL), for per-launch timing
 int launch_qd_stage(const Ctx& c, const QdStageBufs& b, int pos, const QdStageIn& in, hipStream_t s,
-                    std::vector<unsigned char>* rec = nullptr);
+                    std::vector<unsigned char>* rec = nullptr, hipEvent_t evStart = nullptr,
+                    hipEvent_t evStop = nullptr);
 // fin (optional): a pending residual test run as one more block
 int launch_vars_lohi_err(hipStream_t s, const VarsLoHi& L, const PendingFinish* fin = nullptr,
                          StepGate gate = StepGate{});
@@ -268,6 +271,15 @@ struct QdDriver {
   std::vector<unsigned char> stageKey;
   long long graphBuilds = 0;
   void drop_stage_graph();
+  // Set during sc_qd_integrate_timed: the device loop launches its attempts
+  // one by one (no graph) with HIP events around each stage launch and sums
+  // their durations over the attempts that ran (positions 0, 1: the plain
+  // stage kernels; 2: the attempt's end).
+  struct StageTiming {
+    double ms[3] = {0, 0, 0};
+    long long attempts = 0;
+  };
+  StageTiming* timing = nullptr;
   // runs steps from the current one (r.stepCount, started) on the device
   // until the run ends (*finished) or a missed guess / fault failure halts
   // it (*halted: the host redoes the current step with checked solves)

@@ -13,6 +13,8 @@
 //                   sums, one block per short field
 //   k_step_end      the end of a device-controlled RK32 attempt: the last
 //                   stage's residual test, the pair, the step control
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -897,7 +899,7 @@ int launch_stage_plan(const Ctx& c, QdStageBufs& b, const double* rfix, hipStrea
 }
 
 int launch_qd_stage(const Ctx& c, const QdStageBufs& b, int pos, const QdStageIn& in, hipStream_t s,
-                    std::vector<unsigned char>* rec) {
+                    std::vector<unsigned char>* rec, hipEvent_t evStart, hipEvent_t evStop) {
   const int M = c.projCount, Nz = c.d.Nz;
   if (!stage_fits(c) || b.nz != Nz || pos < 0 || pos >= kStagePos)
     return fail(SC_ERR_ARG, "stage kernel: buffers or position");
@@ -975,11 +977,18 @@ int launch_qd_stage(const Ctx& c, const QdStageBufs& b, int pos, const QdStageIn
     rec->insert(rec->end(), abuf, abuf + sizeof abuf);
     return SC_OK;
   }
+  const bool ev = evStart || evStop;
   switch (M) {
-#define SC_ST(m)                                                             \
-  case m:                                                                    \
-    if (in.end) hipLaunchKernelGGL((k_qd_stage<m, true>), g, bl, 0, s, a);   \
-    else hipLaunchKernelGGL((k_qd_stage<m, false>), g, bl, 0, s, a);         \
+#define SC_ST(m)                                                                                   \
+  case m:                                                                                          \
+    if (ev) {                                                                                      \
+      if (in.end) hipExtLaunchKernelGGL((k_qd_stage<m, true>), g, bl, 0, s, evStart, evStop, 0, a); \
+      else hipExtLaunchKernelGGL((k_qd_stage<m, false>), g, bl, 0, s, evStart, evStop, 0, a);       \
+    } else if (in.end) {                                                                           \
+      hipLaunchKernelGGL((k_qd_stage<m, true>), g, bl, 0, s, a);                                   \
+    } else {                                                                                       \
+      hipLaunchKernelGGL((k_qd_stage<m, false>), g, bl, 0, s, a);                                  \
+    }                                                                                              \
     break;
     SC_ST(1) SC_ST(2) SC_ST(3) SC_ST(4) SC_ST(5) SC_ST(6) SC_ST(7) SC_ST(8)
 #undef SC_ST

@@ -944,8 +944,9 @@ static void bench_frame(const Ctx& c, const DevOp<T>& op, const T* a, T* b) {
 // kernel only.
 extern "C" int sc_bench_apply(sc_ctx* h, int which, int reps, double* ms_per_launch,
                               double* nodes) {
-  if (!h || reps <= 0 || !ms_per_launch || which < 0 || which > 8)
+  if (!h || reps <= 0 || !ms_per_launch || which < 0 || which > 10)
     return fail(SC_ERR_ARG, "sc_bench_apply: bad argument");
+  if (which >= 9 && h->c.d.laplace != 0) return fail(SC_ERR_ARG, "sc_bench_apply: the wave step needs laplace type yz");
   if (which == 8) {
     double words = 0;
     int e = sc::bench_proj_guess(h->c, reps, ms_per_launch, &words);
@@ -953,20 +954,37 @@ extern "C" int sc_bench_apply(sc_ctx* h, int which, int reps, double* ms_per_lau
     return e;
   }
   sc::Ctx& c = h->c;
-  const bool f64 = which % 2 == 0;
+  const bool f64 = which == 9 || (which < 9 && which % 2 == 0);
   const size_t N = (size_t)c.N, esz = f64 ? 8 : 4;
   const int Ny = c.d.Ny, Nz = c.d.Nz;
   const int niy = Ny - 2 * sc::kFrame, niz = Nz - 2 * sc::kFrame;
   if (which >= 2 && (niy <= 0 || niz <= 0)) return fail(SC_ERR_ARG, "grid has no interior block");
   if (nodes) {
-    if (which < 2) *nodes = (double)N;
+    if (which < 2 || which >= 9) *nodes = (double)N;
     else if (which < 6) *nodes = (double)niy * niz;
     else *nodes = (double)N - (double)niy * niz;
   }
-  void *a = nullptr, *b = nullptr;
+  // apply: a -> b; wave step (9 FP64, 10 FP32): u = a, uPrev = b, uNext = w,
+  // rho = 1 in r, ay = 0 in y (zero u: the timing does not depend on values)
+  void *a = nullptr, *b = nullptr, *w = nullptr, *r = nullptr, *y = nullptr, *d2 = nullptr;
   SC_HIP(hipMalloc(&a, N * esz));
   SC_HIP(hipMalloc(&b, N * esz));
   SC_HIP(hipMemsetAsync(a, 0, N * esz, c.stream));
+  if (which >= 9) {
+    SC_HIP(hipMalloc(&w, N * esz));
+    SC_HIP(hipMalloc(&r, N * esz));
+    SC_HIP(hipMalloc(&y, N * esz));
+    SC_HIP(hipMalloc(&d2, (size_t)Nz * esz));
+    SC_HIP(hipMemsetAsync(b, 0, N * esz, c.stream));
+    SC_HIP(hipMemsetAsync(y, 0, N * esz, c.stream));
+    if (f64) {
+      std::vector<double> one(N, 1.0);
+      SC_HIP(hipMemcpy(r, one.data(), N * esz, hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> one(N, 1.0f);
+      SC_HIP(hipMemcpy(r, one.data(), N * esz, hipMemcpyHostToDevice));
+    }
+  }
   hipEvent_t e0, e1;
   SC_HIP(hipEventCreate(&e0));
   SC_HIP(hipEventCreate(&e1));
@@ -981,7 +999,13 @@ extern "C" int sc_bench_apply(sc_ctx* h, int which, int reps, double* ms_per_lau
       case 4: sc::bench_interior<double, 0>(c, c.op64, a64, (double*)b); return 0;
       case 5: sc::bench_interior<float, 0>(c, c.op32, a32, (float*)b); return 0;
       case 6: sc::bench_frame<double>(c, c.op64, a64, (double*)b); return 0;
-      default: sc::bench_frame<float>(c, c.op32, a32, (float*)b); return 0;
+      case 7: sc::bench_frame<float>(c, c.op32, a32, (float*)b); return 0;
+      case 9:
+        return sc::launch_wave64(c, {a64, (const double*)b, (double*)w, (const double*)r, (const double*)y,
+                                     (double*)d2, 1e-3}, c.stream);
+      default:
+        return sc::launch_wave32(c, {a32, (const float*)b, (float*)w, (const float*)r, (const float*)y,
+                                     (float*)d2, 1e-3f}, c.stream);
     }
   };
   int err = 0;
@@ -996,7 +1020,7 @@ extern "C" int sc_bench_apply(sc_ctx* h, int which, int reps, double* ms_per_lau
   *ms_per_launch = ms / reps;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  (void)hipFree(a);
-  (void)hipFree(b);
+  for (void* p : {a, b, w, r, y, d2})
+    if (p) (void)hipFree(p);
   return err;
 }

@@ -109,13 +109,33 @@ class QdMediator:
         curr_err, prev_err: the PID error history at the start (a restart's
         chkpt_currErr / chkpt_prevErr, see load_checkpoint); step_control:
         "auto" (on the device where it can) or "host" (sc_qd_run.stepControl)."""
+        return self.integrate_run(self._run(max_steps, integrator, tol, init_dt, min_dt, max_dt, init_time,
+                                            max_time, control, norm, replay_dt, defer, curr_err, prev_err,
+                                            step_control))
+
+    def _run(self, max_steps, integrator="RK32", tol=1e-8, init_dt=1e-3, min_dt=0.0, max_dt=1e10,
+             init_time=0.0, max_time=1e15, control="PID", norm="L2_absolute", replay_dt=None,
+             defer="auto", curr_err=0.0, prev_err=0.0, step_control="auto"):
         rep = None if replay_dt is None else np.ascontiguousarray(replay_dt, dtype=np.float64)
-        r = QdRun(0 if integrator == "RK32" else 1, 0 if control == "P" else 1,
-                  0 if norm == "L2_absolute" else 1, int(max_steps), init_time, max_time,
-                  init_dt, min_dt, max_dt, tol, 0 if rep is None else len(rep), dptr(rep),
-                  {"auto": 0, "never": 1, "always": 2}[defer], curr_err, prev_err,
-                  {"auto": 0, "host": 1}[step_control])
-        return self.integrate_run(r)
+        self._replay_keep = rep
+        return QdRun(0 if integrator == "RK32" else 1, 0 if control == "P" else 1,
+                     0 if norm == "L2_absolute" else 1, int(max_steps), init_time, max_time,
+                     init_dt, min_dt, max_dt, tol, 0 if rep is None else len(rep), dptr(rep),
+                     {"auto": 0, "never": 1, "always": 2}[defer], curr_err, prev_err,
+                     {"auto": 0, "host": 1}[step_control])
+
+    def integrate_timed(self, max_steps, **kw):
+        """integrate() with the device loop's stage launches timed one by one
+        with HIP events (sc_qd_integrate_timed); returns (tEnd, steps, dts,
+        mean in-step ms of the plain stage kernel, of the attempt-end kernel,
+        attempts timed)."""
+        r = self._run(max_steps, **kw)
+        tEnd, steps = C.c_double(), C.c_int()
+        dts = np.zeros(max(int(r.maxSteps), 1))
+        ms_s, ms_e, att = C.c_double(), C.c_double(), C.c_longlong()
+        check(load().sc_qd_integrate_timed(self.h, C.byref(r), C.byref(tEnd), C.byref(steps), dptr(dts),
+                                           len(dts), C.byref(ms_s), C.byref(ms_e), C.byref(att)))
+        return tEnd.value, steps.value, dts[:steps.value].copy(), ms_s.value, ms_e.value, att.value
 
     def integrate_run(self, r):
         """sc_qd_integrate with a QdRun (e.g. file_run()); returns (tEnd,
