@@ -32,14 +32,59 @@ __device__ __forceinline__ double slip_law_psi(double psi, double V, double a, d
   return -absV / Dc * (f - fss);
 }
 
+// Quasi-dynamic residual F(V) = strength(V) - (tauQS - eta V)
+// (ComputeVel_qd::getResid, fault.cpp:1038-1059).  jac_mode 0 keeps the
+// reference's slope expression A*V/sqrt(B^2V^2+1)+eta; 1 uses A*B/sqrt(..)+eta.
+struct QdResid {
+  double psi, a, sN, eta, tauQS, v0;
+  int jac_mode;
+  double ex;  // exp(psi / a), fixed over the root solve
+  // the strength at the last point evaluated: strength_psi(sN, psi, V, a, v0)
+  // with the same operations, so the caller need not evaluate it again
+  mutable double strength = 0.0;
+  __device__ QdResid(double psi_, double a_, double sN_, double eta_, double tauQS_, double v0_, int jac)
+      : psi(psi_), a(a_), sN(sN_), eta(eta_), tauQS(tauQS_), v0(v0_), jac_mode(jac), ex(exp(psi_ / a_)) {}
+  __device__ double strength_at(double V) const {
+    const double z = (0.5 * V / v0) * ex;
+    // asinh(+-0) = +-0: the bracket's left end V = 0 needs no evaluation
+    return a * sN * (z == 0.0 ? z : asinh(z));
+  }
+  __device__ void operator()(double V, double* f, double* J) const {
+    strength = strength_at(V);
+    const double stress = tauQS - eta * V;
+    *f = strength - stress;
+    const double A = a * sN;
+    const double B = ex / (2. * v0);
+    const double den = sqrt(B * B * V * V + 1.);
+    *J = (jac_mode == 0 ? A * V / den : A * B / den) + eta;
+  }
+  // residual only, at a bracket end (leaves `strength` alone)
+  __device__ void at_bracket_end(double V, double* f) const { *f = strength_at(V) - (tauQS - eta * V); }
+};
+
+// the residual at a bracket end: the residual functor, or QdResid's cheaper
+// form
+template <class R>
+__device__ __forceinline__ void bracket_end_resid(const R& r, double x, double* f) {
+  double dummy;
+  r(x, f, &dummy);
+}
+__device__ __forceinline__ void bracket_end_resid(const QdResid& r, double x, double* f) {
+  r.at_bracket_end(x, f);
+}
+
 // Newton with bisection safeguard on a bracket (rootFinder.cpp:174-257);
 // R(x, &f, &J) evaluates the residual and slope.  Returns 0 when |f| < atol.
 template <class R>
 __device__ int bracketed_newton(const R& resid, double left, double right, double x0, double atol,
                                 int maxIts, double* out, int* its) {
-  double f, fp, fLeft, fRight, dummy;
-  resid(left, &fLeft, &dummy);
-  resid(right, &fRight, &dummy);
+  double f, fp, fLeft, fRight;
+  // the residuals at both bracket ends and at the start are independent:
+  // evaluated together, their latency chains overlap (the start's is used
+  // only when neither end is already a root, as in the reference)
+  bracket_end_resid(resid, left, &fLeft);
+  bracket_end_resid(resid, right, &fRight);
+  resid(x0, &f, &fp);
   *its = 0;
   if (sqrt(fLeft * fLeft) <= atol) { *out = left; return 0; }
   if (sqrt(fRight * fRight) <= atol) { *out = right; return 0; }
@@ -49,7 +94,6 @@ __device__ int bracketed_newton(const R& resid, double left, double right, doubl
   }
   int n = 0;
   double x = x0, dxOld = fabs(right - left), dx = dxOld;
-  resid(x, &f, &fp);
   while (n <= maxIts && fabs(f) >= atol) {
     if (((x - right) * fp - f) * ((x - left) * fp - f) > 0.0 || fabs(2.0 * f) > fabs(dxOld * fp)) {
       dxOld = dx;
@@ -69,36 +113,42 @@ __device__ int bracketed_newton(const R& resid, double left, double right, doubl
   return fabs(f) > atol ? 1 : 0;
 }
 
-// Quasi-dynamic residual F(V) = strength(V) - (tauQS - eta V)
-// (ComputeVel_qd::getResid, fault.cpp:1038-1059).  jac_mode 0 keeps the
-// reference's slope expression A*V/sqrt(B^2V^2+1)+eta; 1 uses A*B/sqrt(..)+eta.
-struct QdResid {
-  double psi, a, sN, eta, tauQS, v0;
-  int jac_mode;
-  double ex;  // exp(psi / a), fixed over the root solve
-  __device__ QdResid(double psi_, double a_, double sN_, double eta_, double tauQS_, double v0_, int jac)
-      : psi(psi_), a(a_), sN(sN_), eta(eta_), tauQS(tauQS_), v0(v0_), jac_mode(jac), ex(exp(psi_ / a_)) {}
-  __device__ void operator()(double V, double* f, double* J) const {
-    const double strength = a * sN * asinh((0.5 * V / v0) * ex);  // strength_psi
-    const double stress = tauQS - eta * V;
-    *f = strength - stress;
-    const double A = a * sN;
-    const double B = ex / (2. * v0);
-    const double den = sqrt(B * B * V * V + 1.);
-    *J = (jac_mode == 0 ? A * V / den : A * B / den) + eta;
-  }
+// The per-node inputs of Fault_qd::d_dt that do not depend on the stage's
+// stress, loaded ahead of the work that forms the stress.
+struct FaultNodeIn {
+  double psi, a, sN, eta, lk, V, pre, b, Dc;
 };
+__device__ __forceinline__ FaultNodeIn fault_node_load(const sc_fault_qd_args& a, int k, double psi) {
+  FaultNodeIn in;
+  in.psi = psi;
+  in.a = a.a_d[k];
+  in.sN = a.sN_d[k];
+  in.eta = a.eta_d[k];
+  in.lk = a.locked_d ? a.locked_d[k] : 0.0;
+  in.V = a.V_d[k];
+  in.pre = a.prestress_d ? a.prestress_d[k] : 0.0;
+  in.b = a.b_d[k];
+  in.Dc = a.Dc_d[k];
+  return in;
+}
 
 // Fault_qd::d_dt at fault node k (fault.cpp:861-918) from the quasi-static
-// shear stress tq0 (before the prestress): slip velocity from the root solve,
-// state rate, shear stress and strength; returns V.
-__device__ __forceinline__ double fault_qd_node(const sc_fault_qd_args& a, int k, double tq0) {
-  const double tq = tq0 + 1.0 * (a.prestress_d ? a.prestress_d[k] : 0.0);
+// shear stress tq0 (before the prestress) and the node's loaded inputs:
+// slip velocity from the root solve, state rate, shear stress and strength;
+// returns V.
+__device__ __forceinline__ double fault_qd_node_in(const sc_fault_qd_args& a, int k, double tq0,
+                                                   const FaultNodeIn& in) {
+  const double tq = tq0 + 1.0 * in.pre;
   a.tauQS_d[k] = tq;
-  const double psi = a.psi_d[k], aa = a.a_d[k], sN = a.sN_d[k], eta = a.eta_d[k];
-  const double lk = a.locked_d ? a.locked_d[k] : 0.0;
-  double V = a.V_d[k];
+  const double psi = in.psi, aa = in.a, sN = in.sN, eta = in.eta;
+  const double lk = in.lk;
+  double V = in.V;
   int bad = 0;
+  // the aging law's exponential does not depend on V: formed beside the
+  // root solve
+  const double agingA = a.stateLaw == SC_AGING_LAW ? exp((a.f0 - psi) / in.b) : 0.0;
+  double strength = 0.0;
+  bool haveStrength = false;
   if (lk > 0.5) {
     V = 0.;
   } else if (lk < -0.5) {
@@ -119,17 +169,33 @@ __device__ __forceinline__ double fault_qd_node(const sc_fault_qd_args& a, int k
       if (!isfinite(V)) V = 0.5 * (left + right);
       if (bracketed_newton(r, left, right, V, a.rootTol, a.maxNumIts, &out, &its)) bad = SC_ERR_NOCONV;
       V = out;
+      // the last point the solve evaluated is the root unless it stopped at a
+      // bracket end
+      if (out != left && out != right) {
+        strength = r.strength;
+        haveStrength = true;
+      }
     }
   }
   a.V_d[k] = V;
-  double dpsi = a.stateLaw == SC_AGING_LAW ? aging_law_psi(psi, V, a.b_d[k], a.f0, a.v0, a.Dc_d[k])
-                                           : slip_law_psi(psi, V, aa, a.b_d[k], a.f0, a.v0, a.Dc_d[k]);
+  double dpsi;
+  if (a.stateLaw == SC_AGING_LAW) {  // aging_law_psi with its exponential formed above
+    dpsi = 0.;
+    if (!isinf(agingA) && in.b > 1e-3) dpsi = (in.b * a.v0 / in.Dc) * (agingA - V / a.v0);
+  } else {
+    dpsi = slip_law_psi(psi, V, aa, in.b, a.f0, a.v0, in.Dc);
+  }
   if (!isfinite(dpsi)) bad = SC_ERR_NONFINITE;
   a.dpsi_d[k] = dpsi;
   a.tau_d[k] = -1.0 * (eta * V) + tq;
-  a.strength_d[k] = strength_psi(sN, psi, V, aa, a.v0);
+  a.strength_d[k] = haveStrength ? strength : strength_psi(sN, psi, V, aa, a.v0);
   if (bad && a.status_d) atomicOr(a.status_d, bad);
   return V;
+}
+
+// the same from memory: the node's inputs loaded here
+__device__ __forceinline__ double fault_qd_node(const sc_fault_qd_args& a, int k, double tq0) {
+  return fault_qd_node_in(a, k, tq0, fault_node_load(a, k, a.psi_d[k]));
 }
 
 }  // namespace sc

@@ -526,8 +526,9 @@ struct QdStageArgs {
 // [nfb, nfb + nstream): the residual b - sum c_m (A y_m) over the grid with
 // its r.r, b.b partials, b = aL l + aR r on the band rows; one more block:
 // the previous stage's residual test.
-template <int M, bool END>
-__global__ __launch_bounds__(kStageThreads) void k_qd_stage(QdStageArgs a) {
+// KQ: fault rows per thread in the prologue (Nz <= KQ * kStageThreads)
+template <int M, bool END, int KQ>
+__global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
   const int blk = blockIdx.x;
   if (!END && blk >= a.nfb + a.nstream) {
     if (!(a.gate.ctl && a.gate.ctl->halt)) finish_irres_block(a.fin);
@@ -564,7 +565,8 @@ __global__ __launch_bounds__(kStageThreads) void k_qd_stage(QdStageArgs a) {
   };
   const size_t iFirst = sblk >= 0 ? (size_t)sblk * kStageThreads + tid : a.N;
   load_pass(iFirst);
-  constexpr int kQ = kStageNzMax / kStageThreads;
+  constexpr int kQ = KQ;
+  static_assert(KQ * kStageThreads <= kStageNzMax, "the LDS rows hold kStageNzMax nodes");
   double sb[kQ], sf[kQ][kStageNT], pl[kQ][M], rk[kQ];
 #pragma unroll
   for (int q = 0; q < kQ; ++q) {
@@ -585,10 +587,49 @@ __global__ __launch_bounds__(kStageThreads) void k_qd_stage(QdStageArgs a) {
       for (int m = 0; m < M; ++m) pl[q][m] = a.PL[(size_t)m * Nz + k];
     }
   }
+  // fault lanes: the node's step-independent operands (tractions of the
+  // basis, friction inputs, the previous V) and the operands of the stage's
+  // psi, so that none of them is a memory round trip after the gate
+  const int kf = blk * 64 + tid;
+  const bool faultLane = sblk < 0 && tid < 64 && kf < Nz;
+  FaultNodeIn fnode{};
+  double tm[M], psiB = 0.0, psiF[kStageNT];
+#pragma unroll
+  for (int m = 0; m < M; ++m) tm[m] = 0.0;
+#pragma unroll
+  for (int u = 0; u < kStageNT; ++u) psiF[u] = 0.0;
+  if (faultLane) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) tm[m] = a.T.v[m][kf];
+    fnode = fault_node_load(a.fuse.a, kf, 0.0);
+    if (!END) {
+      if (a.L.nv) {
+        psiB = a.L.base[a.psiIdx][kf];
+#pragma unroll
+        for (int u = 0; u < kStageNT; ++u)
+          if (u < a.L.nt) psiF[u] = a.L.f[a.psiIdx][u][kf];
+      } else {
+        psiB = a.fuse.a.psi_d[kf];
+      }
+    }
+  }
+  // the coefficient step's per-basis plan sums and weights
+  double wP = 0.0, sR0 = 0.0, sR1 = 0.0;
+  if (tid < M) {
+    wP = a.w[tid];
+    sR0 = a.sR[tid];
+    sR1 = a.sR[M + tid];
+  }
   double cc[kStageNT];
 #pragma unroll
   for (int i = 0; i < kStageNT; ++i) cc[i] = a.L.c[i];
   double t = a.t0;
+  // the attempt's end: the control block's copy, the error sums, and
+  // whether the accepted attempt's controller is left to block 0's second wave
+  constexpr int kCtlW = sizeof(StepCtl) / sizeof(unsigned long long);
+  __shared__ unsigned long long cw[END ? kCtlW : 1];
+  __shared__ double sel[END ? 3 * kMaxVars : 1];
+  __shared__ int fastS;
   if constexpr (END) {
     // The attempt's end, formed by every block alike (the same loads and
     // arithmetic, so every block reaches the same decision): the last
@@ -598,10 +639,9 @@ __global__ __launch_bounds__(kStageThreads) void k_qd_stage(QdStageArgs a) {
     // it; this attempt's kernels read ctlIn), the test's scalars and flag;
     // the fault waves write var <- hi.  An accepted attempt then forms the
     // rates at its solution.
-    constexpr int kCtlW = sizeof(StepCtl) / sizeof(unsigned long long);
-    __shared__ unsigned long long cw[kCtlW];
-    __shared__ double eS[2][3][kStageThreads / 64], fS[2][kRedThreads / 64], sel[3 * kMaxVars];
+    __shared__ double eS[2][3][kStageThreads / 64], fS[2][kRedThreads / 64];
     __shared__ int flg[2], accS;
+    __shared__ double tAccS;
     StepCtl* cs = reinterpret_cast<StepCtl*>(cw);
     if (tid < kCtlW) cw[tid] = reinterpret_cast<const unsigned long long*>(a.ctlIn)[tid];
     if (tid == kCtlW) {
@@ -713,17 +753,34 @@ __global__ __launch_bounds__(kStageThreads) void k_qd_stage(QdStageArgs a) {
       }
     }
     __syncthreads();
+    // An attempt its error accepts goes on to the rates at its end, t + dt,
+    // at once; the next step size (the controller's pow chain) is formed
+    // beside them by block 0's second wave, which also writes the next
+    // control block.  A flagged, rejected or forced attempt runs the whole
+    // controller here.
     if (tid < 64) {
-      const bool ok = step_control_wave(cs, sel, flg[0] || flg[1] || miss3, blk == 0);
-      if (tid == 0) accS = ok ? 1 : 0;
+      const bool flagged = flg[0] || flg[1] || miss3;
+      if (!flagged && step_accepts(cs, sel)) {
+        if (tid == 0) {
+          accS = 1;
+          fastS = 1;
+          tAccS = cs->t + cs->dt;
+        }
+      } else {
+        const bool ok = step_control_wave(cs, sel, flagged, blk == 0);
+        if (tid == 0) {
+          accS = ok ? 1 : 0;
+          fastS = 0;
+        }
+      }
     }
     __syncthreads();
-    if (blk == 0 && tid < kCtlW) reinterpret_cast<unsigned long long*>(a.ctlOut)[tid] = cw[tid];
+    if (!fastS && blk == 0 && tid < kCtlW) reinterpret_cast<unsigned long long*>(a.ctlOut)[tid] = cw[tid];
     if (!accS) {
       pass_state();
       return;
     }
-    t = cs->t;  // the accepted step's end
+    t = fastS ? tAccS : cs->t;  // the accepted step's end
   } else {
     if (gate_closed(a.gate)) return;
     if (a.gate.ctl) {  // this attempt's step and stage time
@@ -766,8 +823,8 @@ __global__ __launch_bounds__(kStageThreads) void k_qd_stage(QdStageArgs a) {
   if (tid < M) {
     double yb = 0.0;
     for (int q = 0; q < kStageThreads / 64; ++q) yb += red[tid][q];
-    yb += a.remote ? (a.vL * t / a.fts) * a.sR[tid] + a.sR[M + tid] : a.sR[M + tid];
-    coef[tid] = -a.w[tid] * yb;
+    yb += a.remote ? (a.vL * t / a.fts) * sR0 + sR1 : sR1;
+    coef[tid] = -wP * yb;
   }
   __syncthreads();
   double c[M];
@@ -775,25 +832,38 @@ __global__ __launch_bounds__(kStageThreads) void k_qd_stage(QdStageArgs a) {
   for (int m = 0; m < M; ++m) c[m] = coef[m];
   if (sblk < 0) {
     // fault nodes: the stage state, tau = sum c_m T_m, the rates
-    const int k = blk * 64 + tid;
-    if (tid < 64 && k < Nz) {
+    const int k = kf;
+    if (faultLane) {
+      double psi = psiB;
       if (END) {  // the accepted solution: the next attempt's state
-        a.pvOut[0][k] = pS[k];
+        psi = pS[k];
+        a.pvOut[0][k] = psi;
         a.pvOut[1][k] = sS[k];
       } else if (a.L.nv) {  // psi of the stage (k_vars_lincomb's order)
-        double x = cc[0] * a.L.f[a.psiIdx][0][k] + a.L.base[a.psiIdx][k];
+        double x = cc[0] * psiF[0] + psiB;
 #pragma unroll
         for (int u = 1; u < kStageNT; ++u)
-          if (u < a.L.nt) x = x + cc[u] * a.L.f[a.psiIdx][u][k];
+          if (u < a.L.nt) x = x + cc[u] * psiF[u];
+        psi = x;
         a.L.out[a.psiIdx][k] = x;
         a.L.out[a.slipIdx][k] = sS[k];
       }
       double tq = 0.0;
 #pragma unroll
-      for (int m = 0; m < M; ++m) tq += c[m] * a.T.v[m][k];
-      a.fuse.dslip[k] = fault_qd_node(a.fuse.a, k, tq);
+      for (int m = 0; m < M; ++m) tq += c[m] * tm[m];
+      fnode.psi = psi;
+      a.fuse.dslip[k] = fault_qd_node_in(a.fuse.a, k, tq, fnode);
       a.bcL[k] = sS[k] * (1.0 / a.fts);
       if (a.remote) a.bcR[k] = rS[k];
+    }
+    if constexpr (END) {
+      // the accepted attempt's controller (the same arithmetic as every
+      // block's full path) and the next control block, beside the root solves
+      if (blk == 0 && wid == 1 && fastS) {
+        StepCtl* cs = reinterpret_cast<StepCtl*>(cw);
+        (void)step_control_wave(cs, sel, false, true);
+        if (lane == 0) *a.ctlOut = *cs;
+      }
     }
     return;
   }
@@ -978,20 +1048,28 @@ int launch_qd_stage(const Ctx& c, const QdStageBufs& b, int pos, const QdStageIn
     return SC_OK;
   }
   const bool ev = evStart || evStop;
+  const bool kq2 = Nz <= 2 * kStageThreads;
   switch (M) {
-#define SC_ST(m)                                                                                   \
-  case m:                                                                                          \
-    if (ev) {                                                                                      \
-      if (in.end) hipExtLaunchKernelGGL((k_qd_stage<m, true>), g, bl, 0, s, evStart, evStop, 0, a); \
-      else hipExtLaunchKernelGGL((k_qd_stage<m, false>), g, bl, 0, s, evStart, evStop, 0, a);       \
-    } else if (in.end) {                                                                           \
-      hipLaunchKernelGGL((k_qd_stage<m, true>), g, bl, 0, s, a);                                   \
-    } else {                                                                                       \
-      hipLaunchKernelGGL((k_qd_stage<m, false>), g, bl, 0, s, a);                                  \
-    }                                                                                              \
+#define SC_STQ(m, q)                                                                                  \
+  if (ev) {                                                                                           \
+    if (in.end) hipExtLaunchKernelGGL((k_qd_stage<m, true, q>), g, bl, 0, s, evStart, evStop, 0, a); \
+    else hipExtLaunchKernelGGL((k_qd_stage<m, false, q>), g, bl, 0, s, evStart, evStop, 0, a);       \
+  } else if (in.end) {                                                                                \
+    hipLaunchKernelGGL((k_qd_stage<m, true, q>), g, bl, 0, s, a);                                     \
+  } else {                                                                                            \
+    hipLaunchKernelGGL((k_qd_stage<m, false, q>), g, bl, 0, s, a);                                    \
+  }
+#define SC_ST(m)       \
+  case m:              \
+    if (kq2) {         \
+      SC_STQ(m, 2)     \
+    } else {           \
+      SC_STQ(m, 4)     \
+    }                  \
     break;
     SC_ST(1) SC_ST(2) SC_ST(3) SC_ST(4) SC_ST(5) SC_ST(6) SC_ST(7) SC_ST(8)
 #undef SC_ST
+#undef SC_STQ
   }
   SC_HIP(hipGetLastError());
   return SC_OK;

@@ -172,6 +172,26 @@ __device__ inline double step_size_wave(bool pCtl, int ord, double kappa, double
 // rejected attempt's retry and an accepted step's next size (computeStepSize
 // sees the same state in both), so it is computed once; an attempt accepted
 // at minDt or the attempt limit proposes again from the step it took.
+// computeError's total over the selected fields from their sums (err: 3 per
+// field)
+__device__ inline double step_error_total(const StepCtl* c, const double* err) {
+  const int ord = c->ord, nsel = c->nsel, normType = c->normType;
+  double tot = 0;
+  for (int q = 0; q < nsel; ++q) {
+    const double* v = err + 3 * q;
+    if (normType == 0) tot += sqrt(v[0]) / (sqrt((double)c->selN[q]) * 1.0);
+    else if (ord == 3) tot += sqrt(v[0]) / (sqrt(v[1]) * 1.0);
+    else tot += v[2] / 1.0;
+  }
+  return tot;
+}
+
+// the attempt is accepted by its error alone (not flagged, not forced)
+__device__ inline bool step_accepts(const StepCtl* c, const double* err) {
+  const double tot = step_error_total(c, err);
+  return c->ord == 3 ? tot <= c->tol : tot < c->tol;
+}
+
 __device__ inline bool step_control_wave(StepCtl* c, const double* err, bool flagged, bool writeDts = true) {
   const bool lead = (threadIdx.x & 63) == 0;
   if (c->halt || c->done) {
@@ -185,14 +205,8 @@ __device__ inline bool step_control_wave(StepCtl* c, const double* err, bool fla
     }
     return false;
   }
-  const int ord = c->ord, nsel = c->nsel, normType = c->normType;
-  double tot = 0;
-  for (int q = 0; q < nsel; ++q) {
-    const double* v = err + 3 * q;
-    if (normType == 0) tot += sqrt(v[0]) / (sqrt((double)c->selN[q]) * 1.0);
-    else if (ord == 3) tot += sqrt(v[0]) / (sqrt(v[1]) * 1.0);
-    else tot += v[2] / 1.0;
-  }
+  const int ord = c->ord;
+  const double tot = step_error_total(c, err);
   const double tol = c->tol, minDt = c->minDt, finalT = c->finalT;
   double t = c->t, dt = c->dt, newDt = c->newDt;
   int attempts = c->attempts + 1, rejected = c->rejected, stepCount = c->stepCount;
