@@ -526,6 +526,9 @@ struct QdStageArgs {
 // [nfb, nfb + nstream): the residual b - sum c_m (A y_m) over the grid with
 // its r.r, b.b partials, b = aL l + aR r on the band rows; one more block:
 // the previous stage's residual test.
+// the gate words a stage launch without a gate reads (never written)
+__device__ StepCtl g_no_gate;
+
 // KQ: fault rows per thread in the prologue (Nz <= KQ * kStageThreads)
 template <int M, bool END, int KQ>
 __global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
@@ -541,9 +544,36 @@ __global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int Nz = a.Nz;
   const int sblk = blk - a.nfb;  // < 0: a fault block
+  constexpr int kCtlW = sizeof(StepCtl) / sizeof(unsigned long long);
+  // The control block's words go out first: loads return in the order they
+  // were issued, so a gate read issued after the prefetches below would wait
+  // for all of them.  Plain stages read the gate's words; the attempt's end
+  // reads the whole block (one word a thread) and the flags.
+  // (loaded unconditionally, from a zeroed record when there is no gate,
+  // so that no branch makes the compiler wait for them here)
+  int gHalt = 0, gAcc = 0, gDone = 0;
+  double gDt = 0.0, gT = 0.0;
+  unsigned long long cwReg = 0;
+  int flgReg0 = 0, flgReg1 = 0;
+  if constexpr (END) {
+    if (tid < kCtlW) cwReg = reinterpret_cast<const unsigned long long*>(a.ctlIn)[tid];
+    if (tid == kCtlW) {
+      flgReg0 = a.flags[0];
+      flgReg1 = a.flags[1];
+    }
+  } else {
+    const StepCtl* g = a.gate.ctl ? a.gate.ctl : &g_no_gate;
+    gHalt = g->halt;
+    gAcc = g->accepted;
+    gDone = g->done;
+    gDt = g->dt;
+    gT = g->t;
+  }
+  __builtin_amdgcn_sched_barrier(0);
   // Every load that does not depend on the step goes out before the gate is
-  // read: the streaming blocks' first pass (A y_m, and the unit-datum rhs on
-  // band rows), and each thread's fault nodes' slip operands and PL entries.
+  // waited for: the streaming blocks' first pass (A y_m, and the unit-datum
+  // rhs on band rows), and each thread's fault nodes' slip operands and PL
+  // entries.
   constexpr int kU = M <= 2 ? 4 : (M <= 4 ? 2 : 1);
   const size_t stride = (size_t)a.nstream * kStageThreads;
   double zv[kU][M], av[kU][2];
@@ -564,7 +594,8 @@ __global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
     }
   };
   const size_t iFirst = sblk >= 0 ? (size_t)sblk * kStageThreads + tid : a.N;
-  load_pass(iFirst);
+  // (the streaming blocks' first pass waits for the gate: issued with
+  // everyone's gate reads it would queue them behind 16 MB of loads)
   constexpr int kQ = KQ;
   static_assert(KQ * kStageThreads <= kStageNzMax, "the LDS rows hold kStageNzMax nodes");
   double sb[kQ], sf[kQ][kStageNT], pl[kQ][M], rk[kQ];
@@ -626,7 +657,6 @@ __global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
   double t = a.t0;
   // the attempt's end: the control block's copy, the error sums, and
   // whether the accepted attempt's controller is left to block 0's second wave
-  constexpr int kCtlW = sizeof(StepCtl) / sizeof(unsigned long long);
   __shared__ unsigned long long cw[END ? kCtlW : 1];
   __shared__ double sel[END ? 3 * kMaxVars : 1];
   __shared__ int fastS;
@@ -643,10 +673,10 @@ __global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
     __shared__ int flg[2], accS;
     __shared__ double tAccS;
     StepCtl* cs = reinterpret_cast<StepCtl*>(cw);
-    if (tid < kCtlW) cw[tid] = reinterpret_cast<const unsigned long long*>(a.ctlIn)[tid];
+    if (tid < kCtlW) cw[tid] = cwReg;
     if (tid == kCtlW) {
-      flg[0] = a.flags[0];
-      flg[1] = a.flags[1];
+      flg[0] = flgReg0;
+      flg[1] = flgReg1;
     }
     double w0 = 0.0, w1 = 0.0;
     if (tid < kRedThreads)
@@ -782,10 +812,11 @@ __global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
     }
     t = fastS ? tAccS : cs->t;  // the accepted step's end
   } else {
-    if (gate_closed(a.gate)) return;
+    // gate_closed
+    if (a.gate.ctl && (gHalt || (a.gate.mode == 2 ? !gAcc : (a.gate.mode == 1 && gDone)))) return;
     if (a.gate.ctl) {  // this attempt's step and stage time
-      const double h = a.gate.ctl->dt;
-      t = a.gate.ctl->t + a.tfrac * h;
+      const double h = gDt;
+      t = gT + a.tfrac * h;
 #pragma unroll
       for (int i = 0; i < kStageNT; ++i) cc[i] = (a.L.c[i] * h) / a.L.cd[i];
     }
@@ -877,7 +908,7 @@ __global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
   if (sblk == 0 && tid < M) a.coefOut[tid] = c[tid];
   double r2 = 0.0, b2 = 0.0;
   for (size_t i0 = iFirst; i0 < a.N; i0 += kU * stride) {
-    if (i0 != iFirst) load_pass(i0);
+    load_pass(i0);
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const size_t i = i0 + u * stride;
