@@ -291,6 +291,78 @@ def test_qd_device_step_control(defer):
     assert max(e.values()) <= 1e-7, e
 
 
+@pytest.mark.parametrize("case", ["l2_relative", "p_control", "final_time", "forced_min_dt", "rejections"])
+def test_qd_device_step_control_branches(case):
+    """The device loop's step control on the branches the default run does
+    not take, against the host-controlled loop from the same state: the
+    L2_relative norm (the error sums of hi^2 as well), P control
+    throughout, a run that ends on maxTime (the last step clamped to it),
+    and, continuing a run with a tighter tolerance, rejected attempts and
+    acceptances forced at minDt (a tolerance no step down to minDt meets).
+    Step sizes agree to 1e-12, the end time to 1e-12 (exactly on maxTime)
+    and the fields to 1e-10."""
+    from scycle_amd import QdMediator
+    N = 25
+    cfg = _small_cfg(N, N, 4, True)
+    steps = 60
+    kw = dict(tol=1e-8, init_dt=1e-3, max_dt=1e10, defer="auto")
+    slip0 = cfg.slip0
+    if case == "l2_relative":  # a relative norm needs a slip away from zero
+        kw["norm"] = "L2_relative"
+        slip0 = 1e-3 * np.linspace(1.0, 0.5, N)
+    elif case == "p_control":
+        kw["control"] = "P"
+    elif case == "final_time":
+        g = QdMediator(cfg, solve="cg64", rtol=1e-13, jac_mode=0, precond="fdiag64", reuse=2)
+        g.set_state(cfg.psi0, cfg.slip0, np.zeros(N))
+        t60, _, _ = g.integrate(steps, step_control="host", **kw)
+        g.close()
+        kw["max_time"] = 0.37 * t60
+    warm = None
+    if case in ("forced_min_dt", "rejections"):
+        # both runs continue from the same state after 30 host-controlled
+        # steps (the basis then predicts the solutions, so the attempts run
+        # on the device): with a tolerance the current step size cannot meet
+        # (rejected attempts), or one no step down to min_dt meets (forced
+        # acceptances at min_dt)
+        warm = dict(tol=1e-8, init_dt=1e-3, max_dt=1e10, defer="auto", step_control="host")
+    out = {}
+    for mode in ("host", "auto"):
+        g = QdMediator(cfg, solve="cg64", rtol=1e-13, jac_mode=0, precond="fdiag64", reuse=2)
+        g.set_state(cfg.psi0, slip0, np.zeros(N))
+        if warm:
+            t1, _, d1 = g.integrate(30, **warm)
+            kw = dict(max_dt=1e10, defer="always", init_time=t1, init_dt=float(d1[-1]))
+            if case == "forced_min_dt":
+                kw.update(tol=1e-14, min_dt=0.5 * float(d1[-1]))
+            else:
+                kw.update(tol=1e-10)
+        tEnd, nst, dts = g.integrate(steps, step_control=mode, **kw)
+        out[mode] = dict(t=tEnd, n=nst, dts=dts, st=g.get_state(), u=g.get_u(), c=g.counters())
+        g.close()
+    h, d = out["host"], out["auto"]
+    print(case, "steps", d["n"], "device steps", d["c"]["device_steps"], "rejected", d["c"]["rejected_steps"],
+          h["c"]["rejected_steps"], "t", d["t"], h["t"], flush=True)
+    assert h["c"]["device_steps"] == 0 and d["c"]["device_steps"] > 0
+    assert h["n"] == d["n"] and d["c"]["rejected_steps"] == h["c"]["rejected_steps"]
+    assert np.allclose(d["dts"], h["dts"], rtol=1e-12, atol=0)
+    assert abs(d["t"] / h["t"] - 1) <= 1e-12
+    for k in ("psi", "slip", "V", "tau"):
+        ref = h["st"][k]
+        assert np.abs(d["st"][k] - ref).max() <= 1e-10 * np.abs(ref).max(), k
+    assert np.abs(d["u"] - h["u"]).max() <= 1e-10 * np.abs(h["u"]).max()
+    if case == "final_time":
+        assert d["n"] < steps and d["t"] == kw["max_time"] and h["t"] == kw["max_time"]
+    elif case == "forced_min_dt":
+        # every accepted step after the first rejections is at the minimum:
+        # the tolerance is never met
+        assert d["dts"][-1] == kw["min_dt"] and d["c"]["rejected_steps"] >= d["n"] // 2
+    elif case == "rejections":
+        assert d["c"]["rejected_steps"] > 0 and d["dts"][0] < kw["init_dt"]
+    else:
+        assert d["n"] == steps
+
+
 @pytest.mark.parametrize("N,order,var_grid", [(25, 4, True), (33, 4, True), (29, 2, False)])
 def test_qd_stage_kernel_rates(N, order, var_grid):
     """The device loop's stage kernel (k_qd_stage: stage state, projections
