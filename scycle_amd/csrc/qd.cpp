@@ -95,12 +95,13 @@ int QdDriver::init(const sc_qd_desc& d, const double* mu, const double* y, const
   tau_d = dalloc(Nz, nullptr);
   strength_d = dalloc(Nz, nullptr);
   V_d = dalloc(Nz, nullptr);
-  // one read-back block: 12 error sums, then the fault status and the
-  // missed-guess flag
+  // one read-back block: the error sums, then the fault status, the
+  // missed-guess flag and the attempt's-end status pair
   err_d = dalloc(kChkDoubles, nullptr);
   if (!err_d) return fail(SC_ERR_HIP, "device allocation failed");
   status_d = (int*)(err_d + kErrSlots);
   miss_d = status_d + 1;
+  endStatus_d = status_d + 2;
   for (const char* key : {"psi", "slip"})
     if ((e = add_var(key, Nz))) return e;
   err_part_d = dalloc((size_t)3 * kErrBlocks, nullptr);
@@ -158,9 +159,9 @@ int QdDriver::read_flags(int* status, int* miss) {
   int e = flush_finish();
   if (!e) e = materialize_u();
   if (e) return e;
-  SC_HIP(hipMemcpyAsync(host_scratch, status_d, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(host_scratch, status_d, kFlagInts * sizeof(int), hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
-  *status = ((const int*)host_scratch)[0];
+  *status = fault_status((const int*)host_scratch);
   *miss = ((const int*)host_scratch)[1];
   return SC_OK;
 }
@@ -528,9 +529,13 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
       if (!e) {
         QdStageIn in = stage_in(nullptr, 0, gp, 0.0, PendingFinish{});
         in.fuse = QdFuse{fault_args(varB[1 - p][0], dvarB[1 - p][0]), dvarB[1 - p][1]};
+        // its root solves report where the next attempt's end reads (blocks
+        // of one launch do not see one another's writes in order)
+        in.fuse.a.status_d = endStatus_d + (1 - p);
         in.end = true;
         in.ctlOut = ctl_d + (1 - p);
         in.flags = status_d;
+        in.flagEnd = endStatus_d + p;
         in.fin3 = stage_finish(stageBufs, 1, miss_d);
         for (int f = 0; f < 2; ++f) {
           in.pv[f] = varB[p][f];
@@ -698,12 +703,12 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
     if ((e = read_flags(&st, &miss))) return e;
     note_checked(true);
     if (st) {
-      (void)hipMemsetAsync(status_d, 0, 2 * sizeof(int), s);
+      (void)hipMemsetAsync(status_d, 0, kFlagInts * sizeof(int), s);
       return fail(st & SC_ERR_NONFINITE ? SC_ERR_NONFINITE : SC_ERR_NOCONV,
                   "fault root solve failed (status " + std::to_string(st) + ")");
     }
     counters.missed_guesses++;
-    SC_HIP(hipMemsetAsync(status_d, 0, 2 * sizeof(int), s));
+    SC_HIP(hipMemsetAsync(status_d, 0, kFlagInts * sizeof(int), s));
     *halted = true;
     return SC_OK;
   }
@@ -713,11 +718,11 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
 
 int QdDriver::check_status() {
   hipStream_t s = ctx->c.stream;
-  SC_HIP(hipMemcpyAsync(host_scratch, status_d, sizeof(int), hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(host_scratch, status_d, kFlagInts * sizeof(int), hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
-  int st = *(int*)host_scratch;
+  const int st = fault_status((const int*)host_scratch);
   if (st) {
-    (void)hipMemsetAsync(status_d, 0, sizeof(int), s);
+    (void)hipMemsetAsync(status_d, 0, kFlagInts * sizeof(int), s);
     return fail(st & SC_ERR_NONFINITE ? SC_ERR_NONFINITE : SC_ERR_NOCONV,
                 "fault root solve failed (status " + std::to_string(st) + ")");
   }
@@ -804,18 +809,18 @@ int QdDriver::lohi_error(const RkState& r, int ntlo, const int* filo, const doub
   SC_HIP(hipMemcpyAsync(host_scratch, err_d, kChkDoubles * sizeof(double), hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
   const double* h = (const double*)host_scratch;
-  const int status = ((const int*)(h + kErrSlots))[0];
+  const int status = fault_status((const int*)(h + kErrSlots));
   const int miss = ((const int*)(h + kErrSlots))[1];
   *missed = miss != 0;
   note_checked(*missed);
   if (miss) {
     counters.missed_guesses++;
-    SC_HIP(hipMemsetAsync(status_d, 0, 2 * sizeof(int), s));
+    SC_HIP(hipMemsetAsync(status_d, 0, kFlagInts * sizeof(int), s));
     *totErr = 0;
     return SC_OK;
   }
   if (status) {
-    (void)hipMemsetAsync(status_d, 0, sizeof(int), s);
+    (void)hipMemsetAsync(status_d, 0, kFlagInts * sizeof(int), s);
     return fail(status & SC_ERR_NONFINITE ? SC_ERR_NONFINITE : SC_ERR_NOCONV,
                 "fault root solve failed (status " + std::to_string(status) + ")");
   }
@@ -919,7 +924,7 @@ int QdDriver::integrate(const sc_qd_run& run, double* tEnd, int* steps, double* 
   // a run that ended in an error may have left the device flags set (and a
   // residual test pending)
   finPending = false;
-  SC_HIP(hipMemsetAsync(status_d, 0, 2 * sizeof(int), s));
+  SC_HIP(hipMemsetAsync(status_d, 0, kFlagInts * sizeof(int), s));
   // solves with a projected guess are checked on the device; the flag is read
   // with the error estimate of each attempt
   const bool may_defer = desc.solve.reuse > 0 && run.deferChecks != 1;
@@ -1057,7 +1062,7 @@ int QdDriver::settle(double t) {
   note_checked(missed);
   if (missed) {
     counters.missed_guesses++;
-    SC_HIP(hipMemsetAsync(status_d, 0, 2 * sizeof(int), s));
+    SC_HIP(hipMemsetAsync(status_d, 0, kFlagInts * sizeof(int), s));
     const bool d = defer;
     defer = false;
     const int e = d_dt_vars(t);

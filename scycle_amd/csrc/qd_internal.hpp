@@ -86,6 +86,9 @@ struct QdStageIn {
   bool end = false;
   StepCtl* ctlOut = nullptr;
   int* flags = nullptr;
+  // the previous attempt's end's root-solve status (read); this launch's
+  // own root solves report into the other word of the pair (fuse.a.status_d)
+  const int* flagEnd = nullptr;
   PendingFinish fin3{};
   const double* pv[2] = {};
   double* pvOut[2] = {};
@@ -171,13 +174,21 @@ struct QdDriver {
   double *rhs_d = nullptr, *u_d = nullptr;
   double *tauQS_d = nullptr, *tau_d = nullptr, *strength_d = nullptr, *V_d = nullptr;
   static constexpr int kErrSlots = 16;  // 3 error sums per selected field (<= 4), then 3 scratch
-  static constexpr int kChkDoubles = 17;  // + fault status and missed-guess flag (2 ints)
+  // + fault status, missed-guess flag and the attempt's-end status pair (4 ints)
+  static constexpr int kChkDoubles = 18;
+  static constexpr int kFlagInts = 4;
   static constexpr int kShortField = 16384;  // fields up to this length: one block each
   double* err_d = nullptr;              // the read-back block
   double* err_part_d = nullptr;         // block partials of the error sums
   static constexpr int kErrBlocks = 512;
   int* status_d = nullptr;             // inside the read-back block
   int* miss_d = nullptr;               // set by a deferred guess check that missed
+  // status_d[2 + p]: root solves of the attempt's end launched at parity 1 - p
+  // (read by the next attempt's end, never by the launch that writes it)
+  int* endStatus_d = nullptr;
+  // the fault status of a read-back of the flag words: stage launches', and
+  // both attempt's-end words
+  static int fault_status(const int* w) { return w[0] | w[2] | w[3]; }
   bool defer = false;                  // solves may defer their check (integrate only)
   // Deferring pays only while the projected guesses keep meeting the
   // tolerance: after a miss the driver checks every solve again until

@@ -513,6 +513,7 @@ struct QdStageArgs {
   const StepCtl* ctlIn;
   StepCtl* ctlOut;
   int* flags;          // fault status, missed guess
+  const int* flagEnd;  // the previous attempt's end's root-solve status
   PendingFinish fin3;  // the last stage's residual test
   const double* pv[2];      // the attempt's state (read)
   double* pvOut[2];         // the next attempt's state (written)
@@ -558,7 +559,7 @@ __global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
   if constexpr (END) {
     if (tid < kCtlW) cwReg = reinterpret_cast<const unsigned long long*>(a.ctlIn)[tid];
     if (tid == kCtlW) {
-      flgReg0 = a.flags[0];
+      flgReg0 = a.flags[0] | a.flagEnd[0];
       flgReg1 = a.flags[1];
     }
   } else {
@@ -1048,11 +1049,15 @@ int launch_qd_stage(const Ctx& c, const QdStageBufs& b, int pos, const QdStageIn
   a.t0 = in.t0;
   a.fin = in.fin;
   if (in.end) {
-    if (!in.ctlOut || !in.gate.ctl || !in.flags || !in.fin3.sc || in.fin.sc || in.L.nv)
+    if (!in.ctlOut || !in.gate.ctl || !in.flags || !in.flagEnd || !in.fin3.sc || in.fin.sc || in.L.nv)
       return fail(SC_ERR_ARG, "stage kernel: an attempt's end needs its control blocks, flags and test");
+    // its own root solves must not report into a word that it reads
+    if (in.fuse.a.status_d == in.flags || in.fuse.a.status_d == in.flagEnd)
+      return fail(SC_ERR_ARG, "stage kernel: an attempt's end reports its root solves into a word it reads");
     a.ctlIn = in.gate.ctl;
     a.ctlOut = in.ctlOut;
     a.flags = in.flags;
+    a.flagEnd = in.flagEnd;
     a.fin3 = in.fin3;
     for (int f = 0; f < 2; ++f) {
       a.pv[f] = in.pv[f];
