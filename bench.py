@@ -230,23 +230,33 @@ def oracle_c2(cfg, q, label, t0, dt0, k):
 def c1_leg(steps):
     """C1: 101^2, order 2, constant grid, mu = 36 GPa, 2000 adaptive RK32
     steps from t = 0.  GPU: the headline path (FP64 CG, FP64 preconditioner,
-    reuse 2).  CPU: the oracle with the banded Cholesky factorisation (the
-    stand-in for the reference's MUMPS Cholesky) on one thread, whole steps.
-    Accuracy: the GPU replays the oracle's step sizes, fields compared at the
-    end."""
+    reuse 2) with the exact slope dF/dV in the root solve (jac_mode 1), and
+    again with the reference's slope expression A V / sqrt(B^2 V^2 + 1) + eta
+    (jac_mode 0, fault.cpp:1051), whose Newton steps lean on the bracket's
+    bisections.  CPU: the oracle with the reference's slope and the banded
+    Cholesky factorisation (the stand-in for the reference's MUMPS Cholesky) on
+    one thread, whole steps.  Accuracy: the GPU with the reference's slope
+    replays the oracle's step sizes, fields compared at the end."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
     from scycle_amd import QdMediator
     from scycle_amd.configs import config_c1
     cfg = config_c1()
-    q = QdMediator(cfg, solve="cg64", rtol=1e-12, jac_mode=0, precond="fdiag64", reuse=2)
-    q.set_state(cfg.psi0, cfg.slip0, np.zeros(cfg.Nz))
-    q.integrate(10, tol=1e-8, init_dt=1e-3, max_dt=1e10)  # warm-up (kernel loads, graph)
-    q.set_state(cfg.psi0, cfg.slip0, np.zeros(cfg.Nz))
-    tic = time.perf_counter()
-    tg, ng, dtg = q.integrate(steps, tol=1e-8, init_dt=1e-3, max_dt=1e10)
-    sec_gpu = time.perf_counter() - tic
-    q.close()
+
+    def gpu(jac_mode):
+        q = QdMediator(cfg, solve="cg64", rtol=1e-12, jac_mode=jac_mode, precond="fdiag64", reuse=2)
+        q.set_state(cfg.psi0, cfg.slip0, np.zeros(cfg.Nz))
+        q.integrate(10, tol=1e-8, init_dt=1e-3, max_dt=1e10)  # warm-up (kernel loads, graph)
+        q.set_state(cfg.psi0, cfg.slip0, np.zeros(cfg.Nz))
+        tic = time.perf_counter()
+        tg, ng, dtg = q.integrate(steps, tol=1e-8, init_dt=1e-3, max_dt=1e10)
+        sec = time.perf_counter() - tic
+        c = q.counters()
+        q.close()
+        return tg, ng, dtg, sec, c
+
+    tg, ng, dtg, sec_gpu, cnt = gpu(1)
+    tg0, ng0, dtg0, sec_gpu0, _ = gpu(0)
     o = ol.OracleQd(cfg, solver=0, jac_mode=0)
     st = [cfg.psi0.copy(), cfg.slip0.copy()]
     V, tau, u = np.zeros(cfg.Nz), np.zeros(cfg.Nz), np.zeros(cfg.Ny * cfg.Nz)
@@ -267,12 +277,16 @@ def c1_leg(steps):
     return dict(workload=f"C1 QD {cfg.Ny}x{cfg.Nz} order-2 constant grid mu 36 GPa aging-law RK32 tol 1e-8, "
                          f"{steps} steps from t=0",
                 gpu_steps_per_s=ng / sec_gpu, gpu_steps=ng, gpu_t_end=tg,
+                gpu_device_steps=cnt["device_steps"], gpu_missed_guesses=cnt["missed_guesses"],
+                gpu_cg_iters=cnt["cg_iters"],
+                gpu_steps_per_s_reference_slope=ng0 / sec_gpu0, gpu_t_end_reference_slope=tg0,
                 cpu_steps_per_s=n / secs.value, cpu_steps=n, cpu_seconds=secs.value, cpu_t_end=tE.value,
-                cpu_solver="oracle banded Cholesky (MUMPS-Cholesky stand-in), 1 thread",
+                cpu_solver="oracle banded Cholesky (MUMPS-Cholesky stand-in), reference slope, 1 thread",
                 gpu_over_cpu=(ng / sec_gpu) / (n / secs.value),
                 replay_errors_vs_oracle=dict(slip=_rel(g["slip"], st[1]), psi=_rel(g["psi"], st[0]),
                                              V=_rel(g["V"], V), u=_rel(gu, u), t=abs(tr / tE.value - 1)),
-                free_run_dt_rel_diff=float(np.abs(dtg[:min(ng, n)] / odts[:min(ng, n)] - 1).max()))
+                free_run_dt_rel_diff=float(np.abs(dtg0[:min(ng0, n)] / odts[:min(ng0, n)] - 1).max()),
+                free_run_dt_rel_diff_exact_slope=float(np.abs(dtg[:min(ng, n)] / odts[:min(ng, n)] - 1).max()))
 
 
 def c3_leg(n, steps):

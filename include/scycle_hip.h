@@ -74,33 +74,51 @@ int sc_ctx_sync(sc_ctx* ctx);
 /* Returns the context's hipStream_t (as void*). */
 int sc_ctx_stream(sc_ctx* ctx, void** stream);
 
-/* SbpOps::updateVarCoeff (sbpOps.hpp:65; sbpOps_m_varGrid.cpp:531-560). */
+/* SbpOps::updateVarCoeff (sbpOps.hpp:63; sbpOps_m_varGrid.cpp:531-560). */
 int sc_update_var_coeff(sc_ctx* ctx, const double* mu_h);
-/* SbpOps::changeBCTypes (sbpOps.hpp:61; sbpOps_m_varGrid.cpp:187-213). */
+/* SbpOps::changeBCTypes (sbpOps.hpp:58; sbpOps_m_varGrid.cpp:187-213). */
 int sc_change_bc_types(sc_ctx* ctx, int bcR, int bcT, int bcL, int bcB);
-/* SbpOps::geth11 (sbpOps.hpp:68). */
+/* SbpOps::geth11 (sbpOps.hpp:66). */
 int sc_geth11(sc_ctx* ctx, double* h11y, double* h11z);
 /* Diagonals of SbpOps::getCoordTrans yq (Ny values) and zr (Nz values)
- * (sbpOps.hpp:81; sbpOps_m_varGrid.cpp:333-385). */
+ * (sbpOps.hpp:79; sbpOps_m_varGrid.cpp:333-385). */
 int sc_get_metric(sc_ctx* ctx, double* yq_h, double* zr_h);
 
-/* out = A*in, A = getA (sbpOps.hpp:71) as used by KSPSolve and by
+/* out = A*in, A = getA (sbpOps.hpp:69) as used by KSPSolve and by
  * propagateWaves (strikeSlip_linearElastic_fd.cpp:459-460).
  * prec = SC_FP64 (double buffers) or SC_FP32 (float buffers). */
 int sc_sbp_apply(sc_ctx* ctx, int prec, const void* in_d, void* out_d);
 /* diag(A) (FP64). */
 int sc_sbp_diag(sc_ctx* ctx, double* out_d);
-/* SbpOps::setRhs(rhs, bcL, bcR, bcT, bcB) (sbpOps.hpp:84;
+/* SbpOps::setRhs(rhs, bcL, bcR, bcT, bcB) (sbpOps.hpp:82;
  * sbpOps_m_varGrid.cpp:1046-1083).  bcL/bcR: Nz values, bcT/bcB: Ny. */
 int sc_sbp_set_rhs(sc_ctx* ctx, double* rhs_d, const double* bcL_d, const double* bcR_d,
                    const double* bcT_d, const double* bcB_d);
-/* SbpOps::Dy, muxDy, Dz, muxDz, H, Hinv (sbpOps.hpp:87-96). */
+/* SbpOps::Dy, muxDy, Dz, muxDz, H, Hinv (sbpOps.hpp:85-94). */
 int sc_sbp_dy(sc_ctx* ctx, const double* in_d, double* out_d);
 int sc_sbp_mux_dy(sc_ctx* ctx, const double* in_d, double* out_d);
 int sc_sbp_dz(sc_ctx* ctx, const double* in_d, double* out_d);
 int sc_sbp_mux_dz(sc_ctx* ctx, const double* in_d, double* out_d);
 int sc_sbp_h(sc_ctx* ctx, const double* in_d, double* out_d);
 int sc_sbp_hinv(sc_ctx* ctx, const double* in_d, double* out_d);
+/* SbpOps::Dyxmu, Dzxmu: out = Dy (mu in), Dz (mu in) (sbpOps.hpp:87,90;
+ * sbpOps_m_varGrid.cpp:1641-1660, 1704-1723; sbpOps_m_constGrid.cpp:1503+). */
+int sc_sbp_dyxmu(sc_ctx* ctx, const double* in_d, double* out_d);
+int sc_sbp_dzxmu(sc_ctx* ctx, const double* in_d, double* out_d);
+/* SbpOps::Hyinvxe0y, HyinvxeNy (sbpOps.hpp:95-96; sbpOps_m_varGrid.cpp
+ * :1760-1801; sbpOps_m_constGrid.cpp:1622+): in_d holds Nz values (one
+ * face), out_d the grid: Hy^-1 e0y in is in / (h11 dy) on the face j = 0 and
+ * zero elsewhere (j = Ny-1 for eNy). */
+int sc_sbp_hyinvx_e0y(sc_ctx* ctx, const double* in_d, double* out_d);
+int sc_sbp_hyinvx_eNy(sc_ctx* ctx, const double* in_d, double* out_d);
+/* SbpOps::HyinvxE0y, HyinvxENy, HzinvxE0z, HzinvxENz (sbpOps.hpp:97-100;
+ * sbpOps_m_varGrid.cpp:1804-1889): in_d and out_d are grid fields; the face
+ * j = 0 (j = Ny-1, k = 0, k = Nz-1) of in scaled by the 1D inverse norm's
+ * weight there, zero elsewhere. */
+int sc_sbp_hyinvx_E0y(sc_ctx* ctx, const double* in_d, double* out_d);
+int sc_sbp_hyinvx_ENy(sc_ctx* ctx, const double* in_d, double* out_d);
+int sc_sbp_hzinvx_E0z(sc_ctx* ctx, const double* in_d, double* out_d);
+int sc_sbp_hzinvx_ENz(sc_ctx* ctx, const double* in_d, double* out_d);
 
 /* Measurement: mean duration (ms) of `reps` back-to-back launches timed with
  * HIP events on the context stream, and the nodes they write.  which:

@@ -96,6 +96,11 @@ void orc_sbp_muxdy(void* h, const double* in, double* out) { sbp2d_muxDy(((OrcSb
 void orc_sbp_muxdz(void* h, const double* in, double* out) { sbp2d_muxDz(((OrcSbp*)h)->s, in, out); }
 void orc_sbp_dy(void* h, const double* in, double* out) { sbp2d_Dy(((OrcSbp*)h)->s, in, out); }
 void orc_sbp_dz(void* h, const double* in, double* out) { sbp2d_Dz(((OrcSbp*)h)->s, in, out); }
+void orc_sbp_dyxmu(void* h, const double* in, double* out) { sbp2d_Dyxmu(((OrcSbp*)h)->s, in, out); }
+void orc_sbp_dzxmu(void* h, const double* in, double* out) { sbp2d_Dzxmu(((OrcSbp*)h)->s, in, out); }
+void orc_sbp_bnd(void* h, int which, const double* in, double* out) {
+  sbp2d_bnd(((OrcSbp*)h)->s, which, in, out);
+}
 void orc_sbp_h(void* h, const double* in, double* out) { spmv(((OrcSbp*)h)->s.H, in, out); }
 void orc_sbp_hinv(void* h, const double* in, double* out) { spmv(((OrcSbp*)h)->s.Hinv, in, out); }
 void orc_sbp_diag(void* h, double* out) {

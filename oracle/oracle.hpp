@@ -130,6 +130,9 @@ void sbp2d_muxDy(const Sbp2D& s, const double* in, double* out);
 void sbp2d_muxDz(const Sbp2D& s, const double* in, double* out);
 void sbp2d_Dy(const Sbp2D& s, const double* in, double* out);
 void sbp2d_Dz(const Sbp2D& s, const double* in, double* out);
+void sbp2d_Dyxmu(const Sbp2D& s, const double* in, double* out);
+void sbp2d_Dzxmu(const Sbp2D& s, const double* in, double* out);
+void sbp2d_bnd(const Sbp2D& s, int which, const double* in, double* out);
 
 // ----------------------------------------------------------- solvers -----
 struct PcgStats { int iters = 0; double relres = 0; };

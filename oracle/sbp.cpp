@@ -546,5 +546,27 @@ void sbp2d_muxDz(const Sbp2D& s, const double* in, double* out) {
   spmv(s.Iy_Dz, in, t.data());
   spmv(s.muM, t.data(), out);
 }
+// Dyxmu, Dzxmu: mu first, then the derivative (sbpOps_m_varGrid.cpp:1641-1660,
+// 1704-1723)
+void sbp2d_Dyxmu(const Sbp2D& s, const double* in, double* out) {
+  std::vector<double> t(s.N());
+  spmv(s.muM, in, t.data());
+  spmv(s.Dy_Iz, t.data(), out);
+}
+void sbp2d_Dzxmu(const Sbp2D& s, const double* in, double* out) {
+  std::vector<double> t(s.N());
+  spmv(s.muM, in, t.data());
+  spmv(s.Iy_Dz, t.data(), out);
+}
+// Hyinvxe0y ... HzinvxENz: the boundary restriction, then the Kronecker
+// inverse 1-D norm (sbpOps_m_varGrid.cpp:1760-1889).  which 0, 1 take Nz
+// values (e0y, eNy); 2..5 a grid field (E0y, ENy, E0z, ENz).
+void sbp2d_bnd(const Sbp2D& s, int which, const double* in, double* out) {
+  const Csr* e[6] = {&s.e0y_Iz, &s.eNy_Iz, &s.E0y_Iz, &s.ENy_Iz, &s.Iy_E0z, &s.Iy_ENz};
+  const Csr& Hi = which < 4 ? s.Hyinv_Iz : s.Iy_Hzinv;
+  std::vector<double> t(s.N());
+  spmv(*e[which], in, t.data());
+  spmv(Hi, t.data(), out);
+}
 
 }  // namespace orc

@@ -159,6 +159,14 @@ SIGNATURES = {
     "sc_sbp_mux_dz": (C.c_int, [_vp, _vp, _vp]),
     "sc_sbp_h": (C.c_int, [_vp, _vp, _vp]),
     "sc_sbp_hinv": (C.c_int, [_vp, _vp, _vp]),
+    "sc_sbp_dyxmu": (C.c_int, [_vp, _vp, _vp]),
+    "sc_sbp_dzxmu": (C.c_int, [_vp, _vp, _vp]),
+    "sc_sbp_hyinvx_e0y": (C.c_int, [_vp, _vp, _vp]),
+    "sc_sbp_hyinvx_eNy": (C.c_int, [_vp, _vp, _vp]),
+    "sc_sbp_hyinvx_E0y": (C.c_int, [_vp, _vp, _vp]),
+    "sc_sbp_hyinvx_ENy": (C.c_int, [_vp, _vp, _vp]),
+    "sc_sbp_hzinvx_E0z": (C.c_int, [_vp, _vp, _vp]),
+    "sc_sbp_hzinvx_ENz": (C.c_int, [_vp, _vp, _vp]),
     "sc_bench_apply": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
     "sc_qd_bench_stage": (C.c_int, [_vp, C.c_double, C.c_int, _dp, _dp]),
     "sc_qd_integrate_timed": (C.c_int, [_vp, C.POINTER(QdRun), _dp, C.POINTER(C.c_int), _dp, C.c_int,

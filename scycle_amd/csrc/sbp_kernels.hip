@@ -832,9 +832,51 @@ __global__ __launch_bounds__(256) void k_deriv(DevOp<double> p, int which,
       break;
     }
     case 4: v = (p.y.H[j] * p.z.H[k]) * u[I]; break;
-    default: v = (p.y.Hinv[j] * p.z.Hinv[k]) * u[I]; break;
+    case 5: v = (p.y.Hinv[j] * p.z.Hinv[k]) * u[I]; break;
+    // Dy (mu u), Dz (mu u): the product formed at each stencil point
+    case 6: {
+      v = p.y.imet[j] * d1_row(p.y, j, [&](int m) {
+            const size_t i = (size_t)m * Nz + k;
+            return p.mu[i] * u[i];
+          });
+      break;
+    }
+    default: {
+      v = p.z.imet[k] * d1_row(p.z, k, [&](int m) {
+            const size_t i = (size_t)j * Nz + m;
+            return p.mu[i] * u[i];
+          });
+      break;
+    }
   }
   out[I] = v;
+}
+
+// Hy^-1 e0y etc.: the 1D inverse norm is diagonal, so each operator keeps one
+// face of the grid, scaled by that face's inverse weight, and zeroes the rest
+__global__ __launch_bounds__(256) void k_bnd(DevOp<double> p, int which, const double* __restrict__ in,
+                                             double* __restrict__ out) {
+  const size_t I = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int Ny = p.Ny, Nz = p.Nz;
+  if (I >= (size_t)Ny * Nz) return;
+  const int j = (int)(I / Nz), k = (int)(I % Nz);
+  double v = 0.0;
+  switch (which) {
+    case 0: if (j == 0) v = p.y.Hinv[0] * in[k]; break;
+    case 1: if (j == Ny - 1) v = p.y.Hinv[Ny - 1] * in[k]; break;
+    case 2: if (j == 0) v = p.y.Hinv[0] * in[I]; break;
+    case 3: if (j == Ny - 1) v = p.y.Hinv[Ny - 1] * in[I]; break;
+    case 4: if (k == 0) v = p.z.Hinv[0] * in[I]; break;
+    default: if (k == Nz - 1) v = p.z.Hinv[Nz - 1] * in[I]; break;
+  }
+  out[I] = v;
+}
+
+int launch_bnd(const Ctx& c, int which, const double* in, double* out, hipStream_t s) {
+  const size_t N = (size_t)c.N;
+  hipLaunchKernelGGL(k_bnd, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, c.op64, which, in, out);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
 }
 
 int launch_deriv(const Ctx& c, int which, const double* in, double* out, hipStream_t s) {
@@ -895,6 +937,20 @@ extern "C" int sc_sbp_dz(sc_ctx* h, const double* in, double* out) { return deri
 extern "C" int sc_sbp_mux_dz(sc_ctx* h, const double* in, double* out) { return deriv(h, 3, in, out); }
 extern "C" int sc_sbp_h(sc_ctx* h, const double* in, double* out) { return deriv(h, 4, in, out); }
 extern "C" int sc_sbp_hinv(sc_ctx* h, const double* in, double* out) { return deriv(h, 5, in, out); }
+extern "C" int sc_sbp_dyxmu(sc_ctx* h, const double* in, double* out) { return deriv(h, 6, in, out); }
+extern "C" int sc_sbp_dzxmu(sc_ctx* h, const double* in, double* out) { return deriv(h, 7, in, out); }
+
+static int bnd(sc_ctx* h, int which, const double* in, double* out) {
+  if (!h || !in || !out) return fail(SC_ERR_ARG, "null argument");
+  if (in == out) return fail(SC_ERR_ARG, "in and out must differ");
+  return sc::launch_bnd(h->c, which, in, out, h->c.stream);
+}
+extern "C" int sc_sbp_hyinvx_e0y(sc_ctx* h, const double* in, double* out) { return bnd(h, 0, in, out); }
+extern "C" int sc_sbp_hyinvx_eNy(sc_ctx* h, const double* in, double* out) { return bnd(h, 1, in, out); }
+extern "C" int sc_sbp_hyinvx_E0y(sc_ctx* h, const double* in, double* out) { return bnd(h, 2, in, out); }
+extern "C" int sc_sbp_hyinvx_ENy(sc_ctx* h, const double* in, double* out) { return bnd(h, 3, in, out); }
+extern "C" int sc_sbp_hzinvx_E0z(sc_ctx* h, const double* in, double* out) { return bnd(h, 4, in, out); }
+extern "C" int sc_sbp_hzinvx_ENz(sc_ctx* h, const double* in, double* out) { return bnd(h, 5, in, out); }
 
 extern "C" int sc_wave_step(sc_ctx* h, int prec, double dt, const void* u, const void* uPrev,
                             void* uNext, const void* rho, const void* ay, void* d2uFault) {

@@ -151,8 +151,12 @@ int launch_wave32(const Ctx& c, const WaveArgs<float>& a, hipStream_t s);
 int ctx_build_frame(Ctx& c);
 int launch_set_rhs(const Ctx& c, double* rhs, const double* bcL, const double* bcR,
                    const double* bcT, const double* bcB, hipStream_t s, bool frame_only = false);
-// which: 0 Dy, 1 muxDy, 2 Dz, 3 muxDz, 4 H, 5 Hinv
+// which: 0 Dy, 1 muxDy, 2 Dz, 3 muxDz, 4 H, 5 Hinv, 6 Dyxmu, 7 Dzxmu
 int launch_deriv(const Ctx& c, int which, const double* in, double* out, hipStream_t s);
+// the boundary restrictions scaled by the 1D inverse norm, out on the grid:
+// 0 Hy^-1 e0y, 1 Hy^-1 eNy (in: Nz values of a face), 2 Hy^-1 E0y,
+// 3 Hy^-1 ENy, 4 Hz^-1 E0z, 5 Hz^-1 ENz (in: a grid field)
+int launch_bnd(const Ctx& c, int which, const double* in, double* out, hipStream_t s);
 // sxy on the fault line j = 0 only: out[k] = mu(0,k)*qy[0]*(D1y u)(0,k)
 int launch_fault_sxy(const Ctx& c, const double* u, double* out, hipStream_t s);
 

@@ -77,6 +77,37 @@ class SbpOps:
     def muxDz(self, u):
         return self._unary(load().sc_sbp_mux_dz, u)
 
+    def Dyxmu(self, u):
+        return self._unary(load().sc_sbp_dyxmu, u)
+
+    def Dzxmu(self, u):
+        return self._unary(load().sc_sbp_dzxmu, u)
+
+    def _face(self, fn, face):
+        """fn on Nz values of a face (Hyinvxe0y, HyinvxeNy): a grid field."""
+        a = DeviceArray(self.Nz, host=np.ascontiguousarray(face, dtype=np.float64))
+        check(fn(self.h, a.ptr, self._b.ptr))
+        self.sync()
+        return self._b.to_host()
+
+    def Hyinvxe0y(self, face):
+        return self._face(load().sc_sbp_hyinvx_e0y, face)
+
+    def HyinvxeNy(self, face):
+        return self._face(load().sc_sbp_hyinvx_eNy, face)
+
+    def HyinvxE0y(self, u):
+        return self._unary(load().sc_sbp_hyinvx_E0y, u)
+
+    def HyinvxENy(self, u):
+        return self._unary(load().sc_sbp_hyinvx_ENy, u)
+
+    def HzinvxE0z(self, u):
+        return self._unary(load().sc_sbp_hzinvx_E0z, u)
+
+    def HzinvxENz(self, u):
+        return self._unary(load().sc_sbp_hzinvx_ENz, u)
+
     def H(self, u):
         return self._unary(load().sc_sbp_h, u)
 

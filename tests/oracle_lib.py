@@ -33,8 +33,9 @@ def lib():
         L.orc_sbp_shape.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_sbp_csr.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_int), _dp]
         for name in ("orc_sbp_apply", "orc_sbp_muxdy", "orc_sbp_muxdz", "orc_sbp_dy",
-                     "orc_sbp_dz", "orc_sbp_h", "orc_sbp_hinv"):
+                     "orc_sbp_dz", "orc_sbp_h", "orc_sbp_hinv", "orc_sbp_dyxmu", "orc_sbp_dzxmu"):
             getattr(L, name).argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_sbp_bnd.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.orc_sbp_set_rhs.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
         L.orc_sbp_diag.argtypes = [C.c_void_p, _dp]
         L.orc_sbp_metric.argtypes = [C.c_void_p, _dp, _dp]
@@ -153,6 +154,20 @@ class OracleSbp:
 
     def dz(self, u):
         return self._unary(lib().orc_sbp_dz, u)
+
+    def dyxmu(self, u):
+        return self._unary(lib().orc_sbp_dyxmu, u)
+
+    def dzxmu(self, u):
+        return self._unary(lib().orc_sbp_dzxmu, u)
+
+    def bnd(self, which, x):
+        """Hy^-1 e0y, Hy^-1 eNy (x: Nz face values), Hy^-1 E0y, Hy^-1 ENy,
+        Hz^-1 E0z, Hz^-1 ENz (x: a grid field), for which = 0..5."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(self.N)
+        lib().orc_sbp_bnd(self.h, int(which), ptr(x), ptr(out))
+        return out
 
     def H(self, u):
         return self._unary(lib().orc_sbp_h, u)

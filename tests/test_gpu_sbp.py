@@ -96,6 +96,25 @@ def test_rhs_derivs_diag_parity(order, var_grid):
         assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max(), name
 
 
+@pytest.mark.parametrize("order,var_grid", [(2, False), (2, True), (4, True), (4, False)])
+def test_dyxmu_and_boundary_helpers_parity(order, var_grid):
+    """Dyxmu, Dzxmu and the six boundary inverse-norm helpers
+    (sbpOps.hpp:87-100) against the oracle on random fields."""
+    Ny, Nz = 33, 26
+    c = _case(order, var_grid, Ny, Nz, (0, 1, 0, 1), seed=13)
+    g, o = _pair(c)
+    rng = c["rng"]
+    u = rng.standard_normal(Ny * Nz)
+    x = rng.standard_normal(Nz)
+    for name, ref in (("Dyxmu", o.dyxmu(u)), ("Dzxmu", o.dzxmu(u)),
+                      ("Hyinvxe0y", o.bnd(0, x)), ("HyinvxeNy", o.bnd(1, x)),
+                      ("HyinvxE0y", o.bnd(2, u)), ("HyinvxENy", o.bnd(3, u)),
+                      ("HzinvxE0z", o.bnd(4, u)), ("HzinvxENz", o.bnd(5, u))):
+        got = getattr(g, name)(x if name in ("Hyinvxe0y", "HyinvxeNy") else u)
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max(), name
+    g.close()
+
+
 def test_change_bc_and_update_coeff():
     c = _case(4, True, 29, 31, (0, 1, 0, 1), seed=9)
     g, o = _pair(c)

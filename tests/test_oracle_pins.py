@@ -120,6 +120,66 @@ def test_dy_dz_muxdy_muxdz_of_the_coordinates(order):
     assert np.abs(o.dz(np.cos(y / 9.0))).max() < 1e-12
 
 
+@pytest.mark.parametrize("order", [2, 4])
+@pytest.mark.parametrize("var_grid", [False, True])
+def test_dyxmu_dzxmu_of_linear_coefficients(order, var_grid):
+    """Dy (mu u) and Dz (mu u) differentiate the product: with mu linear in the
+    coordinate, Dyxmu(1) is mu's slope (the metric is formed with the same
+    D1, so Dy of a linear function of y is exact on either grid), where
+    muxDy(1) is zero; and Dyxmu(g(z)) for mu = a + b y is b g(z)."""
+    Ny, Nz, Ly, Lz = 29, 33, 30.0, 25.0
+    if var_grid:
+        y, z = _stretched(Ny, Nz, Ly, Lz, 3.0, 5.0)
+    else:
+        y = np.repeat(np.linspace(0, Ly, Ny), Nz)
+        z = np.tile(np.linspace(0, Lz, Nz), Ny)
+    g = 1.0 + 0.1 * np.cos(z / 7.0)
+    for b, c in ((0.3, 0.0), (0.0, -0.2)):
+        mu = 30.0 + b * y + c * z
+        o = OracleSbp(order, Ny, Nz, Ly, Lz, mu, y if var_grid else None, z if var_grid else None,
+                      var_grid=var_grid)
+        one = np.ones(Ny * Nz)
+        assert np.abs(o.dyxmu(one) - b).max() < 1e-11
+        assert np.abs(o.dzxmu(one) - c).max() < 1e-11
+        assert np.abs(o.muxdy(one)).max() < 1e-11
+        if c == 0.0:
+            assert np.abs(o.dyxmu(g) - b * g).max() < 1e-11
+        else:
+            h = 1.0 + 0.1 * np.sin(y / 9.0)
+            assert np.abs(o.dzxmu(h) - c * h).max() < 1e-11
+
+
+@pytest.mark.parametrize("order", [2, 4])
+@pytest.mark.parametrize("var_grid", [False, True])
+def test_boundary_inverse_norm_helpers(order, var_grid):
+    """Hy^-1 e0y x is x / (h11 dy) on the face j = 0 and zero elsewhere, with
+    the published boundary weight h11 = 1/2 (order 2) or 17/48 (order 4) and
+    dy the computational spacing (Ly/(Ny-1) on a constant grid, 1/(Ny-1) on a
+    mapped one); likewise eNy on j = Ny-1, and E0y, ENy, E0z, ENz on the faces
+    of a grid field."""
+    Ny, Nz, Ly, Lz = 23, 27, 30.0, 25.0
+    y, z = _stretched(Ny, Nz, Ly, Lz, 3.0, 5.0) if var_grid else (None, None)
+    mu = np.full(Ny * Nz, 32.0)
+    o = OracleSbp(order, Ny, Nz, Ly, Lz, mu, y, z, var_grid=var_grid)
+    h11 = 0.5 if order == 2 else 17.0 / 48.0
+    dy = 1.0 / (Ny - 1) if var_grid else Ly / (Ny - 1)
+    dz = 1.0 / (Nz - 1) if var_grid else Lz / (Nz - 1)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(Nz)
+    u = rng.standard_normal(Ny * Nz)
+    U = u.reshape(Ny, Nz)
+    for which, face, src in ((0, 0, x), (1, Ny - 1, x), (2, 0, U[0]), (3, Ny - 1, U[Ny - 1])):
+        want = np.zeros((Ny, Nz))
+        want[face] = src / (h11 * dy)
+        got = o.bnd(which, x if which < 2 else u).reshape(Ny, Nz)
+        assert np.abs(got - want).max() < 1e-12 * np.abs(want).max(), which
+    for which, face in ((4, 0), (5, Nz - 1)):
+        want = np.zeros((Ny, Nz))
+        want[:, face] = U[:, face] / (h11 * dz)
+        got = o.bnd(which, u).reshape(Ny, Nz)
+        assert np.abs(got - want).max() < 1e-12 * np.abs(want).max(), which
+
+
 @pytest.mark.parametrize("order,rate", [(2, 1.35), (4, 2.0)])
 def test_dy_dz_converge_on_stretched_grids(order, rate):
     """H-norm error of Dy sin(y/5), Dz cos(z/4) on sinh grids falls at the
