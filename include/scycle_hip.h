@@ -444,8 +444,16 @@ int sc_qd_solve_ss(sc_qd* q, double* psi_h, int have_state_vals, double* slip_h)
  * Each record is of the rates at the accepted state, with any deferred guess
  * check settled first.  Series start truncated, or appended when
  * enableCheckpointing > 0 and outputDir+ckptNumber holds a number > 0
- * (Domain, domain.cpp:26-28).  o = NULL switches output off.  Not available
- * for the power-law mediator (SC_ERR_ARG). */
+ * (Domain, domain.cpp:26-28).  o = NULL switches output off.
+ * The power-law mediator (StrikeSlip_PowerLaw_qd::timeMonitor,
+ * strikeSlip_powerLaw_qd.cpp:352-408) writes the same 1-D series and, as the
+ * 2-D series, PowerLaw::writeStep2D's files (powerLaw.cpp:2312-2372):
+ * momBal_u, momBal_sxy, momBal_sxz, momBal_sdev, momBal_gTxy, momBal_gTxz,
+ * momBal_gxy, momBal_gxz, momBal_dgVxy, momBal_dgVxz, momBal_effVisc,
+ * momBal_T, disl_invEffVisc, disl_dgVdev; like the reference it replaces the
+ * strides by 1 below step 50 and by 100 from there on (a stride <= 0 still
+ * switches a series off), and it writes no checkpoints (enableCheckpointing
+ * must be 0). */
 typedef struct sc_qd_output {
   const char* outputDir;      /* prefix of every file name                   */
   int stride1D, stride2D;     /* <= 0: that series is not written            */
@@ -466,6 +474,25 @@ int sc_qd_set_output(sc_qd* q, const sc_qd_output* o);
  * count before it reaches the solver (:732-737).  *ckptNumber gets the number
  * (0: nothing loaded). */
 int sc_qd_load_checkpoint(sc_qd* q, sc_qd_run* run, int* ckptNumber);
+
+/* Output series of the fully dynamic mediator
+ * (strikeSlip_linearElastic_fd::timeMonitor, strikeSlip_linearElastic_fd.cpp
+ * :218-251, writeStep1D/2D :254-305): the same files as the QD mediator's
+ * series (med_time1D.txt, med_dt1D.txt, surfDisp, momBal_bcR/T/L/B,
+ * momBal_bcRShift, slip, slipVel, tauP, tauQSP, strength, psi, sNEff;
+ * med_time2D.txt, med_dt2D.txt, momBal_u, momBal_sxy).  As in the reference,
+ * surfDisp keeps the surface of the state given to sc_fd_set_state (the FD
+ * mediator never updates it), the boundary vectors stay zero, and tauQSP
+ * equals tauP.  enableCheckpointing must be 0 (the reference's FD mediator
+ * writes no checkpoints); o = NULL switches output off. */
+int sc_fd_set_output(sc_fd* fd, const sc_qd_output* o);
+/* OdeSolver_WaveEq::integrate (odeSolver_WaveEq.cpp:147-186): the output of
+ * the initial state (step 0), then steps of deltaT (deltaT = 0: (maxTime -
+ * initTime) / maxSteps) while stepCount < maxSteps and t < maxTime, the time
+ * clamped to maxTime, the output after each step; *tEnd, *steps at the end.
+ * Steps between two written steps run as one batch of launches. */
+int sc_fd_integrate(sc_fd* fd, double initTime, double deltaT, int maxSteps, double maxTime,
+                    double* tEnd, int* steps);
 
 /* ------------------------------------------------ QD <-> FD switching */
 /* strikeSlip_linearElastic_qd_fd (strikeSlip_linearElastic_qd_fd.cpp): QD

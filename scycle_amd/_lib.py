@@ -203,6 +203,8 @@ SIGNATURES = {
     "sc_qd_file_run": (C.c_int, [_vp, C.POINTER(QdRun)]),
     "sc_qd_solve_ss": (C.c_int, [_vp, _dp, C.c_int, _dp]),
     "sc_qd_set_output": (C.c_int, [_vp, C.POINTER(QdOutput)]),
+    "sc_fd_set_output": (C.c_int, [_vp, C.POINTER(QdOutput)]),
+    "sc_fd_integrate": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int, C.c_double, _dp, C.POINTER(C.c_int)]),
     "sc_qdfd_create": (C.c_int, [C.POINTER(QdFdDesc)] + [_dp] * 13 + [C.POINTER(_vp)]),
     "sc_qdfd_destroy": (C.c_int, [_vp]),
     "sc_qdfd_time_step": (C.c_int, [_vp, _dp]),

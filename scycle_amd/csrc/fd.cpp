@@ -58,6 +58,9 @@ extern "C" int sc_fd_set_state(sc_fd* fd, const double* uPrev, const double* u,
   if (V) SC_HIP(hipMemcpyAsync(d.V, V, Nz * 8, hipMemcpyHostToDevice, s));
   else SC_HIP(hipMemsetAsync(d.V, 0, Nz * 8, s));
   SC_HIP(hipStreamSynchronize(s));
+  // the initial state's surface displacement u(y, z = 0)
+  d.surf0.resize((size_t)d.Ny);
+  for (int j = 0; j < d.Ny; ++j) d.surf0[(size_t)j] = u[(size_t)j * Nz];
   return SC_OK;
 }
 
@@ -82,31 +85,6 @@ extern "C" int sc_fd_get_state(sc_fd* fd, double* u, double* psi, double* slip, 
   if (V) SC_HIP(hipMemcpyAsync(V, d.V, Nz * 8, hipMemcpyDeviceToHost, s));
   if (strength) SC_HIP(hipMemcpyAsync(strength, d.strength, Nz * 8, hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
-  if (tau) {
-    // tau = sxy on the fault + prestress + tau0 (strikeSlip_linearElastic_fd.cpp:425-434);
-    // sxy needs the first rows of u only, taken in FP64 for an FP32 body
-    double* tmp = nullptr;
-    const double* ud = (const double*)d.u[d.cur];
-    if (d.f32) {
-      const int nr = std::min(sc::kBand, d.Ny);
-      std::vector<double> full((size_t)d.N, 0.0);
-      e = d.downBody(full.data(), d.u[d.cur], (size_t)nr * Nz);
-      if (e) return e;
-      SC_HIP(hipMalloc(&tmp, (size_t)d.N * 8));
-      SC_HIP(hipMemcpyAsync(tmp, full.data(), (size_t)d.N * 8, hipMemcpyHostToDevice, s));
-      ud = tmp;
-    }
-    std::vector<double> sx(Nz), pre(Nz), t0(Nz);
-    e = sc::launch_fault_sxy(d.ctx->c, ud, d.sxy, s);
-    if (!e) {
-      (void)hipMemcpyAsync(sx.data(), d.sxy, Nz * 8, hipMemcpyDeviceToHost, s);
-      (void)hipMemcpyAsync(pre.data(), d.prestress, Nz * 8, hipMemcpyDeviceToHost, s);
-      (void)hipMemcpyAsync(t0.data(), d.tau0, Nz * 8, hipMemcpyDeviceToHost, s);
-    }
-    (void)hipStreamSynchronize(s);
-    if (tmp) (void)hipFree(tmp);
-    if (e) return e;
-    for (size_t k = 0; k < Nz; ++k) tau[k] = sx[k] + 1.0 * pre[k] + 1.0 * t0[k];
-  }
+  if (tau && (e = d.fault_tau(tau))) return e;
   return SC_OK;
 }

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "sc_internal.hpp"
@@ -31,6 +32,27 @@ struct FdDriver {
   double *V = nullptr, *strength = nullptr, *sxy = nullptr;
   int* status = nullptr;
   int prev = 0, cur = 1, next = 2;
+  // output series (strikeSlip_linearElastic_fd::timeMonitor): the files'
+  // directory prefix and strides, and whether each series has started
+  struct Output {
+    std::string dir;
+    int stride1D = 0, stride2D = 0;
+    bool started1D = false, started2D = false;
+  };
+  std::unique_ptr<Output> out;
+  // LinearElastic's surfDisp is set from u at construction and by computeU,
+  // which the FD mediator never calls: it keeps the initial state's surface
+  std::vector<double> surf0;
+  double* u64 = nullptr;    // FP64 copy of an FP32 body for the 2-D series
+  double* sxy2d = nullptr;  // muxDy u over the grid for the 2-D series
+
+  int set_output(const sc_qd_output* o);
+  int monitor(double t, double dt, int step, double maxTime);
+  int integrate(double t0, double dt, int maxSteps, double maxTime, double* tEnd, int* nSteps);
+  // tau = sxy on the fault + prestress + tau0 at the current level
+  int fault_tau(double* tau);
+  // the current body level in FP64 on the device (the level itself, or a copy)
+  int body64(const double** ud);
 
   ~FdDriver() {
     if (ctx) {

@@ -1,5 +1,8 @@
-// Output series and checkpoints of the linear-elastic QD mediator: the
-// reference's timeMonitor (strikeSlip_linearElastic_qd.cpp:417-463) with
+// Output series and checkpoints of the linear-elastic QD mediator, and the
+// series of the power-law QD mediator (StrikeSlip_PowerLaw_qd::timeMonitor,
+// strikeSlip_powerLaw_qd.cpp:352-408, with PowerLaw::writeStep1D/2D
+// powerLaw.cpp:2276-2372): the linear-elastic mediator's timeMonitor
+// (strikeSlip_linearElastic_qd.cpp:417-463) with
 // writeStep1D / writeStep2D (:466-525) and writeCheckpoint (:527-557), the
 // fault's and the momentum balance's series and checkpoints (Fault::writeStep
 // fault.cpp:512-557, writeCheckpoint :561-584, loadCheckpoint :144-176;
@@ -15,33 +18,10 @@
 
 #include "input.hpp"
 #include "qd_internal.hpp"
+#include "series_io.hpp"
 
 namespace sc {
 namespace {
-
-std::string fmt_e(double v) {
-  char b[64];
-  std::snprintf(b, sizeof b, "%.15e\n", v);
-  return b;
-}
-std::string fmt_i(int v) {
-  char b[32];
-  std::snprintf(b, sizeof b, "%d\n", v);
-  return b;
-}
-
-int write_text(const std::string& path, const std::string& text, bool append) {
-  std::FILE* f = std::fopen(path.c_str(), append ? "a" : "w");
-  if (!f) return fail(SC_ERR_IO, "cannot write " + path);
-  const size_t w = std::fwrite(text.data(), 1, text.size(), f);
-  std::fclose(f);
-  return w == text.size() ? SC_OK : fail(SC_ERR_IO, "short write to " + path);
-}
-
-int write_vec(const std::string& path, const std::vector<double>& v, bool append) {
-  if (write_petsc_vec(path, v.data(), (int)v.size(), append)) return fail(SC_ERR_IO, "cannot write " + path);
-  return SC_OK;
-}
 
 // loadValueFromCheckpoint (genFuncs.cpp:889-934): the first number of an
 // ASCII file; false (value kept) when the file is missing
@@ -72,12 +52,6 @@ int read_vec(const std::string& path, size_t n, std::vector<double>& out, bool* 
   return SC_OK;
 }
 
-int d2h(std::vector<double>& h, const double* d, size_t n) {
-  h.resize(n);
-  SC_HIP(hipMemcpy(h.data(), d, n * sizeof(double), hipMemcpyDeviceToHost));
-  return SC_OK;
-}
-
 int h2d(double* d, const std::vector<double>& h) {
   SC_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
   return SC_OK;
@@ -90,8 +64,10 @@ int QdDriver::set_output(const sc_qd_output* o) {
     out.reset();
     return SC_OK;
   }
-  if (pl) return fail(SC_ERR_ARG, "output series and checkpoints are not available for the power-law mediator");
   if (!o->outputDir) return fail(SC_ERR_ARG, "sc_qd_set_output: outputDir is null");
+  // StrikeSlip_PowerLaw_qd writes series only (no writeCheckpoint)
+  if (pl && o->enableCheckpointing > 0)
+    return fail(SC_ERR_ARG, "the power-law mediator writes no checkpoints");
   if (o->enableCheckpointing > 0 && o->interval < 1)
     return fail(SC_ERR_ARG, "checkpoint interval must be >= 1");
   std::unique_ptr<QdOutput> n(new QdOutput);
@@ -106,9 +82,14 @@ int QdDriver::set_output(const sc_qd_output* o) {
     n->append = n->ckptNumber > 0;
   }
   n->sxy_d = out ? out->sxy_d : nullptr;
-  if (n->stride2D > 0 && !n->sxy_d) {
+  n->plf_d = out ? out->plf_d : nullptr;
+  if (n->stride2D > 0 && !pl && !n->sxy_d) {
     n->sxy_d = dalloc((size_t)N, nullptr);
     if (!n->sxy_d) return fail(SC_ERR_HIP, "device allocation failed");
+  }
+  if (n->stride2D > 0 && pl && !n->plf_d) {
+    n->plf_d = dalloc((size_t)8 * N, nullptr);
+    if (!n->plf_d) return fail(SC_ERR_HIP, "device allocation failed");
   }
   out = std::move(n);
   return SC_OK;
@@ -117,8 +98,13 @@ int QdDriver::set_output(const sc_qd_output* o) {
 int QdDriver::monitor(double t, double dt, int step, const double* errA, double newDt, int maxSteps,
                       double maxTime, bool initial, int* stop) {
   QdOutput& o = *out;
-  const bool w1 = o.stride1D > 0 && (t == maxTime || step % o.stride1D == 0);
-  const bool w2 = o.stride2D > 0 && (t == maxTime || step % o.stride2D == 0);
+  // the power-law mediator's timeMonitor replaces both strides by 1 below
+  // step 50 and 100 from there on (strikeSlip_powerLaw_qd.cpp:367-368); a
+  // stride <= 0 still switches its series off
+  const int s1 = pl ? (step < 50 ? 1 : 100) : o.stride1D;
+  const int s2 = pl ? (step < 50 ? 1 : 100) : o.stride2D;
+  const bool w1 = o.stride1D > 0 && (t == maxTime || step % s1 == 0);
+  const bool w2 = o.stride2D > 0 && (t == maxTime || step % s2 == 0);
   const bool ck = o.ckpt > 0 && (step % o.interval == 0 || step >= maxSteps || t >= maxTime);
   if (!(w1 || w2 || ck)) return SC_OK;
   hipStream_t s = ctx->c.stream;
@@ -154,7 +140,35 @@ int QdDriver::monitor(double t, double dt, int step, const double* errA, double 
       return e;
     o.started1D = true;
   }
-  if (w2) {
+  if (w2 && pl) {
+    const bool app = o.started2D || o.append;
+    const size_t n = (size_t)N;
+    double* f = o.plf_d;
+    const PlFields pf{u_d, vars["gVxy"].var, vars["gVxz"].var, pl->T, pl->A, pl->B, pl->n, pl->cap,
+                      f, f + n, f + 2 * n, f + 3 * n, f + 4 * n, f + 5 * n, f + 6 * n, f + 7 * n};
+    if ((e = launch_pl_fields(ctx->c, pf, s))) return e;
+    SC_HIP(hipStreamSynchronize(s));
+    // PowerLaw::writeStep2D's files, with the dislocation-creep extras
+    struct Field {
+      const char* file;
+      const double* src;
+    };
+    const Field fields[] = {
+        {"momBal_u", u_d},           {"momBal_sxy", f + 2 * n},        {"momBal_sxz", f + 3 * n},
+        {"momBal_sdev", f + 4 * n},  {"momBal_gTxy", f},               {"momBal_gTxz", f + n},
+        {"momBal_gxy", vars["gVxy"].var}, {"momBal_gxz", vars["gVxz"].var},
+        {"momBal_dgVxy", vars["gVxy"].dvar}, {"momBal_dgVxz", vars["gVxz"].dvar},
+        {"momBal_effVisc", f + 5 * n}, {"momBal_T", pl->T},            {"disl_invEffVisc", f + 6 * n},
+        {"disl_dgVdev", f + 7 * n},
+    };
+    const std::string& d = o.dir;
+    if ((e = write_text(d + "med_time2D.txt", fmt_e(t), app)) || (e = write_text(d + "med_dt2D.txt", fmt_e(dt), app)))
+      return e;
+    std::vector<double> h;
+    for (const Field& fl : fields)
+      if ((e = d2h(h, fl.src, n)) || (e = write_vec(d + fl.file, h, app))) return e;
+    o.started2D = true;
+  } else if (w2) {
     const bool app = o.started2D || o.append;
     if ((e = launch_deriv(ctx->c, 1, u_d, o.sxy_d, s))) return e;
     SC_HIP(hipStreamSynchronize(s));

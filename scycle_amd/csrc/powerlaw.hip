@@ -137,6 +137,48 @@ int launch_pl_source(const Ctx& c, const double* gxy, const double* gxz, double*
   return SC_OK;
 }
 
+// The 2-D output fields of PowerLaw::writeStep2D (powerLaw.cpp:2312-2372) at
+// the current state: total strains (computeTotalStrains :1806-1820),
+// stresses and their deviatoric norm (computeStresses :1823-1887), the
+// dislocation-creep inverse viscosity and effective viscosity
+// (computeViscosity :1674-1700) and the deviatoric strain rate of the
+// dislocation creep (computeDevViscStrainRates :1919-1938).
+__global__ __launch_bounds__(256) void k_pl_fields(DevOp<double> p, PlFields f) {
+  const int Ny = p.Ny, Nz = p.Nz;
+  const long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= (long long)Ny * Nz) return;
+  const int j = (int)(I / Nz), k = (int)(I - (long long)j * Nz);
+  const double* u = f.u;
+  const double gTxy = p.y.imet[j] * d1_line(p.y.D1, Ny, j, [&](int m) { return u[(size_t)m * Nz + k]; });
+  const double gTxz = p.z.imet[k] * d1_line(p.z.D1, Nz, k, [&](int m) { return u[(size_t)j * Nz + m]; });
+  const double muI = p.mu[I];
+  const double sxy = (gTxy + (-1.0) * f.gVxy[I]) * muI;
+  const double sxz = (gTxz + (-1.0) * f.gVxz[I]) * muI;
+  double sdev = sxy * sxy;
+  sdev = sdev + 1.0 * (sxz * sxz);
+  sdev = sqrt(fabs(sdev));
+  const double inv = 1e3 * f.A[I] * pow(sdev, f.n[I] - 1.0) * exp(-f.B[I] / f.T[I]);
+  double ev = 1.0 / f.cap;
+  ev = ev + 1.0 * inv;
+  ev = 1.0 / ev;
+  const double ry = sxy * inv, rz = sxz * inv;
+  f.gTxy[I] = gTxy;
+  f.gTxz[I] = gTxz;
+  f.sxy[I] = sxy;
+  f.sxz[I] = sxz;
+  f.sdev[I] = sdev;
+  f.effVisc[I] = ev;
+  f.invDisl[I] = inv;
+  f.dgVdev[I] = sqrt(ry * ry + rz * rz);
+}
+
+int launch_pl_fields(const Ctx& c, const PlFields& f, hipStream_t s) {
+  const long long N = (long long)c.N;
+  hipLaunchKernelGGL(k_pl_fields, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, c.op64, f);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
 int launch_pl_strain(const Ctx& c, const double* u, const double* gVxy, const double* gVxz,
                      const double* T, const double* A, const double* B, const double* n,
                      const double* bcL, const double* bcR, double cap, double* dgxy, double* dgxz,

@@ -153,6 +153,7 @@ struct QdOutput {
   bool append = false;                  // series start appended (a restart), else truncated
   bool started1D = false, started2D = false;
   double* sxy_d = nullptr;              // muxDy u for the 2-D series (N)
+  double* plf_d = nullptr;              // the power law's eight 2-D fields (8 N)
 };
 
 struct QdDriver;

@@ -263,6 +263,13 @@ int proj_frame_plan(Ctx& c, const sc_solve_opts& o, ProjFramePlan* plan);
 int bench_proj_guess(Ctx& c, int reps, double* ms, double* words);
 // power law: rhs += B gxy + C gxz; strains -> stresses -> viscosity -> rates
 int launch_pl_source(const Ctx& c, const double* gxy, const double* gxz, double* rhs, hipStream_t s);
+// the power law's 2-D output fields at the current state (N each)
+struct PlFields {
+  const double *u, *gVxy, *gVxz, *T, *A, *B, *n;
+  double cap;
+  double *gTxy, *gTxz, *sxy, *sxz, *sdev, *effVisc, *invDisl, *dgVdev;
+};
+int launch_pl_fields(const Ctx& c, const PlFields& f, hipStream_t s);
 int launch_pl_strain(const Ctx& c, const double* u, const double* gVxy, const double* gVxz,
                      const double* T, const double* A, const double* B, const double* n,
                      const double* bcL, const double* bcR, double cap, double* dgxy, double* dgxz,

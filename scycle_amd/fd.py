@@ -6,7 +6,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import FdDesc, GridDesc, check, dptr, load
+from ._lib import FdDesc, GridDesc, QdOutput, check, dptr, load
 
 BULK = {"fp64": 0, "fp32": 1}
 
@@ -48,6 +48,25 @@ class FdMediator:
 
     def steps(self, n, dt, t0=0.0):
         check(load().sc_fd_steps(self.h, t0, dt, int(n)))
+
+    def set_output(self, output_dir, stride1D=1, stride2D=0):
+        """Output series written by integrate (timeMonitor of
+        strikeSlip_linearElastic_fd.cpp:218-251); output_dir is the prefix of
+        every file name.  None switches output off."""
+        if output_dir is None:
+            check(load().sc_fd_set_output(self.h, None))
+            return
+        self._out_dir = str(output_dir).encode()
+        o = QdOutput(self._out_dir, int(stride1D), int(stride2D), 0, 1)
+        check(load().sc_fd_set_output(self.h, C.byref(o)))
+
+    def integrate(self, max_steps, dt, init_time=0.0, max_time=1e15):
+        """OdeSolver_WaveEq::integrate from the current state; returns
+        (tEnd, steps)."""
+        tEnd, n = C.c_double(), C.c_int()
+        check(load().sc_fd_integrate(self.h, float(init_time), float(dt), int(max_steps), float(max_time),
+                                     C.byref(tEnd), C.byref(n)))
+        return tEnd.value, n.value
 
     def get_state(self):
         u = np.empty(self.N)
