@@ -200,9 +200,12 @@ namespace {
 // Size (in elements) of one direction's table block on the device.
 size_t dir_elems(int n) { return (size_t)n * (4 + 5 * kBand + 2) + 2 * kBand; }
 
+// false when the interior row of a line with interior tiles does not have the
+// structure the tile kernel's factored form assumes (it always has for the
+// operators built above; the check guards that form against a table change)
 template <typename T>
-void pack_dir(const Dir1D& d, const std::vector<double>& met, const std::vector<double>& imet,
-              T* base, DevDir<T>& dd) {
+bool pack_dir(const Dir1D& d, const std::vector<double>& met, const std::vector<double>& imet,
+              T* base, DevDir<T>& dd, int order) {
   const int n = d.n;
   std::vector<T> h;
   h.reserve(dir_elems(n));
@@ -232,6 +235,30 @@ void pack_dir(const Dir1D& d, const std::vector<double>& met, const std::vector<
   dd.wr = (T)d.wR[m];
   dd.w4c = (T)d.w4[m];
   dd.Hc = (T)d.H[m];
+  auto at = [&](const std::vector<double>& t, int s) { return t[(size_t)m * kBand + s + 2]; };
+  const double a = at(d.D1, 3), b = at(d.D1, 4), hc = d.H[m];
+  dd.ia = (T)a;
+  dd.ib = (T)b;
+  dd.Ha = (T)(hc * a);
+  dd.Hb = (T)(hc * b);
+  bool ok = at(d.D1, 0) == -b && at(d.D1, 1) == -a && at(d.D1, 2) == 0.0;
+  if (order == 4) {
+    const double r[5] = {0, -1, 3, -3, 1}, rt[5] = {1, -3, 3, -1, 0}, q[5] = {1, -4, 6, -4, 1};
+    for (int s = 0; s < 5; ++s)
+      ok = ok && at(d.DR, s) == r[s] && at(d.DRT, s) == rt[s] && at(d.D4, s) == q[s] && at(d.D4T, s) == q[s];
+    dd.g3s = (T)(0.5 * d.wR[m]);
+    dd.g4s = (T)d.w4[m];
+    dd.rts = (T)1;
+  } else {
+    const double s2 = at(d.DR, 1);
+    const double r[5] = {0, 1, -2, 1, 0};
+    for (int s = 0; s < 5; ++s) ok = ok && at(d.DR, s) == r[s] * s2 && at(d.DRT, s) == r[s] * s2;
+    ok = ok && b == 0.0;
+    dd.g3s = (T)(d.wR[m] * s2);
+    dd.g4s = (T)0;
+    dd.rts = (T)s2;
+  }
+  return ok || n <= 2 * kFrame;
 }
 
 }  // namespace
@@ -279,10 +306,15 @@ int ctx_upload_tables(Ctx& c) {
   const size_t ey = dir_elems(c.d.Ny), ez = dir_elems(c.d.Nz);
   if (!c.tab64) SC_HIP(hipMalloc(&c.tab64, (ey + ez) * sizeof(double)));
   if (!c.tab32) SC_HIP(hipMalloc(&c.tab32, (ey + ez) * sizeof(float)));
-  pack_dir<double>(c.ty, c.yq, c.qy, (double*)c.tab64, c.op64.y);
-  pack_dir<double>(c.tz, c.zr, c.rz, (double*)c.tab64 + ey, c.op64.z);
-  pack_dir<float>(c.ty, c.yq, c.qy, (float*)c.tab32, c.op32.y);
-  pack_dir<float>(c.tz, c.zr, c.rz, (float*)c.tab32 + ey, c.op32.z);
+  const int ord = c.d.order;
+  bool ok = pack_dir<double>(c.ty, c.yq, c.qy, (double*)c.tab64, c.op64.y, ord);
+  ok = pack_dir<double>(c.tz, c.zr, c.rz, (double*)c.tab64 + ey, c.op64.z, ord) && ok;
+  ok = pack_dir<float>(c.ty, c.yq, c.qy, (float*)c.tab32, c.op32.y, ord) && ok;
+  ok = pack_dir<float>(c.tz, c.zr, c.rz, (float*)c.tab32 + ey, c.op32.z, ord) && ok;
+  if (!ok) return fail(SC_ERR_ARG, "SBP interior stencil of an unexpected form");
+  // the tile kernel addresses the grid with 32-bit byte offsets
+  if ((double)c.d.Ny * c.d.Nz * sizeof(double) >= 2147483648.0)
+    return fail(SC_ERR_ARG, "grids of 2^28 nodes or more are not supported");
   int e = upload_closure<double>(c, c.ctab64, c.op64);
   if (!e) e = upload_closure<float>(c, c.ctab32, c.op32);
   if (e) return e;

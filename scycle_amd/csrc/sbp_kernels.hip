@@ -264,27 +264,43 @@ __global__ __launch_bounds__(256) void k_apply_frame(DevOp<T> p, FrameMap fm,
 // Whole-wave lane shifts by one (DPP wave_shr:1 / wave_shl:1 of the GFX9
 // family): lane i receives lane i-1 (up) or lane i+1 (down), as
 // __shfl_up/__shfl_down(x, 1, 64) but as a VALU operand modifier rather than
-// an LDS permute.  The end lane keeps its own value; the kernels never use
-// the values shifted in at the wave's edges.
+// an LDS permute.  The end lane receives zero (bound_ctrl, which also spares
+// the copy of the old value into the destination); the kernels never use the
+// values shifted in at the wave's edges.
 constexpr int kDppWaveShr1 = 0x138;
 constexpr int kDppWaveShl1 = 0x130;
 template <int CTRL>
 __device__ __forceinline__ float dpp_shift(float x) {
   const int v = __float_as_int(x);
-  return __int_as_float(__builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true));
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp_shift(double x) {
   const long long b = __double_as_longlong(x);
   const int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
-  const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
-  const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+  const int lo2 = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xf, 0xf, true);
+  const int hi2 = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xf, 0xf, true);
   return __longlong_as_double(((long long)hi2 << 32) | (unsigned int)lo2);
+}
+// a quad permutation (every lane of the quad has a source)
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp_quad(T x) {
+  return dpp_shift<CTRL>(x);
 }
 template <typename T>
 __device__ __forceinline__ T lane_up1(T x) { return dpp_shift<kDppWaveShr1>(x); }
 template <typename T>
 __device__ __forceinline__ T lane_down1(T x) { return dpp_shift<kDppWaveShl1>(x); }
+
+// Raw buffer access (stride 0, the range in bytes) for the tile kernel.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  if constexpr (sizeof(T) == 8) return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  else return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
 
 // ------------------------------------------------------ interior kernel ---
 // One wave per block.  Lanes cover columns [k0-4, k0+60); lanes 4..59 write
@@ -310,30 +326,45 @@ __device__ __forceinline__ T lane_down1(T x) { return dpp_shift<kDppWaveShl1>(x)
 constexpr int kCross = 17;
 constexpr int kFrameCoefs = 2 * kCross;
 
+// Four lanes per frame node: lane s of the quad takes the y-taps 0..8 (s = 0),
+// 9..16 (s = 1) or the z-taps 0..8 (s = 2), 9..16 (s = 3) of the node's two
+// 17-point stencils (18 loads a lane instead of 68 for one lane), and a quad
+// sum over two lane swaps gives lane 0 the node's value.
+constexpr int kFrameQuad = 4;
+constexpr int kDppQuadSwap1 = 0xB1;  // quad_perm [1,0,3,2]
+constexpr int kDppQuadSwap2 = 0x4E;  // quad_perm [2,3,0,1]
+
 template <typename T, class EP>
-__device__ __forceinline__ void frame_cf_node(const DevOp<T>& p, const FrameMap& fm, const T* __restrict__ coef,
-                                              const T* __restrict__ u, const EP& ep, long long t) {
+__device__ __forceinline__ void frame_cf_quad(const DevOp<T>& p, const FrameMap& fm, const T* __restrict__ coef,
+                                              const T* __restrict__ u, const EP& ep, long long t, int sub) {
   const long long nf = fm.count();
-  if (t >= nf) return;
+  const bool valid = t < nf;
+  const long long tc = valid ? t : nf - 1;  // every lane takes part in the quad sums
   int j, k;
-  fm.node(t, j, k);
+  fm.node(tc, j, k);
   const int Ny = p.Ny, Nz = p.Nz;
+  const bool ydir = sub < 2;
+  const int r0 = (sub & 1) ? 9 : 0;
+  const size_t cbase = (size_t)(ydir ? 0 : kCross) * (size_t)nf + (size_t)tc;
   T v = 0;
 #pragma unroll
-  for (int r = 0; r < kCross; ++r) {  // clamped loads, then a select: all in flight together
-    const int jj = j - 8 + r;
-    const T x = u[(size_t)min(max(jj, 0), Ny - 1) * Nz + k];
-    v += coef[(size_t)r * nf + t] * ((jj >= 0 && jj < Ny) ? x : T(0));
+  for (int i = 0; i < 9; ++i) {  // clamped loads, then selects: all in flight together
+    const int r = r0 + i;
+    const bool tap = r < kCross;
+    const int rc = tap ? r : kCross - 1;
+    const int jj = ydir ? j - 8 + rc : j;
+    const int kk = ydir ? k : k - 8 + rc;
+    const bool in = tap && jj >= 0 && jj < Ny && kk >= 0 && kk < Nz;
+    const T x = u[(size_t)min(max(jj, 0), Ny - 1) * Nz + min(max(kk, 0), Nz - 1)];
+    const T w = coef[cbase + (size_t)rc * (size_t)nf];
+    v += w * (in ? x : T(0));
   }
-  const size_t row = (size_t)j * Nz;
-#pragma unroll
-  for (int r = 0; r < kCross; ++r) {
-    const int kk = k - 8 + r;
-    const T x = u[row + min(max(kk, 0), Nz - 1)];
-    v += coef[(size_t)(kCross + r) * nf + t] * ((kk >= 0 && kk < Nz) ? x : T(0));
+  v += dpp_quad<kDppQuadSwap1>(v);
+  v += dpp_quad<kDppQuadSwap2>(v);
+  if (valid && sub == 0) {
+    const size_t I = (size_t)j * Nz + k;
+    ep(j, k, I, v, u[I], true);
   }
-  const size_t I = row + k;
-  ep(j, k, I, v, u[I], true);
 }
 
 constexpr int kJC = 8;
@@ -360,7 +391,8 @@ __global__ __launch_bounds__(64) void k_apply_interior(DevOp<T> p, const T* __re
   const int Ny = p.Ny, Nz = p.Nz;
   const int lane = threadIdx.x;
   if ((int)blockIdx.x < nfb) {
-    frame_cf_node(p, fm, fcoef, u, ep, (long long)blockIdx.x * 64 + lane);
+    frame_cf_quad(p, fm, fcoef, u, ep, (long long)blockIdx.x * (64 / kFrameQuad) + lane / kFrameQuad,
+                  lane % kFrameQuad);
     return;
   }
   const int tile = tile_of_block((int)blockIdx.x - nfb, nbx * nby, XCD);
@@ -368,87 +400,107 @@ __global__ __launch_bounds__(64) void k_apply_interior(DevOp<T> p, const T* __re
   const int k = kFrame + bx * kLanesOut - 4 + lane;
   const int j0 = kFrame + by * JC;
   const int kc = k < Nz ? k : Nz - 1;  // k >= kFrame-4 >= 0 always
+  // u and mu through buffer loads: the lane's column is the one vector
+  // offset, each row's start a scalar offset (rows clamped to the grid on the
+  // scalar side), so the loads cost no vector address arithmetic
+  const int voff = kc * (int)sizeof(T);
+  const int rowB = Nz * (int)sizeof(T);
+  const int nB = Ny * rowB;
+  const __amdgpu_buffer_rsrc_t ru = buf_rsrc(u, nB), rm = buf_rsrc(p.mu, nB);
   T uw[JC + 8];
 #pragma unroll
-  for (int r = 0; r < JC + 8; ++r) {
-    const int jj = min(j0 - 4 + r, Ny - 1);
-    uw[r] = u[(size_t)jj * Nz + kc];
-  }
+  for (int r = 0; r < JC + 8; ++r) uw[r] = buf_ld<T>(ru, voff, min(j0 - 4 + r, Ny - 1) * rowB);
   T mw[JC + 4];
 #pragma unroll
-  for (int r = 0; r < JC + 4; ++r) {
-    const int jj = min(j0 - 2 + r, Ny - 1);
-    mw[r] = p.mu[(size_t)jj * Nz + kc];
-  }
+  for (int r = 0; r < JC + 4; ++r) mw[r] = buf_ld<T>(rm, voff, min(j0 - 2 + r, Ny - 1) * rowB);
   const DevDir<T>& dy = p.y;
   const DevDir<T>& dz = p.z;
-  // ---- y-direction fluxes at rows j0-2 .. j0+JC+1
+  // ---- y-direction fluxes at rows j0-2 .. j0+JC+1 (factored interior rows:
+  // D1 u = ia (u[+1]-u[-1]) + ib (u[+2]-u[-2]), DR = [-1 3 -3 1] or the
+  // three-point D2, D4 = [1 -4 6 -4 1])
   T g1[JC + 4], g3[JC + 4], g4[JC + 4];
 #pragma unroll
   for (int r = 0; r < JC + 4; ++r) {
     const int m = min(j0 - 2 + r, Ny - 1);
     const T qm = dy.imet[m];
     const T cm = mw[r] * qm;
-    const T du = dy.d1[0] * uw[r] + dy.d1[1] * uw[r + 1] + dy.d1[3] * uw[r + 3] + dy.d1[4] * uw[r + 4];
+    T du = dy.ia * (uw[r + 3] - uw[r + 1]);
+    if (ORDER == 4) du += dy.ib * (uw[r + 4] - uw[r]);
     g1[r] = cm * du;
     if (ORDER == 4) {
       const T mnext = lane_down1(mw[r]);
-      const T c3 = T(0.5) * (cm + mnext * qm);
-      const T d3u = dy.dr[1] * uw[r + 1] + dy.dr[2] * uw[r + 2] + dy.dr[3] * uw[r + 3] + dy.dr[4] * uw[r + 4];
-      g3[r] = dy.wr * c3 * d3u;
-      const T d4u = dy.d4[0] * uw[r] + dy.d4[1] * uw[r + 1] + dy.d4[2] * uw[r + 2] +
-                    dy.d4[3] * uw[r + 3] + dy.d4[4] * uw[r + 4];
-      g4[r] = dy.w4c * cm * d4u;
+      const T c3 = mnext * qm + cm;  // twice the two-point average (the 1/2 is in g3s)
+      const T d3u = (uw[r + 4] - uw[r + 1]) + T(3) * (uw[r + 2] - uw[r + 3]);
+      g3[r] = (dy.g3s * c3) * d3u;
+      const T d4u = (uw[r] + uw[r + 4]) + T(6) * uw[r + 2] - T(4) * (uw[r + 1] + uw[r + 3]);
+      g4[r] = (dy.g4s * cm) * d4u;
     } else {
-      const T d2u = dy.dr[1] * uw[r + 1] + dy.dr[2] * uw[r + 2] + dy.dr[3] * uw[r + 3];
-      g3[r] = dy.wr * cm * d2u;
+      const T d2u = (uw[r + 1] + uw[r + 3]) - T(2) * uw[r + 2];
+      g3[r] = (dy.g3s * cm) * d2u;
       g4[r] = T(0);
     }
   }
-  const T preY = dz.H[kc] * dz.met[kc];
+  // the laplace type's direction switches as zero weights (no branches in
+  // the row loop)
+  const T preY = p.doY ? dz.H[kc] * dz.met[kc] : T(0);
   const T rzk = dz.imet[kc];
   const T rzk1 = dz.imet[kc + 1 < Nz ? kc + 1 : Nz - 1];
+  // the output rows' z-weights, loaded here with the rest (a load inside the
+  // row loop would follow its stores and wait for every load in flight)
+  T hzr[JC];
+#pragma unroll
+  for (int ro = 0; ro < JC; ++ro) {
+    const int jc = min(j0 + ro, Ny - 1);
+    hzr[ro] = p.doZ ? dy.H[jc] * dy.met[jc] : T(0);
+  }
   const bool writer = lane >= 4 && lane < 4 + kLanesOut && k < Nz - kFrame;
 #pragma unroll
   for (int ro = 0; ro < JC; ++ro) {
     const int j = j0 + ro;
     const int rr = ro + 2;
-    T v = 0;
-    if (p.doY) {
-      T Ly = dy.Hc * (dy.d1[0] * g1[rr - 2] + dy.d1[1] * g1[rr - 1] + dy.d1[3] * g1[rr + 1] + dy.d1[4] * g1[rr + 2]);
+    T v;
+    {
+      // transposed combination: H D1^T on g1, DR^T on g3, D4^T on g4
+      T Ly = dy.Ha * (g1[rr + 1] - g1[rr - 1]);
       if (ORDER == 4) {
-        Ly -= (dy.drt[0] * g3[rr - 2] + dy.drt[1] * g3[rr - 1] + dy.drt[2] * g3[rr] + dy.drt[3] * g3[rr + 1]) +
-              (dy.d4t[0] * g4[rr - 2] + dy.d4t[1] * g4[rr - 1] + dy.d4t[2] * g4[rr] +
-               dy.d4t[3] * g4[rr + 1] + dy.d4t[4] * g4[rr + 2]);
+        Ly += dy.Hb * (g1[rr + 2] - g1[rr - 2]);
+        const T r3 = (g3[rr - 2] - g3[rr + 1]) + T(3) * (g3[rr] - g3[rr - 1]);
+        const T r4 = (g4[rr - 2] + g4[rr + 2]) + T(6) * g4[rr] - T(4) * (g4[rr - 1] + g4[rr + 1]);
+        Ly -= r3 + r4;
       } else {
-        Ly -= dy.drt[1] * g3[rr - 1] + dy.drt[2] * g3[rr] + dy.drt[3] * g3[rr + 1];
+        Ly -= dy.rts * ((g3[rr - 1] + g3[rr + 1]) - T(2) * g3[rr]);
       }
       v = preY * Ly;
     }
-    if (p.doZ) {
+    {
       const T uc = uw[ro + 4];
       const T um1 = lane_up1(uc), um2 = lane_up1(um1);
       const T up1 = lane_down1(uc), up2 = lane_down1(up1);
       const T cmz = mw[ro + 2] * rzk;
-      const T g1z = cmz * (dz.d1[0] * um2 + dz.d1[1] * um1 + dz.d1[3] * up1 + dz.d1[4] * up2);
-      T g3z, g4z;
+      T duz = dz.ia * (up1 - um1);
+      if (ORDER == 4) duz += dz.ib * (up2 - um2);
+      const T g1z = cmz * duz;
+      // w_s: this lane's fluxes' share of the lane at offset 2 - s
+      T Lz;
       if (ORDER == 4) {
         const T mz1 = lane_down1(mw[ro + 2]);
-        const T c3z = T(0.5) * (cmz + mz1 * rzk1);
-        g3z = dz.wr * c3z * (dz.dr[1] * um1 + dz.dr[2] * uc + dz.dr[3] * up1 + dz.dr[4] * up2);
-        g4z = dz.w4c * cmz * (dz.d4[0] * um2 + dz.d4[1] * um1 + dz.d4[2] * uc + dz.d4[3] * up1 + dz.d4[4] * up2);
+        const T g3z = (dz.g3s * (mz1 * rzk1 + cmz)) * ((up2 - um1) + T(3) * (uc - up1));
+        const T g4z = (dz.g4s * cmz) * ((um2 + up2) + T(6) * uc - T(4) * (um1 + up1));
+        const T t3 = T(3) * g3z, t4 = T(4) * g4z;
+        const T w0 = -(dz.Hb * g1z) - (g3z + g4z);
+        const T w1 = (t3 + t4) - dz.Ha * g1z;
+        const T w2 = -(t3 + T(6) * g4z);
+        const T w3 = dz.Ha * g1z + (g3z + t4);
+        const T w4 = dz.Hb * g1z - g4z;
+        Lz = w2 + lane_up1(w1 + lane_up1(w0)) + lane_down1(w3 + lane_down1(w4));
       } else {
-        g3z = dz.wr * cmz * (dz.dr[1] * um1 + dz.dr[2] * uc + dz.dr[3] * up1);
-        g4z = T(0);
+        const T g3z = (dz.g3s * cmz) * ((um1 + up1) - T(2) * uc);
+        const T hr = dz.rts * g3z;
+        const T w1 = -(dz.Ha * g1z) - hr;
+        const T w3 = dz.Ha * g1z - hr;
+        Lz = T(2) * hr + lane_up1(w1) + lane_down1(w3);
       }
-      // contribution of this lane's fluxes to the lane at offset -t (t = s-2)
-      const T w0 = dz.Hc * dz.d1[0] * g1z - (dz.drt[0] * g3z + dz.d4t[0] * g4z);
-      const T w1 = dz.Hc * dz.d1[1] * g1z - (dz.drt[1] * g3z + dz.d4t[1] * g4z);
-      const T w2 = -(dz.drt[2] * g3z + dz.d4t[2] * g4z);
-      const T w3 = dz.Hc * dz.d1[3] * g1z - (dz.drt[3] * g3z + dz.d4t[3] * g4z);
-      const T w4 = dz.Hc * dz.d1[4] * g1z - (dz.drt[4] * g3z + dz.d4t[4] * g4z);
-      const T Lz = w2 + lane_up1(w1) + lane_up1(lane_up1(w0)) + lane_down1(w3) + lane_down1(lane_down1(w4));
-      v += (dy.H[j < Ny ? j : Ny - 1] * dy.met[j < Ny ? j : Ny - 1]) * Lz;
+      v += hzr[ro] * Lz;
     }
     if (writer && j < Ny - kFrame) ep(j, k, (size_t)j * Nz + k, v, uw[ro + 4], false);
   }
@@ -656,7 +708,8 @@ __global__ __launch_bounds__(kFrameThreads) void k_frame_coefs(DevOp<T> p, Frame
 template <typename T, class EP>
 __global__ __launch_bounds__(256) void k_apply_frame_cf(DevOp<T> p, FrameMap fm, const T* __restrict__ coef,
                                                        const T* __restrict__ u, EP ep) {
-  frame_cf_node(p, fm, coef, u, ep, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+  frame_cf_quad(p, fm, coef, u, ep, ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kFrameQuad,
+                (int)(threadIdx.x % kFrameQuad));
 }
 
 template <typename T>
@@ -707,7 +760,8 @@ static int launch_apply_t(const Ctx& c, const DevOp<T>& op, const T* in, const E
   const long long nf = fm.count();
   if (Ny > 2 * kFrame && Nz > 2 * kFrame) {  // frame and interior in one launch
     const int nbx = (Nz - 2 * kFrame + kLanesOut - 1) / kLanesOut, nby = (Ny - 2 * kFrame + kJC - 1) / kJC;
-    const int nfb = (int)((nf + 64 * kXcd - 1) / (64 * kXcd)) * kXcd;
+    const int npb = 64 / kFrameQuad;  // frame nodes per one-wave block
+    const int nfb = (int)((nf + npb * kXcd - 1) / (npb * kXcd)) * kXcd;
     const T* cf = frame_coefs<T>(c);
     const dim3 g((unsigned)(nfb + nbx * nby));
     if (c.d.order == 4)
@@ -986,7 +1040,7 @@ static void bench_interior(const Ctx& c, const DevOp<T>& op, const T* a, T* b) {
 template <typename T>
 static void bench_frame(const Ctx& c, const DevOp<T>& op, const T* a, T* b) {
   FrameMap fm = FrameMap::make(c.d.Ny, c.d.Nz);
-  const unsigned nb = (unsigned)((fm.count() + 255) / 256);
+  const unsigned nb = (unsigned)((fm.count() * kFrameQuad + 255) / 256);
   hipLaunchKernelGGL((k_apply_frame_cf<T, StoreAu<T>>), dim3(nb), dim3(256), 0, c.stream, op, fm,
                      frame_coefs<T>(c), a, StoreAu<T>{b});
 }

@@ -50,6 +50,13 @@ struct DevDir {
   T alphaD;
   // interior constants (row n/2 of the tables)
   T d1[5], dr[5], d4[5], drt[5], d4t[5], wr, w4c, Hc;
+  // the interior rows in the factored form the tile kernel evaluates (the
+  // stencils' integer parts are literals there): D1 u = ia (u[+1] - u[-1]) +
+  // ib (u[+2] - u[-2]); the same with H: Ha, Hb; the DR flux scale g3s (order
+  // 4: wR/2, the c3 average's 1/2 folded in; order 2: wR times the D2 scale),
+  // the D4 flux scale g4s = w4, and rts, the scale of the transposed D2 of
+  // order 2
+  T ia, ib, Ha, Hb, g3s, g4s, rts;
 };
 
 // Boundary-closure tables of one direction and one end, in the layout the
