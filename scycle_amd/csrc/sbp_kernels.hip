@@ -418,11 +418,15 @@ __global__ __launch_bounds__(64) void k_apply_interior(DevOp<T> p, const T* __re
   // ---- y-direction fluxes at rows j0-2 .. j0+JC+1 (factored interior rows:
   // D1 u = ia (u[+1]-u[-1]) + ib (u[+2]-u[-2]), DR = [-1 3 -3 1] or the
   // three-point D2, D4 = [1 -4 6 -4 1])
-  T g1[JC + 4], g3[JC + 4], g4[JC + 4];
+  // the flux rows' metric (loaded with the rest, before any store)
+  T qmr[JC + 4];
 #pragma unroll
-  for (int r = 0; r < JC + 4; ++r) {
-    const int m = min(j0 - 2 + r, Ny - 1);
-    const T qm = dy.imet[m];
+  for (int r = 0; r < JC + 4; ++r) qmr[r] = dy.imet[min(j0 - 2 + r, Ny - 1)];
+  T g1[JC + 4], g3[JC + 4], g4[JC + 4];
+  // flux row r (grid row j0-2+r); each is formed just before the first
+  // output row that needs it, so that only five rows of fluxes are live
+  auto flux = [&](int r) {
+    const T qm = qmr[r];
     const T cm = mw[r] * qm;
     T du = dy.ia * (uw[r + 3] - uw[r + 1]);
     if (ORDER == 4) du += dy.ib * (uw[r + 4] - uw[r]);
@@ -439,7 +443,7 @@ __global__ __launch_bounds__(64) void k_apply_interior(DevOp<T> p, const T* __re
       g3[r] = (dy.g3s * cm) * d2u;
       g4[r] = T(0);
     }
-  }
+  };
   // the laplace type's direction switches as zero weights (no branches in
   // the row loop)
   const T preY = p.doY ? dz.H[kc] * dz.met[kc] : T(0);
@@ -455,9 +459,12 @@ __global__ __launch_bounds__(64) void k_apply_interior(DevOp<T> p, const T* __re
   }
   const bool writer = lane >= 4 && lane < 4 + kLanesOut && k < Nz - kFrame;
 #pragma unroll
+  for (int r = 0; r < 4; ++r) flux(r);
+#pragma unroll
   for (int ro = 0; ro < JC; ++ro) {
     const int j = j0 + ro;
     const int rr = ro + 2;
+    flux(ro + 4);
     T v;
     {
       // transposed combination: H D1^T on g1, DR^T on g3, D4^T on g4
