@@ -291,14 +291,15 @@ def c1_leg(steps):
 
 def c3_leg(n, steps):
     """C3: 801^2 power-law viscoelastic QD (dislocation creep), 300 adaptive
-    RK32 steps on the GPU; and the pointwise creep update (strain rates and
-    effective viscosity) with FP64 and FP32 arrays on the mediator's own
-    stresses and temperature, its error in FP32 against FP64."""
+    RK32 steps on the GPU (FP64 CG, FP64 preconditioner, reuse 2, the exact
+    slope dF/dV in the root solve); and the pointwise creep update (strain
+    rates and effective viscosity) with FP64 and FP32 arrays on the
+    mediator's own stresses and temperature, its error in FP32 against FP64."""
     from scycle_amd import DeviceArray, PowerLawMediator, load
     from scycle_amd._lib import PowerLawArgs, check
     from scycle_amd.configs import config_c3
     cfg = config_c3(n, n)
-    m = PowerLawMediator(cfg, solve="cg64", rtol=1e-12, jac_mode=0, precond="fdiag64", reuse=2)
+    m = PowerLawMediator(cfg, solve="cg64", rtol=1e-12, jac_mode=1, precond="fdiag64", reuse=2)
     m.set_state(cfg.psi0, cfg.slip0, np.zeros(cfg.Nz))
     m.integrate(3, tol=1e-8, init_dt=1e-3, max_dt=1e10)
     m.set_state(cfg.psi0, cfg.slip0, np.zeros(cfg.Nz))
@@ -524,6 +525,8 @@ def main():
                     help="timed steps of the plain FP64 Jacobi-PCG path")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the FP64 apply and a calibration copy (for counter runs)")
+    ap.add_argument("--only", choices=("c1", "c3", "fd", "adjoint"),
+                    help="run this one leg and print its record (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus > 1 is not supported "
@@ -532,6 +535,12 @@ def main():
         sys.exit(2)
     if args.kernels_only:
         kernels_only(args.n, 50, args.reuse)
+        return
+    if args.only:
+        leg = {"c1": lambda: c1_leg(args.c1_steps), "c3": lambda: c3_leg(args.c3_n, args.c3_steps),
+               "fd": lambda: fd_c4(args.fd_n, args.fd_steps),
+               "adjoint": lambda: adjoint_c5(args.adj_n, args.adj_steps, args.adj_ckpt)}[args.only]
+        print(json.dumps({args.only: leg()}))
         return
     from scycle_amd._lib import load
     from scycle_amd.configs import config_c2
