@@ -54,9 +54,13 @@ __global__ __launch_bounds__(256) void k_pl_rates(int N, const TS* __restrict__ 
   if (gxz_t) gxz_t[i] = (TS)gz;
 }
 
-// D1 along y at (j, k) applied to f(j') (table row j, clamped loads)
+// D1 of a direction at line index i applied to f: the interior row in its
+// factored form ia (f[i+1] - f[i-1]) + ib (f[i+2] - f[i-2]) (four loads, no
+// table), the boundary rows from the table (clamped loads)
 template <class F>
-__device__ __forceinline__ double d1_line(const double* __restrict__ D1, int n, int i, const F& f) {
+__device__ __forceinline__ double d1_line(const DevDir<double>& d, int n, int i, const F& f) {
+  if (i >= 4 && i < n - 4) return d.ia * (f(i + 1) - f(i - 1)) + d.ib * (f(i + 2) - f(i - 2));
+  const double* __restrict__ D1 = d.D1;
   double s = 0.0;
 #pragma unroll
   for (int q = 0; q < kBand; ++q) {
@@ -79,8 +83,8 @@ __global__ __launch_bounds__(256) void k_pl_source(DevOp<double> p, const double
   const int j = (int)(I / Nz), k = (int)(I - (long long)j * Nz);
   const double* mu = p.mu;
   const double wgt = p.y.met[j] * p.z.met[k] * p.y.H[j] * p.z.H[k];
-  const double sy = d1_line(p.y.D1, Ny, j, [&](int m) { return mu[(size_t)m * Nz + k] * gxy[(size_t)m * Nz + k]; });
-  const double sz = d1_line(p.z.D1, Nz, k, [&](int m) { return mu[(size_t)j * Nz + m] * gxz[(size_t)j * Nz + m]; });
+  const double sy = d1_line(p.y, Ny, j, [&](int m) { return mu[(size_t)m * Nz + k] * gxy[(size_t)m * Nz + k]; });
+  const double sz = d1_line(p.z, Nz, k, [&](int m) { return mu[(size_t)j * Nz + m] * gxz[(size_t)j * Nz + m]; });
   double v = wgt * (p.y.imet[j] * sy) + wgt * (p.z.imet[k] * sz);
   const double muI = mu[I];
   if (j == 0 && p.bcL == SC_NEUMANN) v += p.z.met[k] * p.z.H[k] * muI * gxy[I];
@@ -103,8 +107,8 @@ __global__ __launch_bounds__(256) void k_pl_strain(DevOp<double> p, PlStrainArgs
   if (I >= (long long)Ny * Nz) return;
   const int j = (int)(I / Nz), k = (int)(I - (long long)j * Nz);
   const double* u = a.u;
-  const double gTxy = p.y.imet[j] * d1_line(p.y.D1, Ny, j, [&](int m) { return u[(size_t)m * Nz + k]; });
-  const double gTxz = p.z.imet[k] * d1_line(p.z.D1, Nz, k, [&](int m) { return u[(size_t)j * Nz + m]; });
+  const double gTxy = p.y.imet[j] * d1_line(p.y, Ny, j, [&](int m) { return u[(size_t)m * Nz + k]; });
+  const double gTxz = p.z.imet[k] * d1_line(p.z, Nz, k, [&](int m) { return u[(size_t)j * Nz + m]; });
   const double muI = p.mu[I];
   const double sxy = (gTxy + (-1.0) * a.gVxy[I]) * muI;
   const double sxz = (gTxz + (-1.0) * a.gVxz[I]) * muI;
@@ -149,8 +153,8 @@ __global__ __launch_bounds__(256) void k_pl_fields(DevOp<double> p, PlFields f) 
   if (I >= (long long)Ny * Nz) return;
   const int j = (int)(I / Nz), k = (int)(I - (long long)j * Nz);
   const double* u = f.u;
-  const double gTxy = p.y.imet[j] * d1_line(p.y.D1, Ny, j, [&](int m) { return u[(size_t)m * Nz + k]; });
-  const double gTxz = p.z.imet[k] * d1_line(p.z.D1, Nz, k, [&](int m) { return u[(size_t)j * Nz + m]; });
+  const double gTxy = p.y.imet[j] * d1_line(p.y, Ny, j, [&](int m) { return u[(size_t)m * Nz + k]; });
+  const double gTxz = p.z.imet[k] * d1_line(p.z, Nz, k, [&](int m) { return u[(size_t)j * Nz + m]; });
   const double muI = p.mu[I];
   const double sxy = (gTxy + (-1.0) * f.gVxy[I]) * muI;
   const double sxz = (gTxz + (-1.0) * f.gVxz[I]) * muI;
