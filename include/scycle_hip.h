@@ -396,6 +396,16 @@ int sc_qd_get_counters(sc_qd* q, sc_qd_counters* c);
  * basis (an integration with reuse > 0) and Nz <= 2048.  No counterpart in
  * the reference (its timers are MPI_Wtime accumulators). */
 int sc_qd_bench_stage(sc_qd* q, double t, int reps, double* ms_per_launch, double* words);
+/* Diagnostic: the squared residual of the projected guess at the current
+ * state and time t, r = b - A x, x = sum c_m y_m, as the stage kernel's
+ * residual test forms it (rr_bb[0] = r.r, rr_bb[1] = b.b: node by node on
+ * the band rows where b is nonzero, and c^T G c with the off-band Gram
+ * matrix G = <A y, A y> in double-double elsewhere), and
+ * the same two sums by an explicit pass (x formed, b from the boundary data,
+ * A x applied: rr_bb[2], rr_bb[3]).  The KSP convergence test on a guess
+ * (||r|| < rtol ||b||) is what the stage kernel decides with these.  Same
+ * requirements as sc_qd_bench_stage; no counterpart in the reference. */
+int sc_qd_guess_residual(sc_qd* q, double t, double* rr_bb);
 /* Measurement: sc_qd_integrate with the device loop's attempts launched one
  * by one (no graph) and HIP events around each stage launch on the context
  * stream; returns the mean duration (ms) of the plain stage kernels

@@ -169,6 +169,7 @@ SIGNATURES = {
     "sc_sbp_hzinvx_ENz": (C.c_int, [_vp, _vp, _vp]),
     "sc_bench_apply": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
     "sc_qd_bench_stage": (C.c_int, [_vp, C.c_double, C.c_int, _dp, _dp]),
+    "sc_qd_guess_residual": (C.c_int, [_vp, C.c_double, _dp]),
     "sc_qd_integrate_timed": (C.c_int, [_vp, C.POINTER(QdRun), _dp, C.POINTER(C.c_int), _dp, C.c_int,
                                         _dp, _dp, C.POINTER(C.c_longlong)]),
     "sc_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),

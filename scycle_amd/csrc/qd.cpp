@@ -226,6 +226,59 @@ QdStageIn QdDriver::stage_input(const VarsLinComb* L, int st, StepGate g, double
   return in;
 }
 
+int QdDriver::guess_residual(double t, double* out4) {
+  if (pl) return fail(SC_ERR_ARG, "the stage kernel is the linear-elastic mediator's");
+  Ctx& c = ctx->c;
+  if (!stage_fits(c)) return fail(SC_ERR_ARG, "the stage kernel needs a solution basis (integrate with reuse > 0) and Nz <= 2048");
+  hipStream_t s = c.stream;
+  int e = flush_finish();
+  if (!e) e = stage_bufs_ensure(stageBufs, Nz, band_rows());
+  if (!e) e = launch_stage_plan(c, stageBufs, desc.remoteLoadingR ? shift_d : bcR_d, s);
+  if (e) return e;
+  QdStageIn in = stage_input(nullptr, 0, StepGate{}, 0.0, PendingFinish{});
+  in.t0 = t;
+  if ((e = launch_qd_stage(c, stageBufs, 2, in, s))) return e;
+  uLazy = true;  // u is the launch's guess at the current state
+  // the residual blocks' partials, summed as the test of the next launch sums them
+  const PendingFinish pf = stage_finish(stageBufs, 2, nullptr);
+  std::vector<double> gp(2 * (size_t)pf.nb);
+  SC_HIP(hipMemcpyAsync(gp.data(), pf.partials, gp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  // explicit: the guess, the rhs of the stage's boundary data, A u, r = b - Au
+  if ((e = materialize_u())) return e;
+  double* Au = nullptr;
+  double* parts = nullptr;
+  constexpr int nb = 512;
+  if (hipMalloc(&Au, (size_t)N * sizeof(double)) != hipSuccess || hipMalloc(&parts, 2 * nb * sizeof(double)) != hipSuccess) {
+    if (Au) (void)hipFree(Au);
+    return fail(SC_ERR_HIP, "device allocation failed");
+  }
+  e = launch_set_rhs(c, rhs_d, bcL_d, bcR_d, bcT_d, bcB_d, s);
+  if (!e) e = launch_apply64(c, u_d, Au, s);
+  if (!e) e = launch_resid_norms(rhs_d, Au, (size_t)N, parts, nb, s);
+  std::vector<double> h(2 * nb);
+  if (!e && hipMemcpyAsync(h.data(), parts, h.size() * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
+    e = fail(SC_ERR_HIP, "copy of the residual partials failed");
+  if (!e && hipStreamSynchronize(s) != hipSuccess) e = fail(SC_ERR_HIP, "stream synchronisation failed");
+  (void)hipFree(Au);
+  (void)hipFree(parts);
+  if (e) return e;
+  double rr = 0.0, bb = 0.0;
+  for (int i = 0; i < nb; ++i) {
+    rr += h[2 * i];
+    bb += h[2 * i + 1];
+  }
+  double g0 = 0.0, g1 = 0.0;
+  for (int i = 0; i < pf.nb; ++i) {
+    g0 += gp[2 * i];
+    g1 += gp[2 * i + 1];
+  }
+  out4[0] = g0;
+  out4[1] = g1;
+  out4[2] = rr;
+  out4[3] = bb;
+  return SC_OK;
+}
+
 int QdDriver::bench_stage(double t, int reps, double* ms, double* words) {
   if (pl) return fail(SC_ERR_ARG, "the stage kernel is the linear-elastic mediator's");
   if (reps < 1) return fail(SC_ERR_ARG, "reps must be >= 1");
@@ -233,7 +286,7 @@ int QdDriver::bench_stage(double t, int reps, double* ms, double* words) {
   if (!stage_fits(c)) return fail(SC_ERR_ARG, "the stage kernel needs a solution basis (integrate with reuse > 0) and Nz <= 2048");
   hipStream_t s = c.stream;
   int e = flush_finish();
-  if (!e) e = stage_bufs_ensure(stageBufs, Nz, (size_t)N);
+  if (!e) e = stage_bufs_ensure(stageBufs, Nz, band_rows());
   if (!e) e = launch_stage_plan(c, stageBufs, desc.remoteLoadingR ? shift_d : bcR_d, s);
   if (e) return e;
   QdStageIn in = stage_input(nullptr, 0, StepGate{}, 0.0, PendingFinish{});
@@ -253,13 +306,12 @@ int QdDriver::bench_stage(double t, int reps, double* ms, double* words) {
   if (e) return e;
   uLazy = true;  // u is the last launch's guess at the current state
   *ms = (double)msTot / reps;
-  // the M operator products over the grid, the band nodes' unit-datum rhs
-  // (2 per node), and per fault node PL, T (M each), slip, the right datum,
-  // nine friction and state inputs and eight outputs
-  int nLow, nHigh;
-  qd_band_rows(c.d.Ny, &nLow, &nHigh);
   const double M = c.projCount;
-  *words = M * N + 2.0 * (nLow + nHigh) * Nz + (2.0 * M + 19.0) * Nz;
+  // per fault node the plan and tractions (2 M), slip and right datum, nine
+  // friction and state inputs and eight outputs (19); per band node the M
+  // operator products and the two unit-datum rhs values; the off-band Gram
+  // matrix (2 M^2)
+  *words = (2.0 * M + 19.0) * Nz + (M + 2.0) * band_rows() * Nz + 2.0 * M * M;
   return SC_OK;
 }
 
@@ -448,7 +500,7 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
   // each d_dt is the rhs and guess kernels and k_step_end ends the attempt.
   const bool oneKernel = stage_fits(ctx->c);
   if (oneKernel) {
-    int e0 = stage_bufs_ensure(stageBufs, Nz, (size_t)N);
+    int e0 = stage_bufs_ensure(stageBufs, Nz, band_rows());
     if (!e0) e0 = launch_stage_plan(ctx->c, stageBufs, desc.remoteLoadingR ? shift_d : bcR_d, s);
     if (e0) return e0;
   }
@@ -1174,6 +1226,11 @@ extern "C" int sc_qd_integrate_timed(sc_qd* h, const sc_qd_run* run, double* tEn
   *ms_stage = t.attempts > 0 ? (t.ms[0] + t.ms[1]) / (2.0 * t.attempts) : 0.0;
   *ms_end = t.attempts > 0 ? t.ms[2] / t.attempts : 0.0;
   return e;
+}
+
+extern "C" int sc_qd_guess_residual(sc_qd* h, double t, double* rr_bb) {
+  if (!h || !rr_bb) return fail(SC_ERR_ARG, "null argument");
+  return h->q->guess_residual(t, rr_bb);
 }
 
 extern "C" int sc_qd_bench_stage(sc_qd* h, double t, int reps, double* ms_per_launch, double* words) {

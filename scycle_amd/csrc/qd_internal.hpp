@@ -62,6 +62,9 @@ struct QdStageBufs {
   // the second state of the device loop's ping-pong: psi, slip, their rates
   // (Nz each)
   double* alt = nullptr;
+  // double-double sums of the guess residual (QL, QR, S3, G; the Gram
+  // matrix's block partials)
+  double* gram = nullptr;
   int nz = 0, nstream = 0;
 };
 struct QdStageIn {
@@ -96,10 +99,14 @@ struct QdStageIn {
   double clo[2] = {}, cdlo[2] = {1, 1}, chi[3] = {}, cdhi[3] = {1, 1, 1};
   int selSlot[2] = {-1, -1};
 };
-int stage_bufs_ensure(QdStageBufs& b, int Nz, size_t N);
+// buffers of the stage kernel for a fault of Nz nodes and nBandRows band rows
+int stage_bufs_ensure(QdStageBufs& b, int Nz, int nBandRows);
 void stage_bufs_free(QdStageBufs& b);
 // a basis and Nz within the kernel's LDS tables
 bool stage_fits(const Ctx& c);
+// r = b - Au over the grid: block partials of r.r and b.b (2 per block, nb
+// blocks of 256 threads; nb <= 1024)
+int launch_resid_norms(const double* b, const double* Au, size_t N, double* parts, int nb, hipStream_t s);
 // the residual test a stage launch at position pos leaves
 PendingFinish stage_finish(const QdStageBufs& b, int pos, int* miss);
 int launch_stage_plan(const Ctx& c, QdStageBufs& b, const double* rfix, hipStream_t s);
@@ -241,6 +248,10 @@ struct QdDriver {
   QdStageIn stage_input(const VarsLinComb* L, int st, StepGate g, double tfrac, PendingFinish f);
   // times the stage kernel in its accepted-state form (sc_qd_bench_stage)
   int bench_stage(double t, int reps, double* ms, double* words);              // on var -> dvar of every integrated variable
+  // the stage kernel's guess residual at time t (r.r, b.b from its Gram
+  // form) and the same two sums by an explicit pass over the grid
+  // (sc_qd_guess_residual)
+  int guess_residual(double t, double* out4);
   int check_status();
   // fault status and missed-guess flag in one read-back (the flags are left set)
   int read_flags(int* status, int* miss);
@@ -262,6 +273,12 @@ struct QdDriver {
   // u_d of the last deferred solve is formed only on demand (solve's x_lazy)
   bool uLazy = false;
   int materialize_u();
+  // rows next to the y-sides where the QD rhs is nonzero (qd_band_rows)
+  int band_rows() const {
+    int lo, hi;
+    qd_band_rows(Ny, &lo, &hi);
+    return lo + hi;
+  }
   // RK32 step control on the device (linear-elastic mediator, deferred
   // checks, no replay / output / hook): the host enqueues attempts in
   // batches and reads the control block once per batch.

@@ -21,6 +21,7 @@
 #include <new>
 
 #include "cg_dev.hpp"
+#include "dd_dev.hpp"
 #include "fault_dev.hpp"
 #include "qd_internal.hpp"
 #include "reduce_dev.hpp"
@@ -458,6 +459,69 @@ __global__ __launch_bounds__(256) void k_stage_plan(DevOp<double> p, int nLow, i
   }
 }
 
+// G_mn = <A y_m, A y_n> over the nodes off the band rows (where b is zero),
+// in double-double: per-block partials (grid-stride), then one block sums
+// them.  A y_m is the rhs of an earlier solve up to its residual there, so
+// these entries are of the size of solve residuals squared, and
+// c^T G c of the guess's coefficients keeps its digits.
+constexpr int kGramBlocks = 256;
+template <int M>
+__global__ __launch_bounds__(256) void k_stage_gram(VecList Z, size_t zLo, size_t zHi, double* __restrict__ part) {
+  constexpr int P = M * (M + 1) / 2;
+  DD g[P];
+#pragma unroll
+  for (int q = 0; q < P; ++q) g[q] = DD{0.0, 0.0};
+  for (size_t i = zLo + (size_t)blockIdx.x * 256 + threadIdx.x; i < zHi; i += (size_t)kGramBlocks * 256) {
+    double z[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) z[m] = Z.v[m][i];
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+#pragma unroll
+      for (int n = m; n < M; ++n) {
+        const int q = m * M - m * (m - 1) / 2 + (n - m);  // pair (m, n), n >= m, row by row
+        g[q] = dd_add(g[q], dd_prod(z[m], z[n]));
+      }
+  }
+  __shared__ double sh[P][4][2];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < P; ++q) {
+    const DD w = dd_wave_sum(g[q]);
+    if (lane == 0) {
+      sh[q][wid][0] = w.hi;
+      sh[q][wid][1] = w.lo;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < P) {
+    DD t{0.0, 0.0};
+    for (int w = 0; w < 4; ++w) t = dd_add(t, DD{sh[threadIdx.x][w][0], sh[threadIdx.x][w][1]});
+    part[((size_t)blockIdx.x * P + threadIdx.x) * 2] = t.hi;
+    part[((size_t)blockIdx.x * P + threadIdx.x) * 2 + 1] = t.lo;
+  }
+}
+
+// one block: G (M x M, both triangles) from the partials
+template <int M>
+__global__ __launch_bounds__(64) void k_stage_gram_sum(const double* __restrict__ part, double* __restrict__ G) {
+  constexpr int P = M * (M + 1) / 2;
+  const int q = threadIdx.x;
+  if (q >= P) return;
+  DD t{0.0, 0.0};
+  for (int b = 0; b < kGramBlocks; ++b) t = dd_add(t, DD{part[((size_t)b * P + q) * 2], part[((size_t)b * P + q) * 2 + 1]});
+  int m = 0, r = q;
+  while (r >= M - m) {
+    r -= M - m;
+    ++m;
+  }
+  const int n = m + r;
+  G[((size_t)m * M + n) * 2] = t.hi;
+  G[((size_t)m * M + n) * 2 + 1] = t.lo;
+  G[((size_t)n * M + m) * 2] = t.hi;
+  G[((size_t)n * M + m) * 2 + 1] = t.lo;
+}
+
 // one block: sR[m] = sum_k PR_m(k), sR[M + m] = sum_k PR_m(k) rfix(k)
 template <int M>
 __global__ __launch_bounds__(256) void k_stage_plan_sums(const double* __restrict__ PR, const double* __restrict__ rfix,
@@ -493,6 +557,8 @@ struct QdStageArgs {
   const double* sR;  // 2 M
   const double* aL;  // band nodes' rhs for a unit left datum
   const double* aR;  // and a unit right datum
+  // the off-band Gram matrix G_mn = <A y_m, A y_n> (M x M, double-double)
+  const double* G;
   double* coefOut;   // the coefficients, for proj_materialize
   double* parts;     // this stage's residual partials (2 per streaming block)
   CgScal* sc;        // this stage's scalars
@@ -524,9 +590,11 @@ struct QdStageArgs {
 
 // Blocks [0, nfb): the stage state and fault rates of 64 fault nodes each
 // (one wave: the root solves are FP64 VALU work, spread one wave per CU);
-// [nfb, nfb + nstream): the residual b - sum c_m (A y_m) over the grid with
-// its r.r, b.b partials, b = aL l + aR r on the band rows; one more block:
-// the previous stage's residual test.
+// [nfb, nfb + nstream): the guess residual's r.r and b.b partials over the
+// band rows, b = aL l + aR r there, the first adding the nodes off the band
+// (b = 0 there) as c^T G c from the plan's off-band Gram matrix, so that no
+// block passes over the grid; one more block: the previous stage's residual
+// test.
 // the gate words a stage launch without a gate reads (never written)
 __device__ StepCtl g_no_gate;
 
@@ -572,31 +640,23 @@ __global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
   }
   __builtin_amdgcn_sched_barrier(0);
   // Every load that does not depend on the step goes out before the gate is
-  // waited for: the streaming blocks' first pass (A y_m, and the unit-datum
-  // rhs on band rows), and each thread's fault nodes' slip operands and PL
-  // entries.
-  constexpr int kU = M <= 2 ? 4 : (M <= 4 ? 2 : 1);
-  const size_t stride = (size_t)a.nstream * kStageThreads;
-  double zv[kU][M], av[kU][2];
-  const size_t nBand0 = a.zLo;
-  auto load_pass = [&](size_t i0) {
+  // waited for: the residual blocks' band nodes (A y_m and the unit-datum
+  // rhs), and each thread's fault nodes' slip operands and PL entries.
+  constexpr int kBN = 2;  // band nodes a thread of a residual block
+  const size_t nBand = a.zLo + (a.N - a.zHi);
+  double zb[kBN][M], ab[kBN][2];
+  size_t bi_[kBN];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const size_t i = i0 + u * stride;
-      if (i < a.N) {
+  for (int u = 0; u < kBN; ++u) {
+    const size_t q = (size_t)(sblk >= 0 ? sblk : 0) * (kBN * kStageThreads) + (size_t)u * kStageThreads + tid;
+    bi_[u] = (sblk >= 0 && q < nBand) ? (q < a.zLo ? q : a.zHi + (q - a.zLo)) : a.N;
+    if (bi_[u] < a.N) {
 #pragma unroll
-        for (int m = 0; m < M; ++m) zv[u][m] = a.Z.v[m][i];
-        if (i < a.zLo || i >= a.zHi) {
-          const size_t q = i < a.zLo ? i : nBand0 + (i - a.zHi);
-          av[u][0] = a.aL[q];
-          av[u][1] = a.aR[q];
-        }
-      }
+      for (int m = 0; m < M; ++m) zb[u][m] = a.Z.v[m][bi_[u]];
+      ab[u][0] = a.aL[q];
+      ab[u][1] = a.aR[q];
     }
-  };
-  const size_t iFirst = sblk >= 0 ? (size_t)sblk * kStageThreads + tid : a.N;
-  // (the streaming blocks' first pass waits for the gate: issued with
-  // everyone's gate reads it would queue them behind 16 MB of loads)
+  }
   constexpr int kQ = KQ;
   static_assert(KQ * kStageThreads <= kStageNzMax, "the LDS rows hold kStageNzMax nodes");
   double sb[kQ], sf[kQ][kStageNT], pl[kQ][M], rk[kQ];
@@ -907,28 +967,21 @@ __global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
     a.sc->cnt = 0;
   }
   if (sblk == 0 && tid < M) a.coefOut[tid] = c[tid];
+  // ||b - sum c_m A y_m||^2 on the band nodes, node by node, b = aL l + aR r
   double r2 = 0.0, b2 = 0.0;
-  for (size_t i0 = iFirst; i0 < a.N; i0 += kU * stride) {
-    load_pass(i0);
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const size_t i = i0 + u * stride;
-      if (i < a.N) {
-        double az = 0.0;
+  for (int u = 0; u < kBN; ++u) {
+    if (bi_[u] < a.N) {
+      double az = 0.0;
 #pragma unroll
-        for (int m = 0; m < M; ++m) az += c[m] * zv[u][m];
-        double bi = 0.0;
-        if (i < a.zLo || i >= a.zHi) {  // a band row
-          const int k = (int)(i % (size_t)Nz);
-          bi = av[u][0] * (sS[k] * (1.0 / a.fts)) + av[u][1] * rS[k];
-        }
-        const double ri = bi - az;
-        r2 += ri * ri;
-        b2 += bi * bi;
-      }
+      for (int m = 0; m < M; ++m) az += c[m] * zb[u][m];
+      const int k = (int)(bi_[u] % (size_t)Nz);
+      const double bi = ab[u][0] * (sS[k] * (1.0 / a.fts)) + ab[u][1] * rS[k];
+      const double ri = bi - az;
+      r2 += ri * ri;
+      b2 += bi * bi;
     }
   }
-  // block partials: waves in order
   r2 = wave_sum(r2);
   b2 = wave_sum(b2);
   __syncthreads();  // red is reused
@@ -940,12 +993,33 @@ __global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
   if (tid < 2) {
     double v = 0.0;
     for (int q = 0; q < kStageThreads / 64; ++q) v += red[tid][q];
+    if (tid == 0 && sblk == 0) {
+      // the nodes off the band rows, where b = 0: ||sum c_m A y_m||^2 =
+      // c^T G c with the off-band Gram matrix (double-double)
+      DD q2{0.0, 0.0};
+#pragma unroll
+      for (int m = 0; m < M; ++m)
+#pragma unroll
+        for (int n = 0; n < M; ++n) {
+          const size_t o = ((size_t)m * M + n) * 2;
+          q2 = dd_add(q2, dd_mul_d(dd_mul_d(DD{a.G[o], a.G[o + 1]}, c[m]), c[n]));
+        }
+      const double off = q2.hi + q2.lo;
+      v += off > 0.0 ? off : 0.0;
+    }
     a.parts[(size_t)sblk * 2 + tid] = v;
   }
 }
 
-int stage_bufs_ensure(QdStageBufs& b, int Nz, size_t N) {
-  const int nstream = (int)((N + 4 * kStageThreads - 1) / (4 * kStageThreads));  // 4 entries per thread
+// the gram buffer: G (2 SC_REUSE_MAX^2 doubles), the Gram kernel's partials
+static size_t gram_doubles() {
+  constexpr size_t R = SC_REUSE_MAX;
+  return 2 * R * R + (size_t)kGramBlocks * (R * (R + 1) / 2) * 2;
+}
+
+int stage_bufs_ensure(QdStageBufs& b, int Nz, int nBandRows) {
+  // residual blocks: 2 band nodes a thread
+  const int nstream = (int)(((size_t)nBandRows * Nz + 2 * kStageThreads - 1) / (2 * kStageThreads));
   if (b.plan && b.nz == Nz && b.nstream == nstream) return SC_OK;
   stage_bufs_free(b);
   // PL, PR (SC_REUSE_MAX x Nz each), sums (2 SC_REUSE_MAX), aL, aR (2 kBand x Nz each)
@@ -955,6 +1029,7 @@ int stage_bufs_ensure(QdStageBufs& b, int Nz, size_t N) {
   SC_HIP(hipMalloc(&b.scal, kStagePos * sizeof(CgScal)));
   SC_HIP(hipMemset(b.scal, 0, kStagePos * sizeof(CgScal)));
   SC_HIP(hipMalloc(&b.alt, 4 * (size_t)Nz * sizeof(double)));
+  SC_HIP(hipMalloc(&b.gram, gram_doubles() * sizeof(double)));
   b.nz = Nz;
   b.nstream = nstream;
   return SC_OK;
@@ -965,13 +1040,43 @@ void stage_bufs_free(QdStageBufs& b) {
   if (b.parts) (void)hipFree(b.parts);
   if (b.scal) (void)hipFree(b.scal);
   if (b.alt) (void)hipFree(b.alt);
+  if (b.gram) (void)hipFree(b.gram);
   b = QdStageBufs{};
 }
 
 bool stage_fits(const Ctx& c) { return c.d.Nz <= kStageNzMax && c.projCount > 0; }
 
+__global__ __launch_bounds__(256) void k_resid_norms(const double* __restrict__ b, const double* __restrict__ Au,
+                                                     size_t N, double* __restrict__ parts) {
+  double v[2] = {0.0, 0.0};
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t)gridDim.x * 256) {
+    const double r = b[i] - Au[i];
+    v[0] += r * r;
+    v[1] += b[i] * b[i];
+  }
+  block_partials(v, parts);
+}
+
+int launch_resid_norms(const double* b, const double* Au, size_t N, double* parts, int nb, hipStream_t s) {
+  if (nb < 1 || nb > 1024) return fail(SC_ERR_ARG, "resid norms: 1..1024 blocks");
+  hipLaunchKernelGGL(k_resid_norms, dim3(nb), dim3(256), 0, s, b, Au, N, parts);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
 PendingFinish stage_finish(const QdStageBufs& b, int pos, int* miss) {
   return PendingFinish{(CgScal*)b.scal + pos, b.parts + (size_t)pos * 2 * b.nstream, b.nstream, miss};
+}
+
+struct GramPtrs {
+  double *G, *part;
+};
+static GramPtrs gram_ptrs(const QdStageBufs& b) {
+  constexpr size_t R = SC_REUSE_MAX;
+  GramPtrs p;
+  p.G = b.gram;
+  p.part = p.G + 2 * R * R;
+  return p;
 }
 
 int launch_stage_plan(const Ctx& c, QdStageBufs& b, const double* rfix, hipStream_t s) {
@@ -979,19 +1084,26 @@ int launch_stage_plan(const Ctx& c, QdStageBufs& b, const double* rfix, hipStrea
   if (!stage_fits(c)) return fail(SC_ERR_ARG, "stage kernel: no basis or Nz too large");
   int nLow, nHigh;
   qd_band_rows(c.d.Ny, &nLow, &nHigh);
-  VecList Y{};
-  for (int m = 0; m < M; ++m) Y.v[m] = c.projY[m];
+  VecList Y{}, Z{};
+  for (int m = 0; m < M; ++m) {
+    Y.v[m] = c.projY[m];
+    Z.v[m] = c.projZ[m];
+  }
   double* PL = b.plan;
   double* PR = b.plan + (size_t)SC_REUSE_MAX * Nz;
   double* sR = b.plan + 2 * (size_t)SC_REUSE_MAX * Nz;
   double* aL = sR + 2 * SC_REUSE_MAX;
   double* aR = aL + 2 * (size_t)kBand * Nz;
+  GramPtrs gp = gram_ptrs(b);
+  const size_t zLo = (size_t)nLow * Nz, zHi = (size_t)(c.d.Ny - nHigh) * Nz;
   const dim3 g((Nz + 255) / 256), bl(256);
   switch (M) {
 #define SC_SP(m)                                                                                    \
   case m:                                                                                           \
     hipLaunchKernelGGL(k_stage_plan<m>, g, bl, 0, s, c.op64, nLow, nHigh, Y, PL, PR, aL, aR);       \
     hipLaunchKernelGGL(k_stage_plan_sums<m>, dim3(1), dim3(256), 0, s, (const double*)PR, rfix, Nz, sR); \
+    hipLaunchKernelGGL(k_stage_gram<m>, dim3(kGramBlocks), dim3(256), 0, s, Z, zLo, zHi, gp.part);  \
+    hipLaunchKernelGGL(k_stage_gram_sum<m>, dim3(1), dim3(64), 0, s, (const double*)gp.part, gp.G); \
     break;
     SC_SP(1) SC_SP(2) SC_SP(3) SC_SP(4) SC_SP(5) SC_SP(6) SC_SP(7) SC_SP(8)
 #undef SC_SP
@@ -1022,6 +1134,7 @@ int launch_qd_stage(const Ctx& c, const QdStageBufs& b, int pos, const QdStageIn
   a.zHi = (size_t)(c.d.Ny - nHigh) * Nz;
   a.aL = b.plan + 2 * (size_t)SC_REUSE_MAX * Nz + 2 * SC_REUSE_MAX;
   a.aR = a.aL + 2 * (size_t)kBand * Nz;
+  a.G = gram_ptrs(b).G;
   for (int m = 0; m < M; ++m) {
     a.Z.v[m] = c.projZ[m];
     a.T.v[m] = c.projT[m];
