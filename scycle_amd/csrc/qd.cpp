@@ -50,8 +50,12 @@ QdDriver::~QdDriver() {
 void QdDriver::drop_stage_graph() {
   if (stageExec) (void)hipGraphExecDestroy(stageExec);
   if (stageGraph) (void)hipGraphDestroy(stageGraph);
+  if (tailExec) (void)hipGraphExecDestroy(tailExec);
+  if (tailGraph) (void)hipGraphDestroy(tailGraph);
   stageExec = nullptr;
   stageGraph = nullptr;
+  tailExec = nullptr;
+  tailGraph = nullptr;
   stageKey.clear();
 }
 
@@ -514,7 +518,9 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
   // no launch writes what its other blocks read.  The direct attempt has
   // p = 0 and a graph holds an even number of attempts starting at p = 1:
   // after every batch the latest state and control block are those of p = 1.
-  static_assert(kAttemptsPerGraph % 2 == 0, "the stage loop's ping-pong needs an even number of attempts per graph");
+  static_assert(kAttemptsPerGraph % 2 == 0 && kAttemptsPerTail % 2 == 0,
+                "the stage loop's ping-pong needs an even number of attempts per graph");
+  static_assert(kAttemptsPerGraph % kAttemptsPerTail == 0, "the full graph repeats the short one's launches");
   double* varB[2][2] = {{P.var, Q.var}, {nullptr, nullptr}};
   double* dvarB[2][2] = {{P.dvar, Q.dvar}, {nullptr, nullptr}};
   if (oneKernel) {
@@ -624,25 +630,25 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
   // The first attempt goes in directly (it also takes any residual test left
   // from the host's last solve); every later one repeats the same launches
   // with the same arguments (all that changes lives in the control block),
-  // so a captured run of kAttemptsPerGraph attempts is replayed as a graph
-  // (attempts past the run's end find the gate closed).  The capture holds
+  // so a captured run of attempts is replayed as a graph (attempts past the
+  // run's end find the gate closed).  The capture holds
   // no memset nodes.  The one-kernel loop's graph is kept across calls and
   // captured again only when the arguments of its launches change.
   int e = slot();
   if (!e && timed) e = collect();
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  auto capture = [&](hipGraph_t* gr, hipGraphExec_t* ex) -> int {
+  auto capture = [&](hipGraph_t* gr, hipGraphExec_t* ex, int nAtt) -> int {
     SC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
     int ec = 0;
-    for (int a = 0; a < kAttemptsPerGraph && !ec; ++a) ec = slot();
+    for (int a = 0; a < nAtt && !ec; ++a) ec = slot();
     hipError_t ce = hipStreamEndCapture(s, gr);
     if (!ec && ce != hipSuccess) ec = fail(SC_ERR_HIP, std::string("stream capture: ") + hipGetErrorString(ce));
     if (!ec && hipGraphInstantiate(ex, *gr, nullptr, nullptr, 0) != hipSuccess)
       ec = fail(SC_ERR_HIP, "graph instantiation failed");
     return ec;
   };
-  hipGraphExec_t run = nullptr;
+  hipGraphExec_t run = nullptr, tail = nullptr;
   if (timed) {
     // no graph: the attempts go in one by one
   } else if (!e && oneKernel) {
@@ -651,15 +657,20 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
     const bool finPending0 = finPending, uLazy0 = uLazy;
     std::vector<unsigned char> key;
     rec = &key;
-    for (int a = 0; a < kAttemptsPerGraph && !e; ++a) e = slot();
+    for (int a = 0; a < kAttemptsPerTail && !e; ++a) e = slot();
     rec = nullptr;
-    if (!e && (!stageExec || key != stageKey)) {
-      drop_stage_graph();
+    auto restore = [&]() {
       parity = parity0;
       fin = fin0;
       finPending = finPending0;
       uLazy = uLazy0;
-      e = capture(&stageGraph, &stageExec);
+    };
+    restore();
+    if (!e && (!stageExec || !tailExec || key != stageKey)) {
+      drop_stage_graph();
+      e = capture(&stageGraph, &stageExec, kAttemptsPerGraph);
+      restore();
+      if (!e) e = capture(&tailGraph, &tailExec, kAttemptsPerTail);
       if (e) drop_stage_graph();
       else {
         stageKey.swap(key);
@@ -667,9 +678,10 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
       }
     }
     run = stageExec;
+    tail = tailExec;
   } else if (!e) {
-    e = capture(&graph, &exec);
-    run = exec;
+    e = capture(&graph, &exec, kAttemptsPerTail);
+    run = tail = exec;
   }
   struct GraphGuard {
     hipGraph_t& g;
@@ -691,8 +703,12 @@ int QdDriver::device_steps(RkState& r, double finalT, int maxSteps, double& newD
         if (!e) e = collect();
       }
     } else {
-      for (int k = 0; k < (slots + kAttemptsPerGraph - 1) / kAttemptsPerGraph; ++k) {
-        if (hipGraphLaunch(run, s) != hipSuccess) {
+      // whole batches as the full graph, the rest as the short one
+      const bool full = run != tail;
+      const int nFull = full ? slots / kAttemptsPerGraph : 0;
+      const int nTail = (slots - nFull * kAttemptsPerGraph + kAttemptsPerTail - 1) / kAttemptsPerTail;
+      for (int k = 0; k < nFull + nTail; ++k) {
+        if (hipGraphLaunch(k < nFull ? run : tail, s) != hipSuccess) {
           e = fail(SC_ERR_HIP, "graph launch failed");
           break;
         }

@@ -289,14 +289,20 @@ struct QdDriver {
   QdStageBufs stageBufs;  // the device loop's one-kernel stages
   static constexpr int kSlotsPerBatch = 24;
   // attempts captured in one graph: a relaunch of a graph waits for its
-  // previous run, so each launch carries several attempts
-  static constexpr int kAttemptsPerGraph = 4;
-  // The instantiated graph of the one-kernel stage loop, kept across
-  // device_steps calls; stageKey holds the arguments of every launch it
-  // contains, and it is captured again only when they change (a new basis
-  // size or new basis vectors).
+  // previous run (about 8 us between the two on the device), so a full batch
+  // is one launch of kAttemptsPerGraph attempts and only a batch's tail is
+  // made of launches of the short graph (kAttemptsPerTail attempts)
+  static constexpr int kAttemptsPerGraph = kSlotsPerBatch;
+  static constexpr int kAttemptsPerTail = 4;
+  // The instantiated graphs of the one-kernel stage loop (the full batch and
+  // the short tail), kept across device_steps calls; stageKey holds the
+  // arguments of every launch of the short graph (the full one repeats them),
+  // and both are captured again only when they change (a new basis size or
+  // new basis vectors).
   hipGraph_t stageGraph = nullptr;
   hipGraphExec_t stageExec = nullptr;
+  hipGraph_t tailGraph = nullptr;
+  hipGraphExec_t tailExec = nullptr;
   std::vector<unsigned char> stageKey;
   long long graphBuilds = 0;
   void drop_stage_graph();

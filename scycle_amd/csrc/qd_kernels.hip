@@ -567,6 +567,16 @@ struct QdStageArgs {
   VarsLinComb L;     // the stage combination (nv = 0: the state itself)
   int psiIdx, slipIdx;
   const double* slipVar;  // the slip when L.nv = 0
+  // the stage's slip and psi operands resolved by the host (no indexed
+  // argument loads in the kernel): base, the combination's rate terms (the
+  // base again past ntS terms), the outputs of the combination
+  int ntS;  // terms of the combination (0: the state itself)
+  const double* sBase;
+  const double* sF[kStageNT];
+  const double* pBase;
+  const double* pF[kStageNT];
+  double* sOut;
+  double* pOut;
   double fts, vL;
   int remote;
   const double* rfix;  // shift (remote loading) or the fixed bcR
@@ -600,7 +610,7 @@ __device__ StepCtl g_no_gate;
 
 // KQ: fault rows per thread in the prologue (Nz <= KQ * kStageThreads)
 template <int M, bool END, int KQ>
-__global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
+__global__ __launch_bounds__(kStageThreads, 2) void k_qd_stage(QdStageArgs a) {
   const int blk = blockIdx.x;
   if (!END && blk >= a.nfb + a.nstream) {
     if (!(a.gate.ctl && a.gate.ctl->halt)) finish_irres_block(a.fin);
@@ -649,35 +659,58 @@ __global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
 #pragma unroll
   for (int u = 0; u < kBN; ++u) {
     const size_t q = (size_t)(sblk >= 0 ? sblk : 0) * (kBN * kStageThreads) + (size_t)u * kStageThreads + tid;
-    bi_[u] = (sblk >= 0 && q < nBand) ? (q < a.zLo ? q : a.zHi + (q - a.zLo)) : a.N;
-    if (bi_[u] < a.N) {
+    const bool use = sblk >= 0 && q < nBand;
+    const size_t qc = use ? q : 0;  // (unconditional loads: none waits in a branch)
+    const size_t ic = qc < a.zLo ? qc : a.zHi + (qc - a.zLo);
+    bi_[u] = use ? ic : a.N;
 #pragma unroll
-      for (int m = 0; m < M; ++m) zb[u][m] = a.Z.v[m][bi_[u]];
-      ab[u][0] = a.aL[q];
-      ab[u][1] = a.aR[q];
-    }
+    for (int m = 0; m < M; ++m) zb[u][m] = a.Z.v[m][ic];
+    ab[u][0] = a.aL[qc];
+    ab[u][1] = a.aR[qc];
   }
   constexpr int kQ = KQ;
   static_assert(KQ * kStageThreads <= kStageNzMax, "the LDS rows hold kStageNzMax nodes");
   double sb[kQ], sf[kQ][kStageNT], pl[kQ][M], rk[kQ];
+  // the attempt's end: its state and its stages' rates (base, f0, f1, f2 of
+  // psi and slip), and the last stage's first residual partials
+  double ev[END ? kQ : 1][2][4];
+  double w0 = 0.0, w1 = 0.0;
+  if constexpr (END) {
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+      const int k = min(tid + q * kStageThreads, Nz - 1);
+#pragma unroll
+      for (int f = 0; f < 2; ++f) {
+        ev[q][f][0] = a.pv[f][k];
+        ev[q][f][1] = a.pf[f][0][k];
+        ev[q][f][2] = a.pf[f][1][k];
+        ev[q][f][3] = a.pf[f][2][k];
+      }
+    }
+    const int b0 = min(tid, a.fin3.nb - 1);
+    const double p0 = a.fin3.partials[(size_t)b0 * 2], p1 = a.fin3.partials[(size_t)b0 * 2 + 1];
+    if (tid < a.fin3.nb) {
+      w0 = p0;
+      w1 = p1;
+    }
+  }
 #pragma unroll
   for (int q = 0; q < kQ; ++q) {
-    const int k = tid + q * kStageThreads;
-    if (k < Nz) {
-      if (END) {
-        sb[q] = 0.0;  // the state is the attempt's solution, formed below
-      } else if (a.L.nv) {
-        sb[q] = a.L.base[a.slipIdx][k];
+    // (clamped rows: every load is unconditional, so that none waits in a
+    // branch; rows past Nz are not used)
+    const int k = min(tid + q * kStageThreads, Nz - 1);
+    if constexpr (END) {
+      sb[q] = 0.0;  // the state is the attempt's solution, formed below
 #pragma unroll
-        for (int u = 0; u < kStageNT; ++u)
-          if (u < a.L.nt) sf[q][u] = a.L.f[a.slipIdx][u][k];
-      } else {
-        sb[q] = a.slipVar[k];
-      }
-      rk[q] = a.rfix[k];
+      for (int u = 0; u < kStageNT; ++u) sf[q][u] = 0.0;
+    } else {
+      sb[q] = a.sBase[k];
 #pragma unroll
-      for (int m = 0; m < M; ++m) pl[q][m] = a.PL[(size_t)m * Nz + k];
+      for (int u = 0; u < kStageNT; ++u) sf[q][u] = a.sF[u][k];
     }
+    rk[q] = a.rfix[k];
+#pragma unroll
+    for (int m = 0; m < M; ++m) pl[q][m] = a.PL[(size_t)m * Nz + k];
   }
   // fault lanes: the node's step-independent operands (tractions of the
   // basis, friction inputs, the previous V) and the operands of the stage's
@@ -694,15 +727,10 @@ __global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
 #pragma unroll
     for (int m = 0; m < M; ++m) tm[m] = a.T.v[m][kf];
     fnode = fault_node_load(a.fuse.a, kf, 0.0);
-    if (!END) {
-      if (a.L.nv) {
-        psiB = a.L.base[a.psiIdx][kf];
+    if constexpr (!END) {
+      psiB = a.pBase[kf];
 #pragma unroll
-        for (int u = 0; u < kStageNT; ++u)
-          if (u < a.L.nt) psiF[u] = a.L.f[a.psiIdx][u][kf];
-      } else {
-        psiB = a.fuse.a.psi_d[kf];
-      }
+      for (int u = 0; u < kStageNT; ++u) psiF[u] = a.pF[u][kf];
     }
   }
   // the coefficient step's per-basis plan sums and weights
@@ -739,9 +767,8 @@ __global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
       flg[0] = flgReg0;
       flg[1] = flgReg1;
     }
-    double w0 = 0.0, w1 = 0.0;
     if (tid < kRedThreads)
-      for (int bq = tid; bq < a.fin3.nb; bq += kRedThreads) {
+      for (int bq = tid + kRedThreads; bq < a.fin3.nb; bq += kRedThreads) {
         w0 += a.fin3.partials[(size_t)bq * 2];
         w1 += a.fin3.partials[(size_t)bq * 2 + 1];
       }
@@ -784,7 +811,7 @@ __global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
       if (k < Nz) {
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
-          const double base = a.pv[f][k], f0 = a.pf[f][0][k], f1 = a.pf[f][1][k], f2 = a.pf[f][2][k];
+          const double base = ev[q][f][0], f0 = ev[q][f][1], f1 = ev[q][f][2], f2 = ev[q][f][3];
           double lo = clo[0] * f0 + base;  // y + h/2 (f0 + f2)
           lo = lo + clo[1] * f2;
           double hi = chi[0] * f0 + base;  // y + h (f0/6 + 2 f1/3 + f2/6)
@@ -892,11 +919,11 @@ __global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
     const int k = tid + q * kStageThreads;
     if (k < Nz) {
       double sk = END ? sS[k] : sb[q];
-      if (!END && a.L.nv) {  // the combination (k_vars_lincomb's order)
+      if (!END && a.ntS > 0) {  // the combination (k_vars_lincomb's order)
         sk = cc[0] * sf[q][0] + sb[q];
 #pragma unroll
         for (int u = 1; u < kStageNT; ++u)
-          if (u < a.L.nt) sk = sk + cc[u] * sf[q][u];
+          if (u < a.ntS) sk = sk + cc[u] * sf[q][u];
       }
       const double l = sk * (1.0 / a.fts);
       const double r = a.remote ? a.vL * t / a.fts + 1.0 * rk[q] : rk[q];
@@ -931,14 +958,14 @@ __global__ __launch_bounds__(kStageThreads, 4) void k_qd_stage(QdStageArgs a) {
         psi = pS[k];
         a.pvOut[0][k] = psi;
         a.pvOut[1][k] = sS[k];
-      } else if (a.L.nv) {  // psi of the stage (k_vars_lincomb's order)
+      } else if (a.ntS > 0) {  // psi of the stage (k_vars_lincomb's order)
         double x = cc[0] * psiF[0] + psiB;
 #pragma unroll
         for (int u = 1; u < kStageNT; ++u)
-          if (u < a.L.nt) x = x + cc[u] * psiF[u];
+          if (u < a.ntS) x = x + cc[u] * psiF[u];
         psi = x;
-        a.L.out[a.psiIdx][k] = x;
-        a.L.out[a.slipIdx][k] = sS[k];
+        a.pOut[k] = x;
+        a.sOut[k] = sS[k];
       }
       double tq = 0.0;
 #pragma unroll
@@ -1148,6 +1175,21 @@ int launch_qd_stage(const Ctx& c, const QdStageBufs& b, int pos, const QdStageIn
   a.tol2 = in.tol2;
   a.fuse = in.fuse;
   a.L = in.L;
+  {
+    // the stage's operands, resolved here (no indexed argument loads in the kernel)
+    const bool comb = in.L.nv > 0;
+    a.ntS = comb ? in.L.nt : 0;
+    a.sBase = comb ? in.L.base[in.slipIdx] : in.slipVar;
+    a.pBase = comb ? in.L.base[in.psiIdx] : in.fuse.a.psi_d;
+    for (int u = 0; u < kStageNT; ++u) {
+      a.sF[u] = comb && u < in.L.nt ? in.L.f[in.slipIdx][u] : a.sBase;
+      a.pF[u] = comb && u < in.L.nt ? in.L.f[in.psiIdx][u] : a.pBase;
+    }
+    a.sOut = comb ? in.L.out[in.slipIdx] : nullptr;
+    a.pOut = comb ? in.L.out[in.psiIdx] : nullptr;
+    if (!in.end && (!a.sBase || !a.pBase))
+      return fail(SC_ERR_ARG, "stage kernel: the stage's slip or psi is missing");
+  }
   a.psiIdx = in.psiIdx;
   a.slipIdx = in.slipIdx;
   a.slipVar = in.slipVar;
@@ -1172,12 +1214,17 @@ int launch_qd_stage(const Ctx& c, const QdStageBufs& b, int pos, const QdStageIn
     a.flags = in.flags;
     a.flagEnd = in.flagEnd;
     a.fin3 = in.fin3;
+    if (in.fin3.nb < 1 || !in.fin3.partials)
+      return fail(SC_ERR_ARG, "stage kernel: an attempt's end needs the last stage's residual partials");
     for (int f = 0; f < 2; ++f) {
       a.pv[f] = in.pv[f];
       a.pvOut[f] = in.pvOut[f];
       if (!a.pv[f] || !a.pvOut[f] || a.pv[f] == a.pvOut[f])
         return fail(SC_ERR_ARG, "stage kernel: an attempt's end reads and writes different states");
-      for (int r = 0; r < 3; ++r) a.pf[f][r] = in.pf[f][r];
+      for (int r = 0; r < 3; ++r) {
+        a.pf[f][r] = in.pf[f][r];
+        if (!a.pf[f][r]) return fail(SC_ERR_ARG, "stage kernel: an attempt's end needs its three stages' rates");
+      }
       a.selSlot[f] = in.selSlot[f];
     }
     for (int i = 0; i < 2; ++i) {
