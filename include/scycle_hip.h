@@ -127,7 +127,10 @@ int sc_sbp_hzinvx_ENz(sc_ctx* ctx, const double* in_d, double* out_d);
  * 6 FP64 frame kernel, 7 FP32 frame kernel, 8 the projected-guess kernel of
  * sc_solve on the context's current basis (needs a solve with reuse > 0;
  * `nodes` then returns the 8-byte words it moves per launch), 9 / 10 the
- * FP64 / FP32 wave step of sc_wave_step (laplace type yz).
+ * FP64 / FP32 wave step of sc_wave_step (laplace type yz), 11 / 12 / 13 one
+ * application of the fast-diagonalisation preconditioner with FP64 / FP32 /
+ * bf16 GEMMs on FP64 vectors (sc_precond_apply; `nodes` then returns the
+ * GEMM flops of one application, 4 Ny Nz (Ny + Nz)).
  * No counterpart in the reference (its timers are MPI_Wtime accumulators,
  * e.g. linearElastic.cpp:474-477). */
 int sc_bench_apply(sc_ctx* ctx, int which, int reps, double* ms_per_launch, double* nodes);
@@ -176,6 +179,13 @@ typedef struct sc_solve_stats {
  * with u as the initial guess (KSPSetInitialGuessNonzero) and the solution. */
 int sc_solve(sc_ctx* ctx, const double* rhs_d, double* u_d, const sc_solve_opts* opts,
              sc_solve_stats* stats);
+
+/* PCApply of the solve's preconditioner (KSPGetPC / PCApply, the PC of the
+ * reference's KSP set up in linearElastic.cpp:251-289): z = M^-1 r on the
+ * device for precond SC_PREC_FDIAG64 / FDIAG32 / FDIAG_BF16 (the
+ * fast-diagonalisation inverse of the separable operator, built on first use
+ * and after the operator changes) or SC_PREC_JACOBI (z = r / diag(A)). */
+int sc_precond_apply(sc_ctx* ctx, int precond, const double* r_d, double* z_d);
 
 /* sc_solve without any device allocation: FP64 Jacobi-PCG (SC_SOLVE_CG64,
  * SC_PREC_JACOBI, reuse 0; other options give SC_ERR_ARG) from u as the

@@ -174,6 +174,7 @@ SIGNATURES = {
                                         _dp, _dp, C.POINTER(C.c_longlong)]),
     "sc_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "sc_solve_noalloc": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "sc_precond_apply": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "sc_fault_qd_rates": (C.c_int, [_vp, C.POINTER(FaultQdArgs)]),
     "sc_fault_fd_step": (C.c_int, [_vp, C.POINTER(FaultFdArgs)]),
     "sc_wave_step": (C.c_int, [_vp, C.c_int, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -1145,6 +1145,42 @@ int solve_ir_fdiag(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o,
   return SC_OK;
 }
 
+// the fast-diagonalisation factors for the operator as it stands (built on
+// first use and again after the BC types or the coefficients change)
+int ensure_fdiag(Ctx& c) {
+  const int key = c.d.bcR | c.d.bcT << 1 | c.d.bcL << 2 | c.d.bcB << 3;
+  if (!c.fd) c.fd = new FdDiag;
+  if (c.fd->bcKey != key || c.fd_stale) {
+    const int e = fdiag_build(c, *c.fd);
+    if (e) return e;
+    c.fd_stale = false;
+  }
+  return SC_OK;
+}
+
+__global__ void k_jacobi_apply(size_t n, const double* __restrict__ r, const double* __restrict__ dinv,
+                               double* __restrict__ z) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    z[i] = r[i] * dinv[i];
+}
+
+// z = M^-1 r with one of the solve's preconditioners
+int precond_apply(Ctx& c, int precond, const double* r, double* z) {
+  if (precond == SC_PREC_JACOBI) {
+    if (!c.dinv64) return fail(SC_ERR_ARG, "sc_precond_apply: the context has no diagonal");
+    const size_t N = (size_t)c.N;
+    hipLaunchKernelGGL(k_jacobi_apply, dim3(ew_blocks(N)), dim3(256), 0, c.stream, N, r, c.dinv64, z);
+    SC_HIP(hipGetLastError());
+    return SC_OK;
+  }
+  if (precond != SC_PREC_FDIAG64 && precond != SC_PREC_FDIAG32 && precond != SC_PREC_FDIAG_BF16)
+    return fail(SC_ERR_ARG, "sc_precond_apply: unknown preconditioner");
+  const int e = ensure_fdiag(c);
+  if (e) return e;
+  const int fprec = precond == SC_PREC_FDIAG64 ? FD_F64 : (precond == SC_PREC_FDIAG32 ? FD_F32 : FD_BF16);
+  return fdiag_apply(c, *c.fd, fprec, r, z);
+}
+
 int solve_core(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_solve_stats& st) {
   std::memset(&st, 0, sizeof(st));
   const bool ir = o.mode == SC_SOLVE_IR32;
@@ -1153,13 +1189,8 @@ int solve_core(Ctx& c, const double* rhs, double* u, const sc_solve_opts& o, sc_
   if (o.precond != SC_PREC_JACOBI) {
     if (ir && o.precond == SC_PREC_FDIAG64)
       return fail(SC_ERR_ARG, "SC_SOLVE_IR32 takes the FP32 or bf16 fast-diagonalisation preconditioner");
-    const int key = c.d.bcR | c.d.bcT << 1 | c.d.bcL << 2 | c.d.bcB << 3;
-    if (!c.fd) c.fd = new FdDiag;
-    if (c.fd->bcKey != key || c.fd_stale) {
-      e = fdiag_build(c, *c.fd);
-      if (e) return e;
-      c.fd_stale = false;
-    }
+    e = ensure_fdiag(c);
+    if (e) return e;
     if (!c.z64) SC_HIP(hipMalloc(&c.z64, (size_t)c.N * sizeof(double)));
     if (!c.zold64) SC_HIP(hipMalloc(&c.zold64, (size_t)c.N * sizeof(double)));
     const int fprec = o.precond == SC_PREC_FDIAG64 ? FD_F64 : (o.precond == SC_PREC_FDIAG32 ? FD_F32 : FD_BF16);
@@ -1245,6 +1276,11 @@ extern "C" int sc_solve(sc_ctx* h, const double* rhs_d, double* u_d, const sc_so
   int e = sc::solve(h->c, rhs_d, u_d, *opts, st);
   if (stats) *stats = st;
   return e;
+}
+
+extern "C" int sc_precond_apply(sc_ctx* h, int precond, const double* r_d, double* z_d) {
+  if (!h || !r_d || !z_d) return sc::fail(SC_ERR_ARG, "sc_precond_apply: null argument");
+  return sc::precond_apply(h->c, precond, r_d, z_d);
 }
 
 // ------------------------------------------------ allocation-free solve --

@@ -1,5 +1,6 @@
 // Fast-diagonalisation preconditioner: set-up (1-D operators, generalised
-// eigen-decompositions with rocSOLVER) and application (four GEMMs, rocBLAS).
+// eigen-decompositions with rocSOLVER) and application (four GEMMs on the
+// matrix cores, this file's own MFMA kernels).
 #include <hip/hip_bf16.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -7,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "fdiag.hpp"
 
@@ -132,39 +134,422 @@ int gen_eig(rocblas_handle hb, hipStream_t s, int n, std::vector<double> L,
   return 0;
 }
 
-__global__ void k_to_f32(size_t n, const double* __restrict__ a, float* __restrict__ b) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    b[i] = (float)a[i];
-}
-__global__ void k_to_f64(size_t n, const float* __restrict__ a, double* __restrict__ b) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    b[i] = (double)a[i];
-}
-template <typename T>
-__global__ void k_scale(size_t n, T* __restrict__ a, const T* __restrict__ s) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    a[i] *= s[i];
-}
-__global__ void k_f64_to_bf16(size_t n, const double* __restrict__ a, __hip_bfloat16* __restrict__ b) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    b[i] = __float2bfloat16((float)a[i]);
-}
-__global__ void k_f32_to_bf16(size_t n, const float* __restrict__ a, const float* __restrict__ s,
-                              __hip_bfloat16* __restrict__ b) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    b[i] = __float2bfloat16(s ? a[i] * s[i] : a[i]);
+// ----------------------------------------------------------- MFMA GEMMs --
+// C = A B on the matrix cores, column-major operands (the preconditioner's
+// vectors are z-fast, i.e. column-major Nz x Ny):
+//   A  M x K (lda), zero-padded factors or intermediates: read unchecked;
+//   B  K x N (ldb): rows >= kB or columns >= nB read as zero (the caller's
+//      unpadded vector), converted to the GEMM's input type on load;
+//   C  M x N (ldc): rows >= mC or columns >= nC are not stored (the caller's
+//      unpadded vector), converted from the accumulator on store;
+//   S  optional M x N scale in the accumulator type: C = (A B) .* S.
+// M, N, K are multiples of kFdTile.  FP64 (v_mfma_f64_16x16x4f64) and FP32
+// (v_mfma_f32_16x16x4f32): a block owns a BM x BN tile of C; its waves form
+// KS groups of WM x WN waves, each group multiplying its share of every K
+// slab (split K inside the block, summed through LDS at the end), so that
+// a grid with one tile per CU still runs two or more waves per SIMD.  K
+// runs in slabs of BK through two LDS buffers: after each barrier the next
+// slab is stored and the one after it loaded before the current slab is
+// multiplied.  LDS layout: A as [k][m] (rows padded so the four 16-lane
+// row groups of a fragment read fall on different banks), B as [n][k] in
+// its global k-fast order (pitch BK + 16 bytes: conflict-free fragment
+// reads and aligned 16-byte stores); 16-byte global loads wherever the
+// operand is padded.  bf16 (v_mfma_f32_16x16x16bf16_1k, four consecutive k
+// a lane): 64 x 64 tiles, A and B both k-fast in LDS.
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef short s4v __attribute__((ext_vector_type(4)));
+
+template <int KIND>
+struct Mfma;
+template <>
+struct Mfma<0> {
+  using T = double;
+  using Acc = double;
+  using V = d4v;
+  __device__ static V op(T a, T b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  // row of accumulator r of lane group g inside a 16 x 16 tile (interleaved
+  // for the FP64 instruction)
+  __device__ static int row(int g, int r) { return g + 4 * r; }
+};
+template <>
+struct Mfma<1> {
+  using T = float;
+  using Acc = float;
+  using V = f4v;
+  __device__ static V op(T a, T b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int g, int r) { return 4 * g + r; }
+};
+template <>
+struct Mfma<2> {
+  using T = unsigned short;  // bf16 bits
+  using Acc = float;
+  using V = f4v;
+  __device__ static int row(int g, int r) { return 4 * g + r; }
+};
+
+// round to nearest even FP32 -> bf16 (NaN kept quiet)
+__device__ __forceinline__ unsigned short bf16_bits(float x) {
+  const unsigned u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
+  return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
-unsigned nblk(size_t n) {
-  size_t b = (n + 255) / 256;
-  return (unsigned)std::min<size_t>(std::max<size_t>(b, 1), 4096);
+template <int KIND, typename TB>
+__device__ __forceinline__ typename Mfma<KIND>::T to_op(TB x) {
+  if constexpr (KIND == 0) return (double)x;
+  else if constexpr (KIND == 1) return (float)x;
+  else if constexpr (std::is_same<TB, unsigned short>::value) return x;
+  else return bf16_bits((float)x);
+}
+template <typename TC, typename Acc>
+__device__ __forceinline__ TC from_acc(Acc x) {
+  if constexpr (std::is_same<TC, unsigned short>::value) return bf16_bits((float)x);
+  else return (TC)x;
+}
+
+template <int KIND, typename TB, typename TC>
+struct GemmArgs {
+  const typename Mfma<KIND>::T* A;
+  int lda;
+  const TB* B;
+  int ldb, kB, nB;
+  TC* C;
+  int ldc, mC, nC;
+  const typename Mfma<KIND>::Acc* S;
+  int lds;
+  int M, N, K;
+};
+
+// the tile configurations (measured at 448^2 and 1024^2 on MI355X)
+template <int BM_, int BN_, int BK_, int WM_, int WN_, int KS_>
+struct GemmCfg {
+  static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_, KS = KS_;
+  static constexpr int NT = 64 * WM * WN * KS;
+};
+using CfgF64Large = GemmCfg<64, 64, 32, 2, 4, 1>;
+using CfgF64Small = GemmCfg<32, 32, 32, 2, 2, 1>;
+using CfgF32Large = GemmCfg<64, 64, 32, 2, 2, 2>;
+using CfgF32Small = GemmCfg<32, 32, 32, 2, 2, 2>;
+
+// the block's tile: each XCD (block id mod 8) takes a contiguous run of
+// tiles, M fastest
+__device__ __forceinline__ void gemm_tile(int M, int N, int BM, int BN, int* m0, int* n0) {
+  const int tilesM = M / BM, nt = tilesM * (N / BN);
+  int b = blockIdx.x;
+  if ((nt & 7) == 0) b = (b & 7) * (nt >> 3) + (b >> 3);
+  *m0 = (b % tilesM) * BM;
+  *n0 = (b / tilesM) * BN;
+}
+
+// FP64 / FP32.  VECB: 0 for the caller's unpadded vector as B (element
+// loads, bounds-checked), 1 for a padded B (loads of up to 16 bytes)
+template <int KIND, class CF, int VECB, typename TB, typename TC>
+__global__ __launch_bounds__(CF::NT) void k_fd_gemm(GemmArgs<KIND, TB, TC> g) {
+  using P = Mfma<KIND>;
+  using T = typename P::T;
+  using V = typename P::V;
+  using Acc = typename P::Acc;
+  constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, WM = CF::WM, WN = CF::WN, KS = CF::KS, NT = CF::NT;
+  constexpr int V16 = 16 / (int)sizeof(T);    // elements of a 16-byte load
+  // elements a thread loads at once: up to 16 bytes, at most its share of a slab
+  constexpr int VE = BM * BK / NT < V16 ? BM * BK / NT : V16;
+  constexpr int VB = VECB ? (BN * BK / NT < V16 ? BN * BK / NT : V16) : 1;
+  constexpr int TM = BM / WM / 16, TN = BN / WN / 16;
+  constexpr int PMN = BM + 16;                 // [k][m] rows of A
+  constexpr int PNK = BK + V16;                // [n][k] rows of B
+  constexpr int ASZ = BK * PMN, BSZ = BN * PNK;
+  static_assert(BM * BN <= 2 * (ASZ + BSZ), "the k-groups' reduction fits in the slab buffers");
+  __shared__ __attribute__((aligned(16))) T smem[2 * (ASZ + BSZ)];
+  T* const As0 = smem;
+  T* const Bs0 = smem + 2 * ASZ;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int m0, n0;
+  gemm_tile(g.M, g.N, BM, BN, &m0, &n0);
+  // staging: A VE consecutive m a thread; B VB consecutive k a thread
+  constexpr int LA = BM * BK / NT / VE, LB = BN * BK / NT / VB;
+  static_assert(LA >= 1 && LB >= 1 && BM * BK % (NT * VE) == 0 && BN * BK % (NT * VB) == 0, "staging");
+  constexpr int AT = BM / VE, BT = BK / VB;
+  typedef T TA __attribute__((ext_vector_type(VE)));
+  typedef T TBV __attribute__((ext_vector_type(VB)));
+  const int am = (tid % AT) * VE, ak = tid / AT;
+  const int bk = (tid % BT) * VB, bn = tid / BT;
+  TA ra[2][LA];
+  TBV rb[2][LB];
+  const T* pa = g.A + (size_t)ak * g.lda + m0 + am;
+  auto gload = [&](TA (&xa)[LA], TBV (&xb)[LB], int k0) {
+    const T* qa = pa + (size_t)k0 * g.lda;
+#pragma unroll
+    for (int i = 0; i < LA; ++i) xa[i] = *reinterpret_cast<const TA*>(qa + (size_t)i * (NT / AT) * g.lda);
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+      const int n = n0 + bn + (NT / BT) * i, k = k0 + bk;
+      if constexpr (!VECB) {
+        const bool ok = k < g.kB && n < g.nB;
+        const TB v = g.B[(size_t)(ok ? n : 0) * g.ldb + (ok ? k : 0)];  // unconditional, clamped
+        xb[i] = ok ? to_op<KIND>(v) : to_op<KIND>((TB)0);
+      } else {
+        static_assert(std::is_same<TB, T>::value, "16-byte B loads take the operand type");
+        xb[i] = *reinterpret_cast<const TBV*>(g.B + (size_t)n * g.ldb + k);
+      }
+    }
+  };
+  auto sstore = [&](const TA (&xa)[LA], const TBV (&xb)[LB], int buf) {
+#pragma unroll
+    for (int i = 0; i < LA; ++i)
+      *reinterpret_cast<TA*>(As0 + buf * ASZ + (ak + (NT / AT) * i) * PMN + am) = xa[i];
+#pragma unroll
+    for (int i = 0; i < LB; ++i)
+      *reinterpret_cast<TBV*>(Bs0 + buf * BSZ + (bn + (NT / BT) * i) * PNK + bk) = xb[i];
+  };
+  V acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = V{0, 0, 0, 0};
+  const int kg = wid / (WM * WN), wi = wid % (WM * WN);
+  const int wm = (wi % WM) * (BM / WM), wn = (wi / WM) * (BN / WN);
+  const int lr = lane & 15, lk = lane >> 4;
+  constexpr int KPG = BK / KS;
+  auto compute = [&](int buf) {
+    const T* as = As0 + buf * ASZ;
+    const T* bs = Bs0 + buf * BSZ;
+#pragma unroll
+    for (int kk = 0; kk < KPG; kk += 4) {
+      const int k = kg * KPG + kk + lk;
+      T a[TM], b[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) a[i] = as[k * PMN + wm + 16 * i + lr];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) b[j] = bs[(wn + 16 * j + lr) * PNK + k];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = P::op(a[i], b[j], acc[i][j]);
+    }
+  };
+  const int S = g.K / BK;
+  gload(ra[0], rb[0], 0);
+  if (S > 1) gload(ra[1], rb[1], BK);
+  sstore(ra[0], rb[0], 0);
+  for (int s0 = 0; s0 < S; s0 += 2) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int s = s0 + p;
+      if (s < S) {
+        __syncthreads();
+        if (s + 1 < S) sstore(ra[p ^ 1], rb[p ^ 1], (s + 1) & 1);
+        if (s + 2 < S) gload(ra[p], rb[p], (s + 2) * BK);
+        compute(s & 1);
+      }
+    }
+  }
+  // the k-groups' partial tiles summed through LDS into group 0
+  if constexpr (KS > 1) {
+    Acc* red = reinterpret_cast<Acc*>(smem);
+    for (int q = 1; q < KS; ++q) {
+      __syncthreads();
+      if (kg == q) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              red[(wn + 16 * j + lr) * BM + wm + 16 * i + P::row(lk, r)] = acc[i][j][r];
+      }
+      __syncthreads();
+      if (kg == 0) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[i][j][r] += red[(wn + 16 * j + lr) * BM + wm + 16 * i + P::row(lk, r)];
+      }
+    }
+    if (kg != 0) return;
+  }
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int n = n0 + wn + 16 * j + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm + 16 * i + P::row(lk, r);
+        Acc v = acc[i][j][r];
+        if (g.S) v *= g.S[(size_t)n * g.lds + m];
+        if (m < g.mC && n < g.nC) g.C[(size_t)n * g.ldc + m] = from_acc<TC>(v);
+      }
+    }
+}
+
+// bf16 inputs, FP32 accumulation.  Both operands arrive k-fast (A as the
+// transpose of its column-major form, lda its row length; B column-major),
+// so both go to LDS in their global order with 16-byte loads and stores:
+// A as [m][k], B as [n][k] (rows of BK + 8, conflict-free 8-byte fragment
+// reads).  64 x 64 tiles, four waves of 32 x 32, K in slabs of 64.  TRC:
+// C is stored transposed (row-major, ldc its row length), as the k-fast A
+// of the next GEMM.
+template <bool TRC, typename TC>
+__global__ __launch_bounds__(256) void k_fd_gemm_bf16(GemmArgs<2, unsigned short, TC> g) {
+  using T = unsigned short;
+  using V = f4v;
+  constexpr int BM = 64, BN = 64, BK = 64, PK = BK + 8, ASZ = BM * PK;
+  typedef T T8 __attribute__((ext_vector_type(8)));
+  __shared__ __attribute__((aligned(16))) T As[2][ASZ];
+  __shared__ __attribute__((aligned(16))) T Bs[2][ASZ];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int m0, n0;
+  gemm_tile(g.M, g.N, BM, BN, &m0, &n0);
+  // a thread moves 8 consecutive k of two rows of A and two columns of B
+  const int kq = (tid & 7) * 8, rq = tid >> 3;  // rq in [0, 32)
+  T8 ra[2][2], rb[2][2];
+  auto gload = [&](T8 (&xa)[2], T8 (&xb)[2], int k0) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      xa[i] = *reinterpret_cast<const T8*>(g.A + (size_t)(m0 + rq + 32 * i) * g.lda + k0 + kq);
+      xb[i] = *reinterpret_cast<const T8*>(g.B + (size_t)(n0 + rq + 32 * i) * g.ldb + k0 + kq);
+    }
+  };
+  auto sstore = [&](const T8 (&xa)[2], const T8 (&xb)[2], int buf) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      *reinterpret_cast<T8*>(&As[buf][(rq + 32 * i) * PK + kq]) = xa[i];
+      *reinterpret_cast<T8*>(&Bs[buf][(rq + 32 * i) * PK + kq]) = xb[i];
+    }
+  };
+  V acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = V{0, 0, 0, 0};
+  const int wm = (wid & 1) * 32, wn = (wid >> 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  auto compute = [&](int buf) {
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 16) {
+      const int k = kk + 4 * lk;
+      const s4v a0 = *reinterpret_cast<const s4v*>(&As[buf][(wm + lr) * PK + k]);
+      const s4v a1 = *reinterpret_cast<const s4v*>(&As[buf][(wm + 16 + lr) * PK + k]);
+      const s4v b0 = *reinterpret_cast<const s4v*>(&Bs[buf][(wn + lr) * PK + k]);
+      const s4v b1 = *reinterpret_cast<const s4v*>(&Bs[buf][(wn + 16 + lr) * PK + k]);
+      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1, acc[1][1], 0, 0, 0);
+    }
+  };
+  const int S = g.K / BK;
+  gload(ra[0], rb[0], 0);
+  if (S > 1) gload(ra[1], rb[1], BK);
+  sstore(ra[0], rb[0], 0);
+  for (int s0 = 0; s0 < S; s0 += 2) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int s = s0 + p;
+      if (s < S) {
+        __syncthreads();
+        if (s + 1 < S) sstore(ra[p ^ 1], rb[p ^ 1], (s + 1) & 1);
+        if (s + 2 < S) gload(ra[p], rb[p], (s + 2) * BK);
+        compute(s & 1);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int n = n0 + wn + 16 * j + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm + 16 * i + Mfma<2>::row(lk, r);
+        float v = acc[i][j][r];
+        if (g.S) v *= g.S[(size_t)n * g.lds + m];
+        if (m < g.mC && n < g.nC) g.C[TRC ? (size_t)m * g.ldc + n : (size_t)n * g.ldc + m] = from_acc<TC>(v);
+      }
+    }
+}
+
+// the vector r (Nz x Ny, z fast) into the padded Pz x Py operand layout of
+// the first GEMM, converted to its type, zero in the padding
+template <int KIND, typename TI>
+__global__ __launch_bounds__(256) void k_fd_pad(const TI* __restrict__ r, int Nz, int Ny, int Pz, int Py,
+                                                typename Mfma<KIND>::T* __restrict__ out) {
+  const size_t n = (size_t)Pz * Py;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(i % Pz), j = (int)(i / Pz);
+    out[i] = (k < Nz && j < Ny) ? to_op<KIND>(r[(size_t)j * Nz + k]) : to_op<KIND>((TI)0);
+  }
+}
+
+// one GEMM with the configuration for its size: 64 x 64 tiles when the
+// grid has one per CU or more, 32 x 32 tiles below
+template <int KIND, int VECB, typename TB, typename TC>
+void gemm(hipStream_t s, const typename Mfma<KIND>::T* A, int lda, const TB* B, int ldb, int kB, int nB,
+          TC* C, int ldc, int mC, int nC, const typename Mfma<KIND>::Acc* S, int lds, int M, int N, int K) {
+  GemmArgs<KIND, TB, TC> g{A, lda, B, ldb, kB, nB, C, ldc, mC, nC, S, lds, M, N, K};
+  if constexpr (KIND == 2) {
+    static_assert(KIND != 2, "bf16 GEMMs go through gemm_bf16");
+  } else {
+    using Large = std::conditional_t<KIND == 0, CfgF64Large, CfgF32Large>;
+    using Small = std::conditional_t<KIND == 0, CfgF64Small, CfgF32Small>;
+    if ((M / 64) * (N / 64) >= 200) {
+      const unsigned nb = (unsigned)((M / Large::BM) * (N / Large::BN));
+      hipLaunchKernelGGL((k_fd_gemm<KIND, Large, VECB, TB, TC>), dim3(nb), dim3(Large::NT), 0, s, g);
+    } else {
+      const unsigned nb = (unsigned)((M / Small::BM) * (N / Small::BN));
+      hipLaunchKernelGGL((k_fd_gemm<KIND, Small, VECB, TB, TC>), dim3(nb), dim3(Small::NT), 0, s, g);
+    }
+  }
+}
+
+template <bool TRC, typename TC>
+void gemm_bf16(hipStream_t s, const unsigned short* A, int lda, const unsigned short* B, int ldb, TC* C, int ldc,
+               int mC, int nC, const float* S, int lds, int M, int N, int K) {
+  GemmArgs<2, unsigned short, TC> g{A, lda, B, ldb, K, N, C, ldc, mC, nC, S, lds, M, N, K};
+  hipLaunchKernelGGL((k_fd_gemm_bf16<TRC, TC>), dim3((unsigned)((M / 64) * (N / 64))), dim3(256), 0, s, g);
+}
+
+unsigned pad_blocks(const FdDiag& f) {
+  const size_t b = ((size_t)f.Pz * f.Py + 255) / 256;
+  return (unsigned)(b < 4096 ? b : 4096);
+}
+
+// z = W (inv .* (W^T R V)) V^T with R the z-fast vector r (column-major
+// Nz x Ny): r padded into t, then four GEMMs, the scale in the second one's
+// store.  TI / TO: the vectors' types; the intermediates are in the GEMM's
+// operand type.
+template <int KIND, typename TI, typename TO>
+void fd_transform(hipStream_t s, const FdDiag& f, const typename Mfma<KIND>::T* V,
+                  const typename Mfma<KIND>::T* VT, const typename Mfma<KIND>::T* W,
+                  const typename Mfma<KIND>::T* WT, const typename Mfma<KIND>::Acc* inv,
+                  typename Mfma<KIND>::T* g, typename Mfma<KIND>::T* t, const TI* r, TO* z) {
+  using TM = typename Mfma<KIND>::T;
+  using Acc = typename Mfma<KIND>::Acc;
+  const int Ny = f.Ny, Nz = f.Nz, Py = f.Py, Pz = f.Pz;
+  hipLaunchKernelGGL((k_fd_pad<KIND, TI>), dim3(pad_blocks(f)), dim3(256), 0, s, r, Nz, Ny, Pz, Py, t);
+  if constexpr (KIND == 2) {
+    // k-fast operands: W^T's rows are W's columns, and the first and third
+    // products are stored transposed for the GEMM after them
+    gemm_bf16<true, TM>(s, W, Pz, t, Pz, g, Py, Pz, Py, (const Acc*)nullptr, 0, Pz, Py, Pz);
+    gemm_bf16<false, TM>(s, g, Py, V, Py, t, Pz, Pz, Py, inv, Pz, Pz, Py, Py);
+    gemm_bf16<true, TM>(s, WT, Pz, t, Pz, g, Py, Pz, Py, (const Acc*)nullptr, 0, Pz, Py, Pz);
+    gemm_bf16<false, TO>(s, g, Py, VT, Py, z, Nz, Nz, Ny, (const Acc*)nullptr, 0, Pz, Py, Py);
+  } else {
+    gemm<KIND, 1, TM, TM>(s, WT, Pz, t, Pz, Pz, Py, g, Pz, Pz, Py, (const Acc*)nullptr, 0, Pz, Py, Pz);
+    gemm<KIND, 1, TM, TM>(s, g, Pz, V, Py, Py, Py, t, Pz, Pz, Py, inv, Pz, Pz, Py, Py);
+    gemm<KIND, 1, TM, TM>(s, W, Pz, t, Pz, Pz, Py, g, Pz, Pz, Py, (const Acc*)nullptr, 0, Pz, Py, Pz);
+    gemm<KIND, 1, TM, TO>(s, g, Pz, VT, Py, Py, Py, z, Nz, Nz, Ny, (const Acc*)nullptr, 0, Pz, Py, Py);
+  }
 }
 
 }  // namespace
 
 FdDiag::~FdDiag() {
   void* bufs[] = {V64, W64, VT64, WT64, inv64, V32, W32, VT32, WT32, inv32, Vb, Wb, VTb, WTb,
-                  g64, t64, r32, g32, t32, rb, gb};
+                  g64, t64, g32, t32, gb, tb};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (blas) rocblas_destroy_handle((rocblas_handle)blas);
@@ -172,8 +557,9 @@ FdDiag::~FdDiag() {
 
 int fdiag_build(Ctx& c, FdDiag& f) {
   const int Ny = c.d.Ny, Nz = c.d.Nz;
-  const size_t N = (size_t)Ny * Nz;
   if (c.d.laplace != 0) return fail(SC_ERR_ARG, "fast-diagonalisation preconditioner needs laplace type yz");
+  const int Py = (Ny + kFdTile - 1) / kFdTile * kFdTile, Pz = (Nz + kFdTile - 1) / kFdTile * kFdTile;
+  const bool resize = f.Py != Py || f.Pz != Pz;
   f.Ny = Ny;
   f.Nz = Nz;
   // depth profile of mu (column mean over y when mu is not a function of z alone)
@@ -213,13 +599,35 @@ int fdiag_build(Ctx& c, FdDiag& f) {
   if (int e = gen_eig((rocblas_handle)f.blas, c.stream, Ny, std::move(Ly), Dy, f.lam, V, &a1)) return e;
   if (int e = gen_eig((rocblas_handle)f.blas, c.stream, Nz, std::move(Lz), Dz, f.theta, W, &a2)) return e;
   f.maxAsym = std::max(a1, a2);
-  std::vector<double> inv(N);
+  // the factors zero-padded to Py x Py / Pz x Pz (column-major), their
+  // transposes, and the inverse eigenvalue sums at (k, j) of a Pz x Py array
+  auto pad = [](const std::vector<double>& a, int n, int p, bool trans) {
+    std::vector<double> o((size_t)p * p, 0.0);
+    for (int col = 0; col < n; ++col)
+      for (int row = 0; row < n; ++row)
+        o[(size_t)col * p + row] = trans ? a[(size_t)row * n + col] : a[(size_t)col * n + row];
+    return o;
+  };
+  const std::vector<double> Vp = pad(V, Ny, Py, false), VTp = pad(V, Ny, Py, true);
+  const std::vector<double> Wp = pad(W, Nz, Pz, false), WTp = pad(W, Nz, Pz, true);
+  std::vector<double> inv((size_t)Pz * Py, 0.0);
   const double lmax = std::max(std::fabs(f.lam.back()), std::fabs(f.theta.back()));
   for (int j = 0; j < Ny; ++j)
     for (int k = 0; k < Nz; ++k) {
       const double den = -(f.lam[j] + f.theta[k]);
-      inv[(size_t)j * Nz + k] = std::fabs(den) > 1e-13 * lmax ? 1.0 / den : 0.0;
+      inv[(size_t)j * Pz + k] = std::fabs(den) > 1e-13 * lmax ? 1.0 / den : 0.0;
     }
+  if (resize) {
+    void* bufs[] = {f.V64, f.W64, f.VT64, f.WT64, f.inv64, f.V32, f.W32, f.VT32, f.WT32, f.inv32,
+                    f.Vb, f.Wb, f.VTb, f.WTb, f.g64, f.t64, f.g32, f.t32, f.gb, f.tb};
+    for (void* p : bufs)
+      if (p) (void)hipFree(p);
+    f.V64 = f.W64 = f.VT64 = f.WT64 = f.inv64 = f.g64 = f.t64 = nullptr;
+    f.V32 = f.W32 = f.VT32 = f.WT32 = f.inv32 = f.g32 = f.t32 = nullptr;
+    f.Vb = f.Wb = f.VTb = f.WTb = f.gb = f.tb = nullptr;
+  }
+  f.Py = Py;
+  f.Pz = Pz;
   auto up64 = [&](double*& p, const std::vector<double>& h) -> int {
     if (!p) SC_HIP(hipMalloc(&p, h.size() * sizeof(double)));
     SC_HIP(hipMemcpy(p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -238,131 +646,58 @@ int fdiag_build(Ctx& c, FdDiag& f) {
     SC_HIP(hipMemcpy(p, t.data(), t.size() * sizeof(__hip_bfloat16), hipMemcpyHostToDevice));
     return 0;
   };
-  // explicit transposes, so that all four GEMMs run in the faster NN form
-  auto transpose = [](const std::vector<double>& a, int n) {
-    std::vector<double> t(a.size());
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j) t[(size_t)j * n + i] = a[(size_t)i * n + j];
-    return t;
-  };
-  const std::vector<double> VT = transpose(V, Ny), WT = transpose(W, Nz);
-  int e = up64(f.V64, V);
-  if (!e) e = up64(f.W64, W);
-  if (!e) e = up64(f.VT64, VT);
-  if (!e) e = up64(f.WT64, WT);
+  int e = up64(f.V64, Vp);
+  if (!e) e = up64(f.W64, Wp);
+  if (!e) e = up64(f.VT64, VTp);
+  if (!e) e = up64(f.WT64, WTp);
   if (!e) e = up64(f.inv64, inv);
-  if (!e) e = up32(f.V32, V);
-  if (!e) e = up32(f.W32, W);
-  if (!e) e = up32(f.VT32, VT);
-  if (!e) e = up32(f.WT32, WT);
+  if (!e) e = up32(f.V32, Vp);
+  if (!e) e = up32(f.W32, Wp);
+  if (!e) e = up32(f.VT32, VTp);
+  if (!e) e = up32(f.WT32, WTp);
   if (!e) e = up32(f.inv32, inv);
-  if (!e) e = upb(f.Vb, V);
-  if (!e) e = upb(f.Wb, W);
-  if (!e) e = upb(f.VTb, VT);
-  if (!e) e = upb(f.WTb, WT);
+  if (!e) e = upb(f.Vb, Vp);
+  if (!e) e = upb(f.Wb, Wp);
+  if (!e) e = upb(f.VTb, VTp);
+  if (!e) e = upb(f.WTb, WTp);
   if (e) return e;
+  const size_t P = (size_t)Pz * Py;
   if (!f.g64) {
-    SC_HIP(hipMalloc(&f.g64, N * sizeof(double)));
-    SC_HIP(hipMalloc(&f.t64, N * sizeof(double)));
-    SC_HIP(hipMalloc(&f.r32, N * sizeof(float)));
-    SC_HIP(hipMalloc(&f.g32, N * sizeof(float)));
-    SC_HIP(hipMalloc(&f.t32, N * sizeof(float)));
-    SC_HIP(hipMalloc(&f.rb, N * sizeof(__hip_bfloat16)));
-    SC_HIP(hipMalloc(&f.gb, N * sizeof(__hip_bfloat16)));
+    SC_HIP(hipMalloc(&f.g64, P * sizeof(double)));
+    SC_HIP(hipMalloc(&f.t64, P * sizeof(double)));
+    SC_HIP(hipMalloc(&f.g32, P * sizeof(float)));
+    SC_HIP(hipMalloc(&f.t32, P * sizeof(float)));
+    SC_HIP(hipMalloc(&f.gb, P * sizeof(__hip_bfloat16)));
+    SC_HIP(hipMalloc(&f.tb, P * sizeof(__hip_bfloat16)));
   }
-  rocblas_set_stream((rocblas_handle)f.blas, c.stream);
   f.bcKey = c.d.bcR | c.d.bcT << 1 | c.d.bcL << 2 | c.d.bcB << 3;
   return SC_OK;
 }
 
-#define SC_BLAS(call)                                                                  \
-  do {                                                                                 \
-    rocblas_status _s = (call);                                                        \
-    if (_s != rocblas_status_success)                                                  \
-      return fail(SC_ERR_HIP, std::string(#call " -> ") + rocblas_status_to_string(_s)); \
-  } while (0)
-
-// Column-major view: r as R_c (Nz x Ny, ld Nz).  z = W (inv .* (W^T R_c V)) V^T.
 int fdiag_apply(Ctx& c, FdDiag& f, int prec, const double* r, double* z) {
-  const int Ny = f.Ny, Nz = f.Nz;
-  const size_t N = (size_t)Ny * Nz;
   hipStream_t s = c.stream;
-  rocblas_handle hb = (rocblas_handle)f.blas;
-  rocblas_set_stream(hb, s);
-  const rocblas_operation nN = rocblas_operation_none;
-  if (prec == FD_F64) {
-    const double one = 1.0, zero = 0.0;
-    SC_BLAS(rocblas_dgemm(hb, nN, nN, Nz, Ny, Nz, &one, f.WT64, Nz, r, Nz, &zero, f.g64, Nz));
-    SC_BLAS(rocblas_dgemm(hb, nN, nN, Nz, Ny, Ny, &one, f.g64, Nz, f.V64, Ny, &zero, f.t64, Nz));
-    hipLaunchKernelGGL(k_scale<double>, dim3(nblk(N)), dim3(256), 0, s, N, f.t64, f.inv64);
-    SC_BLAS(rocblas_dgemm(hb, nN, nN, Nz, Ny, Nz, &one, f.W64, Nz, f.t64, Nz, &zero, f.g64, Nz));
-    SC_BLAS(rocblas_dgemm(hb, nN, nN, Nz, Ny, Ny, &one, f.g64, Nz, f.VT64, Ny, &zero, z, Nz));
-  } else if (prec == FD_F32) {
-    const float one = 1.0f, zero = 0.0f;
-    hipLaunchKernelGGL(k_to_f32, dim3(nblk(N)), dim3(256), 0, s, N, r, f.r32);
-    SC_BLAS(rocblas_sgemm(hb, nN, nN, Nz, Ny, Nz, &one, f.WT32, Nz, f.r32, Nz, &zero, f.g32, Nz));
-    SC_BLAS(rocblas_sgemm(hb, nN, nN, Nz, Ny, Ny, &one, f.g32, Nz, f.V32, Ny, &zero, f.t32, Nz));
-    hipLaunchKernelGGL(k_scale<float>, dim3(nblk(N)), dim3(256), 0, s, N, f.t32, f.inv32);
-    SC_BLAS(rocblas_sgemm(hb, nN, nN, Nz, Ny, Nz, &one, f.W32, Nz, f.t32, Nz, &zero, f.g32, Nz));
-    SC_BLAS(rocblas_sgemm(hb, nN, nN, Nz, Ny, Ny, &one, f.g32, Nz, f.VT32, Ny, &zero, f.t32, Nz));
-    hipLaunchKernelGGL(k_to_f64, dim3(nblk(N)), dim3(256), 0, s, N, f.t32, z);
-  } else {
-    // bf16 inputs, FP32 accumulation and output; intermediates re-rounded to bf16
-    const float one = 1.0f, zero = 0.0f;
-    const rocblas_datatype bf = rocblas_datatype_bf16_r, f32 = rocblas_datatype_f32_r;
-    __hip_bfloat16* rb = (__hip_bfloat16*)f.rb;
-    __hip_bfloat16* gb = (__hip_bfloat16*)f.gb;
-    hipLaunchKernelGGL(k_f64_to_bf16, dim3(nblk(N)), dim3(256), 0, s, N, r, rb);
-    SC_BLAS(rocblas_gemm_ex(hb, nN, nN, Nz, Ny, Nz, &one, f.WTb, bf, Nz, rb, bf, Nz, &zero, f.g32, f32,
-                            Nz, f.g32, f32, Nz, f32, rocblas_gemm_algo_standard, 0, 0));
-    hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk(N)), dim3(256), 0, s, N, f.g32, (const float*)nullptr, gb);
-    SC_BLAS(rocblas_gemm_ex(hb, nN, nN, Nz, Ny, Ny, &one, gb, bf, Nz, f.Vb, bf, Ny, &zero, f.t32, f32,
-                            Nz, f.t32, f32, Nz, f32, rocblas_gemm_algo_standard, 0, 0));
-    hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk(N)), dim3(256), 0, s, N, f.t32, (const float*)f.inv32, rb);
-    SC_BLAS(rocblas_gemm_ex(hb, nN, nN, Nz, Ny, Nz, &one, f.Wb, bf, Nz, rb, bf, Nz, &zero, f.g32, f32,
-                            Nz, f.g32, f32, Nz, f32, rocblas_gemm_algo_standard, 0, 0));
-    hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk(N)), dim3(256), 0, s, N, f.g32, (const float*)nullptr, gb);
-    SC_BLAS(rocblas_gemm_ex(hb, nN, nN, Nz, Ny, Ny, &one, gb, bf, Nz, f.VTb, bf, Ny, &zero, f.t32, f32,
-                            Nz, f.t32, f32, Nz, f32, rocblas_gemm_algo_standard, 0, 0));
-    hipLaunchKernelGGL(k_to_f64, dim3(nblk(N)), dim3(256), 0, s, N, f.t32, z);
-  }
+  using u16 = unsigned short;
+  if (prec == FD_F64)
+    fd_transform<0, double, double>(s, f, f.V64, f.VT64, f.W64, f.WT64, f.inv64, f.g64, f.t64, r, z);
+  else if (prec == FD_F32)
+    fd_transform<1, double, double>(s, f, f.V32, f.VT32, f.W32, f.WT32, f.inv32, f.g32, f.t32, r, z);
+  else
+    fd_transform<2, double, double>(s, f, (const u16*)f.Vb, (const u16*)f.VTb, (const u16*)f.Wb,
+                                         (const u16*)f.WTb, f.inv32, (u16*)f.gb, (u16*)f.tb, r, z);
   SC_HIP(hipGetLastError());
   return SC_OK;
 }
 
 int fdiag_apply_f32(Ctx& c, FdDiag& f, int prec, const float* r, float* z) {
-  const int Ny = f.Ny, Nz = f.Nz;
-  const size_t N = (size_t)Ny * Nz;
   hipStream_t s = c.stream;
-  rocblas_handle hb = (rocblas_handle)f.blas;
-  rocblas_set_stream(hb, s);
-  const rocblas_operation nN = rocblas_operation_none;
-  const float one = 1.0f, zero = 0.0f;
-  if (prec == FD_F32) {
-    SC_BLAS(rocblas_sgemm(hb, nN, nN, Nz, Ny, Nz, &one, f.WT32, Nz, r, Nz, &zero, f.g32, Nz));
-    SC_BLAS(rocblas_sgemm(hb, nN, nN, Nz, Ny, Ny, &one, f.g32, Nz, f.V32, Ny, &zero, f.t32, Nz));
-    hipLaunchKernelGGL(k_scale<float>, dim3(nblk(N)), dim3(256), 0, s, N, f.t32, f.inv32);
-    SC_BLAS(rocblas_sgemm(hb, nN, nN, Nz, Ny, Nz, &one, f.W32, Nz, f.t32, Nz, &zero, f.g32, Nz));
-    SC_BLAS(rocblas_sgemm(hb, nN, nN, Nz, Ny, Ny, &one, f.g32, Nz, f.VT32, Ny, &zero, z, Nz));
-  } else if (prec == FD_BF16) {
-    const rocblas_datatype bf = rocblas_datatype_bf16_r, f32 = rocblas_datatype_f32_r;
-    __hip_bfloat16* rb = (__hip_bfloat16*)f.rb;
-    __hip_bfloat16* gb = (__hip_bfloat16*)f.gb;
-    hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk(N)), dim3(256), 0, s, N, r, (const float*)nullptr, rb);
-    SC_BLAS(rocblas_gemm_ex(hb, nN, nN, Nz, Ny, Nz, &one, f.WTb, bf, Nz, rb, bf, Nz, &zero, f.g32, f32,
-                            Nz, f.g32, f32, Nz, f32, rocblas_gemm_algo_standard, 0, 0));
-    hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk(N)), dim3(256), 0, s, N, f.g32, (const float*)nullptr, gb);
-    SC_BLAS(rocblas_gemm_ex(hb, nN, nN, Nz, Ny, Ny, &one, gb, bf, Nz, f.Vb, bf, Ny, &zero, f.t32, f32,
-                            Nz, f.t32, f32, Nz, f32, rocblas_gemm_algo_standard, 0, 0));
-    hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk(N)), dim3(256), 0, s, N, f.t32, (const float*)f.inv32, rb);
-    SC_BLAS(rocblas_gemm_ex(hb, nN, nN, Nz, Ny, Nz, &one, f.Wb, bf, Nz, rb, bf, Nz, &zero, f.g32, f32,
-                            Nz, f.g32, f32, Nz, f32, rocblas_gemm_algo_standard, 0, 0));
-    hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk(N)), dim3(256), 0, s, N, f.g32, (const float*)nullptr, gb);
-    SC_BLAS(rocblas_gemm_ex(hb, nN, nN, Nz, Ny, Ny, &one, gb, bf, Nz, f.VTb, bf, Ny, &zero, z, f32,
-                            Nz, z, f32, Nz, f32, rocblas_gemm_algo_standard, 0, 0));
-  } else {
+  using u16 = unsigned short;
+  if (prec == FD_F32)
+    fd_transform<1, float, float>(s, f, f.V32, f.VT32, f.W32, f.WT32, f.inv32, f.g32, f.t32, r, z);
+  else if (prec == FD_BF16)
+    fd_transform<2, float, float>(s, f, (const u16*)f.Vb, (const u16*)f.VTb, (const u16*)f.Wb,
+                                       (const u16*)f.WTb, f.inv32, (u16*)f.gb, (u16*)f.tb, r, z);
+  else
     return fail(SC_ERR_ARG, "the FP32 preconditioner application takes FP32 or bf16 GEMMs");
-  }
   SC_HIP(hipGetLastError());
   return SC_OK;
 }

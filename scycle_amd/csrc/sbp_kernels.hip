@@ -1061,7 +1061,7 @@ static void bench_frame(const Ctx& c, const DevOp<T>& op, const T* a, T* b) {
 // kernel only.
 extern "C" int sc_bench_apply(sc_ctx* h, int which, int reps, double* ms_per_launch,
                               double* nodes) {
-  if (!h || reps <= 0 || !ms_per_launch || which < 0 || which > 10)
+  if (!h || reps <= 0 || !ms_per_launch || which < 0 || which > 13)
     return fail(SC_ERR_ARG, "sc_bench_apply: bad argument");
   if (which >= 9 && h->c.d.laplace != 0) return fail(SC_ERR_ARG, "sc_bench_apply: the wave step needs laplace type yz");
   if (which == 8) {
@@ -1071,6 +1071,35 @@ extern "C" int sc_bench_apply(sc_ctx* h, int which, int reps, double* ms_per_lau
     return e;
   }
   sc::Ctx& c = h->c;
+  if (which >= 11) {  // the fast-diagonalisation preconditioner, r -> z
+    const int prec = which == 11 ? SC_PREC_FDIAG64 : (which == 12 ? SC_PREC_FDIAG32 : SC_PREC_FDIAG_BF16);
+    const size_t N = (size_t)c.N;
+    double *r = nullptr, *z = nullptr;
+    SC_HIP(hipMalloc(&r, N * sizeof(double)));
+    SC_HIP(hipMalloc(&z, N * sizeof(double)));
+    std::vector<double> rh(N);
+    for (size_t i = 0; i < N; ++i) rh[i] = 1.0 + 1e-3 * (double)(i % 97);
+    SC_HIP(hipMemcpy(r, rh.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    int err = 0;
+    for (int w = 0; w < 3 && !err; ++w) err = sc::precond_apply(c, prec, r, z);
+    hipEvent_t e0, e1;
+    SC_HIP(hipEventCreate(&e0));
+    SC_HIP(hipEventCreate(&e1));
+    SC_HIP(hipEventRecord(e0, c.stream));
+    for (int q = 0; q < reps && !err; ++q) err = sc::precond_apply(c, prec, r, z);
+    SC_HIP(hipEventRecord(e1, c.stream));
+    SC_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    SC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_launch = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(r);
+    (void)hipFree(z);
+    // four GEMMs: two Nz x Nz x Ny, two Nz x Ny x Ny
+    if (nodes) *nodes = 4.0 * (double)c.d.Nz * c.d.Ny * ((double)c.d.Ny + c.d.Nz);
+    return err;
+  }
   const bool f64 = which == 9 || (which < 9 && which % 2 == 0);
   const size_t N = (size_t)c.N, esz = f64 ? 8 : 4;
   const int Ny = c.d.Ny, Nz = c.d.Nz;

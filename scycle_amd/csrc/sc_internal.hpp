@@ -268,6 +268,8 @@ struct ProjFramePlan {
 };
 int proj_frame_plan(Ctx& c, const sc_solve_opts& o, ProjFramePlan* plan);
 int bench_proj_guess(Ctx& c, int reps, double* ms, double* words);
+// z = M^-1 r with one of the solve's preconditioners (SC_PREC_*)
+int precond_apply(Ctx& c, int precond, const double* r, double* z);
 // power law: rhs += B gxy + C gxz; strains -> stresses -> viscosity -> rates
 int launch_pl_source(const Ctx& c, const double* gxy, const double* gxz, double* rhs, hipStream_t s);
 // the power law's 2-D output fields at the current state (N each)

@@ -141,6 +141,13 @@ class SbpOps:
         check(load().sc_get_metric(self.h, dptr(yq), dptr(zr)))
         return yq, zr
 
+    def pc_apply(self, r, precond="fdiag64"):
+        """PCApply: z = M^-1 r with one of the solve's preconditioners."""
+        self._a.upload(r)
+        check(load().sc_precond_apply(self.h, PRECONDS[precond], self._a.ptr, self._b.ptr))
+        self.sync()
+        return self._b.to_host()
+
     def solve(self, rhs, u0=None, mode="cg64", rtol=1e-12, inner_rtol=1e-5, max_iters=200000,
               check_every=64, precond="jacobi", reuse=0):
         """KSPSolve(A, rhs, u) with u0 as the initial guess; returns (u, stats).

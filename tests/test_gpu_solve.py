@@ -186,3 +186,33 @@ def test_reuse_projection_guess(mode, precond):
     u, st = g.solve(b, mode=mode, rtol=1e-11, precond=precond, reuse=3, max_iters=20000)
     uref = o2.chol_solve(b)
     assert np.abs(u - uref).max() <= 1e-8 * np.abs(uref).max()
+
+
+@pytest.mark.parametrize("Ny,Nz,var_grid,bc", [(37, 53, True, (0, 1, 0, 1)), (64, 128, False, (0, 1, 0, 1)),
+                                               (130, 70, True, (1, 1, 0, 0)), (21, 21, False, (0, 0, 0, 0))])
+def test_fdiag_apply_is_the_separable_inverse(Ny, Nz, var_grid, bc):
+    """With a constant shear modulus the operator is separable (the two-point
+    mu average is mu itself), so the fast-diagonalisation preconditioner is
+    A^-1: z = M^-1 r must match the oracle's direct solve of A z = r.  Sizes
+    off and on the GEMM tiles' 64 (padding, several tiles, Ny != Nz), both
+    grids and BC sets.  Tolerances: FP64 GEMMs 1e-10; FP32 and bf16 GEMMs by
+    their unit roundoff (6e-8, 4e-3) times the operator's condition number
+    (< 1e4 at these sizes, the error entering through the inverse eigenvalue
+    scaling)."""
+    from scycle_amd import SbpOps
+    y, z = _grid(Ny, Nz, 30.0, 24.0, 5.0) if var_grid else (None, None)
+    mu = np.full(Ny * Nz, 32.0)
+    o = OracleSbp(4, Ny, Nz, 30.0, 24.0, mu, y, z, var_grid=var_grid, bc=bc)
+    g = SbpOps(4, Ny, Nz, 30.0, 24.0, mu, y, z, var_grid=var_grid, bc=bc)
+    rng = np.random.default_rng(Ny * 1000 + Nz)
+    r = rng.standard_normal(Ny * Nz)
+    zref = o.chol_solve(r)
+    tol = {"fdiag64": 1e-10, "fdiag32": 5e-4, "fdiag_bf16": 0.5}
+    for prec in ("fdiag64", "fdiag32", "fdiag_bf16"):
+        zz = g.pc_apply(r, prec)
+        err = np.abs(zz - zref).max() / np.abs(zref).max()
+        print(Ny, Nz, var_grid, bc, prec, f"rel err vs direct solve {err:.3e}", flush=True)
+        assert err <= tol[prec], (prec, err)
+    # Jacobi: r / diag(A)
+    zj = g.pc_apply(r, "jacobi")
+    assert np.allclose(zj, r / o.diag(), rtol=1e-13, atol=0)
