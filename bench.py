@@ -428,16 +428,36 @@ def adjoint_c5(n, steps, ckpt):
                         f"64-node a/b/Dc profiles, checkpoints every {ckpt} steps",
                J=J, forward_seconds=t_fwd)
     grads = {}
-    for label, pc, rtol in (("fp64", "fdiag64", 1e-12), ("mixed", "fdiag32", 1e-10)):
+
+    def timed_adjoint(pc, rtol, green):
         tic = time.perf_counter()
-        ga, gb, gdc, st = m.adjoint(dts, prob.stations, d, ckpt_every=ckpt, rtol=rtol, precond=pc, reuse=4)
+        ga, gb, gdc, st = m.adjoint(dts, prob.stations, d, ckpt_every=ckpt, rtol=rtol, precond=pc, reuse=4,
+                                    green=green)
         sec = time.perf_counter() - tic
-        grads[label] = {q: P.T @ g for q, g in zip(("a", "b", "Dc"), (ga, gb, gdc))}
-        out[label] = dict(adjoint_seconds=sec, cost_over_forward=sec / t_fwd, adj_rtol=rtol, precond=pc,
-                          forward_part_s=st["forward_seconds"], reverse_part_s=st["reverse_seconds"],
-                          adj_solves=st["adj_solves"],
-                          adj_cg_iters_per_solve=st["adj_cg_iters"] / max(st["adj_solves"], 1),
-                          recomputed_steps=st["recomputed_steps"])
+        g = {q: P.T @ x for q, x in zip(("a", "b", "Dc"), (ga, gb, gdc))}
+        rec = dict(adjoint_seconds=sec, cost_over_forward=sec / t_fwd,
+                   forward_part_s=st["forward_seconds"], reverse_part_s=st["reverse_seconds"],
+                   adj_solves=st["adj_solves"],
+                   adj_cg_iters_per_solve=st["adj_cg_iters"] / max(st["adj_solves"], 1),
+                   recomputed_steps=st["recomputed_steps"], green_solves=st["green_solves"],
+                   green_seconds=st["green_seconds"])
+        return g, rec
+
+    for label, pc, rtol in (("fp64", "fdiag64", 1e-12), ("mixed", "fdiag32", 1e-10)):
+        # the reverse pass through the boundary operator: the first gradient
+        # forms it (Nz adjoint solves), the next reuses it
+        g, first = timed_adjoint(pc, rtol, "always")
+        g2, again = timed_adjoint(pc, rtol, "always")
+        grads[label] = g
+        out[label] = dict(adj_rtol=rtol, precond=pc, cost_over_forward=first["cost_over_forward"],
+                          cost_over_forward_reused=again["cost_over_forward"], first=first, reused=again,
+                          reused_vs_first_grad_rel_err=max(
+                              float(np.abs(g2[q] - g[q]).max() / np.abs(g[q]).max()) for q in g))
+    # the same FP64 gradient with one adjoint solve per stage
+    gs, per_stage = timed_adjoint("fdiag64", 1e-12, "never")
+    out["fp64"]["per_stage_solves"] = per_stage
+    out["fp64"]["per_stage_vs_operator_grad_rel_err"] = max(
+        float(np.abs(gs[q] - grads["fp64"][q]).max() / np.abs(grads["fp64"][q]).max()) for q in gs)
     # central difference along a seeded direction, h = 1e-6 relative: the
     # forward's 1e-12 solve tolerance leaves ~1e-10 relative noise on J, so
     # the quotient itself carries ~1e-5 (much larger h is not usable: the

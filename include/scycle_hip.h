@@ -629,13 +629,24 @@ typedef struct sc_adj_desc {
   const int* stationJ_h;      /* station y-indices j_s (node j_s*Nz, z = 0) */
   const double* data_h;       /* d_s (NULL: zero)                           */
   sc_solve_opts adjSolve;
+  /* How the reverse pass applies (1/fts) R_L^T A^-1 S^T to a stage's
+   * fault-traction cotangent.  1: as the Nz x Nz boundary operator formed
+   * once by Nz adjoint solves of unit tractions (kept by the mediator and
+   * reused by later gradients until mu, the BC types or the adjoint solve
+   * options change); 2: one adjoint solve per stage; 0: the operator when
+   * it is already formed or the run has at least Nz / 3 steps. */
+  int greenMode;
 } sc_adj_desc;
 
 typedef struct sc_adj_stats {
   double J;
   double forward_seconds;     /* checkpointed forward                        */
-  double reverse_seconds;     /* recomputation + adjoint steps               */
+  double reverse_seconds;     /* recomputation + adjoint steps (+ forming the
+                                 boundary operator when this call formed it) */
   long long fwd_solves, adj_solves, adj_cg_iters, recomputed_steps;
+  long long green_solves;     /* adjoint solves that formed the boundary
+                                 operator in this call (0: reused or unused)  */
+  double green_seconds;       /* their wall clock, inside reverse_seconds    */
 } sc_adj_stats;
 
 /* Replace the friction parameters a, b, Dc (host arrays, Nz; NULL keeps one). */

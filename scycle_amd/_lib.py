@@ -124,14 +124,15 @@ class QdCounters(C.Structure):
 class AdjDesc(C.Structure):
     _fields_ = [("nSteps", C.c_int), ("dts", C.c_void_p), ("t0", C.c_double),
                 ("ckptEvery", C.c_int), ("nStations", C.c_int), ("stationJ", C.c_void_p),
-                ("data", C.c_void_p), ("adjSolve", SolveOpts)]
+                ("data", C.c_void_p), ("adjSolve", SolveOpts), ("greenMode", C.c_int)]
 
 
 class AdjStats(C.Structure):
     _fields_ = [("J", C.c_double), ("forward_seconds", C.c_double),
                 ("reverse_seconds", C.c_double), ("fwd_solves", C.c_longlong),
                 ("adj_solves", C.c_longlong), ("adj_cg_iters", C.c_longlong),
-                ("recomputed_steps", C.c_longlong)]
+                ("recomputed_steps", C.c_longlong), ("green_solves", C.c_longlong),
+                ("green_seconds", C.c_double)]
 
 
 # Every symbol include/scycle_hip.h declares, with (restype, argtypes).

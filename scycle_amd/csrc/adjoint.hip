@@ -21,12 +21,20 @@
 // The reverse pass recomputes each checkpoint segment forward, keeping psi
 // and V of the three stages of each step (all the VJPs need), then walks the
 // segment backwards.
+// S^T puts a traction cotangent on the five rows next to the fault and R_L^T
+// reads lambda on the nine rows there, both node by node along z, so the
+// map taub -> slipb of a stage is one fixed Nz x Nz matrix
+//   B = (1/fts) R_L^T A^{-1} S^T,
+// column k the adjoint solve of a unit traction at node k.  The reverse pass
+// either solves per stage or applies B, formed once by Nz such solves and
+// kept by the mediator (A does not depend on a, b, Dc).
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
 
 #include "qd_internal.hpp"
+#include "reduce_dev.hpp"
 #include "sbp_dev.hpp"
 
 namespace sc {
@@ -49,7 +57,8 @@ __global__ __launch_bounds__(256) void k_adj_fault(DevOp<double> p, AdjFault P, 
                                                    const double* __restrict__ V, const double* __restrict__ pb,
                                                    const double* __restrict__ sb, double* __restrict__ psi_bar,
                                                    double* __restrict__ ga, double* __restrict__ gb,
-                                                   double* __restrict__ gDc, double* __restrict__ rhs) {
+                                                   double* __restrict__ gDc, double* __restrict__ rhs,
+                                                   double* __restrict__ taub_out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= P.Nz) return;
   const double ps = psi[k], v = V[k], a = P.a[k], b = P.b[k], Dc = P.Dc[k];
@@ -85,10 +94,35 @@ __global__ __launch_bounds__(256) void k_adj_fault(DevOp<double> p, AdjFault P, 
     taub = w;
   }
   psi_bar[k] = psib;
+  if (taub_out) taub_out[k] = taub;
+  if (!rhs) return;
   // S^T: tau_qs[k] = mu_k imet0 sum_{q>=4} D1[q] u[(q-4) Nz + k]
   const double t = p.mu[k] * (p.y.imet[0] * taub);
 #pragma unroll
   for (int q = 4; q < kBand; ++q) rhs[(size_t)(q - 4) * P.Nz + k] = p.y.D1[q] * t;
+}
+
+// S^T e_kk: the rhs rows 0..4 of a unit traction cotangent at fault node kk
+// (k_adj_fault's arithmetic with taub = 1)
+__global__ __launch_bounds__(256) void k_green_rhs(DevOp<double> p, int Nz, int kk, double* __restrict__ rhs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 5 * Nz) return;
+  const int q = i / Nz, k = i - q * Nz;
+  rhs[i] = k == kk ? p.y.D1[q + 4] * (p.mu[kk] * p.y.imet[0]) : 0.0;
+}
+
+// column kk of B: (1/fts) R_L^T lam (k_adj_combine's arithmetic)
+__global__ __launch_bounds__(256) void k_green_col(DevOp<double> p, int Nz, int kk, const double* __restrict__ lam,
+                                                   double invfts, double* __restrict__ B) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= Nz) return;
+  const int nj = min(kBand, p.Ny);
+  double s = 0.0;
+  for (int j = 0; j < nj; ++j) {
+    const double c = rhs_node(p, j, k, [](int side, int) { return side == 0 ? 1.0 : 0.0; });
+    s += c * lam[(size_t)j * Nz + k];
+  }
+  B[(size_t)k * Nz + kk] = invfts * s;
 }
 
 // kb = (psi_bar, (1/fts) R_L^T lam); targets t_i += c_i kb (psi and slip parts).
@@ -114,6 +148,26 @@ __global__ __launch_bounds__(256) void k_adj_combine(DevOp<double> p, int Nz, co
   for (int i = 0; i < T.n; ++i) {
     T.tp[i][k] += T.c[i] * pbv;
     T.ts[i][k] += T.c[i] * sb;
+  }
+}
+
+// the stage's slip cotangent through B (one wave per fault node: row k of
+// B against taub), then the targets as in k_adj_combine
+__global__ __launch_bounds__(256) void k_adj_green(int Nz, const double* __restrict__ B,
+                                                   const double* __restrict__ taub,
+                                                   const double* __restrict__ psi_bar, AdjCombine T) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= Nz) return;  // the whole wave
+  const double* b = B + (size_t)row * Nz;
+  double s = 0.0;
+  for (int k = lane; k < Nz; k += 64) s += b[k] * taub[k];
+  s = wave_sum(s);
+  if (lane == 0) {
+    const double pbv = psi_bar ? psi_bar[row] : 0.0;
+    for (int i = 0; i < T.n; ++i) {
+      T.tp[i][row] += T.c[i] * pbv;
+      T.ts[i][row] += T.c[i] * s;
+    }
   }
 }
 
@@ -186,6 +240,10 @@ class AdjDriver {
   double *rhs_ = nullptr, *rhsQ_ = nullptr, *lam_ = nullptr;
   double *yb_ = nullptr, *fb_ = nullptr, *pbar_ = nullptr, *grad_ = nullptr;  // yb: 2Nz, fb: 3x2Nz, grad: 3Nz
   long long adjSolves_ = 0, adjIters_ = 0, recomputed_ = 0;
+  double* taub_ = nullptr;     // a stage's traction cotangent (the B path)
+  bool green_ = false;         // the reverse pass applies B
+  long long greenSolves_ = 0;
+  double greenSec_ = 0.0;
 
   bool allocFailed_ = false;
   double* dal(size_t n) {
@@ -206,13 +264,12 @@ class AdjDriver {
   int adj_vjp(const double* psi, const double* V, const double* pb, const double* sb, const AdjCombine& T);
   int adj_solve(const double* rhs);
   int adj_step(int local, double h);
+  bool green_ready() const;
+  int build_green();
 };
 
 AdjDriver::~AdjDriver() {
-  if (adj_) {
-    (void)hipStreamSynchronize(adj_->c.stream);
-    sc_ctx_destroy(adj_);
-  }
+  if (adj_) (void)hipStreamSynchronize(adj_->c.stream);  // the context is the mediator's
   (void)hipStreamSynchronize(q_.ctx->c.stream);
   for (void* p : owned_) (void)hipFree(p);
 }
@@ -244,10 +301,66 @@ int AdjDriver::setup(bool reverse) {
   fb_ = dal(6 * (size_t)Nz_);
   pbar_ = dal((size_t)Nz_);
   grad_ = dal(3 * (size_t)Nz_);
+  taub_ = dal((size_t)Nz_);
   if (allocFailed_) return fail(SC_ERR_HIP, "adjoint: device allocation failed");
-  int e = sc_ctx_create(&q_.desc.grid, q_.y_host.empty() ? nullptr : q_.y_host.data(),
-                        q_.z_host.empty() ? nullptr : q_.z_host.data(), q_.mu_host.data(), &adj_);
-  return e;
+  // the adjoint's own context for the mediator's operator as it stands
+  const Ctx& c = q_.ctx->c;
+  if (!q_.adjCtx || q_.adjCtxGen != c.opGen) {
+    if (q_.adjCtx) {
+      sc_ctx_destroy(q_.adjCtx);
+      q_.adjCtx = nullptr;
+    }
+    q_.adjCtxGen = -1;
+    int e = sc_ctx_create(&c.d, q_.y_host.empty() ? nullptr : q_.y_host.data(),
+                          q_.z_host.empty() ? nullptr : q_.z_host.data(), c.mu_h.data(), &q_.adjCtx);
+    if (e) return e;
+    q_.adjCtxGen = c.opGen;
+  }
+  adj_ = q_.adjCtx;
+  // every gradient starts its adjoint solves from an empty projection basis,
+  // so that the same inputs give the same gradient
+  adj_->c.projCount = adj_->c.projNext = 0;
+  return SC_OK;
+}
+
+bool AdjDriver::green_ready() const {
+  const sc_solve_opts& o = d_.adjSolve;
+  return q_.adjB && q_.adjBGen == q_.ctx->c.opGen && q_.adjBPrec == o.precond && q_.adjBMode == o.mode &&
+         q_.adjBRtol == o.rtol;
+}
+
+// B column by column: the adjoint solve of S^T e_k from zero with the
+// adjoint solve options (no projected guess), then (1/fts) R_L^T of it
+int AdjDriver::build_green() {
+  if (green_ready()) return SC_OK;
+  const double t0 = now_s();
+  const size_t N = (size_t)q_.N;
+  if (!q_.adjB) SC_HIP(hipMalloc(&q_.adjB, (size_t)Nz_ * Nz_ * sizeof(double)));
+  q_.adjBGen = -1;  // incomplete until the last column
+  hipStream_t s = adj_->c.stream;
+  sc_solve_opts o = d_.adjSolve;
+  o.reuse = 0;
+  const double invfts = 1.0 / q_.desc.faultTypeScale;
+  for (int kk = 0; kk < Nz_; ++kk) {
+    hipLaunchKernelGGL(k_green_rhs, dim3((5 * Nz_ + 255) / 256), dim3(256), 0, s, adj_->c.op64, Nz_, kk, rhs_);
+    SC_HIP(hipMemsetAsync(lam_, 0, N * sizeof(double), s));
+    sc_solve_stats st;
+    int e = solve(adj_->c, rhs_, lam_, o, st, nullptr, nullptr, true);
+    if (e) return e;
+    greenSolves_ += 1;
+    hipLaunchKernelGGL(k_green_col, dim3((Nz_ + 255) / 256), dim3(256), 0, s, adj_->c.op64, Nz_, kk,
+                       (const double*)lam_, invfts, q_.adjB);
+  }
+  SC_HIP(hipGetLastError());
+  // the next adjoint solve starts from zero, as without the operator
+  SC_HIP(hipMemsetAsync(lam_, 0, N * sizeof(double), s));
+  SC_HIP(hipStreamSynchronize(s));
+  q_.adjBGen = q_.ctx->c.opGen;
+  q_.adjBPrec = d_.adjSolve.precond;
+  q_.adjBMode = d_.adjSolve.mode;
+  q_.adjBRtol = d_.adjSolve.rtol;
+  greenSec_ = now_s() - t0;
+  return SC_OK;
 }
 
 // One RK32 step of size h from the current state at time t (the stages of
@@ -376,8 +489,16 @@ int AdjDriver::adj_vjp(const double* psi, const double* V, const double* pb, con
   hipStream_t s = adj_->c.stream;
   const AdjFault P{Nz_, q_.a_d, q_.b_d, q_.Dc_d, q_.sN_d, q_.eta_d, q_.locked_d, q_.desc.f0, q_.desc.v0};
   const dim3 g((Nz_ + 255) / 256), b(256);
+  if (green_) {
+    hipLaunchKernelGGL(k_adj_fault, g, b, 0, s, adj_->c.op64, P, psi, V, pb, sb, pbar_, grad_, grad_ + Nz_,
+                       grad_ + 2 * Nz_, (double*)nullptr, taub_);
+    hipLaunchKernelGGL(k_adj_green, dim3((Nz_ + 3) / 4), b, 0, s, Nz_, (const double*)q_.adjB,
+                       (const double*)taub_, (const double*)pbar_, T);
+    SC_HIP(hipGetLastError());
+    return SC_OK;
+  }
   hipLaunchKernelGGL(k_adj_fault, g, b, 0, s, adj_->c.op64, P, psi, V, pb, sb, pbar_, grad_, grad_ + Nz_,
-                     grad_ + 2 * Nz_, rhs_);
+                     grad_ + 2 * Nz_, rhs_, (double*)nullptr);
   SC_HIP(hipGetLastError());
   int e = adj_solve(rhs_);
   if (e) return e;
@@ -429,6 +550,10 @@ int AdjDriver::gradient(double* ga_h, double* gb_h, double* gDc_h, sc_adj_stats*
   const double t1 = now_s();
   const long long fwdSolves = q_.counters.solves - c0.solves;
   hipStream_t s = adj_->c.stream;
+  // the reverse pass's adjoint solves: through B when asked, when B is
+  // already formed, or when the run needs at least Nz of them anyway
+  green_ = d_.greenMode == 1 || (d_.greenMode == 0 && (green_ready() || 3LL * d_.nSteps >= Nz_));
+  if (green_ && (e = build_green())) return e;
   // terminal cotangent: yb = (0, (1/fts) R_L^T A^{-1} Q^T (Qu - d))
   if ((e = adj_solve(rhsQ_))) return e;
   {
@@ -467,13 +592,15 @@ int AdjDriver::gradient(double* ga_h, double* gb_h, double* gDc_h, sc_adj_stats*
     st->adj_solves = adjSolves_;
     st->adj_cg_iters = adjIters_;
     st->recomputed_steps = recomputed_;
+    st->green_solves = greenSolves_;
+    st->green_seconds = greenSec_;
   }
   return SC_OK;
 }
 
 int check_adj(const QdDriver& q, const sc_adj_desc* d) {
   if (!d || !d->dts_h || d->nSteps < 1 || d->ckptEvery < 1 || d->nStations < 0 ||
-      (d->nStations > 0 && !d->stationJ_h))
+      (d->nStations > 0 && !d->stationJ_h) || d->greenMode < 0 || d->greenMode > 2)
     return fail(SC_ERR_ARG, "adjoint: bad description");
   if (q.pl) return fail(SC_ERR_ARG, "adjoint: the power-law mediator is not supported");
   if (q.desc.stateLaw != SC_AGING_LAW) return fail(SC_ERR_ARG, "adjoint: only the aging law is differentiated");

@@ -43,6 +43,8 @@ QdDriver::~QdDriver() {
     if (ctlDts_d) (void)hipFree(ctlDts_d);
     drop_stage_graph();
     stage_bufs_free(stageBufs);
+    if (adjB) (void)hipFree(adjB);
+    if (adjCtx) sc_ctx_destroy(adjCtx);
     sc_ctx_destroy(ctx);
   }
 }

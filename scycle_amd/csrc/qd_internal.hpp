@@ -223,6 +223,15 @@ struct QdDriver {
   // (OdeSolver::setInitialStepCount, odeSolver.cpp:25-29)
   int stepBase = 0;
   bool pidFresh = false;  // the PID controller counts its four P steps from stepBase
+  // the discrete adjoint's second context (its own solve workspace and
+  // projection basis) and the boundary operator (1/fts) R_L^T A^-1 S^T of
+  // its reverse pass (Nz x Nz, row-major), kept across gradients; the keys
+  // say which operator (ctx opGen) and adjoint solve options they belong to
+  sc_ctx* adjCtx = nullptr;
+  int adjCtxGen = -1;
+  double* adjB = nullptr;
+  int adjBGen = -1, adjBPrec = -1, adjBMode = -1;
+  double adjBRtol = 0.0;
 
   ~QdDriver();
   double* dalloc(size_t n, const double* init_h);

@@ -522,6 +522,7 @@ extern "C" int sc_update_var_coeff(sc_ctx* h, const double* mu_h) {
   SC_HIP(hipStreamSynchronize(c.stream));
   c.mu_h.assign(mu_h, mu_h + c.N);
   c.fd_stale = true;
+  ++c.opGen;
   c.projCount = c.projNext = 0;
   int e = sc::ctx_upload_mu(c);
   if (!e) e = sc::ctx_compute_diag(c);
@@ -543,6 +544,7 @@ extern "C" int sc_change_bc_types(sc_ctx* h, int bcR, int bcT, int bcL, int bcB)
   c.op64.bcL = c.op32.bcL = bcL;
   c.op64.bcB = c.op32.bcB = bcB;
   c.projCount = c.projNext = 0;
+  ++c.opGen;
   return sc::ctx_compute_diag(c);
 }
 

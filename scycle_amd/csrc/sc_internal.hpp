@@ -115,6 +115,7 @@ struct Ctx {
   double* zold64 = nullptr;            // previous z (flexible CG)
   struct FdDiag* fd = nullptr;         // fast-diagonalisation factors (lazy)
   bool fd_stale = false;
+  int opGen = 0;                       // bumped whenever mu or the BC types change
   double* partials = nullptr;          // reduction scratch
   // previous solutions kept for the initial guess (ring buffer of A-orthogonal
   // vectors y_i with weights w_i = 1 / (y_i^T (-A) y_i) on the device)

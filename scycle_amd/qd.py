@@ -9,6 +9,7 @@ from ._lib import (AdjDesc, AdjStats, GridDesc, QdCounters, QdDesc, QdOutput, Qd
                    dptr, load)
 
 SOLVE_MODES = {"cg64": 0, "ir32": 1}
+GREEN_MODES = {"auto": 0, "always": 1, "never": 2}
 
 
 class QdMediator:
@@ -177,14 +178,14 @@ class QdMediator:
         arrs = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (a, b, Dc)]
         check(load().sc_qd_set_friction(self.h, *[dptr(x) for x in arrs]))
 
-    def _adj_desc(self, dts, t0, stations, data, ckpt_every, solve, rtol, precond, reuse):
+    def _adj_desc(self, dts, t0, stations, data, ckpt_every, solve, rtol, precond, reuse, green="auto"):
         from .sbp_ops import PRECONDS
         dts = np.ascontiguousarray(dts, dtype=np.float64)
         js = np.ascontiguousarray(stations, dtype=np.int32)
         dd = None if data is None else np.ascontiguousarray(data, dtype=np.float64)
         so = SolveOpts(SOLVE_MODES[solve], rtol, 1e-5, 400000, 50, 64, PRECONDS[precond], reuse)
         d = AdjDesc(len(dts), dts.ctypes.data, float(t0), int(ckpt_every), len(js), js.ctypes.data,
-                    None if dd is None else dd.ctypes.data, so)
+                    None if dd is None else dd.ctypes.data, so, GREEN_MODES[green])
         return d, (dts, js, dd)
 
     def forward_misfit(self, dts, stations, data=None, t0=0.0, ckpt_every=50):
@@ -198,11 +199,13 @@ class QdMediator:
         return J.value, us[:len(keep[1])]
 
     def adjoint(self, dts, stations, data=None, t0=0.0, ckpt_every=50, solve="cg64",
-                rtol=1e-12, precond="fdiag64", reuse=4):
+                rtol=1e-12, precond="fdiag64", reuse=4, green="auto"):
         """dJ/da, dJ/db, dJ/dDc per fault node by the discrete adjoint; the
-        adjoint's elasticity solves use (solve, rtol, precond, reuse).  Returns
-        (ga, gb, gDc, stats)."""
-        d, keep = self._adj_desc(dts, t0, stations, data, ckpt_every, solve, rtol, precond, reuse)
+        adjoint's elasticity solves use (solve, rtol, precond, reuse).  green:
+        "always" applies the adjoint solves of the reverse pass through the
+        precomputed Nz x Nz boundary operator, "never" solves per stage,
+        "auto" chooses (sc_adj_desc.greenMode).  Returns (ga, gb, gDc, stats)."""
+        d, keep = self._adj_desc(dts, t0, stations, data, ckpt_every, solve, rtol, precond, reuse, green)
         g = [np.zeros(self.Nz) for _ in range(3)]
         st = AdjStats()
         check(load().sc_qd_adjoint(self.h, C.byref(d), *[dptr(x) for x in g], C.byref(st)))

@@ -143,3 +143,37 @@ def test_adjoint_torch_function():
     print("J", float(J), "after a descent step", float(J2), flush=True)
     assert float(J2) < float(J)
     m.close()
+
+
+def test_adjoint_boundary_operator_matches_per_stage_solves():
+    """The reverse pass through the precomputed boundary operator
+    B = (1/fts) R_L^T A^-1 S^T gives the gradient of the pass that solves
+    per stage (both to the solve tolerance, 1e-12 relative residual); B is
+    formed once (Nz solves), reused by the next gradient and after the
+    friction parameters change (A does not depend on them), and formed
+    again for other adjoint solve options."""
+    cfg, prob, m, dts, d = _problem(N=33, steps=24)
+    gs = m.adjoint(dts, prob.stations, d, ckpt_every=7, green="never")
+    gb1 = m.adjoint(dts, prob.stations, d, ckpt_every=7, green="always")
+    gb2 = m.adjoint(dts, prob.stations, d, ckpt_every=7, green="always")
+    print("per-stage solves", gs[3], "\nboundary operator", gb1[3], "\nreused", gb2[3], flush=True)
+    assert gs[3]["green_solves"] == 0 and gs[3]["adj_solves"] == 3 * 24 + 1
+    assert gb1[3]["green_solves"] == cfg.Nz and gb1[3]["adj_solves"] == 1
+    assert gb2[3]["green_solves"] == 0
+    for x, y in zip(gs[:3], gb1[:3]):
+        err = np.abs(x - y).max() / np.abs(x).max()
+        print("per-stage vs operator", err, flush=True)
+        assert err <= 1e-9
+    for x, y in zip(gb1[:3], gb2[:3]):
+        assert np.abs(x - y).max() <= 1e-12 * np.abs(x).max()
+    m.set_friction(prob.P @ (1.01 * prob.theta["a"]), None, None)
+    gb3 = m.adjoint(dts, prob.stations, d, ckpt_every=7, green="always")
+    gs3 = m.adjoint(dts, prob.stations, d, ckpt_every=7, green="never")
+    assert gb3[3]["green_solves"] == 0
+    for x, y in zip(gs3[:3], gb3[:3]):
+        assert np.abs(x - y).max() <= 1e-9 * np.abs(x).max()
+    gb4 = m.adjoint(dts, prob.stations, d, ckpt_every=7, green="always", precond="fdiag32", rtol=1e-11)
+    assert gb4[3]["green_solves"] == cfg.Nz
+    for x, y in zip(gb3[:3], gb4[:3]):
+        assert np.abs(x - y).max() <= 1e-8 * np.abs(x).max()
+    m.close()
