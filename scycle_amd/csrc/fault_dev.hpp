@@ -6,6 +6,63 @@
 
 namespace sc {
 
+// 2 atanh(s) for |s| <= 0.23 by its Taylor series 2 (s + s^3/3 + s^5/5 + ...)
+// through s^25 (the first omitted term is below 2^-60 of the sum there).
+__device__ __forceinline__ double two_atanh_small(double s) {
+  const double s2 = s * s;
+  double p = 1.0 / 25.0;
+  p = fma(p, s2, 1.0 / 23.0);
+  p = fma(p, s2, 1.0 / 21.0);
+  p = fma(p, s2, 1.0 / 19.0);
+  p = fma(p, s2, 1.0 / 17.0);
+  p = fma(p, s2, 1.0 / 15.0);
+  p = fma(p, s2, 1.0 / 13.0);
+  p = fma(p, s2, 1.0 / 11.0);
+  p = fma(p, s2, 1.0 / 9.0);
+  p = fma(p, s2, 1.0 / 7.0);
+  p = fma(p, s2, 1.0 / 5.0);
+  p = fma(p, s2, 1.0 / 3.0);
+  const double t = 2.0 * s;
+  return fma(t * s2, p, t);
+}
+
+// log x for finite x > 0: x = 2^k m with m in [sqrt(1/2), sqrt(2)), log m =
+// 2 atanh((m - 1) / (m + 1)) with |(m - 1) / (m + 1)| <= 0.172 (m - 1 is exact),
+// and k ln 2 from a two-part ln 2 (the high part has 32 significant bits, so
+// that k times it is exact).
+__device__ __forceinline__ double log_pos(double x) {
+  int k;
+  double m = frexp(x, &k);  // m in [0.5, 1)
+  const bool lo = m < 0.70710678118654752440;
+  m = lo ? 2.0 * m : m;
+  k = lo ? k - 1 : k;
+  const double r = two_atanh_small((m - 1.0) / (m + 1.0));
+  constexpr double kLn2Hi = 0x1.62e42ffp-1;             // ln 2 rounded to 32 bits
+  constexpr double kLn2Lo = -0x1.718432a1b0e26p-35;      // ln 2 - kLn2Hi
+  const double kd = (double)k;
+  return fma(kd, kLn2Hi, fma(kd, kLn2Lo, r));
+}
+
+// asinh z from asinh z = log(z + sqrt(z^2 + 1)): for |z| >= 1/2 with log_pos
+// (log(2|z|) once z^2 + 1 rounds to z^2, from 2^28 on), below 1/2 as log1p(t),
+// t = |z| + z^2 / (1 + sqrt(1 + z^2)), = 2 atanh(t / (2 + t)) with
+// t / (2 + t) <= 0.23, so that small arguments keep their relative accuracy.
+// Odd in z.  About a fifth of the instructions of the library's asinh, to a
+// few units in the last place.
+__device__ __forceinline__ double asinh_qd(double z) {
+  const double az = fabs(z);
+  double r;
+  if (az < 0.5) {
+    const double q = sqrt(fma(az, az, 1.0));
+    const double t = az + (az * az) / (1.0 + q);
+    r = two_atanh_small(t / (2.0 + t));
+  } else {
+    const double w = az > 268435456.0 ? 2.0 * az : az + sqrt(fma(az, az, 1.0));
+    r = log_pos(w);
+  }
+  return copysign(r, z);
+}
+
 // a*sN*asinh(V/(2 v0) * exp(psi/a))  (fault.cpp:2246-2250)
 __device__ __forceinline__ double strength_psi(double sN, double psi, double V, double a, double v0) {
   return a * sN * asinh((0.5 * V / v0) * exp(psi / a));
@@ -35,28 +92,31 @@ __device__ __forceinline__ double slip_law_psi(double psi, double V, double a, d
 // Quasi-dynamic residual F(V) = strength(V) - (tauQS - eta V)
 // (ComputeVel_qd::getResid, fault.cpp:1038-1059).  jac_mode 0 keeps the
 // reference's slope expression A*V/sqrt(B^2V^2+1)+eta; 1 uses A*B/sqrt(..)+eta.
+//
+// Evaluated with the per-solve constants A = a sN and B = exp(psi/a) / (2 v0)
+// formed once, z = B V, the library-free asinh_qd above and the slope's
+// 1/sqrt(z^2 + 1) as one reciprocal square root: the same function and slope
+// to a few units in the last place, in about a third of the instructions of
+// the direct form, which matters because the fault waves' Newton chains are
+// the critical path of a QD stage.
 struct QdResid {
   double psi, a, sN, eta, tauQS, v0;
   int jac_mode;
-  double ex;  // exp(psi / a), fixed over the root solve
-  // the strength at the last point evaluated: strength_psi(sN, psi, V, a, v0)
-  // with the same operations, so the caller need not evaluate it again
+  double A, B;  // a sN and exp(psi / a) / (2 v0), fixed over the root solve
+  // the strength at the last point evaluated, so the caller need not
+  // evaluate it again
   mutable double strength = 0.0;
   __device__ QdResid(double psi_, double a_, double sN_, double eta_, double tauQS_, double v0_, int jac)
-      : psi(psi_), a(a_), sN(sN_), eta(eta_), tauQS(tauQS_), v0(v0_), jac_mode(jac), ex(exp(psi_ / a_)) {}
-  __device__ double strength_at(double V) const {
-    const double z = (0.5 * V / v0) * ex;
-    // asinh(+-0) = +-0: the bracket's left end V = 0 needs no evaluation
-    return a * sN * (z == 0.0 ? z : asinh(z));
-  }
+      : psi(psi_), a(a_), sN(sN_), eta(eta_), tauQS(tauQS_), v0(v0_), jac_mode(jac), A(a_ * sN_),
+        B(exp(psi_ / a_) / (2. * v0_)) {}
+  __device__ double strength_at(double V) const { return A * asinh_qd(B * V); }
   __device__ void operator()(double V, double* f, double* J) const {
-    strength = strength_at(V);
+    const double z = B * V;
+    strength = A * asinh_qd(z);
     const double stress = tauQS - eta * V;
     *f = strength - stress;
-    const double A = a * sN;
-    const double B = ex / (2. * v0);
-    const double den = sqrt(B * B * V * V + 1.);
-    *J = (jac_mode == 0 ? A * V / den : A * B / den) + eta;
+    const double inv = rsqrt(fma(z, z, 1.0));
+    *J = (jac_mode == 0 ? A * V : A * B) * inv + eta;
   }
   // residual only, at a bracket end (leaves `strength` alone)
   __device__ void at_bracket_end(double V, double* f) const { *f = strength_at(V) - (tauQS - eta * V); }

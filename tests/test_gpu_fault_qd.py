@@ -67,6 +67,15 @@ def test_fault_kernel_parity(jac_mode, state_law):
     F = inp["a"] * inp["sN"] * np.arcsinh(g["V"] / 2e-6 * np.exp(inp["psi"] / inp["a"])) \
         - (inp["tauQS"] - inp["eta"] * g["V"])
     assert np.abs(F).max() < 1e-11
+    # the strength comes from the root solve's own asinh (log-based, no library
+    # call): against numpy's asinh over z from ~1e-4 to ~1e30 (both of its
+    # branches), to the rounding of z and of the exponential
+    z = g["V"] / 2e-6 * np.exp(inp["psi"] / inp["a"])
+    print("z range", z.min(), z.max(), "below 1/2:", int((z < 0.5).sum()), flush=True)
+    sref = inp["a"] * inp["sN"] * np.arcsinh(z)
+    err = np.abs(g["strength"] - sref) / np.abs(sref)
+    print("strength rel err max", err.max(), flush=True)
+    assert err.max() < 1e-14
     L = olib()
     dref = np.array([L.orc_aging(p, v, b, 0.6, 1e-6, dc) if state_law == 0 else
                      L.orc_sliplaw(p, v, a, b, 0.6, 1e-6, dc)
