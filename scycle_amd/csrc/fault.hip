@@ -61,9 +61,12 @@ __global__ __launch_bounds__(256) void k_fault_fd(sc_fault_fd_args a, const TD* 
     } else {
       const double P = Phi < 0 ? -Phi : Phi;
       const double A = aa * sN, B = exp(psiC / aa) / (2. * v0);
+      // the strength A asinh(B V) with the library-free asinh_qd and the
+      // slope's 1/sqrt as one reciprocal square root (as the QD residual)
       auto resid = [&](double vel, double* f, double* J) {
-        *f = fricPen * strength_psi(sN, psiC, vel, aa, v0) - (P - vel);
-        *J = 1 + fricPen * A * B / sqrt(1. + B * B * vel * vel);
+        const double z = B * vel;
+        *f = fricPen * (A * asinh_qd(z)) - (P - vel);
+        *J = 1 + fricPen * A * B * rsqrt(fma(z, z, 1.0));
       };
       int its;
       if (bracketed_newton(resid, left, right, fabs(a.V_d[k]), a.rootTol, a.maxNumIts, &V, &its))
@@ -72,7 +75,7 @@ __global__ __launch_bounds__(256) void k_fault_fd(sc_fault_fd_args a, const TD* 
   }
   // fault displacement with the frictional damping of the slip velocity
   if (!(V < 1e-14)) {
-    const double fric = strength_psi(sN, psiC, V, aa, v0);
+    const double fric = aa * sN * asinh_qd(exp(psiC / aa) / (2. * v0) * V);
     const double alpha = 1.0 / (rho * alphay) * fric / V;
     const double Aden = 1.0 + alpha * dt;
     V = Phi / (1. + dt * alpha);
@@ -81,17 +84,25 @@ __global__ __launch_bounds__(256) void k_fault_fd(sc_fault_fd_args a, const TD* 
   // state: psiNext - psiPrev = 2 dt G((psiPrev + psiNext)/2, V) on [-2, 2]
   double psiN;
   {
+    // one exponential per evaluation, shared by the residual and its slope;
+    // the slip law's steady-state friction and |V| terms fixed over the solve
+    const double absV = fabs(V);
+    const double fss = absV > 0.0 ? a.f0 + (aa - bb) * log(absV / v0) : 0.0;
+    const double xV = absV / 2. / v0, gV = absV / Dc;
     auto resid = [&](double state, double* f, double* J) {
       const double mid = (psiP + state) / 2.0;
       if (a.stateLaw == SC_AGING_LAW) {
-        const double G = aging_law_psi(mid, V, bb, a.f0, v0, Dc);
+        // aging_law_psi: zero where b <= 1e-3 or the exponential overflows
+        const double E = exp((a.f0 - mid) / bb);
+        const double G = (!isinf(E) && bb > 1e-3) ? (bb * v0 / Dc) * (E - V / v0) : 0.0;
         *f = -2 * dt * G + (state - psiP);
-        *J = 1 + dt * v0 / Dc * exp((a.f0 - mid) / bb);
+        *J = 1 + dt * v0 / Dc * E;
       } else {
-        const double G = slip_law_psi(mid, V, aa, bb, a.f0, v0, Dc);
+        // slip_law_psi: zero at V = 0
+        const double x = xV * exp(mid / aa);
+        const double G = absV > 0.0 ? -gV * (aa * asinh_qd(x) - fss) : 0.0;
         *f = -2 * dt * G + (state - psiP);
-        const double A = fabs(V) / 2. / v0 * exp((psiP + state) / 2.0 / aa);
-        *J = 1 + dt * fabs(V) / Dc * A / sqrt(1 + A * A);
+        *J = 1 + dt * gV * x * rsqrt(fma(x, x, 1.0));
       }
     };
     int its;
