@@ -206,20 +206,41 @@ struct FdDriver {
     return f;
   }
 
+  // dt^2 / rho per node for the wave update, formed again when dt changes
+  void* c1 = nullptr;
+  double c1dt = -1.0;
+  int ensure_c1(double dt) {
+    if (c1 && c1dt == dt) return SC_OK;
+    hipStream_t s = ctx->c.stream;
+    if (!c1) {
+      c1 = dalloc((size_t)N * (f32 ? sizeof(float) : sizeof(double)));
+      if (!c1) return fail(SC_ERR_HIP, "hipMalloc failed");
+    }
+    int e = f32 ? launch_wave_c1((const float*)rho, (float*)c1, (size_t)N, (float)dt, s)
+                : launch_wave_c1((const double*)rho, (double*)c1, (size_t)N, dt, s);
+    if (e) return e;
+    c1dt = dt;
+    return SC_OK;
+  }
+
   int steps(double t0, double dt, int n) {
     (void)t0;  // no time-dependent forcing on this path (updatePrestress needs initialConditions "tau")
     const Ctx& c = ctx->c;
     hipStream_t s = c.stream;
+    if (n > 0) {
+      int e = ensure_c1(dt);
+      if (e) return e;
+    }
     for (int it = 0; it < n; ++it) {
       int e;
       if (f32) {
-        WaveArgs<float> w{(const float*)u[cur], (const float*)u[prev], (float*)u[next], (const float*)rho,
-                          (const float*)ay, (float*)d2u, (float)dt};
+        WaveArgs<float> w{(const float*)u[cur], (const float*)u[prev], (float*)u[next], (const float*)c1,
+                          (const float*)ay, (float*)d2u, (float)dt, 1};
         e = launch_wave32(c, w, s);
         if (!e) e = fault_fd_step32(s, fault_args(dt), (const float*)d2u, (float*)u[next]);
       } else {
         WaveArgs<double> w{(const double*)u[cur], (const double*)u[prev], (double*)u[next],
-                           (const double*)rho, (const double*)ay, (double*)d2u, dt};
+                           (const double*)c1, (const double*)ay, (double*)d2u, dt, 1};
         e = launch_wave64(c, w, s);
         if (!e) e = fault_fd_step64(s, fault_args(dt), (double*)u[next]);
       }
