@@ -206,7 +206,8 @@ struct FdDriver {
     return f;
   }
 
-  // dt^2 / rho per node for the wave update, formed again when dt changes
+  // the wave update's per-node scale (dt^2/rho and the grid's H^-1 and
+  // metric factors), formed again when dt changes
   void* c1 = nullptr;
   double c1dt = -1.0;
   int ensure_c1(double dt) {
@@ -216,8 +217,8 @@ struct FdDriver {
       c1 = dalloc((size_t)N * (f32 ? sizeof(float) : sizeof(double)));
       if (!c1) return fail(SC_ERR_HIP, "hipMalloc failed");
     }
-    int e = f32 ? launch_wave_c1((const float*)rho, (float*)c1, (size_t)N, (float)dt, s)
-                : launch_wave_c1((const double*)rho, (double*)c1, (size_t)N, dt, s);
+    int e = f32 ? launch_wave_c1(ctx->c, (const float*)rho, (float*)c1, (float)dt, s)
+                : launch_wave_c1(ctx->c, (const double*)rho, (double*)c1, dt, s);
     if (e) return e;
     c1dt = dt;
     return SC_OK;

@@ -205,44 +205,103 @@ __device__ __forceinline__ T eval_node(const DevOp<T>& p, const U& u, int j, int
   return v + sat_terms(p, u, j, k);
 }
 
+// Raw buffer access (stride 0, the range in bytes) for the tile kernel: a
+// lane's column is the vector offset, a row's start the scalar one.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  if constexpr (sizeof(T) == 8) return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  else return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <typename T>
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, T x, int voff, int soff) {
+  if constexpr (sizeof(T) == 8) {
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, x), r, voff, soff, 0);
+  } else {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, voff, soff, 0);
+  }
+}
+
 // ------------------------------------------------------------ epilogues --
 // What the apply kernels do with (A u) at node (j, k), index I: `uc` is u at
-// the node, `edge` tells whether the node may lie on the domain boundary
-// (frame kernels) or certainly does not (interior kernel).
+// the node.  operator() serves the frame nodes (any node, the boundary's
+// terms included); the interior tile kernel instead fetches each output
+// row's operands (pre) two rows before it stores that row (interior), all
+// through buffer resources made once per wave (rows), so that no row's store
+// waits for a load issued just before it.
 template <typename T>
 struct StoreAu {  // out = A u
   T* __restrict__ out;
   __device__ __forceinline__ void operator()(int, int, size_t I, T Au, T, bool) const { out[I] = Au; }
+  struct Rows {
+    __amdgpu_buffer_rsrc_t o;
+  };
+  struct Pre {};
+  __device__ __forceinline__ Rows rows(int bytes) const { return {buf_rsrc(out, bytes)}; }
+  __device__ __forceinline__ Pre pre(const Rows&, int, int) const { return {}; }
+  __device__ __forceinline__ void interior(const Rows& r, int, int, int voff, int soff, T Au, T,
+                                           const Pre&) const {
+    buf_st<T>(r.o, Au, voff, soff);
+  }
 };
 
 // propagateWaves (strikeSlip_linearElastic_fd.cpp:445-516):
 //   D2u = Jinv * Hinv * (A u),  Jinv = qy*rz, Hinv = Hy^-1 (x) Hz^-1
 //   uNext = (dt^2/rho * D2u + 2u + (dt*ay - 1)*uPrev) / (dt*ay + 1)
 // a_y is zero off the domain boundary, where the update reduces exactly to
-// dt^2/rho*D2u + 2u - uPrev.  D2u on the fault (j = 0) is kept for the
-// fault update.
+// dt^2/rho*D2u + 2u - uPrev (every interior tile node).  D2u on the fault
+// (j = 0) is kept for the fault update.  C1: `rho` holds the whole per-node
+// scale s = dt^2/rho * Hy^-1 Hz^-1 (* qy rz on a variable grid), formed once
+// for a run at constant dt (k_wave_c1), so the update is s (A u) + 2u - uPrev.
 template <typename T, bool C1>
 struct WaveUpdate {
   T* __restrict__ uNext;
   const T* __restrict__ uPrev;
-  const T* __restrict__ rho;  // C1: dt^2 / rho, formed once for the run
+  const T* __restrict__ rho;  // C1: the per-node scale s
   const T* __restrict__ ay;
   const T *hinvY, *hinvZ, *qy, *rz;
   T* __restrict__ d2uFault;
   T dt;
   int varGrid;
-  __device__ __forceinline__ void operator()(int j, int k, size_t I, T Au, T uc, bool edge) const {
+  __device__ __forceinline__ T d2u_of(int j, int k, T Au) const {
     T d2u = (hinvY[j] * hinvZ[k]) * Au;
     if (varGrid) d2u = (qy[j] * rz[k]) * d2u;
-    if (j == 0) d2uFault[k] = d2u;
-    const T c1 = C1 ? rho[I] : dt * dt / rho[I];
+    return d2u;
+  }
+  // the update's (dt^2/rho) D2u term at node I, given rho or s there
+  __device__ __forceinline__ T scaled(int j, int k, T r, T Au) const {
+    if constexpr (C1) return r * Au;
+    else return (dt * dt / r) * d2u_of(j, k, Au);
+  }
+  __device__ __forceinline__ void operator()(int j, int k, size_t I, T Au, T uc, bool edge) const {
+    if (j == 0) d2uFault[k] = d2u_of(j, k, Au);
+    const T sAu = scaled(j, k, rho[I], Au);
     if (edge) {
       const T a = ay[I];
       const T c2 = dt * a - T(1), c3 = dt * a + T(1);
-      uNext[I] = (c1 * d2u + T(2) * uc + c2 * uPrev[I]) / c3;
+      uNext[I] = (sAu + T(2) * uc + c2 * uPrev[I]) / c3;
     } else {
-      uNext[I] = c1 * d2u + T(2) * uc + T(-1) * uPrev[I];
+      uNext[I] = sAu + T(2) * uc + T(-1) * uPrev[I];
     }
+  }
+  struct Rows {
+    __amdgpu_buffer_rsrc_t n, p, r;
+  };
+  struct Pre {
+    T r, up;
+  };
+  __device__ __forceinline__ Rows rows(int bytes) const {
+    return {buf_rsrc(uNext, bytes), buf_rsrc(uPrev, bytes), buf_rsrc(rho, bytes)};
+  }
+  __device__ __forceinline__ Pre pre(const Rows& b, int voff, int soff) const {
+    return {buf_ld<T>(b.r, voff, soff), buf_ld<T>(b.p, voff, soff)};
+  }
+  __device__ __forceinline__ void interior(const Rows& b, int j, int k, int voff, int soff, T Au, T uc,
+                                           const Pre& q) const {
+    buf_st<T>(b.n, scaled(j, k, q.r, Au) + T(2) * uc + T(-1) * q.up, voff, soff);
   }
 };
 
@@ -291,16 +350,6 @@ template <typename T>
 __device__ __forceinline__ T lane_up1(T x) { return dpp_shift<kDppWaveShr1>(x); }
 template <typename T>
 __device__ __forceinline__ T lane_down1(T x) { return dpp_shift<kDppWaveShl1>(x); }
-
-// Raw buffer access (stride 0, the range in bytes) for the tile kernel.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, int bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, bytes, 0x00020000);
-}
-template <typename T>
-__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  if constexpr (sizeof(T) == 8) return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-  else return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-}
 
 // ------------------------------------------------------ interior kernel ---
 // One wave per block.  Lanes cover columns [k0-4, k0+60); lanes 4..59 write
@@ -367,7 +416,14 @@ __device__ __forceinline__ void frame_cf_quad(const DevOp<T>& p, const FrameMap&
   }
 }
 
+// output rows of an interior tile, and how many output rows ahead of the
+// one it computes a tile's wave issues its loads: u and mu (In; kJC: every
+// row at the start, measured faster than streaming them in) and the
+// epilogue's operands (Out; two rows, so that no row's store waits on a load
+// issued just before it)
 constexpr int kJC = 8;
+constexpr int kAheadIn = kJC;
+constexpr int kAheadOut = 2;
 constexpr int kLanesOut = 56;
 constexpr int kXcd = 8;
 
@@ -384,7 +440,7 @@ __device__ __forceinline__ int tile_of_block(int b, int total, int xcd) {
 // one launch covers the whole grid, and the frame's latency-bound loads
 // overlap the interior's streaming.
 template <typename T, int ORDER, class EP, int XCD = 1>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_apply_interior(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_apply_interior(
                                                        DevOp<T> p, const T* __restrict__ u, EP ep,
                                                        int nbx, int nby, FrameMap fm,
                                                        const T* __restrict__ fcoef, int nfb) {
@@ -408,21 +464,55 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   const int rowB = Nz * (int)sizeof(T);
   const int nB = Ny * rowB;
   const __amdgpu_buffer_rsrc_t ru = buf_rsrc(u, nB), rm = buf_rsrc(p.mu, nB);
-  T uw[JC + 8];
-#pragma unroll
-  for (int r = 0; r < JC + 8; ++r) uw[r] = buf_ld<T>(ru, voff, min(j0 - 4 + r, Ny - 1) * rowB);
-  T mw[JC + 4];
-#pragma unroll
-  for (int r = 0; r < JC + 4; ++r) mw[r] = buf_ld<T>(rm, voff, min(j0 - 2 + r, Ny - 1) * rowB);
   const DevDir<T>& dy = p.y;
   const DevDir<T>& dz = p.z;
+  // the lane's column weights.  The laplace type's direction switches are
+  // zero factors: no load sits in a branch (a branch's loads are waited for
+  // inside it).
+  const T selY = p.doY ? T(1) : T(0), selZ = p.doZ ? T(1) : T(0);
+  const T hzk = dz.H[kc], mzk = dz.met[kc];
+  const T rzk = dz.imet[kc];
+  const T rzk1 = dz.imet[kc + 1 < Nz ? kc + 1 : Nz - 1];
+  // The tile's rows: u rows j0-4 .. j0+JC+3 (uw), mu and the y-metric at the
+  // flux rows j0-2 .. j0+JC+1 (mw, qmr), the output rows' z-weights (myr) and
+  // the epilogue's operands (pq), each loaded kAheadIn or kAheadOut output
+  // rows before the row that first reads it.
+  // (the per-row factors qmr, myr are uniform: scalar loads where no
+  // store of the tile precedes them, vector loads after)
+  T uw[JC + 8], mw[JC + 4], qmr[JC + 4], myr[JC];
+  auto ld_u = [&](int r) {
+    if (r < JC + 8) uw[r] = buf_ld<T>(ru, voff, min(j0 - 4 + r, Ny - 1) * rowB);
+  };
+  auto ld_m = [&](int r) {
+    if (r < JC + 4) {
+      const int jm = min(j0 - 2 + r, Ny - 1);
+      mw[r] = buf_ld<T>(rm, voff, jm * rowB);
+      qmr[r] = dy.imet[jm];
+    }
+  };
+  const typename EP::Rows er = ep.rows(nB);
+  typename EP::Pre pq[JC];
+  auto ld_out = [&](int ro) {
+    if (ro < JC) {
+      const int jc = min(j0 + ro, Ny - 1);
+      myr[ro] = dy.met[jc];  // (H is the interior weight Hc on these rows)
+      pq[ro] = ep.pre(er, voff, jc * rowB);
+    }
+  };
+  // what the flux prologue and output row 0 read, and the rows ahead
+#pragma unroll
+  for (int r = 0; r < 8 + kAheadIn; ++r) ld_u(r);
+#pragma unroll
+  for (int r = 0; r < 4 + kAheadIn; ++r) ld_m(r);
+#pragma unroll
+  for (int ro = 0; ro < kAheadOut; ++ro) ld_out(ro);
+  // (the loads stay where they are written: the scheduler would otherwise
+  // sink them towards their uses and wait for each group in turn)
+  __builtin_amdgcn_sched_barrier(0);
+  const T preY = selY * (hzk * mzk);
   // ---- y-direction fluxes at rows j0-2 .. j0+JC+1 (factored interior rows:
   // D1 u = ia (u[+1]-u[-1]) + ib (u[+2]-u[-2]), DR = [-1 3 -3 1] or the
   // three-point D2, D4 = [1 -4 6 -4 1])
-  // the flux rows' metric (loaded with the rest, before any store)
-  T qmr[JC + 4];
-#pragma unroll
-  for (int r = 0; r < JC + 4; ++r) qmr[r] = dy.imet[min(j0 - 2 + r, Ny - 1)];
   T g1[JC + 4], g3[JC + 4], g4[JC + 4];
   // flux row r (grid row j0-2+r); each is formed just before the first
   // output row that needs it, so that only five rows of fluxes are live
@@ -445,19 +535,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
       g4[r] = T(0);
     }
   };
-  // the laplace type's direction switches as zero weights (no branches in
-  // the row loop)
-  const T preY = p.doY ? dz.H[kc] * dz.met[kc] : T(0);
-  const T rzk = dz.imet[kc];
-  const T rzk1 = dz.imet[kc + 1 < Nz ? kc + 1 : Nz - 1];
-  // the output rows' z-weights, loaded here with the rest (a load inside the
-  // row loop would follow its stores and wait for every load in flight)
-  T hzr[JC];
-#pragma unroll
-  for (int ro = 0; ro < JC; ++ro) {
-    const int jc = min(j0 + ro, Ny - 1);
-    hzr[ro] = p.doZ ? dy.H[jc] * dy.met[jc] : T(0);
-  }
   const bool writer = lane >= 4 && lane < 4 + kLanesOut && k < Nz - kFrame;
 #pragma unroll
   for (int r = 0; r < 4; ++r) flux(r);
@@ -465,6 +542,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   for (int ro = 0; ro < JC; ++ro) {
     const int j = j0 + ro;
     const int rr = ro + 2;
+    ld_u(ro + 8 + kAheadIn);
+    ld_m(ro + 4 + kAheadIn);
+    ld_out(ro + kAheadOut);
     flux(ro + 4);
     T v;
     {
@@ -508,9 +588,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const T w3 = dz.Ha * g1z - hr;
         Lz = T(2) * hr + lane_up1(w1) + lane_down1(w3);
       }
-      v += hzr[ro] * Lz;
+      v += (selZ * (dy.Hc * myr[ro])) * Lz;
     }
-    if (writer && j < Ny - kFrame) ep(j, k, (size_t)j * Nz + k, v, uw[ro + 4], false);
+    // (v formed by every lane: a value only the store reads would be sunk
+    // into the store's branch, its loads with it)
+    asm volatile("" : "+v"(v));
+    if (writer && j < Ny - kFrame) ep.interior(er, j, k, voff, j * rowB, v, uw[ro + 4], pq[ro]);
   }
 }
 
@@ -803,23 +886,32 @@ static int launch_wave_t(const Ctx& c, const DevOp<T>& op, const WaveArgs<T>& a,
   return launch_apply_t(c, op, a.u, ep, s);
 }
 
+// the wave update's per-node scale s = dt^2/rho * Hy^-1 Hz^-1 (* qy rz on a
+// variable grid): the same factors WaveUpdate forms from rho, multiplied out once
 template <typename T>
-__global__ __launch_bounds__(256) void k_wave_c1(const T* __restrict__ rho, T* __restrict__ c1, size_t n, T dt) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    c1[i] = dt * dt / rho[i];
+__global__ __launch_bounds__(256) void k_wave_c1(DevOp<T> p, const T* __restrict__ rho, T* __restrict__ c1,
+                                                 T dt, int varGrid) {
+  const size_t n = (size_t)p.Ny * p.Nz;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(i / (size_t)p.Nz), k = (int)(i - (size_t)j * p.Nz);
+    T h = p.y.Hinv[j] * p.z.Hinv[k];
+    if (varGrid) h = h * (p.y.imet[j] * p.z.imet[k]);
+    c1[i] = (dt * dt / rho[i]) * h;
+  }
 }
 template <typename T>
-static int launch_wave_c1_t(const T* rho, T* c1, size_t n, T dt, hipStream_t s) {
-  const size_t b = (n + 255) / 256;
-  hipLaunchKernelGGL(k_wave_c1<T>, dim3((unsigned)(b < 4096 ? b : 4096)), dim3(256), 0, s, rho, c1, n, dt);
+static int launch_wave_c1_t(const Ctx& c, const DevOp<T>& op, const T* rho, T* c1, T dt, hipStream_t s) {
+  const size_t b = ((size_t)c.N + 255) / 256;
+  hipLaunchKernelGGL(k_wave_c1<T>, dim3((unsigned)(b < 4096 ? b : 4096)), dim3(256), 0, s, op, rho, c1, dt,
+                     c.d.var_grid);
   SC_HIP(hipGetLastError());
   return SC_OK;
 }
-int launch_wave_c1(const double* rho, double* c1, size_t n, double dt, hipStream_t s) {
-  return launch_wave_c1_t(rho, c1, n, dt, s);
+int launch_wave_c1(const Ctx& c, const double* rho, double* c1, double dt, hipStream_t s) {
+  return launch_wave_c1_t(c, c.op64, rho, c1, dt, s);
 }
-int launch_wave_c1(const float* rho, float* c1, size_t n, float dt, hipStream_t s) {
-  return launch_wave_c1_t(rho, c1, n, dt, s);
+int launch_wave_c1(const Ctx& c, const float* rho, float* c1, float dt, hipStream_t s) {
+  return launch_wave_c1_t(c, c.op32, rho, c1, dt, s);
 }
 int launch_wave64(const Ctx& c, const WaveArgs<double>& a, hipStream_t s) {
   return launch_wave_t(c, c.op64, a, s);
@@ -1171,7 +1263,7 @@ extern "C" int sc_bench_apply(sc_ctx* h, int which, int reps, double* ms_per_lau
       case 5: sc::bench_interior<float, 0>(c, c.op32, a32, (float*)b); return 0;
       case 6: sc::bench_frame<double>(c, c.op64, a64, (double*)b); return 0;
       case 7: sc::bench_frame<float>(c, c.op32, a32, (float*)b); return 0;
-      case 9:  // the FD driver's form: r holds dt^2 / rho
+      case 9:  // the FD driver's form: r holds the per-node scale
         return sc::launch_wave64(c, {a64, (const double*)b, (double*)w, (const double*)r, (const double*)y,
                                      (double*)d2, 1e-3, 1}, c.stream);
       default:

@@ -146,16 +146,16 @@ struct WaveArgs {
   const T* u;
   const T* uPrev;
   T* uNext;
-  const T* rho;  // rho, or dt^2 / rho per node when rhoIsC1
+  const T* rho;  // rho, or the per-node scale of launch_wave_c1 when rhoIsC1
   const T* ay;
   T* d2uFault;
   T dt;
   int rhoIsC1 = 0;
 };
-// c1 = dt^2 / rho per node (the wave update's coefficient, formed once for a
-// run at constant dt)
-int launch_wave_c1(const double* rho, double* c1, size_t n, double dt, hipStream_t s);
-int launch_wave_c1(const float* rho, float* c1, size_t n, float dt, hipStream_t s);
+// c1 = dt^2/rho * Hy^-1 Hz^-1 (* qy rz on a variable grid) per node: the
+// wave update's whole scale of A u, formed once for a run at constant dt
+int launch_wave_c1(const Ctx& c, const double* rho, double* c1, double dt, hipStream_t s);
+int launch_wave_c1(const Ctx& c, const float* rho, float* c1, float dt, hipStream_t s);
 int launch_wave64(const Ctx& c, const WaveArgs<double>& a, hipStream_t s);
 int launch_wave32(const Ctx& c, const WaveArgs<float>& a, hipStream_t s);
 // frame_only: write only the nodes within kFrame of a boundary, for a
