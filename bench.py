@@ -348,11 +348,14 @@ def c3_leg(n, steps):
 
 
 def fd_c4(n, steps):
-    """Fully dynamic 2001^2 rupture: constant-dt wave + fault steps, FP64 and
-    FP32 bulk; field errors of the FP32-bulk run against the FP64 run; the
-    wave step's roofline (40 algorithmic bytes per node: read u, u_prev, mu,
-    rho, write u_next; ay and the frame coefficients are boundary-only) timed
-    with HIP events on the context stream."""
+    """Fully dynamic 2001^2 rupture: constant-dt steps, FP64 and FP32 bulk;
+    a step is one launch, the wave update with the fault's step fused in
+    (the fault's root solves beside the interior tiles).  Field errors of the
+    FP32-bulk run against the FP64 run; the wave launch's roofline (40
+    algorithmic bytes per node: read u, u_prev, mu, the per-node scale,
+    write u_next; ay and the frame coefficients are boundary-only) timed with
+    HIP events on the context stream without the fault's part, and the same
+    bytes over the timed step (`step_frac`)."""
     from scycle_amd import FdMediator
     from scycle_amd._lib import check, load
     from scycle_amd.configs import config_c4
@@ -383,6 +386,7 @@ def fd_c4(n, steps):
                 rl[key] = dict(ms_per_launch=ms.value, bytes=bpn * nodes.value,
                                GBps=bpn * nodes.value / (ms.value * 1e-3) / 1e9,
                                frac=bpn * nodes.value / (ms.value * 1e-3) / HBM_PEAK)
+            rl["step_frac_fp64"] = 40.0 * N / (out["fp64"]["ms_per_step"] * 1e-3) / HBM_PEAK
             out["roofline_wave"] = rl
         m.close()
     out["fp32_bulk_vs_fp64_rel_err"] = {k: _rel(states["fp32"][k], states["fp64"][k])

@@ -26,111 +26,23 @@ int fault_qd_rates(hipStream_t s, const sc_fault_qd_args& a) {
 }
 
 // ------------------------------------------------------------ FD fault --
-// Fault_fd::d_dt for one node (fault.cpp:1436-1523, setPhi :1527-1575,
-// ComputeVel_fd :1588-1685, ComputeAging_fd / ComputeSlipLaw_fd :1697-1879).
-// uBody (may be null) receives the new fault displacement in the body array's
-// type: the fault column of uNext (strikeSlip_linearElastic_fd.cpp:421-423).
-template <typename TB, typename TD>
-__global__ __launch_bounds__(256) void k_fault_fd(sc_fault_fd_args a, const TD* __restrict__ d2u,
-                                                  TB* __restrict__ uBody) {
+// Fault_fd::d_dt, one thread per node (the node's update is fault_fd_node;
+// the FD driver runs it inside its wave-update launch instead).  uBody (may
+// be null) receives the new fault displacement: the fault column of uNext
+// (strikeSlip_linearElastic_fd.cpp:421-423).
+__global__ __launch_bounds__(256) void k_fault_fd(sc_fault_fd_args a, const double* __restrict__ d2u,
+                                                  double* __restrict__ uBody) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.N) return;
-  const double dt = a.dt, fts = a.faultTypeScale, v0 = a.v0;
-  const double psiC = a.psi_d[k], psiP = a.psiPrev_d[k];
-  const double aa = a.a_d[k], bb = a.b_d[k], Dc = a.Dc_d[k], sN = a.sNEff_d[k];
-  const double rho = a.rho_d[k], alphay = a.alphay_d[k], slip0 = a.slip0_d[k];
-  int bad = 0;
-  // u, uPrev from slip: (slip - slip0) / faultTypeScale
-  const double uPrev = (-1.0 * slip0 + a.slipPrev_d[k]) * (1.0 / fts);
-  double u = (-1.0 * slip0 + a.slip_d[k]) * (1.0 / fts);
-  // setPhi
-  const double an = (double)d2u[k] + a.tau0_d[k] / alphay;
-  const double Phi = 2.0 / dt * (u - uPrev) + dt * an / rho;
-  const double fricPen = dt / alphay / rho;
-  // slip velocity: fricPen*strength(V) = |Phi| - V on [0, |Phi|]
-  const double lk = a.locked_d ? a.locked_d[k] : 0.0;
-  double V;
-  if (lk > 0.5) {
-    V = 0.;
-  } else if (lk < -0.5) {
-    V = a.vL;
-  } else {
-    const double left = 0., right = fabs(Phi);
-    if (fabs(left - right) < 1e-14) {
-      V = left;
-    } else {
-      const double P = Phi < 0 ? -Phi : Phi;
-      const double A = aa * sN, B = exp(psiC / aa) / (2. * v0);
-      // the strength A asinh(B V) with the library-free asinh_qd and the
-      // slope's 1/sqrt as one reciprocal square root (as the QD residual)
-      auto resid = [&](double vel, double* f, double* J) {
-        const double z = B * vel;
-        *f = fricPen * (A * asinh_qd(z)) - (P - vel);
-        *J = 1 + fricPen * A * B * rsqrt(fma(z, z, 1.0));
-      };
-      int its;
-      if (bracketed_newton(resid, left, right, fabs(a.V_d[k]), a.rootTol, a.maxNumIts, &V, &its))
-        bad = SC_ERR_NOCONV;
-    }
-  }
-  // fault displacement with the frictional damping of the slip velocity
-  if (!(V < 1e-14)) {
-    const double fric = aa * sN * asinh_qd(exp(psiC / aa) / (2. * v0) * V);
-    const double alpha = 1.0 / (rho * alphay) * fric / V;
-    const double Aden = 1.0 + alpha * dt;
-    V = Phi / (1. + dt * alpha);
-    u = (2. * u + (an * dt * dt / rho) + (dt * alpha - 1.) * uPrev) / Aden;
-  }
-  // state: psiNext - psiPrev = 2 dt G((psiPrev + psiNext)/2, V) on [-2, 2]
-  double psiN;
-  {
-    // one exponential per evaluation, shared by the residual and its slope;
-    // the slip law's steady-state friction and |V| terms fixed over the solve
-    const double absV = fabs(V);
-    const double fss = absV > 0.0 ? a.f0 + (aa - bb) * log(absV / v0) : 0.0;
-    const double xV = absV / 2. / v0, gV = absV / Dc;
-    auto resid = [&](double state, double* f, double* J) {
-      const double mid = (psiP + state) / 2.0;
-      if (a.stateLaw == SC_AGING_LAW) {
-        // aging_law_psi: zero where b <= 1e-3 or the exponential overflows
-        const double E = exp((a.f0 - mid) / bb);
-        const double G = (!isinf(E) && bb > 1e-3) ? (bb * v0 / Dc) * (E - V / v0) : 0.0;
-        *f = -2 * dt * G + (state - psiP);
-        *J = 1 + dt * v0 / Dc * E;
-      } else {
-        // slip_law_psi: zero at V = 0
-        const double x = xV * exp(mid / aa);
-        const double G = absV > 0.0 ? -gV * (aa * asinh_qd(x) - fss) : 0.0;
-        *f = -2 * dt * G + (state - psiP);
-        *J = 1 + dt * gV * x * rsqrt(fma(x, x, 1.0));
-      }
-    };
-    int its;
-    if (bracketed_newton(resid, -2., 2., psiC, a.rootTol, a.maxNumIts, &psiN, &its)) bad = SC_ERR_NOCONV;
-  }
-  const double slipN = fts * u + slip0;
-  a.V_d[k] = V;
-  a.psiNext_d[k] = psiN;
-  a.slipNext_d[k] = slipN;
-  if (a.strength_d) a.strength_d[k] = strength_psi(sN, psiN, V, aa, v0);
-  if (a.uFault_d) a.uFault_d[k] = u;
-  if (uBody) uBody[k] = (TB)u;
-  if (!isfinite(V) || !isfinite(psiN) || !isfinite(u)) bad = SC_ERR_NONFINITE;
-  if (bad && a.status_d) atomicOr(a.status_d, bad);
+  const double u = fault_fd_node(a, k, d2u[k]);
+  if (uBody) uBody[k] = u;
 }
 
-template <typename TB, typename TD>
-int fault_fd_step_t(hipStream_t s, const sc_fault_fd_args& a, const TD* d2u, TB* uBody) {
+int fault_fd_step64(hipStream_t s, const sc_fault_fd_args& a, double* uBody) {
   if (a.N <= 0) return SC_OK;
-  hipLaunchKernelGGL((k_fault_fd<TB, TD>), dim3((a.N + 255) / 256), dim3(256), 0, s, a, d2u, uBody);
+  hipLaunchKernelGGL(k_fault_fd, dim3((a.N + 255) / 256), dim3(256), 0, s, a, a.d2u_d, uBody);
   SC_HIP(hipGetLastError());
   return SC_OK;
-}
-int fault_fd_step64(hipStream_t s, const sc_fault_fd_args& a, double* uBody) {
-  return fault_fd_step_t<double, double>(s, a, a.d2u_d, uBody);
-}
-int fault_fd_step32(hipStream_t s, const sc_fault_fd_args& a, const float* d2u32, float* uBody) {
-  return fault_fd_step_t<float, float>(s, a, d2u32, uBody);
 }
 
 // ------------------------------------------------------ QD driver helpers

@@ -232,18 +232,18 @@ struct FdDriver {
       int e = ensure_c1(dt);
       if (e) return e;
     }
+    // one launch a step: the wave update with the fault's step fused in
     for (int it = 0; it < n; ++it) {
       int e;
+      const sc_fault_fd_args fa = fault_args(dt);
       if (f32) {
         WaveArgs<float> w{(const float*)u[cur], (const float*)u[prev], (float*)u[next], (const float*)c1,
-                          (const float*)ay, (float*)d2u, (float)dt, 1};
+                          (const float*)ay, (float*)d2u, (float)dt, 1, &fa};
         e = launch_wave32(c, w, s);
-        if (!e) e = fault_fd_step32(s, fault_args(dt), (const float*)d2u, (float*)u[next]);
       } else {
         WaveArgs<double> w{(const double*)u[cur], (const double*)u[prev], (double*)u[next],
-                           (const double*)c1, (const double*)ay, (double*)d2u, dt, 1};
+                           (const double*)c1, (const double*)ay, (double*)d2u, dt, 1, &fa};
         e = launch_wave64(c, w, s);
-        if (!e) e = fault_fd_step64(s, fault_args(dt), (double*)u[next]);
       }
       if (e) return e;
       const int p = prev;

@@ -17,6 +17,7 @@
 // node: the boundary closures, prefactors and SAT terms folded in).  Grids
 // with a side of at most 2*kFrame nodes use a per-node generic kernel
 // instead.
+#include "fault_dev.hpp"
 #include "sbp_dev.hpp"
 
 namespace sc {
@@ -302,6 +303,37 @@ struct WaveUpdate {
   __device__ __forceinline__ void interior(const Rows& b, int j, int k, int voff, int soff, T Au, T uc,
                                            const Pre& q) const {
     buf_st<T>(b.n, scaled(j, k, q.r, Au) + T(2) * uc + T(-1) * q.up, voff, soff);
+  }
+};
+
+// The wave update with the fault's step fused in (Fault_fd::d_dt after
+// propagateWaves, strikeSlip_linearElastic_fd.cpp:421-445): at a fault node
+// (j = 0, a frame node) the thread that forms D2u there runs the node's
+// slip-velocity and state solves (fault_fd_node) and writes the fault
+// displacement into uNext, so the fault's root solves run beside the
+// interior tiles of the same launch instead of in a launch of their own.
+// The arithmetic is the two launches' own, so the results are the same bit
+// for bit.
+template <typename T>
+struct WaveFault {
+  WaveUpdate<T, true> w;
+  sc_fault_fd_args f;
+  __device__ __forceinline__ void operator()(int j, int k, size_t I, T Au, T uc, bool edge) const {
+    if (j != 0) {
+      w(j, k, I, Au, uc, edge);
+      return;
+    }
+    const T d2u = w.d2u_of(j, k, Au);
+    w.d2uFault[k] = d2u;
+    w.uNext[I] = (T)fault_fd_node(f, k, (double)d2u);
+  }
+  using Rows = typename WaveUpdate<T, true>::Rows;
+  using Pre = typename WaveUpdate<T, true>::Pre;
+  __device__ __forceinline__ Rows rows(int bytes) const { return w.rows(bytes); }
+  __device__ __forceinline__ Pre pre(const Rows& b, int voff, int soff) const { return w.pre(b, voff, soff); }
+  __device__ __forceinline__ void interior(const Rows& b, int j, int k, int voff, int soff, T Au, T uc,
+                                           const Pre& q) const {
+    w.interior(b, j, k, voff, soff, Au, uc, q);
   }
 };
 
@@ -879,8 +911,13 @@ static int launch_wave_t(const Ctx& c, const DevOp<T>& op, const WaveArgs<T>& a,
   if (a.rhoIsC1) {
     WaveUpdate<T, true> ep{a.uNext, a.uPrev, a.rho, a.ay, op.y.Hinv, op.z.Hinv, op.y.imet, op.z.imet,
                            a.d2uFault, a.dt, c.d.var_grid};
+    if (a.fault) {
+      if (a.fault->N != c.d.Nz) return fail(SC_ERR_ARG, "wave step: the fault has Nz nodes");
+      return launch_apply_t(c, op, a.u, WaveFault<T>{ep, *a.fault}, s);
+    }
     return launch_apply_t(c, op, a.u, ep, s);
   }
+  if (a.fault) return fail(SC_ERR_ARG, "wave step: the fused fault step needs the per-node scale");
   WaveUpdate<T, false> ep{a.uNext, a.uPrev, a.rho, a.ay, op.y.Hinv, op.z.Hinv, op.y.imet, op.z.imet,
                           a.d2uFault, a.dt, c.d.var_grid};
   return launch_apply_t(c, op, a.u, ep, s);

@@ -151,6 +151,9 @@ struct WaveArgs {
   T* d2uFault;
   T dt;
   int rhoIsC1 = 0;
+  // the fault's step fused into the launch (Fault_fd::d_dt at j = 0; needs
+  // rhoIsC1), or null
+  const sc_fault_fd_args* fault = nullptr;
 };
 // c1 = dt^2/rho * Hy^-1 Hz^-1 (* qy rz on a variable grid) per node: the
 // wave update's whole scale of A u, formed once for a run at constant dt
@@ -292,7 +295,6 @@ int launch_pl_strain(const Ctx& c, const double* u, const double* gVxy, const do
 int fault_qd_rates(hipStream_t s, const sc_fault_qd_args& a);
 // FD fault step; uBody (may be null) receives the fault displacement
 int fault_fd_step64(hipStream_t s, const sc_fault_fd_args& a, double* uBody);
-int fault_fd_step32(hipStream_t s, const sc_fault_fd_args& a, const float* d2u32, float* uBody);
 
 }  // namespace sc
 
