@@ -2,9 +2,8 @@
 // root solve, state evolution, shear stress and strength (Fault_qd::d_dt,
 // fault.cpp:861-918), of the fully dynamic step (Fault_fd::d_dt,
 // fault.cpp:1436-1523), plus the small vector kernels of the explicit
-// Runge-Kutta driver (odeSolver.cpp) and the boundary-data update of
-// StrikeSlip_LinearElastic_qd::d_dt (strikeSlip_linearElastic_qd.cpp:763-770).
-// All of it is FP64: the argument of asinh reaches ~1e30.
+// Runge-Kutta driver (odeSolver.cpp).  All of it is FP64: the argument of
+// asinh reaches ~1e30.
 #include <cmath>
 
 #include "fault_dev.hpp"
@@ -46,16 +45,6 @@ int fault_fd_step64(hipStream_t s, const sc_fault_fd_args& a, double* uBody) {
 }
 
 // ------------------------------------------------------ QD driver helpers
-// bcL = slip / fts ; bcR = vL*t/fts + bcRShift
-__global__ void k_qd_bc(int Nz, const double* __restrict__ slip, const double* __restrict__ shift,
-                        double t, double vL, double fts, int remote, double* __restrict__ bcL,
-                        double* __restrict__ bcR) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= Nz) return;
-  bcL[k] = slip[k] * (1.0 / fts);
-  if (remote) bcR[k] = vL * t / fts + 1.0 * shift[k];
-}
-
 // Error measures over one vector pair (lo, hi): [sum (lo-hi)^2, sum hi^2, max |lo-hi|/hi]
 __global__ __launch_bounds__(256) void k_err(int n, const double* __restrict__ lo,
                                              const double* __restrict__ hi, double* __restrict__ out) {
@@ -86,13 +75,6 @@ __global__ __launch_bounds__(256) void k_err(int n, const double* __restrict__ l
   }
 }
 
-int launch_qd_bc(hipStream_t s, int Nz, const double* slip, const double* shift, double t,
-                 double vL, double fts, int remote, double* bcL, double* bcR) {
-  hipLaunchKernelGGL(k_qd_bc, dim3((Nz + 255) / 256), dim3(256), 0, s, Nz, slip, shift, t, vL, fts,
-                     remote, bcL, bcR);
-  SC_HIP(hipGetLastError());
-  return SC_OK;
-}
 
 // Long vectors (the viscous strains, N entries): per-block partial sums, then
 // one block combines them in block order (deterministic).

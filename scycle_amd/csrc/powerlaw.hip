@@ -15,7 +15,8 @@
 // evaluated in FP64 in every variant (exponents reach e^-50).
 #include <cmath>
 
-#include "sc_internal.hpp"
+#include "fault_dev.hpp"
+#include "sbp_dev.hpp"
 
 namespace sc {
 
@@ -72,15 +73,26 @@ __device__ __forceinline__ double d1_line(const DevDir<double>& d, int n, int i,
   return s;
 }
 
-// rhs += B gxy + C gxz, B = yq zr H Dy mu (+ Neumann SATs on L/R), C likewise
-// with Dz (T/B)
+// rhs = setRhs(boundary data) + B gxy + C gxz, B = yq zr H Dy mu (+ Neumann
+// SATs on L/R), C likewise with Dz (T/B): the boundary-data update of the
+// mediator's d_dt (bcL = slip/fts, bcR = vL t/fts + shift when remotely
+// loaded; strikeSlip_powerLaw_qd.cpp d_dt), SbpOps::setRhs and the viscous
+// source in one pass.  Every thread forms the boundary data it reads from the
+// stage's slip (no thread reads what another writes); the j = 0 row writes
+// bcL and bcR for the later kernels.  The same arithmetic as the boundary
+// kernel, setRhs and the source term launched one after the other.
 __global__ __launch_bounds__(256) void k_pl_source(DevOp<double> p, const double* __restrict__ gxy,
                                                    const double* __restrict__ gxz,
-                                                   double* __restrict__ rhs) {
+                                                   double* __restrict__ rhs, PlRhsArgs b) {
   const int Ny = p.Ny, Nz = p.Nz;
   const long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= (long long)Ny * Nz) return;
   const int j = (int)(I / Nz), k = (int)(I - (long long)j * Nz);
+  auto bcL = [&](int i) { return b.slip[i] * (1.0 / b.fts); };
+  auto bcR = [&](int i) { return b.remote ? b.vL * b.t / b.fts + 1.0 * b.shift[i] : b.bcR[i]; };
+  const double r0 = rhs_node(p, j, k, [&](int side, int i) {
+    return side == 0 ? bcL(i) : side == 1 ? bcR(i) : side == 2 ? b.bcT[i] : b.bcB[i];
+  });
   const double* mu = p.mu;
   const double wgt = p.y.met[j] * p.z.met[k] * p.y.H[j] * p.z.H[k];
   const double sy = d1_line(p.y, Ny, j, [&](int m) { return mu[(size_t)m * Nz + k] * gxy[(size_t)m * Nz + k]; });
@@ -91,7 +103,11 @@ __global__ __launch_bounds__(256) void k_pl_source(DevOp<double> p, const double
   if (j == Ny - 1 && p.bcR == SC_NEUMANN) v -= p.z.met[k] * p.z.H[k] * muI * gxy[I];
   if (k == 0 && p.bcT == SC_NEUMANN) v += p.y.met[j] * p.y.H[j] * muI * gxz[I];
   if (k == Nz - 1 && p.bcB == SC_NEUMANN) v -= p.y.met[j] * p.y.H[j] * muI * gxz[I];
-  rhs[I] = rhs[I] + 1.0 * v;
+  rhs[I] = r0 + 1.0 * v;
+  if (j == 0) {
+    b.bcLOut[k] = bcL(k);
+    if (b.remote) b.bcR[k] = bcR(k);
+  }
 }
 
 // Strains, stresses, viscosity and strain rates in one pass per node.
@@ -99,6 +115,11 @@ struct PlStrainArgs {
   const double *u, *gVxy, *gVxz, *T, *A, *B, *n, *bcL, *bcR;
   double cap;
   double *dgxy, *dgxz, *tauQS, *effVisc, *sxyOut;
+  // Fault_qd::d_dt fused in at the fault nodes (fault = 1): the thread that
+  // forms sigma_xy at (0, k) runs the node's root solve with it
+  int fault;
+  sc_fault_qd_args fa;
+  double* dslip;
 };
 
 __global__ __launch_bounds__(256) void k_pl_strain(DevOp<double> p, PlStrainArgs a) {
@@ -131,12 +152,16 @@ __global__ __launch_bounds__(256) void k_pl_strain(DevOp<double> p, PlStrainArgs
   a.dgxz[I] = gz;
   if (a.effVisc) a.effVisc[I] = ev;
   if (a.sxyOut) a.sxyOut[I] = sxy;
-  if (j == 0) a.tauQS[k] = sxy;
+  if (j == 0) {
+    a.tauQS[k] = sxy;
+    if (a.fault) a.dslip[k] = fault_qd_node(a.fa, k, sxy);
+  }
 }
 
-int launch_pl_source(const Ctx& c, const double* gxy, const double* gxz, double* rhs, hipStream_t s) {
+int launch_pl_source(const Ctx& c, const double* gxy, const double* gxz, double* rhs, const PlRhsArgs& b,
+                     hipStream_t s) {
   const long long N = (long long)c.N;
-  hipLaunchKernelGGL(k_pl_source, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, c.op64, gxy, gxz, rhs);
+  hipLaunchKernelGGL(k_pl_source, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, c.op64, gxy, gxz, rhs, b);
   SC_HIP(hipGetLastError());
   return SC_OK;
 }
@@ -186,9 +211,16 @@ int launch_pl_fields(const Ctx& c, const PlFields& f, hipStream_t s) {
 int launch_pl_strain(const Ctx& c, const double* u, const double* gVxy, const double* gVxz,
                      const double* T, const double* A, const double* B, const double* n,
                      const double* bcL, const double* bcR, double cap, double* dgxy, double* dgxz,
-                     double* tauQS, double* effVisc, double* sxyOut, hipStream_t s) {
+                     double* tauQS, double* effVisc, double* sxyOut, const sc_fault_qd_args* fault,
+                     double* dslip, hipStream_t s) {
   const long long N = (long long)c.N;
   PlStrainArgs a{u, gVxy, gVxz, T, A, B, n, bcL, bcR, cap, dgxy, dgxz, tauQS, effVisc, sxyOut};
+  a.fault = fault ? 1 : 0;
+  if (fault) {
+    if (fault->N != c.d.Nz) return fail(SC_ERR_ARG, "power-law strain: the fault has Nz nodes");
+    a.fa = *fault;
+    a.dslip = dslip;
+  }
   hipLaunchKernelGGL(k_pl_strain, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, c.op64, a);
   SC_HIP(hipGetLastError());
   return SC_OK;

@@ -22,8 +22,6 @@
 
 namespace sc {
 
-int launch_qd_bc(hipStream_t s, int Nz, const double* slip, const double* shift, double t,
-                 double vL, double fts, int remote, double* bcL, double* bcR);
 int launch_err(hipStream_t s, int n, const double* lo, const double* hi, double* out3,
                double* partials);
 
@@ -358,11 +356,10 @@ int QdDriver::d_dt(double t, const double* psi, const double* slip, double* dpsi
   } else {
     e = flush_finish();
     if (!e && pre) e = launch_vars_lincomb(s, *pre);
-    if (!e)
-      e = launch_qd_bc(s, Nz, slip, shift_d, t, desc.vL, desc.faultTypeScale, desc.remoteLoadingR, bcL_d,
-                       bcR_d);
-    if (!e) e = launch_set_rhs(ctx->c, rhs_d, bcL_d, bcR_d, bcT_d, bcB_d, s);
-    if (!e) e = launch_pl_source(ctx->c, gxy, gxz, rhs_d, s);
+    // the boundary data, setRhs and the viscous source in one launch
+    const PlRhsArgs rb{slip, shift_d, t, desc.vL, desc.faultTypeScale, desc.remoteLoadingR, bcR_d,
+                       bcT_d, bcB_d, bcL_d};
+    if (!e) e = launch_pl_source(ctx->c, gxy, gxz, rhs_d, rb, s);
   }
   if (e) return e;
   sc_fault_qd_args fa = fault_args(psi, dpsi);
@@ -388,9 +385,11 @@ int QdDriver::d_dt(double t, const double* psi, const double* slip, double* dpsi
   }
   if (e) return e;
   if (pl) {
+    // the fault's rates fused in at the fault nodes
     e = launch_pl_strain(ctx->c, u_d, gxy, gxz, pl->T, pl->A, pl->B, pl->n, bcL_d, bcR_d, pl->cap,
-                         dgxy, dgxz, tauQS_d, pl->effVisc, pl->sxy, s);
+                         dgxy, dgxz, tauQS_d, pl->effVisc, pl->sxy, &fa, dslip, s);
     if (e) return e;
+    fused = true;
   }
   // sigma_xy on the fault from u (the power law's strain kernel wrote it already)
   if (!fused) {

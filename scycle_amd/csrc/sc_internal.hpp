@@ -280,7 +280,20 @@ int bench_proj_guess(Ctx& c, int reps, double* ms, double* words);
 // z = M^-1 r with one of the solve's preconditioners (SC_PREC_*)
 int precond_apply(Ctx& c, int precond, const double* r, double* z);
 // power law: rhs += B gxy + C gxz; strains -> stresses -> viscosity -> rates
-int launch_pl_source(const Ctx& c, const double* gxy, const double* gxz, double* rhs, hipStream_t s);
+// the boundary data of the power-law d_dt, formed in the source kernel:
+// bcL = slip/fts; bcR = vL t/fts + shift when remote, else bcR as given
+struct PlRhsArgs {
+  const double *slip, *shift;
+  double t, vL, fts;
+  int remote;
+  double* bcR;  // in (fixed datum) or out (remote)
+  const double *bcT, *bcB;
+  double* bcLOut;
+};
+// rhs = setRhs + the viscous source (the boundary data, setRhs and
+// computeViscStrainSourceTerms in one launch)
+int launch_pl_source(const Ctx& c, const double* gxy, const double* gxz, double* rhs, const PlRhsArgs& b,
+                     hipStream_t s);
 // the power law's 2-D output fields at the current state (N each)
 struct PlFields {
   const double *u, *gVxy, *gVxz, *T, *A, *B, *n;
@@ -291,7 +304,8 @@ int launch_pl_fields(const Ctx& c, const PlFields& f, hipStream_t s);
 int launch_pl_strain(const Ctx& c, const double* u, const double* gVxy, const double* gVxz,
                      const double* T, const double* A, const double* B, const double* n,
                      const double* bcL, const double* bcR, double cap, double* dgxy, double* dgxz,
-                     double* tauQS, double* effVisc, double* sxyOut, hipStream_t s);
+                     double* tauQS, double* effVisc, double* sxyOut, const sc_fault_qd_args* fault,
+                     double* dslip, hipStream_t s);
 int fault_qd_rates(hipStream_t s, const sc_fault_qd_args& a);
 // FD fault step; uBody (may be null) receives the fault displacement
 int fault_fd_step64(hipStream_t s, const sc_fault_fd_args& a, double* uBody);
