@@ -853,7 +853,10 @@ int QdDriver::lohi_error(const RkState& r, int ntlo, const int* filo, const doub
       for (int i = 0; i < nthi; ++i) S.fhi[q][i] = fsel(fihi[i]);
       S.out[q] = slot[fi] >= 0 ? err_d + 3 * slot[fi] : err_d + 12;
     } else {
+      // the low-order solution of a long field enters only its error sums:
+      // formed only when the norm reads the field
       for (VarsLinComb* L : {&Llo, &Lhi}) {
+        if (L == &Llo && slot[fi] < 0) continue;
         const int q = L->nv++;
         L->n[q] = v.n;
         L->out[q] = L == &Llo ? v.lo : v.hi;
