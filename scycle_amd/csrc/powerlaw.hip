@@ -15,6 +15,7 @@
 // evaluated in FP64 in every variant (exponents reach e^-50).
 #include <cmath>
 
+#include "cg_dev.hpp"
 #include "fault_dev.hpp"
 #include "sbp_dev.hpp"
 
@@ -120,9 +121,15 @@ struct PlStrainArgs {
   int fault;
   sc_fault_qd_args fa;
   double* dslip;
+  // the solve's deferred guess check (sc non-null): one more block
+  PendingFinish fin;
 };
 
-__global__ __launch_bounds__(256) void k_pl_strain(DevOp<double> p, PlStrainArgs a) {
+__global__ __launch_bounds__(256) void k_pl_strain(DevOp<double> p, PlStrainArgs a, int nGrid) {
+  if ((int)blockIdx.x >= nGrid) {
+    finish_irres_block(a.fin);
+    return;
+  }
   const int Ny = p.Ny, Nz = p.Nz;
   const long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= (long long)Ny * Nz) return;
@@ -212,16 +219,19 @@ int launch_pl_strain(const Ctx& c, const double* u, const double* gVxy, const do
                      const double* T, const double* A, const double* B, const double* n,
                      const double* bcL, const double* bcR, double cap, double* dgxy, double* dgxz,
                      double* tauQS, double* effVisc, double* sxyOut, const sc_fault_qd_args* fault,
-                     double* dslip, hipStream_t s) {
+                     double* dslip, const PendingFinish* fin, hipStream_t s) {
   const long long N = (long long)c.N;
   PlStrainArgs a{u, gVxy, gVxz, T, A, B, n, bcL, bcR, cap, dgxy, dgxz, tauQS, effVisc, sxyOut};
+  if (fin && fin->sc) a.fin = *fin;
   a.fault = fault ? 1 : 0;
   if (fault) {
     if (fault->N != c.d.Nz) return fail(SC_ERR_ARG, "power-law strain: the fault has Nz nodes");
     a.fa = *fault;
     a.dslip = dslip;
   }
-  hipLaunchKernelGGL(k_pl_strain, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, c.op64, a);
+  const int nGrid = (int)((N + 255) / 256);
+  hipLaunchKernelGGL(k_pl_strain, dim3((unsigned)nGrid + (a.fin.sc ? 1u : 0u)), dim3(256), 0, s, c.op64, a,
+                     nGrid);
   SC_HIP(hipGetLastError());
   return SC_OK;
 }

@@ -371,7 +371,7 @@ int QdDriver::d_dt(double t, const double* psi, const double* slip, double* dpsi
   // a deferred guess of the linear-elastic step also computes the fault rates
   const QdFuse fuse{fa, dslip};
   e = solve(ctx->c, rhs_d, u_d, desc.solve, st, defer ? miss_d : nullptr, &deferred, !pl, mnb,
-            pl ? nullptr : &fuse, &fused, pl ? nullptr : &fin, gate, pl ? nullptr : &lazy);
+            pl ? nullptr : &fuse, &fused, &fin, gate, pl ? nullptr : &lazy);
   finPending = fin.sc != nullptr;
   uLazy = lazy;  // a solve that wrote u ends the laziness of an earlier one
   if (count) {
@@ -385,10 +385,12 @@ int QdDriver::d_dt(double t, const double* psi, const double* slip, double* dpsi
   }
   if (e) return e;
   if (pl) {
-    // the fault's rates fused in at the fault nodes
+    // the fault's rates fused in at the fault nodes, and the deferred guess
+    // check as one more block
     e = launch_pl_strain(ctx->c, u_d, gxy, gxz, pl->T, pl->A, pl->B, pl->n, bcL_d, bcR_d, pl->cap,
-                         dgxy, dgxz, tauQS_d, pl->effVisc, pl->sxy, &fa, dslip, s);
+                         dgxy, dgxz, tauQS_d, pl->effVisc, pl->sxy, &fa, dslip, finPending ? &fin : nullptr, s);
     if (e) return e;
+    finPending = false;
     fused = true;
   }
   // sigma_xy on the fault from u (the power law's strain kernel wrote it already)

@@ -305,7 +305,7 @@ int launch_pl_strain(const Ctx& c, const double* u, const double* gVxy, const do
                      const double* T, const double* A, const double* B, const double* n,
                      const double* bcL, const double* bcR, double cap, double* dgxy, double* dgxz,
                      double* tauQS, double* effVisc, double* sxyOut, const sc_fault_qd_args* fault,
-                     double* dslip, hipStream_t s);
+                     double* dslip, const PendingFinish* fin, hipStream_t s);
 int fault_qd_rates(hipStream_t s, const sc_fault_qd_args& a);
 // FD fault step; uBody (may be null) receives the fault displacement
 int fault_fd_step64(hipStream_t s, const sc_fault_fd_args& a, double* uBody);
