@@ -192,14 +192,20 @@ __device__ __forceinline__ FaultNodeIn fault_node_load(const sc_fault_qd_args& a
   return in;
 }
 
-// Fault_qd::d_dt at fault node k (fault.cpp:861-918) from the quasi-static
-// shear stress tq0 (before the prestress) and the node's loaded inputs:
-// slip velocity from the root solve, state rate, shear stress and strength;
-// returns V.
-__device__ __forceinline__ double fault_qd_node_in(const sc_fault_qd_args& a, int k, double tq0,
-                                                   const FaultNodeIn& in) {
+// Fault_qd::d_dt at one fault node (fault.cpp:861-918) from the quasi-static
+// shear stress tq0 (before the prestress) and the node's loaded inputs: slip
+// velocity from the root solve, state rate, shear stress and strength, formed
+// in registers (fault_qd_eval) and then written (fault_qd_commit), so that a
+// caller can form them before it knows whether it will keep them.
+struct FaultNodeOut {
+  double tq, V, dpsi, tau, strength;
+  int bad;
+};
+__device__ __forceinline__ FaultNodeOut fault_qd_eval(const sc_fault_qd_args& a, double tq0,
+                                                      const FaultNodeIn& in) {
+  FaultNodeOut o;
   const double tq = tq0 + 1.0 * in.pre;
-  a.tauQS_d[k] = tq;
+  o.tq = tq;
   const double psi = in.psi, aa = in.a, sN = in.sN, eta = in.eta;
   const double lk = in.lk;
   double V = in.V;
@@ -237,7 +243,7 @@ __device__ __forceinline__ double fault_qd_node_in(const sc_fault_qd_args& a, in
       }
     }
   }
-  a.V_d[k] = V;
+  o.V = V;
   double dpsi;
   if (a.stateLaw == SC_AGING_LAW) {  // aging_law_psi with its exponential formed above
     dpsi = 0.;
@@ -246,11 +252,25 @@ __device__ __forceinline__ double fault_qd_node_in(const sc_fault_qd_args& a, in
     dpsi = slip_law_psi(psi, V, aa, in.b, a.f0, a.v0, in.Dc);
   }
   if (!isfinite(dpsi)) bad = SC_ERR_NONFINITE;
-  a.dpsi_d[k] = dpsi;
-  a.tau_d[k] = -1.0 * (eta * V) + tq;
-  a.strength_d[k] = haveStrength ? strength : strength_psi(sN, psi, V, aa, a.v0);
-  if (bad && a.status_d) atomicOr(a.status_d, bad);
-  return V;
+  o.dpsi = dpsi;
+  o.tau = -1.0 * (eta * V) + tq;
+  o.strength = haveStrength ? strength : strength_psi(sN, psi, V, aa, a.v0);
+  o.bad = bad;
+  return o;
+}
+// writes the node's outputs; returns V
+__device__ __forceinline__ double fault_qd_commit(const sc_fault_qd_args& a, int k, const FaultNodeOut& o) {
+  a.tauQS_d[k] = o.tq;
+  a.V_d[k] = o.V;
+  a.dpsi_d[k] = o.dpsi;
+  a.tau_d[k] = o.tau;
+  a.strength_d[k] = o.strength;
+  if (o.bad && a.status_d) atomicOr(a.status_d, o.bad);
+  return o.V;
+}
+__device__ __forceinline__ double fault_qd_node_in(const sc_fault_qd_args& a, int k, double tq0,
+                                                   const FaultNodeIn& in) {
+  return fault_qd_commit(a, k, fault_qd_eval(a, tq0, in));
 }
 
 // the same from memory: the node's inputs loaded here

@@ -744,23 +744,33 @@ __global__ __launch_bounds__(kStageThreads, 2) void k_qd_stage(QdStageArgs a) {
 #pragma unroll
   for (int i = 0; i < kStageNT; ++i) cc[i] = a.L.c[i];
   double t = a.t0;
-  // the attempt's end: the control block's copy, the error sums, and
-  // whether the accepted attempt's controller is left to block 0's second wave
+  // the attempt's end: the control block's copy
   __shared__ unsigned long long cw[END ? kCtlW : 1];
-  __shared__ double sel[END ? 3 * kMaxVars : 1];
-  __shared__ int fastS;
   if constexpr (END) {
     // The attempt's end, formed by every block alike (the same loads and
     // arithmetic, so every block reaches the same decision): the last
     // stage's residual test (k_finish's order), the RK32 pair and its error
-    // sums, the step control on a copy of the control block.  Block 0
-    // writes the updated control block to ctlOut (the next attempt reads
-    // it; this attempt's kernels read ctlIn), the test's scalars and flag;
-    // the fault waves write var <- hi.  An accepted attempt then forms the
-    // rates at its solution.
-    __shared__ double eS[2][3][kStageThreads / 64], fS[2][kRedThreads / 64];
-    __shared__ int flg[2], accS;
+    // sums, the step control on a copy of the control block, by the second
+    // wave.  Beside the decision the other waves form the rates (fault
+    // lanes) and the guess residual (residual blocks) at the pair's solution
+    // as if the attempt were accepted, in registers; the writes follow the
+    // decision: an accepted attempt's rates and next state (var <- hi), or a
+    // rejected or closed attempt's start passed on.  Block 0 writes the
+    // updated control block to ctlOut (the next attempt reads it; this
+    // attempt's kernels read ctlIn), the test's scalars and flag.
+    __shared__ double rred[2][kStageThreads / 64];
+    __shared__ double sel[3 * kMaxVars];
+    __shared__ int accS, fastS;
     __shared__ double tAccS;
+    double wPm[M], sRm0[M], sRm1[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      wPm[m] = a.w[m];
+      sRm0[m] = a.sR[m];
+      sRm1[m] = a.sR[M + m];
+    }
+    __shared__ double eS[2][3][kStageThreads / 64], fS[2][kRedThreads / 64];
+    __shared__ int flg[2];
     StepCtl* cs = reinterpret_cast<StepCtl*>(cw);
     if (tid < kCtlW) cw[tid] = cwReg;
     if (tid == kCtlW) {
@@ -804,7 +814,11 @@ __global__ __launch_bounds__(kStageThreads, 2) void k_qd_stage(QdStageArgs a) {
     for (int i = 0; i < 2; ++i) clo[i] = (a.clo[i] * h) / a.cdlo[i];
 #pragma unroll
     for (int i = 0; i < 3; ++i) chi[i] = (a.chi[i] * h) / a.cdhi[i];
+    const double tSpec = cs->t + cs->dt;  // an accepted step's end (the fast path's)
     double ea[2] = {0.0, 0.0}, eb[2] = {0.0, 0.0}, em[2] = {-INFINITY, -INFINITY};
+    double acc[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) acc[m] = 0.0;
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
       const int k = tid + q * kStageThreads;
@@ -823,6 +837,12 @@ __global__ __launch_bounds__(kStageThreads, 2) void k_qd_stage(QdStageArgs a) {
           em[f] = fmax(em[f], fabs(e) / hi);
           (f == 0 ? pS : sS)[k] = hi;
         }
+        // the pair's solution as the next stage state (the stage path's
+        // arithmetic): right datum and this thread's share of the projections
+        const double l = sS[k] * (1.0 / a.fts);
+        rS[k] = a.remote ? a.vL * tSpec / a.fts + 1.0 * rk[q] : rk[q];
+#pragma unroll
+        for (int m = 0; m < M; ++m) acc[m] += pl[q][m] * l;
       }
     }
     // only the sums the norm reads (computeError, odeSolver.cpp:449-501):
@@ -840,65 +860,172 @@ __global__ __launch_bounds__(kStageThreads, 2) void k_qd_stage(QdStageArgs a) {
         eS[f][2][wid] = z;
       }
     }
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const double sm = wave_sum(acc[m]);
+      if (lane == 0) red[m][wid] = sm;
+    }
     __syncthreads();
-    bool miss3 = false;
-    if (tid < 64) {
+    // the coefficients at the pair's solution, every thread (the stage path's order)
+    double c[M];
+    auto coefs_at = [&](double tt) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        double yb = 0.0;
+        for (int q = 0; q < kStageThreads / 64; ++q) yb += red[m][q];
+        yb += a.remote ? (a.vL * tt / a.fts) * sRm0[m] + sRm1[m] : sRm1[m];
+        c[m] = -wPm[m] * yb;
+      }
+    };
+    coefs_at(tSpec);
+    // the rates (fault lanes) or the guess residual (residual blocks) at c
+    FaultNodeOut fo{};
+    double r2 = 0.0, b2 = 0.0;
+    auto spec = [&]() {
+      if (sblk < 0) {
+        if (faultLane) {
+          double tq = 0.0;
+#pragma unroll
+          for (int m = 0; m < M; ++m) tq += c[m] * tm[m];
+          fnode.psi = pS[kf];
+          fo = fault_qd_eval(a.fuse.a, tq, fnode);
+        }
+      } else {
+        r2 = 0.0;
+        b2 = 0.0;
+#pragma unroll
+        for (int u = 0; u < kBN; ++u) {
+          if (bi_[u] < a.N) {
+            double az = 0.0;
+#pragma unroll
+            for (int m = 0; m < M; ++m) az += c[m] * zb[u][m];
+            const int k = (int)(bi_[u] % (size_t)Nz);
+            const double bi = ab[u][0] * (sS[k] * (1.0 / a.fts)) + ab[u][1] * rS[k];
+            const double ri = bi - az;
+            r2 += ri * ri;
+            b2 += bi * bi;
+          }
+        }
+      }
+    };
+    if (wid == 1) {
+      // the decision: the last stage's residual test and the selected error
+      // sums (each lane forms them), then the controller.  An attempt its
+      // error accepts is accepted at once, block 0 forming the next step
+      // size and control block here; a flagged, rejected or forced attempt
+      // runs the whole controller in every block.
       double t0 = 0.0, t1 = 0.0;
       for (int ww = 0; ww < kRedThreads / 64; ++ww) t0 += fS[0][ww];
       for (int ww = 0; ww < kRedThreads / 64; ++ww) t1 += fS[1][ww];
-      miss3 = !(t0 <= a.tol2 * t1 || t1 == 0.0);
-      if (tid == 0) {
-        for (int q = 0; q < 3 * kMaxVars; ++q) sel[q] = 0.0;
-        for (int f = 0; f < 2; ++f) {
-          const int sl = a.selSlot[f];
-          if (sl < 0) continue;
-          double x = eS[f][0][0], y = eS[f][1][0], z = eS[f][2][0];
-          for (int ww = 1; ww < kStageThreads / 64; ++ww) {
-            x += eS[f][0][ww];
-            y += eS[f][1][ww];
-            z = fmax(z, eS[f][2][ww]);
-          }
-          sel[3 * sl] = x;
-          sel[3 * sl + 1] = y;
-          sel[3 * sl + 2] = z;
+      const bool miss3 = !(t0 <= a.tol2 * t1 || t1 == 0.0);
+      // (every lane writes the same values: the wave reads them back below)
+#pragma unroll
+      for (int q = 0; q < 3 * kMaxVars; ++q) sel[q] = 0.0;
+#pragma unroll
+      for (int f = 0; f < 2; ++f) {
+        const int sl = a.selSlot[f];
+        if (sl < 0) continue;
+        double x = eS[f][0][0], y = eS[f][1][0], z = eS[f][2][0];
+        for (int ww = 1; ww < kStageThreads / 64; ++ww) {
+          x += eS[f][0][ww];
+          y += eS[f][1][ww];
+          z = fmax(z, eS[f][2][ww]);
         }
-        if (blk == 0) {  // the test's scalars and flag, for the host
-          a.fin3.sc->rr = t0;
-          a.fin3.sc->bb = t1;
-          a.fin3.sc->done = miss3 ? 0 : 1;
-          if (miss3 && a.fin3.flag) *a.fin3.flag = 1;
-        }
+        sel[3 * sl] = x;
+        sel[3 * sl + 1] = y;
+        sel[3 * sl + 2] = z;
       }
-    }
-    __syncthreads();
-    // An attempt its error accepts goes on to the rates at its end, t + dt,
-    // at once; the next step size (the controller's pow chain) is formed
-    // beside them by block 0's second wave, which also writes the next
-    // control block.  A flagged, rejected or forced attempt runs the whole
-    // controller here.
-    if (tid < 64) {
+      if (blk == 0 && lane == 0) {  // the test's scalars and flag, for the host
+        a.fin3.sc->rr = t0;
+        a.fin3.sc->bb = t1;
+        a.fin3.sc->done = miss3 ? 0 : 1;
+        if (miss3 && a.fin3.flag) *a.fin3.flag = 1;
+      }
       const bool flagged = flg[0] || flg[1] || miss3;
       if (!flagged && step_accepts(cs, sel)) {
-        if (tid == 0) {
+        if (lane == 0) {
           accS = 1;
           fastS = 1;
           tAccS = cs->t + cs->dt;
         }
+        if (blk == 0) {
+          (void)step_control_wave(cs, sel, false, true);
+          if (lane == 0) *a.ctlOut = *cs;
+        }
       } else {
         const bool ok = step_control_wave(cs, sel, flagged, blk == 0);
-        if (tid == 0) {
+        if (lane == 0) {
           accS = ok ? 1 : 0;
           fastS = 0;
         }
       }
     }
+    spec();
     __syncthreads();
     if (!fastS && blk == 0 && tid < kCtlW) reinterpret_cast<unsigned long long*>(a.ctlOut)[tid] = cw[tid];
     if (!accS) {
       pass_state();
       return;
     }
-    t = fastS ? tAccS : cs->t;  // the accepted step's end
+    const double tEnd = fastS ? tAccS : cs->t;
+    if (a.remote && tEnd != tSpec) {
+      // an acceptance the controller forced ended the step at another time:
+      // the right datum, the coefficients and the rates there
+#pragma unroll
+      for (int q = 0; q < kQ; ++q) {
+        const int k = tid + q * kStageThreads;
+        if (k < Nz) rS[k] = a.vL * tEnd / a.fts + 1.0 * rk[q];
+      }
+      __syncthreads();
+      coefs_at(tEnd);
+      spec();
+    }
+    if (sblk < 0) {
+      if (faultLane) {
+        const int k = kf;
+        a.pvOut[0][k] = pS[k];
+        a.pvOut[1][k] = sS[k];
+        a.fuse.dslip[k] = fault_qd_commit(a.fuse.a, k, fo);
+        a.bcL[k] = sS[k] * (1.0 / a.fts);
+        if (a.remote) a.bcR[k] = rS[k];
+      }
+      return;
+    }
+    if (sblk == 0 && tid == 0) {
+      a.sc->tol2 = a.tol2;
+      a.sc->maxit = 0;
+      a.sc->done = 0;
+      a.sc->iters = 0;
+      a.sc->cnt = 0;
+    }
+    if (sblk == 0 && tid < M) a.coefOut[tid] = c[tid];
+    r2 = wave_sum(r2);
+    b2 = wave_sum(b2);
+    if (lane == 0) {
+      rred[0][wid] = r2;
+      rred[1][wid] = b2;
+    }
+    __syncthreads();
+    if (tid < 2) {
+      double v = 0.0;
+      for (int q = 0; q < kStageThreads / 64; ++q) v += rred[tid][q];
+      if (tid == 0 && sblk == 0) {
+        // the nodes off the band rows, where b = 0: ||sum c_m A y_m||^2 =
+        // c^T G c with the off-band Gram matrix (double-double)
+        DD q2{0.0, 0.0};
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+#pragma unroll
+          for (int n = 0; n < M; ++n) {
+            const size_t o = ((size_t)m * M + n) * 2;
+            q2 = dd_add(q2, dd_mul_d(dd_mul_d(DD{a.G[o], a.G[o + 1]}, c[m]), c[n]));
+          }
+        const double off = q2.hi + q2.lo;
+        v += off > 0.0 ? off : 0.0;
+      }
+      a.parts[(size_t)sblk * 2 + tid] = v;
+    }
+    return;
   } else {
     // gate_closed
     if (a.gate.ctl && (gHalt || (a.gate.mode == 2 ? !gAcc : (a.gate.mode == 1 && gDone)))) return;
@@ -909,6 +1036,7 @@ __global__ __launch_bounds__(kStageThreads, 2) void k_qd_stage(QdStageArgs a) {
       for (int i = 0; i < kStageNT; ++i) cc[i] = (a.L.c[i] * h) / a.L.cd[i];
     }
   }
+  // (the plain stage from here: the attempt's end returned above)
   // the stage's slip and right datum at every fault node (into LDS), and
   // this thread's share of sum_k PL_m(k) l(k)
   double acc[M];
@@ -918,8 +1046,8 @@ __global__ __launch_bounds__(kStageThreads, 2) void k_qd_stage(QdStageArgs a) {
   for (int q = 0; q < kQ; ++q) {
     const int k = tid + q * kStageThreads;
     if (k < Nz) {
-      double sk = END ? sS[k] : sb[q];
-      if (!END && a.ntS > 0) {  // the combination (k_vars_lincomb's order)
+      double sk = sb[q];
+      if (a.ntS > 0) {  // the combination (k_vars_lincomb's order)
         sk = cc[0] * sf[q][0] + sb[q];
 #pragma unroll
         for (int u = 1; u < kStageNT; ++u)
@@ -954,11 +1082,7 @@ __global__ __launch_bounds__(kStageThreads, 2) void k_qd_stage(QdStageArgs a) {
     const int k = kf;
     if (faultLane) {
       double psi = psiB;
-      if (END) {  // the accepted solution: the next attempt's state
-        psi = pS[k];
-        a.pvOut[0][k] = psi;
-        a.pvOut[1][k] = sS[k];
-      } else if (a.ntS > 0) {  // psi of the stage (k_vars_lincomb's order)
+      if (a.ntS > 0) {  // psi of the stage (k_vars_lincomb's order)
         double x = cc[0] * psiF[0] + psiB;
 #pragma unroll
         for (int u = 1; u < kStageNT; ++u)
@@ -974,15 +1098,6 @@ __global__ __launch_bounds__(kStageThreads, 2) void k_qd_stage(QdStageArgs a) {
       a.fuse.dslip[k] = fault_qd_node_in(a.fuse.a, k, tq, fnode);
       a.bcL[k] = sS[k] * (1.0 / a.fts);
       if (a.remote) a.bcR[k] = rS[k];
-    }
-    if constexpr (END) {
-      // the accepted attempt's controller (the same arithmetic as every
-      // block's full path) and the next control block, beside the root solves
-      if (blk == 0 && wid == 1 && fastS) {
-        StepCtl* cs = reinterpret_cast<StepCtl*>(cw);
-        (void)step_control_wave(cs, sel, false, true);
-        if (lane == 0) *a.ctlOut = *cs;
-      }
     }
     return;
   }
