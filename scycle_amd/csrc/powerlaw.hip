@@ -8,7 +8,9 @@
 //                 viscous strain rates, one pass per node (computeTotalStrains,
 //                 computeStresses, computeViscosity, computeViscStrainRates and
 //                 the SAT of computeViscousStrainRateSAT, :1674-1887), plus the
-//                 shear stress on the fault for Fault_qd
+//                 shear stress on the fault and Fault_qd::d_dt there; the
+//                 creep factor 1e3 A exp(-B/T) comes formed once per mediator
+//                 (k_pl_factor)
 //   k_pl_rates    the pointwise update alone (sigma -> eta, strain rates) in
 //                 FP64 or with FP32 arrays
 // The creep law 1/eta = 1e3 A s^(n-1) exp(-B/T) and the capped sum are
@@ -111,9 +113,26 @@ __global__ __launch_bounds__(256) void k_pl_source(DevOp<double> p, const double
   }
 }
 
-// Strains, stresses, viscosity and strain rates in one pass per node.
+// K = 1e3 A exp(-B/T), the dislocation-creep factor that does not change
+// with the stress (formed once per mediator)
+__global__ __launch_bounds__(256) void k_pl_factor(long long N, const double* __restrict__ A,
+                                                   const double* __restrict__ B, const double* __restrict__ T,
+                                                   double* __restrict__ K) {
+  const long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I < N) K[I] = (1e3 * A[I]) * exp(-B[I] / T[I]);
+}
+
+int launch_pl_factor(const Ctx& c, const double* A, const double* B, const double* T, double* K, hipStream_t s) {
+  const long long N = (long long)c.N;
+  hipLaunchKernelGGL(k_pl_factor, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N, A, B, T, K);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
+// Strains, stresses, viscosity and strain rates in one pass per node, with
+// the creep factor K precomputed: 1/eta = 1/cap + K s^(n-1).
 struct PlStrainArgs {
-  const double *u, *gVxy, *gVxz, *T, *A, *B, *n, *bcL, *bcR;
+  const double *u, *gVxy, *gVxz, *K, *n, *bcL, *bcR;
   double cap;
   double *dgxy, *dgxz, *tauQS, *effVisc, *sxyOut;
   // Fault_qd::d_dt fused in at the fault nodes (fault = 1): the thread that
@@ -153,8 +172,15 @@ __global__ __launch_bounds__(256) void k_pl_strain(DevOp<double> p, PlStrainArgs
     sat = sat + 1.0 * (hi * a.bcR[k]);
   }
   sat = p.y.imet[j] * sat;
-  double ev, gy, gz;
-  pl_rates(sxy, sxz, a.T[I], a.A[I], a.B[I], a.n[I], muI, sat, a.cap, &ev, &gy, &gz);
+  double sdev = sxy * sxy;
+  sdev = sdev + 1.0 * (sxz * sxz);
+  sdev = sqrt(fabs(sdev));
+  const double inv = a.K[I] * pow(sdev, a.n[I] - 1.0);
+  double ev = 1.0 / a.cap;
+  ev = ev + 1.0 * inv;
+  ev = 1.0 / ev;
+  const double gy = (muI * sat + 1.0 * sxy) / ev;
+  const double gz = sxz / ev;
   a.dgxy[I] = gy;
   a.dgxz[I] = gz;
   if (a.effVisc) a.effVisc[I] = ev;
@@ -216,12 +242,11 @@ int launch_pl_fields(const Ctx& c, const PlFields& f, hipStream_t s) {
 }
 
 int launch_pl_strain(const Ctx& c, const double* u, const double* gVxy, const double* gVxz,
-                     const double* T, const double* A, const double* B, const double* n,
-                     const double* bcL, const double* bcR, double cap, double* dgxy, double* dgxz,
-                     double* tauQS, double* effVisc, double* sxyOut, const sc_fault_qd_args* fault,
-                     double* dslip, const PendingFinish* fin, hipStream_t s) {
+                     const double* K, const double* n, const double* bcL, const double* bcR, double cap,
+                     double* dgxy, double* dgxz, double* tauQS, double* effVisc, double* sxyOut,
+                     const sc_fault_qd_args* fault, double* dslip, const PendingFinish* fin, hipStream_t s) {
   const long long N = (long long)c.N;
-  PlStrainArgs a{u, gVxy, gVxz, T, A, B, n, bcL, bcR, cap, dgxy, dgxz, tauQS, effVisc, sxyOut};
+  PlStrainArgs a{u, gVxy, gVxz, K, n, bcL, bcR, cap, dgxy, dgxz, tauQS, effVisc, sxyOut};
   if (fin && fin->sc) a.fin = *fin;
   a.fault = fault ? 1 : 0;
   if (fault) {

@@ -139,11 +139,14 @@ int QdDriver::init_powerlaw(const double* T, const double* A, const double* B, c
   pl->A = dalloc(N, A);
   pl->B = dalloc(N, B);
   pl->n = dalloc(N, n);
+  pl->K = dalloc(N, nullptr);
   pl->effVisc = dalloc(N, nullptr);
   pl->sxy = dalloc(N, nullptr);
   pl->cap = cap;
   errMask = mask;
-  int e = add_var("gVxy", N);
+  if (!pl->T || !pl->A || !pl->B || !pl->n || !pl->K) return fail(SC_ERR_HIP, "device allocation failed");
+  int e = launch_pl_factor(ctx->c, pl->A, pl->B, pl->T, pl->K, ctx->c.stream);
+  if (!e) e = add_var("gVxy", N);
   if (!e) e = add_var("gVxz", N);
   SC_HIP(hipStreamSynchronize(ctx->c.stream));
   return e;
@@ -387,7 +390,7 @@ int QdDriver::d_dt(double t, const double* psi, const double* slip, double* dpsi
   if (pl) {
     // the fault's rates fused in at the fault nodes, and the deferred guess
     // check as one more block
-    e = launch_pl_strain(ctx->c, u_d, gxy, gxz, pl->T, pl->A, pl->B, pl->n, bcL_d, bcR_d, pl->cap,
+    e = launch_pl_strain(ctx->c, u_d, gxy, gxz, pl->K, pl->n, bcL_d, bcR_d, pl->cap,
                          dgxy, dgxz, tauQS_d, pl->effVisc, pl->sxy, &fa, dslip, finPending ? &fin : nullptr, s);
     if (e) return e;
     finPending = false;

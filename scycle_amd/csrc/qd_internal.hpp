@@ -148,6 +148,7 @@ struct RkState {
 // Power-law viscoelastic terms (StrikeSlip_PowerLaw_qd with dislocation creep)
 struct PowerLawState {
   double *T = nullptr, *A = nullptr, *B = nullptr, *n = nullptr;  // N each
+  double* K = nullptr;  // 1e3 A exp(-B/T): the creep law's factor that does not depend on stress
   double cap = 1e30;
   double *effVisc = nullptr, *sxy = nullptr;                       // last d_dt
 };

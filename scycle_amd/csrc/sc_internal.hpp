@@ -301,11 +301,12 @@ struct PlFields {
   double *gTxy, *gTxz, *sxy, *sxz, *sdev, *effVisc, *invDisl, *dgVdev;
 };
 int launch_pl_fields(const Ctx& c, const PlFields& f, hipStream_t s);
+// the creep law's stress-free factor K = 1e3 A exp(-B/T) per node, formed once
+int launch_pl_factor(const Ctx& c, const double* A, const double* B, const double* T, double* K, hipStream_t s);
 int launch_pl_strain(const Ctx& c, const double* u, const double* gVxy, const double* gVxz,
-                     const double* T, const double* A, const double* B, const double* n,
-                     const double* bcL, const double* bcR, double cap, double* dgxy, double* dgxz,
-                     double* tauQS, double* effVisc, double* sxyOut, const sc_fault_qd_args* fault,
-                     double* dslip, const PendingFinish* fin, hipStream_t s);
+                     const double* K, const double* n, const double* bcL, const double* bcR, double cap,
+                     double* dgxy, double* dgxz, double* tauQS, double* effVisc, double* sxyOut,
+                     const sc_fault_qd_args* fault, double* dslip, const PendingFinish* fin, hipStream_t s);
 int fault_qd_rates(hipStream_t s, const sc_fault_qd_args& a);
 // FD fault step; uBody (may be null) receives the fault displacement
 int fault_fd_step64(hipStream_t s, const sc_fault_fd_args& a, double* uBody);
